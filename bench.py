@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the linear-static solve path on one MI355X.
+
+Workload: the 606,015-DOF jittered two-layer plate (100 x 100 x 0.8 mm, 100 x 100 x 2 hex cells
+x 6 Tet10, E 3.5 GPa, nu 0.36, clamped at x = 0, 0.01 MPa on the top face, interior vertices
+jittered by U(-5%, +5%) of the cell, seed 20261202). The matrix, the multigrid hierarchy and the
+solve's vectors are assembled and set up once and stay resident in HBM; a step is one full
+conjugate gradient solve from a zero estimate to a true relative residual of 1e-8.
+
+The headline value is ms per solve with the FP32 V-cycle inside the FP64 conjugate gradient
+(the reference's Precision::Single); the plain all-FP64 form (Precision::Double) is timed beside
+it. The dominant kernel -- the fused Chebyshev step on the finest level -- is timed with HIP
+events on its own stream inside the timed region, for the roofline. The CPU baseline is the
+oracle's serial restatement of the reference CPU path timed on a bounded sample of iterations.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "ms/solve to rtol 1e-8 @606k-DOF plate, FP32 vs FP64 V-cycle; served p99 latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, if any."""
+    f = ROOT / "profiles" / "traffic_fine_chebyshev_plate606k.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(workload, gpu_iterations, sample_iterations):
+    """The oracle (serial C++ restatement of the reference CPU path) on the same plate: setup once,
+    then `sample_iterations` PCG iterations with the f32 V-cycle, scaled to a full solve."""
+    import refcpu
+
+    t0 = time.perf_counter()
+    problem = refcpu.Problem(workload)
+    mg = refcpu.Multigrid(problem, refcpu.options(precision=2))
+    prep = time.perf_counter() - t0
+    _, r = mg.solve(1e-8, iteration_limit=sample_iterations)
+    per_iteration = r.solve_seconds / max(1, r.iterations)
+    return {
+        "value": per_iteration * gpu_iterations * 1e3,
+        "unit": "ms/solve",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (serial C++ restatement of the reference CPU path, Precision::Single V-cycle) on the same "
+                   f"606k-DOF plate: {r.iterations} PCG iterations timed ({r.solve_seconds:.2f} s, "
+                   f"{per_iteration * 1e3:.1f} ms/iteration) x {gpu_iterations} iterations of the GPU solve; "
+                   f"assembly+setup {prep:.1f} s not included"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--divisions", type=str, default="100,100,2")
+    ap.add_argument("--cpu-iterations", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
+              "independent solves shard one per GPU, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import solidfem_amd as sf
+    from workloads import plate_config
+
+    divisions = tuple(int(v) for v in args.divisions.split(","))
+    workload = plate_config(divisions=divisions)
+    ctx = sf.Context(0)
+
+    log(f"building the plate {divisions}")
+    t0 = time.perf_counter()
+    pcg_options = sf.PcgOptions(tolerance=1e-8, dot_order=sf.DotOrder.TREE)
+    builder = sf.StudyBuilder.new(ctx, sf.Model.from_workload(workload), pcg_options,
+                                  sf.MultigridOptions(precision=sf.Precision.DOUBLE))
+    builder.step(2 ** 62)
+    study = builder.finish()
+    build_seconds = time.perf_counter() - t0
+    report = study.report()
+    log(f"built in {build_seconds:.1f} s: {report.free_dofs} free DOFs, {report.nonzeros} stored values, "
+        f"{report.multigrid.levels} levels, operator complexity {report.multigrid.operator_complexity:.3f}; "
+        f"assembly {report.assembly_seconds:.2f} s, preconditioner {report.preconditioner_seconds:.2f} s")
+
+    system = study.system()
+    rhs = study.rhs()
+    mg64 = study.multigrid()
+    mg32 = mg64.with_precision(sf.Precision.SINGLE)
+    forms = {
+        "fp64": sf.Pcg.new(ctx, system, mg64, rhs, pcg_options),
+        "fp32": sf.Pcg.new(ctx, system, mg32, rhs, pcg_options),
+    }
+
+    def solve(pcg):
+        pcg.restart()
+        p = pcg.step(2 ** 62)
+        if p.outcome != sf.StepOutcome.CONVERGED:
+            raise RuntimeError(f"solve did not converge: {p}")
+        return p
+
+    results = {}
+    for name in ("fp64", "fp32"):
+        pcg = forms[name]
+        for _ in range(args.warmup):
+            solve(pcg)
+        measure = name == "fp32"
+        if measure:
+            for k in sf.Kernel:
+                sf.profile_enable(k, True)
+            sf.profile_reset()
+        ctx.synchronize()
+        t = time.perf_counter()
+        progress = [solve(pcg) for _ in range(args.steps)]
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t
+        kernels = {}
+        if measure:
+            for k in sf.Kernel:
+                kernels[k.name] = sf.profile_read(k)
+                sf.profile_enable(k, False)
+        it = progress[-1].iteration
+        assert all(p.iteration == it for p in progress), "a repeated solve took a different path"
+        results[name] = dict(ms=elapsed / args.steps * 1e3, iterations=it, residual=progress[-1].relative_residual,
+                             floor=progress[-1].at_rounding_floor, x=pcg.estimate(), kernels=kernels)
+        log(f"{name}: {results[name]['ms']:.2f} ms/solve, {it} iterations, residual {progress[-1].relative_residual:.2e}")
+
+    x64, x32 = results["fp64"]["x"], results["fp32"]["x"]
+    disp_err = float(np.abs(x32 - x64).max() / np.abs(x64).max())
+    n_free = report.free_dofs
+    fp32 = results["fp32"]
+    launches, seconds, nbytes = fp32["kernels"]["FINE_CHEBYSHEV"]
+    per_launch = nbytes / launches
+    achieved = per_launch / (seconds / launches) / 1e9
+    spmv_n, spmv_s, spmv_b = fp32["kernels"]["SYSTEM_PRODUCT"]
+    res_n, res_s, res_b = fp32["kernels"]["FINE_RESIDUAL"]
+
+    line = {
+        "metric": METRIC,
+        "value": round(fp32["ms"], 3),
+        "unit": "ms/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fp32["ms"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle in f64 PCG",
+        "data": "synthetic (seeded jittered plate)",
+        "config": {
+            "workload": f"plate 100x100x0.8 mm, {divisions[0]}x{divisions[1]}x{divisions[2]} hex x 6 Tet10, "
+                        f"{3 * (2 * divisions[0] + 1) * (2 * divisions[1] + 1) * (2 * divisions[2] + 1)} DOFs "
+                        f"({n_free} free), jitter 5% seed 20261202, rtol 1e-8",
+            "levels": int(report.multigrid.levels),
+            "operator_complexity": round(report.multigrid.operator_complexity, 4),
+        },
+        "fp64_ms_per_solve": round(results["fp64"]["ms"], 3),
+        "fp32_over_fp64_speedup": round(results["fp64"]["ms"] / fp32["ms"], 3),
+        "iterations": {"fp32": int(fp32["iterations"]), "fp64": int(results["fp64"]["iterations"])},
+        "relative_residual": {"fp32": fp32["residual"], "fp64": results["fp64"]["residual"]},
+        "fp32_vs_fp64_displacement_max_rel_diff": disp_err,
+        "dof_iterations_per_s": {"fp32": n_free * fp32["iterations"] / (fp32["ms"] * 1e-3),
+                                 "fp64": n_free * results["fp64"]["iterations"] / (results["fp64"]["ms"] * 1e-3)},
+        "solves_per_s": round(1e3 / fp32["ms"], 3),
+        "build_seconds": {"assembly": round(report.assembly_seconds, 3),
+                          "multigrid_setup": round(report.preconditioner_seconds, 3)},
+        "roofline": {
+            "kernel": "cheb_step_kernel<3, float, float> (fused Chebyshev step, finest level)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(),
+            "algorithmic_bytes_per_launch": per_launch,
+            "avg_launch_us": round(seconds / launches * 1e6, 2),
+            "launches": launches,
+        },
+        "other_kernels": {
+            "system_product_f64": {"avg_launch_us": round(spmv_s / max(1, spmv_n) * 1e6, 2),
+                                   "achieved_GBs": round(spmv_b / max(spmv_s, 1e-30) / 1e9, 1), "launches": spmv_n},
+            "fine_residual_f32": {"avg_launch_us": round(res_s / max(1, res_n) * 1e6, 2),
+                                  "achieved_GBs": round(res_b / max(res_s, 1e-30) / 1e9, 1), "launches": res_n},
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline (oracle) on a bounded sample")
+        line["cpu_baseline"] = cpu_baseline(workload, int(fp32["iterations"]), args.cpu_iterations)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

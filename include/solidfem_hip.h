@@ -1,0 +1,396 @@
+/*
+ * solidfem_hip.h -- the C ABI of the MI355X-native linear-static solve path.
+ *
+ * The entry points replace the Rust API of solidfem's solve path one for one; each
+ * declaration cites the reference item it stands in for (paths relative to the reference
+ * repository). Conventions:
+ *   - Opaque handles with explicit create/destroy. Host arrays are borrowed for the call and
+ *     copied; the library owns all device memory.
+ *   - Every call returns an sf_status. SF_OK is 0. On failure sf_last_error() holds a message
+ *     and sf_last_error_payload() the named fields of the reference's error variant.
+ *   - A handle is used by one thread at a time (the reference's Multigrid is !Sync:
+ *     crates/solidfem-solve/src/multigrid/cycle.rs:111-114). Different handles may run
+ *     concurrently on different streams.
+ *   - The cancel callback runs on the calling thread, before every iteration of a solve
+ *     (crates/solidfem-solve/src/pcg.rs:179-183) and between row chunks of a setup
+ *     (crates/solidfem-solve/src/multigrid/setup.rs:191-192). Nothing throws across the ABI.
+ *   - Results never depend on how a resumable step is sliced
+ *     (crates/solidfem-solve/src/lib.rs:15-19).
+ */
+#ifndef SOLIDFEM_HIP_H
+#define SOLIDFEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t sf_status;
+
+enum {
+    SF_OK = 0,
+    SF_ERR_INVALID_ARGUMENT = 1,
+    SF_ERR_HIP = 2,
+    SF_ERR_OUT_OF_MEMORY = 3,
+    SF_ERR_INTERNAL = 4,
+    /* PcgError, crates/solidfem-solve/src/pcg.rs:59-77 */
+    SF_ERR_PCG_NOT_POSITIVE_DEFINITE = 10, /* payload: a=iteration, value=curvature */
+    SF_ERR_PCG_NOT_CONVERGED = 11,         /* payload: a=iterations, value=relative residual */
+    /* MultigridError, crates/solidfem-solve/src/multigrid/mod.rs:197-246 */
+    SF_ERR_MG_BLOCK_STRUCTURE = 20,        /* a=row */
+    SF_ERR_MG_NEAR_NULL_LENGTH = 21,       /* a=expected, b=found */
+    SF_ERR_MG_NOT_POSITIVE_DEFINITE = 22,  /* a=level, b=block, value=pivot */
+    SF_ERR_MG_CANCELLED = 23,
+    SF_ERR_MG_POSITION_LENGTH = 24,        /* a=expected, b=found */
+    SF_ERR_MG_NOT_FINISHED = 25,
+    SF_ERR_MG_COARSEST_NOT_FACTORED = 26,  /* a=values */
+    SF_ERR_MG_COARSE_LENGTH = 27,          /* a=expected, b=found */
+    /* JacobiError, crates/solidfem-solve/src/preconditioner.rs:22-31 */
+    SF_ERR_JACOBI_DIAGONAL = 28,           /* a=row, value=diagonal */
+    /* MeshError, crates/solidfem-mesh/src/error.rs:7-27 */
+    SF_ERR_MESH_EMPTY = 30,
+    SF_ERR_MESH_NON_FINITE_POSITION = 31,  /* a=node */
+    SF_ERR_MESH_NODE_OUT_OF_RANGE = 32,    /* a=tet, b=node */
+    SF_ERR_MESH_UNUSED_NODE = 33,          /* a=node */
+    SF_ERR_MESH_DEGENERATE_TET = 34,       /* a=tet, value=volume */
+    SF_ERR_MESH_FLAT_TET = 35,             /* a=tet, value=relative volume */
+    SF_ERR_MESH_NON_MANIFOLD_FACE = 36,
+    SF_ERR_MESH_INVALID_BOX = 37,
+    /* ModelError, crates/solidfem-elastic/src/error.rs:23-95 (the solve path's variants) */
+    SF_ERR_MODEL_NO_FIXED_NODES = 40,
+    SF_ERR_MODEL_NODE_OUT_OF_RANGE = 41,   /* a=node */
+    SF_ERR_MODEL_NON_FINITE_LOAD = 42,
+    SF_ERR_MODEL_FACE_NOT_ON_BOUNDARY = 43,/* a=tet, b=face */
+    SF_ERR_MODEL_NO_EFFECTIVE_LOAD = 44,
+    SF_ERR_MODEL_INVERTED_JACOBIAN = 45,   /* a=tet, b=point, value=determinant */
+    SF_ERR_MODEL_INVALID_MATERIAL = 46,    /* a=tet, b=point */
+    SF_ERR_MODEL_CANCELLED = 47,
+    SF_ERR_MODEL_NOT_FINISHED = 48,
+    SF_ERR_MODEL_AXIS_OUT_OF_RANGE = 49,   /* a=axis */
+    SF_ERR_MODEL_PRESCRIBED_ON_FIXED = 50, /* a=node */
+    /* serving */
+    SF_ERR_SERVE_OVER_BUDGET = 60,         /* a=bytes needed, b=budget */
+    SF_ERR_SERVE_UNKNOWN_REQUEST = 61
+};
+
+/* Precision of the finished hierarchy, crates/solidfem-solve/src/multigrid/mod.rs:126-141 */
+enum { SF_PRECISION_DOUBLE = 0, SF_PRECISION_SINGLE_STORAGE = 1, SF_PRECISION_SINGLE = 2 };
+
+/* Order of the conjugate gradient's dot products. REFERENCE sums in index order from zero,
+ * exactly crates/solidfem-solve/src/pcg.rs:285-301, so a solve reproduces the reference's bits
+ * (a serial reduction: for parity runs on small systems). TREE sums in a fixed wavefront tree
+ * and a fixed order of partials: the same bits on every run, at full device speed. */
+enum { SF_DOT_TREE = 0, SF_DOT_REFERENCE = 1 };
+
+/* Step outcome, crates/solidfem-solve/src/pcg.rs:47-57 */
+enum { SF_RUNNING = 0, SF_CONVERGED = 1, SF_CANCELLED = 2 };
+
+/* Returns non-zero to cancel. */
+typedef int32_t (*sf_cancel_fn)(void *user);
+
+/* Progress, crates/solidfem-solve/src/pcg.rs:35-45, plus the outcome of the call. */
+typedef struct sf_progress {
+    uint64_t iteration;
+    double relative_residual;
+    int32_t at_rounding_floor;
+    int32_t outcome;
+} sf_progress;
+
+/* PcgOptions, crates/solidfem-solve/src/pcg.rs:14-33 (max_iterations 0 = 10 n + 1000) */
+typedef struct sf_pcg_options {
+    double tolerance;
+    uint64_t max_iterations;
+    int32_t dot_order;
+    int32_t reserved;
+} sf_pcg_options;
+
+/* MultigridOptions, crates/solidfem-solve/src/multigrid/mod.rs:143-182 */
+typedef struct sf_multigrid_options {
+    double neighbour_reach;
+    uint64_t coarse_dofs;
+    uint64_t max_levels;
+    uint64_t smoother_degree;
+    double smoother_range;
+    int32_t precision;
+    int32_t threads; /* host threads for setup; 0 = all */
+} sf_multigrid_options;
+
+/* MultigridStats, crates/solidfem-solve/src/multigrid/mod.rs:184-195 */
+typedef struct sf_multigrid_stats {
+    uint64_t levels;
+    uint64_t coarse_dofs;
+    double operator_complexity;
+    uint64_t memory_bytes;
+} sf_multigrid_stats;
+
+/* Report, crates/solidfem-elastic/src/study.rs:150-176 */
+typedef struct sf_report {
+    uint64_t total_dofs;
+    uint64_t free_dofs;
+    uint64_t nonzeros;
+    int32_t at_rounding_floor;
+    int32_t has_multigrid;
+    uint64_t iterations;
+    double relative_residual;
+    double assembly_seconds;
+    double preconditioner_seconds;
+    sf_multigrid_stats multigrid;
+    double solve_seconds;
+} sf_report;
+
+/* BuildProgress, crates/solidfem-elastic/src/study.rs:377-387 */
+typedef struct sf_build_progress {
+    uint64_t elements_done;
+    uint64_t elements_total;
+    int32_t setup_finished;
+    int32_t reserved;
+} sf_build_progress;
+
+/* The structured box problems the benchmark configurations are built from: a box mesh
+ * (crates/solidfem-mesh/src/box_mesh.rs:22-80) with optional seeded interior jitter, upgraded to
+ * Tet10, clamped on a plane, loaded by tractions on the boundary faces of a plane. */
+enum { SF_TRACTION_UNIFORM = 0, SF_TRACTION_RANDOM = 1, SF_TRACTION_TORSION = 2 };
+enum { SF_MATERIAL_UNIFORM = 0, SF_MATERIAL_SHELL_CORE = 1 };
+typedef struct sf_workload {
+    double origin[3];
+    double size[3];
+    int32_t divisions[3];
+    double jitter_fraction;
+    uint64_t jitter_seed;
+    int32_t clamp_axis;
+    double clamp_value;
+    int32_t load_axis;
+    double load_value;
+    int32_t traction_kind;
+    double traction[3];
+    double traction_scale;
+    uint64_t traction_seed;
+    int32_t load_case;
+    double axis_u, axis_v;
+    int32_t material_kind;
+    double youngs, poisson;
+    double core_youngs, core_poisson;
+    double shell_depth;
+    int32_t quadrature_points;
+} sf_workload;
+
+typedef struct sf_context sf_context;
+typedef struct sf_tet_mesh sf_tet_mesh;
+typedef struct sf_quad_mesh sf_quad_mesh;
+typedef struct sf_model sf_model;
+typedef struct sf_study_builder sf_study_builder;
+typedef struct sf_static_study sf_static_study;
+typedef struct sf_csr sf_csr;
+typedef struct sf_multigrid_setup sf_multigrid_setup;
+typedef struct sf_multigrid sf_multigrid;
+typedef struct sf_pcg sf_pcg;
+typedef struct sf_server sf_server;
+
+/* ---- errors and device ---------------------------------------------------------------- */
+const char *sf_last_error(void);
+void sf_last_error_payload(int64_t *a, int64_t *b, double *value);
+const char *sf_version(void);
+sf_status sf_device_count(int32_t *count);
+/* A device context: one GPU, one stream. No reference counterpart: the reference runs on the
+ * host (crates/solidfem-solve/src/parallel.rs:82-215 is its worker pool). */
+sf_status sf_context_create(int32_t device, sf_context **out);
+void sf_context_destroy(sf_context *ctx);
+sf_status sf_context_synchronize(sf_context *ctx);
+
+/* ---- mesh: crates/solidfem-mesh ------------------------------------------------------- */
+/* TetMesh::new, crates/solidfem-mesh/src/tet.rs:59-109 */
+sf_status sf_tet_mesh_create(const double *positions, uint64_t node_count, const uint32_t *tets,
+                             uint64_t tet_count, sf_tet_mesh **out);
+/* box_tet_mesh, crates/solidfem-mesh/src/box_mesh.rs:22-80 */
+sf_status sf_box_tet_mesh(const double origin[3], const double size[3], const int32_t divisions[3],
+                          sf_tet_mesh **out);
+void sf_tet_mesh_destroy(sf_tet_mesh *mesh);
+uint64_t sf_tet_mesh_node_count(const sf_tet_mesh *mesh);
+uint64_t sf_tet_mesh_tet_count(const sf_tet_mesh *mesh);
+/* QuadraticTetMesh::from_linear, crates/solidfem-mesh/src/quadratic.rs:24-71 */
+sf_status sf_quad_mesh_from_linear(const sf_tet_mesh *mesh, sf_quad_mesh **out);
+void sf_quad_mesh_destroy(sf_quad_mesh *mesh);
+uint64_t sf_quad_mesh_node_count(const sf_quad_mesh *mesh);
+uint64_t sf_quad_mesh_tet_count(const sf_quad_mesh *mesh);
+uint64_t sf_quad_mesh_vertex_count(const sf_quad_mesh *mesh);
+void sf_quad_mesh_nodes(const sf_quad_mesh *mesh, double *out);   /* 3 per node */
+void sf_quad_mesh_tets(const sf_quad_mesh *mesh, uint32_t *out);  /* 10 per element */
+/* QuadraticTetMesh::boundary_faces, crates/solidfem-mesh/src/quadratic.rs:89-124:
+ * (tet, face, 6 nodes) as 8 u32 per face. Returns the count; out may be NULL. */
+uint64_t sf_quad_mesh_boundary_faces(const sf_quad_mesh *mesh, uint32_t *out);
+
+/* ---- model and study: crates/solidfem-elastic ------------------------------------------ */
+/* Model::new, crates/solidfem-elastic/src/study.rs:58-68 (uniform material) */
+sf_status sf_model_create(const sf_quad_mesh *mesh, double youngs_modulus, double poisson_ratio,
+                          sf_model **out);
+/* A two-material field by depth below the long faces (the MaterialField of
+ * crates/solidfem/tests/validation/shell_core.rs:19-21 over material.rs:35-50). */
+sf_status sf_model_set_shell_core(sf_model *model, double shell_depth, double core_youngs,
+                                  double core_poisson);
+/* Model::set_quadrature, study.rs:75-78: 4 or 14 points */
+sf_status sf_model_set_quadrature(sf_model *model, int32_t points);
+/* Model::fix_node, study.rs:80-83 */
+sf_status sf_model_fix_node(sf_model *model, uint32_t node);
+/* Model::add_nodal_force, study.rs:85-88 */
+sf_status sf_model_add_nodal_force(sf_model *model, uint32_t node, const double force[3]);
+/* Model::prescribe_displacement, study.rs:90-99 */
+sf_status sf_model_prescribe_displacement(sf_model *model, uint32_t node, uint32_t axis, double value);
+/* Model::add_traction, study.rs:101-104; face = (tet, face, 6 nodes) as from boundary_faces */
+sf_status sf_model_add_traction(sf_model *model, const uint32_t face[8], const double traction[3]);
+/* A structured box model (mesh, clamp, tractions, material) from a workload description. */
+sf_status sf_model_from_workload(const sf_workload *workload, sf_model **out);
+void sf_model_destroy(sf_model *model);
+const sf_quad_mesh *sf_model_mesh(const sf_model *model);
+/* element_stiffness, crates/solidfem-elastic/src/element.rs:98-133, evaluated on the device for
+ * elements [first, first+count): 900 values each, row-major. */
+sf_status sf_model_element_stiffness(sf_context *ctx, const sf_model *model, uint64_t first,
+                                     uint64_t count, double *ke);
+
+/* The numbered problem of StudyBuilder::new (study.rs:436-449, assembly.rs:55-81) on the host:
+ * free map (node -> free index or UINT32_MAX), restricted right-hand side, the free stiffness's
+ * block pattern (block row starts, block columns) and rigid-body modes (rigid_modes.rs:36-83).
+ * Any output may be NULL; the counts are returned. */
+sf_status sf_model_free_problem(const sf_model *model, uint64_t *free_nodes, uint64_t *blocks, uint32_t *free_map,
+                                double *rhs, int64_t *block_starts, uint32_t *block_columns, double *rigid_modes);
+
+/* StudyBuilder::new, crates/solidfem-elastic/src/study.rs:419-474. use_multigrid=0 chooses
+ * Jacobi (PreconditionerChoice::Jacobi, study.rs:108-117); mg may be NULL then. The model is
+ * copied. */
+sf_status sf_study_builder_create(sf_context *ctx, const sf_model *model, const sf_pcg_options *pcg,
+                                  int32_t use_multigrid, const sf_multigrid_options *mg,
+                                  sf_study_builder **out);
+/* StudyBuilder::step, study.rs:478-512 */
+sf_status sf_study_builder_step(sf_study_builder *builder, uint64_t max_elements, sf_cancel_fn cancel,
+                                void *user, sf_build_progress *progress);
+/* StudyBuilder::finish, study.rs:601-656; consumes the builder on success */
+sf_status sf_study_builder_finish(sf_study_builder *builder, sf_static_study **out);
+void sf_study_builder_destroy(sf_study_builder *builder);
+/* StaticStudy::new, study.rs:191-199 (builder step with an unbounded budget, then finish) */
+sf_status sf_static_study_create(sf_context *ctx, const sf_model *model, const sf_pcg_options *pcg,
+                                 int32_t use_multigrid, const sf_multigrid_options *mg,
+                                 sf_cancel_fn cancel, void *user, sf_static_study **out);
+/* StaticStudy::step, study.rs:202-215 */
+sf_status sf_static_study_step(sf_static_study *study, uint64_t max_iterations, sf_cancel_fn cancel,
+                               void *user, sf_progress *progress);
+/* StaticStudy::displacements, study.rs:218-237: 3 per mesh node, fixed nodes zero */
+sf_status sf_static_study_displacements(const sf_static_study *study, double *out);
+/* StaticStudy::report, study.rs:240-243 */
+sf_status sf_static_study_report(const sf_static_study *study, sf_report *out);
+/* StaticStudy::multigrid, study.rs:362-368 (NULL for Jacobi) */
+const sf_multigrid *sf_static_study_multigrid(const sf_static_study *study);
+/* The conjugate gradient solve behind the study. */
+sf_pcg *sf_static_study_pcg(sf_static_study *study);
+/* The assembled free stiffness (scalar CSR, the reference's CsrMatrix, csr.rs:6-13), read back
+ * from the device: row_starts (n+1), columns and values (nnz); any pointer may be NULL. */
+sf_status sf_static_study_matrix(const sf_static_study *study, uint64_t *size, uint64_t *nonzeros,
+                                 int64_t *row_starts, uint32_t *columns, double *values);
+sf_status sf_static_study_rhs(const sf_static_study *study, double *out);
+void sf_static_study_destroy(sf_static_study *study);
+
+/* ---- linear algebra: crates/solidfem-solve --------------------------------------------- */
+/* CsrMatrix from plain arrays (the pattern of CsrMatrix::from_block_connectivity plus values,
+ * csr.rs:19-71 and :89-96), uploaded to the device; with ctx NULL it stays on the host (enough
+ * for a multigrid setup, which runs on the host). */
+sf_status sf_csr_create(sf_context *ctx, uint64_t size, const int64_t *row_starts,
+                        const uint32_t *columns, const double *values, sf_csr **out);
+void sf_csr_destroy(sf_csr *csr);
+/* CsrMatrix::multiply, csr.rs:112-132, on the device: host x and y of `size` */
+sf_status sf_csr_multiply(sf_csr *csr, const double *x, double *y);
+/* CsrMatrix::multiply_rounding_bound, csr.rs:142-154, on the device */
+sf_status sf_csr_rounding_bound(sf_csr *csr, const double *x, int32_t dot_order, double *bound);
+
+/* MultigridSetup::new, crates/solidfem-solve/src/multigrid/setup.rs:129-189: near_null holds
+ * near_null_len values (6 per scalar row), positions 3 per block row; wrong lengths are refused. */
+sf_status sf_multigrid_setup_create(sf_context *ctx, const sf_csr *matrix, const double *near_null,
+                                    uint64_t near_null_len, const double *positions, uint64_t position_count,
+                                    const sf_multigrid_options *options, sf_multigrid_setup **out);
+/* MultigridSetup::step, setup.rs:193-222: finished = 1 once done */
+sf_status sf_multigrid_setup_step(sf_multigrid_setup *setup, uint64_t budget, sf_cancel_fn cancel,
+                                  void *user, int32_t *finished);
+/* MultigridSetup::finish, setup.rs:230-240: builds the device hierarchy */
+sf_status sf_multigrid_setup_finish(sf_multigrid_setup *setup, sf_multigrid **out);
+void sf_multigrid_setup_destroy(sf_multigrid_setup *setup);
+void sf_multigrid_destroy(sf_multigrid *mg);
+/* Multigrid::stats / precision, cycle.rs:319-330 */
+sf_status sf_multigrid_stats_get(const sf_multigrid *mg, sf_multigrid_stats *out);
+int32_t sf_multigrid_precision(const sf_multigrid *mg);
+/* Preconditioner::apply for Multigrid, cycle.rs:430-446: one V-cycle on the device, host arrays */
+sf_status sf_multigrid_apply(sf_multigrid *mg, const double *residual, double *output);
+/* One Chebyshev smoother application on `level` from a zero guess, cycle.rs:570-651 */
+sf_status sf_multigrid_smooth(sf_multigrid *mg, uint64_t level, const double *b, double *x);
+/* Multigrid::level_sizes, cycle.rs:332-339: (rows, row_block, stored values) per level */
+sf_status sf_multigrid_level_sizes(const sf_multigrid *mg, uint64_t *out, uint64_t capacity, uint64_t *levels);
+/* Multigrid::level_matrix / level_prolongator / level_inverse_diagonal, cycle.rs:363-395, values
+ * widened to f64 as stored on the device. Any output pointer may be NULL; counts are returned. */
+sf_status sf_multigrid_level_matrix(const sf_multigrid *mg, uint64_t level, int32_t prolongator,
+                                    uint64_t *rows, uint64_t *block_columns, uint64_t *row_block,
+                                    uint64_t *col_block, uint64_t *blocks, int64_t *starts,
+                                    uint32_t *columns, double *values);
+sf_status sf_multigrid_level_inverse_diagonal(const sf_multigrid *mg, uint64_t level, double *out);
+/* Multigrid::smoother_spectrum / aggregates, cycle.rs:341-361 */
+sf_status sf_multigrid_smoother_spectrum(const sf_multigrid *mg, uint64_t level, double *lambda_min,
+                                         double *lambda_max);
+sf_status sf_multigrid_aggregates(const sf_multigrid *mg, uint64_t level, uint32_t *out, uint64_t *count);
+/* Multigrid::solve_coarsest, cycle.rs:402-408 */
+sf_status sf_multigrid_solve_coarsest(sf_multigrid *mg, double *values, uint64_t count);
+
+/* The levels of a finished setup, before MultigridSetup::finish moves them to the device: the
+ * same views as the sf_multigrid_level_* calls, in f64 as set up (host only, no device needed). */
+sf_status sf_multigrid_setup_level_count(const sf_multigrid_setup *setup, uint64_t *levels);
+sf_status sf_multigrid_setup_level_matrix(const sf_multigrid_setup *setup, uint64_t level, int32_t prolongator,
+                                          uint64_t *rows, uint64_t *block_columns, uint64_t *row_block,
+                                          uint64_t *col_block, uint64_t *blocks, int64_t *starts,
+                                          uint32_t *columns, double *values);
+sf_status sf_multigrid_setup_level_inverse_diagonal(const sf_multigrid_setup *setup, uint64_t level, double *out);
+sf_status sf_multigrid_setup_smoother_spectrum(const sf_multigrid_setup *setup, uint64_t level, double *lambda_min,
+                                               double *lambda_max);
+sf_status sf_multigrid_setup_aggregates(const sf_multigrid_setup *setup, uint64_t level, uint32_t *out, uint64_t *count);
+/* The coarsest level's Cholesky factor (n x n row-major, lower triangle); *n = 0 when unfactored. */
+sf_status sf_multigrid_setup_coarse_factor(const sf_multigrid_setup *setup, uint64_t *n, double *out);
+
+/* Pcg::new, crates/solidfem-solve/src/pcg.rs:135-164, preconditioned by a multigrid (mg) or,
+ * with mg NULL, by Jacobi (preconditioner.rs:53-66). The solve owns device copies; the matrix
+ * and the multigrid must outlive it. */
+sf_status sf_pcg_create(sf_context *ctx, const sf_csr *matrix, sf_multigrid *mg, const double *rhs,
+                        const sf_pcg_options *options, sf_pcg **out);
+/* Pcg::step, pcg.rs:168-189 */
+sf_status sf_pcg_step(sf_pcg *pcg, uint64_t max_iterations, sf_cancel_fn cancel, void *user,
+                      sf_progress *progress);
+/* Pcg::estimate, pcg.rs:192-194 (copied to the host) */
+sf_status sf_pcg_estimate(const sf_pcg *pcg, double *out);
+/* Pcg::progress / is_converged, pcg.rs:202-213 */
+sf_status sf_pcg_progress(const sf_pcg *pcg, sf_progress *out);
+int32_t sf_pcg_is_converged(const sf_pcg *pcg);
+uint64_t sf_pcg_size(const sf_pcg *pcg);
+void sf_pcg_destroy(sf_pcg *pcg);
+
+/* ---- resident re-use and measurement (no reference counterpart) ------------------------ */
+/* The study's system matrix as a CsrMatrix handle (owned by the study). */
+const sf_csr *sf_static_study_system(const sf_static_study *study);
+/* The same hierarchy stored and applied in another precision (Multigrid::from_levels,
+ * crates/solidfem-solve/src/multigrid/cycle.rs:265-308, over the same f64 levels). */
+sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision, sf_multigrid **out);
+/* Algorithmic HBM bytes of one V-cycle of this hierarchy (every array a kernel touches, once). */
+double sf_multigrid_cycle_bytes(const sf_multigrid *mg);
+/* Restarts a solve from a zero estimate (Pcg::new semantics, pcg.rs:135-164) on the buffers it
+ * already holds; new_rhs NULL keeps the right-hand side. Allocates nothing. */
+sf_status sf_pcg_restart(sf_pcg *pcg, const double *new_rhs);
+/* Kernel launch timing with HIP events on the launching stream. Kernels: 0 = the fused
+ * Chebyshev step on the finest level, 1 = the conjugate gradient's system product A p, 2 = the
+ * residual on the finest level. read returns launches, summed seconds and summed algorithmic
+ * bytes since the last reset. */
+enum { SF_KERNEL_FINE_CHEBYSHEV = 0, SF_KERNEL_SYSTEM_PRODUCT = 1, SF_KERNEL_FINE_RESIDUAL = 2 };
+sf_status sf_profile_enable(int32_t kernel, int32_t enable);
+sf_status sf_profile_reset(void);
+sf_status sf_profile_read(int32_t kernel, uint64_t *launches, double *seconds, double *bytes);
+/* A plain device copy of `bytes` bytes (16-byte vector loads and stores), run `repeats` times on
+ * the context's stream: the achievable HBM bandwidth and the calibration of byte counters.
+ * Returns the average seconds per copy, timed with HIP events. */
+sf_status sf_stream_copy(sf_context *ctx, uint64_t bytes, uint32_t repeats, double *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -593,6 +593,21 @@ void ref_multigrid_stats(const ref_multigrid *m, int64_t *levels, int64_t *coars
 
 uint64_t ref_fingerprint(const double *values, int64_t count) { return fnv1a(values, count); }
 
+static CsrMatrix csr_view(int64_t n, const int64_t *rs, const uint32_t *cols, const double *vals) {
+    CsrMatrix m;
+    m.size = (size_t)n;
+    m.row_starts.assign(rs, rs + n + 1);
+    m.columns.assign(cols, cols + rs[n]);
+    m.values.assign(vals, vals + rs[n]);
+    return m;
+}
+void ref_csr_multiply(int64_t n, const int64_t *rs, const uint32_t *cols, const double *vals, const double *x, double *y) {
+    csr_view(n, rs, cols, vals).multiply(x, y);
+}
+double ref_csr_rounding_bound(int64_t n, const int64_t *rs, const uint32_t *cols, const double *vals, const double *x) {
+    return csr_view(n, rs, cols, vals).multiply_rounding_bound(x);
+}
+
 int ref_solve_with(const ref_problem *rp, const ref_multigrid *rm, double tolerance, uint64_t iteration_limit,
                    double *displacements, ref_result *result) {
     const Problem &p = *rp->p;

@@ -151,6 +151,12 @@ int ref_solve_with(const ref_problem *p, const ref_multigrid *m, double toleranc
 /* Chebyshev smoother on level `level` from a zero guess: x = poly(D^-1 A) D^-1 b (hierarchy precision) */
 void ref_multigrid_smooth(const ref_multigrid *m, int64_t level, const double *b, double *x);
 
+/* CsrMatrix::multiply and multiply_rounding_bound (csr.rs:112-154) on plain arrays. */
+void ref_csr_multiply(int64_t n, const int64_t *row_starts, const uint32_t *columns, const double *values,
+                      const double *x, double *y);
+double ref_csr_rounding_bound(int64_t n, const int64_t *row_starts, const uint32_t *columns, const double *values,
+                              const double *x);
+
 /* FNV-1a over the little-endian bits of `count` doubles */
 uint64_t ref_fingerprint(const double *values, int64_t count);
 

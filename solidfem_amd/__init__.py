@@ -1,0 +1,696 @@
+"""solidfem_amd: the MI355X-native linear-static solve path of solidfem.
+
+A Python mirror of the reference's Rust API (crates/solidfem-solve/src/lib.rs:28-34,
+crates/solidfem-elastic/src/lib.rs:44-66) over the C ABI in include/solidfem_hip.h. Every
+compute call runs in libsolidfem_hip.so on the GPU: Tet10 stiffness assembly into 3x3 blocks,
+the block smoothed-aggregation multigrid V-cycle with its Chebyshev smoother, and the
+resumable, cancellable, slice-invariant conjugate gradient solve. The multigrid setup runs on
+the host threads inside the same library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _capi
+
+__all__ = [
+    "Context", "Precision", "DotOrder", "PcgOptions", "MultigridOptions", "Workload", "TetMesh", "QuadraticTetMesh",
+    "box_tet_mesh", "Model", "StudyBuilder", "StaticStudy", "CsrMatrix", "MultigridSetup", "Multigrid", "Pcg",
+    "StepOutcome", "SolidfemError", "PcgError", "MultigridError", "ModelError", "MeshError", "library",
+]
+
+
+def library():
+    return _capi.load()
+
+
+class Precision(enum.IntEnum):
+    """crates/solidfem-solve/src/multigrid/mod.rs:126-141"""
+    DOUBLE = 0
+    SINGLE_STORAGE = 1
+    SINGLE = 2
+
+
+class DotOrder(enum.IntEnum):
+    TREE = 0
+    REFERENCE = 1
+
+
+class SolidfemError(RuntimeError):
+    def __init__(self, status: int, message: str, a: int = 0, b: int = 0, value: float = 0.0):
+        super().__init__(f"[{status}] {message}")
+        self.status, self.message, self.a, self.b, self.value = status, message, a, b, value
+
+
+class PcgError(SolidfemError):
+    """pcg.rs:59-77: NotPositiveDefinite (10) or NotConverged (11)."""
+
+
+class MultigridError(SolidfemError):
+    """multigrid/mod.rs:197-246"""
+
+
+class ModelError(SolidfemError):
+    """crates/solidfem-elastic/src/error.rs:23-95"""
+
+
+class MeshError(SolidfemError):
+    """crates/solidfem-mesh/src/error.rs:7-27"""
+
+
+def _raise(status: int):
+    if status == 0:
+        return
+    lib = library()
+    a, b, v = C.c_int64(), C.c_int64(), C.c_double()
+    lib.sf_last_error_payload(C.byref(a), C.byref(b), C.byref(v))
+    msg = lib.sf_last_error().decode()
+    cls = SolidfemError
+    if 10 <= status < 20:
+        cls = PcgError
+    elif 20 <= status < 30:
+        cls = MultigridError
+    elif 30 <= status < 40:
+        cls = MeshError
+    elif 40 <= status < 60:
+        cls = ModelError
+    raise cls(status, msg, a.value, b.value, v.value)
+
+
+def _ptr(array: np.ndarray, ctype):
+    return array.ctypes.data_as(C.POINTER(ctype))
+
+
+def _cancel(cancel: Optional[Callable[[], bool]]):
+    if cancel is None:
+        return _capi.CANCEL_FN(lambda _u: 0)
+    return _capi.CANCEL_FN(lambda _u: 1 if cancel() else 0)
+
+
+@dataclass
+class PcgOptions:
+    """pcg.rs:14-33"""
+    tolerance: float = 1e-9
+    max_iterations: int = 0  # 0: 10 n + 1000
+    dot_order: DotOrder = DotOrder.TREE
+
+    def c(self):
+        return _capi.PcgOptions(self.tolerance, self.max_iterations, int(self.dot_order), 0)
+
+
+@dataclass
+class MultigridOptions:
+    """multigrid/mod.rs:143-182"""
+    neighbour_reach: float = 3.0
+    coarse_dofs: int = 1500
+    max_levels: int = 12
+    smoother_degree: int = 3
+    smoother_range: float = 30.0
+    precision: Precision = Precision.SINGLE_STORAGE
+    threads: int = 0
+
+    def c(self):
+        return _capi.MultigridOptions(self.neighbour_reach, self.coarse_dofs, self.max_levels, self.smoother_degree,
+                                      self.smoother_range, int(self.precision), self.threads)
+
+
+@dataclass
+class Workload:
+    """A structured box problem (see include/solidfem_hip.h: sf_workload)."""
+    size: tuple = (1.0, 1.0, 1.0)
+    divisions: tuple = (1, 1, 1)
+    origin: tuple = (0.0, 0.0, 0.0)
+    jitter_fraction: float = 0.0
+    jitter_seed: int = 0
+    clamp_axis: int = 0
+    clamp_value: float = 0.0
+    load_axis: int = 0
+    load_value: float = 1.0
+    traction_kind: int = 0
+    traction: tuple = (0.0, 0.0, 0.0)
+    traction_scale: float = 0.0
+    traction_seed: int = 0
+    load_case: int = 0
+    axis_u: float = 0.0
+    axis_v: float = 0.0
+    material_kind: int = 0
+    youngs: float = 200000.0
+    poisson: float = 0.3
+    core_youngs: float = 1.0
+    core_poisson: float = 0.0
+    shell_depth: float = 0.0
+    quadrature_points: int = 4
+
+    def fields(self):
+        return dict(origin=self.origin, size=self.size, divisions=self.divisions, jitter_fraction=self.jitter_fraction,
+                    jitter_seed=self.jitter_seed, clamp_axis=self.clamp_axis, clamp_value=self.clamp_value,
+                    load_axis=self.load_axis, load_value=self.load_value, traction_kind=self.traction_kind,
+                    traction=self.traction, traction_scale=self.traction_scale, traction_seed=self.traction_seed,
+                    load_case=self.load_case, axis_u=self.axis_u, axis_v=self.axis_v, material_kind=self.material_kind,
+                    youngs=self.youngs, poisson=self.poisson, core_youngs=self.core_youngs,
+                    core_poisson=self.core_poisson, shell_depth=self.shell_depth,
+                    quadrature_points=self.quadrature_points)
+
+    def c(self, struct=None):
+        w = (struct or _capi.Workload)()
+        for name, value in self.fields().items():
+            if isinstance(value, tuple):
+                arr = getattr(w, name)
+                for i, x in enumerate(value):
+                    arr[i] = x
+            else:
+                setattr(w, name, value)
+        return w
+
+
+class _Handle:
+    _destroy = ""
+
+    def __init__(self, ptr):
+        self._ptr = ptr
+
+    def close(self):
+        if getattr(self, "_ptr", None):
+            getattr(library(), self._destroy)(self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context(_Handle):
+    """One GPU and one stream."""
+    _destroy = "sf_context_destroy"
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _raise(library().sf_context_create(device, C.byref(h)))
+        super().__init__(h)
+
+    def synchronize(self):
+        _raise(library().sf_context_synchronize(self._ptr))
+
+    def stream_copy(self, nbytes: int, repeats: int = 10) -> float:
+        """Average seconds of a plain device copy of `nbytes` (read + write)."""
+        t = C.c_double()
+        _raise(library().sf_stream_copy(self._ptr, nbytes, repeats, C.byref(t)))
+        return t.value
+
+
+class TetMesh(_Handle):
+    """crates/solidfem-mesh/src/tet.rs:50-130"""
+    _destroy = "sf_tet_mesh_destroy"
+
+    @classmethod
+    def new(cls, positions, tets):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        tt = np.ascontiguousarray(tets, dtype=np.uint32).reshape(-1, 4)
+        h = C.c_void_p()
+        _raise(library().sf_tet_mesh_create(_ptr(pos, C.c_double), len(pos), _ptr(tt, C.c_uint32), len(tt), C.byref(h)))
+        return cls(h)
+
+
+def box_tet_mesh(origin, size, divisions) -> TetMesh:
+    """crates/solidfem-mesh/src/box_mesh.rs:22-80"""
+    o = (C.c_double * 3)(*origin)
+    s = (C.c_double * 3)(*size)
+    d = (C.c_int32 * 3)(*divisions)
+    h = C.c_void_p()
+    _raise(library().sf_box_tet_mesh(o, s, d, C.byref(h)))
+    return TetMesh(h)
+
+
+class QuadraticTetMesh(_Handle):
+    """crates/solidfem-mesh/src/quadratic.rs:13-125"""
+    _destroy = "sf_quad_mesh_destroy"
+
+    def __init__(self, ptr, owned=True):
+        super().__init__(ptr)
+        self._owned = owned
+
+    def close(self):
+        if self._owned:
+            super().close()
+
+    @classmethod
+    def from_linear(cls, mesh: TetMesh):
+        h = C.c_void_p()
+        _raise(library().sf_quad_mesh_from_linear(mesh._ptr, C.byref(h)))
+        return cls(h)
+
+    def nodes(self) -> np.ndarray:
+        n = library().sf_quad_mesh_node_count(self._ptr)
+        out = np.empty((n, 3), dtype=np.float64)
+        library().sf_quad_mesh_nodes(self._ptr, _ptr(out, C.c_double))
+        return out
+
+    def tets(self) -> np.ndarray:
+        n = library().sf_quad_mesh_tet_count(self._ptr)
+        out = np.empty((n, 10), dtype=np.uint32)
+        library().sf_quad_mesh_tets(self._ptr, _ptr(out, C.c_uint32))
+        return out
+
+    def vertex_count(self) -> int:
+        return library().sf_quad_mesh_vertex_count(self._ptr)
+
+    def boundary_faces(self) -> np.ndarray:
+        """(tet, face, 6 nodes) per face, in the reference's order."""
+        n = library().sf_quad_mesh_boundary_faces(self._ptr, None)
+        out = np.empty((n, 8), dtype=np.uint32)
+        library().sf_quad_mesh_boundary_faces(self._ptr, _ptr(out, C.c_uint32))
+        return out
+
+
+class Model(_Handle):
+    """crates/solidfem-elastic/src/study.rs:45-105"""
+    _destroy = "sf_model_destroy"
+
+    @classmethod
+    def new(cls, mesh: QuadraticTetMesh, youngs_modulus: float, poisson_ratio: float):
+        h = C.c_void_p()
+        _raise(library().sf_model_create(mesh._ptr, youngs_modulus, poisson_ratio, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_workload(cls, workload: Workload):
+        h = C.c_void_p()
+        w = workload.c()
+        _raise(library().sf_model_from_workload(C.byref(w), C.byref(h)))
+        return cls(h)
+
+    def mesh(self) -> QuadraticTetMesh:
+        return QuadraticTetMesh(library().sf_model_mesh(self._ptr), owned=False)
+
+    def set_quadrature(self, points: int):
+        _raise(library().sf_model_set_quadrature(self._ptr, points))
+
+    def fix_node(self, node: int):
+        _raise(library().sf_model_fix_node(self._ptr, node))
+
+    def add_nodal_force(self, node: int, force):
+        _raise(library().sf_model_add_nodal_force(self._ptr, node, (C.c_double * 3)(*force)))
+
+    def prescribe_displacement(self, node: int, axis: int, value: float):
+        _raise(library().sf_model_prescribe_displacement(self._ptr, node, axis, value))
+
+    def add_traction(self, face, traction):
+        _raise(library().sf_model_add_traction(self._ptr, (C.c_uint32 * 8)(*[int(v) for v in face]),
+                                               (C.c_double * 3)(*traction)))
+
+    def set_shell_core(self, shell_depth: float, core_youngs: float, core_poisson: float):
+        _raise(library().sf_model_set_shell_core(self._ptr, shell_depth, core_youngs, core_poisson))
+
+    def element_stiffness(self, ctx: Context, first: int, count: int) -> np.ndarray:
+        """element.rs:98-133 on the device: (count, 30, 30)."""
+        out = np.empty((count, 30, 30), dtype=np.float64)
+        _raise(library().sf_model_element_stiffness(ctx._ptr, self._ptr, first, count, _ptr(out, C.c_double)))
+        return out
+
+    def free_problem(self):
+        """(free_map, rhs, block_starts, block_columns, rigid_modes) on the host."""
+        lib = library()
+        nf, nb = C.c_uint64(), C.c_uint64()
+        _raise(lib.sf_model_free_problem(self._ptr, C.byref(nf), C.byref(nb), None, None, None, None, None))
+        nn = self.mesh().nodes().shape[0]
+        free_map = np.empty(nn, dtype=np.uint32)
+        rhs = np.empty(3 * nf.value)
+        starts = np.empty(nf.value + 1, dtype=np.int64)
+        cols = np.empty(nb.value, dtype=np.uint32)
+        modes = np.empty(18 * nf.value)
+        _raise(lib.sf_model_free_problem(self._ptr, None, None, _ptr(free_map, C.c_uint32), _ptr(rhs, C.c_double),
+                                         _ptr(starts, C.c_int64), _ptr(cols, C.c_uint32), _ptr(modes, C.c_double)))
+        return free_map, rhs, starts, cols, modes
+
+
+class StepOutcome(enum.IntEnum):
+    """pcg.rs:47-57"""
+    RUNNING = 0
+    CONVERGED = 1
+    CANCELLED = 2
+
+
+@dataclass
+class Progress:
+    iteration: int
+    relative_residual: float
+    at_rounding_floor: bool
+    outcome: StepOutcome
+
+    @classmethod
+    def of(cls, p):
+        return cls(p.iteration, p.relative_residual, bool(p.at_rounding_floor), StepOutcome(p.outcome))
+
+
+class StaticStudy(_Handle):
+    """crates/solidfem-elastic/src/study.rs:178-372"""
+    _destroy = "sf_static_study_destroy"
+
+    @classmethod
+    def new(cls, ctx: Context, model: Model, pcg: PcgOptions = None, multigrid: Optional[MultigridOptions] = None,
+            cancel=None):
+        pcg = pcg or PcgOptions()
+        h = C.c_void_p()
+        po = pcg.c()
+        mo = multigrid.c() if multigrid else None
+        cb = _cancel(cancel)
+        _raise(library().sf_static_study_create(ctx._ptr, model._ptr, C.byref(po), 1 if multigrid else 0,
+                                                C.byref(mo) if mo else None, cb, None, C.byref(h)))
+        s = cls(h)
+        s._ctx = ctx
+        return s
+
+    def step(self, max_iterations: int, cancel=None) -> Progress:
+        p = _capi.Progress()
+        cb = _cancel(cancel)
+        _raise(library().sf_static_study_step(self._ptr, max_iterations, cb, None, C.byref(p)))
+        return Progress.of(p)
+
+    def solve(self, slice_iterations: int = 2**62) -> Progress:
+        while True:
+            p = self.step(slice_iterations)
+            if p.outcome != StepOutcome.RUNNING:
+                return p
+
+    def displacements(self) -> np.ndarray:
+        n = self.report().total_dofs // 3
+        out = np.empty((n, 3))
+        _raise(library().sf_static_study_displacements(self._ptr, _ptr(out, C.c_double)))
+        return out
+
+    def report(self) -> _capi.Report:
+        r = _capi.Report()
+        _raise(library().sf_static_study_report(self._ptr, C.byref(r)))
+        return r
+
+    def matrix(self):
+        """The assembled free stiffness as scalar CSR (row_starts, columns, values)."""
+        lib = library()
+        n, nnz = C.c_uint64(), C.c_uint64()
+        _raise(lib.sf_static_study_matrix(self._ptr, C.byref(n), C.byref(nnz), None, None, None))
+        rs = np.empty(n.value + 1, dtype=np.int64)
+        cols = np.empty(nnz.value, dtype=np.uint32)
+        vals = np.empty(nnz.value)
+        _raise(lib.sf_static_study_matrix(self._ptr, None, None, _ptr(rs, C.c_int64), _ptr(cols, C.c_uint32),
+                                          _ptr(vals, C.c_double)))
+        return rs, cols, vals
+
+    def rhs(self) -> np.ndarray:
+        out = np.empty(self.report().free_dofs)
+        _raise(library().sf_static_study_rhs(self._ptr, _ptr(out, C.c_double)))
+        return out
+
+    def system(self) -> "CsrMatrix":
+        """The study's system matrix on the device (owned by the study)."""
+        m = CsrMatrix(library().sf_static_study_system(self._ptr))
+        m._owned, m._parent, m._ctx, m.size = False, self, self._ctx, self.report().free_dofs
+        return m
+
+    def multigrid(self) -> Optional["Multigrid"]:
+        h = library().sf_static_study_multigrid(self._ptr)
+        if not h:
+            return None
+        m = Multigrid(h, owned=False)
+        m._parent = self
+        return m
+
+
+class StudyBuilder(_Handle):
+    """crates/solidfem-elastic/src/study.rs:389-656"""
+    _destroy = "sf_study_builder_destroy"
+
+    @classmethod
+    def new(cls, ctx: Context, model: Model, pcg: PcgOptions = None, multigrid: Optional[MultigridOptions] = None):
+        pcg = pcg or PcgOptions()
+        h = C.c_void_p()
+        po = pcg.c()
+        mo = multigrid.c() if multigrid else None
+        _raise(library().sf_study_builder_create(ctx._ptr, model._ptr, C.byref(po), 1 if multigrid else 0,
+                                                 C.byref(mo) if mo else None, C.byref(h)))
+        b = cls(h)
+        b._ctx = ctx
+        return b
+
+    def step(self, max_elements: int, cancel=None):
+        p = _capi.BuildProgress()
+        cb = _cancel(cancel)
+        _raise(library().sf_study_builder_step(self._ptr, max_elements, cb, None, C.byref(p)))
+        return p
+
+    def finish(self) -> StaticStudy:
+        h = C.c_void_p()
+        _raise(library().sf_study_builder_finish(self._ptr, C.byref(h)))
+        self._ptr = None  # consumed
+        s = StaticStudy(h)
+        s._ctx = self._ctx
+        return s
+
+
+class CsrMatrix(_Handle):
+    """crates/solidfem-solve/src/csr.rs:6-168 (on the device when ctx is given)."""
+    _destroy = "sf_csr_destroy"
+    _owned = True
+
+    def close(self):
+        if self._owned:
+            super().close()
+
+    @classmethod
+    def new(cls, ctx: Optional[Context], row_starts, columns, values):
+        rs = np.ascontiguousarray(row_starts, dtype=np.int64)
+        cols = np.ascontiguousarray(columns, dtype=np.uint32)
+        vals = np.ascontiguousarray(values, dtype=np.float64)
+        h = C.c_void_p()
+        _raise(library().sf_csr_create(ctx._ptr if ctx else None, len(rs) - 1, _ptr(rs, C.c_int64),
+                                       _ptr(cols, C.c_uint32), _ptr(vals, C.c_double), C.byref(h)))
+        m = cls(h)
+        m._ctx, m.size = ctx, len(rs) - 1
+        return m
+
+    def multiply(self, x) -> np.ndarray:
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.size)
+        _raise(library().sf_csr_multiply(self._ptr, _ptr(xx, C.c_double), _ptr(y, C.c_double)))
+        return y
+
+    def multiply_rounding_bound(self, x, dot_order=DotOrder.REFERENCE) -> float:
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        b = C.c_double()
+        _raise(library().sf_csr_rounding_bound(self._ptr, _ptr(xx, C.c_double), int(dot_order), C.byref(b)))
+        return b.value
+
+
+class _Levels:
+    """Shared level views of MultigridSetup (finished) and Multigrid."""
+    _prefix = ""
+
+    def _fn(self, name):
+        return getattr(library(), self._prefix + name)
+
+    def level_matrix(self, level: int, prolongator: bool = False):
+        rows, bc, rb, cb, nb = (C.c_uint64() for _ in range(5))
+        f = self._fn("level_matrix")
+        _raise(f(self._ptr, level, int(prolongator), C.byref(rows), C.byref(bc), C.byref(rb), C.byref(cb), C.byref(nb),
+                 None, None, None))
+        starts = np.empty(rows.value + 1, dtype=np.int64)
+        cols = np.empty(nb.value, dtype=np.uint32)
+        vals = np.empty(nb.value * rb.value * cb.value)
+        _raise(f(self._ptr, level, int(prolongator), None, None, None, None, None, _ptr(starts, C.c_int64),
+                 _ptr(cols, C.c_uint32), _ptr(vals, C.c_double)))
+        return dict(rows=rows.value, block_columns=bc.value, row_block=rb.value, col_block=cb.value, starts=starts,
+                    columns=cols, values=vals)
+
+    def level_inverse_diagonal(self, level: int) -> np.ndarray:
+        m = self.level_matrix(level)
+        out = np.empty(m["rows"] * m["row_block"] ** 2)
+        _raise(self._fn("level_inverse_diagonal")(self._ptr, level, _ptr(out, C.c_double)))
+        return out
+
+    def smoother_spectrum(self, level: int):
+        lo, hi = C.c_double(), C.c_double()
+        _raise(self._fn("smoother_spectrum")(self._ptr, level, C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+    def aggregates(self, level: int) -> np.ndarray:
+        n = C.c_uint64()
+        f = self._fn("aggregates")
+        _raise(f(self._ptr, level, None, C.byref(n)))
+        out = np.empty(n.value, dtype=np.uint32)
+        _raise(f(self._ptr, level, _ptr(out, C.c_uint32), None))
+        return out
+
+
+class MultigridSetup(_Handle, _Levels):
+    """crates/solidfem-solve/src/multigrid/setup.rs:81-240"""
+    _destroy = "sf_multigrid_setup_destroy"
+    _prefix = "sf_multigrid_setup_"
+
+    @classmethod
+    def new(cls, ctx: Optional[Context], matrix: CsrMatrix, near_null, positions, options: MultigridOptions = None):
+        options = options or MultigridOptions()
+        nn = np.ascontiguousarray(near_null, dtype=np.float64).ravel()
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        o = options.c()
+        h = C.c_void_p()
+        _raise(library().sf_multigrid_setup_create(ctx._ptr if ctx else None, matrix._ptr, _ptr(nn, C.c_double), len(nn),
+                                                   _ptr(pos, C.c_double), len(pos), C.byref(o), C.byref(h)))
+        s = cls(h)
+        s._ctx = ctx
+        return s
+
+    def step(self, budget: int, cancel=None) -> bool:
+        done = C.c_int32()
+        cb = _cancel(cancel)
+        _raise(library().sf_multigrid_setup_step(self._ptr, budget, cb, None, C.byref(done)))
+        return bool(done.value)
+
+    def level_count(self) -> int:
+        n = C.c_uint64()
+        _raise(library().sf_multigrid_setup_level_count(self._ptr, C.byref(n)))
+        return n.value
+
+    def coarse_factor(self) -> Optional[np.ndarray]:
+        n = C.c_uint64()
+        _raise(library().sf_multigrid_setup_coarse_factor(self._ptr, C.byref(n), None))
+        if n.value == 0:
+            return None
+        out = np.empty((n.value, n.value))
+        _raise(library().sf_multigrid_setup_coarse_factor(self._ptr, C.byref(n), _ptr(out, C.c_double)))
+        return out
+
+    def finish(self) -> "Multigrid":
+        h = C.c_void_p()
+        _raise(library().sf_multigrid_setup_finish(self._ptr, C.byref(h)))
+        m = Multigrid(h)
+        m._ctx = self._ctx
+        return m
+
+
+class Multigrid(_Handle, _Levels):
+    """crates/solidfem-solve/src/multigrid/cycle.rs:253-458"""
+    _destroy = "sf_multigrid_destroy"
+    _prefix = "sf_multigrid_"
+
+    def __init__(self, ptr, owned=True):
+        super().__init__(ptr)
+        self._owned = owned
+
+    def close(self):
+        if getattr(self, "_owned", True):
+            super().close()
+
+    def stats(self) -> _capi.MultigridStats:
+        s = _capi.MultigridStats()
+        _raise(library().sf_multigrid_stats_get(self._ptr, C.byref(s)))
+        return s
+
+    def precision(self) -> Precision:
+        return Precision(library().sf_multigrid_precision(self._ptr))
+
+    def level_sizes(self):
+        n = C.c_uint64()
+        out = np.zeros(3 * 64, dtype=np.uint64)
+        _raise(library().sf_multigrid_level_sizes(self._ptr, _ptr(out, C.c_uint64), 64, C.byref(n)))
+        return [tuple(int(v) for v in out[3 * i:3 * i + 3]) for i in range(n.value)]
+
+    def with_precision(self, precision: Precision) -> "Multigrid":
+        """The same hierarchy in another precision (cycle.rs:265-308 over the same levels)."""
+        h = C.c_void_p()
+        _raise(library().sf_multigrid_with_precision(self._ptr, int(precision), C.byref(h)))
+        m = Multigrid(h)
+        m._parent = self
+        return m
+
+    def cycle_bytes(self) -> float:
+        return library().sf_multigrid_cycle_bytes(self._ptr)
+
+    def apply(self, residual) -> np.ndarray:
+        r = np.ascontiguousarray(residual, dtype=np.float64)
+        z = np.empty_like(r)
+        _raise(library().sf_multigrid_apply(self._ptr, _ptr(r, C.c_double), _ptr(z, C.c_double)))
+        return z
+
+    def smooth(self, level: int, b) -> np.ndarray:
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(bb)
+        _raise(library().sf_multigrid_smooth(self._ptr, level, _ptr(bb, C.c_double), _ptr(x, C.c_double)))
+        return x
+
+    def solve_coarsest(self, values) -> np.ndarray:
+        v = np.array(values, dtype=np.float64)
+        _raise(library().sf_multigrid_solve_coarsest(self._ptr, _ptr(v, C.c_double), len(v)))
+        return v
+
+
+class Pcg(_Handle):
+    """crates/solidfem-solve/src/pcg.rs:113-283, preconditioned by a Multigrid or by Jacobi."""
+    _destroy = "sf_pcg_destroy"
+
+    @classmethod
+    def new(cls, ctx: Context, matrix: CsrMatrix, preconditioner: Optional[Multigrid], rhs,
+            options: PcgOptions = None):
+        options = options or PcgOptions()
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        o = options.c()
+        h = C.c_void_p()
+        _raise(library().sf_pcg_create(ctx._ptr, matrix._ptr, preconditioner._ptr if preconditioner else None,
+                                       _ptr(b, C.c_double), C.byref(o), C.byref(h)))
+        p = cls(h)
+        p._keep = (ctx, matrix, preconditioner)
+        return p
+
+    def step(self, max_iterations: int, cancel=None) -> Progress:
+        p = _capi.Progress()
+        cb = _cancel(cancel)
+        _raise(library().sf_pcg_step(self._ptr, max_iterations, cb, None, C.byref(p)))
+        return Progress.of(p)
+
+    def restart(self, rhs=None):
+        """Pcg::new again on the resident buffers (optionally a new right-hand side)."""
+        if rhs is None:
+            _raise(library().sf_pcg_restart(self._ptr, None))
+        else:
+            b = np.ascontiguousarray(rhs, dtype=np.float64)
+            _raise(library().sf_pcg_restart(self._ptr, _ptr(b, C.c_double)))
+
+    def estimate(self) -> np.ndarray:
+        n = library().sf_pcg_size(self._ptr)
+        out = np.empty(n)
+        _raise(library().sf_pcg_estimate(self._ptr, _ptr(out, C.c_double)))
+        return out
+
+    def progress(self) -> Progress:
+        p = _capi.Progress()
+        _raise(library().sf_pcg_progress(self._ptr, C.byref(p)))
+        return Progress.of(p)
+
+    def is_converged(self) -> bool:
+        return bool(library().sf_pcg_is_converged(self._ptr))
+
+
+class Kernel(enum.IntEnum):
+    FINE_CHEBYSHEV = 0
+    SYSTEM_PRODUCT = 1
+    FINE_RESIDUAL = 2
+
+
+def profile_enable(kernel: Kernel, enable: bool = True):
+    _raise(library().sf_profile_enable(int(kernel), 1 if enable else 0))
+
+
+def profile_reset():
+    _raise(library().sf_profile_reset())
+
+
+def profile_read(kernel: Kernel):
+    """(launches, summed seconds, summed algorithmic bytes) since the last reset."""
+    n, t, b = C.c_uint64(), C.c_double(), C.c_double()
+    _raise(library().sf_profile_read(int(kernel), C.byref(n), C.byref(t), C.byref(b)))
+    return n.value, t.value, b.value

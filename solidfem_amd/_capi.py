@@ -1,0 +1,199 @@
+"""ctypes declarations of include/solidfem_hip.h and the loader of libsolidfem_hip.so.
+
+The library is loaded from this package's directory only. There is no fallback: a missing
+library raises at import of the binding, so nothing can silently run elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIBRARY = Path(__file__).resolve().parent / "libsolidfem_hip.so"
+
+u32, u64, i32, i64, f64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_int64, C.c_double
+P = C.POINTER
+vp = C.c_void_p
+
+CANCEL_FN = C.CFUNCTYPE(i32, vp)
+
+
+class Progress(C.Structure):
+    _fields_ = [("iteration", u64), ("relative_residual", f64), ("at_rounding_floor", i32), ("outcome", i32)]
+
+
+class PcgOptions(C.Structure):
+    _fields_ = [("tolerance", f64), ("max_iterations", u64), ("dot_order", i32), ("reserved", i32)]
+
+
+class MultigridOptions(C.Structure):
+    _fields_ = [
+        ("neighbour_reach", f64),
+        ("coarse_dofs", u64),
+        ("max_levels", u64),
+        ("smoother_degree", u64),
+        ("smoother_range", f64),
+        ("precision", i32),
+        ("threads", i32),
+    ]
+
+
+class MultigridStats(C.Structure):
+    _fields_ = [("levels", u64), ("coarse_dofs", u64), ("operator_complexity", f64), ("memory_bytes", u64)]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("total_dofs", u64),
+        ("free_dofs", u64),
+        ("nonzeros", u64),
+        ("at_rounding_floor", i32),
+        ("has_multigrid", i32),
+        ("iterations", u64),
+        ("relative_residual", f64),
+        ("assembly_seconds", f64),
+        ("preconditioner_seconds", f64),
+        ("multigrid", MultigridStats),
+        ("solve_seconds", f64),
+    ]
+
+
+class BuildProgress(C.Structure):
+    _fields_ = [("elements_done", u64), ("elements_total", u64), ("setup_finished", i32), ("reserved", i32)]
+
+
+class Workload(C.Structure):
+    _fields_ = [
+        ("origin", f64 * 3),
+        ("size", f64 * 3),
+        ("divisions", i32 * 3),
+        ("jitter_fraction", f64),
+        ("jitter_seed", u64),
+        ("clamp_axis", i32),
+        ("clamp_value", f64),
+        ("load_axis", i32),
+        ("load_value", f64),
+        ("traction_kind", i32),
+        ("traction", f64 * 3),
+        ("traction_scale", f64),
+        ("traction_seed", u64),
+        ("load_case", i32),
+        ("axis_u", f64),
+        ("axis_v", f64),
+        ("material_kind", i32),
+        ("youngs", f64),
+        ("poisson", f64),
+        ("core_youngs", f64),
+        ("core_poisson", f64),
+        ("shell_depth", f64),
+        ("quadrature_points", i32),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "sf_last_error": (C.c_char_p, []),
+    "sf_last_error_payload": (None, [P(i64), P(i64), P(f64)]),
+    "sf_version": (C.c_char_p, []),
+    "sf_device_count": (i32, [P(i32)]),
+    "sf_context_create": (i32, [i32, P(vp)]),
+    "sf_context_destroy": (None, [vp]),
+    "sf_context_synchronize": (i32, [vp]),
+    "sf_tet_mesh_create": (i32, [P(f64), u64, P(u32), u64, P(vp)]),
+    "sf_box_tet_mesh": (i32, [P(f64), P(f64), P(i32), P(vp)]),
+    "sf_tet_mesh_destroy": (None, [vp]),
+    "sf_tet_mesh_node_count": (u64, [vp]),
+    "sf_tet_mesh_tet_count": (u64, [vp]),
+    "sf_quad_mesh_from_linear": (i32, [vp, P(vp)]),
+    "sf_quad_mesh_destroy": (None, [vp]),
+    "sf_quad_mesh_node_count": (u64, [vp]),
+    "sf_quad_mesh_tet_count": (u64, [vp]),
+    "sf_quad_mesh_vertex_count": (u64, [vp]),
+    "sf_quad_mesh_nodes": (None, [vp, P(f64)]),
+    "sf_quad_mesh_tets": (None, [vp, P(u32)]),
+    "sf_quad_mesh_boundary_faces": (u64, [vp, P(u32)]),
+    "sf_model_create": (i32, [vp, f64, f64, P(vp)]),
+    "sf_model_set_shell_core": (i32, [vp, f64, f64, f64]),
+    "sf_model_set_quadrature": (i32, [vp, i32]),
+    "sf_model_fix_node": (i32, [vp, u32]),
+    "sf_model_add_nodal_force": (i32, [vp, u32, P(f64)]),
+    "sf_model_prescribe_displacement": (i32, [vp, u32, u32, f64]),
+    "sf_model_add_traction": (i32, [vp, P(u32), P(f64)]),
+    "sf_model_from_workload": (i32, [P(Workload), P(vp)]),
+    "sf_model_destroy": (None, [vp]),
+    "sf_model_mesh": (vp, [vp]),
+    "sf_model_element_stiffness": (i32, [vp, vp, u64, u64, P(f64)]),
+    "sf_model_free_problem": (i32, [vp, P(u64), P(u64), P(u32), P(f64), P(i64), P(u32), P(f64)]),
+    "sf_study_builder_create": (i32, [vp, vp, P(PcgOptions), i32, P(MultigridOptions), P(vp)]),
+    "sf_study_builder_step": (i32, [vp, u64, CANCEL_FN, vp, P(BuildProgress)]),
+    "sf_study_builder_finish": (i32, [vp, P(vp)]),
+    "sf_study_builder_destroy": (None, [vp]),
+    "sf_static_study_create": (i32, [vp, vp, P(PcgOptions), i32, P(MultigridOptions), CANCEL_FN, vp, P(vp)]),
+    "sf_static_study_step": (i32, [vp, u64, CANCEL_FN, vp, P(Progress)]),
+    "sf_static_study_displacements": (i32, [vp, P(f64)]),
+    "sf_static_study_report": (i32, [vp, P(Report)]),
+    "sf_static_study_multigrid": (vp, [vp]),
+    "sf_static_study_pcg": (vp, [vp]),
+    "sf_static_study_matrix": (i32, [vp, P(u64), P(u64), P(i64), P(u32), P(f64)]),
+    "sf_static_study_rhs": (i32, [vp, P(f64)]),
+    "sf_static_study_destroy": (None, [vp]),
+    "sf_csr_create": (i32, [vp, u64, P(i64), P(u32), P(f64), P(vp)]),
+    "sf_csr_destroy": (None, [vp]),
+    "sf_csr_multiply": (i32, [vp, P(f64), P(f64)]),
+    "sf_csr_rounding_bound": (i32, [vp, P(f64), i32, P(f64)]),
+    "sf_multigrid_setup_create": (i32, [vp, vp, P(f64), u64, P(f64), u64, P(MultigridOptions), P(vp)]),
+    "sf_multigrid_setup_step": (i32, [vp, u64, CANCEL_FN, vp, P(i32)]),
+    "sf_multigrid_setup_finish": (i32, [vp, P(vp)]),
+    "sf_multigrid_setup_destroy": (None, [vp]),
+    "sf_multigrid_setup_level_count": (i32, [vp, P(u64)]),
+    "sf_multigrid_setup_level_matrix": (i32, [vp, u64, i32, P(u64), P(u64), P(u64), P(u64), P(u64), P(i64), P(u32), P(f64)]),
+    "sf_multigrid_setup_level_inverse_diagonal": (i32, [vp, u64, P(f64)]),
+    "sf_multigrid_setup_smoother_spectrum": (i32, [vp, u64, P(f64), P(f64)]),
+    "sf_multigrid_setup_aggregates": (i32, [vp, u64, P(u32), P(u64)]),
+    "sf_multigrid_setup_coarse_factor": (i32, [vp, P(u64), P(f64)]),
+    "sf_multigrid_destroy": (None, [vp]),
+    "sf_multigrid_stats_get": (i32, [vp, P(MultigridStats)]),
+    "sf_multigrid_precision": (i32, [vp]),
+    "sf_multigrid_apply": (i32, [vp, P(f64), P(f64)]),
+    "sf_multigrid_smooth": (i32, [vp, u64, P(f64), P(f64)]),
+    "sf_multigrid_level_sizes": (i32, [vp, P(u64), u64, P(u64)]),
+    "sf_multigrid_level_matrix": (i32, [vp, u64, i32, P(u64), P(u64), P(u64), P(u64), P(u64), P(i64), P(u32), P(f64)]),
+    "sf_multigrid_level_inverse_diagonal": (i32, [vp, u64, P(f64)]),
+    "sf_multigrid_smoother_spectrum": (i32, [vp, u64, P(f64), P(f64)]),
+    "sf_multigrid_aggregates": (i32, [vp, u64, P(u32), P(u64)]),
+    "sf_multigrid_solve_coarsest": (i32, [vp, P(f64), u64]),
+    "sf_pcg_create": (i32, [vp, vp, vp, P(f64), P(PcgOptions), P(vp)]),
+    "sf_pcg_step": (i32, [vp, u64, CANCEL_FN, vp, P(Progress)]),
+    "sf_pcg_estimate": (i32, [vp, P(f64)]),
+    "sf_pcg_progress": (i32, [vp, P(Progress)]),
+    "sf_pcg_is_converged": (i32, [vp]),
+    "sf_pcg_size": (u64, [vp]),
+    "sf_pcg_destroy": (None, [vp]),
+    "sf_static_study_system": (vp, [vp]),
+    "sf_multigrid_with_precision": (i32, [vp, i32, P(vp)]),
+    "sf_multigrid_cycle_bytes": (f64, [vp]),
+    "sf_pcg_restart": (i32, [vp, P(f64)]),
+    "sf_profile_enable": (i32, [i32, i32]),
+    "sf_profile_reset": (i32, []),
+    "sf_profile_read": (i32, [i32, P(u64), P(f64), P(f64)]),
+    "sf_stream_copy": (i32, [vp, u64, u32, P(f64)]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """The HIP library, loaded once. Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIBRARY.exists():
+            raise ImportError(
+                f"{LIBRARY} is missing: build it with __graft_entry__.build() or python solidfem_amd/build.py"
+            )
+        lib = C.CDLL(os.fspath(LIBRARY), mode=C.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib

@@ -1,0 +1,938 @@
+// The C ABI of the solve path (include/solidfem_hip.h): thin wrappers over the host and device
+// objects. No exception crosses this boundary; every failure becomes an sf_status with its
+// message and payload kept per thread.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "sf_device.hpp"
+
+using namespace sf;
+
+namespace {
+
+thread_local std::string g_message;
+thread_local int64_t g_a = 0, g_b = 0;
+thread_local double g_value = 0;
+
+sf_status fail(sf_status s, const std::string &m, int64_t a = 0, int64_t b = 0, double v = 0) {
+    g_message = m;
+    g_a = a;
+    g_b = b;
+    g_value = v;
+    return s;
+}
+
+template <typename F>
+sf_status guard(F &&f) {
+    try {
+        f();
+        return SF_OK;
+    } catch (const Error &e) {
+        return fail(e.status, e.what(), e.a, e.b, e.value);
+    } catch (const std::bad_alloc &) {
+        return fail(SF_ERR_OUT_OF_MEMORY, "out of host memory");
+    } catch (const std::exception &e) {
+        return fail(SF_ERR_INTERNAL, e.what());
+    } catch (...) {
+        return fail(SF_ERR_INTERNAL, "unknown failure");
+    }
+}
+
+double seconds() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+SetupOptions setup_options(const sf_multigrid_options *o) {
+    SetupOptions s;
+    if (!o) return s;
+    s.reach = o->neighbour_reach;
+    s.coarse_dofs = (size_t)o->coarse_dofs;
+    s.max_levels = (size_t)o->max_levels;
+    s.degree = (size_t)o->smoother_degree;
+    s.range = o->smoother_range;
+    s.precision = o->precision;
+    if (s.precision < 0 || s.precision > 2) throw Error(SF_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (s.degree < 1) throw Error(SF_ERR_INVALID_ARGUMENT, "the smoother degree must be at least one");
+    return s;
+}
+
+// Scalar CSR -> block rows of 3 when every node's three rows share a pattern of whole blocks
+// (BlockCsr::from_nodal, crates/solidfem-solve/src/multigrid/block.rs:51-99). Returns the first
+// offending scalar row, or -1 when the matrix is nodal.
+int64_t to_nodal(size_t size, const int64_t *rs, const u32 *cols, const double *vals, HostBsr<double> &out) {
+    if (size % 3 != 0) return (int64_t)size;
+    const size_t rows = size / 3;
+    out = HostBsr<double>();
+    out.rows = out.cols = rows;
+    out.rb = out.cb = 3;
+    out.columns.reserve((size_t)rs[size] / 9);
+    out.values.reserve((size_t)rs[size]);
+    for (size_t node = 0; node < rows; ++node) {
+        const int64_t s0 = rs[3 * node], e0 = rs[3 * node + 1];
+        const int64_t len = e0 - s0;
+        for (int r = 1; r < 3; ++r) {
+            const int64_t sr = rs[3 * node + r], er = rs[3 * node + r + 1];
+            if (er - sr != len || !std::equal(cols + s0, cols + e0, cols + sr)) return (int64_t)(3 * node + r);
+        }
+        if (len % 3 != 0) return (int64_t)(3 * node);
+        for (int64_t at = 0; at < len; at += 3) {
+            const u32 c = cols[s0 + at];
+            if (c % 3 != 0 || cols[s0 + at + 1] != c + 1 || cols[s0 + at + 2] != c + 2) return (int64_t)(3 * node);
+            out.columns.push_back(c / 3);
+        }
+        for (int64_t b = 0; b < len / 3; ++b)
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q) out.values.push_back(vals[rs[3 * node + r] + 3 * b + q]);
+        out.starts.push_back((int64_t)out.columns.size());
+    }
+    return -1;
+}
+
+void to_scalar_csr(const HostBsr<double> &m, int64_t *row_starts, u32 *columns, double *values) {
+    // the reference's CsrMatrix view: every scalar row of a block row lists the same columns
+    size_t at = 0, row = 0;
+    if (row_starts) row_starts[0] = 0;
+    for (size_t br = 0; br < m.rows; ++br)
+        for (size_t p = 0; p < m.rb; ++p) {
+            for (int64_t b = m.starts[br]; b < m.starts[br + 1]; ++b)
+                for (size_t q = 0; q < m.cb; ++q) {
+                    if (columns) columns[at] = (u32)(m.columns[(size_t)b] * m.cb + q);
+                    if (values) values[at] = m.values[(size_t)b * m.rb * m.cb + p * m.cb + q];
+                    ++at;
+                }
+            ++row;
+            if (row_starts) row_starts[row] = (int64_t)at;
+        }
+}
+
+}  // namespace
+
+// ---- handle types -------------------------------------------------------------------------------
+
+struct sf_context {
+    std::unique_ptr<Context> ctx;
+};
+struct sf_tet_mesh {
+    TetMesh mesh;
+};
+struct sf_quad_mesh {
+    QuadMesh mesh;
+};
+struct sf_model {
+    Model model;
+    sf_quad_mesh view;  // the mesh as a handle
+};
+struct sf_csr {
+    sf_context *ctx = nullptr;
+    DeviceSystem system;
+};
+struct sf_multigrid {
+    sf_context *ctx = nullptr;
+    std::unique_ptr<DeviceMultigrid> mg;
+};
+struct sf_multigrid_setup {
+    sf_context *ctx = nullptr;
+    std::unique_ptr<MultigridSetup> setup;
+};
+struct sf_pcg {
+    std::unique_ptr<DevicePcg> pcg;
+};
+
+struct sf_static_study {
+    sf_context *ctx = nullptr;
+    Model model;
+    FreeProblem problem;
+    sf_csr system;
+    sf_multigrid mg;
+    std::unique_ptr<sf_pcg> pcg;
+    sf_report report{};
+};
+
+struct sf_study_builder {
+    sf_context *ctx = nullptr;
+    Model model;
+    FreeProblem problem;
+    AssemblyPattern pattern;
+    DeviceAssembly assembly;
+    std::unique_ptr<sf_static_study> study;  // the system being assembled
+    sf_pcg_options pcg{};
+    bool use_multigrid = false;
+    SetupOptions mg_options;
+    size_t next = 0;
+    bool prescribed_applied = false;
+    std::unique_ptr<MultigridSetup> setup;
+    bool setup_finished = false;
+    double assembly_seconds = 0, preconditioner_seconds = 0;
+};
+
+namespace {
+
+const std::vector<HostLevel> &setup_levels(const sf_multigrid_setup *setup) {
+    if (!setup->setup->finished()) throw Error(SF_ERR_MG_NOT_FINISHED, "the multigrid setup has not finished");
+    return setup->setup->levels();
+}
+
+void level_matrix_view(const std::vector<HostLevel> &h, bool single, uint64_t level, int32_t prolongator, uint64_t *rows,
+                       uint64_t *block_columns, uint64_t *row_block, uint64_t *col_block, uint64_t *blocks, int64_t *starts,
+                       uint32_t *columns, double *values) {
+    if (level >= h.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
+    if (prolongator && !h[level].has_prolongator) throw Error(SF_ERR_INVALID_ARGUMENT, "the coarsest level has no prolongator");
+    const HostBsr<double> &m = prolongator ? h[level].prolongator : h[level].a;
+    if (rows) *rows = m.rows;
+    if (block_columns) *block_columns = m.cols;
+    if (row_block) *row_block = m.rb;
+    if (col_block) *col_block = m.cb;
+    if (blocks) *blocks = m.blocks();
+    if (starts) std::copy(m.starts.begin(), m.starts.end(), starts);
+    if (columns) std::copy(m.columns.begin(), m.columns.end(), columns);
+    if (values)
+        for (size_t i = 0; i < m.values.size(); ++i) values[i] = single ? (double)(float)m.values[i] : m.values[i];
+}
+
+void level_inverse_diagonal_view(const std::vector<HostLevel> &h, bool single, uint64_t level, double *out) {
+    if (level >= h.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
+    const auto &d = h[level].inverse_diagonal;
+    for (size_t i = 0; i < d.size(); ++i) out[i] = single ? (double)(float)d[i] : d[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sf_last_error(void) { return g_message.c_str(); }
+void sf_last_error_payload(int64_t *a, int64_t *b, double *value) {
+    if (a) *a = g_a;
+    if (b) *b = g_b;
+    if (value) *value = g_value;
+}
+const char *sf_version(void) { return "solidfem-hip 0.1.0 (gfx950)"; }
+
+sf_status sf_device_count(int32_t *count) {
+    return guard([&] {
+        int n = 0;
+        SF_HIP(hipGetDeviceCount(&n));
+        *count = n;
+    });
+}
+
+sf_status sf_context_create(int32_t device, sf_context **out) {
+    return guard([&] {
+        auto c = std::make_unique<sf_context>();
+        c->ctx = std::make_unique<Context>(device);
+        *out = c.release();
+    });
+}
+void sf_context_destroy(sf_context *ctx) { delete ctx; }
+sf_status sf_context_synchronize(sf_context *ctx) {
+    return guard([&] { SF_HIP(hipStreamSynchronize(ctx->ctx->stream)); });
+}
+
+// ---- mesh ------------------------------------------------------------------------------------
+
+sf_status sf_tet_mesh_create(const double *positions, uint64_t node_count, const uint32_t *tets, uint64_t tet_count,
+                             sf_tet_mesh **out) {
+    return guard([&] {
+        auto m = std::make_unique<sf_tet_mesh>();
+        m->mesh.positions.resize(node_count);
+        for (size_t n = 0; n < node_count; ++n)
+            for (int a = 0; a < 3; ++a) m->mesh.positions[n][a] = positions[3 * n + a];
+        m->mesh.tets.resize(tet_count);
+        for (size_t t = 0; t < tet_count; ++t)
+            for (int a = 0; a < 4; ++a) m->mesh.tets[t][a] = tets[4 * t + a];
+        validate_tet_mesh(m->mesh);
+        *out = m.release();
+    });
+}
+sf_status sf_box_tet_mesh(const double origin[3], const double size[3], const int32_t divisions[3], sf_tet_mesh **out) {
+    return guard([&] {
+        auto m = std::make_unique<sf_tet_mesh>();
+        m->mesh = make_box_mesh(origin, size, divisions);
+        validate_tet_mesh(m->mesh);
+        *out = m.release();
+    });
+}
+void sf_tet_mesh_destroy(sf_tet_mesh *m) { delete m; }
+uint64_t sf_tet_mesh_node_count(const sf_tet_mesh *m) { return m->mesh.positions.size(); }
+uint64_t sf_tet_mesh_tet_count(const sf_tet_mesh *m) { return m->mesh.tets.size(); }
+sf_status sf_quad_mesh_from_linear(const sf_tet_mesh *mesh, sf_quad_mesh **out) {
+    return guard([&] {
+        auto q = std::make_unique<sf_quad_mesh>();
+        q->mesh = upgrade_to_quadratic(mesh->mesh);
+        *out = q.release();
+    });
+}
+void sf_quad_mesh_destroy(sf_quad_mesh *m) { delete m; }
+uint64_t sf_quad_mesh_node_count(const sf_quad_mesh *m) { return m->mesh.nodes.size(); }
+uint64_t sf_quad_mesh_tet_count(const sf_quad_mesh *m) { return m->mesh.tets.size(); }
+uint64_t sf_quad_mesh_vertex_count(const sf_quad_mesh *m) { return m->mesh.vertex_count; }
+void sf_quad_mesh_nodes(const sf_quad_mesh *m, double *out) {
+    for (size_t n = 0; n < m->mesh.nodes.size(); ++n)
+        for (int a = 0; a < 3; ++a) out[3 * n + a] = m->mesh.nodes[n][a];
+}
+void sf_quad_mesh_tets(const sf_quad_mesh *m, uint32_t *out) {
+    for (size_t t = 0; t < m->mesh.tets.size(); ++t)
+        for (int a = 0; a < 10; ++a) out[10 * t + a] = m->mesh.tets[t][a];
+}
+uint64_t sf_quad_mesh_boundary_faces(const sf_quad_mesh *m, uint32_t *out) {
+    auto faces = m->mesh.boundary_faces();
+    if (out)
+        for (size_t f = 0; f < faces.size(); ++f) {
+            out[8 * f] = faces[f].tet;
+            out[8 * f + 1] = faces[f].face;
+            for (int s = 0; s < 6; ++s) out[8 * f + 2 + s] = faces[f].nodes[s];
+        }
+    return faces.size();
+}
+
+// ---- model -----------------------------------------------------------------------------------
+
+sf_status sf_model_create(const sf_quad_mesh *mesh, double e, double nu, sf_model **out) {
+    return guard([&] {
+        auto m = std::make_unique<sf_model>();
+        m->model.mesh = mesh->mesh;
+        m->model.field.kind = SF_MATERIAL_UNIFORM;
+        m->model.field.e = e;
+        m->model.field.nu = nu;
+        m->view.mesh = m->model.mesh;
+        *out = m.release();
+    });
+}
+sf_status sf_model_set_shell_core(sf_model *model, double depth, double core_e, double core_nu) {
+    return guard([&] {
+        auto &f = model->model.field;
+        const auto &nodes = model->model.mesh.nodes;
+        f.kind = SF_MATERIAL_SHELL_CORE;
+        f.shell_depth = depth;
+        f.core_e = core_e;
+        f.core_nu = core_nu;
+        for (int a = 0; a < 3; ++a) {
+            f.lo[a] = 1e300;
+            f.hi[a] = -1e300;
+        }
+        for (const auto &p : nodes)
+            for (int a = 0; a < 3; ++a) {
+                f.lo[a] = std::min(f.lo[a], p[a]);
+                f.hi[a] = std::max(f.hi[a], p[a]);
+            }
+        int longest = 0;
+        for (int a = 1; a < 3; ++a)
+            if (f.hi[a] - f.lo[a] > f.hi[longest] - f.lo[longest]) longest = a;
+        f.transverse[0] = (longest + 1) % 3;
+        f.transverse[1] = (longest + 2) % 3;
+    });
+}
+sf_status sf_model_set_quadrature(sf_model *model, int32_t points) {
+    return guard([&] {
+        if (points != 4 && points != 14) throw Error(SF_ERR_INVALID_ARGUMENT, "the rule has 4 or 14 points");
+        model->model.quadrature = points;
+    });
+}
+sf_status sf_model_fix_node(sf_model *model, uint32_t node) {
+    return guard([&] { model->model.fixed.insert(node); });
+}
+sf_status sf_model_add_nodal_force(sf_model *model, uint32_t node, const double force[3]) {
+    return guard([&] { model->model.nodal_forces.push_back({node, {force[0], force[1], force[2]}}); });
+}
+sf_status sf_model_prescribe_displacement(sf_model *model, uint32_t node, uint32_t axis, double value) {
+    return guard([&] { model->model.prescribed[{node, axis}] = value; });
+}
+sf_status sf_model_add_traction(sf_model *model, const uint32_t face[8], const double t[3]) {
+    return guard([&] {
+        Face f;
+        f.tet = face[0];
+        f.face = face[1];
+        for (int s = 0; s < 6; ++s) f.nodes[s] = face[2 + s];
+        model->model.tractions.push_back({f, {t[0], t[1], t[2]}});
+    });
+}
+sf_status sf_model_from_workload(const sf_workload *w, sf_model **out) {
+    return guard([&] {
+        auto m = std::make_unique<sf_model>();
+        m->model = model_from_workload(*w);
+        m->view.mesh = m->model.mesh;
+        *out = m.release();
+    });
+}
+void sf_model_destroy(sf_model *m) { delete m; }
+const sf_quad_mesh *sf_model_mesh(const sf_model *m) { return &m->view; }
+
+sf_status sf_model_element_stiffness(sf_context *ctx, const sf_model *model, uint64_t first, uint64_t count, double *ke) {
+    return guard([&] {
+        if (first + count > model->model.mesh.tets.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "element range out of bounds");
+        DeviceAssembly a;
+        AssemblyPattern empty;
+        empty.contribution_starts.assign(1, 0);
+        a.prepare(model->model, empty, ctx->ctx->stream);
+        a.element_matrices(first, count, ke, ctx->ctx->stream);
+        a.check_fault(ctx->ctx->stream);
+    });
+}
+
+// ---- study -----------------------------------------------------------------------------------
+
+sf_status sf_study_builder_create(sf_context *ctx, const sf_model *model, const sf_pcg_options *pcg, int32_t use_multigrid,
+                                  const sf_multigrid_options *mg, sf_study_builder **out) {
+    return guard([&] {
+        const double t0 = seconds();
+        auto b = std::make_unique<sf_study_builder>();
+        b->ctx = ctx;
+        b->model = model->model;
+        b->pcg = pcg ? *pcg : sf_pcg_options{1e-9, 0, SF_DOT_TREE, 0};
+        b->use_multigrid = use_multigrid != 0;
+        if (b->use_multigrid) b->mg_options = setup_options(mg);
+        // StudyBuilder::new, study.rs:419-474
+        b->problem = number_and_load(b->model);
+        b->pattern = build_pattern(b->model.mesh, b->problem.free, b->problem.free_count);
+        b->study = std::make_unique<sf_static_study>();
+        b->study->ctx = ctx;
+        auto &sys = b->study->system.system;
+        b->study->system.ctx = ctx;
+        sys.size = 3 * b->problem.free_count;
+        sys.rb = 3;
+        sys.host = b->pattern.pattern;
+        hipStream_t s = ctx->ctx->stream;
+        sys.values.resize(std::max<size_t>(sys.host.blocks() * 9, 1));
+        SF_HIP(hipMemsetAsync(sys.values.ptr, 0, sys.host.blocks() * 9 * sizeof(double), s));
+        b->assembly.prepare(b->model, b->pattern, s);
+        b->assembly_seconds = seconds() - t0;
+        *out = b.release();
+    });
+}
+
+sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_cancel_fn cancel_fn, void *user,
+                                sf_build_progress *progress) {
+    return guard([&] {
+        const Cancel cancel{cancel_fn, user};
+        hipStream_t s = b->ctx->ctx->stream;
+        const size_t total = b->model.mesh.tets.size();
+        const double t0 = seconds();
+        auto &sys = b->study->system.system;
+        // study.rs:483-500: element stiffness in element order; the cancel check is read before
+        // each chunk of elements
+        const size_t end = std::min<u64>(total, b->next + std::min<u64>(max_elements, total));
+        constexpr size_t POLL = 1u << 16;
+        while (b->next < end) {
+            if (cancel()) throw Error(SF_ERR_MODEL_CANCELLED, "the study build was cancelled");
+            const size_t stop = std::min(end, b->next + POLL);
+            b->assembly.add_elements(b->next, stop, sys.values.ptr, sys.host.blocks(), s);
+            b->next = stop;
+        }
+        SF_HIP(hipStreamSynchronize(s));
+        b->assembly_seconds += seconds() - t0;
+        if (b->next == total) {
+            if (!b->prescribed_applied) {
+                b->assembly.check_fault(s);
+                // the host copy the setup reads
+                sys.host.values.resize(sys.host.blocks() * 9);
+                sys.values.download(sys.host.values.data(), sys.host.values.size(), s);
+                SF_HIP(hipStreamSynchronize(s));
+                if (!b->problem.prescribed.empty()) {
+                    // study.rs:567-593: symmetric elimination on the scalar entries
+                    auto &h = sys.host;
+                    auto entry = [&](size_t row, size_t col) -> double * {
+                        const size_t br = row / 3, bc = col / 3;
+                        auto first = h.columns.begin() + h.starts[br], last = h.columns.begin() + h.starts[br + 1];
+                        auto it = std::lower_bound(first, last, (u32)bc);
+                        if (it == last || *it != bc) return nullptr;
+                        return &h.values[(size_t)(it - h.columns.begin()) * 9 + (row % 3) * 3 + col % 3];
+                    };
+                    std::vector<char> constrained(sys.size, 0);
+                    for (auto &pr : b->problem.prescribed) constrained[pr.first] = 1;
+                    for (auto &pr : b->problem.prescribed) {
+                        const size_t dof = pr.first, br = dof / 3;
+                        std::vector<std::pair<size_t, double>> entries;
+                        for (int64_t at = h.starts[br]; at < h.starts[br + 1]; ++at)
+                            for (size_t q = 0; q < 3; ++q) {
+                                const size_t col = h.columns[(size_t)at] * 3 + q;
+                                if (col != dof) entries.push_back({col, h.values[(size_t)at * 9 + (dof % 3) * 3 + q]});
+                            }
+                        for (auto &e : entries) {
+                            if (!constrained[e.first]) b->problem.rhs[e.first] -= e.second * pr.second;
+                            *entry(e.first, dof) += -e.second;
+                            *entry(dof, e.first) += -e.second;
+                        }
+                    }
+                    for (auto &pr : b->problem.prescribed) b->problem.rhs[pr.first] = *entry(pr.first, pr.first) * pr.second;
+                    sys.values.upload(sys.host.values.data(), sys.host.values.size(), s);
+                }
+                b->prescribed_applied = true;
+            }
+            if (b->use_multigrid && !b->setup_finished) {
+                const double t1 = seconds();
+                if (!b->setup) {
+                    // study.rs:529-549
+                    auto modes = rigid_body_modes(b->model.mesh, b->problem.free, b->problem.free_count);
+                    for (auto &pr : b->problem.prescribed) std::fill(&modes[pr.first * 6], &modes[pr.first * 6] + 6, 0.0);
+                    std::vector<Vec3> positions;
+                    positions.reserve(b->problem.free_count);
+                    for (size_t n = 0; n < b->model.mesh.nodes.size(); ++n)
+                        if (b->problem.free[n] != FIXED) positions.push_back(b->model.mesh.nodes[n]);
+                    b->setup = std::make_unique<MultigridSetup>(sys.host, modes, positions, b->mg_options);
+                }
+                b->setup_finished = b->setup->step(max_elements, cancel);
+                b->preconditioner_seconds += seconds() - t1;
+            }
+        }
+        if (progress) {
+            progress->elements_done = b->next;
+            progress->elements_total = total;
+            progress->setup_finished = (b->next == total && (!b->use_multigrid || b->setup_finished)) ? 1 : 0;
+        }
+    });
+}
+
+sf_status sf_study_builder_finish(sf_study_builder *b, sf_static_study **out) {
+    return guard([&] {
+        const size_t total = b->model.mesh.tets.size();
+        if (b->next != total || (b->use_multigrid && !b->setup_finished))
+            throw Error(SF_ERR_MODEL_NOT_FINISHED, "the study build has not finished");
+        const double t0 = seconds();
+        hipStream_t s = b->ctx->ctx->stream;
+        auto &st = *b->study;
+        st.system.system.build_from_device_values(s);
+        double pre = b->preconditioner_seconds;
+        if (b->use_multigrid) {
+            const double t1 = seconds();
+            st.mg.ctx = b->ctx;
+            st.mg.mg = make_device_multigrid(b->setup->take_levels(), b->mg_options.degree, b->mg_options.precision, s);
+            pre += seconds() - t1;
+        }
+        st.pcg = std::make_unique<sf_pcg>();
+        st.pcg->pcg = std::make_unique<DevicePcg>(*b->ctx->ctx, st.system.system, st.mg.mg.get(), b->problem.rhs, b->pcg);
+        st.model = std::move(b->model);
+        st.problem = std::move(b->problem);
+        auto &r = st.report;
+        r.total_dofs = 3 * st.model.mesh.nodes.size();
+        r.free_dofs = 3 * st.problem.free_count;
+        r.nonzeros = st.system.system.host.blocks() * 9;
+        r.iterations = 0;
+        r.relative_residual = 1.0;
+        r.assembly_seconds = b->assembly_seconds + (seconds() - t0) - (pre - b->preconditioner_seconds);
+        r.preconditioner_seconds = pre;
+        r.has_multigrid = b->use_multigrid ? 1 : 0;
+        if (b->use_multigrid) r.multigrid = st.mg.mg->stats;
+        *out = b->study.release();
+        delete b;
+    });
+}
+void sf_study_builder_destroy(sf_study_builder *b) { delete b; }
+
+sf_status sf_static_study_create(sf_context *ctx, const sf_model *model, const sf_pcg_options *pcg, int32_t use_multigrid,
+                                 const sf_multigrid_options *mg, sf_cancel_fn cancel, void *user, sf_static_study **out) {
+    sf_study_builder *b = nullptr;
+    sf_status st = sf_study_builder_create(ctx, model, pcg, use_multigrid, mg, &b);
+    if (st != SF_OK) return st;
+    sf_build_progress progress{};
+    st = sf_study_builder_step(b, UINT64_MAX, cancel, user, &progress);
+    if (st != SF_OK) {
+        sf_study_builder_destroy(b);
+        return st;
+    }
+    st = sf_study_builder_finish(b, out);
+    if (st != SF_OK) sf_study_builder_destroy(b);
+    return st;
+}
+
+sf_status sf_static_study_step(sf_static_study *study, uint64_t max_iterations, sf_cancel_fn cancel, void *user,
+                               sf_progress *progress) {
+    const double t0 = seconds();
+    int outcome = SF_RUNNING;
+    sf_status st = guard([&] { outcome = study->pcg->pcg->step(max_iterations, Cancel{cancel, user}); });
+    study->report.solve_seconds += seconds() - t0;
+    sf_progress p = study->pcg->pcg->progress();
+    study->report.iterations = p.iteration;
+    study->report.relative_residual = p.relative_residual;
+    study->report.at_rounding_floor = p.at_rounding_floor;
+    if (progress) {
+        *progress = p;
+        progress->outcome = outcome;
+    }
+    return st;
+}
+
+sf_status sf_static_study_displacements(const sf_static_study *study, double *out) {
+    return guard([&] {
+        // study.rs:218-237
+        std::vector<double> x(study->problem.free_count * 3);
+        study->pcg->pcg->estimate(x.data());
+        const auto &free = study->problem.free;
+        for (size_t n = 0; n < free.size(); ++n)
+            for (int a = 0; a < 3; ++a) out[3 * n + a] = free[n] == FIXED ? 0.0 : x[3 * (size_t)free[n] + a];
+        for (const auto &pr : study->model.prescribed) out[3 * (size_t)pr.first.first + pr.first.second] = pr.second;
+    });
+}
+sf_status sf_static_study_report(const sf_static_study *study, sf_report *out) {
+    return guard([&] { *out = study->report; });
+}
+const sf_multigrid *sf_static_study_multigrid(const sf_static_study *study) { return study->mg.mg ? &study->mg : nullptr; }
+sf_pcg *sf_static_study_pcg(sf_static_study *study) { return study->pcg.get(); }
+sf_status sf_static_study_matrix(const sf_static_study *study, uint64_t *size, uint64_t *nonzeros, int64_t *row_starts,
+                                 uint32_t *columns, double *values) {
+    return guard([&] {
+        const auto &sys = study->system.system;
+        HostBsr<double> h = sys.host;
+        h.values.resize(h.blocks() * 9);
+        sys.values.download(h.values.data(), h.values.size(), study->ctx->ctx->stream);
+        SF_HIP(hipStreamSynchronize(study->ctx->ctx->stream));
+        if (size) *size = sys.size;
+        if (nonzeros) *nonzeros = h.blocks() * 9;
+        to_scalar_csr(h, row_starts, columns, values);
+    });
+}
+sf_status sf_static_study_rhs(const sf_static_study *study, double *out) {
+    return guard([&] { std::copy(study->problem.rhs.begin(), study->problem.rhs.end(), out); });
+}
+void sf_static_study_destroy(sf_static_study *study) { delete study; }
+
+// ---- linear algebra -------------------------------------------------------------------------
+
+sf_status sf_csr_create(sf_context *ctx, uint64_t size, const int64_t *row_starts, const uint32_t *columns, const double *values,
+                        sf_csr **out) {
+    return guard([&] {
+        auto c = std::make_unique<sf_csr>();
+        c->ctx = ctx;
+        auto &sys = c->system;
+        sys.size = size;
+        if (to_nodal(size, row_starts, columns, values, sys.host) < 0) {
+            sys.rb = 3;
+        } else {
+            sys.rb = 1;
+            sys.host = HostBsr<double>();
+            sys.host.rows = sys.host.cols = size;
+            sys.host.rb = sys.host.cb = 1;
+            sys.host.starts.assign(row_starts, row_starts + size + 1);
+            sys.host.columns.assign(columns, columns + row_starts[size]);
+            sys.host.values.assign(values, values + row_starts[size]);
+        }
+        if (ctx) sys.upload_from_host(ctx->ctx->stream);
+        *out = c.release();
+    });
+}
+void sf_csr_destroy(sf_csr *c) { delete c; }
+
+sf_status sf_csr_multiply(sf_csr *c, const double *x, double *y) {
+    return guard([&] {
+        hipStream_t s = c->ctx->ctx->stream;
+        const size_t n = c->system.size;
+        DeviceBuffer<double> dx, dy(std::max<size_t>(n, 1));
+        dx.upload(x, n, s);
+        launch_spmv<double, double>(c->system.sliced, dx.ptr, dy.ptr, nullptr, dev::EPI_STORE, s);
+        dy.download(y, n, s);
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+sf_status sf_csr_rounding_bound(sf_csr *c, const double *x, int32_t dot_order, double *bound) {
+    return guard([&] {
+        hipStream_t s = c->ctx->ctx->stream;
+        const size_t n = c->system.size;
+        DeviceBuffer<double> dx, rows(std::max<size_t>(n, 1)), partials(dev::DOT_BLOCKS), result(1);
+        DeviceBuffer<uint32_t> ticket(1);
+        SF_HIP(hipMemsetAsync(ticket.ptr, 0, 4, s));
+        dx.upload(x, n, s);
+        const dim3 grid((unsigned)((c->system.sliced.slices + 3) / 4)), block(256);
+        if (c->system.rb == 3)
+            hipLaunchKernelGGL((dev::rounding_bound_kernel<3, 3>), grid, block, 0, s, c->system.sliced.view(), dx.ptr, rows.ptr);
+        else
+            hipLaunchKernelGGL((dev::rounding_bound_kernel<1, 1>), grid, block, 0, s, c->system.sliced.view(), dx.ptr, rows.ptr);
+        if (dot_order == SF_DOT_REFERENCE)
+            hipLaunchKernelGGL((dev::dot_serial_kernel<1>), dim3(1), dim3(64), 0, s, n, rows.ptr, rows.ptr, result.ptr);
+        else
+            hipLaunchKernelGGL((dev::dot_tree_kernel<1>), dim3(dev::DOT_BLOCKS), dim3(dev::DOT_THREADS), 0, s, n, rows.ptr, rows.ptr,
+                               partials.ptr, ticket.ptr, result.ptr);
+        SF_HIP(hipGetLastError());
+        double sum = 0;
+        result.download(&sum, 1, s);
+        SF_HIP(hipStreamSynchronize(s));
+        *bound = std::sqrt(sum);
+    });
+}
+
+sf_status sf_multigrid_setup_create(sf_context *ctx, const sf_csr *matrix, const double *near_null, uint64_t near_null_len,
+                                    const double *positions, uint64_t position_count, const sf_multigrid_options *options,
+                                    sf_multigrid_setup **out) {
+    return guard([&] {
+        const auto &sys = matrix->system;
+        if (sys.rb != 3) {
+            HostBsr<double> tmp;
+            const int64_t bad = to_nodal(sys.size, sys.host.starts.data(), sys.host.columns.data(), sys.host.values.data(), tmp);
+            throw Error(SF_ERR_MG_BLOCK_STRUCTURE, "the matrix does not group into 3x3 nodal blocks", bad < 0 ? 0 : bad);
+        }
+        std::vector<double> modes(near_null, near_null + near_null_len);
+        std::vector<Vec3> pos(position_count);
+        for (size_t r = 0; r < position_count; ++r)
+            for (int a = 0; a < 3; ++a) pos[r][a] = positions[3 * r + a];
+        auto s = std::make_unique<sf_multigrid_setup>();
+        s->ctx = ctx;
+        s->setup = std::make_unique<MultigridSetup>(sys.host, modes, pos, setup_options(options));
+        *out = s.release();
+    });
+}
+sf_status sf_multigrid_setup_step(sf_multigrid_setup *setup, uint64_t budget, sf_cancel_fn cancel, void *user, int32_t *finished) {
+    return guard([&] {
+        const bool done = setup->setup->step(budget, Cancel{cancel, user});
+        if (finished) *finished = done ? 1 : 0;
+    });
+}
+sf_status sf_multigrid_setup_finish(sf_multigrid_setup *setup, sf_multigrid **out) {
+    return guard([&] {
+        auto m = std::make_unique<sf_multigrid>();
+        m->ctx = setup->ctx;
+        const auto opt = setup->setup->options();
+        m->mg = make_device_multigrid(setup->setup->take_levels(), opt.degree, opt.precision, setup->ctx->ctx->stream);
+        *out = m.release();
+    });
+}
+void sf_multigrid_setup_destroy(sf_multigrid_setup *s) { delete s; }
+void sf_multigrid_destroy(sf_multigrid *m) { delete m; }
+
+sf_status sf_multigrid_stats_get(const sf_multigrid *mg, sf_multigrid_stats *out) {
+    return guard([&] { *out = mg->mg->stats; });
+}
+int32_t sf_multigrid_precision(const sf_multigrid *mg) { return mg->mg->precision; }
+
+sf_status sf_multigrid_apply(sf_multigrid *mg, const double *r, double *z) {
+    return guard([&] {
+        hipStream_t s = mg->ctx->ctx->stream;
+        const size_t n = mg->mg->dofs();
+        DeviceBuffer<double> dr, dz(std::max<size_t>(n, 1));
+        dr.upload(r, n, s);
+        mg->mg->apply(dr.ptr, dz.ptr, s);
+        dz.download(z, n, s);
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+sf_status sf_multigrid_smooth(sf_multigrid *mg, uint64_t level, const double *b, double *x) {
+    return guard([&] {
+        if (level >= mg->mg->host.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
+        hipStream_t s = mg->ctx->ctx->stream;
+        const size_t n = mg->mg->host[level].a.dofs();
+        DeviceBuffer<double> db, dx(std::max<size_t>(n, 1));
+        db.upload(b, n, s);
+        mg->mg->smooth_level(level, db.ptr, dx.ptr, s);
+        dx.download(x, n, s);
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+sf_status sf_multigrid_level_sizes(const sf_multigrid *mg, uint64_t *out, uint64_t capacity, uint64_t *levels) {
+    return guard([&] {
+        const auto &h = mg->mg->host;
+        if (levels) *levels = h.size();
+        for (size_t l = 0; l < h.size() && l < capacity; ++l) {
+            out[3 * l] = h[l].a.rows;
+            out[3 * l + 1] = h[l].a.rb;
+            out[3 * l + 2] = h[l].a.values.size();
+        }
+    });
+}
+
+sf_status sf_multigrid_level_matrix(const sf_multigrid *mg, uint64_t level, int32_t prolongator, uint64_t *rows,
+                                    uint64_t *block_columns, uint64_t *row_block, uint64_t *col_block, uint64_t *blocks,
+                                    int64_t *starts, uint32_t *columns, double *values) {
+    // values widened to f64 from the stored precision (cycle.rs:185-203)
+    return guard([&] {
+        level_matrix_view(mg->mg->host, mg->mg->precision != SF_PRECISION_DOUBLE, level, prolongator, rows, block_columns, row_block,
+                          col_block, blocks, starts, columns, values);
+    });
+}
+
+sf_status sf_multigrid_level_inverse_diagonal(const sf_multigrid *mg, uint64_t level, double *out) {
+    return guard([&] { level_inverse_diagonal_view(mg->mg->host, mg->mg->precision != SF_PRECISION_DOUBLE, level, out); });
+}
+
+sf_status sf_multigrid_smoother_spectrum(const sf_multigrid *mg, uint64_t level, double *lmin, double *lmax) {
+    return guard([&] {
+        const auto &h = mg->mg->host;
+        if (level >= h.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
+        *lmin = h[level].lambda_min;
+        *lmax = h[level].lambda_max;
+    });
+}
+
+sf_status sf_multigrid_aggregates(const sf_multigrid *mg, uint64_t level, uint32_t *out, uint64_t *count) {
+    return guard([&] {
+        const auto &h = mg->mg->host;
+        if (level >= h.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
+        if (count) *count = h[level].aggregates.size();
+        if (out) std::copy(h[level].aggregates.begin(), h[level].aggregates.end(), out);
+    });
+}
+
+sf_status sf_multigrid_solve_coarsest(sf_multigrid *mg, double *values, uint64_t count) {
+    return guard([&] {
+        const auto &h = mg->mg->host;
+        const size_t n = h.back().a.dofs();
+        if (!h.back().has_factor) throw Error(SF_ERR_MG_COARSEST_NOT_FACTORED, "the coarsest level has no direct factorization", (int64_t)n);
+        if (count != n) throw Error(SF_ERR_MG_COARSE_LENGTH, "the coarse solve was given the wrong length", (int64_t)n, (int64_t)count);
+        hipStream_t s = mg->ctx->ctx->stream;
+        DeviceBuffer<double> b, x(std::max<size_t>(n, 1));
+        b.upload(values, n, s);
+        mg->mg->coarse_solve(b.ptr, x.ptr, s);
+        x.download(values, n, s);
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+sf_status sf_pcg_create(sf_context *ctx, const sf_csr *matrix, sf_multigrid *mg, const double *rhs, const sf_pcg_options *options,
+                        sf_pcg **out) {
+    return guard([&] {
+        auto p = std::make_unique<sf_pcg>();
+        const sf_pcg_options o = options ? *options : sf_pcg_options{1e-9, 0, SF_DOT_TREE, 0};
+        std::vector<double> b(rhs, rhs + matrix->system.size);
+        p->pcg = std::make_unique<DevicePcg>(*ctx->ctx, matrix->system, mg ? mg->mg.get() : nullptr, b, o);
+        *out = p.release();
+    });
+}
+sf_status sf_pcg_step(sf_pcg *pcg, uint64_t max_iterations, sf_cancel_fn cancel, void *user, sf_progress *progress) {
+    int outcome = SF_RUNNING;
+    sf_status st = guard([&] { outcome = pcg->pcg->step(max_iterations, Cancel{cancel, user}); });
+    if (progress) {
+        *progress = pcg->pcg->progress();
+        progress->outcome = outcome;
+    }
+    return st;
+}
+sf_status sf_pcg_estimate(const sf_pcg *pcg, double *out) {
+    return guard([&] { pcg->pcg->estimate(out); });
+}
+sf_status sf_pcg_progress(const sf_pcg *pcg, sf_progress *out) {
+    return guard([&] { *out = pcg->pcg->progress(); });
+}
+int32_t sf_pcg_is_converged(const sf_pcg *pcg) { return pcg->pcg->converged() ? 1 : 0; }
+uint64_t sf_pcg_size(const sf_pcg *pcg) { return pcg->pcg->size(); }
+void sf_pcg_destroy(sf_pcg *pcg) { delete pcg; }
+
+}  // extern "C"
+
+extern "C" {
+
+sf_status sf_multigrid_setup_level_count(const sf_multigrid_setup *setup, uint64_t *levels) {
+    return guard([&] { *levels = setup_levels(setup).size(); });
+}
+sf_status sf_multigrid_setup_level_matrix(const sf_multigrid_setup *setup, uint64_t level, int32_t prolongator, uint64_t *rows,
+                                          uint64_t *block_columns, uint64_t *row_block, uint64_t *col_block, uint64_t *blocks,
+                                          int64_t *starts, uint32_t *columns, double *values) {
+    return guard([&] {
+        level_matrix_view(setup_levels(setup), false, level, prolongator, rows, block_columns, row_block, col_block, blocks, starts,
+                          columns, values);
+    });
+}
+sf_status sf_multigrid_setup_level_inverse_diagonal(const sf_multigrid_setup *setup, uint64_t level, double *out) {
+    return guard([&] { level_inverse_diagonal_view(setup_levels(setup), false, level, out); });
+}
+sf_status sf_multigrid_setup_smoother_spectrum(const sf_multigrid_setup *setup, uint64_t level, double *lmin, double *lmax) {
+    return guard([&] {
+        const auto &h = setup_levels(setup);
+        if (level >= h.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
+        *lmin = h[level].lambda_min;
+        *lmax = h[level].lambda_max;
+    });
+}
+sf_status sf_multigrid_setup_aggregates(const sf_multigrid_setup *setup, uint64_t level, uint32_t *out, uint64_t *count) {
+    return guard([&] {
+        const auto &h = setup_levels(setup);
+        if (level >= h.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
+        if (count) *count = h[level].aggregates.size();
+        if (out) std::copy(h[level].aggregates.begin(), h[level].aggregates.end(), out);
+    });
+}
+sf_status sf_multigrid_setup_coarse_factor(const sf_multigrid_setup *setup, uint64_t *n, double *out) {
+    return guard([&] {
+        const auto &h = setup_levels(setup);
+        const HostLevel &l = h.back();
+        *n = l.has_factor ? l.a.dofs() : 0;
+        if (out && l.has_factor) std::copy(l.factor.begin(), l.factor.end(), out);
+    });
+}
+
+sf_status sf_model_free_problem(const sf_model *model, uint64_t *free_nodes, uint64_t *blocks, uint32_t *free_map, double *rhs,
+                                int64_t *block_starts, uint32_t *block_columns, double *rigid_modes) {
+    return guard([&] {
+        FreeProblem p = number_and_load(model->model);
+        if (free_nodes) *free_nodes = p.free_count;
+        if (free_map) std::copy(p.free.begin(), p.free.end(), free_map);
+        if (rhs) std::copy(p.rhs.begin(), p.rhs.end(), rhs);
+        if (blocks || block_starts || block_columns) {
+            AssemblyPattern pat = build_pattern(model->model.mesh, p.free, p.free_count);
+            if (blocks) *blocks = pat.pattern.blocks();
+            if (block_starts) std::copy(pat.pattern.starts.begin(), pat.pattern.starts.end(), block_starts);
+            if (block_columns) std::copy(pat.pattern.columns.begin(), pat.pattern.columns.end(), block_columns);
+        }
+        if (rigid_modes) {
+            auto modes = rigid_body_modes(model->model.mesh, p.free, p.free_count);
+            std::copy(modes.begin(), modes.end(), rigid_modes);
+        }
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+const sf_csr *sf_static_study_system(const sf_static_study *study) { return &study->system; }
+
+sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision, sf_multigrid **out) {
+    return guard([&] {
+        auto m = std::make_unique<sf_multigrid>();
+        m->ctx = mg->ctx;
+        m->mg = make_device_multigrid(mg->mg->host, mg->mg->degree, precision, mg->ctx->ctx->stream);
+        *out = m.release();
+    });
+}
+
+double sf_multigrid_cycle_bytes(const sf_multigrid *mg) { return mg->mg->cycle_bytes; }
+
+sf_status sf_pcg_restart(sf_pcg *pcg, const double *new_rhs) {
+    return guard([&] { pcg->pcg->restart(new_rhs); });
+}
+
+sf_status sf_profile_enable(int32_t kernel, int32_t enable) {
+    return guard([&] {
+        if (kernel < 0 || kernel >= Profiler::KERNELS) throw Error(SF_ERR_INVALID_ARGUMENT, "no such kernel");
+        std::lock_guard<std::mutex> g(Profiler::get().lock);
+        Profiler::get().slots[kernel].on = enable != 0;
+    });
+}
+
+sf_status sf_profile_reset(void) {
+    return guard([&] { Profiler::get().reset(); });
+}
+
+sf_status sf_stream_copy(sf_context *ctx, uint64_t bytes, uint32_t repeats, double *seconds) {
+    return guard([&] {
+        hipStream_t s = ctx->ctx->stream;
+        const size_t n16 = (size_t)(bytes / 16);
+        if (n16 == 0 || repeats == 0) throw Error(SF_ERR_INVALID_ARGUMENT, "nothing to copy");
+        DeviceBuffer<uint4> a(n16), b(n16);
+        SF_HIP(hipMemsetAsync(a.ptr, 1, n16 * 16, s));
+        hipEvent_t e0, e1;
+        SF_HIP(hipEventCreate(&e0));
+        SF_HIP(hipEventCreate(&e1));
+        const unsigned grid = 256 * 8 * 4;
+        hipLaunchKernelGGL(dev::stream_copy_kernel, dim3(grid), dim3(256), 0, s, n16, a.ptr, b.ptr);  // warm
+        SF_HIP(hipEventRecord(e0, s));
+        for (uint32_t r = 0; r < repeats; ++r) hipLaunchKernelGGL(dev::stream_copy_kernel, dim3(grid), dim3(256), 0, s, n16, a.ptr, b.ptr);
+        SF_HIP(hipEventRecord(e1, s));
+        SF_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *seconds = ms * 1e-3 / repeats;
+    });
+}
+
+sf_status sf_profile_read(int32_t kernel, uint64_t *launches, double *seconds, double *bytes) {
+    return guard([&] {
+        if (kernel < 0 || kernel >= Profiler::KERNELS) throw Error(SF_ERR_INVALID_ARGUMENT, "no such kernel");
+        Profiler::get().collect(kernel);
+        auto &slot = Profiler::get().slots[kernel];
+        if (launches) *launches = slot.launches;
+        if (seconds) *seconds = slot.seconds;
+        if (bytes) *bytes = slot.bytes;
+    });
+}
+
+}  // extern "C"

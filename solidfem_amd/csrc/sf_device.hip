@@ -1,0 +1,668 @@
+// Device objects of the solve path: sliced matrices, the V-cycle, the conjugate gradient.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "sf_device.hpp"
+
+namespace sf {
+
+using namespace dev;
+
+static inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+Context::Context(int dev) : device(dev) {
+    int count = 0;
+    SF_HIP(hipGetDeviceCount(&count));
+    if (dev < 0 || dev >= count) throw Error(SF_ERR_INVALID_ARGUMENT, "no such device");
+    SF_HIP(hipSetDevice(dev));
+    SF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+Context::~Context() {
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+}
+
+// ---- launch timing -----------------------------------------------------------------------------
+
+Profiler &Profiler::get() {
+    static Profiler p;
+    return p;
+}
+
+int Profiler::begin(int kernel, hipStream_t s, double bytes) {
+    if (kernel < 0 || kernel >= KERNELS) return -1;
+    std::lock_guard<std::mutex> g(lock);
+    Slot &slot = slots[kernel];
+    if (!slot.on) return -1;
+    Pending p{};
+    for (hipEvent_t *e : {&p.start, &p.stop}) {
+        if (!slot.pool.empty()) {
+            *e = slot.pool.back();
+            slot.pool.pop_back();
+        } else {
+            SF_HIP(hipEventCreate(e));
+        }
+    }
+    p.bytes = bytes;
+    SF_HIP(hipEventRecord(p.start, s));
+    slot.pending.push_back(p);
+    return (int)slot.pending.size() - 1;
+}
+
+void Profiler::end(int kernel, int token, hipStream_t s) {
+    if (token < 0) return;
+    std::lock_guard<std::mutex> g(lock);
+    SF_HIP(hipEventRecord(slots[kernel].pending[(size_t)token].stop, s));
+}
+
+void Profiler::collect(int kernel) {
+    std::lock_guard<std::mutex> g(lock);
+    Slot &slot = slots[kernel];
+    for (auto &p : slot.pending) {
+        SF_HIP(hipEventSynchronize(p.stop));
+        float ms = 0;
+        SF_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
+        slot.seconds += ms * 1e-3;
+        slot.bytes += p.bytes;
+        slot.launches += 1;
+        slot.pool.push_back(p.start);
+        slot.pool.push_back(p.stop);
+    }
+    slot.pending.clear();
+}
+
+void Profiler::reset() {
+    for (int k = 0; k < KERNELS; ++k) {
+        collect(k);
+        std::lock_guard<std::mutex> g(lock);
+        slots[k].launches = 0;
+        slots[k].seconds = slots[k].bytes = 0;
+    }
+}
+
+// ---- sliced layout ---------------------------------------------------------------------------
+
+SlicedPlan plan_sliced(const int64_t *starts, const u32 *columns, size_t rows, int rb, int cb) {
+    SlicedPlan p;
+    p.rb = rb;
+    p.cb = cb;
+    p.npw = WAVE / rb;
+    p.rows = rows;
+    p.blocks = (size_t)starts[rows];
+    const size_t npw = (size_t)p.npw;
+    p.slices = (uint32_t)((rows + npw - 1) / npw);
+    // Rows sorted by block count (descending, stable) inside windows of 32 slices: a slice's
+    // lanes then run loops of nearly equal length while rows stay near their neighbours.
+    std::vector<uint32_t> order(rows);
+    std::iota(order.begin(), order.end(), 0u);
+    const size_t window = 32 * npw;
+    for (size_t w0 = 0; w0 < rows; w0 += window) {
+        const size_t w1 = std::min(rows, w0 + window);
+        std::stable_sort(order.begin() + w0, order.begin() + w1, [&](uint32_t a, uint32_t b) {
+            return (starts[a + 1] - starts[a]) > (starts[b + 1] - starts[b]);
+        });
+    }
+    p.perm.assign((size_t)p.slices * npw, NONE);
+    p.row_len.assign((size_t)p.slices * npw, 0);
+    p.slice_start.assign((size_t)p.slices + 1, 0);
+    for (uint32_t s = 0; s < p.slices; ++s) {
+        uint32_t width = 0;
+        for (size_t j = 0; j < npw; ++j) {
+            const size_t at = (size_t)s * npw + j;
+            if (at >= rows) break;
+            const uint32_t row = order[at];
+            p.perm[at] = row;
+            p.row_len[at] = (uint32_t)(starts[row + 1] - starts[row]);
+            width = std::max(width, p.row_len[at]);
+        }
+        p.slice_start[s + 1] = p.slice_start[s] + width;
+    }
+    const size_t slots = p.slice_start[p.slices];
+    p.cols.assign(slots * npw, 0);
+    p.block_of_slot.assign(slots * npw, NONE);
+    for (uint32_t s = 0; s < p.slices; ++s)
+        for (size_t j = 0; j < npw; ++j) {
+            const size_t at = (size_t)s * npw + j;
+            if (p.perm[at] == NONE) continue;
+            const uint32_t row = p.perm[at];
+            for (uint32_t k = 0; k < p.row_len[at]; ++k) {
+                const size_t slot = (size_t)(p.slice_start[s] + k) * npw + j;
+                p.cols[slot] = columns[(size_t)starts[row] + k];
+                p.block_of_slot[slot] = (uint32_t)(starts[row] + k);
+            }
+        }
+    return p;
+}
+
+template <typename T>
+void build_sliced(DeviceSliced<T> &out, const SlicedPlan &plan, size_t block_cols, const double *src, hipStream_t s) {
+    out.rb = plan.rb;
+    out.cb = plan.cb;
+    out.rows = plan.rows;
+    out.block_cols = block_cols;
+    out.blocks = plan.blocks;
+    out.slices = plan.slices;
+    out.slice_start.upload(plan.slice_start, s);
+    out.row_len.upload(plan.row_len, s);
+    out.perm.upload(plan.perm, s);
+    out.cols.upload(plan.cols, s);
+    DeviceBuffer<uint32_t> map;
+    map.upload(plan.block_of_slot, s);
+    const size_t n = (size_t)plan.slice_start[plan.slices] * plan.cb * WAVE;
+    out.vals.resize(std::max<size_t>(n, 1));
+    if (n) hipLaunchKernelGGL(gather_convert_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, map.ptr, plan.rb, plan.cb, src, out.vals.ptr);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipStreamSynchronize(s));  // `map` is released on return
+}
+
+template <typename T>
+void upload_sliced(DeviceSliced<T> &out, const HostBsr<double> &m, hipStream_t s) {
+    SlicedPlan plan = plan_sliced(m.starts.data(), m.columns.data(), m.rows, (int)m.rb, (int)m.cb);
+    DeviceBuffer<double> src;
+    src.upload(m.values, s);
+    build_sliced(out, plan, m.cols, src.ptr, s);
+}
+
+template void upload_sliced<double>(DeviceSliced<double> &, const HostBsr<double> &, hipStream_t);
+template void upload_sliced<float>(DeviceSliced<float> &, const HostBsr<double> &, hipStream_t);
+template void build_sliced<double>(DeviceSliced<double> &, const SlicedPlan &, size_t, const double *, hipStream_t);
+template void build_sliced<float>(DeviceSliced<float> &, const SlicedPlan &, size_t, const double *, hipStream_t);
+
+template <int RB, int CB, typename S, typename V>
+static void spmv_shape(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+    const dim3 grid(grid_for(a.slices, 4)), block(256);
+    if (epi == EPI_STORE) hipLaunchKernelGGL((spmv_kernel<RB, CB, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b);
+    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((spmv_kernel<RB, CB, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b);
+    else hipLaunchKernelGGL((spmv_kernel<RB, CB, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b);
+}
+
+template <typename S, typename V>
+void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+    if (a.slices == 0) return;
+    if (a.rb == 3 && a.cb == 3) spmv_shape<3, 3>(a, x, y, b, epi, s);
+    else if (a.rb == 6 && a.cb == 6) spmv_shape<6, 6>(a, x, y, b, epi, s);
+    else if (a.rb == 3 && a.cb == 6) spmv_shape<3, 6>(a, x, y, b, epi, s);
+    else if (a.rb == 6 && a.cb == 3) spmv_shape<6, 3>(a, x, y, b, epi, s);
+    else if (a.rb == 1 && a.cb == 1) spmv_shape<1, 1>(a, x, y, b, epi, s);
+    else throw Error(SF_ERR_INTERNAL, "unsupported block shape");
+    SF_HIP(hipGetLastError());
+}
+
+template void launch_spmv<double, double>(const DeviceSliced<double> &, const double *, double *, const double *, int, hipStream_t);
+template void launch_spmv<float, double>(const DeviceSliced<float> &, const double *, double *, const double *, int, hipStream_t);
+template void launch_spmv<float, float>(const DeviceSliced<float> &, const float *, float *, const float *, int, hipStream_t);
+
+// ---- the device hierarchy -------------------------------------------------------------------
+
+namespace {
+
+// Chebyshev scalars computed in f64 as the reference's cycle does (cycle.rs:578-650,
+// crates/solidfem-solve/src/gpu/layout.rs:445-468).
+void chebyshev(double lmin, double lmax, size_t degree, double &theta, std::vector<std::pair<double, double>> &steps) {
+    theta = 0.5 * (lmax + lmin);
+    const double delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    steps.clear();
+    for (size_t s = 1; s < degree; ++s) {
+        const double next = 1.0 / (2.0 * sigma - rho);
+        steps.push_back({next * rho, 2.0 * next / delta});
+        rho = next;
+    }
+}
+
+template <typename S, typename V>
+struct LevelDev {
+    int rb = 3;
+    size_t rows = 0, n = 0;
+    DeviceSliced<S> a, p, r;
+    DeviceBuffer<S> dinv;
+    DeviceBuffer<double> lt;  // coarsest: transposed Cholesky factor
+    bool has_p = false, has_factor = false;
+    double theta = 0;
+    std::vector<std::pair<double, double>> steps;
+    DeviceBuffer<V> b, x, d0, d1, r0, r1, res, px;
+};
+
+template <typename S, typename V>
+class Hierarchy final : public DeviceMultigrid {
+   public:
+    Hierarchy(std::vector<HostLevel> levels, size_t deg, int prec, hipStream_t s) {
+        precision = prec;
+        degree = deg;
+        host = std::move(levels);
+        lv_.resize(host.size());
+        for (size_t l = 0; l < host.size(); ++l) {
+            const HostLevel &h = host[l];
+            LevelDev<S, V> &d = lv_[l];
+            d.rb = (int)h.a.rb;
+            d.rows = h.a.rows;
+            d.n = h.a.dofs();
+            upload_sliced(d.a, h.a, s);
+            std::vector<S> dinv(h.inverse_diagonal.size());
+            for (size_t i = 0; i < dinv.size(); ++i) dinv[i] = (S)h.inverse_diagonal[i];
+            d.dinv.upload(dinv, s);
+            SF_HIP(hipStreamSynchronize(s));
+            chebyshev(h.lambda_min, h.lambda_max, degree, d.theta, d.steps);
+            if (h.has_prolongator) {
+                d.has_p = true;
+                upload_sliced(d.p, h.prolongator, s);
+                // the stored transpose: block row c holds block column c of P in ascending fine
+                // row order, each block transposed (crates/solidfem-solve/src/gpu/layout.rs:254-291)
+                HostBsr<double> t;
+                const auto &P = h.prolongator;
+                t.rows = P.cols;
+                t.cols = P.rows;
+                t.rb = P.cb;
+                t.cb = P.rb;
+                t.starts.assign(t.rows + 1, 0);
+                for (u32 c : P.columns) t.starts[(size_t)c + 1] += 1;
+                for (size_t i = 0; i < t.rows; ++i) t.starts[i + 1] += t.starts[i];
+                std::vector<int64_t> fill(t.starts.begin(), t.starts.end() - 1);
+                t.columns.assign(P.columns.size(), 0);
+                t.values.assign(P.values.size(), 0.0);
+                const size_t bl = P.rb * P.cb;
+                for (size_t row = 0; row < P.rows; ++row)
+                    for (int64_t at = P.starts[row]; at < P.starts[row + 1]; ++at) {
+                        const size_t to = (size_t)fill[P.columns[(size_t)at]]++;
+                        t.columns[to] = (u32)row;
+                        for (size_t pp = 0; pp < P.rb; ++pp)
+                            for (size_t q = 0; q < P.cb; ++q) t.values[to * bl + q * P.rb + pp] = P.values[(size_t)at * bl + pp * P.cb + q];
+                    }
+                upload_sliced(d.r, t, s);
+            }
+            if (h.has_factor) {
+                d.has_factor = true;
+                const size_t n = d.n;
+                if (n > COARSE_MAX) throw Error(SF_ERR_INTERNAL, "coarse factor larger than the device solve");
+                std::vector<double> lt(n * n, 0.0);
+                for (size_t i = 0; i < n; ++i)
+                    for (size_t k = 0; k <= i; ++k) lt[k * n + i] = h.factor[i * n + k];
+                d.lt.upload(lt, s);
+            }
+            for (auto *buf : {&d.b, &d.x, &d.d0, &d.d1, &d.r0, &d.r1, &d.res, &d.px}) buf->resize(d.n);
+        }
+        in_.resize(host.empty() ? 0 : host[0].a.dofs());
+        out_.resize(in_.count);
+        SF_HIP(hipStreamSynchronize(s));
+        compute_stats();
+    }
+
+    void apply(const double *r, double *z, hipStream_t s) override {
+        if (lv_.empty()) return;
+        if constexpr (std::is_same<V, double>::value) {
+            cycle(0, r, z, s);
+        } else {
+            const size_t n = lv_[0].n;
+            hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, r, in_.ptr);
+            cycle(0, in_.ptr, out_.ptr, s);
+            hipLaunchKernelGGL((convert_kernel<double, V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const V *)out_.ptr, z);
+        }
+        SF_HIP(hipGetLastError());
+    }
+
+    void smooth_level(size_t level, const double *b, double *x, hipStream_t s) override {
+        LevelDev<S, V> &d = lv_.at(level);
+        if constexpr (std::is_same<V, double>::value) {
+            smooth(d, b, x, nullptr, s);
+        } else {
+            hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, b, d.b.ptr);
+            smooth(d, d.b.ptr, d.x.ptr, nullptr, s);
+            hipLaunchKernelGGL((convert_kernel<double, V>), dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, (const V *)d.x.ptr, x);
+        }
+        SF_HIP(hipGetLastError());
+    }
+
+    void coarse_solve(const double *b, double *x, hipStream_t s) override {
+        LevelDev<S, V> &d = lv_.back();
+        if (!d.has_factor) throw Error(SF_ERR_MG_COARSEST_NOT_FACTORED, "the coarsest level has no direct factorization", (int64_t)d.n);
+        hipLaunchKernelGGL(coarse_solve_kernel<double>, dim3(1), dim3(1024), 0, s, (uint32_t)d.n, d.lt.ptr, b, x);
+        SF_HIP(hipGetLastError());
+    }
+
+   private:
+    // cycle.rs:570-651: start, then degree - 1 fused steps; `sum_into` takes the final add.
+    void smooth(LevelDev<S, V> &d, const V *b, V *x, V *sum_into, hipStream_t s, bool finest = false) {
+        const unsigned g = grid_for(d.n, 256);
+        if (d.rb == 3)
+            hipLaunchKernelGGL((cheb_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
+        else
+            hipLaunchKernelGGL((cheb_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
+        V *dd[2] = {d.d0.ptr, d.d1.ptr};
+        V *rr[2] = {d.r0.ptr, d.r1.ptr};
+        const size_t steps = d.steps.size();
+        for (size_t k = 1; k <= steps; ++k) {
+            const V *d_in = dd[(k - 1) & 1];
+            const V *r_in = k == 1 ? b : rr[(k - 1) & 1];
+            V *d_out = dd[k & 1];
+            V *r_out = rr[k & 1];
+            const V keep = (V)d.steps[k - 1].first, stp = (V)d.steps[k - 1].second;
+            V *into = k == steps ? sum_into : nullptr;
+            const dim3 grid(grid_for(d.a.slices, 4)), block(256);
+            const double vec = (double)d.n * sizeof(V);
+            const int token = finest ? Profiler::get().begin(0, s, (double)d.a.algorithmic_matrix_bytes() +
+                                                                       (double)d.rows * d.rb * d.rb * sizeof(S) + 6 * vec +
+                                                                       (into ? 2 * vec : 0.0))
+                                     : -1;
+            if (d.rb == 3)
+                hipLaunchKernelGGL((cheb_step_kernel<3, S, V>), grid, block, 0, s, d.a.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+            else
+                hipLaunchKernelGGL((cheb_step_kernel<6, S, V>), grid, block, 0, s, d.a.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+            Profiler::get().end(0, token, s);
+        }
+        if (steps == 0 && sum_into)
+            hipLaunchKernelGGL((add_kernel<V>), dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, sum_into, (const V *)x);
+    }
+
+    // cycle.rs:461-539
+    void cycle(size_t l, const V *b, V *x, hipStream_t s) {
+        LevelDev<S, V> &d = lv_[l];
+        if (!d.has_p) {
+            if (d.has_factor) {
+                hipLaunchKernelGGL(coarse_solve_kernel<V>, dim3(1), dim3(1024), 0, s, (uint32_t)d.n, d.lt.ptr, b, x);
+            } else {
+                smooth(d, b, x, nullptr, s);
+            }
+            return;
+        }
+        LevelDev<S, V> &c = lv_[l + 1];
+        const bool finest = l == 0;
+        const double res_bytes = (double)d.a.algorithmic_matrix_bytes() + 3.0 * d.n * sizeof(V);
+        smooth(d, b, x, nullptr, s, finest);
+        int token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
+        launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s);
+        Profiler::get().end(2, token, s);
+        launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
+        cycle(l + 1, c.b.ptr, c.x.ptr, s);
+        launch_spmv<S, V>(d.p, c.x.ptr, x, nullptr, EPI_ADD, s);
+        token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
+        launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s);
+        Profiler::get().end(2, token, s);
+        smooth(d, d.res.ptr, d.px.ptr, x, s, finest);
+    }
+
+    void compute_stats() {
+        // MultigridStats::of_levels, cycle.rs:676-699, with stored values in S
+        const double finest = host.empty() ? 1.0 : (double)host[0].a.values.size();
+        size_t stored = 0, memory = 0;
+        for (const auto &h : host) {
+            stored += h.a.values.size();
+            memory += h.a.starts.size() * 8 + h.a.columns.size() * 4 + h.a.values.size() * sizeof(S);
+            memory += h.inverse_diagonal.size() * sizeof(S);
+            if (h.has_prolongator)
+                memory += h.prolongator.starts.size() * 8 + h.prolongator.columns.size() * 4 + h.prolongator.values.size() * sizeof(S);
+            if (h.has_factor) memory += h.factor.size() * sizeof(double);
+        }
+        stats.levels = host.size();
+        stats.coarse_dofs = host.empty() ? 0 : host.back().a.dofs();
+        stats.operator_complexity = (double)stored / finest;
+        stats.memory_bytes = memory;
+        // algorithmic bytes of one V-cycle
+        const double sv = sizeof(V), ss = sizeof(S);
+        double bytes = 0;
+        const double k = (double)degree;
+        for (const auto &d : lv_) {
+            const double n = (double)d.n;
+            const double abytes = (double)d.a.algorithmic_matrix_bytes();
+            const double dbytes = (double)d.rows * d.rb * d.rb * ss;
+            if (!d.has_p) {
+                if (d.has_factor) bytes += n * n * 8 + 2 * n * sv;
+                else bytes += dbytes + 3 * n * sv + (k - 1) * (abytes + dbytes + 6 * n * sv);
+                continue;
+            }
+            const double smooth_bytes = (dbytes + 3 * n * sv) + (k - 1) * (abytes + dbytes + 6 * n * sv);
+            bytes += 2 * smooth_bytes + n * sv;                                   // + final add of x
+            bytes += 2 * (abytes + 3 * n * sv);                                   // two residuals
+            bytes += (double)d.r.algorithmic_matrix_bytes() + n * sv + (double)d.r.dofs() * sv;  // restrict
+            bytes += (double)d.p.algorithmic_matrix_bytes() + (double)d.r.dofs() * sv + 2 * n * sv;  // prolong
+        }
+        cycle_bytes = bytes;
+    }
+
+    std::vector<LevelDev<S, V>> lv_;
+    DeviceBuffer<V> in_, out_;
+};
+
+}  // namespace
+
+std::unique_ptr<DeviceMultigrid> make_device_multigrid(std::vector<HostLevel> levels, size_t degree, int precision, hipStream_t s) {
+    if (degree < 1) throw Error(SF_ERR_INVALID_ARGUMENT, "the smoother degree must be at least one");
+    switch (precision) {
+        case SF_PRECISION_DOUBLE:
+            return std::make_unique<Hierarchy<double, double>>(std::move(levels), degree, precision, s);
+        case SF_PRECISION_SINGLE_STORAGE:
+            return std::make_unique<Hierarchy<float, double>>(std::move(levels), degree, precision, s);
+        case SF_PRECISION_SINGLE:
+            return std::make_unique<Hierarchy<float, float>>(std::move(levels), degree, precision, s);
+        default:
+            throw Error(SF_ERR_INVALID_ARGUMENT, "unknown precision");
+    }
+}
+
+// ---- the system matrix ------------------------------------------------------------------------
+
+void DeviceSystem::upload_from_host(hipStream_t s) {
+    values.upload(host.values, s);
+    build_from_device_values(s);
+}
+
+void DeviceSystem::build_from_device_values(hipStream_t s) {
+    SlicedPlan plan = plan_sliced(host.starts.data(), host.columns.data(), host.rows, (int)host.rb, (int)host.cb);
+    build_sliced(sliced, plan, host.cols, values.ptr, s);
+}
+
+// ---- the conjugate gradient -------------------------------------------------------------------
+
+DevicePcg::DevicePcg(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
+                     const sf_pcg_options &options)
+    : ctx_(ctx), a_(a), mg_(mg), n_(a.size), dot_order_(options.dot_order), tolerance_(options.tolerance) {
+    if (rhs.size() != n_) throw Error(SF_ERR_INVALID_ARGUMENT, "the right-hand side does not match the matrix");
+    if (mg_ && mg_->dofs() != n_) throw Error(SF_ERR_INVALID_ARGUMENT, "the multigrid does not match the matrix");
+    hipStream_t s = ctx_.stream;
+    max_iterations_ = options.max_iterations ? options.max_iterations : 10 * (u64)n_ + 1000;
+    for (auto *buf : {&b_, &x_, &r_, &z_, &p_, &ap_, &tmp_, &tmp2_}) buf->resize(std::max<size_t>(n_, 1));
+    b_.upload(rhs.data(), n_, s);
+    scalars_.resize(1);
+    partials_.resize(DOT_BLOCKS);
+    norm_.resize(1);
+    tickets_.resize(4);
+    SF_HIP(hipMemsetAsync(tickets_.ptr, 0, 4 * sizeof(uint32_t), s));
+    SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocDefault));
+    SF_HIP(hipEventCreateWithFlags(&copied_, hipEventDisableTiming));
+    if (!mg_) {
+        // Jacobi::new, preconditioner.rs:53-66
+        std::vector<double> inv(n_);
+        const auto &h = a_.host;
+        for (size_t row = 0; row < n_; ++row) {
+            const size_t brow = row / h.rb, p = row % h.rb;
+            double diag = 0.0;
+            for (int64_t at = h.starts[brow]; at < h.starts[brow + 1]; ++at)
+                if (h.columns[(size_t)at] * h.cb <= row && row < (h.columns[(size_t)at] + 1) * h.cb)
+                    diag = h.values[(size_t)at * h.rb * h.cb + p * h.cb + (row - h.columns[(size_t)at] * h.cb)];
+            if (std::isnan(diag) || diag <= 0.0 || !std::isfinite(diag))
+                throw Error(SF_ERR_JACOBI_DIAGONAL, "a stiffness matrix needs a positive diagonal", (int64_t)row, 0, diag);
+            inv[row] = 1.0 / diag;
+        }
+        jacobi_.upload(inv, s);
+    }
+    initialise();
+}
+
+void DevicePcg::initialise() {
+    // pcg.rs:135-164: r = b, x = 0, then the first direction from the preconditioned residual
+    hipStream_t s = ctx_.stream;
+    SF_HIP(hipMemcpyAsync(r_.ptr, b_.ptr, n_ * 8, hipMemcpyDeviceToDevice, s));
+    SF_HIP(hipMemsetAsync(x_.ptr, 0, n_ * 8, s));
+    SF_HIP(hipMemsetAsync(z_.ptr, 0, n_ * 8, s));
+    SF_HIP(hipMemsetAsync(p_.ptr, 0, n_ * 8, s));
+    SF_HIP(hipMemsetAsync(scalars_.ptr, 0, sizeof(PcgScalars), s));
+    iterations_ = 0;
+    relative_ = 0;
+    floor_ = false;
+    status_ = PcgStatus::Active;
+    dot(b_.ptr, b_.ptr, norm_.ptr);
+    double bb = 0;
+    norm_.download(&bb, 1, s);
+    SF_HIP(hipStreamSynchronize(s));
+    rhs_norm_ = std::sqrt(bb);
+    if (rhs_norm_ == 0.0) {
+        status_ = PcgStatus::Converged;
+    } else {
+        reset_direction();
+        relative_ = 1.0;
+    }
+    SF_HIP(hipStreamSynchronize(s));
+}
+
+void DevicePcg::restart(const double *new_rhs) {
+    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    if (new_rhs) b_.upload(new_rhs, n_, ctx_.stream);
+    initialise();
+}
+
+DevicePcg::~DevicePcg() {
+    (void)hipStreamSynchronize(ctx_.stream);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (copied_) (void)hipEventDestroy(copied_);
+}
+
+void DevicePcg::dot(const double *a, const double *b, double *result) {
+    if (dot_order_ == SF_DOT_REFERENCE)
+        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_.stream, n_, a, b, result);
+    else
+        hipLaunchKernelGGL((dot_tree_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_.stream, n_, a, b, partials_.ptr,
+                           tickets_.ptr, result);
+    SF_HIP(hipGetLastError());
+}
+
+void DevicePcg::sum(const double *a, double *result) {
+    if (dot_order_ == SF_DOT_REFERENCE)
+        hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_.stream, n_, a, a, result);
+    else
+        hipLaunchKernelGGL((dot_tree_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_.stream, n_, a, a, partials_.ptr,
+                           tickets_.ptr, result);
+    SF_HIP(hipGetLastError());
+}
+
+void DevicePcg::precondition(const double *r, double *z) {
+    if (mg_) mg_->apply(r, z, ctx_.stream);
+    else hipLaunchKernelGGL(jacobi_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, ctx_.stream, n_, jacobi_.ptr, r, z);
+    SF_HIP(hipGetLastError());
+}
+
+void DevicePcg::reset_direction() {
+    // pcg.rs:277-282
+    precondition(r_.ptr, z_.ptr);
+    SF_HIP(hipMemcpyAsync(p_.ptr, z_.ptr, n_ * 8, hipMemcpyDeviceToDevice, ctx_.stream));
+    dot(r_.ptr, z_.ptr, &scalars_.ptr->rz);
+}
+
+PcgScalars DevicePcg::read_scalars() {
+    SF_HIP(hipMemcpyAsync(pinned_, scalars_.ptr, sizeof(PcgScalars), hipMemcpyDeviceToHost, ctx_.stream));
+    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    return *pinned_;
+}
+
+void DevicePcg::iterate() {
+    // pcg.rs:216-275. The branch taken when the recursive residual is above the tolerance (the
+    // preconditioner, r.z and the new direction) is enqueued before the residual is read back, so
+    // the readback overlaps the V-cycle. When the other branch is taken, that work is discarded:
+    // it changes only z, p and rz, which the true-residual branch resets or never reads again.
+    hipStream_t s = ctx_.stream;
+    PcgScalars *sc = scalars_.ptr;
+    const int token = Profiler::get().begin(1, s, (double)a_.sliced.algorithmic_matrix_bytes() + 2.0 * n_ * 8);
+    launch_spmv<double, double>(a_.sliced, p_.ptr, ap_.ptr, nullptr, EPI_STORE, s);
+    Profiler::get().end(1, token, s);
+    dot(p_.ptr, ap_.ptr, &sc->curvature);
+    hipLaunchKernelGGL(pcg_update_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, sc, p_.ptr, ap_.ptr, x_.ptr, r_.ptr);
+    dot(r_.ptr, r_.ptr, &sc->rr);
+    SF_HIP(hipMemcpyAsync(pinned_, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipEventRecord(copied_, s));
+    precondition(r_.ptr, z_.ptr);
+    dot(r_.ptr, z_.ptr, &sc->rz_next);
+    hipLaunchKernelGGL(pcg_direction_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, sc, z_.ptr, p_.ptr, tickets_.ptr + 1);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipEventSynchronize(copied_));
+    const PcgScalars h = *pinned_;
+    if (!h.valid) {
+        SF_HIP(hipStreamSynchronize(s));
+        status_ = PcgStatus::NotPositiveDefinite;
+        failure_ = Error(SF_ERR_PCG_NOT_POSITIVE_DEFINITE, "the system is not positive definite", (int64_t)iterations_, 0, h.curvature);
+        throw failure_;
+    }
+    iterations_ += 1;
+    relative_ = std::sqrt(h.rr) / rhs_norm_;
+    if (relative_ <= tolerance_) {
+        launch_spmv<double, double>(a_.sliced, x_.ptr, tmp_.ptr, b_.ptr, EPI_RESIDUAL, s);
+        dot(tmp_.ptr, tmp_.ptr, &sc->true_rr);
+        const dim3 grid(grid_for(a_.sliced.slices, 4)), block(256);
+        if (a_.rb == 3) hipLaunchKernelGGL((rounding_bound_kernel<3, 3>), grid, block, 0, s, a_.sliced.view(), x_.ptr, tmp2_.ptr);
+        else hipLaunchKernelGGL((rounding_bound_kernel<1, 1>), grid, block, 0, s, a_.sliced.view(), x_.ptr, tmp2_.ptr);
+        sum(tmp2_.ptr, &sc->bound2);
+        const PcgScalars t = read_scalars();
+        relative_ = std::sqrt(t.true_rr) / rhs_norm_;
+        const double floor = std::sqrt(t.bound2) / rhs_norm_;
+        floor_ = relative_ > tolerance_ && relative_ <= floor;
+        if (relative_ <= tolerance_ || floor_) {
+            status_ = PcgStatus::Converged;
+            return;
+        }
+        // rounding made the recursive residual over-optimistic: restart from the true one
+        SF_HIP(hipMemcpyAsync(r_.ptr, tmp_.ptr, n_ * 8, hipMemcpyDeviceToDevice, s));
+        reset_direction();
+    }
+    if (iterations_ >= max_iterations_) {
+        SF_HIP(hipStreamSynchronize(s));
+        status_ = PcgStatus::NotConverged;
+        failure_ = Error(SF_ERR_PCG_NOT_CONVERGED, "the solver did not converge", (int64_t)iterations_, 0, relative_);
+        throw failure_;
+    }
+}
+
+int DevicePcg::step(u64 max, const Cancel &cancel) {
+    // pcg.rs:168-189
+    switch (status_) {
+        case PcgStatus::Active:
+            break;
+        case PcgStatus::Converged:
+            return SF_CONVERGED;
+        case PcgStatus::Cancelled:
+            return SF_CANCELLED;
+        default:
+            throw failure_;
+    }
+    for (u64 i = 0; i < max; ++i) {
+        if (cancel()) {
+            SF_HIP(hipStreamSynchronize(ctx_.stream));
+            status_ = PcgStatus::Cancelled;
+            return SF_CANCELLED;
+        }
+        iterate();
+        if (status_ == PcgStatus::Converged) {
+            SF_HIP(hipStreamSynchronize(ctx_.stream));
+            return SF_CONVERGED;
+        }
+    }
+    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    return SF_RUNNING;
+}
+
+void DevicePcg::estimate(double *out) const {
+    x_.download(out, n_, ctx_.stream);
+    SF_HIP(hipStreamSynchronize(ctx_.stream));
+}
+
+sf_progress DevicePcg::progress() const {
+    sf_progress p{};
+    p.iteration = iterations_;
+    p.relative_residual = relative_;
+    p.at_rounding_floor = floor_ ? 1 : 0;
+    p.outcome = status_ == PcgStatus::Converged ? SF_CONVERGED : status_ == PcgStatus::Cancelled ? SF_CANCELLED : SF_RUNNING;
+    return p;
+}
+
+}  // namespace sf

@@ -1,0 +1,180 @@
+// Device objects of the solve path: sliced block matrices, the multigrid hierarchy and its
+// V-cycle, the conjugate gradient solve and the stiffness assembly, all resident in HBM.
+#pragma once
+
+#include "sf_host.hpp"
+#include "sf_kernels.hpp"
+#include "sf_setup.hpp"
+
+namespace sf {
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    explicit Context(int dev);
+    ~Context();
+};
+
+// The row order and slot layout of a sliced matrix, planned on the host.
+struct SlicedPlan {
+    int rb = 3, cb = 3, npw = 21;
+    size_t rows = 0, blocks = 0;
+    uint32_t slices = 0;
+    std::vector<uint32_t> slice_start, row_len, perm, cols, block_of_slot;
+};
+SlicedPlan plan_sliced(const int64_t *starts, const u32 *columns, size_t rows, int rb, int cb);
+
+template <typename T>
+struct DeviceSliced {
+    int rb = 3, cb = 3;
+    size_t rows = 0, block_cols = 0, blocks = 0;
+    uint32_t slices = 0;
+    DeviceBuffer<uint32_t> slice_start, row_len, perm, cols;
+    DeviceBuffer<T> vals;
+    dev::Sliced<T> view() const { return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices}; }
+    size_t dofs() const { return rows * rb; }
+    // Bytes one product reads by the algorithmic count: values, block columns, row offsets.
+    size_t algorithmic_matrix_bytes() const { return blocks * (rb * cb * sizeof(T) + 4) + (rows + 1) * 4; }
+};
+
+// Builds a sliced matrix from row-major f64 blocks already on the device (`src_values`).
+template <typename T>
+void build_sliced(DeviceSliced<T> &out, const SlicedPlan &plan, size_t block_cols, const double *src_values, hipStream_t s);
+// ... or from host blocks.
+template <typename T>
+void upload_sliced(DeviceSliced<T> &out, const HostBsr<double> &m, hipStream_t s);
+
+// y = A x (epilogue per dev::Epilogue) for any supported block shape.
+template <typename S, typename V>
+void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s);
+
+// ---- launch timing ------------------------------------------------------------------------------
+
+// Brackets launches of a chosen kernel with HIP events on the launching stream and accumulates
+// their durations and algorithmic bytes (read back on request).
+struct Profiler {
+    static constexpr int KERNELS = 3;
+    struct Pending {
+        hipEvent_t start, stop;
+        double bytes;
+    };
+    struct Slot {
+        bool on = false;
+        std::vector<Pending> pending;
+        std::vector<hipEvent_t> pool;
+        u64 launches = 0;
+        double seconds = 0, bytes = 0;
+    };
+    Slot slots[KERNELS];
+    std::mutex lock;
+    static Profiler &get();
+    // Returns an index into the pending list, or -1 when off.
+    int begin(int kernel, hipStream_t s, double bytes);
+    void end(int kernel, int token, hipStream_t s);
+    void collect(int kernel);
+    void reset();
+};
+
+// ---- multigrid --------------------------------------------------------------------------------
+
+class DeviceMultigrid {
+   public:
+    virtual ~DeviceMultigrid() = default;
+    // z = M^-1 r, device pointers, f64 in and out.
+    virtual void apply(const double *r, double *z, hipStream_t s) = 0;
+    // One smoother application on `level` from a zero guess (f64 device vectors in and out).
+    virtual void smooth_level(size_t level, const double *b, double *x, hipStream_t s) = 0;
+    virtual void coarse_solve(const double *b, double *x, hipStream_t s) = 0;
+    int precision = SF_PRECISION_SINGLE_STORAGE;
+    size_t degree = 3;
+    std::vector<HostLevel> host;  // the f64 hierarchy as set up (for inspection and export)
+    sf_multigrid_stats stats{};
+    size_t dofs() const { return host.empty() ? 0 : host[0].a.dofs(); }
+    // Algorithmic HBM bytes of one V-cycle (every array each kernel touches, counted once).
+    double cycle_bytes = 0;
+};
+std::unique_ptr<DeviceMultigrid> make_device_multigrid(std::vector<HostLevel> levels, size_t degree, int precision,
+                                                       hipStream_t s);
+
+// ---- system matrix and conjugate gradient ---------------------------------------------------
+
+// The system matrix on the device: 3x3 blocks when the matrix groups into nodal blocks, else
+// scalar rows. Keeps the host scalar-CSR view for export.
+struct DeviceSystem {
+    size_t size = 0;  // scalar rows
+    int rb = 3;
+    HostBsr<double> host;  // block (or scalar) rows with f64 values, as assembled
+    DeviceBuffer<double> values;  // row-major blocks on the device (assembly output)
+    DeviceSliced<double> sliced;
+    void upload_from_host(hipStream_t s);
+    void build_from_device_values(hipStream_t s);
+};
+
+enum class PcgStatus { Active, Converged, Cancelled, NotPositiveDefinite, NotConverged };
+
+class DevicePcg {
+   public:
+    DevicePcg(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
+              const sf_pcg_options &options);
+    // Pcg::new again on the buffers already held, optionally with a new right-hand side.
+    void restart(const double *new_rhs);
+    ~DevicePcg();
+    // Pcg::step: up to `max` iterations, cancel polled before each. Throws Error on failure.
+    int step(u64 max, const Cancel &cancel);
+    void estimate(double *out) const;
+    sf_progress progress() const;
+    bool converged() const { return status_ == PcgStatus::Converged; }
+    size_t size() const { return n_; }
+    const double *x_device() const { return x_.ptr; }
+    u64 iterations() const { return iterations_; }
+
+   private:
+    void precondition(const double *r, double *z);
+    void dot(const double *a, const double *b, double *result);
+    void sum(const double *a, double *result);
+    void iterate();
+    void reset_direction();
+    void initialise();
+    dev::PcgScalars read_scalars();
+
+    Context &ctx_;
+    const DeviceSystem &a_;
+    DeviceMultigrid *mg_;
+    size_t n_;
+    int dot_order_;
+    double tolerance_, rhs_norm_ = 0, relative_ = 0;
+    u64 iterations_ = 0, max_iterations_;
+    bool floor_ = false;
+    PcgStatus status_ = PcgStatus::Active;
+    Error failure_{SF_OK, ""};
+    DeviceBuffer<double> b_, x_, r_, z_, p_, ap_, tmp_, tmp2_, jacobi_;
+    DeviceBuffer<dev::PcgScalars> scalars_;
+    DeviceBuffer<double> partials_, norm_;
+    DeviceBuffer<uint32_t> tickets_;
+    dev::PcgScalars *pinned_ = nullptr;
+    hipEvent_t copied_ = nullptr;
+};
+
+// ---- assembly -------------------------------------------------------------------------------
+
+// Element stiffness (crates/solidfem-elastic/src/element.rs:98-133) on the device, and the
+// gather of each block's contributions in ascending element order (assembly.rs:87-130).
+struct DeviceAssembly {
+    DeviceBuffer<double> nodes;     // 3 per mesh node
+    DeviceBuffer<uint32_t> tets;    // 10 per element
+    DeviceBuffer<int64_t> contribution_starts;
+    DeviceBuffer<uint32_t> contributions;
+    DeviceBuffer<double> ke;        // 55 upper blocks x 9 per element of a chunk
+    DeviceBuffer<int32_t> fault;    // first failing element, point and kind
+    size_t elements = 0;
+    MaterialField field;
+    int quadrature = 4;
+    void prepare(const Model &model, const AssemblyPattern &pattern, hipStream_t s);
+    // Adds elements [first, last) into `values` (row-major 3x3 blocks, zeroed before the first).
+    void add_elements(size_t first, size_t last, double *values, size_t blocks, hipStream_t s);
+    // The 900-value element stiffness of elements [first, first + count) into host `ke900`.
+    void element_matrices(size_t first, size_t count, double *ke900, hipStream_t s);
+    void check_fault(hipStream_t s);
+};
+
+}  // namespace sf

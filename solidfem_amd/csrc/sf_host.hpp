@@ -1,0 +1,124 @@
+// Host-side structures of the solve path: meshes, the model, the free numbering, the load, the
+// rigid-body modes and the block sparsity pattern with its assembly map. These feed the device
+// (assembly, multigrid cycle, conjugate gradient) and the host multigrid setup.
+#pragma once
+
+#include <map>
+#include <set>
+
+#include "sf_common.hpp"
+
+namespace sf {
+
+struct Vec3 {
+    double v[3];
+    double &operator[](int i) { return v[i]; }
+    double operator[](int i) const { return v[i]; }
+};
+
+// ---- meshes ------------------------------------------------------------------------------
+
+// Four-node tetrahedra, validated as TetMesh::new does (crates/solidfem-mesh/src/tet.rs:59-109).
+struct TetMesh {
+    std::vector<Vec3> positions;
+    std::vector<std::array<u32, 4>> tets;
+};
+void validate_tet_mesh(const TetMesh &mesh);  // throws Error(SF_ERR_MESH_*)
+TetMesh make_box_mesh(const double origin[3], const double size[3], const int32_t divisions[3]);
+
+struct Face {
+    u32 tet;
+    u32 face;
+    u32 nodes[6];
+    bool operator<(const Face &o) const {
+        if (tet != o.tet) return tet < o.tet;
+        if (face != o.face) return face < o.face;
+        for (int i = 0; i < 6; ++i)
+            if (nodes[i] != o.nodes[i]) return nodes[i] < o.nodes[i];
+        return false;
+    }
+    bool operator==(const Face &o) const {
+        if (tet != o.tet || face != o.face) return false;
+        for (int i = 0; i < 6; ++i)
+            if (nodes[i] != o.nodes[i]) return false;
+        return true;
+    }
+};
+
+// Ten-node tetrahedra: corners then edge midpoints numbered by sorted corner pair
+// (crates/solidfem-mesh/src/quadratic.rs:24-71).
+struct QuadMesh {
+    std::vector<Vec3> nodes;
+    std::vector<std::array<u32, 10>> tets;
+    std::vector<std::array<u32, 4>> corners;
+    size_t vertex_count = 0;
+    std::vector<Face> boundary_faces() const;  // quadratic.rs:89-124 order
+};
+QuadMesh upgrade_to_quadratic(const TetMesh &mesh);
+
+// Local node indices of each face (crates/solidfem-mesh/src/boundary.rs:8-13).
+extern const int FACE_LOCAL[4][6];
+extern const int EDGE_PAIRS[6][2];
+
+// ---- model -------------------------------------------------------------------------------
+
+struct MaterialField {
+    // Uniform (E, nu) or a shell of depth `shell_depth` below the two long-face pairs with a core.
+    int kind = SF_MATERIAL_UNIFORM;
+    double e = 1, nu = 0, core_e = 1, core_nu = 0, shell_depth = 0;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    int transverse[2] = {1, 2};
+};
+
+struct Model {
+    QuadMesh mesh;
+    MaterialField field;
+    int quadrature = 4;
+    std::set<u32> fixed;
+    std::vector<std::pair<u32, std::array<double, 3>>> nodal_forces;
+    std::vector<std::pair<Face, std::array<double, 3>>> tractions;
+    std::map<std::pair<u32, u32>, double> prescribed;
+};
+
+Model model_from_workload(const sf_workload &w);
+
+// The validated, numbered problem of StudyBuilder::new (crates/solidfem-elastic/src/study.rs:419-474).
+struct FreeProblem {
+    std::vector<u32> free;  // node -> free index or FIXED
+    size_t free_count = 0;
+    std::vector<double> rhs;  // 3 per free node
+    std::vector<std::pair<size_t, double>> prescribed;  // (free dof, value), sorted
+};
+FreeProblem number_and_load(const Model &model);  // throws Error(SF_ERR_MODEL_*)
+
+// Six rigid-body modes per free unknown (crates/solidfem-elastic/src/rigid_modes.rs:36-83).
+std::vector<double> rigid_body_modes(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count);
+
+// ---- block pattern and assembly map ----------------------------------------------------
+
+// A block sparse matrix on the host: rows of `rb` x `cb` blocks, sorted block columns,
+// row-major block values.
+template <typename T>
+struct HostBsr {
+    size_t rows = 0, cols = 0, rb = 0, cb = 0;
+    std::vector<int64_t> starts{0};
+    std::vector<u32> columns;
+    std::vector<T> values;
+    size_t blocks() const { return columns.size(); }
+    size_t block_len() const { return rb * cb; }
+    size_t dofs() const { return rows * rb; }
+    const T *block(size_t at) const { return &values[at * rb * cb]; }
+};
+
+// The free-node block pattern (the pattern of CsrMatrix::from_block_connectivity over the
+// elements' free nodes, crates/solidfem-elastic/src/assembly.rs:55-81), built row by row from
+// the node-to-element adjacency, and the assembly map: for every stored block, its element
+// contributions in ascending element order, packed as (element << 7) | (10 a + b).
+struct AssemblyPattern {
+    HostBsr<double> pattern;               // values empty
+    std::vector<int64_t> contribution_starts;  // blocks + 1
+    std::vector<u32> contributions;
+};
+AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count);
+
+}  // namespace sf

@@ -1,0 +1,396 @@
+// Device kernels of the solve path for gfx950 (CDNA4, wave64).
+//
+// Sliced block layout ("node slices"). A block sparse matrix of RB x CB blocks is cut into
+// slices of NPW = 64 / RB block rows (21 nodes of 3 rows, 10 of 6, 64 scalar rows), one
+// wavefront per slice and one lane per scalar row. Within a slice, the k-th block of every row
+// is stored side by side: block columns at cols[(start + k) * NPW + j] and values at
+// vals[((start + k) * CB + q) * 64 + j * RB + p] for row j, block row p, block column q. Each
+// value load of a wavefront is then 64 consecutive elements, the RB lanes of a row read the same
+// column index (one fetch), and every lane sums its own row's blocks in stored order, column by
+// column, from zero: the summation order of the reference's block product
+// (crates/solidfem-solve/src/multigrid/block.rs:136-154), so the result is the same bits.
+// Rows are permuted inside windows so that a slice's rows have similar block counts; perm[]
+// maps a slice position back to the block row.
+//
+// Every kernel is compiled with -ffp-contract=off: no multiply-add is fused, as the reference
+// refuses mul_add (clippy.toml:8-9), so the device repeats the reference's roundings.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sf {
+namespace dev {
+
+constexpr int WAVE = 64;
+constexpr uint32_t NONE = 0xffffffffu;
+
+template <int RB>
+struct Shape {
+    static constexpr int NPW = WAVE / RB;  // block rows per slice
+    static constexpr int LANES = NPW * RB; // lanes holding a scalar row
+};
+
+template <typename T>
+struct Sliced {
+    const uint32_t *slice_start;  // slices + 1, in block slots per row
+    const uint32_t *row_len;      // per slice position: blocks in the row (0 for padding)
+    const uint32_t *perm;         // per slice position: block row, NONE for padding
+    const uint32_t *cols;         // per block slot and slice row
+    const T *vals;                // per block slot, block column and lane
+    uint32_t slices;
+};
+
+template <typename V, typename S>
+__device__ __forceinline__ V widen(S s) {
+    return (V)(double)s;
+}
+
+// Sum of one lane's row of A x, from zero in stored order: the row product of the reference.
+template <int RB, int CB, typename S, typename V>
+__device__ __forceinline__ V row_product(const Sliced<S> &a, uint32_t slot0, uint32_t len, int j, int lane,
+                                         const V *__restrict__ x) {
+    constexpr int NPW = Shape<RB>::NPW;
+    V sum = (V)0;
+    const uint32_t *cp = a.cols + (size_t)slot0 * NPW + j;
+    const S *vp = a.vals + (size_t)slot0 * CB * WAVE + lane;
+#pragma unroll 2
+    for (uint32_t k = 0; k < len; ++k) {
+        const uint32_t col = __builtin_nontemporal_load(cp + (size_t)k * NPW);
+        const V *xc = x + (size_t)col * CB;
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+            const S v = __builtin_nontemporal_load(vp + ((size_t)k * CB + q) * WAVE);
+            sum = sum + widen<V>(v) * xc[q];
+        }
+    }
+    return sum;
+}
+
+enum Epilogue { EPI_STORE = 0, EPI_RESIDUAL = 1, EPI_ADD = 2 };
+
+// y = A x (STORE), y = b - A x (RESIDUAL) or y = y + A x (ADD): the block products, the
+// residual (crates/solidfem-solve/src/multigrid/cycle.rs:548-564), the gather restriction over
+// the stored transpose (crates/solidfem-solve/src/gpu/layout.rs:249-291) and the prolongation
+// with correction (cycle.rs:512-521).
+template <int RB, int CB, typename S, typename V, int EPI>
+__global__ __launch_bounds__(256) void spmv_kernel(Sliced<S> a, const V *__restrict__ x, V *__restrict__ y,
+                                                   const V *__restrict__ b) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const uint32_t slice = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (slice >= a.slices) return;
+    const int lane = threadIdx.x % WAVE;
+    const int j = lane / RB, p = lane - j * RB;
+    if (j >= NPW) return;
+    const uint32_t pos = slice * NPW + j;
+    const uint32_t row = a.perm[pos];
+    if (row == NONE) return;
+    const V sum = row_product<RB, CB, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, x);
+    const size_t out = (size_t)row * RB + p;
+    if (EPI == EPI_STORE) y[out] = sum;
+    else if (EPI == EPI_RESIDUAL) y[out] = b[out] - sum;
+    else y[out] = y[out] + sum;
+}
+
+// Per scalar row, the rounding bound term of CsrMatrix::multiply_rounding_bound
+// (crates/solidfem-solve/src/csr.rs:142-154): (terms * eps) * sum |a x|, squared.
+template <int RB, int CB>
+__global__ __launch_bounds__(256) void rounding_bound_kernel(Sliced<double> a, const double *__restrict__ x,
+                                                             double *__restrict__ out) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const uint32_t slice = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (slice >= a.slices) return;
+    const int lane = threadIdx.x % WAVE;
+    const int j = lane / RB, p = lane - j * RB;
+    if (j >= NPW) return;
+    const uint32_t pos = slice * NPW + j;
+    const uint32_t row = a.perm[pos];
+    if (row == NONE) return;
+    const uint32_t len = a.row_len[pos], slot0 = a.slice_start[slice];
+    double sum = 0.0;
+    for (uint32_t k = 0; k < len; ++k) {
+        const uint32_t col = a.cols[(size_t)(slot0 + k) * NPW + j];
+        for (int q = 0; q < CB; ++q) sum = sum + fabs(a.vals[((size_t)(slot0 + k) * CB + q) * WAVE + lane] * x[(size_t)col * CB + q]);
+    }
+    const double bound = (double)(len * CB) * 2.220446049250313e-16 * sum;
+    out[(size_t)row * RB + p] = bound * bound;
+}
+
+// The block-Jacobi product of one lane's row: sum_q Dinv[p][q] r_q from zero, with the row's
+// RB residual values shuffled in from the lanes of the same block row (cycle.rs:655-674).
+template <int RB, typename S, typename V>
+__device__ __forceinline__ V block_jacobi_row(const S *__restrict__ dinv_row, V r_own, int lane_base) {
+    V z = (V)0;
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        const V rq = __shfl(r_own, lane_base + q, WAVE);
+        z = z + widen<V>(dinv_row[q]) * rq;
+    }
+    return z;
+}
+
+// The Chebyshev start (cycle.rs:591-610): z = D^-1 b, d = z / theta, x = d.
+template <int RB, typename S, typename V>
+__global__ __launch_bounds__(256) void cheb_start_kernel(uint32_t rows, const S *__restrict__ dinv, const V *__restrict__ b,
+                                                         V theta, V *__restrict__ d, V *__restrict__ x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * RB) return;
+    const size_t row = i / RB, p = i - row * RB;
+    const S *dr = dinv + row * RB * RB + p * RB;
+    V z = (V)0;
+#pragma unroll
+    for (int q = 0; q < RB; ++q) z = z + widen<V>(dr[q]) * b[row * RB + q];
+    const V dv = z / theta;
+    d[i] = dv;
+    x[i] = dv;
+}
+
+// One later Chebyshev step, fused (cycle.rs:611-650): ad = A d_in; r_out = r_in - ad;
+// z = D^-1 r_out; d_out = keep d_in + step z; x += d_out. With `sum_into` set, the step is the
+// last of a post-smoothing and also adds the smoothed correction to the cycle's x
+// (cycle.rs:534-536): sum_into += x.
+template <int RB, typename S, typename V>
+__global__ __launch_bounds__(256) void cheb_step_kernel(Sliced<S> a, const S *__restrict__ dinv, const V *__restrict__ d_in,
+                                                        const V *__restrict__ r_in, V keep, V step, V *__restrict__ d_out,
+                                                        V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const uint32_t slice = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (slice >= a.slices) return;  // whole wave leaves together
+    const int lane = threadIdx.x % WAVE;
+    const int j = lane / RB, p = lane - j * RB;
+    const bool in_wave = j < NPW;
+    const uint32_t pos = slice * NPW + (in_wave ? j : 0);
+    const uint32_t row = in_wave ? a.perm[pos] : NONE;
+    const bool live = row != NONE;
+    V ad = (V)0;
+    if (live) ad = row_product<RB, RB, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, d_in);
+    const size_t i = live ? (size_t)row * RB + p : 0;
+    const V r_new = live ? r_in[i] - ad : (V)0;
+    // every lane takes part in the shuffle; padding lanes carry zeros and store nothing
+    const S *drow = dinv + (live ? (size_t)row * RB * RB + (size_t)p * RB : 0);
+    V z = (V)0;
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        const V rq = __shfl(r_new, (in_wave ? j * RB : 0) + q, WAVE);
+        if (live) z = z + widen<V>(drow[q]) * rq;
+    }
+    if (!live) return;
+    const V dn = keep * d_in[i] + step * z;
+    const V xn = x[i] + dn;
+    d_out[i] = dn;
+    r_out[i] = r_new;
+    x[i] = xn;
+    if (sum_into) sum_into[i] = sum_into[i] + xn;
+}
+
+// x = x + y (cycle.rs:541-545), for a post-smoothing of degree one.
+template <typename V>
+__global__ void add_kernel(size_t n, V *__restrict__ x, const V *__restrict__ y) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + y[i];
+}
+
+template <typename To, typename From>
+__global__ void convert_kernel(size_t n, const From *__restrict__ in, To *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (To)in[i];
+}
+
+// The coarsest level's direct solve with the f64 Cholesky factor (block.rs:357-373), the
+// right-hand side widened from V and the solution rounded to V (cycle.rs:475-486). One
+// workgroup; `lt` is the factor transposed (column k of the factor contiguous), `xs` LDS.
+// Forward substitution runs right-looking: once x_k is final every later row subtracts its
+// term, so each row still subtracts in ascending k, the reference's order. Back substitution
+// also sums each row's terms in ascending k, which can only start once every later x is known:
+// the products are formed in parallel and one thread folds them in order. Every barrier is
+// reached by the whole workgroup.
+constexpr uint32_t COARSE_MAX = 3072;
+template <typename V>
+__global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t n, const double *__restrict__ lt,
+                                                             const V *__restrict__ rhs, V *__restrict__ out) {
+    __shared__ double xs[COARSE_MAX];
+    __shared__ double prod[COARSE_MAX];
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    constexpr int PER = COARSE_MAX / 1024;
+    double s[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const uint32_t i = t + m * nt;
+        s[m] = i < n ? (double)rhs[i] : 0.0;
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        if ((k % nt) == t) {
+            const uint32_t m = k / nt;
+            double v = 0.0;
+#pragma unroll
+            for (int mm = 0; mm < PER; ++mm)
+                if ((uint32_t)mm == m) v = s[mm];
+            xs[k] = v / lt[(size_t)k * n + k];
+        }
+        __syncthreads();
+        const double xv = xs[k];
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const uint32_t i = t + m * nt;
+            if (i > k && i < n) s[m] = s[m] - lt[(size_t)k * n + i] * xv;
+        }
+    }
+    __syncthreads();
+    for (uint32_t ii = n; ii-- > 0;) {
+        for (uint32_t k = ii + 1 + t; k < n; k += nt) prod[k] = lt[(size_t)ii * n + k] * xs[k];
+        __syncthreads();
+        if (t == 0) {
+            double sum = xs[ii];
+            for (uint32_t k = ii + 1; k < n; ++k) sum = sum - prod[k];
+            xs[ii] = sum / lt[(size_t)ii * n + ii];
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = t; i < n; i += nt) out[i] = (V)xs[i];
+}
+
+// ---- conjugate gradient vectors -------------------------------------------------------------
+
+// Scalars of a solve, resident on the device.
+struct PcgScalars {
+    double rz;         // r . z
+    double curvature;  // p . A p
+    double rr;         // r . r (recursive residual)
+    double rz_next;
+    double true_rr;    // ||b - A x||^2
+    double bound2;     // sum of squared row rounding bounds
+    double alpha;
+    int32_t valid;     // the curvature test of pcg.rs:219-227 passed
+    int32_t pad;
+};
+
+// Deterministic dot products: TREE order is a fixed two-stage reduction (each block sums a
+// fixed stride pattern by a fixed shuffle tree, then the last block to finish sums the block
+// partials in index order); REFERENCE order is one thread summing in index order from zero,
+// exactly crates/solidfem-solve/src/pcg.rs:285-291.
+constexpr int DOT_BLOCKS = 512;
+constexpr int DOT_THREADS = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    // fixed butterfly: the same pairs add in the same order on every run
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v = v + __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// mode: 0 = a.b, 1 = sum of a (b unused)
+template <int MODE>
+__global__ __launch_bounds__(DOT_THREADS) void dot_tree_kernel(size_t n, const double *__restrict__ a,
+                                                               const double *__restrict__ b, double *__restrict__ partials,
+                                                               uint32_t *__restrict__ ticket, double *__restrict__ result) {
+    __shared__ double wsum[DOT_THREADS / WAVE];
+    __shared__ bool last;
+    double v = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        v = v + (MODE == 0 ? a[i] * b[i] : a[i]);
+    v = wave_sum(v);
+    if ((threadIdx.x % WAVE) == 0) wsum[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < DOT_THREADS / WAVE; ++w) s = s + wsum[w];
+        partials[blockIdx.x] = s;
+        __threadfence();
+        const uint32_t done = atomicAdd(ticket, 1u);
+        last = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // the last block: partials in index order through the same fixed tree
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) s = s + __builtin_nontemporal_load(&partials[i]);
+    s = wave_sum(s);
+    if ((threadIdx.x % WAVE) == 0) wsum[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < DOT_THREADS / WAVE; ++w) t = t + wsum[w];
+        *result = t;
+        *ticket = 0u;
+    }
+}
+
+template <int MODE>
+__global__ void dot_serial_kernel(size_t n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ result) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) sum = sum + (MODE == 0 ? a[i] * b[i] : a[i]);
+    *result = sum;
+}
+
+// The update of pcg.rs:219-230: the curvature test, then x += alpha p and r += (-alpha) A p.
+static __global__ void pcg_update_kernel(size_t n, PcgScalars *__restrict__ s, const double *__restrict__ p,
+                                  const double *__restrict__ ap, double *__restrict__ x, double *__restrict__ r) {
+    const double curvature = s->curvature, rz = s->rz;
+    const bool valid = !(isnan(curvature) || curvature <= 0.0 || !isfinite(rz));
+    const double alpha = rz / curvature;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        s->valid = valid ? 1 : 0;
+        s->alpha = alpha;
+    }
+    if (!valid || i >= n) return;
+    x[i] = x[i] + alpha * p[i];
+    r[i] = r[i] + (-alpha) * ap[i];
+}
+
+// p = z + beta p with beta = rz_next / rz, then rz = rz_next (pcg.rs:257-263). The block that
+// owns index 0 retires the scalar after every block has read it: rz is read before any write.
+static __global__ void pcg_direction_kernel(size_t n, PcgScalars *__restrict__ s, const double *__restrict__ z, double *__restrict__ p,
+                                     uint32_t *__restrict__ ticket) {
+    const double beta = s->rz_next / s->rz;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(ticket, 1u);
+        if (done == gridDim.x - 1) {
+            s->rz = s->rz_next;
+            *ticket = 0u;
+        }
+    }
+}
+
+// Jacobi: z = r * inverse diagonal (preconditioner.rs:237-247).
+static __global__ void jacobi_kernel(size_t n, const double *__restrict__ inv, const double *__restrict__ r, double *__restrict__ z) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = r[i] * inv[i];
+}
+
+// Sliced values from a row-major block array: slot i of the sliced layout takes value
+// src[map[i]] rounded to T (the precision conversion of block.rs:26-35).
+template <typename T>
+__global__ void gather_convert_kernel(size_t n, const uint32_t *__restrict__ block_of_slot, int rb, int cb,
+                                      const double *__restrict__ src, T *__restrict__ dst) {
+    // dst index = ((slot * cb + q) * 64 + lane), lane = j * rb + p ; map gives the source block
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t lane = i % WAVE;
+    const size_t q = (i / WAVE) % cb;
+    const size_t slot_row = i / ((size_t)WAVE * cb);  // (slot, j) pair index = slot * npw + j
+    const int npw = WAVE / rb;
+    const int j = (int)(lane / rb), p = (int)(lane % rb);
+    if (j >= npw) {
+        dst[i] = (T)0;
+        return;
+    }
+    const uint32_t blk = block_of_slot[slot_row * npw + j];
+    dst[i] = blk == NONE ? (T)0 : (T)src[(size_t)blk * rb * cb + (size_t)p * cb + q];
+}
+
+// A plain copy, 16 bytes per lane per step, grid-stride.
+static __global__ void stream_copy_kernel(size_t n16, const uint4 *__restrict__ src, uint4 *__restrict__ dst) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+}  // namespace dev
+}  // namespace sf

@@ -1,0 +1,499 @@
+// Host input side of the solve path: meshes (crates/solidfem-mesh), the model's numbering, load
+// and rigid-body modes (crates/solidfem-elastic/src/{study,loads,rigid_modes}.rs), the block
+// pattern with its assembly map, and the deterministic host thread pool.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "sf_host.hpp"
+
+namespace sf {
+
+// ---- thread pool ----------------------------------------------------------------------------
+
+size_t host_threads() {
+    static size_t n = [] {
+        size_t h = std::thread::hardware_concurrency();
+        if (const char *env = std::getenv("SOLIDFEM_HOST_THREADS")) h = (size_t)std::max(1, std::atoi(env));
+        return std::max<size_t>(1, std::min<size_t>(h, 64));
+    }();
+    return n;
+}
+
+void parallel_for(size_t n, size_t min_chunk, const std::function<void(size_t, size_t)> &body) {
+    size_t threads = host_threads();
+    if (n == 0) return;
+    size_t chunks = std::min(threads, (n + min_chunk - 1) / std::max<size_t>(min_chunk, 1));
+    if (chunks <= 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::exception_ptr failure;
+    std::mutex m;
+    size_t per = (n + chunks - 1) / chunks;
+    for (size_t c = 0; c < chunks; ++c) {
+        size_t first = c * per, last = std::min(n, first + per);
+        if (first >= last) break;
+        pool.emplace_back([&, first, last] {
+            try {
+                body(first, last);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(m);
+                if (!failure) failure = std::current_exception();
+            }
+        });
+    }
+    for (auto &t : pool) t.join();
+    if (failure) std::rethrow_exception(failure);
+}
+
+// ---- meshes -----------------------------------------------------------------------------------
+
+const int FACE_LOCAL[4][6] = {{1, 2, 3, 5, 9, 8}, {0, 3, 2, 7, 9, 6}, {0, 1, 3, 4, 8, 7}, {0, 2, 1, 6, 5, 4}};
+const int EDGE_PAIRS[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+static double signed_volume(const Vec3 &a, const Vec3 &b, const Vec3 &c, const Vec3 &d) {
+    // crates/solidfem-mesh/src/tet.rs:9-16
+    double u0 = b[0] - a[0], u1 = b[1] - a[1], u2 = b[2] - a[2];
+    double v0 = c[0] - a[0], v1 = c[1] - a[1], v2 = c[2] - a[2];
+    double w0 = d[0] - a[0], w1 = d[1] - a[1], w2 = d[2] - a[2];
+    double det = u0 * (v1 * w2 - v2 * w1) - u1 * (v0 * w2 - v2 * w0) + u2 * (v0 * w1 - v1 * w0);
+    return det / 6.0;
+}
+
+static std::array<u32, 3> sorted_corners(const std::array<u32, 4> &t, int face) {
+    std::array<u32, 3> k = {t[FACE_LOCAL[face][0]], t[FACE_LOCAL[face][1]], t[FACE_LOCAL[face][2]]};
+    std::sort(k.begin(), k.end());
+    return k;
+}
+
+void validate_tet_mesh(const TetMesh &m) {
+    // crates/solidfem-mesh/src/tet.rs:59-109
+    if (m.tets.empty()) throw Error(SF_ERR_MESH_EMPTY, "the mesh has no tetrahedra");
+    for (size_t n = 0; n < m.positions.size(); ++n)
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(m.positions[n][a]))
+                throw Error(SF_ERR_MESH_NON_FINITE_POSITION, "node has a NaN or infinite coordinate", (int64_t)n);
+    std::vector<char> used(m.positions.size(), 0);
+    for (size_t t = 0; t < m.tets.size(); ++t) {
+        const auto &tet = m.tets[t];
+        for (u32 node : tet) {
+            if (node >= m.positions.size())
+                throw Error(SF_ERR_MESH_NODE_OUT_OF_RANGE, "tetrahedron names a node that does not exist", (int64_t)t, node);
+            used[node] = 1;
+        }
+        const Vec3 &a = m.positions[tet[0]], &b = m.positions[tet[1]], &c = m.positions[tet[2]], &d = m.positions[tet[3]];
+        double volume = signed_volume(a, b, c, d);
+        if (std::isnan(volume) || volume <= 0.0)
+            throw Error(SF_ERR_MESH_DEGENERATE_TET, "tetrahedron has no positive volume", (int64_t)t, 0, volume);
+        auto l2 = [](const Vec3 &p, const Vec3 &q) {
+            return (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]);
+        };
+        double longest = 0.0;
+        for (double e : {l2(a, b), l2(a, c), l2(a, d), l2(b, c), l2(b, d), l2(c, d)}) longest = std::max(longest, e);
+        double relative = 6.0 * volume / (longest * std::sqrt(longest));
+        if (std::isnan(relative) || relative < 1e-12)
+            throw Error(SF_ERR_MESH_FLAT_TET, "tetrahedron is flat to rounding", (int64_t)t, 0, relative);
+    }
+    for (size_t n = 0; n < used.size(); ++n)
+        if (!used[n]) throw Error(SF_ERR_MESH_UNUSED_NODE, "node belongs to no tetrahedron", (int64_t)n);
+    std::vector<std::array<u32, 3>> faces;
+    faces.reserve(4 * m.tets.size());
+    for (const auto &t : m.tets)
+        for (int f = 0; f < 4; ++f) faces.push_back(sorted_corners(t, f));
+    std::sort(faces.begin(), faces.end());
+    for (size_t i = 2; i < faces.size(); ++i)
+        if (faces[i] == faces[i - 2]) throw Error(SF_ERR_MESH_NON_MANIFOLD_FACE, "a face is shared by more than two tetrahedra");
+}
+
+TetMesh make_box_mesh(const double origin[3], const double size[3], const int32_t divisions[3]) {
+    // crates/solidfem-mesh/src/box_mesh.rs:22-90: six tetrahedra per cell around the cell's
+    // low-to-high diagonal, one per axis order of the walk; odd orders swap two corners.
+    static const int WALKS[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}};
+    for (int a = 0; a < 3; ++a) {
+        if (divisions[a] <= 0) throw Error(SF_ERR_MESH_INVALID_BOX, "each axis needs at least one division");
+        if (std::isnan(size[a]) || size[a] <= 0.0) throw Error(SF_ERR_MESH_INVALID_BOX, "each box length must be positive");
+    }
+    const size_t nx = (size_t)divisions[0] + 1, ny = (size_t)divisions[1] + 1, nz = (size_t)divisions[2] + 1;
+    auto line = [](double o, double s, size_t i, size_t d) { return i == d ? o + s : o + s * ((double)i / (double)d); };
+    TetMesh m;
+    m.positions.resize(nx * ny * nz);
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i)
+                m.positions[i + nx * (j + ny * k)] = Vec3{{line(origin[0], size[0], i, nx - 1), line(origin[1], size[1], j, ny - 1),
+                                                           line(origin[2], size[2], k, nz - 1)}};
+    m.tets.reserve(6 * (nx - 1) * (ny - 1) * (nz - 1));
+    for (size_t k = 0; k + 1 < nz; ++k)
+        for (size_t j = 0; j + 1 < ny; ++j)
+            for (size_t i = 0; i + 1 < nx; ++i)
+                for (int w = 0; w < 6; ++w) {
+                    size_t at[3] = {i, j, k};
+                    u32 v[4];
+                    v[0] = (u32)(at[0] + nx * (at[1] + ny * at[2]));
+                    for (int s = 0; s < 3; ++s) {
+                        at[WALKS[w][s]] += 1;
+                        v[s + 1] = (u32)(at[0] + nx * (at[1] + ny * at[2]));
+                    }
+                    if (w < 3) m.tets.push_back({v[0], v[1], v[2], v[3]});
+                    else m.tets.push_back({v[0], v[2], v[1], v[3]});
+                }
+    return m;
+}
+
+QuadMesh upgrade_to_quadratic(const TetMesh &mesh) {
+    // crates/solidfem-mesh/src/quadratic.rs:24-71: one midpoint per distinct edge, numbered
+    // after the corners in ascending (min, max) corner order. Edges are packed into u64 keys.
+    QuadMesh q;
+    q.vertex_count = mesh.positions.size();
+    std::vector<u64> edges(mesh.tets.size() * 6);
+    parallel_for(mesh.tets.size(), 4096, [&](size_t first, size_t last) {
+        for (size_t t = first; t < last; ++t)
+            for (int e = 0; e < 6; ++e) {
+                u32 x = mesh.tets[t][EDGE_PAIRS[e][0]], y = mesh.tets[t][EDGE_PAIRS[e][1]];
+                edges[6 * t + e] = ((u64)std::min(x, y) << 32) | std::max(x, y);
+            }
+    });
+    std::vector<u64> unique = edges;
+    std::sort(unique.begin(), unique.end());
+    unique.erase(std::unique(unique.begin(), unique.end()), unique.end());
+    q.nodes.resize(q.vertex_count + unique.size());
+    std::copy(mesh.positions.begin(), mesh.positions.end(), q.nodes.begin());
+    parallel_for(unique.size(), 8192, [&](size_t first, size_t last) {
+        for (size_t e = first; e < last; ++e) {
+            const Vec3 &p = mesh.positions[unique[e] >> 32], &r = mesh.positions[unique[e] & 0xffffffffu];
+            q.nodes[q.vertex_count + e] = Vec3{{0.5 * (p[0] + r[0]), 0.5 * (p[1] + r[1]), 0.5 * (p[2] + r[2])}};
+        }
+    });
+    q.tets.resize(mesh.tets.size());
+    parallel_for(mesh.tets.size(), 4096, [&](size_t first, size_t last) {
+        for (size_t t = first; t < last; ++t) {
+            for (int c = 0; c < 4; ++c) q.tets[t][c] = mesh.tets[t][c];
+            for (int e = 0; e < 6; ++e) {
+                size_t at = std::lower_bound(unique.begin(), unique.end(), edges[6 * t + e]) - unique.begin();
+                q.tets[t][4 + e] = (u32)(q.vertex_count + at);
+            }
+        }
+    });
+    q.corners = mesh.tets;
+    return q;
+}
+
+std::vector<Face> QuadMesh::boundary_faces() const {
+    // crates/solidfem-mesh/src/quadratic.rs:89-124: faces met once, ascending by sorted corners,
+    // then by (tet, face).
+    struct Key {
+        std::array<u32, 3> k;
+        u32 tet, face;
+        bool operator<(const Key &o) const {
+            if (k != o.k) return k < o.k;
+            if (tet != o.tet) return tet < o.tet;
+            return face < o.face;
+        }
+    };
+    std::vector<Key> keys(corners.size() * 4);
+    for (size_t t = 0; t < corners.size(); ++t)
+        for (int f = 0; f < 4; ++f) keys[4 * t + f] = Key{sorted_corners(corners[t], f), (u32)t, (u32)f};
+    std::sort(keys.begin(), keys.end());
+    std::vector<Face> out;
+    for (size_t at = 0; at < keys.size();) {
+        size_t end = at + 1;
+        while (end < keys.size() && keys[end].k == keys[at].k) ++end;
+        if (end == at + 1) {
+            Face f;
+            f.tet = keys[at].tet;
+            f.face = keys[at].face;
+            for (int s = 0; s < 6; ++s) f.nodes[s] = tets[f.tet][FACE_LOCAL[f.face][s]];
+            out.push_back(f);
+        }
+        at = end;
+    }
+    return out;
+}
+
+// ---- workload inputs ------------------------------------------------------------------------
+
+static u64 mix64(u64 x) {
+    // SplitMix64 finaliser: the counter-based generator of every seeded workload input.
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+static double uniform01(u64 bits) { return (double)(bits >> 11) * (1.0 / 9007199254740992.0); }
+static double gaussian(u64 seed, u64 counter) {
+    double u1 = ((double)(mix64(seed + 2 * counter) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+    double u2 = uniform01(mix64(seed + 2 * counter + 1));
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+Model model_from_workload(const sf_workload &w) {
+    TetMesh linear = make_box_mesh(w.origin, w.size, w.divisions);
+    if (w.jitter_fraction > 0.0) {
+        // Interior vertices move by (2u - 1) * (fraction * h) per axis, u from (seed ^ (3 v + axis)).
+        const size_t nx = (size_t)w.divisions[0] + 1, ny = (size_t)w.divisions[1] + 1;
+        double h[3];
+        for (int a = 0; a < 3; ++a) h[a] = w.size[a] / (double)w.divisions[a];
+        for (size_t v = 0; v < linear.positions.size(); ++v) {
+            size_t i = v % nx, j = (v / nx) % ny, k = v / (nx * ny);
+            if (i == 0 || j == 0 || k == 0 || i == (size_t)w.divisions[0] || j == (size_t)w.divisions[1] ||
+                k == (size_t)w.divisions[2])
+                continue;
+            for (int a = 0; a < 3; ++a) {
+                double u = uniform01(mix64(w.jitter_seed ^ (u64)(3 * v + a)));
+                linear.positions[v][a] = linear.positions[v][a] + (2.0 * u - 1.0) * (w.jitter_fraction * h[a]);
+            }
+        }
+    }
+    validate_tet_mesh(linear);
+    Model model;
+    model.mesh = upgrade_to_quadratic(linear);
+    const auto &nodes = model.mesh.nodes;
+    for (size_t n = 0; n < nodes.size(); ++n)
+        if (std::fabs(nodes[n][w.clamp_axis] - w.clamp_value) < 1e-9) model.fixed.insert((u32)n);
+    int u = (w.load_axis + 1) % 3, v = (w.load_axis + 2) % 3;
+    u64 index = 0;
+    for (const Face &f : model.mesh.boundary_faces()) {
+        bool on = true;
+        for (int c = 0; c < 3; ++c)
+            if (!(std::fabs(nodes[f.nodes[c]][w.load_axis] - w.load_value) < 1e-9)) on = false;
+        if (!on) continue;
+        std::array<double, 3> t{0, 0, 0};
+        if (w.traction_kind == SF_TRACTION_UNIFORM) {
+            t = {w.traction[0], w.traction[1], w.traction[2]};
+        } else if (w.traction_kind == SF_TRACTION_RANDOM) {
+            for (int a = 0; a < 3; ++a)
+                t[a] = w.traction_scale * gaussian(w.traction_seed, ((u64)(uint32_t)w.load_case << 40) ^ (index << 2) ^ (u64)a);
+        } else {
+            const Vec3 &p = nodes[f.nodes[0]], &q = nodes[f.nodes[1]], &r = nodes[f.nodes[2]];
+            double cu = (p[u] + q[u] + r[u]) / 3.0 - w.axis_u;
+            double cv = (p[v] + q[v] + r[v]) / 3.0 - w.axis_v;
+            double radius = std::sqrt(cu * cu + cv * cv);
+            t[w.load_axis] = 0.0;
+            t[u] = -w.traction_scale * cv / radius;
+            t[v] = w.traction_scale * cu / radius;
+        }
+        model.tractions.push_back({f, t});
+        ++index;
+    }
+    model.field.kind = w.material_kind;
+    model.field.e = w.youngs;
+    model.field.nu = w.poisson;
+    model.field.core_e = w.core_youngs;
+    model.field.core_nu = w.core_poisson;
+    model.field.shell_depth = w.shell_depth;
+    int longest = 0;
+    for (int a = 1; a < 3; ++a)
+        if (w.size[a] > w.size[longest]) longest = a;
+    model.field.transverse[0] = (longest + 1) % 3;
+    model.field.transverse[1] = (longest + 2) % 3;
+    for (int a = 0; a < 3; ++a) {
+        model.field.lo[a] = w.origin[a];
+        model.field.hi[a] = w.origin[a] + w.size[a];
+    }
+    model.quadrature = w.quadrature_points == 14 ? 14 : 4;
+    return model;
+}
+
+// ---- numbering and load ------------------------------------------------------------------
+
+// Consistent nodal load of a uniform traction over a 6-node face with the 6-point triangle rule
+// (crates/solidfem-elastic/src/loads.rs:10-71, crates/solidfem-elastic/src/quadrature.rs:87-94).
+static void face_load(const QuadMesh &mesh, const Face &face, const std::array<double, 3> &t, std::vector<double> &load) {
+    static const double TA = 0.091576213509771, TB = 0.445948490915965;
+    static const double WA = 0.109951743655322, WB = 0.223381589678011;
+    static const double RULE[6][3] = {{TA, TA, WA}, {1.0 - 2.0 * TA, TA, WA}, {TA, 1.0 - 2.0 * TA, WA},
+                                      {TB, TB, WB}, {1.0 - 2.0 * TB, TB, WB}, {TB, 1.0 - 2.0 * TB, WB}};
+    const Vec3 *p[6];
+    for (int n = 0; n < 6; ++n) p[n] = &mesh.nodes[face.nodes[n]];
+    for (const auto &qp : RULE) {
+        const double r = qp[0], s = qp[1], l = 1.0 - r - s;
+        const double n[6] = {l * (2.0 * l - 1.0), r * (2.0 * r - 1.0), s * (2.0 * s - 1.0), 4.0 * l * r, 4.0 * r * s, 4.0 * s * l};
+        const double dr[6] = {-(4.0 * l - 1.0), 4.0 * r - 1.0, 0.0, 4.0 * (l - r), 4.0 * s, -4.0 * s};
+        const double ds[6] = {-(4.0 * l - 1.0), 0.0, 4.0 * s - 1.0, -4.0 * r, 4.0 * r, 4.0 * (l - s)};
+        double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        for (int k = 0; k < 6; ++k)
+            for (int x = 0; x < 3; ++x) {
+                a[x] += dr[k] * (*p[k])[x];
+                b[x] += ds[k] * (*p[k])[x];
+            }
+        const double c0 = a[1] * b[2] - a[2] * b[1], c1 = a[2] * b[0] - a[0] * b[2], c2 = a[0] * b[1] - a[1] * b[0];
+        const double measure = 0.5 * std::sqrt(c0 * c0 + c1 * c1 + c2 * c2) * qp[2];
+        for (int k = 0; k < 6; ++k)
+            for (int x = 0; x < 3; ++x) load[3 * (size_t)face.nodes[k] + x] += t[x] * n[k] * measure;
+    }
+}
+
+FreeProblem number_and_load(const Model &model) {
+    const size_t nn = model.mesh.nodes.size();
+    if (model.fixed.empty() && model.prescribed.empty()) throw Error(SF_ERR_MODEL_NO_FIXED_NODES, "no node is fixed");
+    for (u32 node : model.fixed)
+        if (node >= nn) throw Error(SF_ERR_MODEL_NODE_OUT_OF_RANGE, "fixed node out of range", node);
+    // crates/solidfem-elastic/src/study.rs:696-735: nodal forces, then tractions, in the order given
+    std::vector<double> load(3 * nn, 0.0);
+    for (const auto &f : model.nodal_forces) {
+        if (f.first >= nn) throw Error(SF_ERR_MODEL_NODE_OUT_OF_RANGE, "force node out of range", f.first);
+        for (double c : f.second)
+            if (!std::isfinite(c)) throw Error(SF_ERR_MODEL_NON_FINITE_LOAD, "a load is not finite");
+        for (int a = 0; a < 3; ++a) load[3 * (size_t)f.first + a] += f.second[a];
+    }
+    if (!model.tractions.empty()) {
+        std::vector<Face> boundary = model.mesh.boundary_faces();
+        std::sort(boundary.begin(), boundary.end());
+        for (const auto &t : model.tractions) {
+            if (!std::binary_search(boundary.begin(), boundary.end(), t.first))
+                throw Error(SF_ERR_MODEL_FACE_NOT_ON_BOUNDARY, "the face is not on the boundary", t.first.tet, t.first.face);
+            for (double c : t.second)
+                if (!std::isfinite(c)) throw Error(SF_ERR_MODEL_NON_FINITE_LOAD, "a traction is not finite");
+            face_load(model.mesh, t.first, t.second, load);
+        }
+    }
+    FreeProblem p;
+    p.free.assign(nn, FIXED);
+    for (size_t n = 0; n < nn; ++n)
+        if (!model.fixed.count((u32)n)) p.free[n] = (u32)p.free_count++;
+    p.rhs.reserve(3 * p.free_count);
+    for (size_t n = 0; n < nn; ++n)
+        if (p.free[n] != FIXED)
+            for (int a = 0; a < 3; ++a) p.rhs.push_back(load[3 * n + a]);
+    // study.rs:662-683
+    for (const auto &pr : model.prescribed) {
+        if (pr.first.second > 2) throw Error(SF_ERR_MODEL_AXIS_OUT_OF_RANGE, "axis out of range", pr.first.second);
+        if (!std::isfinite(pr.second)) throw Error(SF_ERR_MODEL_NON_FINITE_LOAD, "a prescribed value is not finite");
+        if (pr.first.first >= nn) throw Error(SF_ERR_MODEL_NODE_OUT_OF_RANGE, "prescribed node out of range", pr.first.first);
+        u32 index = p.free[pr.first.first];
+        if (index == FIXED) throw Error(SF_ERR_MODEL_PRESCRIBED_ON_FIXED, "prescribed node is also fixed", pr.first.first);
+        p.prescribed.push_back({3 * (size_t)index + pr.first.second, pr.second});
+    }
+    std::sort(p.prescribed.begin(), p.prescribed.end(), [](auto &a, auto &b) { return a.first < b.first; });
+    bool any = false;
+    for (double r : p.rhs) any = any || r != 0.0;
+    for (auto &pr : p.prescribed) any = any || pr.second != 0.0;
+    if (!any) throw Error(SF_ERR_MODEL_NO_EFFECTIVE_LOAD, "the model has no effective load");
+    return p;
+}
+
+std::vector<double> rigid_body_modes(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count) {
+    // crates/solidfem-elastic/src/rigid_modes.rs:36-83: rotations about the free-node centroid,
+    // scaled by the largest coordinate offset from it.
+    double c[3] = {0, 0, 0};
+    for (size_t n = 0; n < free.size(); ++n)
+        if (free[n] != FIXED)
+            for (int a = 0; a < 3; ++a) c[a] += mesh.nodes[n][a];
+    const double divisor = (double)std::max<size_t>(free_count, 1);
+    for (double &x : c) x /= divisor;
+    double reach = 0.0;
+    for (size_t n = 0; n < free.size(); ++n)
+        if (free[n] != FIXED)
+            for (int a = 0; a < 3; ++a) reach = std::max(reach, std::fabs(mesh.nodes[n][a] - c[a]));
+    const double scale = reach > 0.0 ? 1.0 / reach : 1.0;
+    std::vector<double> modes(18 * free_count, 0.0);
+    parallel_for(free.size(), 16384, [&](size_t first, size_t last) {
+        for (size_t n = first; n < last; ++n) {
+            if (free[n] == FIXED) continue;
+            const double x = (mesh.nodes[n][0] - c[0]) * scale, y = (mesh.nodes[n][1] - c[1]) * scale,
+                         z = (mesh.nodes[n][2] - c[2]) * scale;
+            double *m = &modes[18 * (size_t)free[n]];
+            const double rows[18] = {1, 0, 0, 0, z, -y, 0, 1, 0, -z, 0, x, 0, 0, 1, y, -x, 0};
+            std::memcpy(m, rows, sizeof(rows));
+        }
+    });
+    return modes;
+}
+
+// ---- pattern ----------------------------------------------------------------------------
+
+AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count) {
+    // Node -> elements adjacency (ascending elements), by counting sort.
+    const size_t ne = mesh.tets.size();
+    std::vector<int64_t> adj_start(free_count + 1, 0);
+    for (const auto &t : mesh.tets)
+        for (u32 node : t)
+            if (free[node] != FIXED) adj_start[(size_t)free[node] + 1] += 1;
+    for (size_t i = 0; i < free_count; ++i) adj_start[i + 1] += adj_start[i];
+    std::vector<u32> adj((size_t)adj_start[free_count]);  // (element << 4) | local
+    {
+        std::vector<int64_t> fill(adj_start.begin(), adj_start.end() - 1);
+        for (size_t e = 0; e < ne; ++e)
+            for (int a = 0; a < 10; ++a) {
+                u32 f = free[mesh.tets[e][a]];
+                if (f != FIXED) adj[(size_t)fill[f]++] = ((u32)e << 4) | (u32)a;
+            }
+    }
+    AssemblyPattern out;
+    auto &pat = out.pattern;
+    pat.rows = pat.cols = free_count;
+    pat.rb = pat.cb = 3;
+    // Pass 1: per row, its sorted unique free neighbour set and contribution count.
+    std::vector<int64_t> row_len(free_count, 0), row_contrib(free_count, 0);
+    parallel_for(free_count, 2048, [&](size_t first, size_t last) {
+        std::vector<u32> cols;
+        for (size_t r = first; r < last; ++r) {
+            cols.clear();
+            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at) {
+                const auto &t = mesh.tets[adj[(size_t)at] >> 4];
+                for (u32 node : t)
+                    if (free[node] != FIXED) cols.push_back(free[node]);
+            }
+            row_contrib[r] = (int64_t)cols.size();
+            std::sort(cols.begin(), cols.end());
+            row_len[r] = std::unique(cols.begin(), cols.end()) - cols.begin();
+        }
+    });
+    pat.starts.assign(free_count + 1, 0);
+    std::vector<int64_t> contrib_row(free_count + 1, 0);
+    for (size_t r = 0; r < free_count; ++r) {
+        pat.starts[r + 1] = pat.starts[r] + row_len[r];
+        contrib_row[r + 1] = contrib_row[r] + row_contrib[r];
+    }
+    const size_t nnzb = (size_t)pat.starts[free_count];
+    pat.columns.assign(nnzb, 0);
+    out.contribution_starts.assign(nnzb + 1, 0);
+    out.contributions.assign((size_t)contrib_row[free_count], 0);
+    // Pass 2: columns, then each block's contributions in ascending element order.
+    parallel_for(free_count, 2048, [&](size_t first, size_t last) {
+        std::vector<u32> cols;
+        std::vector<int64_t> count;
+        for (size_t r = first; r < last; ++r) {
+            cols.clear();
+            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at) {
+                const auto &t = mesh.tets[adj[(size_t)at] >> 4];
+                for (u32 node : t)
+                    if (free[node] != FIXED) cols.push_back(free[node]);
+            }
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            std::copy(cols.begin(), cols.end(), pat.columns.begin() + pat.starts[r]);
+            count.assign(cols.size(), 0);
+            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at) {
+                const auto &t = mesh.tets[adj[(size_t)at] >> 4];
+                for (u32 node : t)
+                    if (free[node] != FIXED) count[std::lower_bound(cols.begin(), cols.end(), free[node]) - cols.begin()] += 1;
+            }
+            // block-local offsets within this row's contribution range
+            int64_t base = contrib_row[r];
+            for (size_t k = 0; k < cols.size(); ++k) {
+                out.contribution_starts[(size_t)pat.starts[r] + k] = base;
+                base += count[k];
+            }
+            std::vector<int64_t> fill(cols.size());
+            for (size_t k = 0; k < cols.size(); ++k) fill[k] = out.contribution_starts[(size_t)pat.starts[r] + k];
+            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at) {
+                const u32 e = adj[(size_t)at] >> 4, a = adj[(size_t)at] & 15u;
+                const auto &t = mesh.tets[e];
+                for (u32 b = 0; b < 10; ++b) {
+                    u32 f = free[t[b]];
+                    if (f == FIXED) continue;
+                    size_t k = std::lower_bound(cols.begin(), cols.end(), f) - cols.begin();
+                    out.contributions[(size_t)fill[k]++] = (e << 7) | (10 * a + b);
+                }
+            }
+        }
+    });
+    out.contribution_starts[nnzb] = contrib_row[free_count];
+    return out;
+}
+
+}  // namespace sf

@@ -1,0 +1,235 @@
+"""The HIP path against the oracle on the GPU, through the C ABI.
+
+Bit-exact wherever the reference fixes the order (integer/index work and every floating-point
+kernel here is compiled without contraction): assembly and loads, the BSR product, the
+Chebyshev smoother, the V-cycle in all three precisions, the coarse solve, and -- with the
+dot products in the reference's order -- the whole solve, which must reproduce the reference's
+own fingerprints (crates/solidfem/tests/validation/multigrid_precision.rs:231-250).
+"""
+import numpy as np
+import pytest
+
+import refcpu
+import solidfem_amd as sf
+from workloads import FINGERPRINT_CANTILEVER, FINGERPRINT_PLATE, bar_config, block_config, loaded, plate_config
+
+pytestmark = pytest.mark.gpu
+
+WORKLOADS = {
+    "cantilever": loaded(*FINGERPRINT_CANTILEVER[:2]),
+    "plate": loaded(*FINGERPRINT_PLATE[:2]),
+    "jittered-plate": plate_config(divisions=(12, 12, 2)),
+    "random-block": block_config(3, divisions=(6, 4, 4)),
+    "shell-core-torsion": bar_config(divisions=(16, 4, 4)),
+}
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+def test_device_assembly_is_bit_identical(ctx, name):
+    w = WORKLOADS[name]
+    ref = refcpu.Problem(w)
+    study = sf.StaticStudy.new(ctx, sf.Model.from_workload(w))  # Jacobi: no setup
+    rs, cols, vals = study.matrix()
+    r_rs, r_cols, r_vals = ref.csr()
+    assert np.array_equal(rs, r_rs) and np.array_equal(cols, r_cols)
+    assert np.array_equal(bits(vals), bits(r_vals))
+    assert np.array_equal(bits(study.rhs()), bits(ref.rhs()))
+
+
+@pytest.mark.parametrize("points", [4, 14])
+def test_device_element_stiffness_is_bit_identical(ctx, points):
+    w = sf.Workload(**{**bar_config(divisions=(4, 2, 2)).__dict__, "quadrature_points": points,
+                       "jitter_fraction": 0.08, "jitter_seed": 11})
+    ref = refcpu.Problem(w)
+    model = sf.Model.from_workload(w)
+    ke = model.element_stiffness(ctx, 0, 96)
+    for e in range(96):
+        assert np.array_equal(bits(ke[e]), bits(ref.element_stiffness(e))), e
+
+
+def test_device_product_and_rounding_bound_are_bit_identical(ctx):
+    ref = refcpu.Problem(plate_config(divisions=(12, 12, 2)))
+    rs, cols, vals = ref.csr()
+    csr = sf.CsrMatrix.new(ctx, rs, cols, vals)
+    x = np.random.default_rng(1).standard_normal(len(rs) - 1)
+    assert np.array_equal(bits(csr.multiply(x)), bits(refcpu.csr_multiply(rs, cols, vals, x)))
+    assert csr.multiply_rounding_bound(x) == refcpu.csr_rounding_bound(rs, cols, vals, x)
+
+
+def device_multigrid(ctx, ref, precision, coarse):
+    rs, cols, vals = ref.csr()
+    csr = sf.CsrMatrix.new(ctx, rs, cols, vals)
+    setup = sf.MultigridSetup.new(ctx, csr, ref.rigid_modes(), ref.free_positions(),
+                                  sf.MultigridOptions(coarse_dofs=coarse, precision=precision))
+    while not setup.step(2 ** 62):
+        pass
+    return csr, setup.finish()
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2], ids=["double", "single-storage", "single"])
+@pytest.mark.parametrize("name,coarse", [("cantilever", 100), ("plate", 100), ("jittered-plate", 150),
+                                         ("shell-core-torsion", 90)])
+def test_device_v_cycle_is_bit_identical(ctx, name, coarse, precision):
+    ref = refcpu.Problem(WORKLOADS[name])
+    mg_ref = refcpu.Multigrid(ref, refcpu.options(precision=precision, coarse_dofs=coarse))
+    _, mg = device_multigrid(ctx, ref, precision, coarse)
+    assert mg.stats().levels == mg_ref.levels >= 2
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal(ref.free_dofs)
+    for level in range(mg_ref.levels):
+        b = rng.standard_normal(mg_ref.info(level)["rows"] * mg_ref.info(level)["rb"])
+        assert np.array_equal(bits(mg.smooth(level, b)), bits(mg_ref.smooth(level, b))), f"smoother level {level}"
+    z = mg.apply(r)
+    assert np.array_equal(bits(z), bits(mg_ref.apply(r)))
+    assert np.array_equal(bits(mg.apply(r)), bits(z)), "a second application differs"
+
+
+def test_device_coarse_solve_matches_the_factor(ctx):
+    ref = refcpu.Problem(WORKLOADS["plate"])
+    mg_ref = refcpu.Multigrid(ref, refcpu.options(precision=0, coarse_dofs=100))
+    _, mg = device_multigrid(ctx, ref, 0, 100)
+    factor = np.tril(mg_ref.coarse_factor())
+    b = np.random.default_rng(2).standard_normal(factor.shape[0])
+    x = mg.solve_coarsest(b)
+    # forward then back substitution in the reference's order (block.rs:357-373)
+    y = b.copy()
+    n = len(y)
+    for i in range(n):
+        s = y[i]
+        for k in range(i):
+            s -= factor[i, k] * y[k]
+        y[i] = s / factor[i, i]
+    for i in reversed(range(n)):
+        s = y[i]
+        for k in range(i + 1, n):
+            s -= factor[k, i] * y[k]
+        y[i] = s / factor[i, i]
+    assert np.array_equal(bits(x), bits(y))
+
+
+def study(ctx, w, precision, coarse=100, dot=sf.DotOrder.REFERENCE, tol=1e-9, slice_rows=2 ** 62, degree=3):
+    b = sf.StudyBuilder.new(ctx, sf.Model.from_workload(w), sf.PcgOptions(tolerance=tol, dot_order=dot),
+                            sf.MultigridOptions(coarse_dofs=coarse, precision=precision, smoother_degree=degree))
+    while True:
+        p = b.step(slice_rows)
+        if p.elements_done == p.elements_total and p.setup_finished:
+            break
+    return b.finish()
+
+
+@pytest.mark.parametrize("case", [FINGERPRINT_CANTILEVER, FINGERPRINT_PLATE], ids=["cantilever", "plate"])
+def test_gpu_solve_reproduces_the_reference_fingerprint(ctx, case):
+    size, div, iterations, fingerprint = case
+    s = study(ctx, loaded(size, div), sf.Precision.DOUBLE)
+    p = s.solve()
+    assert p.outcome == sf.StepOutcome.CONVERGED
+    disp = s.displacements()
+    assert (p.iteration, refcpu.fingerprint(disp)) == (iterations, fingerprint)
+
+
+@pytest.mark.parametrize("case,precision,expected", [
+    (FINGERPRINT_CANTILEVER, sf.Precision.SINGLE, 65), (FINGERPRINT_PLATE, sf.Precision.SINGLE, 90),
+    (FINGERPRINT_CANTILEVER, sf.Precision.SINGLE_STORAGE, 61), (FINGERPRINT_PLATE, sf.Precision.SINGLE_STORAGE, 85),
+], ids=["cantilever-single", "plate-single", "cantilever-storage", "plate-storage"])
+def test_gpu_single_precisions_match_the_pinned_iterations_and_the_oracle_bits(ctx, case, precision, expected):
+    size, div, _, _ = case
+    s = study(ctx, loaded(size, div), precision)
+    p = s.solve()
+    ref_disp, r = refcpu.solve(loaded(size, div), precision=int(precision), coarse_dofs=100)
+    assert p.iteration == r.iterations == expected
+    assert np.array_equal(bits(s.displacements()), bits(ref_disp))
+
+
+def test_tree_order_solve_is_deterministic_and_slice_invariant(ctx):
+    w = plate_config(divisions=(16, 16, 2))
+    first = study(ctx, w, sf.Precision.SINGLE, coarse=200, dot=sf.DotOrder.TREE, tol=1e-8)
+    p = first.solve()
+    assert p.outcome == sf.StepOutcome.CONVERGED
+    d = first.displacements()
+    again = study(ctx, w, sf.Precision.SINGLE, coarse=200, dot=sf.DotOrder.TREE, tol=1e-8)
+    again.solve()
+    assert np.array_equal(bits(again.displacements()), bits(d))
+    for slice_it in (1, 7, 53):
+        s = study(ctx, w, sf.Precision.SINGLE, coarse=200, dot=sf.DotOrder.TREE, tol=1e-8)
+        while True:
+            q = s.step(slice_it)
+            if q.outcome != sf.StepOutcome.RUNNING:
+                break
+        assert q.iteration == p.iteration
+        assert np.array_equal(bits(s.displacements()), bits(d)), slice_it
+
+
+def test_tree_order_solve_matches_the_oracle_in_the_energy_norm(ctx):
+    # Both stop at a true relative residual of 1e-10; the energy-norm difference of two such
+    # solutions is at most (||r1|| + ||r2||) / sqrt(lambda_min), bounded here against the
+    # solution's own energy by the measured residuals.
+    w = plate_config(divisions=(16, 16, 2))
+    s = study(ctx, w, sf.Precision.DOUBLE, coarse=200, dot=sf.DotOrder.TREE, tol=1e-10)
+    p = s.solve()
+    ref = refcpu.Problem(w)
+    mg = refcpu.Multigrid(ref, refcpu.options(precision=0, coarse_dofs=200))
+    ref_disp, r = mg.solve(1e-10)
+    free = ref.free_map() != 0xFFFFFFFF
+    e = (s.displacements() - ref_disp)[free].ravel()
+    u = ref_disp[free].ravel()
+    rs, cols, vals = ref.csr()
+    energy_err = np.sqrt(abs(e @ refcpu.csr_multiply(rs, cols, vals, e)))
+    energy = np.sqrt(u @ refcpu.csr_multiply(rs, cols, vals, u))
+    print(f"energy-norm difference {energy_err / energy:.3e}, iterations {p.iteration} vs {r.iterations}")
+    assert abs(int(p.iteration) - int(r.iterations)) <= 2
+    assert energy_err <= 1e-6 * energy
+
+
+def laplacian(n):
+    rs = np.zeros(n + 1, dtype=np.int64)
+    cols, vals = [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                cols.append(j)
+                vals.append(2.0 if i == j else -1.0)
+        rs[i + 1] = len(cols)
+    return rs, np.array(cols, dtype=np.uint32), np.array(vals)
+
+
+@pytest.mark.parametrize("dot", [sf.DotOrder.REFERENCE, sf.DotOrder.TREE])
+def test_jacobi_pcg_on_the_laplacian(ctx, dot):
+    # solver/pcg.rs:43-126: closed form, slicing, cancel, the rounding floor
+    n = 50
+    rs, cols, vals = laplacian(n)
+    b = np.zeros(n)
+    b[-1] = 1.0
+    csr = sf.CsrMatrix.new(ctx, rs, cols, vals)
+    pcg = sf.Pcg.new(ctx, csr, None, b, sf.PcgOptions(dot_order=dot))
+    assert pcg.step(10000).outcome == sf.StepOutcome.CONVERGED
+    x = pcg.estimate()
+    assert np.all(np.abs(x - (np.arange(n) + 1) / (n + 1)) < 1e-8)
+    if dot == sf.DotOrder.REFERENCE:
+        xr, _ = refcpu.pcg_jacobi(rs, cols, vals, b)
+        assert np.array_equal(bits(x), bits(xr))
+    sliced = sf.Pcg.new(ctx, csr, None, b, sf.PcgOptions(dot_order=dot))
+    while sliced.step(3).outcome == sf.StepOutcome.RUNNING:
+        pass
+    assert np.array_equal(bits(sliced.estimate()), bits(x))
+    c = sf.Pcg.new(ctx, csr, None, b, sf.PcgOptions(dot_order=dot))
+    c.step(3)
+    out = c.step(1000, cancel=lambda: True)
+    assert out.outcome == sf.StepOutcome.CANCELLED and out.iteration == 3
+    assert c.step(1000).outcome == sf.StepOutcome.CANCELLED
+    floor = sf.Pcg.new(ctx, csr, None, b, sf.PcgOptions(tolerance=1e-18, dot_order=dot))
+    p = floor.step(100000)
+    assert p.outcome == sf.StepOutcome.CONVERGED and p.at_rounding_floor and p.relative_residual > 1e-18
+
+
+def test_an_indefinite_matrix_is_refused_by_name(ctx):
+    rs = np.array([0, 2, 4], dtype=np.int64)
+    cols = np.array([0, 1, 0, 1], dtype=np.uint32)
+    csr = sf.CsrMatrix.new(ctx, rs, cols, np.array([1.0, 3.0, 3.0, 1.0]))
+    pcg = sf.Pcg.new(ctx, csr, None, np.array([1.0, -1.0]))
+    with pytest.raises(sf.PcgError) as e:
+        pcg.step(100)
+    assert e.value.status == 10

@@ -1,0 +1,44 @@
+"""The workloads the tests and the benchmark share (sizes and seeds stated here)."""
+from solidfem_amd import Workload
+
+
+def loaded(size, divisions):
+    """crates/solidfem/tests/validation/multigrid_precision.rs:24-33: steel, clamped at x = 0,
+    traction (0, 3, -7) on the far end."""
+    return Workload(size=tuple(size), divisions=tuple(divisions), clamp_axis=0, clamp_value=0.0, load_axis=0,
+                    load_value=float(size[0]), traction=(0.0, 3.0, -7.0), youngs=200000.0, poisson=0.3)
+
+
+FINGERPRINT_CANTILEVER = ([10.0, 1.0, 1.0], [12, 3, 3], 61, 4639424340864456019)
+FINGERPRINT_PLATE = ([20.0, 20.0, 1.0], [8, 8, 2], 85, 5373319653865270473)
+
+
+def cantilever_config(divisions=(40, 4, 4)):
+    """Config 1: 100x10x10 mm, E 210 GPa, nu 0.3, 1000 N shear on x = 100 (10 MPa over 100 mm^2)."""
+    return Workload(size=(100.0, 10.0, 10.0), divisions=tuple(divisions), clamp_axis=0, clamp_value=0.0, load_axis=0,
+                    load_value=100.0, traction=(0.0, 0.0, -10.0), youngs=210000.0, poisson=0.3)
+
+
+def plate_config(divisions=(100, 100, 2), seed=20261202):
+    """Config 2: 100x100x0.8 mm plate, E 3.5 GPa, nu 0.36, clamped x = 0, 0.01 MPa pressure on
+    z = 0.8, interior vertices jittered by U(-5%, 5%) of the cell size (seed 20261202)."""
+    return Workload(size=(100.0, 100.0, 0.8), divisions=tuple(divisions), jitter_fraction=0.05, jitter_seed=seed,
+                    clamp_axis=0, clamp_value=0.0, load_axis=2, load_value=0.8, traction=(0.0, 0.0, -0.01),
+                    youngs=3500.0, poisson=0.36)
+
+
+def block_config(case, divisions=(30, 20, 20), seed=7):
+    """Config 3: 60x40x40 mm block, E 70 GPa, nu 0.33, clamped z = 0, random tractions N(0, 1 MPa)
+    per component on the z = 40 faces, load case `case` (seed 7)."""
+    return Workload(size=(60.0, 40.0, 40.0), divisions=tuple(divisions), clamp_axis=2, clamp_value=0.0, load_axis=2,
+                    load_value=40.0, traction_kind=1, traction_scale=1.0, traction_seed=seed, load_case=case,
+                    youngs=70000.0, poisson=0.33)
+
+
+def bar_config(divisions=(160, 16, 16)):
+    """Config 4: 200x20x20 mm bar, 2 mm shell E 3.5 GPa nu 0.36 around a core E 0.7 GPa nu 0.3,
+    clamped x = 0, tangential traction 0.5 MPa about (y, z) = (10, 10) on x = 200."""
+    return Workload(size=(200.0, 20.0, 20.0), divisions=tuple(divisions), clamp_axis=0, clamp_value=0.0, load_axis=0,
+                    load_value=200.0, traction_kind=2, traction_scale=0.5, axis_u=10.0, axis_v=10.0,
+                    material_kind=1, youngs=3500.0, poisson=0.36, core_youngs=700.0, core_poisson=0.3,
+                    shell_depth=2.0)
