@@ -115,7 +115,17 @@ typedef struct sf_multigrid_options {
     double smoother_range;
     int32_t precision;
     int32_t threads; /* host threads for setup; 0 = all */
+    int32_t coarse_solve; /* SF_COARSE_REFERENCE or SF_COARSE_INVERSE */
+    int32_t reserved;
 } sf_multigrid_options;
+
+/* The coarsest level's solve. REFERENCE runs forward then back substitution with the Cholesky
+ * factor in the reference's order (crates/solidfem-solve/src/multigrid/block.rs:357-373), so a
+ * cycle repeats the reference's bits; its back substitution is a serial chain of n^2/2
+ * subtractions. INVERSE applies the f64 inverse L^-T L^-1, formed once at setup, as a dense
+ * product summed in a fixed order: the same bits on every run, differing from REFERENCE by
+ * rounding only, in a few microseconds. */
+enum { SF_COARSE_REFERENCE = 0, SF_COARSE_INVERSE = 1 };
 
 /* MultigridStats, crates/solidfem-solve/src/multigrid/mod.rs:184-195 */
 typedef struct sf_multigrid_stats {
@@ -370,8 +380,10 @@ void sf_pcg_destroy(sf_pcg *pcg);
 /* The study's system matrix as a CsrMatrix handle (owned by the study). */
 const sf_csr *sf_static_study_system(const sf_static_study *study);
 /* The same hierarchy stored and applied in another precision (Multigrid::from_levels,
- * crates/solidfem-solve/src/multigrid/cycle.rs:265-308, over the same f64 levels). */
-sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision, sf_multigrid **out);
+ * crates/solidfem-solve/src/multigrid/cycle.rs:265-308, over the same f64 levels), with the
+ * given coarse solve (SF_COARSE_*, or -1 to keep this one's). */
+sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision, int32_t coarse_solve,
+                                      sf_multigrid **out);
 /* Algorithmic HBM bytes of one V-cycle of this hierarchy (every array a kernel touches, once). */
 double sf_multigrid_cycle_bytes(const sf_multigrid *mg);
 /* Restarts a solve from a zero estimate (Pcg::new semantics, pcg.rs:135-164) on the buffers it

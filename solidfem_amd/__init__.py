@@ -113,10 +113,11 @@ class MultigridOptions:
     smoother_range: float = 30.0
     precision: Precision = Precision.SINGLE_STORAGE
     threads: int = 0
+    coarse_solve: int = 0  # 0: reference order, 1: explicit inverse
 
     def c(self):
         return _capi.MultigridOptions(self.neighbour_reach, self.coarse_dofs, self.max_levels, self.smoother_degree,
-                                      self.smoother_range, int(self.precision), self.threads)
+                                      self.smoother_range, int(self.precision), self.threads, self.coarse_solve, 0)
 
 
 @dataclass
@@ -600,10 +601,11 @@ class Multigrid(_Handle, _Levels):
         _raise(library().sf_multigrid_level_sizes(self._ptr, _ptr(out, C.c_uint64), 64, C.byref(n)))
         return [tuple(int(v) for v in out[3 * i:3 * i + 3]) for i in range(n.value)]
 
-    def with_precision(self, precision: Precision) -> "Multigrid":
-        """The same hierarchy in another precision (cycle.rs:265-308 over the same levels)."""
+    def with_precision(self, precision: Precision, coarse_solve: int = -1) -> "Multigrid":
+        """The same hierarchy in another precision (cycle.rs:265-308 over the same levels); the coarse
+        solve mode is kept unless given."""
         h = C.c_void_p()
-        _raise(library().sf_multigrid_with_precision(self._ptr, int(precision), C.byref(h)))
+        _raise(library().sf_multigrid_with_precision(self._ptr, int(precision), coarse_solve, C.byref(h)))
         m = Multigrid(h)
         m._parent = self
         return m

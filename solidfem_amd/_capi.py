@@ -35,6 +35,8 @@ class MultigridOptions(C.Structure):
         ("smoother_range", f64),
         ("precision", i32),
         ("threads", i32),
+        ("coarse_solve", i32),
+        ("reserved", i32),
     ]
 
 
@@ -170,7 +172,7 @@ SIGNATURES = {
     "sf_pcg_size": (u64, [vp]),
     "sf_pcg_destroy": (None, [vp]),
     "sf_static_study_system": (vp, [vp]),
-    "sf_multigrid_with_precision": (i32, [vp, i32, P(vp)]),
+    "sf_multigrid_with_precision": (i32, [vp, i32, i32, P(vp)]),
     "sf_multigrid_cycle_bytes": (f64, [vp]),
     "sf_pcg_restart": (i32, [vp, P(f64)]),
     "sf_profile_enable": (i32, [i32, i32]),

@@ -51,7 +51,10 @@ SetupOptions setup_options(const sf_multigrid_options *o) {
     s.degree = (size_t)o->smoother_degree;
     s.range = o->smoother_range;
     s.precision = o->precision;
+    s.coarse_solve = o->coarse_solve;
     if (s.precision < 0 || s.precision > 2) throw Error(SF_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (s.coarse_solve != SF_COARSE_REFERENCE && s.coarse_solve != SF_COARSE_INVERSE)
+        throw Error(SF_ERR_INVALID_ARGUMENT, "unknown coarse solve");
     if (s.degree < 1) throw Error(SF_ERR_INVALID_ARGUMENT, "the smoother degree must be at least one");
     return s;
 }
@@ -494,7 +497,8 @@ sf_status sf_study_builder_finish(sf_study_builder *b, sf_static_study **out) {
         if (b->use_multigrid) {
             const double t1 = seconds();
             st.mg.ctx = b->ctx;
-            st.mg.mg = make_device_multigrid(b->setup->take_levels(), b->mg_options.degree, b->mg_options.precision, s);
+            st.mg.mg = make_device_multigrid(b->setup->take_levels(), b->mg_options.degree, b->mg_options.precision,
+                                             b->mg_options.coarse_solve, s);
             pre += seconds() - t1;
         }
         st.pcg = std::make_unique<sf_pcg>();
@@ -627,19 +631,21 @@ sf_status sf_csr_rounding_bound(sf_csr *c, const double *x, int32_t dot_order, d
         hipStream_t s = c->ctx->ctx->stream;
         const size_t n = c->system.size;
         DeviceBuffer<double> dx, rows(std::max<size_t>(n, 1)), partials(dev::DOT_BLOCKS), result(1);
-        DeviceBuffer<uint32_t> ticket(1);
-        SF_HIP(hipMemsetAsync(ticket.ptr, 0, 4, s));
         dx.upload(x, n, s);
         const dim3 grid((unsigned)((c->system.sliced.slices + 3) / 4)), block(256);
         if (c->system.rb == 3)
             hipLaunchKernelGGL((dev::rounding_bound_kernel<3, 3>), grid, block, 0, s, c->system.sliced.view(), dx.ptr, rows.ptr);
         else
             hipLaunchKernelGGL((dev::rounding_bound_kernel<1, 1>), grid, block, 0, s, c->system.sliced.view(), dx.ptr, rows.ptr);
-        if (dot_order == SF_DOT_REFERENCE)
-            hipLaunchKernelGGL((dev::dot_serial_kernel<1>), dim3(1), dim3(64), 0, s, n, rows.ptr, rows.ptr, result.ptr);
-        else
-            hipLaunchKernelGGL((dev::dot_tree_kernel<1>), dim3(dev::DOT_BLOCKS), dim3(dev::DOT_THREADS), 0, s, n, rows.ptr, rows.ptr,
-                               partials.ptr, ticket.ptr, result.ptr);
+        if (dot_order == SF_DOT_REFERENCE) {
+            hipLaunchKernelGGL((dev::dot_serial_kernel<1>), dim3(1), dim3(64), 0, s, n, rows.ptr, rows.ptr, (int)dev::FINAL_STORE,
+                               result.ptr, (dev::PcgScalars *)nullptr);
+        } else {
+            hipLaunchKernelGGL((dev::dot_partials_kernel<1>), dim3(dev::DOT_BLOCKS), dim3(dev::DOT_THREADS), 0, s, n, rows.ptr, rows.ptr,
+                               partials.ptr);
+            hipLaunchKernelGGL(dev::dot_final_kernel, dim3(1), dim3(dev::DOT_THREADS), 0, s, dev::DOT_BLOCKS,
+                               (const double *)partials.ptr, (int)dev::FINAL_STORE, result.ptr, (dev::PcgScalars *)nullptr);
+        }
         SF_HIP(hipGetLastError());
         double sum = 0;
         result.download(&sum, 1, s);
@@ -679,7 +685,7 @@ sf_status sf_multigrid_setup_finish(sf_multigrid_setup *setup, sf_multigrid **ou
         auto m = std::make_unique<sf_multigrid>();
         m->ctx = setup->ctx;
         const auto opt = setup->setup->options();
-        m->mg = make_device_multigrid(setup->setup->take_levels(), opt.degree, opt.precision, setup->ctx->ctx->stream);
+        m->mg = make_device_multigrid(setup->setup->take_levels(), opt.degree, opt.precision, opt.coarse_solve, setup->ctx->ctx->stream);
         *out = m.release();
     });
 }
@@ -873,11 +879,12 @@ extern "C" {
 
 const sf_csr *sf_static_study_system(const sf_static_study *study) { return &study->system; }
 
-sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision, sf_multigrid **out) {
+sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision, int32_t coarse_solve, sf_multigrid **out) {
     return guard([&] {
         auto m = std::make_unique<sf_multigrid>();
         m->ctx = mg->ctx;
-        m->mg = make_device_multigrid(mg->mg->host, mg->mg->degree, precision, mg->ctx->ctx->stream);
+        m->mg = make_device_multigrid(mg->mg->host, mg->mg->degree, precision, coarse_solve < 0 ? mg->mg->coarse_mode : coarse_solve,
+                                      mg->ctx->ctx->stream);
         *out = m.release();
     });
 }

@@ -1,6 +1,7 @@
 // Device objects of the solve path: sliced matrices, the V-cycle, the conjugate gradient.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 
@@ -25,6 +26,17 @@ Context::~Context() {
         (void)hipStreamDestroy(stream);
     }
 }
+
+template <typename T>
+uint32_t DeviceSliced<T>::xcd_order() {
+    static const uint32_t on = [] {
+        const char *e = std::getenv("SOLIDFEM_XCD_ORDER");
+        return e ? (uint32_t)(std::atoi(e) != 0) : 0u;  // measured slower on the plate; off unless asked
+    }();
+    return on;
+}
+template struct DeviceSliced<double>;
+template struct DeviceSliced<float>;
 
 // ---- launch timing -----------------------------------------------------------------------------
 
@@ -172,29 +184,86 @@ template void upload_sliced<float>(DeviceSliced<float> &, const HostBsr<double> 
 template void build_sliced<double>(DeviceSliced<double> &, const SlicedPlan &, size_t, const double *, hipStream_t);
 template void build_sliced<float>(DeviceSliced<float> &, const SlicedPlan &, size_t, const double *, hipStream_t);
 
-template <int RB, int CB, typename S, typename V>
-static void spmv_shape(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+template <int RB, int CB, int BATCH, typename S, typename V>
+static void spmv_batch(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
     const dim3 grid(grid_for(a.slices, 4)), block(256);
-    if (epi == EPI_STORE) hipLaunchKernelGGL((spmv_kernel<RB, CB, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b);
-    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((spmv_kernel<RB, CB, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b);
-    else hipLaunchKernelGGL((spmv_kernel<RB, CB, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b);
+    if (epi == EPI_STORE) hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b);
+    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b);
+    else hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b);
+}
+
+template <int RB, int CB, typename S, typename V>
+static void spmv_shape(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch) {
+    switch (batch) {
+        case 1: spmv_batch<RB, CB, 1>(a, x, y, b, epi, s); break;
+        case 2: spmv_batch<RB, CB, 2>(a, x, y, b, epi, s); break;
+        case 8: spmv_batch<RB, CB, 8>(a, x, y, b, epi, s); break;
+        default: spmv_batch<RB, CB, 4>(a, x, y, b, epi, s); break;
+    }
 }
 
 template <typename S, typename V>
-void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch) {
     if (a.slices == 0) return;
-    if (a.rb == 3 && a.cb == 3) spmv_shape<3, 3>(a, x, y, b, epi, s);
-    else if (a.rb == 6 && a.cb == 6) spmv_shape<6, 6>(a, x, y, b, epi, s);
-    else if (a.rb == 3 && a.cb == 6) spmv_shape<3, 6>(a, x, y, b, epi, s);
-    else if (a.rb == 6 && a.cb == 3) spmv_shape<6, 3>(a, x, y, b, epi, s);
-    else if (a.rb == 1 && a.cb == 1) spmv_shape<1, 1>(a, x, y, b, epi, s);
-    else throw Error(SF_ERR_INTERNAL, "unsupported block shape");
+    if (batch == 0) batch = a.batch_spmv;
+    if (a.rb == 3 && a.cb == 3) spmv_shape<3, 3>(a, x, y, b, epi, s, batch);
+    else if (a.rb == 6 && a.cb == 6) spmv_shape<6, 6>(a, x, y, b, epi, s, batch);
+    else if (a.rb == 3 && a.cb == 6) spmv_shape<3, 6>(a, x, y, b, epi, s, batch);
+    else if (a.rb == 6 && a.cb == 3) spmv_shape<6, 3>(a, x, y, b, epi, s, batch);
+    else if constexpr (std::is_same<S, double>::value && std::is_same<V, double>::value) {
+        if (a.rb == 1 && a.cb == 1) spmv_shape<1, 1>(a, x, y, b, epi, s, batch);
+        else throw Error(SF_ERR_INTERNAL, "unsupported block shape");
+    } else {
+        throw Error(SF_ERR_INTERNAL, "unsupported block shape");
+    }
     SF_HIP(hipGetLastError());
 }
 
-template void launch_spmv<double, double>(const DeviceSliced<double> &, const double *, double *, const double *, int, hipStream_t);
-template void launch_spmv<float, double>(const DeviceSliced<float> &, const double *, double *, const double *, int, hipStream_t);
-template void launch_spmv<float, float>(const DeviceSliced<float> &, const float *, float *, const float *, int, hipStream_t);
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch) {
+    static const int forced = [] {
+        const char *e = std::getenv("SOLIDFEM_BATCH");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (forced == 1 || forced == 2 || forced == 4 || forced == 8) return forced;
+    hipEvent_t e0, e1;
+    SF_HIP(hipEventCreate(&e0));
+    SF_HIP(hipEventCreate(&e1));
+    static const bool verbose = std::getenv("SOLIDFEM_TUNE_LOG") != nullptr;
+    int best = 4;
+    float best_ms = 1e30f;
+    for (int batch : {1, 2, 4, 8}) {
+        launch(batch);  // warm
+        SF_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) launch(batch);
+        SF_HIP(hipEventRecord(e1, s));
+        SF_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (verbose) std::fprintf(stderr, "[tune] batch %d: %.2f us\n", batch, ms * 1e3f / 3);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = batch;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+template <typename S, typename V>
+void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s) {
+    if (a.slices == 0) return;
+    DeviceBuffer<V> x(std::max<size_t>(a.block_cols * a.cb, 1)), y(std::max<size_t>(a.rows * a.rb, 1));
+    SF_HIP(hipMemsetAsync(x.ptr, 0, x.bytes(), s));
+    SF_HIP(hipMemsetAsync(y.ptr, 0, y.bytes(), s));
+    a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(a, x.ptr, y.ptr, y.ptr, epi, s, batch); });
+    SF_HIP(hipStreamSynchronize(s));
+}
+
+template void launch_spmv<double, double>(const DeviceSliced<double> &, const double *, double *, const double *, int, hipStream_t, int);
+template void launch_spmv<float, double>(const DeviceSliced<float> &, const double *, double *, const double *, int, hipStream_t, int);
+template void launch_spmv<float, float>(const DeviceSliced<float> &, const float *, float *, const float *, int, hipStream_t, int);
+template void tune_spmv<double, double>(DeviceSliced<double> &, int, hipStream_t);
 
 // ---- the device hierarchy -------------------------------------------------------------------
 
@@ -221,7 +290,10 @@ struct LevelDev {
     size_t rows = 0, n = 0;
     DeviceSliced<S> a, p, r;
     DeviceBuffer<S> dinv;
-    DeviceBuffer<double> lt;  // coarsest: transposed Cholesky factor
+    DeviceBuffer<double> lt;      // coarsest: transposed Cholesky factor
+    DeviceBuffer<double> packed;  // coarsest: the factor packed by columns, for the LDS solve
+    DeviceBuffer<double> inverse; // coarsest: the f64 inverse, by columns (SF_COARSE_INVERSE)
+    size_t lds_bytes = 0;         // nonzero when the LDS solve is used
     bool has_p = false, has_factor = false;
     double theta = 0;
     std::vector<std::pair<double, double>> steps;
@@ -231,8 +303,9 @@ struct LevelDev {
 template <typename S, typename V>
 class Hierarchy final : public DeviceMultigrid {
    public:
-    Hierarchy(std::vector<HostLevel> levels, size_t deg, int prec, hipStream_t s) {
+    Hierarchy(std::vector<HostLevel> levels, size_t deg, int prec, int coarse, hipStream_t s) {
         precision = prec;
+        coarse_mode = coarse;
         degree = deg;
         host = std::move(levels);
         lv_.resize(host.size());
@@ -283,12 +356,51 @@ class Hierarchy final : public DeviceMultigrid {
                 for (size_t i = 0; i < n; ++i)
                     for (size_t k = 0; k <= i; ++k) lt[k * n + i] = h.factor[i * n + k];
                 d.lt.upload(lt, s);
+                if (coarse_mode == SF_COARSE_INVERSE) {
+                    // inverse = L^-T L^-1 in f64: the columns of L^-1 by forward substitution, then
+                    // entry (i, j) = sum_k Linv[k][i] Linv[k][j], symmetric by construction
+                    std::vector<double> linv(n * n, 0.0), inv(n * n, 0.0);
+                    parallel_for(n, 8, [&](size_t c0, size_t c1) {
+                        for (size_t j = c0; j < c1; ++j)
+                            for (size_t i = j; i < n; ++i) {
+                                double v = i == j ? 1.0 : 0.0;
+                                for (size_t k = j; k < i; ++k) v -= h.factor[i * n + k] * linv[k * n + j];
+                                linv[i * n + j] = v / h.factor[i * n + i];
+                            }
+                    });
+                    parallel_for(n, 8, [&](size_t r0, size_t r1) {
+                        for (size_t i = r0; i < r1; ++i)
+                            for (size_t j = 0; j < n; ++j) {
+                                double v = 0.0;
+                                for (size_t k = std::max(i, j); k < n; ++k) v += linv[k * n + i] * linv[k * n + j];
+                                inv[j * n + i] = v;
+                            }
+                    });
+                    d.inverse.upload(inv, s);
+                }
+                const size_t need = (n * (n + 1) / 2 + 2 * n) * sizeof(double);
+                int dev = 0, limit = 0;
+                SF_HIP(hipGetDevice(&dev));
+                SF_HIP(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+                if (need <= (size_t)limit) {
+                    std::vector<double> packed;
+                    packed.reserve(n * (n + 1) / 2);
+                    for (size_t k = 0; k < n; ++k)
+                        for (size_t i = k; i < n; ++i) packed.push_back(h.factor[i * n + k]);
+                    d.packed.upload(packed, s);
+                    d.lds_bytes = need;
+                    if (need > 65536) {
+                        SF_HIP(hipFuncSetAttribute((const void *)coarse_solve_lds_kernel<V>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+                        SF_HIP(hipFuncSetAttribute((const void *)coarse_solve_lds_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+                    }
+                }
             }
             for (auto *buf : {&d.b, &d.x, &d.d0, &d.d1, &d.r0, &d.r1, &d.res, &d.px}) buf->resize(d.n);
         }
         in_.resize(host.empty() ? 0 : host[0].a.dofs());
         out_.resize(in_.count);
         SF_HIP(hipStreamSynchronize(s));
+        tune(s);
         compute_stats();
     }
 
@@ -320,11 +432,57 @@ class Hierarchy final : public DeviceMultigrid {
     void coarse_solve(const double *b, double *x, hipStream_t s) override {
         LevelDev<S, V> &d = lv_.back();
         if (!d.has_factor) throw Error(SF_ERR_MG_COARSEST_NOT_FACTORED, "the coarsest level has no direct factorization", (int64_t)d.n);
-        hipLaunchKernelGGL(coarse_solve_kernel<double>, dim3(1), dim3(1024), 0, s, (uint32_t)d.n, d.lt.ptr, b, x);
+        if (d.lds_bytes)
+            hipLaunchKernelGGL(coarse_solve_lds_kernel<double>, dim3(1), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x);
+        else
+            hipLaunchKernelGGL(coarse_solve_kernel<double>, dim3(1), dim3(WAVE), 0, s, (uint32_t)d.n, d.lt.ptr, b, x);
         SF_HIP(hipGetLastError());
     }
 
    private:
+    template <int RB, int BATCH>
+    static void cheb_step_launch(LevelDev<S, V> &d, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out, V *x,
+                                 V *into, hipStream_t s) {
+        hipLaunchKernelGGL((cheb_step_kernel<RB, BATCH, S, V>), dim3(grid_for(d.a.slices, 4)), dim3(256), 0, s, d.a.view(),
+                           d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+    }
+    template <int RB>
+    static void cheb_step_rb(LevelDev<S, V> &d, int batch, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out,
+                             V *x, V *into, hipStream_t s) {
+        switch (batch) {
+            case 1: cheb_step_launch<RB, 1>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+            case 2: cheb_step_launch<RB, 2>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+            case 8: cheb_step_launch<RB, 8>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+            default: cheb_step_launch<RB, 4>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+        }
+    }
+    static void launch_cheb_step(LevelDev<S, V> &d, int batch, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out,
+                                 V *x, V *into, hipStream_t s) {
+        if (d.rb == 3) cheb_step_rb<3>(d, batch, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+        else cheb_step_rb<6>(d, batch, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+    }
+
+    // Picks each level's batch depths by timing its own kernels on its own matrices.
+    void tune(hipStream_t s) {
+        for (size_t l = 0; l < lv_.size(); ++l) {
+            LevelDev<S, V> &d = lv_[l];
+            V *vs[8] = {d.b.ptr, d.x.ptr, d.d0.ptr, d.d1.ptr, d.r0.ptr, d.r1.ptr, d.res.ptr, d.px.ptr};
+            for (V *v : vs) SF_HIP(hipMemsetAsync(v, 0, d.n * sizeof(V), s));
+            d.a.batch_cheb = tune_batch(s, [&](int batch) {
+                launch_cheb_step(d, batch, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
+            });
+            d.a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, batch); });
+            if (d.has_p) {
+                LevelDev<S, V> &c = lv_[l + 1];
+                SF_HIP(hipMemsetAsync(c.b.ptr, 0, c.n * sizeof(V), s));
+                SF_HIP(hipMemsetAsync(c.x.ptr, 0, c.n * sizeof(V), s));
+                d.r.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, batch); });
+                d.p.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.p, c.x.ptr, d.px.ptr, nullptr, EPI_ADD, s, batch); });
+            }
+        }
+        SF_HIP(hipStreamSynchronize(s));
+    }
+
     // cycle.rs:570-651: start, then degree - 1 fused steps; `sum_into` takes the final add.
     void smooth(LevelDev<S, V> &d, const V *b, V *x, V *sum_into, hipStream_t s, bool finest = false) {
         const unsigned g = grid_for(d.n, 256);
@@ -342,16 +500,12 @@ class Hierarchy final : public DeviceMultigrid {
             V *r_out = rr[k & 1];
             const V keep = (V)d.steps[k - 1].first, stp = (V)d.steps[k - 1].second;
             V *into = k == steps ? sum_into : nullptr;
-            const dim3 grid(grid_for(d.a.slices, 4)), block(256);
             const double vec = (double)d.n * sizeof(V);
             const int token = finest ? Profiler::get().begin(0, s, (double)d.a.algorithmic_matrix_bytes() +
                                                                        (double)d.rows * d.rb * d.rb * sizeof(S) + 6 * vec +
                                                                        (into ? 2 * vec : 0.0))
                                      : -1;
-            if (d.rb == 3)
-                hipLaunchKernelGGL((cheb_step_kernel<3, S, V>), grid, block, 0, s, d.a.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
-            else
-                hipLaunchKernelGGL((cheb_step_kernel<6, S, V>), grid, block, 0, s, d.a.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+            launch_cheb_step(d, d.a.batch_cheb, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
             Profiler::get().end(0, token, s);
         }
         if (steps == 0 && sum_into)
@@ -363,7 +517,12 @@ class Hierarchy final : public DeviceMultigrid {
         LevelDev<S, V> &d = lv_[l];
         if (!d.has_p) {
             if (d.has_factor) {
-                hipLaunchKernelGGL(coarse_solve_kernel<V>, dim3(1), dim3(1024), 0, s, (uint32_t)d.n, d.lt.ptr, b, x);
+                if (d.inverse.ptr)
+                    hipLaunchKernelGGL(coarse_inverse_kernel<V>, dim3(grid_for(d.n, 256)), dim3(256), 0, s, (uint32_t)d.n, d.inverse.ptr, b, x);
+                else if (d.lds_bytes)
+                    hipLaunchKernelGGL(coarse_solve_lds_kernel<V>, dim3(1), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x);
+                else
+                    hipLaunchKernelGGL(coarse_solve_kernel<V>, dim3(1), dim3(WAVE), 0, s, (uint32_t)d.n, d.lt.ptr, b, x);
             } else {
                 smooth(d, b, x, nullptr, s);
             }
@@ -429,15 +588,18 @@ class Hierarchy final : public DeviceMultigrid {
 
 }  // namespace
 
-std::unique_ptr<DeviceMultigrid> make_device_multigrid(std::vector<HostLevel> levels, size_t degree, int precision, hipStream_t s) {
+std::unique_ptr<DeviceMultigrid> make_device_multigrid(std::vector<HostLevel> levels, size_t degree, int precision,
+                                                       int coarse_mode, hipStream_t s) {
     if (degree < 1) throw Error(SF_ERR_INVALID_ARGUMENT, "the smoother degree must be at least one");
+    if (coarse_mode != SF_COARSE_REFERENCE && coarse_mode != SF_COARSE_INVERSE)
+        throw Error(SF_ERR_INVALID_ARGUMENT, "unknown coarse solve");
     switch (precision) {
         case SF_PRECISION_DOUBLE:
-            return std::make_unique<Hierarchy<double, double>>(std::move(levels), degree, precision, s);
+            return std::make_unique<Hierarchy<double, double>>(std::move(levels), degree, precision, coarse_mode, s);
         case SF_PRECISION_SINGLE_STORAGE:
-            return std::make_unique<Hierarchy<float, double>>(std::move(levels), degree, precision, s);
+            return std::make_unique<Hierarchy<float, double>>(std::move(levels), degree, precision, coarse_mode, s);
         case SF_PRECISION_SINGLE:
-            return std::make_unique<Hierarchy<float, float>>(std::move(levels), degree, precision, s);
+            return std::make_unique<Hierarchy<float, float>>(std::move(levels), degree, precision, coarse_mode, s);
         default:
             throw Error(SF_ERR_INVALID_ARGUMENT, "unknown precision");
     }
@@ -453,6 +615,7 @@ void DeviceSystem::upload_from_host(hipStream_t s) {
 void DeviceSystem::build_from_device_values(hipStream_t s) {
     SlicedPlan plan = plan_sliced(host.starts.data(), host.columns.data(), host.rows, (int)host.rb, (int)host.cb);
     build_sliced(sliced, plan, host.cols, values.ptr, s);
+    tune_spmv<double, double>(sliced, EPI_STORE, s);
 }
 
 // ---- the conjugate gradient -------------------------------------------------------------------
@@ -469,8 +632,6 @@ DevicePcg::DevicePcg(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, c
     scalars_.resize(1);
     partials_.resize(DOT_BLOCKS);
     norm_.resize(1);
-    tickets_.resize(4);
-    SF_HIP(hipMemsetAsync(tickets_.ptr, 0, 4 * sizeof(uint32_t), s));
     SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocDefault));
     SF_HIP(hipEventCreateWithFlags(&copied_, hipEventDisableTiming));
     if (!mg_) {
@@ -530,21 +691,25 @@ DevicePcg::~DevicePcg() {
     if (copied_) (void)hipEventDestroy(copied_);
 }
 
-void DevicePcg::dot(const double *a, const double *b, double *result) {
-    if (dot_order_ == SF_DOT_REFERENCE)
-        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_.stream, n_, a, b, result);
-    else
-        hipLaunchKernelGGL((dot_tree_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_.stream, n_, a, b, partials_.ptr,
-                           tickets_.ptr, result);
+void DevicePcg::dot(const double *a, const double *b, double *result, int final_kind) {
+    if (dot_order_ == SF_DOT_REFERENCE) {
+        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_.stream, n_, a, b, final_kind, result, scalars_.ptr);
+    } else {
+        hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_.stream, n_, a, b, partials_.ptr);
+        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_.stream, DOT_BLOCKS, (const double *)partials_.ptr,
+                           final_kind, result, scalars_.ptr);
+    }
     SF_HIP(hipGetLastError());
 }
 
 void DevicePcg::sum(const double *a, double *result) {
-    if (dot_order_ == SF_DOT_REFERENCE)
-        hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_.stream, n_, a, a, result);
-    else
-        hipLaunchKernelGGL((dot_tree_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_.stream, n_, a, a, partials_.ptr,
-                           tickets_.ptr, result);
+    if (dot_order_ == SF_DOT_REFERENCE) {
+        hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_.stream, n_, a, a, (int)FINAL_STORE, result, scalars_.ptr);
+    } else {
+        hipLaunchKernelGGL((dot_partials_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_.stream, n_, a, a, partials_.ptr);
+        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_.stream, DOT_BLOCKS, (const double *)partials_.ptr,
+                           (int)FINAL_STORE, result, scalars_.ptr);
+    }
     SF_HIP(hipGetLastError());
 }
 
@@ -583,8 +748,8 @@ void DevicePcg::iterate() {
     SF_HIP(hipMemcpyAsync(pinned_, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     SF_HIP(hipEventRecord(copied_, s));
     precondition(r_.ptr, z_.ptr);
-    dot(r_.ptr, z_.ptr, &sc->rz_next);
-    hipLaunchKernelGGL(pcg_direction_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, sc, z_.ptr, p_.ptr, tickets_.ptr + 1);
+    dot(r_.ptr, z_.ptr, nullptr, FINAL_RZ);
+    hipLaunchKernelGGL(pcg_direction_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, (const PcgScalars *)sc, z_.ptr, p_.ptr);
     SF_HIP(hipGetLastError());
     SF_HIP(hipEventSynchronize(copied_));
     const PcgScalars h = *pinned_;

@@ -31,7 +31,10 @@ struct DeviceSliced {
     uint32_t slices = 0;
     DeviceBuffer<uint32_t> slice_start, row_len, perm, cols;
     DeviceBuffer<T> vals;
-    dev::Sliced<T> view() const { return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices}; }
+    int batch_spmv = 4;  // blocks in flight per lane, chosen by tune_batch
+    int batch_cheb = 4;
+    dev::Sliced<T> view() const { return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices, xcd_order()}; }
+    static uint32_t xcd_order();
     size_t dofs() const { return rows * rb; }
     // Bytes one product reads by the algorithmic count: values, block columns, row offsets.
     size_t algorithmic_matrix_bytes() const { return blocks * (rb * cb * sizeof(T) + 4) + (rows + 1) * 4; }
@@ -44,9 +47,16 @@ void build_sliced(DeviceSliced<T> &out, const SlicedPlan &plan, size_t block_col
 template <typename T>
 void upload_sliced(DeviceSliced<T> &out, const HostBsr<double> &m, hipStream_t s);
 
-// y = A x (epilogue per dev::Epilogue) for any supported block shape.
+// y = A x (epilogue per dev::Epilogue) for any supported block shape, with the matrix's batch.
 template <typename S, typename V>
-void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s);
+void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch = 0);
+
+// The batch depth (1, 2, 4 or 8 blocks in flight per lane) whose launches `launch(batch)` run
+// fastest; every depth gives the same bits. SOLIDFEM_BATCH forces one.
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch);
+// Tunes a matrix's product on scratch vectors of its shape.
+template <typename S, typename V>
+void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s);
 
 // ---- launch timing ------------------------------------------------------------------------------
 
@@ -86,6 +96,7 @@ class DeviceMultigrid {
     virtual void smooth_level(size_t level, const double *b, double *x, hipStream_t s) = 0;
     virtual void coarse_solve(const double *b, double *x, hipStream_t s) = 0;
     int precision = SF_PRECISION_SINGLE_STORAGE;
+    int coarse_mode = SF_COARSE_REFERENCE;
     size_t degree = 3;
     std::vector<HostLevel> host;  // the f64 hierarchy as set up (for inspection and export)
     sf_multigrid_stats stats{};
@@ -94,7 +105,7 @@ class DeviceMultigrid {
     double cycle_bytes = 0;
 };
 std::unique_ptr<DeviceMultigrid> make_device_multigrid(std::vector<HostLevel> levels, size_t degree, int precision,
-                                                       hipStream_t s);
+                                                       int coarse_mode, hipStream_t s);
 
 // ---- system matrix and conjugate gradient ---------------------------------------------------
 
@@ -130,7 +141,7 @@ class DevicePcg {
 
    private:
     void precondition(const double *r, double *z);
-    void dot(const double *a, const double *b, double *result);
+    void dot(const double *a, const double *b, double *result, int final_kind = dev::FINAL_STORE);
     void sum(const double *a, double *result);
     void iterate();
     void reset_direction();
@@ -150,7 +161,6 @@ class DevicePcg {
     DeviceBuffer<double> b_, x_, r_, z_, p_, ap_, tmp_, tmp2_, jacobi_;
     DeviceBuffer<dev::PcgScalars> scalars_;
     DeviceBuffer<double> partials_, norm_;
-    DeviceBuffer<uint32_t> tickets_;
     dev::PcgScalars *pinned_ = nullptr;
     hipEvent_t copied_ = nullptr;
 };

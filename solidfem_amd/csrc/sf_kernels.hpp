@@ -4,11 +4,14 @@
 // slices of NPW = 64 / RB block rows (21 nodes of 3 rows, 10 of 6, 64 scalar rows), one
 // wavefront per slice and one lane per scalar row. Within a slice, the k-th block of every row
 // is stored side by side: block columns at cols[(start + k) * NPW + j] and values at
-// vals[((start + k) * CB + q) * 64 + j * RB + p] for row j, block row p, block column q. Each
-// value load of a wavefront is then 64 consecutive elements, the RB lanes of a row read the same
-// column index (one fetch), and every lane sums its own row's blocks in stored order, column by
-// column, from zero: the summation order of the reference's block product
+// vals[((start + k) * 64 + j * RB + p) * CB + q] for row j, block row p, block column q. A lane's
+// CB values of a block are contiguous, one vector load, and a wavefront's loads of one slot cover
+// 64 * CB consecutive values; the RB lanes of a row read the same column index (one fetch), and
+// every lane sums its own row's blocks in stored order, column by column, from zero: the
+// summation order of the reference's block product
 // (crates/solidfem-solve/src/multigrid/block.rs:136-154), so the result is the same bits.
+// BATCH blocks are loaded before their terms are added, in order, so BATCH loads are in flight
+// per lane whatever the latency; the batch depth changes no bit and is chosen per matrix by timing.
 // Rows are permuted inside windows so that a slice's rows have similar block counts; perm[]
 // maps a slice position back to the block row.
 //
@@ -39,6 +42,7 @@ struct Sliced {
     const uint32_t *cols;         // per block slot and slice row
     const T *vals;                // per block slot, block column and lane
     uint32_t slices;
+    uint32_t xcd;                 // nonzero: XCD-contiguous slice order (xcd_block)
 };
 
 template <typename V, typename S>
@@ -46,23 +50,59 @@ __device__ __forceinline__ V widen(S s) {
     return (V)(double)s;
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn. Renumbering them so that each XCD takes a
+// contiguous run of slices keeps the rows an XCD works on -- and the vector entries they gather
+// -- together in that XCD's L2. A bijection on [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_block() {
+    constexpr uint32_t XCDS = 8;
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t x = b % XCDS, local = b / XCDS, base = nb / XCDS, rem = nb % XCDS;
+    return x * base + (x < rem ? x : rem) + local;
+}
+
 // Sum of one lane's row of A x, from zero in stored order: the row product of the reference.
-template <int RB, int CB, typename S, typename V>
+// The column indices of the next batch are loaded with the current batch's values, so each
+// batch costs one dependent memory round trip.
+template <int RB, int CB, int BATCH, typename S, typename V>
 __device__ __forceinline__ V row_product(const Sliced<S> &a, uint32_t slot0, uint32_t len, int j, int lane,
                                          const V *__restrict__ x) {
     constexpr int NPW = Shape<RB>::NPW;
     V sum = (V)0;
     const uint32_t *cp = a.cols + (size_t)slot0 * NPW + j;
-    const S *vp = a.vals + (size_t)slot0 * CB * WAVE + lane;
-#pragma unroll 2
-    for (uint32_t k = 0; k < len; ++k) {
-        const uint32_t col = __builtin_nontemporal_load(cp + (size_t)k * NPW);
-        const V *xc = x + (size_t)col * CB;
+    const S *vp = a.vals + ((size_t)slot0 * WAVE + lane) * CB;
+    const uint32_t full = len - len % BATCH;
+    uint32_t c[BATCH];
+    if (full) {
 #pragma unroll
-        for (int q = 0; q < CB; ++q) {
-            const S v = __builtin_nontemporal_load(vp + ((size_t)k * CB + q) * WAVE);
-            sum = sum + widen<V>(v) * xc[q];
+        for (int b = 0; b < BATCH; ++b) c[b] = cp[(size_t)b * NPW];
+    }
+    for (uint32_t k = 0; k < full; k += BATCH) {
+        S v[BATCH][CB];
+        V xv[BATCH][CB];
+        uint32_t cn[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int q = 0; q < CB; ++q) v[b][q] = vp[(size_t)(k + b) * WAVE * CB + q];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int q = 0; q < CB; ++q) xv[b][q] = x[(size_t)c[b] * CB + q];
+        if (k + BATCH < full) {
+#pragma unroll
+            for (int b = 0; b < BATCH; ++b) cn[b] = cp[(size_t)(k + BATCH + b) * NPW];
         }
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int q = 0; q < CB; ++q) sum = sum + widen<V>(v[b][q]) * xv[b][q];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) c[b] = cn[b];
+    }
+    for (uint32_t k = full; k < len; ++k) {
+        const uint32_t col = cp[(size_t)k * NPW];
+#pragma unroll
+        for (int q = 0; q < CB; ++q) sum = sum + widen<V>(vp[(size_t)k * WAVE * CB + q]) * x[(size_t)col * CB + q];
     }
     return sum;
 }
@@ -73,11 +113,11 @@ enum Epilogue { EPI_STORE = 0, EPI_RESIDUAL = 1, EPI_ADD = 2 };
 // residual (crates/solidfem-solve/src/multigrid/cycle.rs:548-564), the gather restriction over
 // the stored transpose (crates/solidfem-solve/src/gpu/layout.rs:249-291) and the prolongation
 // with correction (cycle.rs:512-521).
-template <int RB, int CB, typename S, typename V, int EPI>
+template <int RB, int CB, int BATCH, typename S, typename V, int EPI>
 __global__ __launch_bounds__(256) void spmv_kernel(Sliced<S> a, const V *__restrict__ x, V *__restrict__ y,
                                                    const V *__restrict__ b) {
     constexpr int NPW = Shape<RB>::NPW;
-    const uint32_t slice = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t slice = (a.xcd ? xcd_block() : blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (slice >= a.slices) return;
     const int lane = threadIdx.x % WAVE;
     const int j = lane / RB, p = lane - j * RB;
@@ -85,7 +125,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(Sliced<S> a, const V *__restr
     const uint32_t pos = slice * NPW + j;
     const uint32_t row = a.perm[pos];
     if (row == NONE) return;
-    const V sum = row_product<RB, CB, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, x);
+    const V sum = row_product<RB, CB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, x);
     const size_t out = (size_t)row * RB + p;
     if (EPI == EPI_STORE) y[out] = sum;
     else if (EPI == EPI_RESIDUAL) y[out] = b[out] - sum;
@@ -110,23 +150,10 @@ __global__ __launch_bounds__(256) void rounding_bound_kernel(Sliced<double> a, c
     double sum = 0.0;
     for (uint32_t k = 0; k < len; ++k) {
         const uint32_t col = a.cols[(size_t)(slot0 + k) * NPW + j];
-        for (int q = 0; q < CB; ++q) sum = sum + fabs(a.vals[((size_t)(slot0 + k) * CB + q) * WAVE + lane] * x[(size_t)col * CB + q]);
+        for (int q = 0; q < CB; ++q) sum = sum + fabs(a.vals[((size_t)(slot0 + k) * WAVE + lane) * CB + q] * x[(size_t)col * CB + q]);
     }
     const double bound = (double)(len * CB) * 2.220446049250313e-16 * sum;
     out[(size_t)row * RB + p] = bound * bound;
-}
-
-// The block-Jacobi product of one lane's row: sum_q Dinv[p][q] r_q from zero, with the row's
-// RB residual values shuffled in from the lanes of the same block row (cycle.rs:655-674).
-template <int RB, typename S, typename V>
-__device__ __forceinline__ V block_jacobi_row(const S *__restrict__ dinv_row, V r_own, int lane_base) {
-    V z = (V)0;
-#pragma unroll
-    for (int q = 0; q < RB; ++q) {
-        const V rq = __shfl(r_own, lane_base + q, WAVE);
-        z = z + widen<V>(dinv_row[q]) * rq;
-    }
-    return z;
 }
 
 // The Chebyshev start (cycle.rs:591-610): z = D^-1 b, d = z / theta, x = d.
@@ -146,15 +173,16 @@ __global__ __launch_bounds__(256) void cheb_start_kernel(uint32_t rows, const S 
 }
 
 // One later Chebyshev step, fused (cycle.rs:611-650): ad = A d_in; r_out = r_in - ad;
-// z = D^-1 r_out; d_out = keep d_in + step z; x += d_out. With `sum_into` set, the step is the
+// z = D^-1 r_out (the RB residuals of a block row shuffled in from its lanes, summed from zero,
+// cycle.rs:655-674); d_out = keep d_in + step z; x += d_out. With `sum_into` set, the step is the
 // last of a post-smoothing and also adds the smoothed correction to the cycle's x
 // (cycle.rs:534-536): sum_into += x.
-template <int RB, typename S, typename V>
+template <int RB, int BATCH, typename S, typename V>
 __global__ __launch_bounds__(256) void cheb_step_kernel(Sliced<S> a, const S *__restrict__ dinv, const V *__restrict__ d_in,
                                                         const V *__restrict__ r_in, V keep, V step, V *__restrict__ d_out,
                                                         V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into) {
     constexpr int NPW = Shape<RB>::NPW;
-    const uint32_t slice = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t slice = (a.xcd ? xcd_block() : blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (slice >= a.slices) return;  // whole wave leaves together
     const int lane = threadIdx.x % WAVE;
     const int j = lane / RB, p = lane - j * RB;
@@ -163,7 +191,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(Sliced<S> a, const S *__
     const uint32_t row = in_wave ? a.perm[pos] : NONE;
     const bool live = row != NONE;
     V ad = (V)0;
-    if (live) ad = row_product<RB, RB, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, d_in);
+    if (live) ad = row_product<RB, RB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, d_in);
     const size_t i = live ? (size_t)row * RB + p : 0;
     const V r_new = live ? r_in[i] - ad : (V)0;
     // every lane takes part in the shuffle; padding lanes carry zeros and store nothing
@@ -198,55 +226,105 @@ __global__ void convert_kernel(size_t n, const From *__restrict__ in, To *__rest
 
 // The coarsest level's direct solve with the f64 Cholesky factor (block.rs:357-373), the
 // right-hand side widened from V and the solution rounded to V (cycle.rs:475-486). One
-// workgroup; `lt` is the factor transposed (column k of the factor contiguous), `xs` LDS.
-// Forward substitution runs right-looking: once x_k is final every later row subtracts its
-// term, so each row still subtracts in ascending k, the reference's order. Back substitution
-// also sums each row's terms in ascending k, which can only start once every later x is known:
-// the products are formed in parallel and one thread folds them in order. Every barrier is
-// reached by the whole workgroup.
+// wavefront; `lt` is the factor transposed (column k of the factor contiguous) and the running
+// sums live in LDS. Forward substitution runs right-looking: once x_k is final every later row
+// subtracts its term, so each row still subtracts in ascending k, the reference's order. Back
+// substitution also subtracts each row's terms in ascending k, which can only start once every
+// later x is known: the lanes form the products, then one lane folds them in order.
 constexpr uint32_t COARSE_MAX = 3072;
 template <typename V>
-__global__ __launch_bounds__(1024) void coarse_solve_kernel(uint32_t n, const double *__restrict__ lt,
-                                                             const V *__restrict__ rhs, V *__restrict__ out) {
+__global__ __launch_bounds__(64) void coarse_solve_kernel(uint32_t n, const double *__restrict__ lt,
+                                                           const V *__restrict__ rhs, V *__restrict__ out) {
     __shared__ double xs[COARSE_MAX];
     __shared__ double prod[COARSE_MAX];
-    const uint32_t t = threadIdx.x, nt = blockDim.x;
-    constexpr int PER = COARSE_MAX / 1024;
-    double s[PER];
-#pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const uint32_t i = t + m * nt;
-        s[m] = i < n ? (double)rhs[i] : 0.0;
-    }
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < n; i += WAVE) xs[i] = (double)rhs[i];
+    __syncthreads();
     for (uint32_t k = 0; k < n; ++k) {
-        if ((k % nt) == t) {
-            const uint32_t m = k / nt;
-            double v = 0.0;
-#pragma unroll
-            for (int mm = 0; mm < PER; ++mm)
-                if ((uint32_t)mm == m) v = s[mm];
-            xs[k] = v / lt[(size_t)k * n + k];
-        }
+        if (t == k % WAVE) xs[k] = xs[k] / lt[(size_t)k * n + k];
         __syncthreads();
         const double xv = xs[k];
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const uint32_t i = t + m * nt;
-            if (i > k && i < n) s[m] = s[m] - lt[(size_t)k * n + i] * xv;
-        }
+        const double *col = lt + (size_t)k * n;
+        for (uint32_t i = k + 1 + t; i < n; i += WAVE) xs[i] = xs[i] - col[i] * xv;
+        __syncthreads();
     }
-    __syncthreads();
     for (uint32_t ii = n; ii-- > 0;) {
-        for (uint32_t k = ii + 1 + t; k < n; k += nt) prod[k] = lt[(size_t)ii * n + k] * xs[k];
+        const double *col = lt + (size_t)ii * n;
+        for (uint32_t k = ii + 1 + t; k < n; k += WAVE) prod[k] = col[k] * xs[k];
         __syncthreads();
         if (t == 0) {
             double sum = xs[ii];
             for (uint32_t k = ii + 1; k < n; ++k) sum = sum - prod[k];
-            xs[ii] = sum / lt[(size_t)ii * n + ii];
+            xs[ii] = sum / col[ii];
         }
         __syncthreads();
     }
-    for (uint32_t i = t; i < n; i += nt) out[i] = (V)xs[i];
+    for (uint32_t i = t; i < n; i += WAVE) out[i] = (V)xs[i];
+}
+
+// The same solve with the factor held in LDS, packed by columns (column k holds rows k..n-1 at
+// offset k n - k (k - 1) / 2), so no step of either substitution waits on global memory. Used
+// when the packed factor and two vectors fit in one workgroup's LDS.
+__device__ __forceinline__ size_t packed_column(uint32_t n, uint32_t k) { return (size_t)k * n - ((size_t)k * (k - 1)) / 2; }
+
+template <typename V>
+__global__ __launch_bounds__(64) void coarse_solve_lds_kernel(uint32_t n, const double *__restrict__ packed,
+                                                               const V *__restrict__ rhs, V *__restrict__ out) {
+    extern __shared__ double smem[];
+    const size_t np = (size_t)n * (n + 1) / 2;
+    double *l = smem, *xs = smem + np, *prod = xs + n;
+    const uint32_t t = threadIdx.x;
+    for (size_t i = t; i < np; i += WAVE) l[i] = packed[i];
+    for (uint32_t i = t; i < n; i += WAVE) xs[i] = (double)rhs[i];
+    __syncthreads();
+    for (uint32_t k = 0; k < n; ++k) {
+        const double *col = l + packed_column(n, k) - k;  // col[i] = L[i][k]
+        if (t == k % WAVE) xs[k] = xs[k] / col[k];
+        __syncthreads();
+        const double xv = xs[k];
+        for (uint32_t i = k + 1 + t; i < n; i += WAVE) xs[i] = xs[i] - col[i] * xv;
+        __syncthreads();
+    }
+    for (uint32_t ii = n; ii-- > 0;) {
+        const double *col = l + packed_column(n, ii) - ii;
+        for (uint32_t k = ii + 1 + t; k < n; k += WAVE) prod[k] = col[k] * xs[k];
+        __syncthreads();
+        if (t == 0) {
+            double sum = xs[ii];
+            uint32_t k = ii + 1;
+            for (; k + 8 <= n; k += 8) {
+                double p[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) p[u] = prod[k + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) sum = sum - p[u];
+            }
+            for (; k < n; ++k) sum = sum - prod[k];
+            xs[ii] = sum / col[ii];
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = t; i < n; i += WAVE) out[i] = (V)xs[i];
+}
+
+// The coarsest level's solve as a product with the f64 inverse (stored by columns: inv[k n + i]
+// is entry (i, k)), the right-hand side widened from V and staged through LDS, each row summed
+// in ascending k from zero: a fixed order, so the same bits on every run.
+template <typename V>
+__global__ __launch_bounds__(256) void coarse_inverse_kernel(uint32_t n, const double *__restrict__ inv,
+                                                             const V *__restrict__ rhs, V *__restrict__ out) {
+    __shared__ double bs[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sum = 0.0;
+    for (uint32_t k0 = 0; k0 < n; k0 += 256) {
+        if (k0 + threadIdx.x < n) bs[threadIdx.x] = (double)rhs[k0 + threadIdx.x];
+        __syncthreads();
+        const uint32_t kmax = n - k0 < 256u ? n - k0 : 256u;
+        if (i < n)
+            for (uint32_t kk = 0; kk < kmax; ++kk) sum = sum + inv[(size_t)(k0 + kk) * n + i] * bs[kk];
+        __syncthreads();
+    }
+    if (i < n) out[i] = (V)sum;
 }
 
 // ---- conjugate gradient vectors -------------------------------------------------------------
@@ -260,14 +338,16 @@ struct PcgScalars {
     double true_rr;    // ||b - A x||^2
     double bound2;     // sum of squared row rounding bounds
     double alpha;
+    double beta;
     int32_t valid;     // the curvature test of pcg.rs:219-227 passed
     int32_t pad;
 };
 
-// Deterministic dot products: TREE order is a fixed two-stage reduction (each block sums a
-// fixed stride pattern by a fixed shuffle tree, then the last block to finish sums the block
-// partials in index order); REFERENCE order is one thread summing in index order from zero,
-// exactly crates/solidfem-solve/src/pcg.rs:285-291.
+// Deterministic dot products. TREE order is a fixed two-stage reduction: DOT_BLOCKS blocks each
+// sum a fixed grid-stride pattern through a fixed shuffle tree into one partial, then one block
+// sums the partials in the same fixed pattern (a second kernel: no atomics and no fences).
+// REFERENCE order is one thread summing in index order from zero, exactly
+// crates/solidfem-solve/src/pcg.rs:285-291.
 constexpr int DOT_BLOCKS = 512;
 constexpr int DOT_THREADS = 256;
 
@@ -278,51 +358,57 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// mode: 0 = a.b, 1 = sum of a (b unused)
-template <int MODE>
-__global__ __launch_bounds__(DOT_THREADS) void dot_tree_kernel(size_t n, const double *__restrict__ a,
-                                                               const double *__restrict__ b, double *__restrict__ partials,
-                                                               uint32_t *__restrict__ ticket, double *__restrict__ result) {
+__device__ __forceinline__ double block_sum(double v) {
     __shared__ double wsum[DOT_THREADS / WAVE];
-    __shared__ bool last;
-    double v = 0.0;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        v = v + (MODE == 0 ? a[i] * b[i] : a[i]);
     v = wave_sum(v);
     if ((threadIdx.x % WAVE) == 0) wsum[threadIdx.x / WAVE] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < DOT_THREADS / WAVE; ++w) s = s + wsum[w];
-        partials[blockIdx.x] = s;
-        __threadfence();
-        const uint32_t done = atomicAdd(ticket, 1u);
-        last = done == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // the last block: partials in index order through the same fixed tree
     double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) s = s + __builtin_nontemporal_load(&partials[i]);
-    s = wave_sum(s);
-    if ((threadIdx.x % WAVE) == 0) wsum[threadIdx.x / WAVE] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < DOT_THREADS / WAVE; ++w) t = t + wsum[w];
-        *result = t;
-        *ticket = 0u;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < DOT_THREADS / WAVE; ++w) s = s + wsum[w];
+    return s;  // valid in thread 0
+}
+
+// MODE 0: a . b; MODE 1: sum of a.
+template <int MODE>
+__global__ __launch_bounds__(DOT_THREADS) void dot_partials_kernel(size_t n, const double *__restrict__ a,
+                                                                    const double *__restrict__ b, double *__restrict__ partials) {
+    double v = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v = v + (MODE == 0 ? a[i] * b[i] : a[i]);
+    const double s = block_sum(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// What the final sum feeds: STORE writes it to `out`; RZ makes it the next r.z and forms
+// beta = rz_next / rz, rz = rz_next (pcg.rs:258-260).
+enum DotFinal { FINAL_STORE = 0, FINAL_RZ = 1 };
+
+__device__ __forceinline__ void dot_finish(int kind, double total, double *out, PcgScalars *s) {
+    if (kind == FINAL_STORE) {
+        *out = total;
+    } else {
+        s->rz_next = total;
+        s->beta = total / s->rz;
+        s->rz = total;
     }
 }
 
+static __global__ __launch_bounds__(DOT_THREADS) void dot_final_kernel(int parts, const double *__restrict__ partials, int kind,
+                                                                 double *out, PcgScalars *s) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < parts; i += blockDim.x) v = v + partials[i];
+    const double total = block_sum(v);
+    if (threadIdx.x == 0) dot_finish(kind, total, out, s);
+}
+
 template <int MODE>
-__global__ void dot_serial_kernel(size_t n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ result) {
+__global__ void dot_serial_kernel(size_t n, const double *__restrict__ a, const double *__restrict__ b, int kind,
+                                  double *out, PcgScalars *s) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double sum = 0.0;
     for (size_t i = 0; i < n; ++i) sum = sum + (MODE == 0 ? a[i] * b[i] : a[i]);
-    *result = sum;
+    dot_finish(kind, sum, out, s);
 }
 
 // The update of pcg.rs:219-230: the curvature test, then x += alpha p and r += (-alpha) A p.
@@ -341,22 +427,12 @@ static __global__ void pcg_update_kernel(size_t n, PcgScalars *__restrict__ s, c
     r[i] = r[i] + (-alpha) * ap[i];
 }
 
-// p = z + beta p with beta = rz_next / rz, then rz = rz_next (pcg.rs:257-263). The block that
-// owns index 0 retires the scalar after every block has read it: rz is read before any write.
-static __global__ void pcg_direction_kernel(size_t n, PcgScalars *__restrict__ s, const double *__restrict__ z, double *__restrict__ p,
-                                     uint32_t *__restrict__ ticket) {
-    const double beta = s->rz_next / s->rz;
+// p = z + beta p (pcg.rs:261-263), beta formed by the r.z reduction.
+static __global__ void pcg_direction_kernel(size_t n, const PcgScalars *__restrict__ s, const double *__restrict__ z,
+                                            double *__restrict__ p) {
+    const double beta = s->beta;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = z[i] + beta * p[i];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t done = atomicAdd(ticket, 1u);
-        if (done == gridDim.x - 1) {
-            s->rz = s->rz_next;
-            *ticket = 0u;
-        }
-    }
 }
 
 // Jacobi: z = r * inverse diagonal (preconditioner.rs:237-247).
@@ -370,12 +446,12 @@ static __global__ void jacobi_kernel(size_t n, const double *__restrict__ inv, c
 template <typename T>
 __global__ void gather_convert_kernel(size_t n, const uint32_t *__restrict__ block_of_slot, int rb, int cb,
                                       const double *__restrict__ src, T *__restrict__ dst) {
-    // dst index = ((slot * cb + q) * 64 + lane), lane = j * rb + p ; map gives the source block
+    // dst index = ((slot * 64 + lane) * cb + q), lane = j * rb + p ; map gives the source block
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const size_t lane = i % WAVE;
-    const size_t q = (i / WAVE) % cb;
-    const size_t slot_row = i / ((size_t)WAVE * cb);  // (slot, j) pair index = slot * npw + j
+    const size_t q = i % cb;
+    const size_t lane = (i / cb) % WAVE;
+    const size_t slot_row = i / ((size_t)WAVE * cb);  // the slot
     const int npw = WAVE / rb;
     const int j = (int)(lane / rb), p = (int)(lane % rb);
     if (j >= npw) {

@@ -26,6 +26,7 @@ struct SetupOptions {
     size_t degree = 3;
     double range = 30.0;
     int precision = SF_PRECISION_SINGLE_STORAGE;
+    int coarse_solve = SF_COARSE_REFERENCE;
 };
 
 class MultigridSetup {

@@ -88,6 +88,19 @@ def test_device_v_cycle_is_bit_identical(ctx, name, coarse, precision):
     assert np.array_equal(bits(mg.apply(r)), bits(z)), "a second application differs"
 
 
+@pytest.mark.parametrize("precision", [0, 2], ids=["double", "single"])
+def test_inverse_coarse_solve_is_deterministic_and_within_rounding(ctx, precision):
+    ref = refcpu.Problem(WORKLOADS["plate"])
+    _, mg = device_multigrid(ctx, ref, precision, 100)
+    fast = mg.with_precision(sf.Precision(precision), coarse_solve=1)
+    r = np.random.default_rng(8).standard_normal(ref.free_dofs)
+    z_ref, z_fast = mg.apply(r), fast.apply(r)
+    assert np.array_equal(bits(fast.apply(r)), bits(z_fast))
+    scale = np.abs(z_ref).max()
+    tol = 1e-9 if precision == 0 else 1e-3  # an f64 coarse solve's rounding, carried through f64 or f32 cycles
+    assert np.abs(z_fast - z_ref).max() <= tol * scale
+
+
 def test_device_coarse_solve_matches_the_factor(ctx):
     ref = refcpu.Problem(WORKLOADS["plate"])
     mg_ref = refcpu.Multigrid(ref, refcpu.options(precision=0, coarse_dofs=100))
