@@ -87,6 +87,9 @@ enum { SF_DOT_TREE = 0, SF_DOT_REFERENCE = 1 };
 /* Step outcome, crates/solidfem-solve/src/pcg.rs:47-57 */
 enum { SF_RUNNING = 0, SF_CONVERGED = 1, SF_CANCELLED = 2 };
 
+/* A column of a batched solve that failed: the error is in its sf_column_progress. */
+enum { SF_FAILED = 3 };
+
 /* Returns non-zero to cancel. */
 typedef int32_t (*sf_cancel_fn)(void *user);
 
@@ -97,6 +100,20 @@ typedef struct sf_progress {
     int32_t at_rounding_floor;
     int32_t outcome;
 } sf_progress;
+
+/* One column of a batched solve: its Progress (pcg.rs:35-45) and outcome; for SF_FAILED, the
+ * PcgError variant as a status (SF_ERR_PCG_NOT_POSITIVE_DEFINITE or SF_ERR_PCG_NOT_CONVERGED)
+ * with its iteration and value (curvature or relative residual). */
+typedef struct sf_column_progress {
+    uint64_t iteration;
+    double relative_residual;
+    int32_t at_rounding_floor;
+    int32_t outcome;
+    sf_status error;
+    int32_t reserved;
+    uint64_t error_iteration;
+    double error_value;
+} sf_column_progress;
 
 /* PcgOptions, crates/solidfem-solve/src/pcg.rs:14-33 (max_iterations 0 = 10 n + 1000) */
 typedef struct sf_pcg_options {
@@ -196,6 +213,7 @@ typedef struct sf_csr sf_csr;
 typedef struct sf_multigrid_setup sf_multigrid_setup;
 typedef struct sf_multigrid sf_multigrid;
 typedef struct sf_pcg sf_pcg;
+typedef struct sf_pcg_batch sf_pcg_batch;
 typedef struct sf_server sf_server;
 
 /* ---- errors and device ---------------------------------------------------------------- */
@@ -389,6 +407,31 @@ double sf_multigrid_cycle_bytes(const sf_multigrid *mg);
 /* Restarts a solve from a zero estimate (Pcg::new semantics, pcg.rs:135-164) on the buffers it
  * already holds; new_rhs NULL keeps the right-hand side. Allocates nothing. */
 sf_status sf_pcg_restart(sf_pcg *pcg, const double *new_rhs);
+/* ---- several load cases over one matrix and one hierarchy ------------------------------------
+ * No reference counterpart: the reference solves one right-hand side per Pcg
+ * (crates/solidfem-solve/src/pcg.rs:113-283). A batch runs that recurrence for up to 64
+ * right-hand sides in lockstep, each column on its own scalars, so that every pass over the
+ * matrix and the hierarchy serves all of them. Column m's iterates, iteration count and outcome
+ * are bit for bit those of sf_pcg_create with column m alone (same options), whatever the other
+ * columns hold. Right-hand sides and estimates are column-major: column m at [m * n, (m + 1) * n). */
+#define SF_MAX_COLUMNS 64
+/* Z = M^-1 R for `columns` columns (column-major in and out); column m equals
+ * sf_multigrid_apply of column m bit for bit. */
+sf_status sf_multigrid_apply_batch(sf_multigrid *mg, const double *residuals, uint64_t columns, double *outputs);
+sf_status sf_pcg_batch_create(sf_context *ctx, const sf_csr *matrix, sf_multigrid *mg, const double *rhs,
+                              uint64_t columns, const sf_pcg_options *options, sf_pcg_batch **out);
+/* Up to max_iterations lockstep iterations, the cancel callback polled before each (it cancels
+ * every running column). `progress` receives one entry per column; `running` the columns still
+ * running. Column failures are reported per column, not as the call's status. */
+sf_status sf_pcg_batch_step(sf_pcg_batch *batch, uint64_t max_iterations, sf_cancel_fn cancel, void *user,
+                            sf_column_progress *progress, uint64_t *running);
+sf_status sf_pcg_batch_progress(const sf_pcg_batch *batch, sf_column_progress *progress);
+sf_status sf_pcg_batch_estimate(const sf_pcg_batch *batch, double *out);
+/* Restarts every column from a zero estimate; rhs NULL keeps the right-hand sides. */
+sf_status sf_pcg_batch_restart(sf_pcg_batch *batch, const double *rhs);
+uint64_t sf_pcg_batch_columns(const sf_pcg_batch *batch);
+void sf_pcg_batch_destroy(sf_pcg_batch *batch);
+
 /* Kernel launch timing with HIP events on the launching stream. Kernels: 0 = the fused
  * Chebyshev step on the finest level, 1 = the conjugate gradient's system product A p, 2 = the
  * residual on the finest level. read returns launches, summed seconds and summed algorithmic

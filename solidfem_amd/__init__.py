@@ -619,6 +619,15 @@ class Multigrid(_Handle, _Levels):
         _raise(library().sf_multigrid_apply(self._ptr, _ptr(r, C.c_double), _ptr(z, C.c_double)))
         return z
 
+    def apply_batch(self, residuals) -> np.ndarray:
+        """apply() of each row of `residuals` (columns x n), all through one batched V-cycle."""
+        r = np.ascontiguousarray(residuals, dtype=np.float64)
+        if r.ndim != 2:
+            raise ValueError("residuals must be (columns, n)")
+        z = np.empty_like(r)
+        _raise(library().sf_multigrid_apply_batch(self._ptr, _ptr(r, C.c_double), r.shape[0], _ptr(z, C.c_double)))
+        return z
+
     def smooth(self, level: int, b) -> np.ndarray:
         bb = np.ascontiguousarray(b, dtype=np.float64)
         x = np.empty_like(bb)
@@ -675,6 +684,73 @@ class Pcg(_Handle):
 
     def is_converged(self) -> bool:
         return bool(library().sf_pcg_is_converged(self._ptr))
+
+
+@dataclass
+class ColumnProgress:
+    """One column of a batched solve: pcg.rs:35-45 plus its outcome; `error` is the PcgError
+    status (10 or 11) of a failed column, with its iteration and value."""
+    iteration: int
+    relative_residual: float
+    at_rounding_floor: bool
+    outcome: int
+    error: int = 0
+    error_iteration: int = 0
+    error_value: float = 0.0
+
+    @classmethod
+    def of(cls, p):
+        return cls(p.iteration, p.relative_residual, bool(p.at_rounding_floor), p.outcome, p.error, p.error_iteration,
+                   p.error_value)
+
+
+class PcgBatch(_Handle):
+    """Up to 64 load cases over one matrix and one preconditioner, each column the recurrence of
+    pcg.rs:113-283 on its own scalars, in lockstep: column m's iterates are bit for bit those of
+    Pcg.new with that right-hand side alone."""
+    _destroy = "sf_pcg_batch_destroy"
+    FAILED = 3
+
+    @classmethod
+    def new(cls, ctx: Context, matrix: CsrMatrix, preconditioner: Optional[Multigrid], rhs, options: PcgOptions = None):
+        options = options or PcgOptions()
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        if b.ndim != 2:
+            raise ValueError("rhs must be (columns, n)")
+        o = options.c()
+        h = C.c_void_p()
+        _raise(library().sf_pcg_batch_create(ctx._ptr, matrix._ptr, preconditioner._ptr if preconditioner else None,
+                                             _ptr(b, C.c_double), b.shape[0], C.byref(o), C.byref(h)))
+        p = cls(h)
+        p._keep = (ctx, matrix, preconditioner)
+        p.columns, p.n = b.shape
+        return p
+
+    def step(self, max_iterations: int, cancel=None):
+        """Runs up to `max_iterations` lockstep iterations; returns (running columns, [ColumnProgress])."""
+        progress = (_capi.ColumnProgress * self.columns)()
+        running = C.c_uint64()
+        _raise(library().sf_pcg_batch_step(self._ptr, max_iterations, _cancel(cancel), None, progress, C.byref(running)))
+        return running.value, [ColumnProgress.of(p) for p in progress]
+
+    def progress(self):
+        progress = (_capi.ColumnProgress * self.columns)()
+        _raise(library().sf_pcg_batch_progress(self._ptr, progress))
+        return [ColumnProgress.of(p) for p in progress]
+
+    def restart(self, rhs=None):
+        if rhs is None:
+            _raise(library().sf_pcg_batch_restart(self._ptr, None))
+        else:
+            b = np.ascontiguousarray(rhs, dtype=np.float64)
+            if b.shape != (self.columns, self.n):
+                raise ValueError("rhs must keep the batch's shape")
+            _raise(library().sf_pcg_batch_restart(self._ptr, _ptr(b, C.c_double)))
+
+    def estimate(self) -> np.ndarray:
+        out = np.empty((self.columns, self.n))
+        _raise(library().sf_pcg_batch_estimate(self._ptr, _ptr(out, C.c_double)))
+        return out
 
 
 class Kernel(enum.IntEnum):

@@ -22,6 +22,11 @@ class Progress(C.Structure):
     _fields_ = [("iteration", u64), ("relative_residual", f64), ("at_rounding_floor", i32), ("outcome", i32)]
 
 
+class ColumnProgress(C.Structure):
+    _fields_ = [("iteration", u64), ("relative_residual", f64), ("at_rounding_floor", i32), ("outcome", i32),
+                ("error", i32), ("reserved", i32), ("error_iteration", u64), ("error_value", f64)]
+
+
 class PcgOptions(C.Structure):
     _fields_ = [("tolerance", f64), ("max_iterations", u64), ("dot_order", i32), ("reserved", i32)]
 
@@ -179,6 +184,14 @@ SIGNATURES = {
     "sf_profile_reset": (i32, []),
     "sf_profile_read": (i32, [i32, P(u64), P(f64), P(f64)]),
     "sf_stream_copy": (i32, [vp, u64, u32, P(f64)]),
+    "sf_multigrid_apply_batch": (i32, [vp, P(f64), u64, P(f64)]),
+    "sf_pcg_batch_create": (i32, [vp, vp, vp, P(f64), u64, P(PcgOptions), P(vp)]),
+    "sf_pcg_batch_step": (i32, [vp, u64, CANCEL_FN, vp, P(ColumnProgress), P(u64)]),
+    "sf_pcg_batch_progress": (i32, [vp, P(ColumnProgress)]),
+    "sf_pcg_batch_estimate": (i32, [vp, P(f64)]),
+    "sf_pcg_batch_restart": (i32, [vp, P(f64)]),
+    "sf_pcg_batch_columns": (u64, [vp]),
+    "sf_pcg_batch_destroy": (None, [vp]),
 }
 
 _lib = None

@@ -1,0 +1,366 @@
+// Device kernels for several right-hand sides at once (load cases sharing one matrix and one
+// multigrid hierarchy).
+//
+// Vectors of M columns are stored row-interleaved: entry (row i, column m) at v[i * M + m], so
+// the M values of a row are contiguous. The matrices are the sliced block matrices of the
+// single-column kernels, read with another lane mapping: a group of G lanes (G a power of two,
+// G >= M) takes one block row, lane m of the group its column m, and every lane forms the RB
+// row sums of its block row for its column. In the sliced layout the RB x CB values of the block
+// in slot k of slice row j are contiguous at vals[((start + k) * 64 + j * RB) * CB], so a group
+// reads each block once (one broadcast) and each column of x as G consecutive values.
+//
+// Each row sum runs over the row's blocks in stored order and over a block's columns in order,
+// from zero, exactly as the single-column kernels and the reference's block product
+// (crates/solidfem-solve/src/multigrid/block.rs:136-154) do: column m of a batched product is
+// bit for bit the single-column product of column m. The same holds for every kernel below, so a
+// column's solve does not depend on the columns batched with it.
+#pragma once
+
+#include "sf_kernels.hpp"
+
+namespace sf {
+namespace dev {
+
+// A group's place: the node position (slice * NPW + j) it works on and its column.
+struct GroupLane {
+    uint32_t pos;  // node position in slice order
+    int m;         // column of this lane
+};
+
+__device__ __forceinline__ GroupLane group_lane(int log2g) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    return {t >> log2g, (int)(t & ((1u << log2g) - 1u))};
+}
+
+// Row sums of one block row for column m: sum[p] = sum over stored blocks k and block columns q
+// of A[p][q] x[(col * CB + q) * M + m], each from zero in the reference order.
+template <int RB, int CB, int BATCH, typename S, typename V>
+__device__ __forceinline__ void rows_product(const Sliced<S> &a, uint32_t slot0, uint32_t len, int j, int m, uint32_t M,
+                                             const V *__restrict__ x, V (&sum)[RB]) {
+    constexpr int NPW = Shape<RB>::NPW;
+#pragma unroll
+    for (int p = 0; p < RB; ++p) sum[p] = (V)0;
+    const uint32_t *cp = a.cols + (size_t)slot0 * NPW + j;
+    const S *vp = a.vals + ((size_t)slot0 * WAVE + (size_t)j * RB) * CB;
+    const uint32_t full = len - len % BATCH;
+    for (uint32_t k = 0; k < full; k += BATCH) {
+        uint32_t c[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) c[b] = cp[(size_t)(k + b) * NPW];
+        V xv[BATCH][CB];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int q = 0; q < CB; ++q) xv[b][q] = x[((size_t)c[b] * CB + q) * M + m];
+        S v[BATCH][RB * CB];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int e = 0; e < RB * CB; ++e) v[b][e] = vp[(size_t)(k + b) * WAVE * CB + e];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int p = 0; p < RB; ++p)
+#pragma unroll
+                for (int q = 0; q < CB; ++q) sum[p] = sum[p] + widen<V>(v[b][p * CB + q]) * xv[b][q];
+    }
+    for (uint32_t k = full; k < len; ++k) {
+        const uint32_t col = cp[(size_t)k * NPW];
+        V xv[CB];
+#pragma unroll
+        for (int q = 0; q < CB; ++q) xv[q] = x[((size_t)col * CB + q) * M + m];
+        const S *blk = vp + (size_t)k * WAVE * CB;
+#pragma unroll
+        for (int p = 0; p < RB; ++p)
+#pragma unroll
+            for (int q = 0; q < CB; ++q) sum[p] = sum[p] + widen<V>(blk[p * CB + q]) * xv[q];
+    }
+}
+
+// Y = A X (STORE), Y = B - A X (RESIDUAL) or Y = Y + A X (ADD) for M columns: the batched form
+// of spmv_kernel (products, residuals, restriction over the stored transpose, prolongation).
+template <int RB, int CB, int BATCH, typename S, typename V>
+__global__ __launch_bounds__(256) void spmm_kernel(Sliced<S> a, int log2g, uint32_t M, int epi, const V *__restrict__ x,
+                                                   V *__restrict__ y, const V *__restrict__ b) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const GroupLane g = group_lane(log2g);
+    if (g.pos >= a.slices * (uint32_t)NPW || (uint32_t)g.m >= M) return;
+    const uint32_t row = a.perm[g.pos];
+    if (row == NONE) return;
+    const uint32_t slice = g.pos / NPW;
+    const int j = (int)(g.pos - slice * NPW);
+    V sum[RB];
+    rows_product<RB, CB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[g.pos], j, g.m, M, x, sum);
+#pragma unroll
+    for (int p = 0; p < RB; ++p) {
+        const size_t out = ((size_t)row * RB + p) * M + g.m;
+        if (epi == EPI_STORE) y[out] = sum[p];
+        else if (epi == EPI_RESIDUAL) y[out] = b[out] - sum[p];
+        else y[out] = y[out] + sum[p];
+    }
+}
+
+// The rounding bound terms of CsrMatrix::multiply_rounding_bound (crates/solidfem-solve/src/csr.rs:142-154)
+// for M columns: per row and column, ((terms * eps) * sum |a x|)^2.
+template <int RB, int CB>
+__global__ __launch_bounds__(256) void rounding_bound_batch_kernel(Sliced<double> a, int log2g, uint32_t M,
+                                                                   const double *__restrict__ x, double *__restrict__ out) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const GroupLane g = group_lane(log2g);
+    if (g.pos >= a.slices * (uint32_t)NPW || (uint32_t)g.m >= M) return;
+    const uint32_t row = a.perm[g.pos];
+    if (row == NONE) return;
+    const uint32_t slice = g.pos / NPW;
+    const int j = (int)(g.pos - slice * NPW);
+    const uint32_t len = a.row_len[g.pos], slot0 = a.slice_start[slice];
+    double sum[RB];
+#pragma unroll
+    for (int p = 0; p < RB; ++p) sum[p] = 0.0;
+    for (uint32_t k = 0; k < len; ++k) {
+        const uint32_t col = a.cols[(size_t)(slot0 + k) * NPW + j];
+        const double *blk = a.vals + ((size_t)(slot0 + k) * WAVE + (size_t)j * RB) * CB;
+#pragma unroll
+        for (int p = 0; p < RB; ++p)
+#pragma unroll
+            for (int q = 0; q < CB; ++q) sum[p] = sum[p] + fabs(blk[p * CB + q] * x[((size_t)col * CB + q) * M + g.m]);
+    }
+#pragma unroll
+    for (int p = 0; p < RB; ++p) {
+        const double bound = (double)(len * CB) * 2.220446049250313e-16 * sum[p];
+        out[((size_t)row * RB + p) * M + g.m] = bound * bound;
+    }
+}
+
+// The Chebyshev start for M columns (crates/solidfem-solve/src/multigrid/cycle.rs:591-610):
+// z = D^-1 b, d = z / theta, x = d. One thread per block row and column.
+template <int RB, typename S, typename V>
+__global__ __launch_bounds__(256) void cheb_start_batch_kernel(uint32_t rows, uint32_t M, const S *__restrict__ dinv,
+                                                               const V *__restrict__ b, V theta, V *__restrict__ d,
+                                                               V *__restrict__ x) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)rows * M) return;
+    const size_t row = t / M, m = t - row * M;
+    V bv[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) bv[q] = b[(row * RB + q) * M + m];
+#pragma unroll
+    for (int p = 0; p < RB; ++p) {
+        const S *dr = dinv + row * RB * RB + p * RB;
+        V z = (V)0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) z = z + widen<V>(dr[q]) * bv[q];
+        const V dv = z / theta;
+        d[(row * RB + p) * M + m] = dv;
+        x[(row * RB + p) * M + m] = dv;
+    }
+}
+
+// One later Chebyshev step for M columns, fused as cheb_step_kernel: ad = A d_in;
+// r_out = r_in - ad; z = D^-1 r_out (the lane holds its block row's residuals); d_out = keep d_in
+// + step z; x += d_out; with `sum_into`, sum_into += x (cycle.rs:611-650, :534-536, :655-674).
+template <int RB, int BATCH, typename S, typename V>
+__global__ __launch_bounds__(256) void cheb_step_batch_kernel(Sliced<S> a, int log2g, uint32_t M, const S *__restrict__ dinv,
+                                                              const V *__restrict__ d_in, const V *__restrict__ r_in, V keep,
+                                                              V step, V *__restrict__ d_out, V *__restrict__ r_out,
+                                                              V *__restrict__ x, V *__restrict__ sum_into) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const GroupLane g = group_lane(log2g);
+    if (g.pos >= a.slices * (uint32_t)NPW || (uint32_t)g.m >= M) return;
+    const uint32_t row = a.perm[g.pos];
+    if (row == NONE) return;
+    const uint32_t slice = g.pos / NPW;
+    const int j = (int)(g.pos - slice * NPW);
+    V ad[RB];
+    rows_product<RB, RB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[g.pos], j, g.m, M, d_in, ad);
+    V r_new[RB];
+#pragma unroll
+    for (int p = 0; p < RB; ++p) r_new[p] = r_in[((size_t)row * RB + p) * M + g.m] - ad[p];
+#pragma unroll
+    for (int p = 0; p < RB; ++p) {
+        const S *drow = dinv + (size_t)row * RB * RB + (size_t)p * RB;
+        V z = (V)0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * r_new[q];
+        const size_t i = ((size_t)row * RB + p) * M + g.m;
+        const V dn = keep * d_in[i] + step * z;
+        const V xn = x[i] + dn;
+        d_out[i] = dn;
+        r_out[i] = r_new[p];
+        x[i] = xn;
+        if (sum_into) sum_into[i] = sum_into[i] + xn;
+    }
+}
+
+// The coarsest solve as a product with the f64 inverse for M columns (coarse_inverse_kernel per
+// column): out[i][m] = sum over k ascending of inv(i, k) b[k][m], from zero. A block takes 64
+// rows and all columns; its four waves split the columns, each lane holding up to 16 sums.
+constexpr int COARSE_TILE_K = 32;
+template <typename V>
+__global__ __launch_bounds__(256) void coarse_inverse_batch_kernel(uint32_t n, uint32_t M, const double *__restrict__ inv,
+                                                                   const V *__restrict__ rhs, V *__restrict__ out) {
+    __shared__ double bs[COARSE_TILE_K * 64];
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    const uint32_t i = blockIdx.x * WAVE + lane;
+    const uint32_t per = (M + 3) / 4;  // columns per wave, at most 16
+    const uint32_t m0 = wave * per;
+    double sum[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) sum[c] = 0.0;
+    for (uint32_t k0 = 0; k0 < n; k0 += COARSE_TILE_K) {
+        const uint32_t kn = n - k0 < (uint32_t)COARSE_TILE_K ? n - k0 : (uint32_t)COARSE_TILE_K;
+        for (uint32_t e = threadIdx.x; e < kn * M; e += blockDim.x) bs[e] = (double)rhs[(size_t)k0 * M + e];
+        __syncthreads();
+        if (i < n) {
+            for (uint32_t kk = 0; kk < kn; ++kk) {
+                const double a = inv[(size_t)(k0 + kk) * n + i];
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if ((uint32_t)c < per && m0 + c < M) sum[c] = sum[c] + a * bs[kk * M + m0 + c];
+            }
+        }
+        __syncthreads();
+    }
+    if (i < n)
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+            if ((uint32_t)c < per && m0 + c < M) out[(size_t)i * M + m0 + c] = (V)sum[c];
+}
+
+// ---- conjugate gradient, M columns -----------------------------------------------------------
+
+// Batched dot products whose every column is bit for bit the single-column TREE dot
+// (dot_partials_kernel then dot_final_kernel): block `blk` forms, for each thread slot t of the
+// single kernel and each column, the same grid-stride sum from zero; the sums are staged in LDS
+// and reduced by the same pairs as the single kernel's shuffle butterfly (lane l with l + off for
+// off = 32 .. 1, lane 0's value), then its four wave sums in order from zero. Columns read
+// coalesced: thread u takes column u % M of slots t = u / M, u / M + 256 / M, ...
+template <int MODE>
+__global__ __launch_bounds__(DOT_THREADS) void dot_partials_batch_kernel(size_t n, uint32_t M, const double *__restrict__ a,
+                                                                          const double *__restrict__ b,
+                                                                          double *__restrict__ partials) {
+    extern __shared__ double ls[];  // DOT_THREADS x M
+    const uint32_t u = threadIdx.x;
+    const uint32_t groups = DOT_THREADS / M;
+    const size_t stride = (size_t)gridDim.x * DOT_THREADS;
+    if (u < groups * M) {
+        const uint32_t m = u % M;
+        for (uint32_t t = u / M; t < (uint32_t)DOT_THREADS; t += groups) {
+            double v = 0.0;
+            for (size_t i = (size_t)blockIdx.x * DOT_THREADS + t; i < n; i += stride)
+                v = v + (MODE == 0 ? a[i * M + m] * b[i * M + m] : a[i * M + m]);
+            ls[t * M + m] = v;
+        }
+    }
+    __syncthreads();
+    // the butterfly of wave_sum, as pairs: s[l] = s[l] + s[l + off] for l < off
+    for (uint32_t off = WAVE / 2; off > 0; off >>= 1) {
+        const uint32_t work = (DOT_THREADS / WAVE) * off * M;
+        for (uint32_t e = u; e < work; e += DOT_THREADS) {
+            const uint32_t m = e % M, rest = e / M, l = rest % off, w = rest / off;
+            const uint32_t t = w * WAVE + l;
+            ls[t * M + m] = ls[t * M + m] + ls[(t + off) * M + m];
+        }
+        __syncthreads();
+    }
+    for (uint32_t m = u; m < M; m += DOT_THREADS) {
+        double s = 0.0;
+        for (int w = 0; w < DOT_THREADS / WAVE; ++w) s = s + ls[(size_t)w * WAVE * M + m];
+        partials[(size_t)blockIdx.x * M + m] = s;
+    }
+}
+
+// What a batched final sum feeds, per column: STORE writes the column's `out` field (`field` is a
+// byte offset into PcgScalars); RZ forms rz_next, beta and rz as dot_finish does. Columns whose
+// `mask` entry is zero are left as they are.
+static __global__ __launch_bounds__(DOT_THREADS) void dot_final_batch_kernel(int parts, uint32_t M, const double *__restrict__ partials,
+                                                                             int kind, uint32_t field, PcgScalars *s,
+                                                                             const uint32_t *__restrict__ mask) {
+    const uint32_t m = blockIdx.x;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < parts; i += blockDim.x) v = v + partials[(size_t)i * M + m];
+    const double total = block_sum(v);
+    if (threadIdx.x == 0 && (!mask || mask[m])) {
+        PcgScalars *c = s + m;
+        if (kind == FINAL_STORE) *(double *)((char *)c + field) = total;
+        else dot_finish(FINAL_RZ, total, nullptr, c);
+    }
+}
+
+// REFERENCE order for M columns: one thread per column sums in index order from zero
+// (crates/solidfem-solve/src/pcg.rs:285-291).
+template <int MODE>
+__global__ void dot_serial_batch_kernel(size_t n, uint32_t M, const double *__restrict__ a, const double *__restrict__ b,
+                                        int kind, uint32_t field, PcgScalars *s, const uint32_t *__restrict__ mask) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) sum = sum + (MODE == 0 ? a[i * M + m] * b[i * M + m] : a[i * M + m]);
+    if (mask && !mask[m]) return;
+    PcgScalars *c = s + m;
+    if (kind == FINAL_STORE) *(double *)((char *)c + field) = sum;
+    else dot_finish(FINAL_RZ, sum, nullptr, c);
+}
+
+// pcg_update_kernel per column; columns whose `active` entry is zero are not touched.
+static __global__ void pcg_update_batch_kernel(size_t n, uint32_t M, PcgScalars *__restrict__ s, const uint32_t *__restrict__ active,
+                                               const double *__restrict__ p, const double *__restrict__ ap,
+                                               double *__restrict__ x, double *__restrict__ r) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * M) return;
+    const uint32_t m = (uint32_t)(e % M);
+    if (!active[m]) return;
+    const double curvature = s[m].curvature, rz = s[m].rz;
+    const bool valid = !(isnan(curvature) || curvature <= 0.0 || !isfinite(rz));
+    const double alpha = rz / curvature;
+    if (e < M) {
+        s[m].valid = valid ? 1 : 0;
+        s[m].alpha = alpha;
+    }
+    if (!valid) return;
+    x[e] = x[e] + alpha * p[e];
+    r[e] = r[e] + (-alpha) * ap[e];
+}
+
+// p = z + beta p per active column.
+static __global__ void pcg_direction_batch_kernel(size_t n, uint32_t M, const PcgScalars *__restrict__ s,
+                                                  const uint32_t *__restrict__ active, const double *__restrict__ z,
+                                                  double *__restrict__ p) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * M) return;
+    const uint32_t m = (uint32_t)(e % M);
+    if (active[m]) p[e] = z[e] + s[m].beta * p[e];
+}
+
+// dst = src in the columns whose mask entry is set.
+template <typename T>
+__global__ void copy_columns_kernel(size_t n, uint32_t M, const uint32_t *__restrict__ mask, const T *__restrict__ src,
+                                    T *__restrict__ dst) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n * M && mask[e % M]) dst[e] = src[e];
+}
+
+// Column-major (column m at m * n) to row-interleaved, and back.
+template <typename T>
+__global__ void interleave_kernel(size_t n, uint32_t M, const T *__restrict__ cm, T *__restrict__ il) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * M) return;
+    const size_t i = e / M, m = e - i * M;
+    il[e] = cm[m * n + i];
+}
+template <typename T>
+__global__ void deinterleave_kernel(size_t n, uint32_t M, const T *__restrict__ il, T *__restrict__ cm) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * M) return;
+    const size_t m = e / n, i = e - m * n;
+    cm[e] = il[i * M + m];
+}
+
+// z = r * inverse diagonal for M columns (preconditioner.rs:237-247).
+static __global__ void jacobi_batch_kernel(size_t n, uint32_t M, const double *__restrict__ inv, const double *__restrict__ r,
+                                           double *__restrict__ z) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n * M) z[e] = r[e] * inv[e / M];
+}
+
+}  // namespace dev
+}  // namespace sf

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 
+#include "sf_batch_kernels.hpp"
 #include "sf_device.hpp"
 
 using namespace sf;
@@ -139,6 +140,10 @@ struct sf_multigrid_setup {
 };
 struct sf_pcg {
     std::unique_ptr<DevicePcg> pcg;
+};
+
+struct sf_pcg_batch {
+    std::unique_ptr<DevicePcgBatch> batch;
 };
 
 struct sf_static_study {
@@ -809,6 +814,56 @@ sf_status sf_pcg_progress(const sf_pcg *pcg, sf_progress *out) {
 int32_t sf_pcg_is_converged(const sf_pcg *pcg) { return pcg->pcg->converged() ? 1 : 0; }
 uint64_t sf_pcg_size(const sf_pcg *pcg) { return pcg->pcg->size(); }
 void sf_pcg_destroy(sf_pcg *pcg) { delete pcg; }
+
+sf_status sf_multigrid_apply_batch(sf_multigrid *mg, const double *r, uint64_t columns, double *z) {
+    return guard([&] {
+        if (columns == 0 || columns > MAX_COLUMNS) throw Error(SF_ERR_INVALID_ARGUMENT, "a batch carries 1 to 64 columns");
+        hipStream_t s = mg->ctx->ctx->stream;
+        const size_t n = mg->mg->dofs(), total = n * columns;
+        const uint32_t M = (uint32_t)columns;
+        DeviceBuffer<double> stage, dr(std::max<size_t>(total, 1)), dz(std::max<size_t>(total, 1));
+        stage.upload(r, total, s);
+        const unsigned g = (unsigned)((total + 255) / 256);
+        if (total) {
+            hipLaunchKernelGGL((dev::interleave_kernel<double>), dim3(g), dim3(256), 0, s, n, M, (const double *)stage.ptr, dr.ptr);
+            mg->mg->apply_batch(dr.ptr, dz.ptr, M, s);
+            hipLaunchKernelGGL((dev::deinterleave_kernel<double>), dim3(g), dim3(256), 0, s, n, M, (const double *)dz.ptr, stage.ptr);
+            SF_HIP(hipGetLastError());
+        }
+        stage.download(z, total, s);
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+sf_status sf_pcg_batch_create(sf_context *ctx, const sf_csr *matrix, sf_multigrid *mg, const double *rhs, uint64_t columns,
+                              const sf_pcg_options *options, sf_pcg_batch **out) {
+    return guard([&] {
+        if (columns == 0 || columns > MAX_COLUMNS) throw Error(SF_ERR_INVALID_ARGUMENT, "a batch carries 1 to 64 right-hand sides");
+        auto p = std::make_unique<sf_pcg_batch>();
+        const sf_pcg_options o = options ? *options : sf_pcg_options{1e-9, 0, SF_DOT_TREE, 0};
+        p->batch = std::make_unique<DevicePcgBatch>(*ctx->ctx, matrix->system, mg ? mg->mg.get() : nullptr, rhs, (uint32_t)columns, o);
+        *out = p.release();
+    });
+}
+sf_status sf_pcg_batch_step(sf_pcg_batch *batch, uint64_t max_iterations, sf_cancel_fn cancel, void *user,
+                            sf_column_progress *progress, uint64_t *running) {
+    uint32_t live = 0;
+    sf_status st = guard([&] { live = batch->batch->step(max_iterations, Cancel{cancel, user}); });
+    if (st == SF_OK && running) *running = live;
+    if (progress) batch->batch->progress(progress);
+    return st;
+}
+sf_status sf_pcg_batch_progress(const sf_pcg_batch *batch, sf_column_progress *progress) {
+    return guard([&] { batch->batch->progress(progress); });
+}
+sf_status sf_pcg_batch_estimate(const sf_pcg_batch *batch, double *out) {
+    return guard([&] { batch->batch->estimate(out); });
+}
+sf_status sf_pcg_batch_restart(sf_pcg_batch *batch, const double *rhs) {
+    return guard([&] { batch->batch->restart(rhs); });
+}
+uint64_t sf_pcg_batch_columns(const sf_pcg_batch *batch) { return batch->batch->columns(); }
+void sf_pcg_batch_destroy(sf_pcg_batch *batch) { delete batch; }
 
 }  // extern "C"
 

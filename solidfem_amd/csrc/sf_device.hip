@@ -5,6 +5,7 @@
 #include <cstring>
 #include <numeric>
 
+#include "sf_batch_kernels.hpp"
 #include "sf_device.hpp"
 
 namespace sf {
@@ -219,19 +220,20 @@ void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi
     SF_HIP(hipGetLastError());
 }
 
-int tune_batch(hipStream_t s, const std::function<void(int)> &launch) {
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates) {
     static const int forced = [] {
         const char *e = std::getenv("SOLIDFEM_BATCH");
         return e ? std::atoi(e) : 0;
     }();
-    if (forced == 1 || forced == 2 || forced == 4 || forced == 8) return forced;
+    for (int c : candidates)
+        if (forced == c) return forced;
     hipEvent_t e0, e1;
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
     static const bool verbose = std::getenv("SOLIDFEM_TUNE_LOG") != nullptr;
-    int best = 4;
+    int best = *candidates.begin();
     float best_ms = 1e30f;
-    for (int batch : {1, 2, 4, 8}) {
+    for (int batch : candidates) {
         launch(batch);  // warm
         SF_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < 3; ++r) launch(batch);
@@ -298,6 +300,11 @@ struct LevelDev {
     double theta = 0;
     std::vector<std::pair<double, double>> steps;
     DeviceBuffer<V> b, x, d0, d1, r0, r1, res, px;
+    // the same work vectors for several columns (row-interleaved), grown on demand
+    DeviceBuffer<V> mb, mx, md0, md1, mr0, mr1, mres, mpx;
+    // batch depths of the batched kernels per log2 of the column group: Chebyshev step, residual,
+    // restriction, prolongation (0 = not tuned yet)
+    int mm_depth[8][4] = {};
 };
 
 template <typename S, typename V>
@@ -417,6 +424,21 @@ class Hierarchy final : public DeviceMultigrid {
         SF_HIP(hipGetLastError());
     }
 
+    void apply_batch(const double *r, double *z, uint32_t columns, hipStream_t s) override {
+        if (lv_.empty() || columns == 0) return;
+        if (columns > MAX_COLUMNS) throw Error(SF_ERR_INVALID_ARGUMENT, "more columns than a batch carries");
+        prepare_batch(columns, s);
+        if constexpr (std::is_same<V, double>::value) {
+            cycle_batch(0, r, z, columns, s);
+        } else {
+            const size_t n = lv_[0].n * (size_t)columns;
+            hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, r, min_.ptr);
+            cycle_batch(0, min_.ptr, mout_.ptr, columns, s);
+            hipLaunchKernelGGL((convert_kernel<double, V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const V *)mout_.ptr, z);
+        }
+        SF_HIP(hipGetLastError());
+    }
+
     void smooth_level(size_t level, const double *b, double *x, hipStream_t s) override {
         LevelDev<S, V> &d = lv_.at(level);
         if constexpr (std::is_same<V, double>::value) {
@@ -433,9 +455,9 @@ class Hierarchy final : public DeviceMultigrid {
         LevelDev<S, V> &d = lv_.back();
         if (!d.has_factor) throw Error(SF_ERR_MG_COARSEST_NOT_FACTORED, "the coarsest level has no direct factorization", (int64_t)d.n);
         if (d.lds_bytes)
-            hipLaunchKernelGGL(coarse_solve_lds_kernel<double>, dim3(1), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x);
+            hipLaunchKernelGGL(coarse_solve_lds_kernel<double>, dim3(1), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x, 1u);
         else
-            hipLaunchKernelGGL(coarse_solve_kernel<double>, dim3(1), dim3(WAVE), 0, s, (uint32_t)d.n, d.lt.ptr, b, x);
+            hipLaunchKernelGGL(coarse_solve_kernel<double>, dim3(1), dim3(WAVE), 0, s, (uint32_t)d.n, d.lt.ptr, b, x, 1u);
         SF_HIP(hipGetLastError());
     }
 
@@ -460,6 +482,125 @@ class Hierarchy final : public DeviceMultigrid {
                                  V *x, V *into, hipStream_t s) {
         if (d.rb == 3) cheb_step_rb<3>(d, batch, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
         else cheb_step_rb<6>(d, batch, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+    }
+
+    // ---- several columns ----
+
+    template <int RB, int BATCH>
+    static void cheb_batch_launch(LevelDev<S, V> &d, uint32_t M, const V *d_in, const V *r_in, V keep, V stp, V *d_out,
+                                  V *r_out, V *x, V *into, hipStream_t s) {
+        const int lg = columns_log2(M);
+        const size_t threads = (size_t)d.a.slices * Shape<RB>::NPW << lg;
+        hipLaunchKernelGGL((cheb_step_batch_kernel<RB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.view(),
+                           lg, M, d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+    }
+    template <int RB>
+    static void cheb_batch_rb(LevelDev<S, V> &d, int batch, uint32_t M, const V *d_in, const V *r_in, V keep, V stp,
+                              V *d_out, V *r_out, V *x, V *into, hipStream_t s) {
+        switch (batch) {
+            case 1: cheb_batch_launch<RB, 1>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+            case 2: cheb_batch_launch<RB, 2>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+            default: cheb_batch_launch<RB, 4>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+        }
+    }
+    static void launch_cheb_batch(LevelDev<S, V> &d, int batch, uint32_t M, const V *d_in, const V *r_in, V keep, V stp,
+                                  V *d_out, V *r_out, V *x, V *into, hipStream_t s) {
+        if (d.a.slices == 0) return;
+        if (d.rb == 3) cheb_batch_rb<3>(d, batch, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+        else cheb_batch_rb<6>(d, batch, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+    }
+
+    // Sizes the column work vectors for M columns and tunes the batched kernels for M's group.
+    void prepare_batch(uint32_t M, hipStream_t s) {
+        const int lg = columns_log2(M);
+        if (lv_[0].mb.count < lv_[0].n * (size_t)M) SF_HIP(hipStreamSynchronize(s));  // buffers in use are freed below
+        for (size_t l = 0; l < lv_.size(); ++l) {
+            LevelDev<S, V> &d = lv_[l];
+            const size_t need = d.n * (size_t)M;
+            if (d.mb.count < need)
+                for (auto *buf : {&d.mb, &d.mx, &d.md0, &d.md1, &d.mr0, &d.mr1, &d.mres, &d.mpx}) buf->resize(need);
+        }
+        const size_t top = lv_[0].n * (size_t)M;
+        if (!std::is_same<V, double>::value && min_.count < top) {
+            min_.resize(top);
+            mout_.resize(top);
+        }
+        for (size_t l = 0; l < lv_.size(); ++l) {
+            LevelDev<S, V> &d = lv_[l];
+            if (d.mm_depth[lg][0]) continue;
+            const size_t need = d.n * (size_t)M;
+            V *vs[8] = {d.mb.ptr, d.mx.ptr, d.md0.ptr, d.md1.ptr, d.mr0.ptr, d.mr1.ptr, d.mres.ptr, d.mpx.ptr};
+            for (V *v : vs) SF_HIP(hipMemsetAsync(v, 0, need * sizeof(V), s));
+            d.mm_depth[lg][0] = tune_batch(s, [&](int batch) {
+                launch_cheb_batch(d, batch, M, d.md0.ptr, d.mb.ptr, (V)0, (V)0, d.md1.ptr, d.mr0.ptr, d.mx.ptr, nullptr, s);
+            }, spmm_candidates(d.rb, d.rb, sizeof(S)));
+            d.mm_depth[lg][1] = tune_batch(s, [&](int batch) {
+                launch_spmm<S, V>(d.a, M, d.mx.ptr, d.mres.ptr, d.mb.ptr, EPI_RESIDUAL, s, batch);
+            }, spmm_candidates(d.rb, d.rb, sizeof(S)));
+            if (d.has_p) {
+                LevelDev<S, V> &c = lv_[l + 1];
+                const size_t cneed = c.n * (size_t)M;
+                SF_HIP(hipMemsetAsync(c.mb.ptr, 0, cneed * sizeof(V), s));
+                SF_HIP(hipMemsetAsync(c.mx.ptr, 0, cneed * sizeof(V), s));
+                d.mm_depth[lg][2] = tune_batch(s, [&](int batch) {
+                    launch_spmm<S, V>(d.r, M, d.mres.ptr, c.mb.ptr, nullptr, EPI_STORE, s, batch);
+                }, spmm_candidates(d.r.rb, d.r.cb, sizeof(S)));
+                d.mm_depth[lg][3] = tune_batch(s, [&](int batch) {
+                    launch_spmm<S, V>(d.p, M, c.mx.ptr, d.mpx.ptr, nullptr, EPI_ADD, s, batch);
+                }, spmm_candidates(d.p.rb, d.p.cb, sizeof(S)));
+            }
+        }
+        SF_HIP(hipStreamSynchronize(s));
+    }
+
+    void smooth_batch(LevelDev<S, V> &d, const V *b, V *x, V *sum_into, uint32_t M, hipStream_t s) {
+        const int lg = columns_log2(M);
+        const unsigned g = grid_for(d.rows * (size_t)M, 256);
+        if (d.rb == 3)
+            hipLaunchKernelGGL((cheb_start_batch_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, M, d.dinv.ptr, b, (V)d.theta, d.md0.ptr, x);
+        else
+            hipLaunchKernelGGL((cheb_start_batch_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, M, d.dinv.ptr, b, (V)d.theta, d.md0.ptr, x);
+        V *dd[2] = {d.md0.ptr, d.md1.ptr};
+        V *rr[2] = {d.mr0.ptr, d.mr1.ptr};
+        const size_t steps = d.steps.size();
+        for (size_t k = 1; k <= steps; ++k) {
+            const V *d_in = dd[(k - 1) & 1];
+            const V *r_in = k == 1 ? b : rr[(k - 1) & 1];
+            const V keep = (V)d.steps[k - 1].first, stp = (V)d.steps[k - 1].second;
+            launch_cheb_batch(d, d.mm_depth[lg][0], M, d_in, r_in, keep, stp, dd[k & 1], rr[k & 1], x, k == steps ? sum_into : nullptr, s);
+        }
+        if (steps == 0 && sum_into) {
+            const size_t n = d.n * (size_t)M;
+            hipLaunchKernelGGL((add_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, sum_into, (const V *)x);
+        }
+    }
+
+    // cycle() for M columns.
+    void cycle_batch(size_t l, const V *b, V *x, uint32_t M, hipStream_t s) {
+        LevelDev<S, V> &d = lv_[l];
+        const int lg = columns_log2(M);
+        if (!d.has_p) {
+            if (d.has_factor) {
+                if (d.inverse.ptr)
+                    hipLaunchKernelGGL(coarse_inverse_batch_kernel<V>, dim3(grid_for(d.n, WAVE)), dim3(256), 0, s, (uint32_t)d.n, M,
+                                       d.inverse.ptr, b, x);
+                else if (d.lds_bytes)
+                    hipLaunchKernelGGL(coarse_solve_lds_kernel<V>, dim3(M), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x, M);
+                else
+                    hipLaunchKernelGGL(coarse_solve_kernel<V>, dim3(M), dim3(WAVE), 0, s, (uint32_t)d.n, d.lt.ptr, b, x, M);
+            } else {
+                smooth_batch(d, b, x, nullptr, M, s);
+            }
+            return;
+        }
+        LevelDev<S, V> &c = lv_[l + 1];
+        smooth_batch(d, b, x, nullptr, M, s);
+        launch_spmm<S, V>(d.a, M, x, d.mres.ptr, b, EPI_RESIDUAL, s, d.mm_depth[lg][1]);
+        launch_spmm<S, V>(d.r, M, d.mres.ptr, c.mb.ptr, nullptr, EPI_STORE, s, d.mm_depth[lg][2]);
+        cycle_batch(l + 1, c.mb.ptr, c.mx.ptr, M, s);
+        launch_spmm<S, V>(d.p, M, c.mx.ptr, x, nullptr, EPI_ADD, s, d.mm_depth[lg][3]);
+        launch_spmm<S, V>(d.a, M, x, d.mres.ptr, b, EPI_RESIDUAL, s, d.mm_depth[lg][1]);
+        smooth_batch(d, d.mres.ptr, d.mpx.ptr, x, M, s);
     }
 
     // Picks each level's batch depths by timing its own kernels on its own matrices.
@@ -520,9 +661,9 @@ class Hierarchy final : public DeviceMultigrid {
                 if (d.inverse.ptr)
                     hipLaunchKernelGGL(coarse_inverse_kernel<V>, dim3(grid_for(d.n, 256)), dim3(256), 0, s, (uint32_t)d.n, d.inverse.ptr, b, x);
                 else if (d.lds_bytes)
-                    hipLaunchKernelGGL(coarse_solve_lds_kernel<V>, dim3(1), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x);
+                    hipLaunchKernelGGL(coarse_solve_lds_kernel<V>, dim3(1), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x, 1u);
                 else
-                    hipLaunchKernelGGL(coarse_solve_kernel<V>, dim3(1), dim3(WAVE), 0, s, (uint32_t)d.n, d.lt.ptr, b, x);
+                    hipLaunchKernelGGL(coarse_solve_kernel<V>, dim3(1), dim3(WAVE), 0, s, (uint32_t)d.n, d.lt.ptr, b, x, 1u);
             } else {
                 smooth(d, b, x, nullptr, s);
             }
@@ -584,6 +725,7 @@ class Hierarchy final : public DeviceMultigrid {
 
     std::vector<LevelDev<S, V>> lv_;
     DeviceBuffer<V> in_, out_;
+    DeviceBuffer<V> min_, mout_;  // the finest level's columns in V, for the single precision
 };
 
 }  // namespace

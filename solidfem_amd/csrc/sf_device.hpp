@@ -33,6 +33,7 @@ struct DeviceSliced {
     DeviceBuffer<T> vals;
     int batch_spmv = 4;  // blocks in flight per lane, chosen by tune_batch
     int batch_cheb = 4;
+    mutable int batch_mm[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // batched product, per log2 of the column group
     dev::Sliced<T> view() const { return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices, xcd_order()}; }
     static uint32_t xcd_order();
     size_t dofs() const { return rows * rb; }
@@ -51,9 +52,25 @@ void upload_sliced(DeviceSliced<T> &out, const HostBsr<double> &m, hipStream_t s
 template <typename S, typename V>
 void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch = 0);
 
-// The batch depth (1, 2, 4 or 8 blocks in flight per lane) whose launches `launch(batch)` run
-// fastest; every depth gives the same bits. SOLIDFEM_BATCH forces one.
-int tune_batch(hipStream_t s, const std::function<void(int)> &launch);
+// The batch depth (blocks in flight per lane, one of `candidates`) whose launches
+// `launch(batch)` run fastest; every depth gives the same bits. SOLIDFEM_BATCH forces one.
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates = {1, 2, 4, 8});
+
+// ---- several right-hand sides ------------------------------------------------------------------
+
+// Columns one batched solve carries: a group of lanes per block row, one lane per column.
+constexpr uint32_t MAX_COLUMNS = 64;
+inline int columns_log2(uint32_t m) {
+    int g = 0;
+    while ((1u << g) < m) ++g;
+    return g;
+}
+// Y = A X for M row-interleaved columns (epilogue per dev::Epilogue), `batch` blocks in flight
+// (1, 2 or 4; 0 = the matrix's tuned depth for this column count).
+template <typename S, typename V>
+void launch_spmm(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch = 0);
+// The batch depths worth trying for a matrix's batched product.
+std::initializer_list<int> spmm_candidates(int rb, int cb, size_t value_size);
 // Tunes a matrix's product on scratch vectors of its shape.
 template <typename S, typename V>
 void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s);
@@ -95,6 +112,9 @@ class DeviceMultigrid {
     // One smoother application on `level` from a zero guess (f64 device vectors in and out).
     virtual void smooth_level(size_t level, const double *b, double *x, hipStream_t s) = 0;
     virtual void coarse_solve(const double *b, double *x, hipStream_t s) = 0;
+    // Z = M^-1 R for M row-interleaved f64 columns (entry (i, m) at i * M + m); column m is bit for
+    // bit apply() of column m.
+    virtual void apply_batch(const double *r, double *z, uint32_t columns, hipStream_t s) = 0;
     int precision = SF_PRECISION_SINGLE_STORAGE;
     int coarse_mode = SF_COARSE_REFERENCE;
     size_t degree = 3;
@@ -162,6 +182,60 @@ class DevicePcg {
     DeviceBuffer<dev::PcgScalars> scalars_;
     DeviceBuffer<double> partials_, norm_;
     dev::PcgScalars *pinned_ = nullptr;
+    hipEvent_t copied_ = nullptr;
+};
+
+// Several load cases over one matrix and one preconditioner: every column runs the recurrence of
+// Pcg (crates/solidfem-solve/src/pcg.rs:113-283) on its own scalars, in lockstep with the others
+// so that each matrix pass serves all columns. A column's operations are those of DevicePcg in
+// the same order, so its iterates are bit for bit those of solving it alone; a column that
+// converges, fails or is cancelled is frozen while the rest go on.
+class DevicePcgBatch {
+   public:
+    DevicePcgBatch(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, const double *rhs_columns, uint32_t columns,
+                   const sf_pcg_options &options);
+    ~DevicePcgBatch();
+    // Pcg::new again on the held buffers, optionally with new right-hand sides (column-major).
+    void restart(const double *rhs_columns);
+    // Up to `max` lockstep iterations; cancel is polled before each. Returns the running columns.
+    uint32_t step(u64 max, const Cancel &cancel);
+    void estimate(double *columns_out) const;  // column-major
+    void progress(sf_column_progress *out) const;
+    uint32_t columns() const { return m_; }
+    size_t size() const { return n_; }
+
+   private:
+    struct Column {
+        u64 iterations = 0;
+        double rhs_norm = 0, relative = 0;
+        bool floor = false;
+        PcgStatus status = PcgStatus::Active;
+        Error failure{SF_OK, ""};
+    };
+    void initialise();
+    void iterate();
+    void precondition(const double *r, double *z);
+    void dot(const double *a, const double *b, size_t field, int kind, const uint32_t *mask, int mode = 0);
+    void reset_direction(const uint32_t *mask);
+    void upload_flags(uint32_t *device, const std::vector<uint32_t> &flags);
+    const dev::PcgScalars *read_scalars();
+    std::vector<uint32_t> active_flags() const;
+
+    Context &ctx_;
+    const DeviceSystem &a_;
+    DeviceMultigrid *mg_;
+    size_t n_;
+    uint32_t m_;
+    int dot_order_;
+    double tolerance_;
+    u64 max_iterations_;
+    std::vector<Column> col_;
+    DeviceBuffer<double> b_, x_, r_, z_, p_, ap_, tmp_, tmp2_, jacobi_, stage_;
+    DeviceBuffer<dev::PcgScalars> scalars_;
+    DeviceBuffer<double> partials_;
+    DeviceBuffer<uint32_t> active_, mask_;
+    dev::PcgScalars *pinned_ = nullptr;
+    uint32_t *pinned_flags_ = nullptr;
     hipEvent_t copied_ = nullptr;
 };
 

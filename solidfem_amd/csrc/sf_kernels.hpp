@@ -232,13 +232,16 @@ __global__ void convert_kernel(size_t n, const From *__restrict__ in, To *__rest
 // substitution also subtracts each row's terms in ascending k, which can only start once every
 // later x is known: the lanes form the products, then one lane folds them in order.
 constexpr uint32_t COARSE_MAX = 3072;
+// Several right-hand sides (row-interleaved, `stride` columns) take one workgroup each.
 template <typename V>
 __global__ __launch_bounds__(64) void coarse_solve_kernel(uint32_t n, const double *__restrict__ lt,
-                                                           const V *__restrict__ rhs, V *__restrict__ out) {
+                                                           const V *__restrict__ rhs, V *__restrict__ out, uint32_t stride) {
     __shared__ double xs[COARSE_MAX];
     __shared__ double prod[COARSE_MAX];
     const uint32_t t = threadIdx.x;
-    for (uint32_t i = t; i < n; i += WAVE) xs[i] = (double)rhs[i];
+    rhs += blockIdx.x;
+    out += blockIdx.x;
+    for (uint32_t i = t; i < n; i += WAVE) xs[i] = (double)rhs[(size_t)i * stride];
     __syncthreads();
     for (uint32_t k = 0; k < n; ++k) {
         if (t == k % WAVE) xs[k] = xs[k] / lt[(size_t)k * n + k];
@@ -259,7 +262,7 @@ __global__ __launch_bounds__(64) void coarse_solve_kernel(uint32_t n, const doub
         }
         __syncthreads();
     }
-    for (uint32_t i = t; i < n; i += WAVE) out[i] = (V)xs[i];
+    for (uint32_t i = t; i < n; i += WAVE) out[(size_t)i * stride] = (V)xs[i];
 }
 
 // The same solve with the factor held in LDS, packed by columns (column k holds rows k..n-1 at
@@ -269,13 +272,15 @@ __device__ __forceinline__ size_t packed_column(uint32_t n, uint32_t k) { return
 
 template <typename V>
 __global__ __launch_bounds__(64) void coarse_solve_lds_kernel(uint32_t n, const double *__restrict__ packed,
-                                                               const V *__restrict__ rhs, V *__restrict__ out) {
+                                                               const V *__restrict__ rhs, V *__restrict__ out, uint32_t stride) {
     extern __shared__ double smem[];
     const size_t np = (size_t)n * (n + 1) / 2;
     double *l = smem, *xs = smem + np, *prod = xs + n;
     const uint32_t t = threadIdx.x;
+    rhs += blockIdx.x;
+    out += blockIdx.x;
     for (size_t i = t; i < np; i += WAVE) l[i] = packed[i];
-    for (uint32_t i = t; i < n; i += WAVE) xs[i] = (double)rhs[i];
+    for (uint32_t i = t; i < n; i += WAVE) xs[i] = (double)rhs[(size_t)i * stride];
     __syncthreads();
     for (uint32_t k = 0; k < n; ++k) {
         const double *col = l + packed_column(n, k) - k;  // col[i] = L[i][k]
@@ -304,7 +309,7 @@ __global__ __launch_bounds__(64) void coarse_solve_lds_kernel(uint32_t n, const 
         }
         __syncthreads();
     }
-    for (uint32_t i = t; i < n; i += WAVE) out[i] = (V)xs[i];
+    for (uint32_t i = t; i < n; i += WAVE) out[(size_t)i * stride] = (V)xs[i];
 }
 
 // The coarsest level's solve as a product with the f64 inverse (stored by columns: inv[k n + i]
