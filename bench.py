@@ -70,6 +70,67 @@ def cpu_baseline(workload, gpu_iterations, sample_iterations):
     }
 
 
+def multi_rhs(ctx, sf, np, columns, steps, warmup):
+    """Many load cases over one hierarchy: the 60 x 40 x 40 mm block (30 x 20 x 20 hex x 6 Tet10,
+    E 70 GPa, nu 0.33, clamped z = 0) under `columns` random-traction load cases (N(0, 1 MPa) per
+    component on the z = 40 faces, seed 7, case c), solved as one lockstep batch with the FP32
+    V-cycle, against the same cases solved one at a time on the same hierarchy."""
+    from workloads import block_config
+
+    cases = [block_config(c) for c in range(columns)]
+    pcg_options = sf.PcgOptions(tolerance=1e-8, dot_order=sf.DotOrder.TREE)
+    builder = sf.StudyBuilder.new(ctx, sf.Model.from_workload(cases[0]), pcg_options,
+                                  sf.MultigridOptions(precision=sf.Precision.DOUBLE))
+    builder.step(2 ** 62)
+    study = builder.finish()
+    system = study.system()
+    mg = study.multigrid().with_precision(sf.Precision.SINGLE, coarse_solve=1)
+    rhs = np.stack([sf.Model.from_workload(w).load_vector() for w in cases])
+    log(f"multi-rhs: block with {rhs.shape[1]} free DOFs, {columns} load cases")
+    batch = sf.PcgBatch.new(ctx, system, mg, rhs, pcg_options)
+
+    def run_batch():
+        batch.restart()
+        running, progress = batch.step(2 ** 62)
+        if running or any(p.outcome != sf.StepOutcome.CONVERGED for p in progress):
+            raise RuntimeError("a batched load case did not converge")
+        return progress
+
+    for _ in range(warmup):
+        run_batch()
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        progress = run_batch()
+    ctx.synchronize()
+    batch_ms = (time.perf_counter() - t) / steps * 1e3
+    xb = batch.estimate()
+    singles = [sf.Pcg.new(ctx, system, mg, rhs[m], pcg_options) for m in range(columns)]
+    for p in singles[:2]:
+        p.step(2 ** 62)
+    ctx.synchronize()
+    t = time.perf_counter()
+    iters = []
+    for p in singles:
+        p.restart()
+        iters.append(p.step(2 ** 62).iteration)
+    ctx.synchronize()
+    seq_ms = (time.perf_counter() - t) * 1e3
+    identical = all(np.array_equal(singles[m].estimate().view(np.uint64), xb[m].view(np.uint64)) for m in range(columns))
+    log(f"multi-rhs: batch {batch_ms:.1f} ms for {columns} cases, one at a time {seq_ms:.1f} ms; bit-identical {identical}")
+    return {
+        "workload": f"block 60x40x40 mm, 30x20x20 hex x 6 Tet10, {rhs.shape[1]} free DOFs, {columns} random-traction "
+                    f"load cases (seed 7), FP32 V-cycle in FP64 PCG, rtol 1e-8",
+        "columns": columns,
+        "ms_per_batch": round(batch_ms, 3),
+        "ms_per_load_case": round(batch_ms / columns, 3),
+        "sequential_ms_per_load_case": round(seq_ms / columns, 3),
+        "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
+        "iterations": {"min": int(min(p.iteration for p in progress)), "max": int(max(p.iteration for p in progress))},
+        "columns_bit_identical_to_single_solves": bool(identical),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +139,7 @@ def main():
     ap.add_argument("--divisions", type=str, default="100,100,2")
     ap.add_argument("--cpu-iterations", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--columns", type=int, default=32, help="load cases of the multi-RHS measurement (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -217,6 +279,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.columns:
+        line["multi_rhs"] = multi_rhs(ctx, sf, np, args.columns, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (oracle) on a bounded sample")
         line["cpu_baseline"] = cpu_baseline(workload, int(fp32["iterations"]), args.cpu_iterations)

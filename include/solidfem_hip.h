@@ -432,6 +432,61 @@ sf_status sf_pcg_batch_restart(sf_pcg_batch *batch, const double *rhs);
 uint64_t sf_pcg_batch_columns(const sf_pcg_batch *batch);
 void sf_pcg_batch_destroy(sf_pcg_batch *batch);
 
+/* ---- serving ------------------------------------------------------------------------------
+ * No reference counterpart: the reference solves one study at a time on the caller's thread
+ * (crates/solidfem-elastic/src/study.rs:191-372). A server takes structured box problems
+ * (sf_workload) from any thread and answers each with its displacements. Requests with the same
+ * mesh, constraints and material share one resident system -- the assembled matrix and its
+ * multigrid hierarchy -- kept in HBM under a byte budget (least recently used idle systems are
+ * evicted to admit new ones; a system that cannot fit even alone is refused). A system's requests
+ * are solved as the columns of one lockstep batch (sf_pcg_batch) and new requests take free
+ * columns between slices of iterations, so every answer is bit for bit the answer the request
+ * gets alone. Latency is measured from submission to the answer. */
+enum { SF_SERVE_QUEUED = 0, SF_SERVE_BUILDING = 1, SF_SERVE_SOLVING = 2, SF_SERVE_DONE = 3, SF_SERVE_FAILED = 4 };
+typedef struct sf_server_options {
+    uint64_t hbm_budget;       /* bytes the server may reserve; 0 = 200 GB */
+    int32_t precision;         /* SF_PRECISION_* of the V-cycle */
+    int32_t coarse_solve;      /* SF_COARSE_* */
+    double tolerance;          /* PcgOptions::tolerance, pcg.rs:14-33 */
+    uint32_t max_columns;      /* requests per batch, 1 to 64 */
+    uint32_t slice_iterations; /* lockstep iterations between admissions of new requests */
+    int32_t builders;          /* threads assembling systems and setting up hierarchies */
+    int32_t solvers;           /* threads running batches, one stream each */
+    int32_t dot_order;         /* SF_DOT_* */
+    int32_t reserved;
+} sf_server_options;
+typedef struct sf_serve_status {
+    int32_t state;             /* SF_SERVE_* */
+    sf_status error;           /* for SF_SERVE_FAILED */
+    int32_t cache_hit;         /* the request found its system resident or being built */
+    int32_t at_rounding_floor;
+    uint64_t iterations;
+    double relative_residual;
+    double queued_seconds;     /* waiting for its system or for a column */
+    double build_seconds;      /* building the system this request brought */
+    double solve_seconds;
+    double latency_seconds;    /* submission to answer */
+    uint64_t nodes;
+    uint64_t free_dofs;
+} sf_serve_status;
+typedef struct sf_server_stats {
+    uint64_t submitted, completed, failed, builds, cache_hits, evictions, resident_systems;
+    uint64_t hbm_reserved, hbm_peak;
+    double p50_seconds, p99_seconds, mean_seconds, max_seconds; /* latency of completed requests */
+} sf_server_stats;
+sf_status sf_server_create(int32_t device, const sf_server_options *options, sf_server **out);
+sf_status sf_server_submit(sf_server *server, const sf_workload *workload, uint64_t *id);
+sf_status sf_server_status(sf_server *server, uint64_t id, sf_serve_status *out);
+/* Waits until the request is done or failed, or timeout_seconds pass (negative: no limit). */
+sf_status sf_server_wait(sf_server *server, uint64_t id, double timeout_seconds, sf_serve_status *out);
+/* The displacements of a finished request: 3 per mesh node (len), fixed components zero. */
+sf_status sf_server_result(sf_server *server, uint64_t id, double *displacements, uint64_t len);
+/* Forgets a finished request. */
+sf_status sf_server_release(sf_server *server, uint64_t id);
+sf_status sf_server_stats_get(sf_server *server, sf_server_stats *out);
+/* Stops the threads; requests still open fail. */
+void sf_server_destroy(sf_server *server);
+
 /* Kernel launch timing with HIP events on the launching stream. Kernels: 0 = the fused
  * Chebyshev step on the finest level, 1 = the conjugate gradient's system product A p, 2 = the
  * residual on the finest level. read returns launches, summed seconds and summed algorithmic

@@ -315,6 +315,16 @@ class Model(_Handle):
         _raise(library().sf_model_element_stiffness(ctx._ptr, self._ptr, first, count, _ptr(out, C.c_double)))
         return out
 
+    def load_vector(self) -> np.ndarray:
+        """The right-hand side over the free unknowns of this model's loads (study.rs:444-449,
+        :696-735), without building the matrix pattern."""
+        lib = library()
+        nf = C.c_uint64()
+        _raise(lib.sf_model_free_problem(self._ptr, C.byref(nf), None, None, None, None, None, None))
+        rhs = np.empty(3 * nf.value)
+        _raise(lib.sf_model_free_problem(self._ptr, None, None, None, _ptr(rhs, C.c_double), None, None, None))
+        return rhs
+
     def free_problem(self):
         """(free_map, rhs, block_starts, block_columns, rigid_modes) on the host."""
         lib = library()
@@ -751,6 +761,100 @@ class PcgBatch(_Handle):
         out = np.empty((self.columns, self.n))
         _raise(library().sf_pcg_batch_estimate(self._ptr, _ptr(out, C.c_double)))
         return out
+
+
+class ServeState(enum.IntEnum):
+    QUEUED = 0
+    BUILDING = 1
+    SOLVING = 2
+    DONE = 3
+    FAILED = 4
+
+
+@dataclass
+class ServerOptions:
+    """Serving options (include/solidfem_hip.h: sf_server_options)."""
+    hbm_budget: int = 200 << 30
+    precision: Precision = Precision.SINGLE
+    coarse_solve: int = 1
+    tolerance: float = 1e-8
+    max_columns: int = 32
+    slice_iterations: int = 8
+    builders: int = 2
+    solvers: int = 4
+    dot_order: DotOrder = DotOrder.TREE
+
+    def c(self):
+        return _capi.ServerOptions(self.hbm_budget, int(self.precision), self.coarse_solve, self.tolerance,
+                                   self.max_columns, self.slice_iterations, self.builders, self.solvers,
+                                   int(self.dot_order), 0)
+
+
+@dataclass
+class ServeStatus:
+    state: ServeState
+    error: int
+    cache_hit: bool
+    at_rounding_floor: bool
+    iterations: int
+    relative_residual: float
+    queued_seconds: float
+    build_seconds: float
+    solve_seconds: float
+    latency_seconds: float
+    nodes: int
+    free_dofs: int
+
+    @classmethod
+    def of(cls, s):
+        return cls(ServeState(s.state), s.error, bool(s.cache_hit), bool(s.at_rounding_floor), s.iterations,
+                   s.relative_residual, s.queued_seconds, s.build_seconds, s.solve_seconds, s.latency_seconds,
+                   s.nodes, s.free_dofs)
+
+
+class Server(_Handle):
+    """A solve server on one GPU: submit box problems from any thread, collect displacements.
+    Requests sharing mesh, constraints and material share one resident system and are solved as
+    the columns of one batch; each answer is bit for bit the request's answer alone."""
+    _destroy = "sf_server_destroy"
+
+    @classmethod
+    def new(cls, device: int = 0, options: ServerOptions = None):
+        o = (options or ServerOptions()).c()
+        h = C.c_void_p()
+        _raise(library().sf_server_create(device, C.byref(o), C.byref(h)))
+        return cls(h)
+
+    def submit(self, workload: "Workload") -> int:
+        w = workload.c(_capi.Workload)
+        rid = C.c_uint64()
+        _raise(library().sf_server_submit(self._ptr, C.byref(w), C.byref(rid)))
+        return rid.value
+
+    def status(self, rid: int) -> ServeStatus:
+        s = _capi.ServeStatus()
+        _raise(library().sf_server_status(self._ptr, rid, C.byref(s)))
+        return ServeStatus.of(s)
+
+    def wait(self, rid: int, timeout: float = -1.0) -> ServeStatus:
+        s = _capi.ServeStatus()
+        _raise(library().sf_server_wait(self._ptr, rid, timeout, C.byref(s)))
+        return ServeStatus.of(s)
+
+    def result(self, rid: int) -> np.ndarray:
+        """Displacements, (nodes, 3); raises the request's error if it failed."""
+        s = self.status(rid)
+        out = np.empty((s.nodes, 3))
+        _raise(library().sf_server_result(self._ptr, rid, _ptr(out, C.c_double), out.size))
+        return out
+
+    def release(self, rid: int):
+        _raise(library().sf_server_release(self._ptr, rid))
+
+    def stats(self) -> dict:
+        s = _capi.ServerStats()
+        _raise(library().sf_server_stats_get(self._ptr, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in s._fields_}
 
 
 class Kernel(enum.IntEnum):

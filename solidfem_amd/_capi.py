@@ -27,6 +27,23 @@ class ColumnProgress(C.Structure):
                 ("error", i32), ("reserved", i32), ("error_iteration", u64), ("error_value", f64)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget", u64), ("precision", i32), ("coarse_solve", i32), ("tolerance", f64), ("max_columns", u32),
+                ("slice_iterations", u32), ("builders", i32), ("solvers", i32), ("dot_order", i32), ("reserved", i32)]
+
+
+class ServeStatus(C.Structure):
+    _fields_ = [("state", i32), ("error", i32), ("cache_hit", i32), ("at_rounding_floor", i32), ("iterations", u64),
+                ("relative_residual", f64), ("queued_seconds", f64), ("build_seconds", f64), ("solve_seconds", f64),
+                ("latency_seconds", f64), ("nodes", u64), ("free_dofs", u64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("failed", u64), ("builds", u64), ("cache_hits", u64),
+                ("evictions", u64), ("resident_systems", u64), ("hbm_reserved", u64), ("hbm_peak", u64),
+                ("p50_seconds", f64), ("p99_seconds", f64), ("mean_seconds", f64), ("max_seconds", f64)]
+
+
 class PcgOptions(C.Structure):
     _fields_ = [("tolerance", f64), ("max_iterations", u64), ("dot_order", i32), ("reserved", i32)]
 
@@ -192,6 +209,14 @@ SIGNATURES = {
     "sf_pcg_batch_restart": (i32, [vp, P(f64)]),
     "sf_pcg_batch_columns": (u64, [vp]),
     "sf_pcg_batch_destroy": (None, [vp]),
+    "sf_server_create": (i32, [i32, P(ServerOptions), P(vp)]),
+    "sf_server_submit": (i32, [vp, P(Workload), P(u64)]),
+    "sf_server_status": (i32, [vp, u64, P(ServeStatus)]),
+    "sf_server_wait": (i32, [vp, u64, f64, P(ServeStatus)]),
+    "sf_server_result": (i32, [vp, u64, P(f64), u64]),
+    "sf_server_release": (i32, [vp, u64]),
+    "sf_server_stats_get": (i32, [vp, P(ServerStats)]),
+    "sf_server_destroy": (None, [vp]),
 }
 
 _lib = None

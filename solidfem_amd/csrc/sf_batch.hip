@@ -334,6 +334,72 @@ void DevicePcgBatch::estimate(double *out) const {
     SF_HIP(hipStreamSynchronize(ctx_.stream));
 }
 
+void DevicePcgBatch::estimate_column(uint32_t column, double *out) const {
+    if (column >= m_) throw Error(SF_ERR_INVALID_ARGUMENT, "no such column");
+    hipLaunchKernelGGL((get_column_kernel<double>), dim3(grid_for(n_, 256)), dim3(256), 0, ctx_.stream, n_, m_, column,
+                       (const double *)x_.ptr, stage_.ptr);
+    SF_HIP(hipGetLastError());
+    stage_.download(out, n_, ctx_.stream);
+    SF_HIP(hipStreamSynchronize(ctx_.stream));
+}
+
+void DevicePcgBatch::replace_columns(const std::vector<uint32_t> &columns, const std::vector<const double *> &rhs) {
+    // restart() for the listed columns only. Every other running column has z = M^-1 r from its
+    // last iteration, so the preconditioner application of reset_direction recomputes the same
+    // bits there, and p and r.z change only in the listed columns.
+    hipStream_t s = ctx_.stream;
+    SF_HIP(hipStreamSynchronize(s));
+    if (columns.size() != rhs.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "one right-hand side per column");
+    if (columns.empty()) return;
+    std::vector<uint32_t> listed(m_, 0);
+    for (size_t k = 0; k < columns.size(); ++k) {
+        const uint32_t m = columns[k];
+        if (m >= m_ || listed[m]) throw Error(SF_ERR_INVALID_ARGUMENT, "no such column, or a column listed twice");
+        listed[m] = 1;
+        SF_HIP(hipMemcpyAsync(stage_.ptr, rhs[k], n_ * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((set_column_kernel<double>), dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, m_, m,
+                           (const double *)stage_.ptr, b_.ptr);
+        SF_HIP(hipStreamSynchronize(s));  // the staging is reused for the next column
+    }
+    const size_t total = n_ * m_;
+    const unsigned g = grid_for(total, 256);
+    upload_flags(mask_.ptr, listed);
+    hipLaunchKernelGGL((copy_columns_kernel<double>), dim3(g), dim3(256), 0, s, n_, m_, (const uint32_t *)mask_.ptr,
+                       (const double *)b_.ptr, r_.ptr);
+    for (double *v : {x_.ptr, z_.ptr, p_.ptr})
+        hipLaunchKernelGGL((zero_columns_kernel<double>), dim3(g), dim3(256), 0, s, n_, m_, (const uint32_t *)mask_.ptr, v);
+    hipLaunchKernelGGL(zero_scalars_kernel, dim3(1), dim3(WAVE), 0, s, m_, (const uint32_t *)mask_.ptr, scalars_.ptr);
+    dot(b_.ptr, b_.ptr, offsetof(PcgScalars, rr), FINAL_STORE, mask_.ptr);
+    const PcgScalars *h = read_scalars();
+    std::vector<uint32_t> live(m_, 0);
+    bool any = false;
+    for (uint32_t m = 0; m < m_; ++m) {
+        if (!listed[m]) continue;
+        Column &c = col_[m];
+        c = Column{};
+        c.rhs_norm = std::sqrt(h[m].rr);
+        if (c.rhs_norm == 0.0) {
+            c.status = PcgStatus::Converged;
+        } else {
+            c.relative = 1.0;
+            live[m] = 1;
+            any = true;
+        }
+    }
+    upload_flags(active_.ptr, active_flags());
+    if (any) {
+        upload_flags(mask_.ptr, live);
+        reset_direction(mask_.ptr);
+    }
+    SF_HIP(hipStreamSynchronize(s));
+}
+
+size_t DevicePcgBatch::device_bytes() const {
+    size_t bytes = 0;
+    for (const auto *buf : {&b_, &x_, &r_, &z_, &p_, &ap_, &tmp_, &tmp2_, &jacobi_, &stage_, &partials_}) bytes += buf->bytes();
+    return bytes + scalars_.bytes() + active_.bytes() + mask_.bytes();
+}
+
 void DevicePcgBatch::progress(sf_column_progress *out) const {
     for (uint32_t m = 0; m < m_; ++m) {
         const Column &c = col_[m];

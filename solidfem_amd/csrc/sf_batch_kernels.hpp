@@ -339,6 +339,31 @@ __global__ void copy_columns_kernel(size_t n, uint32_t M, const uint32_t *__rest
     if (e < n * M && mask[e % M]) dst[e] = src[e];
 }
 
+// Column m of a row-interleaved array from a plain vector.
+template <typename T>
+__global__ void set_column_kernel(size_t n, uint32_t M, uint32_t m, const T *__restrict__ src, T *__restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i * M + m] = src[i];
+}
+template <typename T>
+__global__ void get_column_kernel(size_t n, uint32_t M, uint32_t m, const T *__restrict__ src, T *__restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i * M + m];
+}
+
+// Zeroes the masked columns of a row-interleaved array.
+template <typename T>
+__global__ void zero_columns_kernel(size_t n, uint32_t M, const uint32_t *__restrict__ mask, T *__restrict__ dst) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n * M && mask[e % M]) dst[e] = (T)0;
+}
+
+// Zeroes the scalars of the masked columns.
+static __global__ void zero_scalars_kernel(uint32_t M, const uint32_t *__restrict__ mask, PcgScalars *s) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < M && mask[m]) s[m] = PcgScalars{};
+}
+
 // Column-major (column m at m * n) to row-interleaved, and back.
 template <typename T>
 __global__ void interleave_kernel(size_t n, uint32_t M, const T *__restrict__ cm, T *__restrict__ il) {

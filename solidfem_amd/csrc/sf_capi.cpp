@@ -7,7 +7,8 @@
 #include <memory>
 
 #include "sf_batch_kernels.hpp"
-#include "sf_device.hpp"
+#include "sf_handles.hpp"
+#include "sf_serve.hpp"
 
 using namespace sf;
 
@@ -110,68 +111,6 @@ void to_scalar_csr(const HostBsr<double> &m, int64_t *row_starts, u32 *columns, 
 }
 
 }  // namespace
-
-// ---- handle types -------------------------------------------------------------------------------
-
-struct sf_context {
-    std::unique_ptr<Context> ctx;
-};
-struct sf_tet_mesh {
-    TetMesh mesh;
-};
-struct sf_quad_mesh {
-    QuadMesh mesh;
-};
-struct sf_model {
-    Model model;
-    sf_quad_mesh view;  // the mesh as a handle
-};
-struct sf_csr {
-    sf_context *ctx = nullptr;
-    DeviceSystem system;
-};
-struct sf_multigrid {
-    sf_context *ctx = nullptr;
-    std::unique_ptr<DeviceMultigrid> mg;
-};
-struct sf_multigrid_setup {
-    sf_context *ctx = nullptr;
-    std::unique_ptr<MultigridSetup> setup;
-};
-struct sf_pcg {
-    std::unique_ptr<DevicePcg> pcg;
-};
-
-struct sf_pcg_batch {
-    std::unique_ptr<DevicePcgBatch> batch;
-};
-
-struct sf_static_study {
-    sf_context *ctx = nullptr;
-    Model model;
-    FreeProblem problem;
-    sf_csr system;
-    sf_multigrid mg;
-    std::unique_ptr<sf_pcg> pcg;
-    sf_report report{};
-};
-
-struct sf_study_builder {
-    sf_context *ctx = nullptr;
-    Model model;
-    FreeProblem problem;
-    AssemblyPattern pattern;
-    DeviceAssembly assembly;
-    std::unique_ptr<sf_static_study> study;  // the system being assembled
-    sf_pcg_options pcg{};
-    bool use_multigrid = false;
-    SetupOptions mg_options;
-    size_t next = 0;
-    bool prescribed_applied = false;
-    std::unique_ptr<MultigridSetup> setup;
-    bool setup_finished = false;
-    double assembly_seconds = 0, preconditioner_seconds = 0;
-};
 
 namespace {
 
@@ -814,6 +753,43 @@ sf_status sf_pcg_progress(const sf_pcg *pcg, sf_progress *out) {
 int32_t sf_pcg_is_converged(const sf_pcg *pcg) { return pcg->pcg->converged() ? 1 : 0; }
 uint64_t sf_pcg_size(const sf_pcg *pcg) { return pcg->pcg->size(); }
 void sf_pcg_destroy(sf_pcg *pcg) { delete pcg; }
+
+struct sf_server {
+    std::unique_ptr<Server> server;
+};
+
+sf_status sf_server_create(int32_t device, const sf_server_options *options, sf_server **out) {
+    return guard([&] {
+        sf_server_options o{};
+        if (options) o = *options;
+        else o.precision = SF_PRECISION_SINGLE, o.coarse_solve = SF_COARSE_INVERSE;
+        auto s = std::make_unique<sf_server>();
+        s->server = std::make_unique<Server>(device, o);
+        *out = s.release();
+    });
+}
+sf_status sf_server_submit(sf_server *server, const sf_workload *workload, uint64_t *id) {
+    return guard([&] { *id = server->server->submit(*workload); });
+}
+sf_status sf_server_status(sf_server *server, uint64_t id, sf_serve_status *out) {
+    return guard([&] { *out = server->server->status(id); });
+}
+sf_status sf_server_wait(sf_server *server, uint64_t id, double timeout_seconds, sf_serve_status *out) {
+    return guard([&] {
+        const sf_serve_status s = server->server->wait(id, timeout_seconds);
+        if (out) *out = s;
+    });
+}
+sf_status sf_server_result(sf_server *server, uint64_t id, double *displacements, uint64_t len) {
+    return guard([&] { server->server->result(id, displacements, (size_t)len); });
+}
+sf_status sf_server_release(sf_server *server, uint64_t id) {
+    return guard([&] { server->server->release(id); });
+}
+sf_status sf_server_stats_get(sf_server *server, sf_server_stats *out) {
+    return guard([&] { *out = server->server->stats(); });
+}
+void sf_server_destroy(sf_server *server) { delete server; }
 
 sf_status sf_multigrid_apply_batch(sf_multigrid *mg, const double *r, uint64_t columns, double *z) {
     return guard([&] {

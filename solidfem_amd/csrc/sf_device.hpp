@@ -200,7 +200,12 @@ class DevicePcgBatch {
     // Up to `max` lockstep iterations; cancel is polled before each. Returns the running columns.
     uint32_t step(u64 max, const Cancel &cancel);
     void estimate(double *columns_out) const;  // column-major
+    void estimate_column(uint32_t column, double *out) const;
+    // Pcg::new for the listed columns with new right-hand sides (host, n each), between steps;
+    // the other columns go on unchanged. A column of zeros converges at once and stays idle.
+    void replace_columns(const std::vector<uint32_t> &columns, const std::vector<const double *> &rhs);
     void progress(sf_column_progress *out) const;
+    size_t device_bytes() const;
     uint32_t columns() const { return m_; }
     size_t size() const { return n_; }
 

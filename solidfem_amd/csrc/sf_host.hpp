@@ -82,6 +82,11 @@ struct Model {
 
 Model model_from_workload(const sf_workload &w);
 
+using Traction = std::pair<Face, std::array<double, 3>>;
+// The tractions a workload puts on the boundary faces of its load plane; `boundary` lists the
+// mesh's boundary faces in QuadMesh::boundary_faces order (random tractions are drawn by it).
+std::vector<Traction> workload_tractions(const QuadMesh &mesh, const std::vector<Face> &boundary, const sf_workload &w);
+
 // The validated, numbered problem of StudyBuilder::new (crates/solidfem-elastic/src/study.rs:419-474).
 struct FreeProblem {
     std::vector<u32> free;  // node -> free index or FIXED
@@ -90,6 +95,10 @@ struct FreeProblem {
     std::vector<std::pair<size_t, double>> prescribed;  // (free dof, value), sorted
 };
 FreeProblem number_and_load(const Model &model);  // throws Error(SF_ERR_MODEL_*)
+// The right-hand side over the free unknowns of tractions alone: number_and_load's load for a
+// model with no nodal forces and no prescribed displacements, the same bits.
+std::vector<double> traction_rhs(const QuadMesh &mesh, const std::vector<Traction> &tractions, const std::vector<u32> &free,
+                                 size_t free_count);
 
 // Six rigid-body modes per free unknown (crates/solidfem-elastic/src/rigid_modes.rs:36-83).
 std::vector<double> rigid_body_modes(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count);

@@ -229,6 +229,37 @@ static double gaussian(u64 seed, u64 counter) {
     return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
 }
 
+std::vector<Traction> workload_tractions(const QuadMesh &mesh, const std::vector<Face> &boundary, const sf_workload &w) {
+    const auto &nodes = mesh.nodes;
+    std::vector<Traction> out;
+    int u = (w.load_axis + 1) % 3, v = (w.load_axis + 2) % 3;
+    u64 index = 0;
+    for (const Face &f : boundary) {
+        bool on = true;
+        for (int c = 0; c < 3; ++c)
+            if (!(std::fabs(nodes[f.nodes[c]][w.load_axis] - w.load_value) < 1e-9)) on = false;
+        if (!on) continue;
+        std::array<double, 3> t{0, 0, 0};
+        if (w.traction_kind == SF_TRACTION_UNIFORM) {
+            t = {w.traction[0], w.traction[1], w.traction[2]};
+        } else if (w.traction_kind == SF_TRACTION_RANDOM) {
+            for (int a = 0; a < 3; ++a)
+                t[a] = w.traction_scale * gaussian(w.traction_seed, ((u64)(uint32_t)w.load_case << 40) ^ (index << 2) ^ (u64)a);
+        } else {
+            const Vec3 &p = nodes[f.nodes[0]], &q = nodes[f.nodes[1]], &r = nodes[f.nodes[2]];
+            double cu = (p[u] + q[u] + r[u]) / 3.0 - w.axis_u;
+            double cv = (p[v] + q[v] + r[v]) / 3.0 - w.axis_v;
+            double radius = std::sqrt(cu * cu + cv * cv);
+            t[w.load_axis] = 0.0;
+            t[u] = -w.traction_scale * cv / radius;
+            t[v] = w.traction_scale * cu / radius;
+        }
+        out.push_back({f, t});
+        ++index;
+    }
+    return out;
+}
+
 Model model_from_workload(const sf_workload &w) {
     TetMesh linear = make_box_mesh(w.origin, w.size, w.divisions);
     if (w.jitter_fraction > 0.0) {
@@ -253,31 +284,7 @@ Model model_from_workload(const sf_workload &w) {
     const auto &nodes = model.mesh.nodes;
     for (size_t n = 0; n < nodes.size(); ++n)
         if (std::fabs(nodes[n][w.clamp_axis] - w.clamp_value) < 1e-9) model.fixed.insert((u32)n);
-    int u = (w.load_axis + 1) % 3, v = (w.load_axis + 2) % 3;
-    u64 index = 0;
-    for (const Face &f : model.mesh.boundary_faces()) {
-        bool on = true;
-        for (int c = 0; c < 3; ++c)
-            if (!(std::fabs(nodes[f.nodes[c]][w.load_axis] - w.load_value) < 1e-9)) on = false;
-        if (!on) continue;
-        std::array<double, 3> t{0, 0, 0};
-        if (w.traction_kind == SF_TRACTION_UNIFORM) {
-            t = {w.traction[0], w.traction[1], w.traction[2]};
-        } else if (w.traction_kind == SF_TRACTION_RANDOM) {
-            for (int a = 0; a < 3; ++a)
-                t[a] = w.traction_scale * gaussian(w.traction_seed, ((u64)(uint32_t)w.load_case << 40) ^ (index << 2) ^ (u64)a);
-        } else {
-            const Vec3 &p = nodes[f.nodes[0]], &q = nodes[f.nodes[1]], &r = nodes[f.nodes[2]];
-            double cu = (p[u] + q[u] + r[u]) / 3.0 - w.axis_u;
-            double cv = (p[v] + q[v] + r[v]) / 3.0 - w.axis_v;
-            double radius = std::sqrt(cu * cu + cv * cv);
-            t[w.load_axis] = 0.0;
-            t[u] = -w.traction_scale * cv / radius;
-            t[v] = w.traction_scale * cu / radius;
-        }
-        model.tractions.push_back({f, t});
-        ++index;
-    }
+    model.tractions = workload_tractions(model.mesh, model.mesh.boundary_faces(), w);
     model.field.kind = w.material_kind;
     model.field.e = w.youngs;
     model.field.nu = w.poisson;
@@ -373,6 +380,23 @@ FreeProblem number_and_load(const Model &model) {
     for (auto &pr : p.prescribed) any = any || pr.second != 0.0;
     if (!any) throw Error(SF_ERR_MODEL_NO_EFFECTIVE_LOAD, "the model has no effective load");
     return p;
+}
+
+std::vector<double> traction_rhs(const QuadMesh &mesh, const std::vector<Traction> &tractions, const std::vector<u32> &free,
+                                 size_t free_count) {
+    // the traction part of number_and_load, in the same order and arithmetic
+    std::vector<double> load(3 * mesh.nodes.size(), 0.0);
+    for (const auto &t : tractions) {
+        for (double c : t.second)
+            if (!std::isfinite(c)) throw Error(SF_ERR_MODEL_NON_FINITE_LOAD, "a traction is not finite");
+        face_load(mesh, t.first, t.second, load);
+    }
+    std::vector<double> rhs;
+    rhs.reserve(3 * free_count);
+    for (size_t n = 0; n < free.size(); ++n)
+        if (free[n] != FIXED)
+            for (int a = 0; a < 3; ++a) rhs.push_back(load[3 * n + a]);
+    return rhs;
 }
 
 std::vector<double> rigid_body_modes(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count) {
