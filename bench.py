@@ -131,6 +131,89 @@ def multi_rhs(ctx, sf, np, columns, steps, warmup):
     }
 
 
+def serving_requests(sf, np, count, rate, seed):
+    """Synthetic serving traffic: `count` requests with Poisson arrivals at `rate` per second. A
+    request is a bar (10:1:1) or a plate (1:1:0.02) of Tet10 with a log-uniform size between 5k
+    and 600k DOFs, E ~ U[1, 210] GPa, nu ~ U[0.25, 0.4], clamped on a random face and loaded on
+    another by random tractions N(0, 1 MPa) per component and face; 30% of requests reuse the
+    mesh, material and clamp of an earlier request (its cached hierarchy) with new tractions."""
+    rng = np.random.default_rng(seed)
+    out, systems = [], []
+    t = 0.0
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        if systems and rng.random() < 0.3:
+            base = systems[rng.integers(len(systems))]
+        else:
+            dofs = float(np.exp(rng.uniform(np.log(5e3), np.log(6e5))))
+            if rng.random() < 0.5:  # bar: 3 (20k + 1)(2k + 1)^2 DOFs
+                k = max(1, int(round((dofs / 240.0) ** (1.0 / 3.0))))
+                while k > 1 and 3 * (20 * k + 1) * (2 * k + 1) ** 2 > 6e5:
+                    k -= 1
+                size, div = (10.0 * k, 1.0 * k, 1.0 * k), (10 * k, k, k)
+            else:  # plate: 3 (2k + 1)^2 5 DOFs
+                k = max(2, int(round(np.sqrt(dofs / 60.0))))
+                while k > 2 and 15 * (2 * k + 1) ** 2 > 6e5:
+                    k -= 1
+                size, div = (1.0 * k, 1.0 * k, 0.4), (k, k, 2)
+            clamp = int(rng.integers(6))
+            base = dict(size=size, divisions=div, clamp_axis=clamp // 2,
+                        clamp_value=0.0 if clamp % 2 == 0 else size[clamp // 2],
+                        youngs=float(rng.uniform(1e3, 2.1e5)), poisson=float(rng.uniform(0.25, 0.4)))
+            systems.append(base)
+        faces = [f for f in range(6) if f != 2 * base["clamp_axis"] + (0 if base["clamp_value"] == 0.0 else 1)]
+        load = faces[rng.integers(len(faces))]
+        axis = load // 2
+        w = sf.Workload(**base, load_axis=axis, load_value=0.0 if load % 2 == 0 else base["size"][axis],
+                        traction_kind=1, traction_scale=1.0, traction_seed=seed, load_case=i)
+        out.append((t, w))
+    return out
+
+
+def serving(sf, np, count, rate, seed):
+    """Serves the synthetic traffic on one GPU and reports latency percentiles."""
+    traffic = serving_requests(sf, np, count, rate, seed)
+    opts = sf.ServerOptions(hbm_budget=200 << 30, max_columns=32, slice_iterations=8)
+    server = sf.Server.new(0, opts)
+    ids = []
+    t0 = time.perf_counter()
+    for at, w in traffic:
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(server.submit(w))
+        if len(ids) % 64 == 0:
+            log(f"serving: {len(ids)} submitted, {server.stats()['completed']} answered")
+    states = [server.wait(i, 600.0) for i in ids]
+    wall = time.perf_counter() - t0
+    st = server.stats()
+    done = [s for s in states if s.state == sf.ServeState.DONE]
+    dofs = [s.free_dofs for s in done]
+    server.close()
+    log(f"serving: {st['completed']} answered, {st['failed']} failed in {wall:.1f} s; p50 {st['p50_seconds'] * 1e3:.0f} ms, "
+        f"p99 {st['p99_seconds'] * 1e3:.0f} ms; {st['builds']} builds, {st['cache_hits']} cache hits, "
+        f"{st['evictions']} evictions")
+    return {
+        "workload": f"{count} requests, Poisson arrivals at {rate}/s (seed {seed}); bars and plates of Tet10, "
+                    f"log-uniform 5k-600k DOFs, E U[1,210] GPa, nu U[0.25,0.4], random clamp and loaded faces, "
+                    f"random tractions N(0,1 MPa); 30% reuse an earlier system; HBM budget 200 GB; FP32 V-cycle, rtol 1e-8",
+        "arrival_rate_per_s": rate,
+        "p50_ms": round(st["p50_seconds"] * 1e3, 1),
+        "p99_ms": round(st["p99_seconds"] * 1e3, 1),
+        "mean_ms": round(st["mean_seconds"] * 1e3, 1),
+        "max_ms": round(st["max_seconds"] * 1e3, 1),
+        "completed": int(st["completed"]),
+        "failed": int(st["failed"]),
+        "builds": int(st["builds"]),
+        "cache_hits": int(st["cache_hits"]),
+        "evictions": int(st["evictions"]),
+        "hbm_peak_reserved_gb": round(st["hbm_peak"] / 1e9, 2),
+        "wall_s": round(wall, 1),
+        "served_dofs_per_s": round(sum(dofs) / wall, 1),
+        "median_request_dofs": int(np.median(dofs)) if dofs else 0,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +223,8 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--columns", type=int, default=32, help="load cases of the multi-RHS measurement (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=512, help="requests of the serving measurement (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate of the serving measurement, per s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -281,6 +366,9 @@ def main():
     }
     if args.columns:
         line["multi_rhs"] = multi_rhs(ctx, sf, np, args.columns, args.steps, args.warmup)
+    if args.serve_requests:
+        line["serving"] = serving(sf, np, args.serve_requests, args.serve_rate, 20261202)
+        line["served_p99_ms"] = line["serving"]["p99_ms"]
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (oracle) on a bounded sample")
         line["cpu_baseline"] = cpu_baseline(workload, int(fp32["iterations"]), args.cpu_iterations)

@@ -75,12 +75,12 @@ template void launch_spmm<float, float>(const DeviceSliced<float> &, uint32_t, c
 
 // ---- the lockstep conjugate gradient ---------------------------------------------------------
 
-DevicePcgBatch::DevicePcgBatch(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, const double *rhs, uint32_t columns,
+DevicePcgBatch::DevicePcgBatch(std::shared_ptr<Context> ctx, const DeviceSystem &a, DeviceMultigrid *mg, const double *rhs, uint32_t columns,
                                const sf_pcg_options &options)
-    : ctx_(ctx), a_(a), mg_(mg), n_(a.size), m_(columns), dot_order_(options.dot_order), tolerance_(options.tolerance) {
+    : ctx_(std::move(ctx)), a_(a), mg_(mg), n_(a.size), m_(columns), dot_order_(options.dot_order), tolerance_(options.tolerance) {
     if (columns == 0 || columns > MAX_COLUMNS) throw Error(SF_ERR_INVALID_ARGUMENT, "a batch carries 1 to 64 right-hand sides");
     if (mg_ && mg_->dofs() != n_) throw Error(SF_ERR_INVALID_ARGUMENT, "the multigrid does not match the matrix");
-    hipStream_t s = ctx_.stream;
+    hipStream_t s = ctx_->stream;
     max_iterations_ = options.max_iterations ? options.max_iterations : 10 * (u64)n_ + 1000;
     const size_t total = std::max<size_t>(n_ * m_, 1);
     for (auto *buf : {&b_, &x_, &r_, &z_, &p_, &ap_, &tmp_, &tmp2_, &stage_}) buf->resize(total);
@@ -119,7 +119,7 @@ DevicePcgBatch::DevicePcgBatch(Context &ctx, const DeviceSystem &a, DeviceMultig
 }
 
 DevicePcgBatch::~DevicePcgBatch() {
-    (void)hipStreamSynchronize(ctx_.stream);
+    (void)hipStreamSynchronize(ctx_->stream);
     if (pinned_) (void)hipHostFree(pinned_);
     if (pinned_flags_) (void)hipHostFree(pinned_flags_);
     if (copied_) (void)hipEventDestroy(copied_);
@@ -127,10 +127,10 @@ DevicePcgBatch::~DevicePcgBatch() {
 
 void DevicePcgBatch::upload_flags(uint32_t *device, const std::vector<uint32_t> &flags) {
     // the pinned staging is reused: nothing may still read it
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
     std::copy(flags.begin(), flags.end(), pinned_flags_);
-    SF_HIP(hipMemcpyAsync(device, pinned_flags_, sizeof(uint32_t) * m_, hipMemcpyHostToDevice, ctx_.stream));
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    SF_HIP(hipMemcpyAsync(device, pinned_flags_, sizeof(uint32_t) * m_, hipMemcpyHostToDevice, ctx_->stream));
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
 }
 
 std::vector<uint32_t> DevicePcgBatch::active_flags() const {
@@ -140,13 +140,13 @@ std::vector<uint32_t> DevicePcgBatch::active_flags() const {
 }
 
 const PcgScalars *DevicePcgBatch::read_scalars() {
-    SF_HIP(hipMemcpyAsync(pinned_, scalars_.ptr, sizeof(PcgScalars) * m_, hipMemcpyDeviceToHost, ctx_.stream));
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    SF_HIP(hipMemcpyAsync(pinned_, scalars_.ptr, sizeof(PcgScalars) * m_, hipMemcpyDeviceToHost, ctx_->stream));
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
     return pinned_;
 }
 
 void DevicePcgBatch::dot(const double *a, const double *b, size_t field, int kind, const uint32_t *mask, int mode) {
-    hipStream_t s = ctx_.stream;
+    hipStream_t s = ctx_->stream;
     if (dot_order_ == SF_DOT_REFERENCE) {
         if (mode == 0)
             hipLaunchKernelGGL((dot_serial_batch_kernel<0>), dim3(1), dim3(WAVE), 0, s, n_, m_, a, b, kind, (uint32_t)field, scalars_.ptr, mask);
@@ -166,10 +166,10 @@ void DevicePcgBatch::dot(const double *a, const double *b, size_t field, int kin
 
 void DevicePcgBatch::precondition(const double *r, double *z) {
     if (mg_) {
-        mg_->apply_batch(r, z, m_, ctx_.stream);
+        mg_->apply_batch(r, z, m_, ctx_->stream);
     } else {
         const size_t total = n_ * m_;
-        hipLaunchKernelGGL(jacobi_batch_kernel, dim3(grid_for(total, 256)), dim3(256), 0, ctx_.stream, n_, m_, jacobi_.ptr, r, z);
+        hipLaunchKernelGGL(jacobi_batch_kernel, dim3(grid_for(total, 256)), dim3(256), 0, ctx_->stream, n_, m_, jacobi_.ptr, r, z);
     }
     SF_HIP(hipGetLastError());
 }
@@ -179,14 +179,14 @@ void DevicePcgBatch::reset_direction(const uint32_t *mask) {
     // own r, the same bits where it was already current), p = z, rz = r . z
     precondition(r_.ptr, z_.ptr);
     const size_t total = n_ * m_;
-    hipLaunchKernelGGL((copy_columns_kernel<double>), dim3(grid_for(total, 256)), dim3(256), 0, ctx_.stream, n_, m_, mask,
+    hipLaunchKernelGGL((copy_columns_kernel<double>), dim3(grid_for(total, 256)), dim3(256), 0, ctx_->stream, n_, m_, mask,
                        (const double *)z_.ptr, p_.ptr);
     dot(r_.ptr, z_.ptr, offsetof(PcgScalars, rz), FINAL_STORE, mask);
 }
 
 void DevicePcgBatch::restart(const double *rhs) {
     // pcg.rs:135-164 per column: r = b, x = 0, then the first direction
-    hipStream_t s = ctx_.stream;
+    hipStream_t s = ctx_->stream;
     SF_HIP(hipStreamSynchronize(s));
     const size_t total = n_ * m_;
     if (rhs) {
@@ -220,7 +220,7 @@ void DevicePcgBatch::restart(const double *rhs) {
 
 void DevicePcgBatch::iterate() {
     // DevicePcg::iterate for every running column; a column's branch is its own.
-    hipStream_t s = ctx_.stream;
+    hipStream_t s = ctx_->stream;
     const size_t total = n_ * m_;
     const unsigned g = grid_for(total, 256);
     launch_spmm<double, double>(a_.sliced, m_, p_.ptr, ap_.ptr, nullptr, EPI_STORE, s);
@@ -313,7 +313,7 @@ uint32_t DevicePcgBatch::step(u64 max, const Cancel &cancel) {
     };
     for (u64 i = 0; i < max && running() > 0; ++i) {
         if (cancel()) {
-            SF_HIP(hipStreamSynchronize(ctx_.stream));
+            SF_HIP(hipStreamSynchronize(ctx_->stream));
             for (Column &c : col_)
                 if (c.status == PcgStatus::Active) c.status = PcgStatus::Cancelled;
             upload_flags(active_.ptr, active_flags());
@@ -321,33 +321,33 @@ uint32_t DevicePcgBatch::step(u64 max, const Cancel &cancel) {
         }
         iterate();
     }
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
     return running();
 }
 
 void DevicePcgBatch::estimate(double *out) const {
     const size_t total = n_ * m_;
-    hipLaunchKernelGGL((deinterleave_kernel<double>), dim3(grid_for(total, 256)), dim3(256), 0, ctx_.stream, n_, m_,
+    hipLaunchKernelGGL((deinterleave_kernel<double>), dim3(grid_for(total, 256)), dim3(256), 0, ctx_->stream, n_, m_,
                        (const double *)x_.ptr, stage_.ptr);
     SF_HIP(hipGetLastError());
-    stage_.download(out, total, ctx_.stream);
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    stage_.download(out, total, ctx_->stream);
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
 }
 
 void DevicePcgBatch::estimate_column(uint32_t column, double *out) const {
     if (column >= m_) throw Error(SF_ERR_INVALID_ARGUMENT, "no such column");
-    hipLaunchKernelGGL((get_column_kernel<double>), dim3(grid_for(n_, 256)), dim3(256), 0, ctx_.stream, n_, m_, column,
+    hipLaunchKernelGGL((get_column_kernel<double>), dim3(grid_for(n_, 256)), dim3(256), 0, ctx_->stream, n_, m_, column,
                        (const double *)x_.ptr, stage_.ptr);
     SF_HIP(hipGetLastError());
-    stage_.download(out, n_, ctx_.stream);
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    stage_.download(out, n_, ctx_->stream);
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
 }
 
 void DevicePcgBatch::replace_columns(const std::vector<uint32_t> &columns, const std::vector<const double *> &rhs) {
     // restart() for the listed columns only. Every other running column has z = M^-1 r from its
     // last iteration, so the preconditioner application of reset_direction recomputes the same
     // bits there, and p and r.z change only in the listed columns.
-    hipStream_t s = ctx_.stream;
+    hipStream_t s = ctx_->stream;
     SF_HIP(hipStreamSynchronize(s));
     if (columns.size() != rhs.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "one right-hand side per column");
     if (columns.empty()) return;

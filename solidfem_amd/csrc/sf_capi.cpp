@@ -165,7 +165,7 @@ sf_status sf_device_count(int32_t *count) {
 sf_status sf_context_create(int32_t device, sf_context **out) {
     return guard([&] {
         auto c = std::make_unique<sf_context>();
-        c->ctx = std::make_unique<Context>(device);
+        c->ctx = std::make_shared<Context>(device);
         *out = c.release();
     });
 }
@@ -322,7 +322,7 @@ sf_status sf_study_builder_create(sf_context *ctx, const sf_model *model, const 
     return guard([&] {
         const double t0 = seconds();
         auto b = std::make_unique<sf_study_builder>();
-        b->ctx = ctx;
+        b->ctx = ctx->ctx;
         b->model = model->model;
         b->pcg = pcg ? *pcg : sf_pcg_options{1e-9, 0, SF_DOT_TREE, 0};
         b->use_multigrid = use_multigrid != 0;
@@ -331,9 +331,9 @@ sf_status sf_study_builder_create(sf_context *ctx, const sf_model *model, const 
         b->problem = number_and_load(b->model);
         b->pattern = build_pattern(b->model.mesh, b->problem.free, b->problem.free_count);
         b->study = std::make_unique<sf_static_study>();
-        b->study->ctx = ctx;
+        b->study->ctx = ctx->ctx;
         auto &sys = b->study->system.system;
-        b->study->system.ctx = ctx;
+        b->study->system.ctx = ctx->ctx;
         sys.size = 3 * b->problem.free_count;
         sys.rb = 3;
         sys.host = b->pattern.pattern;
@@ -350,7 +350,7 @@ sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_c
                                 sf_build_progress *progress) {
     return guard([&] {
         const Cancel cancel{cancel_fn, user};
-        hipStream_t s = b->ctx->ctx->stream;
+        hipStream_t s = b->ctx->stream;
         const size_t total = b->model.mesh.tets.size();
         const double t0 = seconds();
         auto &sys = b->study->system.system;
@@ -434,7 +434,7 @@ sf_status sf_study_builder_finish(sf_study_builder *b, sf_static_study **out) {
         if (b->next != total || (b->use_multigrid && !b->setup_finished))
             throw Error(SF_ERR_MODEL_NOT_FINISHED, "the study build has not finished");
         const double t0 = seconds();
-        hipStream_t s = b->ctx->ctx->stream;
+        hipStream_t s = b->ctx->stream;
         auto &st = *b->study;
         st.system.system.build_from_device_values(s);
         double pre = b->preconditioner_seconds;
@@ -446,7 +446,7 @@ sf_status sf_study_builder_finish(sf_study_builder *b, sf_static_study **out) {
             pre += seconds() - t1;
         }
         st.pcg = std::make_unique<sf_pcg>();
-        st.pcg->pcg = std::make_unique<DevicePcg>(*b->ctx->ctx, st.system.system, st.mg.mg.get(), b->problem.rhs, b->pcg);
+        st.pcg->pcg = std::make_unique<DevicePcg>(b->ctx, st.system.system, st.mg.mg.get(), b->problem.rhs, b->pcg);
         st.model = std::move(b->model);
         st.problem = std::move(b->problem);
         auto &r = st.report;
@@ -520,8 +520,8 @@ sf_status sf_static_study_matrix(const sf_static_study *study, uint64_t *size, u
         const auto &sys = study->system.system;
         HostBsr<double> h = sys.host;
         h.values.resize(h.blocks() * 9);
-        sys.values.download(h.values.data(), h.values.size(), study->ctx->ctx->stream);
-        SF_HIP(hipStreamSynchronize(study->ctx->ctx->stream));
+        sys.values.download(h.values.data(), h.values.size(), study->ctx->stream);
+        SF_HIP(hipStreamSynchronize(study->ctx->stream));
         if (size) *size = sys.size;
         if (nonzeros) *nonzeros = h.blocks() * 9;
         to_scalar_csr(h, row_starts, columns, values);
@@ -538,7 +538,7 @@ sf_status sf_csr_create(sf_context *ctx, uint64_t size, const int64_t *row_start
                         sf_csr **out) {
     return guard([&] {
         auto c = std::make_unique<sf_csr>();
-        c->ctx = ctx;
+        c->ctx = ctx ? ctx->ctx : nullptr;
         auto &sys = c->system;
         sys.size = size;
         if (to_nodal(size, row_starts, columns, values, sys.host) < 0) {
@@ -560,7 +560,7 @@ void sf_csr_destroy(sf_csr *c) { delete c; }
 
 sf_status sf_csr_multiply(sf_csr *c, const double *x, double *y) {
     return guard([&] {
-        hipStream_t s = c->ctx->ctx->stream;
+        hipStream_t s = c->ctx->stream;
         const size_t n = c->system.size;
         DeviceBuffer<double> dx, dy(std::max<size_t>(n, 1));
         dx.upload(x, n, s);
@@ -572,7 +572,7 @@ sf_status sf_csr_multiply(sf_csr *c, const double *x, double *y) {
 
 sf_status sf_csr_rounding_bound(sf_csr *c, const double *x, int32_t dot_order, double *bound) {
     return guard([&] {
-        hipStream_t s = c->ctx->ctx->stream;
+        hipStream_t s = c->ctx->stream;
         const size_t n = c->system.size;
         DeviceBuffer<double> dx, rows(std::max<size_t>(n, 1)), partials(dev::DOT_BLOCKS), result(1);
         dx.upload(x, n, s);
@@ -613,7 +613,7 @@ sf_status sf_multigrid_setup_create(sf_context *ctx, const sf_csr *matrix, const
         for (size_t r = 0; r < position_count; ++r)
             for (int a = 0; a < 3; ++a) pos[r][a] = positions[3 * r + a];
         auto s = std::make_unique<sf_multigrid_setup>();
-        s->ctx = ctx;
+        s->ctx = ctx ? ctx->ctx : nullptr;
         s->setup = std::make_unique<MultigridSetup>(sys.host, modes, pos, setup_options(options));
         *out = s.release();
     });
@@ -626,10 +626,11 @@ sf_status sf_multigrid_setup_step(sf_multigrid_setup *setup, uint64_t budget, sf
 }
 sf_status sf_multigrid_setup_finish(sf_multigrid_setup *setup, sf_multigrid **out) {
     return guard([&] {
+        if (!setup->ctx) throw Error(SF_ERR_INVALID_ARGUMENT, "a device multigrid needs the setup to have a device context");
         auto m = std::make_unique<sf_multigrid>();
         m->ctx = setup->ctx;
         const auto opt = setup->setup->options();
-        m->mg = make_device_multigrid(setup->setup->take_levels(), opt.degree, opt.precision, opt.coarse_solve, setup->ctx->ctx->stream);
+        m->mg = make_device_multigrid(setup->setup->take_levels(), opt.degree, opt.precision, opt.coarse_solve, setup->ctx->stream);
         *out = m.release();
     });
 }
@@ -643,7 +644,7 @@ int32_t sf_multigrid_precision(const sf_multigrid *mg) { return mg->mg->precisio
 
 sf_status sf_multigrid_apply(sf_multigrid *mg, const double *r, double *z) {
     return guard([&] {
-        hipStream_t s = mg->ctx->ctx->stream;
+        hipStream_t s = mg->ctx->stream;
         const size_t n = mg->mg->dofs();
         DeviceBuffer<double> dr, dz(std::max<size_t>(n, 1));
         dr.upload(r, n, s);
@@ -656,7 +657,7 @@ sf_status sf_multigrid_apply(sf_multigrid *mg, const double *r, double *z) {
 sf_status sf_multigrid_smooth(sf_multigrid *mg, uint64_t level, const double *b, double *x) {
     return guard([&] {
         if (level >= mg->mg->host.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
-        hipStream_t s = mg->ctx->ctx->stream;
+        hipStream_t s = mg->ctx->stream;
         const size_t n = mg->mg->host[level].a.dofs();
         DeviceBuffer<double> db, dx(std::max<size_t>(n, 1));
         db.upload(b, n, s);
@@ -716,7 +717,7 @@ sf_status sf_multigrid_solve_coarsest(sf_multigrid *mg, double *values, uint64_t
         const size_t n = h.back().a.dofs();
         if (!h.back().has_factor) throw Error(SF_ERR_MG_COARSEST_NOT_FACTORED, "the coarsest level has no direct factorization", (int64_t)n);
         if (count != n) throw Error(SF_ERR_MG_COARSE_LENGTH, "the coarse solve was given the wrong length", (int64_t)n, (int64_t)count);
-        hipStream_t s = mg->ctx->ctx->stream;
+        hipStream_t s = mg->ctx->stream;
         DeviceBuffer<double> b, x(std::max<size_t>(n, 1));
         b.upload(values, n, s);
         mg->mg->coarse_solve(b.ptr, x.ptr, s);
@@ -731,7 +732,7 @@ sf_status sf_pcg_create(sf_context *ctx, const sf_csr *matrix, sf_multigrid *mg,
         auto p = std::make_unique<sf_pcg>();
         const sf_pcg_options o = options ? *options : sf_pcg_options{1e-9, 0, SF_DOT_TREE, 0};
         std::vector<double> b(rhs, rhs + matrix->system.size);
-        p->pcg = std::make_unique<DevicePcg>(*ctx->ctx, matrix->system, mg ? mg->mg.get() : nullptr, b, o);
+        p->pcg = std::make_unique<DevicePcg>(ctx->ctx, matrix->system, mg ? mg->mg.get() : nullptr, b, o);
         *out = p.release();
     });
 }
@@ -794,7 +795,7 @@ void sf_server_destroy(sf_server *server) { delete server; }
 sf_status sf_multigrid_apply_batch(sf_multigrid *mg, const double *r, uint64_t columns, double *z) {
     return guard([&] {
         if (columns == 0 || columns > MAX_COLUMNS) throw Error(SF_ERR_INVALID_ARGUMENT, "a batch carries 1 to 64 columns");
-        hipStream_t s = mg->ctx->ctx->stream;
+        hipStream_t s = mg->ctx->stream;
         const size_t n = mg->mg->dofs(), total = n * columns;
         const uint32_t M = (uint32_t)columns;
         DeviceBuffer<double> stage, dr(std::max<size_t>(total, 1)), dz(std::max<size_t>(total, 1));
@@ -817,7 +818,7 @@ sf_status sf_pcg_batch_create(sf_context *ctx, const sf_csr *matrix, sf_multigri
         if (columns == 0 || columns > MAX_COLUMNS) throw Error(SF_ERR_INVALID_ARGUMENT, "a batch carries 1 to 64 right-hand sides");
         auto p = std::make_unique<sf_pcg_batch>();
         const sf_pcg_options o = options ? *options : sf_pcg_options{1e-9, 0, SF_DOT_TREE, 0};
-        p->batch = std::make_unique<DevicePcgBatch>(*ctx->ctx, matrix->system, mg ? mg->mg.get() : nullptr, rhs, (uint32_t)columns, o);
+        p->batch = std::make_unique<DevicePcgBatch>(ctx->ctx, matrix->system, mg ? mg->mg.get() : nullptr, rhs, (uint32_t)columns, o);
         *out = p.release();
     });
 }
@@ -915,7 +916,7 @@ sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision,
         auto m = std::make_unique<sf_multigrid>();
         m->ctx = mg->ctx;
         m->mg = make_device_multigrid(mg->mg->host, mg->mg->degree, precision, coarse_solve < 0 ? mg->mg->coarse_mode : coarse_solve,
-                                      mg->ctx->ctx->stream);
+                                      mg->ctx->stream);
         *out = m.release();
     });
 }

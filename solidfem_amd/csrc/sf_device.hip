@@ -762,12 +762,12 @@ void DeviceSystem::build_from_device_values(hipStream_t s) {
 
 // ---- the conjugate gradient -------------------------------------------------------------------
 
-DevicePcg::DevicePcg(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
+DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
                      const sf_pcg_options &options)
-    : ctx_(ctx), a_(a), mg_(mg), n_(a.size), dot_order_(options.dot_order), tolerance_(options.tolerance) {
+    : ctx_(std::move(ctx)), a_(a), mg_(mg), n_(a.size), dot_order_(options.dot_order), tolerance_(options.tolerance) {
     if (rhs.size() != n_) throw Error(SF_ERR_INVALID_ARGUMENT, "the right-hand side does not match the matrix");
     if (mg_ && mg_->dofs() != n_) throw Error(SF_ERR_INVALID_ARGUMENT, "the multigrid does not match the matrix");
-    hipStream_t s = ctx_.stream;
+    hipStream_t s = ctx_->stream;
     max_iterations_ = options.max_iterations ? options.max_iterations : 10 * (u64)n_ + 1000;
     for (auto *buf : {&b_, &x_, &r_, &z_, &p_, &ap_, &tmp_, &tmp2_}) buf->resize(std::max<size_t>(n_, 1));
     b_.upload(rhs.data(), n_, s);
@@ -797,7 +797,7 @@ DevicePcg::DevicePcg(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, c
 
 void DevicePcg::initialise() {
     // pcg.rs:135-164: r = b, x = 0, then the first direction from the preconditioned residual
-    hipStream_t s = ctx_.stream;
+    hipStream_t s = ctx_->stream;
     SF_HIP(hipMemcpyAsync(r_.ptr, b_.ptr, n_ * 8, hipMemcpyDeviceToDevice, s));
     SF_HIP(hipMemsetAsync(x_.ptr, 0, n_ * 8, s));
     SF_HIP(hipMemsetAsync(z_.ptr, 0, n_ * 8, s));
@@ -822,23 +822,23 @@ void DevicePcg::initialise() {
 }
 
 void DevicePcg::restart(const double *new_rhs) {
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
-    if (new_rhs) b_.upload(new_rhs, n_, ctx_.stream);
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
+    if (new_rhs) b_.upload(new_rhs, n_, ctx_->stream);
     initialise();
 }
 
 DevicePcg::~DevicePcg() {
-    (void)hipStreamSynchronize(ctx_.stream);
+    (void)hipStreamSynchronize(ctx_->stream);
     if (pinned_) (void)hipHostFree(pinned_);
     if (copied_) (void)hipEventDestroy(copied_);
 }
 
 void DevicePcg::dot(const double *a, const double *b, double *result, int final_kind) {
     if (dot_order_ == SF_DOT_REFERENCE) {
-        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_.stream, n_, a, b, final_kind, result, scalars_.ptr);
+        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, b, final_kind, result, scalars_.ptr);
     } else {
-        hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_.stream, n_, a, b, partials_.ptr);
-        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_.stream, DOT_BLOCKS, (const double *)partials_.ptr,
+        hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, b, partials_.ptr);
+        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
                            final_kind, result, scalars_.ptr);
     }
     SF_HIP(hipGetLastError());
@@ -846,31 +846,31 @@ void DevicePcg::dot(const double *a, const double *b, double *result, int final_
 
 void DevicePcg::sum(const double *a, double *result) {
     if (dot_order_ == SF_DOT_REFERENCE) {
-        hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_.stream, n_, a, a, (int)FINAL_STORE, result, scalars_.ptr);
+        hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, a, (int)FINAL_STORE, result, scalars_.ptr);
     } else {
-        hipLaunchKernelGGL((dot_partials_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_.stream, n_, a, a, partials_.ptr);
-        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_.stream, DOT_BLOCKS, (const double *)partials_.ptr,
+        hipLaunchKernelGGL((dot_partials_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, a, partials_.ptr);
+        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
                            (int)FINAL_STORE, result, scalars_.ptr);
     }
     SF_HIP(hipGetLastError());
 }
 
 void DevicePcg::precondition(const double *r, double *z) {
-    if (mg_) mg_->apply(r, z, ctx_.stream);
-    else hipLaunchKernelGGL(jacobi_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, ctx_.stream, n_, jacobi_.ptr, r, z);
+    if (mg_) mg_->apply(r, z, ctx_->stream);
+    else hipLaunchKernelGGL(jacobi_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, ctx_->stream, n_, jacobi_.ptr, r, z);
     SF_HIP(hipGetLastError());
 }
 
 void DevicePcg::reset_direction() {
     // pcg.rs:277-282
     precondition(r_.ptr, z_.ptr);
-    SF_HIP(hipMemcpyAsync(p_.ptr, z_.ptr, n_ * 8, hipMemcpyDeviceToDevice, ctx_.stream));
+    SF_HIP(hipMemcpyAsync(p_.ptr, z_.ptr, n_ * 8, hipMemcpyDeviceToDevice, ctx_->stream));
     dot(r_.ptr, z_.ptr, &scalars_.ptr->rz);
 }
 
 PcgScalars DevicePcg::read_scalars() {
-    SF_HIP(hipMemcpyAsync(pinned_, scalars_.ptr, sizeof(PcgScalars), hipMemcpyDeviceToHost, ctx_.stream));
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    SF_HIP(hipMemcpyAsync(pinned_, scalars_.ptr, sizeof(PcgScalars), hipMemcpyDeviceToHost, ctx_->stream));
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
     return *pinned_;
 }
 
@@ -879,7 +879,7 @@ void DevicePcg::iterate() {
     // preconditioner, r.z and the new direction) is enqueued before the residual is read back, so
     // the readback overlaps the V-cycle. When the other branch is taken, that work is discarded:
     // it changes only z, p and rz, which the true-residual branch resets or never reads again.
-    hipStream_t s = ctx_.stream;
+    hipStream_t s = ctx_->stream;
     PcgScalars *sc = scalars_.ptr;
     const int token = Profiler::get().begin(1, s, (double)a_.sliced.algorithmic_matrix_bytes() + 2.0 * n_ * 8);
     launch_spmv<double, double>(a_.sliced, p_.ptr, ap_.ptr, nullptr, EPI_STORE, s);
@@ -944,23 +944,23 @@ int DevicePcg::step(u64 max, const Cancel &cancel) {
     }
     for (u64 i = 0; i < max; ++i) {
         if (cancel()) {
-            SF_HIP(hipStreamSynchronize(ctx_.stream));
+            SF_HIP(hipStreamSynchronize(ctx_->stream));
             status_ = PcgStatus::Cancelled;
             return SF_CANCELLED;
         }
         iterate();
         if (status_ == PcgStatus::Converged) {
-            SF_HIP(hipStreamSynchronize(ctx_.stream));
+            SF_HIP(hipStreamSynchronize(ctx_->stream));
             return SF_CONVERGED;
         }
     }
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
     return SF_RUNNING;
 }
 
 void DevicePcg::estimate(double *out) const {
-    x_.download(out, n_, ctx_.stream);
-    SF_HIP(hipStreamSynchronize(ctx_.stream));
+    x_.download(out, n_, ctx_->stream);
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
 }
 
 sf_progress DevicePcg::progress() const {

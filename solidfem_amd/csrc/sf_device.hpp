@@ -145,7 +145,7 @@ enum class PcgStatus { Active, Converged, Cancelled, NotPositiveDefinite, NotCon
 
 class DevicePcg {
    public:
-    DevicePcg(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
+    DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
               const sf_pcg_options &options);
     // Pcg::new again on the buffers already held, optionally with a new right-hand side.
     void restart(const double *new_rhs);
@@ -168,7 +168,7 @@ class DevicePcg {
     void initialise();
     dev::PcgScalars read_scalars();
 
-    Context &ctx_;
+    std::shared_ptr<Context> ctx_;  // kept alive while this solve may launch on its stream
     const DeviceSystem &a_;
     DeviceMultigrid *mg_;
     size_t n_;
@@ -192,7 +192,7 @@ class DevicePcg {
 // converges, fails or is cancelled is frozen while the rest go on.
 class DevicePcgBatch {
    public:
-    DevicePcgBatch(Context &ctx, const DeviceSystem &a, DeviceMultigrid *mg, const double *rhs_columns, uint32_t columns,
+    DevicePcgBatch(std::shared_ptr<Context> ctx, const DeviceSystem &a, DeviceMultigrid *mg, const double *rhs_columns, uint32_t columns,
                    const sf_pcg_options &options);
     ~DevicePcgBatch();
     // Pcg::new again on the held buffers, optionally with new right-hand sides (column-major).
@@ -226,7 +226,7 @@ class DevicePcgBatch {
     const dev::PcgScalars *read_scalars();
     std::vector<uint32_t> active_flags() const;
 
-    Context &ctx_;
+    std::shared_ptr<Context> ctx_;  // kept alive while this solve may launch on its stream
     const DeviceSystem &a_;
     DeviceMultigrid *mg_;
     size_t n_;

@@ -8,7 +8,7 @@
 using namespace sf;
 
 struct sf_context {
-    std::unique_ptr<Context> ctx;
+    std::shared_ptr<Context> ctx;  // shared with every object that launches on its stream
 };
 struct sf_tet_mesh {
     TetMesh mesh;
@@ -21,15 +21,15 @@ struct sf_model {
     sf_quad_mesh view;  // the mesh as a handle
 };
 struct sf_csr {
-    sf_context *ctx = nullptr;
+    std::shared_ptr<Context> ctx;
     DeviceSystem system;
 };
 struct sf_multigrid {
-    sf_context *ctx = nullptr;
+    std::shared_ptr<Context> ctx;
     std::unique_ptr<DeviceMultigrid> mg;
 };
 struct sf_multigrid_setup {
-    sf_context *ctx = nullptr;
+    std::shared_ptr<Context> ctx;
     std::unique_ptr<MultigridSetup> setup;
 };
 struct sf_pcg {
@@ -41,7 +41,7 @@ struct sf_pcg_batch {
 };
 
 struct sf_static_study {
-    sf_context *ctx = nullptr;
+    std::shared_ptr<Context> ctx;
     Model model;
     FreeProblem problem;
     sf_csr system;
@@ -51,7 +51,7 @@ struct sf_static_study {
 };
 
 struct sf_study_builder {
-    sf_context *ctx = nullptr;
+    std::shared_ptr<Context> ctx;
     Model model;
     FreeProblem problem;
     AssemblyPattern pattern;

@@ -14,7 +14,9 @@ namespace sf {
 
 size_t host_threads() {
     static size_t n = [] {
+        // the process's CPU share where the environment states one, else the hardware's threads
         size_t h = std::thread::hardware_concurrency();
+        if (const char *omp = std::getenv("OMP_NUM_THREADS")) h = std::min<size_t>(h, (size_t)std::max(1, std::atoi(omp)));
         if (const char *env = std::getenv("SOLIDFEM_HOST_THREADS")) h = (size_t)std::max(1, std::atoi(env));
         return std::max<size_t>(1, std::min<size_t>(h, 64));
     }();

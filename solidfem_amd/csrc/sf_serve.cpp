@@ -242,7 +242,7 @@ void Server::builder_loop() {
     std::unique_ptr<sf_context> ctx;
     try {
         ctx = std::make_unique<sf_context>();
-        ctx->ctx = std::make_unique<Context>(device_);
+        ctx->ctx = std::make_shared<Context>(device_);
     } catch (...) {
         return;
     }
@@ -345,9 +345,9 @@ void Server::builder_loop() {
 }
 
 void Server::solver_loop() {
-    std::unique_ptr<Context> ctx;
+    std::shared_ptr<Context> ctx;
     try {
-        ctx = std::make_unique<Context>(device_);
+        ctx = std::make_shared<Context>(device_);
     } catch (...) {
         return;
     }
@@ -371,7 +371,7 @@ void Server::solver_loop() {
             }
         }
         try {
-            serve(*system, *ctx);
+            serve(*system, ctx);
         } catch (const Error &e) {
             // a device or host failure inside a batch fails the requests it held
             std::lock_guard<std::mutex> g(mu_);
@@ -390,7 +390,7 @@ void Server::solver_loop() {
     }
 }
 
-void Server::serve(ServeSystem &system, Context &ctx) {
+void Server::serve(ServeSystem &system, const std::shared_ptr<Context> &ctx) {
     sf_static_study &st = *system.study;
     const QuadMesh &mesh = st.model.mesh;
     const FreeProblem &numbering = st.problem;

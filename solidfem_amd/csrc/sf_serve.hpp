@@ -57,7 +57,7 @@ class Server {
     void builder_loop();
     void solver_loop();
     void build(ServeSystem &system, const sf_workload &w, sf_context &ctx);
-    void serve(ServeSystem &system, Context &ctx);
+    void serve(ServeSystem &system, const std::shared_ptr<Context> &ctx);
     void finish_request(ServeRequest &r, int state);
     sf_serve_status describe(const ServeRequest &r) const;
     u64 estimate(const sf_workload &w) const;

@@ -176,3 +176,22 @@ def test_batch_limits_are_refused(ctx):
     csr = sf.CsrMatrix.new(ctx, rs, cols, vals)
     with pytest.raises(sf.SolidfemError):
         sf.PcgBatch.new(ctx, csr, None, np.ones((65, n)))
+
+
+def test_solves_outlive_their_context_handle():
+    # Handles caught in a reference cycle are finalized together in any order; a solve keeps the
+    # stream it launches on alive after its context handle is gone.
+    own = sf.Context(0)
+    n = 8
+    rs, cols, vals = laplacian(n)
+    csr = sf.CsrMatrix.new(own, rs, cols, vals)
+    pcg = sf.Pcg.new(own, csr, None, np.ones(n))
+    batch = sf.PcgBatch.new(own, csr, None, np.ones((2, n)))
+    own.close()
+    assert pcg.step(100).outcome == sf.StepOutcome.CONVERGED
+    running, progress = batch.step(100)
+    assert running == 0 and all(p.outcome == sf.StepOutcome.CONVERGED for p in progress)
+    assert np.array_equal(bits(batch.estimate()[0]), bits(pcg.estimate()))
+    csr.close()
+    pcg.close()
+    batch.close()
