@@ -626,6 +626,7 @@ sf_status sf_multigrid_setup_step(sf_multigrid_setup *setup, uint64_t budget, sf
 }
 sf_status sf_multigrid_setup_finish(sf_multigrid_setup *setup, sf_multigrid **out) {
     return guard([&] {
+        if (!setup->setup->finished()) throw Error(SF_ERR_MG_NOT_FINISHED, "the multigrid setup has not finished");
         if (!setup->ctx) throw Error(SF_ERR_INVALID_ARGUMENT, "a device multigrid needs the setup to have a device context");
         auto m = std::make_unique<sf_multigrid>();
         m->ctx = setup->ctx;
