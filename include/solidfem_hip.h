@@ -402,6 +402,11 @@ const sf_csr *sf_static_study_system(const sf_static_study *study);
  * given coarse solve (SF_COARSE_*, or -1 to keep this one's). */
 sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision, int32_t coarse_solve,
                                       sf_multigrid **out);
+/* Launch option, no reference counterpart: with enable set, sf_multigrid_apply and the solves
+ * using this hierarchy replay a HIP graph of one V-cycle's launches, captured on first use for
+ * each pair of input and output buffers, instead of launching the kernels one by one. Results are
+ * the same bits. Off by default; never used while kernel timing (sf_profile_enable) is on. */
+sf_status sf_multigrid_set_graph(sf_multigrid *mg, int32_t enable);
 /* Algorithmic HBM bytes of one V-cycle of this hierarchy (every array a kernel touches, once). */
 double sf_multigrid_cycle_bytes(const sf_multigrid *mg);
 /* Restarts a solve from a zero estimate (Pcg::new semantics, pcg.rs:135-164) on the buffers it

@@ -629,6 +629,10 @@ class Multigrid(_Handle, _Levels):
         _raise(library().sf_multigrid_apply(self._ptr, _ptr(r, C.c_double), _ptr(z, C.c_double)))
         return z
 
+    def set_graph(self, enable: bool = True):
+        """Replay each V-cycle as one captured HIP graph (the same bits); see sf_multigrid_set_graph."""
+        _raise(library().sf_multigrid_set_graph(self._ptr, 1 if enable else 0))
+
     def apply_batch(self, residuals) -> np.ndarray:
         """apply() of each row of `residuals` (columns x n), all through one batched V-cycle."""
         r = np.ascontiguousarray(residuals, dtype=np.float64)

@@ -202,6 +202,7 @@ SIGNATURES = {
     "sf_profile_read": (i32, [i32, P(u64), P(f64), P(f64)]),
     "sf_stream_copy": (i32, [vp, u64, u32, P(f64)]),
     "sf_multigrid_apply_batch": (i32, [vp, P(f64), u64, P(f64)]),
+    "sf_multigrid_set_graph": (i32, [vp, i32]),
     "sf_pcg_batch_create": (i32, [vp, vp, vp, P(f64), u64, P(PcgOptions), P(vp)]),
     "sf_pcg_batch_step": (i32, [vp, u64, CANCEL_FN, vp, P(ColumnProgress), P(u64)]),
     "sf_pcg_batch_progress": (i32, [vp, P(ColumnProgress)]),

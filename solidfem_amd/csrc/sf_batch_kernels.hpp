@@ -211,7 +211,18 @@ __global__ __launch_bounds__(256) void coarse_inverse_batch_kernel(uint32_t n, u
         for (uint32_t e = threadIdx.x; e < kn * M; e += blockDim.x) bs[e] = (double)rhs[(size_t)k0 * M + e];
         __syncthreads();
         if (i < n) {
-            for (uint32_t kk = 0; kk < kn; ++kk) {
+            uint32_t kk = 0;
+            for (; kk + 4 <= kn; kk += 4) {  // four loads in flight, terms in ascending k
+                double a[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u] = inv[(size_t)(k0 + kk + u) * n + i];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        if ((uint32_t)c < per && m0 + c < M) sum[c] = sum[c] + a[u] * bs[(kk + u) * M + m0 + c];
+            }
+            for (; kk < kn; ++kk) {
                 const double a = inv[(size_t)(k0 + kk) * n + i];
 #pragma unroll
                 for (int c = 0; c < 16; ++c)

@@ -647,10 +647,13 @@ sf_status sf_multigrid_apply(sf_multigrid *mg, const double *r, double *z) {
     return guard([&] {
         hipStream_t s = mg->ctx->stream;
         const size_t n = mg->mg->dofs();
-        DeviceBuffer<double> dr, dz(std::max<size_t>(n, 1));
-        dr.upload(r, n, s);
-        mg->mg->apply(dr.ptr, dz.ptr, s);
-        dz.download(z, n, s);
+        if (mg->in.count < std::max<size_t>(n, 1)) {
+            mg->in.resize(std::max<size_t>(n, 1));
+            mg->out.resize(std::max<size_t>(n, 1));
+        }
+        mg->in.upload(r, n, s);
+        mg->mg->apply(mg->in.ptr, mg->out.ptr, s);
+        mg->out.download(z, n, s);
         SF_HIP(hipStreamSynchronize(s));
     });
 }
@@ -792,6 +795,13 @@ sf_status sf_server_stats_get(sf_server *server, sf_server_stats *out) {
     return guard([&] { *out = server->server->stats(); });
 }
 void sf_server_destroy(sf_server *server) { delete server; }
+
+sf_status sf_multigrid_set_graph(sf_multigrid *mg, int32_t enable) {
+    return guard([&] {
+        SF_HIP(hipStreamSynchronize(mg->ctx->stream));
+        mg->mg->use_graph = enable != 0;
+    });
+}
 
 sf_status sf_multigrid_apply_batch(sf_multigrid *mg, const double *r, uint64_t columns, double *z) {
     return guard([&] {

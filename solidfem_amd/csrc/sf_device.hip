@@ -30,9 +30,18 @@ Context::~Context() {
 
 template <typename T>
 uint32_t DeviceSliced<T>::xcd_order() {
-    static const uint32_t on = [] {
+    static const uint32_t order = [] {
         const char *e = std::getenv("SOLIDFEM_XCD_ORDER");
-        return e ? (uint32_t)(std::atoi(e) != 0) : 0u;  // measured slower on the plate; off unless asked
+        const int v = e ? std::atoi(e) : 0;
+        return v == 1 ? (uint32_t)ORDER_XCD_RUNS : v == 2 ? (uint32_t)ORDER_XCD_WINDOWS : 0u;
+    }();
+    return order;
+}
+template <typename T>
+bool DeviceSliced<T>::ping_pong() {
+    static const bool on = [] {
+        const char *e = std::getenv("SOLIDFEM_PING_PONG");
+        return e && std::atoi(e) != 0;
     }();
     return on;
 }
@@ -411,8 +420,46 @@ class Hierarchy final : public DeviceMultigrid {
         compute_stats();
     }
 
+    ~Hierarchy() override {
+        if (!graphs_.empty()) (void)hipDeviceSynchronize();  // a replay may still run on any stream
+        for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    }
+
     void apply(const double *r, double *z, hipStream_t s) override {
         if (lv_.empty()) return;
+        if (use_graph && !Profiler::get().any_on()) {
+            for (const auto &g : graphs_)
+                if (g.r == r && g.z == z && g.stream == s) {
+                    SF_HIP(hipGraphLaunch(g.exec, s));
+                    return;
+                }
+            if (graphs_.size() >= 8) {  // a bounded cache: the oldest goes once its stream is idle
+                SF_HIP(hipStreamSynchronize(graphs_.front().stream));
+                SF_HIP(hipGraphExecDestroy(graphs_.front().exec));
+                graphs_.erase(graphs_.begin());
+            }
+            hipGraph_t graph = nullptr;
+            SF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_apply(r, z, s);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            SF_HIP(hipStreamEndCapture(s, &graph));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            SF_HIP(e);
+            graphs_.push_back({r, z, s, exec});
+            SF_HIP(hipGraphLaunch(exec, s));
+            return;
+        }
+        launch_apply(r, z, s);
+    }
+
+    void launch_apply(const double *r, double *z, hipStream_t s) {
         if constexpr (std::is_same<V, double>::value) {
             cycle(0, r, z, s);
         } else {
@@ -723,6 +770,13 @@ class Hierarchy final : public DeviceMultigrid {
         cycle_bytes = bytes;
     }
 
+    struct Graph {
+        const double *r;
+        double *z;
+        hipStream_t stream;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs_;
     std::vector<LevelDev<S, V>> lv_;
     DeviceBuffer<V> in_, out_;
     DeviceBuffer<V> min_, mout_;  // the finest level's columns in V, for the single precision

@@ -34,8 +34,15 @@ struct DeviceSliced {
     int batch_spmv = 4;  // blocks in flight per lane, chosen by tune_batch
     int batch_cheb = 4;
     mutable int batch_mm[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // batched product, per log2 of the column group
-    dev::Sliced<T> view() const { return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices, xcd_order()}; }
-    static uint32_t xcd_order();
+    mutable uint32_t passes = 0;  // launches so far, for the alternating traversal
+    // The view one launch takes; each call is one pass (see dev::slice_of for the orders).
+    dev::Sliced<T> view() const {
+        uint32_t order = xcd_order();
+        if (ping_pong() && (passes++ & 1u)) order |= dev::ORDER_REVERSE;
+        return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices, order};
+    }
+    static uint32_t xcd_order();  // SOLIDFEM_XCD_ORDER: 0 none, 1 contiguous runs, 2 whole windows
+    static bool ping_pong();      // SOLIDFEM_PING_PONG: alternate passes walk the slices backwards
     size_t dofs() const { return rows * rb; }
     // Bytes one product reads by the algorithmic count: values, block columns, row offsets.
     size_t algorithmic_matrix_bytes() const { return blocks * (rb * cb * sizeof(T) + 4) + (rows + 1) * 4; }
@@ -100,6 +107,11 @@ struct Profiler {
     void end(int kernel, int token, hipStream_t s);
     void collect(int kernel);
     void reset();
+    bool any_on() const {
+        for (const Slot &k : slots)
+            if (k.on) return true;
+        return false;
+    }
 };
 
 // ---- multigrid --------------------------------------------------------------------------------
@@ -123,6 +135,9 @@ class DeviceMultigrid {
     size_t dofs() const { return host.empty() ? 0 : host[0].a.dofs(); }
     // Algorithmic HBM bytes of one V-cycle (every array each kernel touches, counted once).
     double cycle_bytes = 0;
+    // apply() replays a HIP graph of the cycle's launches, captured once per (r, z) pair, instead
+    // of launching them one by one (not while kernel timing is on). The same kernels, the same bits.
+    bool use_graph = false;
 };
 std::unique_ptr<DeviceMultigrid> make_device_multigrid(std::vector<HostLevel> levels, size_t degree, int precision,
                                                        int coarse_mode, hipStream_t s);

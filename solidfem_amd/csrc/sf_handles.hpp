@@ -27,6 +27,7 @@ struct sf_csr {
 struct sf_multigrid {
     std::shared_ptr<Context> ctx;
     std::unique_ptr<DeviceMultigrid> mg;
+    DeviceBuffer<double> in, out;  // apply()'s device copies, kept between calls
 };
 struct sf_multigrid_setup {
     std::shared_ptr<Context> ctx;

@@ -42,22 +42,43 @@ struct Sliced {
     const uint32_t *cols;         // per block slot and slice row
     const T *vals;                // per block slot, block column and lane
     uint32_t slices;
-    uint32_t xcd;                 // nonzero: XCD-contiguous slice order (xcd_block)
+    uint32_t order;               // workgroup -> slice order: ORDER_* bits (slice_of)
 };
+
+// Workgroup-to-slice orders; none changes a bit of any result, only which workgroups run
+// together and where. XCD_RUNS gives each XCD one contiguous run of slices; XCD_WINDOWS gives
+// each XCD whole sort windows (8 workgroups of 4 slices), so the rows a window permutes share one
+// L2; REVERSE walks the slices from the end (alternate passes over a matrix then start where the
+// last one ended, in the cache).
+enum : uint32_t { ORDER_XCD_RUNS = 1, ORDER_XCD_WINDOWS = 2, ORDER_REVERSE = 4 };
 
 template <typename V, typename S>
 __device__ __forceinline__ V widen(S s) {
     return (V)(double)s;
 }
 
-// Workgroups are dispatched to the 8 XCDs in turn. Renumbering them so that each XCD takes a
-// contiguous run of slices keeps the rows an XCD works on -- and the vector entries they gather
-// -- together in that XCD's L2. A bijection on [0, gridDim.x).
-__device__ __forceinline__ uint32_t xcd_block() {
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD b mod 8). The renumberings
+// below are bijections on [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_runs_block() {
     constexpr uint32_t XCDS = 8;
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t x = b % XCDS, local = b / XCDS, base = nb / XCDS, rem = nb % XCDS;
     return x * base + (x < rem ? x : rem) + local;
+}
+__device__ __forceinline__ uint32_t xcd_windows_block() {
+    // within each group of 64 workgroups, XCD x takes logical workgroups 8x .. 8x + 7
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    if (b >= nb - nb % 64) return b;
+    const uint32_t r = b % 64;
+    return b - r + (r % 8) * 8 + r / 8;
+}
+// The slice a wavefront works on under `order` (4 wavefronts of 64 lanes per workgroup).
+__device__ __forceinline__ uint32_t slice_of(uint32_t order) {
+    const uint32_t block = (order & ORDER_XCD_RUNS) ? xcd_runs_block() : (order & ORDER_XCD_WINDOWS) ? xcd_windows_block() : blockIdx.x;
+    const uint32_t waves = blockDim.x / WAVE;
+    uint32_t slice = block * waves + threadIdx.x / WAVE;
+    if (order & ORDER_REVERSE) slice = gridDim.x * waves - 1 - slice;
+    return slice;
 }
 
 // Sum of one lane's row of A x, from zero in stored order: the row product of the reference.
@@ -117,7 +138,7 @@ template <int RB, int CB, int BATCH, typename S, typename V, int EPI>
 __global__ __launch_bounds__(256) void spmv_kernel(Sliced<S> a, const V *__restrict__ x, V *__restrict__ y,
                                                    const V *__restrict__ b) {
     constexpr int NPW = Shape<RB>::NPW;
-    const uint32_t slice = (a.xcd ? xcd_block() : blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t slice = slice_of(a.order);
     if (slice >= a.slices) return;
     const int lane = threadIdx.x % WAVE;
     const int j = lane / RB, p = lane - j * RB;
@@ -182,7 +203,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(Sliced<S> a, const S *__
                                                         const V *__restrict__ r_in, V keep, V step, V *__restrict__ d_out,
                                                         V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into) {
     constexpr int NPW = Shape<RB>::NPW;
-    const uint32_t slice = (a.xcd ? xcd_block() : blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t slice = slice_of(a.order);
     if (slice >= a.slices) return;  // whole wave leaves together
     const int lane = threadIdx.x % WAVE;
     const int j = lane / RB, p = lane - j * RB;
@@ -325,8 +346,18 @@ __global__ __launch_bounds__(256) void coarse_inverse_kernel(uint32_t n, const d
         if (k0 + threadIdx.x < n) bs[threadIdx.x] = (double)rhs[k0 + threadIdx.x];
         __syncthreads();
         const uint32_t kmax = n - k0 < 256u ? n - k0 : 256u;
-        if (i < n)
-            for (uint32_t kk = 0; kk < kmax; ++kk) sum = sum + inv[(size_t)(k0 + kk) * n + i] * bs[kk];
+        if (i < n) {
+            // sixteen loads in flight, then their terms in ascending k
+            uint32_t kk = 0;
+            for (; kk + 16 <= kmax; kk += 16) {
+                double a[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) a[u] = inv[(size_t)(k0 + kk + u) * n + i];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) sum = sum + a[u] * bs[kk + u];
+            }
+            for (; kk < kmax; ++kk) sum = sum + inv[(size_t)(k0 + kk) * n + i] * bs[kk];
+        }
         __syncthreads();
     }
     if (i < n) out[i] = (V)sum;

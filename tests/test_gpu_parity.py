@@ -246,3 +246,30 @@ def test_an_indefinite_matrix_is_refused_by_name(ctx):
     with pytest.raises(sf.PcgError) as e:
         pcg.step(100)
     assert e.value.status == 10
+
+
+@pytest.mark.parametrize("precision", [0, 2], ids=["double", "single"])
+def test_graph_replays_equal_eager_launches(ctx, precision):
+    # every replay of the captured V-cycle, with eager launches between them, and a whole solve
+    # replaying it once per iteration, against the same hierarchy launched kernel by kernel
+    ref = refcpu.Problem(WORKLOADS["jittered-plate"])
+    csr, mg = device_multigrid(ctx, ref, precision, 150)
+    eager = mg.with_precision(sf.Precision(precision), coarse_solve=1)
+    graphed = mg.with_precision(sf.Precision(precision), coarse_solve=1)
+    graphed.set_graph(True)
+    rng = np.random.default_rng(3)
+    for k in range(5):
+        r = rng.standard_normal(ref.free_dofs)
+        z = eager.apply(r)
+        assert np.array_equal(bits(graphed.apply(r)), bits(z)), f"replay {k}"
+        rows, rb, _ = graphed.level_sizes()[1]
+        eager.smooth(1, rng.standard_normal(rows * rb))
+    opts = sf.PcgOptions(tolerance=1e-8, dot_order=sf.DotOrder.TREE)
+    a = sf.Pcg.new(ctx, csr, eager, ref.rhs(), opts)
+    b = sf.Pcg.new(ctx, csr, graphed, ref.rhs(), opts)
+    pa, pb = a.step(10 ** 6), b.step(10 ** 6)
+    assert pa.outcome == pb.outcome == sf.StepOutcome.CONVERGED and pa.iteration == pb.iteration
+    assert np.array_equal(bits(a.estimate()), bits(b.estimate()))
+    b.restart()
+    assert b.step(10 ** 6).iteration == pa.iteration
+    assert np.array_equal(bits(a.estimate()), bits(b.estimate()))
