@@ -504,6 +504,8 @@ sf_status sf_profile_read(int32_t kernel, uint64_t *launches, double *seconds, d
  * the context's stream: the achievable HBM bandwidth and the calibration of byte counters.
  * Returns the average seconds per copy, timed with HIP events. */
 sf_status sf_stream_copy(sf_context *ctx, uint64_t bytes, uint32_t repeats, double *seconds);
+/* The same for a plain read of `bytes` bytes: the achievable HBM read bandwidth. */
+sf_status sf_stream_read(sf_context *ctx, uint64_t bytes, uint32_t repeats, double *seconds);
 
 #ifdef __cplusplus
 }

@@ -205,6 +205,12 @@ class Context(_Handle):
         _raise(library().sf_stream_copy(self._ptr, nbytes, repeats, C.byref(t)))
         return t.value
 
+    def stream_read(self, nbytes: int, repeats: int = 10) -> float:
+        """Average seconds of a plain device read of `nbytes`."""
+        t = C.c_double()
+        _raise(library().sf_stream_read(self._ptr, nbytes, repeats, C.byref(t)))
+        return t.value
+
 
 class TetMesh(_Handle):
     """crates/solidfem-mesh/src/tet.rs:50-130"""

@@ -201,6 +201,7 @@ SIGNATURES = {
     "sf_profile_reset": (i32, []),
     "sf_profile_read": (i32, [i32, P(u64), P(f64), P(f64)]),
     "sf_stream_copy": (i32, [vp, u64, u32, P(f64)]),
+    "sf_stream_read": (i32, [vp, u64, u32, P(f64)]),
     "sf_multigrid_apply_batch": (i32, [vp, P(f64), u64, P(f64)]),
     "sf_multigrid_set_graph": (i32, [vp, i32]),
     "sf_pcg_batch_create": (i32, [vp, vp, vp, P(f64), u64, P(PcgOptions), P(vp)]),

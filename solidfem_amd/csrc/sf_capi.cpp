@@ -950,20 +950,27 @@ sf_status sf_profile_reset(void) {
     return guard([&] { Profiler::get().reset(); });
 }
 
-sf_status sf_stream_copy(sf_context *ctx, uint64_t bytes, uint32_t repeats, double *seconds) {
+static sf_status stream_probe(sf_context *ctx, uint64_t bytes, uint32_t repeats, double *seconds, bool read_only) {
     return guard([&] {
         hipStream_t s = ctx->ctx->stream;
         const size_t n16 = (size_t)(bytes / 16);
         if (n16 == 0 || repeats == 0) throw Error(SF_ERR_INVALID_ARGUMENT, "nothing to copy");
-        DeviceBuffer<uint4> a(n16), b(n16);
+        // 1024 workgroups striding over the buffers: the fastest of the shapes measured for a plain
+        // copy on this GPU (4, 16, 32 and 128 workgroups per CU striding, or one pass per lane)
+        const unsigned grid = 1024;
+        DeviceBuffer<uint4> a(n16), b(read_only ? 0 : n16);
+        DeviceBuffer<uint32_t> sink(read_only ? (size_t)grid * 256 : 0);
         SF_HIP(hipMemsetAsync(a.ptr, 1, n16 * 16, s));
+        auto launch = [&] {
+            if (read_only) hipLaunchKernelGGL(dev::stream_read_kernel, dim3(grid), dim3(256), 0, s, n16, (const uint4 *)a.ptr, sink.ptr);
+            else hipLaunchKernelGGL(dev::stream_copy_kernel, dim3(grid), dim3(256), 0, s, n16, (const uint4 *)a.ptr, b.ptr);
+        };
         hipEvent_t e0, e1;
         SF_HIP(hipEventCreate(&e0));
         SF_HIP(hipEventCreate(&e1));
-        const unsigned grid = 256 * 8 * 4;
-        hipLaunchKernelGGL(dev::stream_copy_kernel, dim3(grid), dim3(256), 0, s, n16, a.ptr, b.ptr);  // warm
+        launch();  // warm
         SF_HIP(hipEventRecord(e0, s));
-        for (uint32_t r = 0; r < repeats; ++r) hipLaunchKernelGGL(dev::stream_copy_kernel, dim3(grid), dim3(256), 0, s, n16, a.ptr, b.ptr);
+        for (uint32_t r = 0; r < repeats; ++r) launch();
         SF_HIP(hipEventRecord(e1, s));
         SF_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -972,6 +979,13 @@ sf_status sf_stream_copy(sf_context *ctx, uint64_t bytes, uint32_t repeats, doub
         (void)hipEventDestroy(e1);
         *seconds = ms * 1e-3 / repeats;
     });
+}
+
+sf_status sf_stream_copy(sf_context *ctx, uint64_t bytes, uint32_t repeats, double *seconds) {
+    return stream_probe(ctx, bytes, repeats, seconds, false);
+}
+sf_status sf_stream_read(sf_context *ctx, uint64_t bytes, uint32_t repeats, double *seconds) {
+    return stream_probe(ctx, bytes, repeats, seconds, true);
 }
 
 sf_status sf_profile_read(int32_t kernel, uint64_t *launches, double *seconds, double *bytes) {

@@ -504,5 +504,18 @@ static __global__ void stream_copy_kernel(size_t n16, const uint4 *__restrict__ 
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// A plain read, 16 bytes per lane per step, grid-stride; one word per lane is written so the
+// loads stay.
+static __global__ void stream_read_kernel(size_t n16, const uint4 *__restrict__ src, uint32_t *__restrict__ sink) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t acc = 0;
+    for (size_t i = t; i < n16; i += stride) {
+        const uint4 v = src[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    sink[t] = acc;
+}
+
 }  // namespace dev
 }  // namespace sf
