@@ -223,7 +223,7 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--columns", type=int, default=32, help="load cases of the multi-RHS measurement (0 = skip)")
-    ap.add_argument("--graph", type=int, default=0, help="replay each V-cycle as a captured HIP graph (1) or launch eagerly (0)")
+    ap.add_argument("--graph", type=int, default=1, help="replay each V-cycle as a captured HIP graph (1) or launch eagerly (0)")
     ap.add_argument("--serve-requests", type=int, default=512, help="requests of the serving measurement (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate of the serving measurement, per s")
     args = ap.parse_args()

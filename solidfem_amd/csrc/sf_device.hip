@@ -31,8 +31,9 @@ Context::~Context() {
 template <typename T>
 uint32_t DeviceSliced<T>::xcd_order() {
     static const uint32_t order = [] {
+        // whole sort windows per XCD by default: measured 11% faster on the plate's Chebyshev step
         const char *e = std::getenv("SOLIDFEM_XCD_ORDER");
-        const int v = e ? std::atoi(e) : 0;
+        const int v = e ? std::atoi(e) : 2;
         return v == 1 ? (uint32_t)ORDER_XCD_RUNS : v == 2 ? (uint32_t)ORDER_XCD_WINDOWS : 0u;
     }();
     return order;
@@ -229,13 +230,55 @@ void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi
     SF_HIP(hipGetLastError());
 }
 
+template <typename T>
+void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s) {
+    if (m.starts.back() > (int64_t)UINT32_MAX) throw Error(SF_ERR_INTERNAL, "a block matrix too large for 32-bit offsets");
+    out.rb = (int)m.rb;
+    out.cb = (int)m.cb;
+    out.rows = (uint32_t)m.rows;
+    std::vector<uint32_t> starts(m.starts.begin(), m.starts.end());
+    std::vector<T> vals(m.values.size());
+    for (size_t i = 0; i < vals.size(); ++i) vals[i] = (T)m.values[i];
+    out.starts.upload(starts, s);
+    out.cols.upload(m.columns.empty() ? std::vector<uint32_t>{0} : m.columns, s);
+    out.vals.upload(vals.empty() ? std::vector<T>{T(0)} : vals, s);
+}
+template void upload_bsr<double>(DeviceBsr<double> &, const HostBsr<double> &, hipStream_t);
+template void upload_bsr<float>(DeviceBsr<float> &, const HostBsr<double> &, hipStream_t);
+
+template <int RB, int CB, typename S, typename V>
+static void long_rows_shape(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+    const dim3 grid(a.rows), block(WAVE);
+    if (epi == EPI_STORE) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b);
+    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b);
+    else hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b);
+}
+
+template <typename S, typename V>
+void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+    if (a.rows == 0) return;
+    if (a.rb == 6 && a.cb == 3) long_rows_shape<6, 3>(a, x, y, b, epi, s);
+    else if (a.rb == 6 && a.cb == 6) long_rows_shape<6, 6>(a, x, y, b, epi, s);
+    else if (a.rb == 3 && a.cb == 3) long_rows_shape<3, 3>(a, x, y, b, epi, s);
+    else if (a.rb == 3 && a.cb == 6) long_rows_shape<3, 6>(a, x, y, b, epi, s);
+    else throw Error(SF_ERR_INTERNAL, "unsupported block shape");
+    SF_HIP(hipGetLastError());
+}
+template void launch_long_rows<double, double>(const DeviceBsr<double> &, const double *, double *, const double *, int, hipStream_t);
+template void launch_long_rows<float, double>(const DeviceBsr<float> &, const double *, double *, const double *, int, hipStream_t);
+template void launch_long_rows<float, float>(const DeviceBsr<float> &, const float *, float *, const float *, int, hipStream_t);
+
 int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates) {
     static const int forced = [] {
         const char *e = std::getenv("SOLIDFEM_BATCH");
         return e ? std::atoi(e) : 0;
     }();
     for (int c : candidates)
-        if (forced == c) return forced;
+        if (forced > 0 && forced == c) return forced;
+    return fastest(s, launch, candidates);
+}
+
+int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates) {
     hipEvent_t e0, e1;
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
@@ -250,7 +293,7 @@ int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initi
         SF_HIP(hipEventSynchronize(e1));
         float ms = 0;
         SF_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (verbose) std::fprintf(stderr, "[tune] batch %d: %.2f us\n", batch, ms * 1e3f / 3);
+        if (verbose) std::fprintf(stderr, "[tune] variant %d: %.2f us\n", batch, ms * 1e3f / 3);
         if (ms < best_ms) {
             best_ms = ms;
             best = batch;
@@ -300,6 +343,8 @@ struct LevelDev {
     int rb = 3;
     size_t rows = 0, n = 0;
     DeviceSliced<S> a, p, r;
+    DeviceBsr<S> r_rows;          // the same transpose in plain block rows, for long_rows_kernel
+    bool restrict_by_rows = false;  // chosen by timing both restriction kernels
     DeviceBuffer<S> dinv;
     DeviceBuffer<double> lt;      // coarsest: transposed Cholesky factor
     DeviceBuffer<double> packed;  // coarsest: the factor packed by columns, for the LDS solve
@@ -363,6 +408,7 @@ class Hierarchy final : public DeviceMultigrid {
                             for (size_t q = 0; q < P.cb; ++q) t.values[to * bl + q * P.rb + pp] = P.values[(size_t)at * bl + pp * P.cb + q];
                     }
                 upload_sliced(d.r, t, s);
+                upload_bsr(d.r_rows, t, s);
             }
             if (h.has_factor) {
                 d.has_factor = true;
@@ -665,6 +711,11 @@ class Hierarchy final : public DeviceMultigrid {
                 SF_HIP(hipMemsetAsync(c.b.ptr, 0, c.n * sizeof(V), s));
                 SF_HIP(hipMemsetAsync(c.x.ptr, 0, c.n * sizeof(V), s));
                 d.r.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, batch); });
+                // the sliced product at its best depth (1) against one wavefront per row (0)
+                d.restrict_by_rows = fastest(s, [&](int sliced) {
+                    if (sliced) launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
+                    else launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
+                }, {1, 0}) == 0;
                 d.p.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.p, c.x.ptr, d.px.ptr, nullptr, EPI_ADD, s, batch); });
             }
         }
@@ -723,7 +774,8 @@ class Hierarchy final : public DeviceMultigrid {
         int token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
         launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s);
         Profiler::get().end(2, token, s);
-        launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
+        if (d.restrict_by_rows) launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
+        else launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
         cycle(l + 1, c.b.ptr, c.x.ptr, s);
         launch_spmv<S, V>(d.p, c.x.ptr, x, nullptr, EPI_ADD, s);
         token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;

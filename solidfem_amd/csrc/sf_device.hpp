@@ -48,6 +48,21 @@ struct DeviceSliced {
     size_t algorithmic_matrix_bytes() const { return blocks * (rb * cb * sizeof(T) + 4) + (rows + 1) * 4; }
 };
 
+// A block matrix in plain block rows (dev::Bsr) in precision T.
+template <typename T>
+struct DeviceBsr {
+    int rb = 0, cb = 0;
+    uint32_t rows = 0;
+    DeviceBuffer<uint32_t> starts, cols;
+    DeviceBuffer<T> vals;
+    dev::Bsr<T> view() const { return {starts.ptr, cols.ptr, vals.ptr, rows}; }
+};
+template <typename T>
+void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s);
+// y = A x (epilogue per dev::Epilogue) with one wavefront per block row (long_rows_kernel).
+template <typename S, typename V>
+void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s);
+
 // Builds a sliced matrix from row-major f64 blocks already on the device (`src_values`).
 template <typename T>
 void build_sliced(DeviceSliced<T> &out, const SlicedPlan &plan, size_t block_cols, const double *src_values, hipStream_t s);
@@ -62,6 +77,8 @@ void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi
 // The batch depth (blocks in flight per lane, one of `candidates`) whose launches
 // `launch(batch)` run fastest; every depth gives the same bits. SOLIDFEM_BATCH forces one.
 int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates = {1, 2, 4, 8});
+// The candidate whose launches `launch(candidate)` run fastest (timed with HIP events on `s`).
+int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates);
 
 // ---- several right-hand sides ------------------------------------------------------------------
 

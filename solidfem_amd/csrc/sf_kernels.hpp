@@ -153,6 +153,67 @@ __global__ __launch_bounds__(256) void spmv_kernel(Sliced<S> a, const V *__restr
     else y[out] = y[out] + sum;
 }
 
+// A block matrix in plain block rows: row r's blocks at [starts[r], starts[r + 1]), their
+// row-major RB x CB values contiguous. Used for matrices with few, long rows.
+template <typename T>
+struct Bsr {
+    const uint32_t *starts;  // rows + 1
+    const uint32_t *cols;
+    const T *vals;
+    uint32_t rows;
+};
+
+// y = A x with an epilogue (as spmv_kernel) for matrices with few, long rows -- the stored
+// transposes of the prolongators, whose rows gather every fine row of an aggregate's
+// neighbourhood. One wavefront (one workgroup) per block row: its 64 lanes load 64 blocks at
+// once and form their terms A[p][q] x[q] into LDS; then lane p adds its row's terms in stored
+// block order and column order, from zero -- the summation of spmv_kernel and of the reference,
+// so the same bits, with one memory round trip per 64 blocks instead of one per few blocks.
+template <int RB, int CB, typename S, typename V, int EPI>
+__global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__restrict__ x, V *__restrict__ y,
+                                                       const V *__restrict__ b) {
+    __shared__ V terms[RB][WAVE * CB];
+    const uint32_t row = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t start = a.starts[row], len = a.starts[row + 1] - start;
+    V sum = (V)0;
+    for (uint32_t k0 = 0; k0 < len; k0 += WAVE) {
+        const uint32_t k = k0 + lane;
+        if (k < len) {
+            const uint32_t col = a.cols[start + k];
+            const S *blk = a.vals + (size_t)(start + k) * RB * CB;
+            V xv[CB];
+#pragma unroll
+            for (int q = 0; q < CB; ++q) xv[q] = x[(size_t)col * CB + q];
+#pragma unroll
+            for (int p = 0; p < RB; ++p)
+#pragma unroll
+                for (int q = 0; q < CB; ++q) terms[p][lane * CB + q] = widen<V>(blk[p * CB + q]) * xv[q];
+        }
+        __syncthreads();
+        if (lane < RB) {
+            const uint32_t n = (len - k0 < (uint32_t)WAVE ? len - k0 : (uint32_t)WAVE) * CB;
+            const V *t = terms[lane];
+            uint32_t i = 0;
+            for (; i + 8 <= n; i += 8) {
+                V v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = t[i + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) sum = sum + v[u];
+            }
+            for (; i < n; ++i) sum = sum + t[i];
+        }
+        __syncthreads();
+    }
+    if (lane < RB) {
+        const size_t out = (size_t)row * RB + lane;
+        if (EPI == EPI_STORE) y[out] = sum;
+        else if (EPI == EPI_RESIDUAL) y[out] = b[out] - sum;
+        else y[out] = y[out] + sum;
+    }
+}
+
 // Per scalar row, the rounding bound term of CsrMatrix::multiply_rounding_bound
 // (crates/solidfem-solve/src/csr.rs:142-154): (terms * eps) * sum |a x|, squared.
 template <int RB, int CB>
