@@ -173,7 +173,7 @@ def serving_requests(sf, np, count, rate, seed):
 def serving(sf, np, count, rate, seed):
     """Serves the synthetic traffic on one GPU and reports latency percentiles."""
     traffic = serving_requests(sf, np, count, rate, seed)
-    opts = sf.ServerOptions(hbm_budget=200 << 30, max_columns=32, slice_iterations=8)
+    opts = sf.ServerOptions(hbm_budget=200 << 30, max_columns=32, slice_iterations=8, builders=6, solvers=8)
     server = sf.Server.new(0, opts)
     ids = []
     t0 = time.perf_counter()
@@ -189,6 +189,14 @@ def serving(sf, np, count, rate, seed):
     st = server.stats()
     done = [s for s in states if s.state == sf.ServeState.DONE]
     dofs = [s.free_dofs for s in done]
+    built = [s for s in done if not s.cache_hit]
+
+    def quantiles(values):
+        if not values:
+            return None
+        v = np.sort(np.asarray(values))
+        return {"p50_ms": round(float(v[int(np.ceil(0.5 * len(v))) - 1]) * 1e3, 1),
+                "p99_ms": round(float(v[int(np.ceil(0.99 * len(v))) - 1]) * 1e3, 1)}
     server.close()
     log(f"serving: {st['completed']} answered, {st['failed']} failed in {wall:.1f} s; p50 {st['p50_seconds'] * 1e3:.0f} ms, "
         f"p99 {st['p99_seconds'] * 1e3:.0f} ms; {st['builds']} builds, {st['cache_hits']} cache hits, "
@@ -211,6 +219,15 @@ def serving(sf, np, count, rate, seed):
         "wall_s": round(wall, 1),
         "served_dofs_per_s": round(sum(dofs) / wall, 1),
         "median_request_dofs": int(np.median(dofs)) if dofs else 0,
+        "builders": opts.builders,
+        "solver_streams": opts.solvers,
+        "latency_parts": {
+            "queued": quantiles([s.queued_seconds for s in done]),
+            "build_of_new_systems": quantiles([s.build_seconds for s in built]),
+            "solve": quantiles([s.solve_seconds for s in done]),
+            "cache_hit_latency": quantiles([s.latency_seconds for s in done if s.cache_hit]),
+            "new_system_latency": quantiles([s.latency_seconds for s in built]),
+        },
     }
 
 
@@ -223,7 +240,7 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--columns", type=int, default=32, help="load cases of the multi-RHS measurement (0 = skip)")
-    ap.add_argument("--graph", type=int, default=1, help="replay each V-cycle as a captured HIP graph (1) or launch eagerly (0)")
+    ap.add_argument("--graph", type=int, default=0, help="replay each V-cycle as a captured HIP graph (1) or launch eagerly (0)")
     ap.add_argument("--serve-requests", type=int, default=512, help="requests of the serving measurement (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate of the serving measurement, per s")
     args = ap.parse_args()

@@ -790,8 +790,8 @@ class ServerOptions:
     tolerance: float = 1e-8
     max_columns: int = 32
     slice_iterations: int = 8
-    builders: int = 2
-    solvers: int = 4
+    builders: int = 4
+    solvers: int = 8
     dot_order: DotOrder = DotOrder.TREE
 
     def c(self):

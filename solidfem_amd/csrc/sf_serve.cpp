@@ -42,8 +42,8 @@ Server::Server(int device, const sf_server_options &o) : device_(device), option
     if (options_.max_columns > MAX_COLUMNS) throw Error(SF_ERR_INVALID_ARGUMENT, "a batch carries 1 to 64 requests");
     if (options_.slice_iterations == 0) options_.slice_iterations = 8;
     if (options_.tolerance <= 0) options_.tolerance = 1e-8;
-    if (options_.builders <= 0) options_.builders = 2;
-    if (options_.solvers <= 0) options_.solvers = 4;
+    if (options_.builders <= 0) options_.builders = 4;
+    if (options_.solvers <= 0) options_.solvers = 8;
     if (options_.precision < 0 || options_.precision > 2) throw Error(SF_ERR_INVALID_ARGUMENT, "unknown precision");
     if (options_.coarse_solve != SF_COARSE_REFERENCE && options_.coarse_solve != SF_COARSE_INVERSE)
         throw Error(SF_ERR_INVALID_ARGUMENT, "unknown coarse solve");
