@@ -345,6 +345,8 @@ struct LevelDev {
     DeviceSliced<S> a, p, r;
     DeviceBsr<S> r_rows;          // the same transpose in plain block rows, for long_rows_kernel
     bool restrict_by_rows = false;  // chosen by timing both restriction kernels
+    DeviceBsr<S> a_rows;          // coarse levels: A in plain block rows, for the row kernels
+    bool cheb_by_rows = false, residual_by_rows = false;  // chosen by timing
     DeviceBuffer<S> dinv;
     DeviceBuffer<double> lt;      // coarsest: transposed Cholesky factor
     DeviceBuffer<double> packed;  // coarsest: the factor packed by columns, for the LDS solve
@@ -377,6 +379,7 @@ class Hierarchy final : public DeviceMultigrid {
             d.rows = h.a.rows;
             d.n = h.a.dofs();
             upload_sliced(d.a, h.a, s);
+            if (l > 0) upload_bsr(d.a_rows, h.a, s);
             std::vector<S> dinv(h.inverse_diagonal.size());
             for (size_t i = 0; i < dinv.size(); ++i) dinv[i] = (S)h.inverse_diagonal[i];
             d.dinv.upload(dinv, s);
@@ -696,6 +699,15 @@ class Hierarchy final : public DeviceMultigrid {
         smooth_batch(d, d.mres.ptr, d.mpx.ptr, x, M, s);
     }
 
+    static void launch_cheb_rows(LevelDev<S, V> &d, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out, V *x,
+                                 V *into, hipStream_t s) {
+        const dim3 grid(d.a_rows.rows), block(WAVE);
+        if (d.rb == 3)
+            hipLaunchKernelGGL((cheb_rows_kernel<3, S, V>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+        else
+            hipLaunchKernelGGL((cheb_rows_kernel<6, S, V>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+    }
+
     // Picks each level's batch depths by timing its own kernels on its own matrices.
     void tune(hipStream_t s) {
         for (size_t l = 0; l < lv_.size(); ++l) {
@@ -706,6 +718,17 @@ class Hierarchy final : public DeviceMultigrid {
                 launch_cheb_step(d, batch, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
             });
             d.a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, batch); });
+            if (d.a_rows.rows) {
+                // each kernel against its row-per-wavefront form (the same bits)
+                d.cheb_by_rows = fastest(s, [&](int sliced) {
+                    if (sliced) launch_cheb_step(d, d.a.batch_cheb, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
+                    else launch_cheb_rows(d, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
+                }, {1, 0}) == 0;
+                d.residual_by_rows = fastest(s, [&](int sliced) {
+                    if (sliced) launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s);
+                    else launch_long_rows<S, V>(d.a_rows, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s);
+                }, {1, 0}) == 0;
+            }
             if (d.has_p) {
                 LevelDev<S, V> &c = lv_[l + 1];
                 SF_HIP(hipMemsetAsync(c.b.ptr, 0, c.n * sizeof(V), s));
@@ -744,11 +767,18 @@ class Hierarchy final : public DeviceMultigrid {
                                                                        (double)d.rows * d.rb * d.rb * sizeof(S) + 6 * vec +
                                                                        (into ? 2 * vec : 0.0))
                                      : -1;
-            launch_cheb_step(d, d.a.batch_cheb, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+            if (d.cheb_by_rows) launch_cheb_rows(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+            else launch_cheb_step(d, d.a.batch_cheb, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
             Profiler::get().end(0, token, s);
         }
         if (steps == 0 && sum_into)
             hipLaunchKernelGGL((add_kernel<V>), dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, sum_into, (const V *)x);
+    }
+
+    // res = b - A x (cycle.rs:548-564) with the level's faster kernel
+    static void residual(LevelDev<S, V> &d, const V *x, const V *b, hipStream_t s) {
+        if (d.residual_by_rows) launch_long_rows<S, V>(d.a_rows, x, d.res.ptr, b, EPI_RESIDUAL, s);
+        else launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s);
     }
 
     // cycle.rs:461-539
@@ -772,14 +802,14 @@ class Hierarchy final : public DeviceMultigrid {
         const double res_bytes = (double)d.a.algorithmic_matrix_bytes() + 3.0 * d.n * sizeof(V);
         smooth(d, b, x, nullptr, s, finest);
         int token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
-        launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s);
+        residual(d, x, b, s);
         Profiler::get().end(2, token, s);
         if (d.restrict_by_rows) launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
         else launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
         cycle(l + 1, c.b.ptr, c.x.ptr, s);
         launch_spmv<S, V>(d.p, c.x.ptr, x, nullptr, EPI_ADD, s);
         token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
-        launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s);
+        residual(d, x, b, s);
         Profiler::get().end(2, token, s);
         smooth(d, d.res.ptr, d.px.ptr, x, s, finest);
     }

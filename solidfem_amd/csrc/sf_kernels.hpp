@@ -214,6 +214,67 @@ __global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__rest
     }
 }
 
+// The fused Chebyshev step of cheb_step_kernel for levels with few rows, one wavefront (one
+// workgroup) per block row as long_rows_kernel: the lanes form the terms of A d_in for 64 blocks
+// at a time, lane p adds its row's terms in stored order, and the RB residuals of the block row
+// meet in LDS for the block-Jacobi product (summed over q in order from zero). The same bits as
+// cheb_step_kernel.
+template <int RB, typename S, typename V>
+__global__ __launch_bounds__(64) void cheb_rows_kernel(Bsr<S> a, const S *__restrict__ dinv, const V *__restrict__ d_in,
+                                                       const V *__restrict__ r_in, V keep, V step, V *__restrict__ d_out,
+                                                       V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into) {
+    __shared__ V terms[RB][WAVE * RB];
+    __shared__ V r_new[RB];
+    const uint32_t row = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t start = a.starts[row], len = a.starts[row + 1] - start;
+    V sum = (V)0;
+    for (uint32_t k0 = 0; k0 < len; k0 += WAVE) {
+        const uint32_t k = k0 + lane;
+        if (k < len) {
+            const uint32_t col = a.cols[start + k];
+            const S *blk = a.vals + (size_t)(start + k) * RB * RB;
+            V xv[RB];
+#pragma unroll
+            for (int q = 0; q < RB; ++q) xv[q] = d_in[(size_t)col * RB + q];
+#pragma unroll
+            for (int p = 0; p < RB; ++p)
+#pragma unroll
+                for (int q = 0; q < RB; ++q) terms[p][lane * RB + q] = widen<V>(blk[p * RB + q]) * xv[q];
+        }
+        __syncthreads();
+        if (lane < RB) {
+            const uint32_t n = (len - k0 < (uint32_t)WAVE ? len - k0 : (uint32_t)WAVE) * RB;
+            const V *t = terms[lane];
+            uint32_t i = 0;
+            for (; i + 8 <= n; i += 8) {
+                V v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = t[i + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) sum = sum + v[u];
+            }
+            for (; i < n; ++i) sum = sum + t[i];
+        }
+        __syncthreads();
+    }
+    const size_t i = (size_t)row * RB + (lane < RB ? lane : 0);
+    if (lane < RB) r_new[lane] = r_in[i] - sum;
+    __syncthreads();
+    if (lane < RB) {
+        const S *drow = dinv + (size_t)row * RB * RB + (size_t)lane * RB;
+        V z = (V)0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * r_new[q];
+        const V dn = keep * d_in[i] + step * z;
+        const V xn = x[i] + dn;
+        d_out[i] = dn;
+        r_out[i] = r_new[lane];
+        x[i] = xn;
+        if (sum_into) sum_into[i] = sum_into[i] + xn;
+    }
+}
+
 // Per scalar row, the rounding bound term of CsrMatrix::multiply_rounding_bound
 // (crates/solidfem-solve/src/csr.rs:142-154): (terms * eps) * sum |a x|, squared.
 template <int RB, int CB>
