@@ -196,33 +196,34 @@ template void build_sliced<double>(DeviceSliced<double> &, const SlicedPlan &, s
 template void build_sliced<float>(DeviceSliced<float> &, const SlicedPlan &, size_t, const double *, hipStream_t);
 
 template <int RB, int CB, int BATCH, typename S, typename V>
-static void spmv_batch(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+static void spmv_batch(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, Start<S, V> st) {
     const dim3 grid(grid_for(a.slices, 4)), block(256);
-    if (epi == EPI_STORE) hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b);
-    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b);
-    else hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b);
+    if (epi == EPI_STORE) hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b, st);
+    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b, st);
+    else hipLaunchKernelGGL((spmv_kernel<RB, CB, BATCH, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b, st);
 }
 
 template <int RB, int CB, typename S, typename V>
-static void spmv_shape(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch) {
+static void spmv_shape(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch, Start<S, V> st) {
     switch (batch) {
-        case 1: spmv_batch<RB, CB, 1>(a, x, y, b, epi, s); break;
-        case 2: spmv_batch<RB, CB, 2>(a, x, y, b, epi, s); break;
-        case 8: spmv_batch<RB, CB, 8>(a, x, y, b, epi, s); break;
-        default: spmv_batch<RB, CB, 4>(a, x, y, b, epi, s); break;
+        case 1: spmv_batch<RB, CB, 1>(a, x, y, b, epi, s, st); break;
+        case 2: spmv_batch<RB, CB, 2>(a, x, y, b, epi, s, st); break;
+        case 8: spmv_batch<RB, CB, 8>(a, x, y, b, epi, s, st); break;
+        default: spmv_batch<RB, CB, 4>(a, x, y, b, epi, s, st); break;
     }
 }
 
 template <typename S, typename V>
-void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch) {
+void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch, Start<S, V> st) {
     if (a.slices == 0) return;
     if (batch == 0) batch = a.batch_spmv;
-    if (a.rb == 3 && a.cb == 3) spmv_shape<3, 3>(a, x, y, b, epi, s, batch);
-    else if (a.rb == 6 && a.cb == 6) spmv_shape<6, 6>(a, x, y, b, epi, s, batch);
-    else if (a.rb == 3 && a.cb == 6) spmv_shape<3, 6>(a, x, y, b, epi, s, batch);
-    else if (a.rb == 6 && a.cb == 3) spmv_shape<6, 3>(a, x, y, b, epi, s, batch);
+    if (st.dinv && a.rb != 3 && a.rb != 6) throw Error(SF_ERR_INTERNAL, "a fused start needs 3 or 6 rows per block");
+    if (a.rb == 3 && a.cb == 3) spmv_shape<3, 3>(a, x, y, b, epi, s, batch, st);
+    else if (a.rb == 6 && a.cb == 6) spmv_shape<6, 6>(a, x, y, b, epi, s, batch, st);
+    else if (a.rb == 3 && a.cb == 6) spmv_shape<3, 6>(a, x, y, b, epi, s, batch, st);
+    else if (a.rb == 6 && a.cb == 3) spmv_shape<6, 3>(a, x, y, b, epi, s, batch, st);
     else if constexpr (std::is_same<S, double>::value && std::is_same<V, double>::value) {
-        if (a.rb == 1 && a.cb == 1) spmv_shape<1, 1>(a, x, y, b, epi, s, batch);
+        if (a.rb == 1 && a.cb == 1) spmv_shape<1, 1>(a, x, y, b, epi, s, batch, st);
         else throw Error(SF_ERR_INTERNAL, "unsupported block shape");
     } else {
         throw Error(SF_ERR_INTERNAL, "unsupported block shape");
@@ -247,26 +248,29 @@ template void upload_bsr<double>(DeviceBsr<double> &, const HostBsr<double> &, h
 template void upload_bsr<float>(DeviceBsr<float> &, const HostBsr<double> &, hipStream_t);
 
 template <int RB, int CB, typename S, typename V>
-static void long_rows_shape(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+static void long_rows_shape(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, Start<S, V> st) {
     const dim3 grid(a.rows), block(WAVE);
-    if (epi == EPI_STORE) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b);
-    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b);
-    else hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b);
+    if (epi == EPI_STORE) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b, st);
+    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b, st);
+    else hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b, st);
 }
 
 template <typename S, typename V>
-void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, Start<S, V> st) {
     if (a.rows == 0) return;
-    if (a.rb == 6 && a.cb == 3) long_rows_shape<6, 3>(a, x, y, b, epi, s);
-    else if (a.rb == 6 && a.cb == 6) long_rows_shape<6, 6>(a, x, y, b, epi, s);
-    else if (a.rb == 3 && a.cb == 3) long_rows_shape<3, 3>(a, x, y, b, epi, s);
-    else if (a.rb == 3 && a.cb == 6) long_rows_shape<3, 6>(a, x, y, b, epi, s);
+    if (a.rb == 6 && a.cb == 3) long_rows_shape<6, 3>(a, x, y, b, epi, s, st);
+    else if (a.rb == 6 && a.cb == 6) long_rows_shape<6, 6>(a, x, y, b, epi, s, st);
+    else if (a.rb == 3 && a.cb == 3) long_rows_shape<3, 3>(a, x, y, b, epi, s, st);
+    else if (a.rb == 3 && a.cb == 6) long_rows_shape<3, 6>(a, x, y, b, epi, s, st);
     else throw Error(SF_ERR_INTERNAL, "unsupported block shape");
     SF_HIP(hipGetLastError());
 }
-template void launch_long_rows<double, double>(const DeviceBsr<double> &, const double *, double *, const double *, int, hipStream_t);
-template void launch_long_rows<float, double>(const DeviceBsr<float> &, const double *, double *, const double *, int, hipStream_t);
-template void launch_long_rows<float, float>(const DeviceBsr<float> &, const float *, float *, const float *, int, hipStream_t);
+template void launch_long_rows<double, double>(const DeviceBsr<double> &, const double *, double *, const double *, int, hipStream_t,
+                                               Start<double, double>);
+template void launch_long_rows<float, double>(const DeviceBsr<float> &, const double *, double *, const double *, int, hipStream_t,
+                                              Start<float, double>);
+template void launch_long_rows<float, float>(const DeviceBsr<float> &, const float *, float *, const float *, int, hipStream_t,
+                                             Start<float, float>);
 
 int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates) {
     static const int forced = [] {
@@ -314,9 +318,12 @@ void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s) {
     SF_HIP(hipStreamSynchronize(s));
 }
 
-template void launch_spmv<double, double>(const DeviceSliced<double> &, const double *, double *, const double *, int, hipStream_t, int);
-template void launch_spmv<float, double>(const DeviceSliced<float> &, const double *, double *, const double *, int, hipStream_t, int);
-template void launch_spmv<float, float>(const DeviceSliced<float> &, const float *, float *, const float *, int, hipStream_t, int);
+template void launch_spmv<double, double>(const DeviceSliced<double> &, const double *, double *, const double *, int, hipStream_t, int,
+                                          Start<double, double>);
+template void launch_spmv<float, double>(const DeviceSliced<float> &, const double *, double *, const double *, int, hipStream_t, int,
+                                         Start<float, double>);
+template void launch_spmv<float, float>(const DeviceSliced<float> &, const float *, float *, const float *, int, hipStream_t, int,
+                                        Start<float, float>);
 template void tune_spmv<double, double>(DeviceSliced<double> &, int, hipStream_t);
 
 // ---- the device hierarchy -------------------------------------------------------------------
@@ -512,10 +519,23 @@ class Hierarchy final : public DeviceMultigrid {
         if constexpr (std::is_same<V, double>::value) {
             cycle(0, r, z, s);
         } else {
-            const size_t n = lv_[0].n;
-            hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, r, in_.ptr);
-            cycle(0, in_.ptr, out_.ptr, s);
-            hipLaunchKernelGGL((convert_kernel<double, V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const V *)out_.ptr, z);
+            LevelDev<S, V> &d = lv_[0];
+            const size_t n = d.n;
+            if (d.has_p) {
+                // the rounding in, fused with the first start; the widening out, fused with the last step
+                const unsigned g = grid_for(d.rows, 256);
+                if (d.rb == 3)
+                    hipLaunchKernelGGL((convert_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr, d.dinv.ptr,
+                                       (V)d.theta, d.d0.ptr, out_.ptr);
+                else
+                    hipLaunchKernelGGL((convert_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr, d.dinv.ptr,
+                                       (V)d.theta, d.d0.ptr, out_.ptr);
+                cycle(0, in_.ptr, out_.ptr, s, true, z);
+            } else {
+                hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, r, in_.ptr);
+                cycle(0, in_.ptr, out_.ptr, s);
+                hipLaunchKernelGGL((convert_kernel<double, V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const V *)out_.ptr, z);
+            }
         }
         SF_HIP(hipGetLastError());
     }
@@ -560,24 +580,24 @@ class Hierarchy final : public DeviceMultigrid {
    private:
     template <int RB, int BATCH>
     static void cheb_step_launch(LevelDev<S, V> &d, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out, V *x,
-                                 V *into, hipStream_t s) {
+                                 V *into, hipStream_t s, double *widen) {
         hipLaunchKernelGGL((cheb_step_kernel<RB, BATCH, S, V>), dim3(grid_for(d.a.slices, 4)), dim3(256), 0, s, d.a.view(),
-                           d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+                           d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into, widen);
     }
     template <int RB>
     static void cheb_step_rb(LevelDev<S, V> &d, int batch, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out,
-                             V *x, V *into, hipStream_t s) {
+                             V *x, V *into, hipStream_t s, double *widen) {
         switch (batch) {
-            case 1: cheb_step_launch<RB, 1>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
-            case 2: cheb_step_launch<RB, 2>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
-            case 8: cheb_step_launch<RB, 8>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
-            default: cheb_step_launch<RB, 4>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+            case 1: cheb_step_launch<RB, 1>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen); break;
+            case 2: cheb_step_launch<RB, 2>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen); break;
+            case 8: cheb_step_launch<RB, 8>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen); break;
+            default: cheb_step_launch<RB, 4>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen); break;
         }
     }
     static void launch_cheb_step(LevelDev<S, V> &d, int batch, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out,
-                                 V *x, V *into, hipStream_t s) {
-        if (d.rb == 3) cheb_step_rb<3>(d, batch, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
-        else cheb_step_rb<6>(d, batch, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+                                 V *x, V *into, hipStream_t s, double *widen = nullptr) {
+        if (d.rb == 3) cheb_step_rb<3>(d, batch, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen);
+        else cheb_step_rb<6>(d, batch, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen);
     }
 
     // ---- several columns ----
@@ -700,12 +720,14 @@ class Hierarchy final : public DeviceMultigrid {
     }
 
     static void launch_cheb_rows(LevelDev<S, V> &d, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out, V *x,
-                                 V *into, hipStream_t s) {
+                                 V *into, hipStream_t s, double *widen = nullptr) {
         const dim3 grid(d.a_rows.rows), block(WAVE);
         if (d.rb == 3)
-            hipLaunchKernelGGL((cheb_rows_kernel<3, S, V>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+            hipLaunchKernelGGL((cheb_rows_kernel<3, S, V>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out,
+                               r_out, x, into, widen);
         else
-            hipLaunchKernelGGL((cheb_rows_kernel<6, S, V>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+            hipLaunchKernelGGL((cheb_rows_kernel<6, S, V>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out,
+                               r_out, x, into, widen);
     }
 
     // Picks each level's batch depths by timing its own kernels on its own matrices.
@@ -746,12 +768,17 @@ class Hierarchy final : public DeviceMultigrid {
     }
 
     // cycle.rs:570-651: start, then degree - 1 fused steps; `sum_into` takes the final add.
-    void smooth(LevelDev<S, V> &d, const V *b, V *x, V *sum_into, hipStream_t s, bool finest = false) {
+    // `started`: the start already ran, fused into the kernel that produced b. `widen`: the last
+    // step also writes the sum into f64 (the V-cycle's widened output).
+    void smooth(LevelDev<S, V> &d, const V *b, V *x, V *sum_into, hipStream_t s, bool finest = false, bool started = false,
+                double *widen = nullptr) {
         const unsigned g = grid_for(d.n, 256);
-        if (d.rb == 3)
-            hipLaunchKernelGGL((cheb_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
-        else
-            hipLaunchKernelGGL((cheb_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
+        if (!started) {
+            if (d.rb == 3)
+                hipLaunchKernelGGL((cheb_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
+            else
+                hipLaunchKernelGGL((cheb_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
+        }
         V *dd[2] = {d.d0.ptr, d.d1.ptr};
         V *rr[2] = {d.r0.ptr, d.r1.ptr};
         const size_t steps = d.steps.size();
@@ -762,27 +789,36 @@ class Hierarchy final : public DeviceMultigrid {
             V *r_out = rr[k & 1];
             const V keep = (V)d.steps[k - 1].first, stp = (V)d.steps[k - 1].second;
             V *into = k == steps ? sum_into : nullptr;
+            double *wide = into ? widen : nullptr;
             const double vec = (double)d.n * sizeof(V);
             const int token = finest ? Profiler::get().begin(0, s, (double)d.a.algorithmic_matrix_bytes() +
                                                                        (double)d.rows * d.rb * d.rb * sizeof(S) + 6 * vec +
-                                                                       (into ? 2 * vec : 0.0))
+                                                                       (into ? 2 * vec : 0.0) + (wide ? (double)d.n * 8 : 0.0))
                                      : -1;
-            if (d.cheb_by_rows) launch_cheb_rows(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
-            else launch_cheb_step(d, d.a.batch_cheb, d_in, r_in, keep, stp, d_out, r_out, x, into, s);
+            if (d.cheb_by_rows) launch_cheb_rows(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, wide);
+            else launch_cheb_step(d, d.a.batch_cheb, d_in, r_in, keep, stp, d_out, r_out, x, into, s, wide);
             Profiler::get().end(0, token, s);
         }
-        if (steps == 0 && sum_into)
+        if (steps == 0 && sum_into) {
             hipLaunchKernelGGL((add_kernel<V>), dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, sum_into, (const V *)x);
+            if (widen)
+                hipLaunchKernelGGL((convert_kernel<double, V>), dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, (const V *)sum_into, widen);
+        }
     }
 
-    // res = b - A x (cycle.rs:548-564) with the level's faster kernel
-    static void residual(LevelDev<S, V> &d, const V *x, const V *b, hipStream_t s) {
-        if (d.residual_by_rows) launch_long_rows<S, V>(d.a_rows, x, d.res.ptr, b, EPI_RESIDUAL, s);
-        else launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s);
+    // res = b - A x (cycle.rs:548-564) with the level's faster kernel, optionally with the start of
+    // the smoothing that follows
+    static void residual(LevelDev<S, V> &d, const V *x, const V *b, hipStream_t s, Start<S, V> st = {nullptr, (V)0, nullptr, nullptr}) {
+        if (d.residual_by_rows) launch_long_rows<S, V>(d.a_rows, x, d.res.ptr, b, EPI_RESIDUAL, s, st);
+        else launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s, 0, st);
     }
+    // Whether a level's cycle begins with a smoothing (and so with a start that can be fused).
+    static bool smooths_first(const LevelDev<S, V> &d) { return d.has_p || !d.has_factor; }
+    static Start<S, V> start_of(LevelDev<S, V> &d, V *x) { return {d.dinv.ptr, (V)d.theta, d.d0.ptr, x}; }
 
-    // cycle.rs:461-539
-    void cycle(size_t l, const V *b, V *x, hipStream_t s) {
+    // cycle.rs:461-539. `started`: this level's first start ran fused upstream; `widen`: the cycle's
+    // output is also written to f64 there (finest level only).
+    void cycle(size_t l, const V *b, V *x, hipStream_t s, bool started = false, double *widen = nullptr) {
         LevelDev<S, V> &d = lv_[l];
         if (!d.has_p) {
             if (d.has_factor) {
@@ -793,25 +829,29 @@ class Hierarchy final : public DeviceMultigrid {
                 else
                     hipLaunchKernelGGL(coarse_solve_kernel<V>, dim3(1), dim3(WAVE), 0, s, (uint32_t)d.n, d.lt.ptr, b, x, 1u);
             } else {
-                smooth(d, b, x, nullptr, s);
+                smooth(d, b, x, nullptr, s, false, started);
             }
             return;
         }
         LevelDev<S, V> &c = lv_[l + 1];
         const bool finest = l == 0;
         const double res_bytes = (double)d.a.algorithmic_matrix_bytes() + 3.0 * d.n * sizeof(V);
-        smooth(d, b, x, nullptr, s, finest);
+        smooth(d, b, x, nullptr, s, finest, started);
         int token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
         residual(d, x, b, s);
         Profiler::get().end(2, token, s);
-        if (d.restrict_by_rows) launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
-        else launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
-        cycle(l + 1, c.b.ptr, c.x.ptr, s);
+        // restriction, with the coarser level's first start when its cycle smooths first
+        const bool fuse = smooths_first(c);
+        const Start<S, V> st = fuse ? start_of(c, c.x.ptr) : Start<S, V>{nullptr, (V)0, nullptr, nullptr};
+        if (d.restrict_by_rows) launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, st);
+        else launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, 0, st);
+        cycle(l + 1, c.b.ptr, c.x.ptr, s, fuse);
         launch_spmv<S, V>(d.p, c.x.ptr, x, nullptr, EPI_ADD, s);
-        token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
-        residual(d, x, b, s);
+        // the residual for the post-smoothing, with its start
+        token = finest ? Profiler::get().begin(2, s, res_bytes + (double)d.rows * d.rb * d.rb * sizeof(S) + 2.0 * d.n * sizeof(V)) : -1;
+        residual(d, x, b, s, start_of(d, d.px.ptr));
         Profiler::get().end(2, token, s);
-        smooth(d, d.res.ptr, d.px.ptr, x, s, finest);
+        smooth(d, d.res.ptr, d.px.ptr, x, s, finest, true, widen);
     }
 
     void compute_stats() {

@@ -61,7 +61,8 @@ template <typename T>
 void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s);
 // y = A x (epilogue per dev::Epilogue) with one wavefront per block row (long_rows_kernel).
 template <typename S, typename V>
-void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s);
+void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s,
+                      dev::Start<S, V> start = {nullptr, (V)0, nullptr, nullptr});
 
 // Builds a sliced matrix from row-major f64 blocks already on the device (`src_values`).
 template <typename T>
@@ -72,7 +73,8 @@ void upload_sliced(DeviceSliced<T> &out, const HostBsr<double> &m, hipStream_t s
 
 // y = A x (epilogue per dev::Epilogue) for any supported block shape, with the matrix's batch.
 template <typename S, typename V>
-void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch = 0);
+void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch = 0,
+                 dev::Start<S, V> start = {nullptr, (V)0, nullptr, nullptr});
 
 // The batch depth (blocks in flight per lane, one of `candidates`) whose launches
 // `launch(batch)` run fastest; every depth gives the same bits. SOLIDFEM_BATCH forces one.

@@ -130,27 +130,57 @@ __device__ __forceinline__ V row_product(const Sliced<S> &a, uint32_t slot0, uin
 
 enum Epilogue { EPI_STORE = 0, EPI_RESIDUAL = 1, EPI_ADD = 2 };
 
+// The Chebyshev start (cycle.rs:591-610) on a vector a kernel has just produced, so that the
+// smoothing that follows needs no launch of its own: z = D^-1 y summed over the block row in
+// order from zero, d = z / theta, x = d. A null `dinv` means no start.
+template <typename S, typename V>
+struct Start {
+    const S *dinv;
+    V theta;
+    V *d;
+    V *x;
+};
+
 // y = A x (STORE), y = b - A x (RESIDUAL) or y = y + A x (ADD): the block products, the
 // residual (crates/solidfem-solve/src/multigrid/cycle.rs:548-564), the gather restriction over
 // the stored transpose (crates/solidfem-solve/src/gpu/layout.rs:249-291) and the prolongation
-// with correction (cycle.rs:512-521).
+// with correction (cycle.rs:512-521); optionally followed by the Chebyshev start on y, the RB
+// values of a block row shuffled in from its lanes.
 template <int RB, int CB, int BATCH, typename S, typename V, int EPI>
 __global__ __launch_bounds__(256) void spmv_kernel(Sliced<S> a, const V *__restrict__ x, V *__restrict__ y,
-                                                   const V *__restrict__ b) {
+                                                   const V *__restrict__ b, Start<S, V> start) {
     constexpr int NPW = Shape<RB>::NPW;
     const uint32_t slice = slice_of(a.order);
-    if (slice >= a.slices) return;
+    if (slice >= a.slices) return;  // whole wave leaves together
     const int lane = threadIdx.x % WAVE;
     const int j = lane / RB, p = lane - j * RB;
-    if (j >= NPW) return;
-    const uint32_t pos = slice * NPW + j;
-    const uint32_t row = a.perm[pos];
-    if (row == NONE) return;
-    const V sum = row_product<RB, CB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, x);
-    const size_t out = (size_t)row * RB + p;
-    if (EPI == EPI_STORE) y[out] = sum;
-    else if (EPI == EPI_RESIDUAL) y[out] = b[out] - sum;
-    else y[out] = y[out] + sum;
+    const bool in_wave = j < NPW;
+    const uint32_t pos = slice * NPW + (in_wave ? j : 0);
+    const uint32_t row = in_wave ? a.perm[pos] : NONE;
+    const bool live = row != NONE;
+    const size_t out = live ? (size_t)row * RB + p : 0;
+    V value = (V)0;
+    if (live) {
+        const V sum = row_product<RB, CB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, x);
+        if (EPI == EPI_STORE) value = sum;
+        else if (EPI == EPI_RESIDUAL) value = b[out] - sum;
+        else value = y[out] + sum;
+        y[out] = value;
+    }
+    if (start.dinv) {  // uniform: every lane takes part in the shuffle
+        const S *drow = start.dinv + (live ? (size_t)row * RB * RB + (size_t)p * RB : 0);
+        V z = (V)0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const V yq = __shfl(value, (in_wave ? j * RB : 0) + q, WAVE);
+            if (live) z = z + widen<V>(drow[q]) * yq;
+        }
+        if (live) {
+            const V dv = z / start.theta;
+            start.d[out] = dv;
+            start.x[out] = dv;
+        }
+    }
 }
 
 // A block matrix in plain block rows: row r's blocks at [starts[r], starts[r + 1]), their
@@ -171,8 +201,9 @@ struct Bsr {
 // so the same bits, with one memory round trip per 64 blocks instead of one per few blocks.
 template <int RB, int CB, typename S, typename V, int EPI>
 __global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__restrict__ x, V *__restrict__ y,
-                                                       const V *__restrict__ b) {
+                                                       const V *__restrict__ b, Start<S, V> then) {
     __shared__ V terms[RB][WAVE * CB];
+    __shared__ V values[RB];
     const uint32_t row = blockIdx.x;
     const int lane = threadIdx.x;
     const uint32_t start = a.starts[row], len = a.starts[row + 1] - start;
@@ -206,11 +237,26 @@ __global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__rest
         }
         __syncthreads();
     }
+    const size_t out = (size_t)row * RB + (lane < RB ? lane : 0);
     if (lane < RB) {
-        const size_t out = (size_t)row * RB + lane;
-        if (EPI == EPI_STORE) y[out] = sum;
-        else if (EPI == EPI_RESIDUAL) y[out] = b[out] - sum;
-        else y[out] = y[out] + sum;
+        V value;
+        if (EPI == EPI_STORE) value = sum;
+        else if (EPI == EPI_RESIDUAL) value = b[out] - sum;
+        else value = y[out] + sum;
+        y[out] = value;
+        values[lane] = value;
+    }
+    if (then.dinv) {  // uniform
+        __syncthreads();
+        if (lane < RB) {
+            const S *drow = then.dinv + (size_t)row * RB * RB + (size_t)lane * RB;
+            V z = (V)0;
+#pragma unroll
+            for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * values[q];
+            const V dv = z / then.theta;
+            then.d[out] = dv;
+            then.x[out] = dv;
+        }
     }
 }
 
@@ -222,7 +268,8 @@ __global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__rest
 template <int RB, typename S, typename V>
 __global__ __launch_bounds__(64) void cheb_rows_kernel(Bsr<S> a, const S *__restrict__ dinv, const V *__restrict__ d_in,
                                                        const V *__restrict__ r_in, V keep, V step, V *__restrict__ d_out,
-                                                       V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into) {
+                                                       V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into,
+                                                       double *__restrict__ widen_into) {
     __shared__ V terms[RB][WAVE * RB];
     __shared__ V r_new[RB];
     const uint32_t row = blockIdx.x;
@@ -271,7 +318,11 @@ __global__ __launch_bounds__(64) void cheb_rows_kernel(Bsr<S> a, const S *__rest
         d_out[i] = dn;
         r_out[i] = r_new[lane];
         x[i] = xn;
-        if (sum_into) sum_into[i] = sum_into[i] + xn;
+        if (sum_into) {
+            const V total = sum_into[i] + xn;
+            sum_into[i] = total;
+            if (widen_into) widen_into[i] = (double)total;
+        }
     }
 }
 
@@ -323,7 +374,8 @@ __global__ __launch_bounds__(256) void cheb_start_kernel(uint32_t rows, const S 
 template <int RB, int BATCH, typename S, typename V>
 __global__ __launch_bounds__(256) void cheb_step_kernel(Sliced<S> a, const S *__restrict__ dinv, const V *__restrict__ d_in,
                                                         const V *__restrict__ r_in, V keep, V step, V *__restrict__ d_out,
-                                                        V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into) {
+                                                        V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into,
+                                                        double *__restrict__ widen_into) {
     constexpr int NPW = Shape<RB>::NPW;
     const uint32_t slice = slice_of(a.order);
     if (slice >= a.slices) return;  // whole wave leaves together
@@ -351,7 +403,11 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(Sliced<S> a, const S *__
     d_out[i] = dn;
     r_out[i] = r_new;
     x[i] = xn;
-    if (sum_into) sum_into[i] = sum_into[i] + xn;
+    if (sum_into) {
+        const V total = sum_into[i] + xn;
+        sum_into[i] = total;
+        if (widen_into) widen_into[i] = (double)total;  // the cycle's correction, widened out
+    }
 }
 
 // x = x + y (cycle.rs:541-545), for a post-smoothing of degree one.
@@ -359,6 +415,32 @@ template <typename V>
 __global__ void add_kernel(size_t n, V *__restrict__ x, const V *__restrict__ y) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] = x[i] + y[i];
+}
+
+// The finest level's right-hand side rounded in from f64 (the conversion of cycle.rs:226-243)
+// and the Chebyshev start on it (cheb_start_kernel), one thread per block row.
+template <int RB, typename S, typename V>
+__global__ __launch_bounds__(256) void convert_start_kernel(uint32_t rows, const double *__restrict__ r, V *__restrict__ b,
+                                                            const S *__restrict__ dinv, V theta, V *__restrict__ d,
+                                                            V *__restrict__ x) {
+    const size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    V bv[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        bv[q] = (V)r[row * RB + q];
+        b[row * RB + q] = bv[q];
+    }
+#pragma unroll
+    for (int p = 0; p < RB; ++p) {
+        const S *dr = dinv + row * RB * RB + p * RB;
+        V z = (V)0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) z = z + widen<V>(dr[q]) * bv[q];
+        const V dv = z / theta;
+        d[row * RB + p] = dv;
+        x[row * RB + p] = dv;
+    }
 }
 
 template <typename To, typename From>
