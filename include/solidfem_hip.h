@@ -412,6 +412,25 @@ double sf_multigrid_cycle_bytes(const sf_multigrid *mg);
 /* Restarts a solve from a zero estimate (Pcg::new semantics, pcg.rs:135-164) on the buffers it
  * already holds; new_rhs NULL keeps the right-hand side. Allocates nothing. */
 sf_status sf_pcg_restart(sf_pcg *pcg, const double *new_rhs);
+/* ---- stress recovery -------------------------------------------------------------------------
+ * Recovery (crates/solidfem-elastic/src/results.rs:307-576 with element.rs:135-192 and patch.rs),
+ * on the device, for nodal displacements (3 per mesh node, fixed components included):
+ *   point_values: per element and integration point, in (element, point) order, 10 values --
+ *     position (3), stress [xx, yy, zz, xy, yz, zx] (6), von Mises (1) -- PointStress;
+ *   nodal_values: per mesh node 7 values -- the recovered nodal stress (6), from the vertex patch
+ *     fits, midside means and the worst incident value where a patch is not fitted, and its von
+ *     Mises (1) -- Solution::nodal_stress and nodal_von_mises.
+ * Either output may be NULL. An inverted element fails with SF_ERR_MODEL_INVERTED_JACOBIAN
+ * (a = the first such element). */
+sf_status sf_recover_stress(sf_context *ctx, const sf_model *model, const double *displacements, double *point_values,
+                            double *nodal_values);
+/* The same for a study's current displacements (Recovery as StaticStudy drives it, study.rs). */
+sf_status sf_static_study_recover(sf_static_study *study, double *point_values, double *nodal_values);
+/* A study's mesh nodes, elements and integration points per element. */
+sf_status sf_static_study_counts(const sf_static_study *study, uint64_t *nodes, uint64_t *elements, uint64_t *points);
+/* Integration points per element of a model's rule (4 or 14). */
+uint64_t sf_model_points_per_element(const sf_model *model);
+
 /* ---- several load cases over one matrix and one hierarchy ------------------------------------
  * No reference counterpart: the reference solves one right-hand side per Pcg
  * (crates/solidfem-solve/src/pcg.rs:113-283). A batch runs that recurrence for up to 64

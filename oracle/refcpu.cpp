@@ -9,6 +9,7 @@
 #include <memory>
 
 #include "refcpu_multigrid.hpp"
+#include "refcpu_recovery.hpp"
 
 namespace refcpu {
 
@@ -508,6 +509,33 @@ void ref_problem_rigid_modes(const ref_problem *p, double *out) {
     auto modes = free_rigid_body_modes(p->p->mesh.nodes, p->p->free, p->p->free_count);
     std::copy(modes.begin(), modes.end(), out);
 }
+// Stress recovery of nodal displacements `u` (3 per node), results.rs:307-576: per (element,
+// point) 10 values (position, stress, von Mises) and per node 7 (stress, von Mises). Returns 0,
+// or -1 with the failing element in *fault_tet.
+int ref_recover(const ref_problem *p, const double *u, double *point_values, double *nodal_values, int64_t *fault_tet) {
+    const Problem &pr = *p->p;
+    std::vector<double> disp(u, u + 3 * pr.mesh.nodes.size());
+    RecoveredField f = recover(pr.mesh, pr.material, pr.quadrature, disp);
+    if (f.fault_tet >= 0) {
+        if (fault_tet) *fault_tet = f.fault_tet;
+        return -1;
+    }
+    const size_t points = f.point_von_mises.size();
+    if (point_values)
+        for (size_t k = 0; k < points; ++k) {
+            for (int a = 0; a < 3; ++a) point_values[10 * k + a] = f.point_position[3 * k + a];
+            for (int c = 0; c < 6; ++c) point_values[10 * k + 3 + c] = f.point_stress[6 * k + c];
+            point_values[10 * k + 9] = f.point_von_mises[k];
+        }
+    if (nodal_values)
+        for (size_t n = 0; n < f.nodal_von_mises.size(); ++n) {
+            for (int c = 0; c < 6; ++c) nodal_values[7 * n + c] = f.nodal_stress[6 * n + c];
+            nodal_values[7 * n + 6] = f.nodal_von_mises[n];
+        }
+    return 0;
+}
+int ref_problem_quadrature(const ref_problem *p) { return p->p->quadrature; }
+
 int ref_problem_element_stiffness(const ref_problem *p, int64_t tet, double *ke) {
     const auto &t = p->p->mesh.tets[(size_t)tet];
     Vec3 en[10];

@@ -117,6 +117,8 @@ void ref_problem_rhs(const ref_problem *p, double *out);
 void ref_problem_rigid_modes(const ref_problem *p, double *out); /* 6 per free dof */
 /* element stiffness of element `tet` (900 values, row-major) */
 int ref_problem_element_stiffness(const ref_problem *p, int64_t tet, double *ke);
+int ref_recover(const ref_problem *p, const double *u, double *point_values, double *nodal_values, int64_t *fault_tet);
+int ref_problem_quadrature(const ref_problem *p);
 double ref_problem_assembly_seconds(const ref_problem *p);
 
 /* Multigrid hierarchy built over the problem's matrix. */

@@ -321,6 +321,25 @@ class Model(_Handle):
         _raise(library().sf_model_element_stiffness(ctx._ptr, self._ptr, first, count, _ptr(out, C.c_double)))
         return out
 
+    def points_per_element(self) -> int:
+        return library().sf_model_points_per_element(self._ptr)
+
+    def recover(self, ctx: "Context", displacements):
+        """Stress recovery on the device (results.rs:307-576): (point_values (elements, points, 10):
+        position, stress [xx, yy, zz, xy, yz, zx], von Mises; nodal_values (nodes, 7): recovered
+        stress and von Mises)."""
+        u = np.ascontiguousarray(displacements, dtype=np.float64).reshape(-1)
+        nodes = self.mesh().nodes().shape[0]
+        if u.size != 3 * nodes:
+            raise ValueError("displacements must hold 3 values per mesh node")
+        q = self.points_per_element()
+        elements = self.mesh().tets().shape[0]
+        points = np.empty((elements, q, 10))
+        nodal = np.empty((nodes, 7))
+        _raise(library().sf_recover_stress(ctx._ptr, self._ptr, _ptr(u, C.c_double), _ptr(points, C.c_double),
+                                           _ptr(nodal, C.c_double)))
+        return points, nodal
+
     def load_vector(self) -> np.ndarray:
         """The right-hand side over the free unknowns of this model's loads (study.rs:444-449,
         :696-735), without building the matrix pattern."""
@@ -418,6 +437,16 @@ class StaticStudy(_Handle):
         _raise(lib.sf_static_study_matrix(self._ptr, None, None, _ptr(rs, C.c_int64), _ptr(cols, C.c_uint32),
                                           _ptr(vals, C.c_double)))
         return rs, cols, vals
+
+    def recover(self):
+        """Stress recovery of the current displacements on the device (results.rs:307-576):
+        (point_values (elements, points, 10), nodal_values (nodes, 7)), as Model.recover."""
+        nodes, elements, points_per = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _raise(library().sf_static_study_counts(self._ptr, C.byref(nodes), C.byref(elements), C.byref(points_per)))
+        points = np.empty((elements.value, points_per.value, 10))
+        nodal = np.empty((nodes.value, 7))
+        _raise(library().sf_static_study_recover(self._ptr, _ptr(points, C.c_double), _ptr(nodal, C.c_double)))
+        return points, nodal
 
     def rhs(self) -> np.ndarray:
         out = np.empty(self.report().free_dofs)

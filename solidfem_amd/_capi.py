@@ -204,6 +204,10 @@ SIGNATURES = {
     "sf_stream_read": (i32, [vp, u64, u32, P(f64)]),
     "sf_multigrid_apply_batch": (i32, [vp, P(f64), u64, P(f64)]),
     "sf_multigrid_set_graph": (i32, [vp, i32]),
+    "sf_recover_stress": (i32, [vp, vp, P(f64), P(f64), P(f64)]),
+    "sf_static_study_recover": (i32, [vp, P(f64), P(f64)]),
+    "sf_model_points_per_element": (u64, [vp]),
+    "sf_static_study_counts": (i32, [vp, P(u64), P(u64), P(u64)]),
     "sf_pcg_batch_create": (i32, [vp, vp, vp, P(f64), u64, P(PcgOptions), P(vp)]),
     "sf_pcg_batch_step": (i32, [vp, u64, CANCEL_FN, vp, P(ColumnProgress), P(u64)]),
     "sf_pcg_batch_progress": (i32, [vp, P(ColumnProgress)]),

@@ -220,6 +220,287 @@ __global__ void gather_kernel(size_t blocks, const int64_t *__restrict__ cstart,
     for (int i = 0; i < 9; ++i) out[i] = v[i];
 }
 
+// ---- stress recovery (crates/solidfem-elastic/src/results.rs:307-576) --------------------------
+
+// crates/solidfem-elastic/src/shape.rs:9-20
+__constant__ double NODE_NATURAL[10][3] = {{0.0, 0.0, 0.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0},
+                                           {0.5, 0.0, 0.0}, {0.5, 0.5, 0.0}, {0.0, 0.5, 0.0}, {0.0, 0.0, 0.5},
+                                           {0.5, 0.0, 0.5}, {0.0, 0.5, 0.5}};
+
+// point_geometry (element.rs:25-80) for one thread: the element's gradients and position at a
+// natural point, the same operations in the same order as element_kernel. False when inverted.
+__device__ bool thread_geometry(const double xe[10][3], const double nat[3], double grad[10][3], double pos[3]) {
+    double l[4] = {1.0 - nat[0] - nat[1] - nat[2], nat[0], nat[1], nat[2]};
+    double j[9];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double s = 0.0;
+            for (int a = 0; a < 10; ++a) s += shape_derivative(a, i, l) * xe[a][k];
+            j[i * 3 + k] = s;
+        }
+    for (int i = 0; i < 3; ++i) {
+        double s = 0.0;
+        for (int a = 0; a < 10; ++a) s += shape_value(a, l) * xe[a][i];
+        pos[i] = s;
+    }
+    double cof[9];
+    cof[0] = j[4] * j[8] - j[5] * j[7];
+    cof[1] = j[2] * j[7] - j[1] * j[8];
+    cof[2] = j[1] * j[5] - j[2] * j[4];
+    cof[3] = j[5] * j[6] - j[3] * j[8];
+    cof[4] = j[0] * j[8] - j[2] * j[6];
+    cof[5] = j[2] * j[3] - j[0] * j[5];
+    cof[6] = j[3] * j[7] - j[4] * j[6];
+    cof[7] = j[1] * j[6] - j[0] * j[7];
+    cof[8] = j[0] * j[4] - j[1] * j[3];
+    const double det = j[0] * cof[0] + j[1] * cof[3] + j[2] * cof[6];
+    if (isnan(det) || det <= 0.0) return false;
+    for (int a = 0; a < 10; ++a)
+        for (int k = 0; k < 3; ++k) {
+            const double sum = cof[k * 3 + 0] * shape_derivative(a, 0, l) + cof[k * 3 + 1] * shape_derivative(a, 1, l) +
+                               cof[k * 3 + 2] * shape_derivative(a, 2, l);
+            grad[a][k] = sum / det;
+        }
+    return true;
+}
+
+// MaterialField::at (material.rs:35-50), as element_kernel evaluates it
+__device__ void material_at(const FieldArgs &field, const double pos[3], double &E, double &nu) {
+    E = field.e;
+    nu = field.nu;
+    if (field.kind == SF_MATERIAL_SHELL_CORE) {
+        const int axes[2] = {field.tr0, field.tr1};
+        double depth = 1e300;
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int ax = axes[s2];
+            depth = fmin(depth, fmin(pos[ax] - field.lo[ax], field.hi[ax] - pos[ax]));
+        }
+        if (!(depth < field.depth)) {
+            E = field.core_e;
+            nu = field.core_nu;
+        }
+    }
+}
+
+// strain_at, stress_at and von_mises (element.rs:135-192) from the gradients and the element's
+// nodal displacements
+__device__ double point_stress(const double grad[10][3], const double ue[10][3], double E, double nu, double s[6]) {
+    double h[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 10; ++a)
+        for (int i = 0; i < 3; ++i)
+            for (int jj = 0; jj < 3; ++jj) h[i][jj] += ue[a][i] * grad[a][jj];
+    const double e0 = h[0][0], e1 = h[1][1], e2 = h[2][2];
+    const double e3 = h[0][1] + h[1][0], e4 = h[1][2] + h[2][1], e5 = h[2][0] + h[0][2];
+    const double lambda = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+    const double trace = e0 + e1 + e2;
+    s[0] = lambda * trace + 2.0 * mu * e0;
+    s[1] = lambda * trace + 2.0 * mu * e1;
+    s[2] = lambda * trace + 2.0 * mu * e2;
+    s[3] = mu * e3;
+    s[4] = mu * e4;
+    s[5] = mu * e5;
+    const double a = s[0] - s[1], b = s[1] - s[2], c = s[2] - s[0];
+    return sqrt(0.5 * (a * a + b * b + c * c) + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+}
+
+__device__ void load_element(const double *__restrict__ nodes, const uint32_t *__restrict__ tets, const double *__restrict__ u,
+                             size_t e, double xe[10][3], double ue[10][3]) {
+    for (int a = 0; a < 10; ++a) {
+        const size_t n = tets[e * 10 + a];
+        for (int i = 0; i < 3; ++i) {
+            xe[a][i] = nodes[n * 3 + i];
+            ue[a][i] = u[n * 3 + i];
+        }
+    }
+}
+
+// Stress at every integration point (Recovery::advance's point loop): one thread per (element,
+// point). `material` keeps each point's (E, nu) for the node values and the patch test of
+// uniformity.
+template <int NQ>
+__global__ __launch_bounds__(256) void recovery_points_kernel(size_t elements, const double *__restrict__ nodes,
+                                                              const uint32_t *__restrict__ tets, const double *__restrict__ u,
+                                                              FieldArgs field, double *__restrict__ out, double *__restrict__ material,
+                                                              int32_t *__restrict__ fault) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= elements * NQ) return;
+    const size_t e = t / NQ;
+    const int q = (int)(t - e * NQ);
+    double xe[10][3], ue[10][3], grad[10][3], pos[3];
+    load_element(nodes, tets, u, e, xe, ue);
+    const QPoint qp = (NQ == 14 ? RULE14 : RULE4)[q];
+    const double nat[3] = {qp.r, qp.s, qp.t};
+    double E, nu;
+    bool ok = thread_geometry(xe, nat, grad, pos);
+    if (ok) {
+        material_at(field, pos, E, nu);
+        ok = isfinite(E) && E > 0.0 && nu > -1.0 && nu < 0.5;
+    }
+    if (!ok) {
+        atomicMin(&fault[0], (int32_t)e);
+        return;
+    }
+    double s[6];
+    const double vm = point_stress(grad, ue, E, nu, s);
+    double *o = out + t * 10;
+    for (int i = 0; i < 3; ++i) o[i] = pos[i];
+    for (int c = 0; c < 6; ++c) o[3 + c] = s[c];
+    o[9] = vm;
+    material[t * 2] = E;
+    material[t * 2 + 1] = nu;
+}
+
+// The element's stress at each of its nodes with the material of its nearest integration point
+// (Recovery::advance's node loop): one thread per (element, node).
+template <int NQ>
+__global__ __launch_bounds__(256) void recovery_nodes_kernel(size_t elements, const double *__restrict__ nodes,
+                                                             const uint32_t *__restrict__ tets, const double *__restrict__ u,
+                                                             const double *__restrict__ material, const int8_t *__restrict__ nearest,
+                                                             double *__restrict__ out, int32_t *__restrict__ fault) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= elements * 10) return;
+    const size_t e = t / 10;
+    const int n = (int)(t - e * 10);
+    double xe[10][3], ue[10][3], grad[10][3], pos[3];
+    load_element(nodes, tets, u, e, xe, ue);
+    const double nat[3] = {NODE_NATURAL[n][0], NODE_NATURAL[n][1], NODE_NATURAL[n][2]};
+    if (!thread_geometry(xe, nat, grad, pos)) {
+        atomicMin(&fault[0], (int32_t)e);
+        return;
+    }
+    const size_t mp = e * NQ + (size_t)nearest[n];
+    double s[6];
+    const double vm = point_stress(grad, ue, material[mp * 2], material[mp * 2 + 1], s);
+    double *o = out + t * 7;
+    for (int c = 0; c < 6; ++c) o[c] = s[c];
+    o[6] = vm;
+}
+
+// The first largest von Mises among a node's incident element values, in element order (the
+// strict comparison of Recovery::advance, from the lowest value).
+__global__ void recovery_worst_kernel(size_t nodes, const uint32_t *__restrict__ starts, const uint32_t *__restrict__ at,
+                                      const double *__restrict__ node_values, double *__restrict__ worst) {
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nodes) return;
+    double best = -1.7976931348623157e308, s[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t k = starts[n]; k < starts[n + 1]; ++k) {
+        const double *v = node_values + (size_t)at[k] * 7;
+        if (v[6] > best) {
+            best = v[6];
+            for (int c = 0; c < 6; ++c) s[c] = v[c];
+        }
+    }
+    for (int c = 0; c < 6; ++c) worst[n * 6 + c] = s[c];
+}
+
+// PatchFit::fit (patch.rs:35-119) at each vertex over its incident elements' points in element
+// order; a patch of more than one material is left unfitted. fit: flag, centre 3, scale,
+// coefficients 4 x 6 (29 values).
+template <int NQ>
+__global__ void recovery_fit_kernel(size_t vertices, const double *__restrict__ nodes, const uint32_t *__restrict__ starts,
+                                    const uint32_t *__restrict__ elems, const double *__restrict__ points,
+                                    const double *__restrict__ material, double *__restrict__ fit) {
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= vertices) return;
+    double *f = fit + v * 29;
+    f[0] = 0.0;
+    const double c[3] = {nodes[v * 3], nodes[v * 3 + 1], nodes[v * 3 + 2]};
+    const uint32_t k0 = starts[v], k1 = starts[v + 1];
+    if (k0 == k1) return;
+    const double e0 = material[(size_t)elems[k0] * NQ * 2], nu0 = material[(size_t)elems[k0] * NQ * 2 + 1];
+    double scale = 0.0;
+    for (uint32_t k = k0; k < k1; ++k)
+        for (int q = 0; q < NQ; ++q) {
+            const size_t p = (size_t)elems[k] * NQ + q;
+            if (__double_as_longlong(material[p * 2]) != __double_as_longlong(e0) ||
+                __double_as_longlong(material[p * 2 + 1]) != __double_as_longlong(nu0))
+                return;  // mixed materials
+            for (int a = 0; a < 3; ++a) scale = fmax(scale, fabs(points[p * 10 + a] - c[a]));
+        }
+    if (scale == 0.0) return;
+    double m[4][4] = {}, rhs[4][6] = {};
+    for (uint32_t k = k0; k < k1; ++k)
+        for (int q = 0; q < NQ; ++q) {
+            const double *pt = points + ((size_t)elems[k] * NQ + q) * 10;
+            const double t[4] = {1.0, (pt[0] - c[0]) / scale, (pt[1] - c[1]) / scale, (pt[2] - c[2]) / scale};
+            for (int i = 0; i < 4; ++i) {
+                for (int j = 0; j < 4; ++j) m[i][j] += t[i] * t[j];
+                for (int cc = 0; cc < 6; ++cc) rhs[i][cc] += t[i] * pt[3 + cc];
+            }
+        }
+    for (int column = 0; column < 4; ++column) {
+        int pivot = column;
+        for (int row = column + 1; row < 4; ++row)
+            if (fabs(m[row][column]) > fabs(m[pivot][column])) pivot = row;
+        if (fabs(m[pivot][column]) <= 1e-12) return;
+        for (int k = 0; k < 4; ++k) {
+            const double x = m[column][k];
+            m[column][k] = m[pivot][k];
+            m[pivot][k] = x;
+        }
+        for (int k = 0; k < 6; ++k) {
+            const double x = rhs[column][k];
+            rhs[column][k] = rhs[pivot][k];
+            rhs[pivot][k] = x;
+        }
+        for (int row = column + 1; row < 4; ++row) {
+            const double factor = m[row][column] / m[column][column];
+            for (int k = column; k < 4; ++k) m[row][k] -= factor * m[column][k];
+            for (int k = 0; k < 6; ++k) rhs[row][k] -= factor * rhs[column][k];
+        }
+    }
+    for (int column = 3; column >= 0; --column) {
+        double value[6];
+        for (int k = 0; k < 6; ++k) value[k] = rhs[column][k];
+        for (int j = column + 1; j < 4; ++j)
+            for (int k = 0; k < 6; ++k) value[k] -= m[column][j] * rhs[j][k];
+        for (int k = 0; k < 6; ++k) rhs[column][k] = value[k] / m[column][column];
+    }
+    f[0] = 1.0;
+    for (int a = 0; a < 3; ++a) f[1 + a] = c[a];
+    f[4] = scale;
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 6; ++k) f[5 + i * 6 + k] = rhs[i][k];
+}
+
+__device__ void fit_at(const double *f, const double p[3], double value[6]) {
+    const double t[4] = {1.0, (p[0] - f[1]) / f[4], (p[1] - f[2]) / f[4], (p[2] - f[3]) / f[4]};
+    for (int k = 0; k < 6; ++k) value[k] = 0.0;
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 6; ++k) value[k] += t[i] * f[5 + i * 6 + k];
+}
+
+// Recovery::into_field (results.rs:376-411): a vertex reads its fit, a midside the mean of its
+// edge's two fits at its own position (midside_mean, results.rs:544-552); a node without fits
+// reads its worst incident value. Then von Mises. out: 7 per node.
+__global__ void recovery_nodal_kernel(size_t nodes, size_t vertices, const double *__restrict__ xyz, const uint32_t *__restrict__ ends,
+                                      const double *__restrict__ fit, const double *__restrict__ worst, double *__restrict__ out) {
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nodes) return;
+    const double p[3] = {xyz[n * 3], xyz[n * 3 + 1], xyz[n * 3 + 2]};
+    double s[6];
+    if (n < vertices) {
+        const double *f = fit + n * 29;
+        if (f[0] != 0.0) fit_at(f, p, s);
+        else
+            for (int c = 0; c < 6; ++c) s[c] = worst[n * 6 + c];
+    } else {
+        const double *fa = fit + (size_t)ends[(n - vertices) * 2] * 29, *fb = fit + (size_t)ends[(n - vertices) * 2 + 1] * 29;
+        if (fa[0] != 0.0 && fb[0] != 0.0) {
+            double a[6], b[6];
+            fit_at(fa, p, a);
+            fit_at(fb, p, b);
+            for (int c = 0; c < 6; ++c) s[c] = 0.5 * (a[c] + b[c]);
+        } else {
+            for (int c = 0; c < 6; ++c) s[c] = worst[n * 6 + c];
+        }
+    }
+    const double a = s[0] - s[1], b = s[1] - s[2], c = s[2] - s[0];
+    double *o = out + n * 7;
+    for (int k = 0; k < 6; ++k) o[k] = s[k];
+    o[6] = sqrt(0.5 * (a * a + b * b + c * c) + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+}
+
 FieldArgs field_args(const MaterialField &f) {
     FieldArgs a{};
     a.kind = f.kind;
@@ -240,6 +521,118 @@ FieldArgs field_args(const MaterialField &f) {
 constexpr size_t CHUNK_ELEMENTS = 1u << 20;
 
 }  // namespace
+
+void recover_stress(const Model &model, const double *u, double *point_values, double *nodal_values, hipStream_t s) {
+    const auto &mesh = model.mesh;
+    const size_t elements = mesh.tets.size(), nodes = mesh.nodes.size(), vertices = mesh.vertex_count;
+    const int nq = model.quadrature == 14 ? 14 : 4;
+    if (elements == 0) return;
+    // the integration point nearest each node in natural coordinates, first on a tie (results.rs:555-571)
+    const double r4[4][3] = {{A4, A4, A4}, {B4, A4, A4}, {A4, B4, A4}, {A4, A4, B4}};
+    const double r14[14][3] = {{A14, A14, A14}, {B14, A14, A14}, {A14, B14, A14}, {A14, A14, B14}, {C14, C14, C14},
+                               {D14, C14, C14}, {C14, D14, C14}, {C14, C14, D14}, {E14, E14, F14}, {E14, F14, E14},
+                               {F14, E14, E14}, {E14, F14, F14}, {F14, E14, F14}, {F14, F14, E14}};
+    const double natural[10][3] = {{0.0, 0.0, 0.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}, {0.5, 0.0, 0.0},
+                                   {0.5, 0.5, 0.0}, {0.0, 0.5, 0.0}, {0.0, 0.0, 0.5}, {0.5, 0.0, 0.5}, {0.0, 0.5, 0.5}};
+    std::vector<int8_t> nearest(10);
+    for (int n = 0; n < 10; ++n) {
+        double best = 1.7976931348623157e308;
+        int index = 0;
+        for (int q = 0; q < nq; ++q) {
+            const double *pt = nq == 14 ? r14[q] : r4[q];
+            double d = 0.0;
+            for (int a = 0; a < 3; ++a) d += (pt[a] - natural[n][a]) * (pt[a] - natural[n][a]);
+            if (d < best) {
+                best = d;
+                index = q;
+            }
+        }
+        nearest[n] = (int8_t)index;
+    }
+    // a node's incident (element, node) values in element order; a vertex's incident elements
+    // (build_patches, results.rs:414-436); a midside's edge ends from the first element reaching it
+    std::vector<uint32_t> node_starts(nodes + 1, 0), vert_starts(vertices + 1, 0);
+    for (const auto &t : mesh.tets) {
+        for (int a = 0; a < 10; ++a) node_starts[t[a] + 1] += 1;
+        for (int a = 0; a < 4; ++a) vert_starts[t[a] + 1] += 1;
+    }
+    for (size_t n = 0; n < nodes; ++n) node_starts[n + 1] += node_starts[n];
+    for (size_t v = 0; v < vertices; ++v) vert_starts[v + 1] += vert_starts[v];
+    std::vector<uint32_t> node_at(node_starts[nodes]), vert_at(vert_starts[vertices]);
+    std::vector<uint32_t> nfill(node_starts.begin(), node_starts.end() - 1), vfill(vert_starts.begin(), vert_starts.end() - 1);
+    std::vector<uint32_t> ends(2 * (nodes - vertices), 0);
+    std::vector<char> seen(nodes - vertices, 0);
+    static const int EA[6] = {0, 1, 2, 0, 1, 2}, EB[6] = {1, 2, 0, 3, 3, 3};
+    for (size_t e = 0; e < elements; ++e) {
+        const auto &t = mesh.tets[e];
+        for (int a = 0; a < 10; ++a) node_at[nfill[t[a]]++] = (uint32_t)(e * 10 + a);
+        for (int a = 0; a < 4; ++a) vert_at[vfill[t[a]]++] = (uint32_t)e;
+        for (int slot = 0; slot < 6; ++slot) {
+            const size_t m = t[4 + slot] - vertices;
+            if (seen[m]) continue;
+            seen[m] = 1;
+            ends[2 * m] = t[EA[slot]];
+            ends[2 * m + 1] = t[EB[slot]];
+        }
+    }
+    std::vector<double> xyz(nodes * 3);
+    for (size_t n = 0; n < nodes; ++n)
+        for (int a = 0; a < 3; ++a) xyz[3 * n + a] = mesh.nodes[n][a];
+    std::vector<uint32_t> tets(elements * 10);
+    for (size_t e = 0; e < elements; ++e)
+        for (int a = 0; a < 10; ++a) tets[10 * e + a] = mesh.tets[e][a];
+    DeviceBuffer<double> d_xyz, d_u, d_points(elements * nq * 10), d_material(elements * nq * 2), d_nodevals(elements * 10 * 7),
+        d_worst(nodes * 6), d_fit(std::max<size_t>(vertices, 1) * 29), d_nodal(nodes * 7);
+    DeviceBuffer<uint32_t> d_tets, d_nstart, d_nat, d_vstart, d_vat, d_ends;
+    DeviceBuffer<int8_t> d_nearest;
+    DeviceBuffer<int32_t> d_fault;
+    d_xyz.upload(xyz, s);
+    d_u.upload(u, nodes * 3, s);
+    d_tets.upload(tets, s);
+    d_nstart.upload(node_starts, s);
+    d_nat.upload(node_at, s);
+    d_vstart.upload(vert_starts, s);
+    d_vat.upload(vert_at, s);
+    d_ends.upload(ends.empty() ? std::vector<uint32_t>{0, 0} : ends, s);
+    d_nearest.upload(nearest, s);
+    const int32_t none = 0x7fffffff;
+    d_fault.upload(&none, 1, s);
+    const FieldArgs fa = field_args(model.field);
+    auto grid = [](size_t n) { return dim3((unsigned)((n + 255) / 256)); };
+    if (nq == 14) {
+        hipLaunchKernelGGL(recovery_points_kernel<14>, grid(elements * 14), dim3(256), 0, s, elements, d_xyz.ptr, d_tets.ptr, d_u.ptr, fa,
+                           d_points.ptr, d_material.ptr, d_fault.ptr);
+        hipLaunchKernelGGL(recovery_nodes_kernel<14>, grid(elements * 10), dim3(256), 0, s, elements, d_xyz.ptr, d_tets.ptr, d_u.ptr,
+                           (const double *)d_material.ptr, (const int8_t *)d_nearest.ptr, d_nodevals.ptr, d_fault.ptr);
+    } else {
+        hipLaunchKernelGGL(recovery_points_kernel<4>, grid(elements * 4), dim3(256), 0, s, elements, d_xyz.ptr, d_tets.ptr, d_u.ptr, fa,
+                           d_points.ptr, d_material.ptr, d_fault.ptr);
+        hipLaunchKernelGGL(recovery_nodes_kernel<4>, grid(elements * 10), dim3(256), 0, s, elements, d_xyz.ptr, d_tets.ptr, d_u.ptr,
+                           (const double *)d_material.ptr, (const int8_t *)d_nearest.ptr, d_nodevals.ptr, d_fault.ptr);
+    }
+    SF_HIP(hipGetLastError());
+    int32_t fault = 0;
+    d_fault.download(&fault, 1, s);
+    SF_HIP(hipStreamSynchronize(s));
+    if (fault != 0x7fffffff)
+        throw Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element has an inverted point or an invalid material", fault);
+    hipLaunchKernelGGL(recovery_worst_kernel, grid(nodes), dim3(256), 0, s, nodes, (const uint32_t *)d_nstart.ptr,
+                       (const uint32_t *)d_nat.ptr, (const double *)d_nodevals.ptr, d_worst.ptr);
+    if (nq == 14)
+        hipLaunchKernelGGL(recovery_fit_kernel<14>, grid(vertices), dim3(256), 0, s, vertices, (const double *)d_xyz.ptr,
+                           (const uint32_t *)d_vstart.ptr, (const uint32_t *)d_vat.ptr, (const double *)d_points.ptr,
+                           (const double *)d_material.ptr, d_fit.ptr);
+    else
+        hipLaunchKernelGGL(recovery_fit_kernel<4>, grid(vertices), dim3(256), 0, s, vertices, (const double *)d_xyz.ptr,
+                           (const uint32_t *)d_vstart.ptr, (const uint32_t *)d_vat.ptr, (const double *)d_points.ptr,
+                           (const double *)d_material.ptr, d_fit.ptr);
+    hipLaunchKernelGGL(recovery_nodal_kernel, grid(nodes), dim3(256), 0, s, nodes, vertices, (const double *)d_xyz.ptr,
+                       (const uint32_t *)d_ends.ptr, (const double *)d_fit.ptr, (const double *)d_worst.ptr, d_nodal.ptr);
+    SF_HIP(hipGetLastError());
+    if (point_values) d_points.download(point_values, elements * nq * 10, s);
+    if (nodal_values) d_nodal.download(nodal_values, nodes * 7, s);
+    SF_HIP(hipStreamSynchronize(s));
+}
 
 void DeviceAssembly::prepare(const Model &model, const AssemblyPattern &pattern, hipStream_t s) {
     const auto &mesh = model.mesh;

@@ -14,13 +14,21 @@ using namespace dev;
 static inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
 std::initializer_list<int> spmm_candidates(int rb, int cb, size_t value_size) {
-    // a lane holds BATCH blocks of rb * cb values: keep that within registers
-    static const std::initializer_list<int> small = {1, 2, 4}, large = {1, 2};
+    // Depths 1, 2, 4 of the lane-group kernels, and NODE_FORM + depth for one wavefront per block
+    // row. A lane holds BATCH blocks of rb * cb values: keep that within registers.
+    static const std::initializer_list<int> small = {1, 2, 4, NODE_FORM + 1, NODE_FORM + 2, NODE_FORM + 4},
+                                            large = {1, 2, NODE_FORM + 1, NODE_FORM + 2};
     return (size_t)rb * cb * value_size > 144 ? large : small;
 }
 
 template <int RB, int CB, int BATCH, typename S, typename V>
-static void spmm_launch(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+static void spmm_launch(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, bool node) {
+    if (node) {
+        const size_t threads = (size_t)a.slices * Shape<RB>::NPW * WAVE;
+        hipLaunchKernelGGL((spmm_node_kernel<RB, CB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a.view(), M, epi,
+                           x, y, b);
+        return;
+    }
     const int lg = columns_log2(M);
     const size_t threads = (size_t)a.slices * Shape<RB>::NPW << lg;
     hipLaunchKernelGGL((spmm_kernel<RB, CB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a.view(), lg, M, epi,
@@ -29,10 +37,11 @@ static void spmm_launch(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, 
 
 template <int RB, int CB, typename S, typename V>
 static void spmm_shape(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch) {
-    switch (batch) {
-        case 1: spmm_launch<RB, CB, 1>(a, M, x, y, b, epi, s); break;
-        case 2: spmm_launch<RB, CB, 2>(a, M, x, y, b, epi, s); break;
-        default: spmm_launch<RB, CB, 4>(a, M, x, y, b, epi, s); break;
+    const bool node = batch >= NODE_FORM;
+    switch (node ? batch - NODE_FORM : batch) {
+        case 1: spmm_launch<RB, CB, 1>(a, M, x, y, b, epi, s, node); break;
+        case 2: spmm_launch<RB, CB, 2>(a, M, x, y, b, epi, s, node); break;
+        default: spmm_launch<RB, CB, 4>(a, M, x, y, b, epi, s, node); break;
     }
 }
 

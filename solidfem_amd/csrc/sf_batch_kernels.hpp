@@ -100,6 +100,110 @@ __global__ __launch_bounds__(256) void spmm_kernel(Sliced<S> a, int log2g, uint3
     }
 }
 
+// The same product with one wavefront per block row: the row's position, block columns and block
+// values are the same for every lane, so they are read through the scalar unit into registers
+// every lane uses, and only the M columns of x are vector loads. Lane m takes column m (lanes
+// from M up idle). Row sums in the same order as rows_product.
+template <int RB, int CB, int BATCH, typename S, typename V>
+__device__ __forceinline__ void node_product(const Sliced<S> &a, uint32_t pos, int m, uint32_t M, const V *__restrict__ x,
+                                             uint32_t &row_out, V (&sum)[RB]) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const uint32_t slice = pos / NPW;
+    const int j = (int)(pos - slice * NPW);
+    const uint32_t row = a.perm[pos];
+    row_out = row;
+#pragma unroll
+    for (int p = 0; p < RB; ++p) sum[p] = (V)0;
+    if (row == NONE) return;
+    const uint32_t slot0 = a.slice_start[slice], len = a.row_len[pos];
+    const uint32_t *__restrict__ cp = a.cols + (size_t)slot0 * NPW + j;
+    const S *__restrict__ vp = a.vals + ((size_t)slot0 * WAVE + (size_t)j * RB) * CB;
+    const uint32_t full = len - len % BATCH;
+    for (uint32_t k = 0; k < full; k += BATCH) {
+        uint32_t c[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) c[b] = __builtin_amdgcn_readfirstlane(cp[(size_t)(k + b) * NPW]);
+        V xv[BATCH][CB];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int q = 0; q < CB; ++q) xv[b][q] = (uint32_t)m < M ? x[((size_t)c[b] * CB + q) * M + m] : (V)0;
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int p = 0; p < RB; ++p)
+#pragma unroll
+                for (int q = 0; q < CB; ++q) sum[p] = sum[p] + widen<V>(vp[(size_t)(k + b) * WAVE * CB + p * CB + q]) * xv[b][q];
+    }
+    for (uint32_t k = full; k < len; ++k) {
+        const uint32_t col = __builtin_amdgcn_readfirstlane(cp[(size_t)k * NPW]);
+        V xv[CB];
+#pragma unroll
+        for (int q = 0; q < CB; ++q) xv[q] = (uint32_t)m < M ? x[((size_t)col * CB + q) * M + m] : (V)0;
+#pragma unroll
+        for (int p = 0; p < RB; ++p)
+#pragma unroll
+            for (int q = 0; q < CB; ++q) sum[p] = sum[p] + widen<V>(vp[(size_t)k * WAVE * CB + p * CB + q]) * xv[q];
+    }
+}
+
+// Position of this wavefront's block row, made uniform for the compiler.
+__device__ __forceinline__ uint32_t node_of_wave() {
+    return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+}
+
+template <int RB, int CB, int BATCH, typename S, typename V>
+__global__ __launch_bounds__(256) void spmm_node_kernel(Sliced<S> a, uint32_t M, int epi, const V *__restrict__ x,
+                                                        V *__restrict__ y, const V *__restrict__ b) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const uint32_t pos = node_of_wave();
+    if (pos >= a.slices * (uint32_t)NPW) return;
+    const int m = threadIdx.x % WAVE;
+    uint32_t row;
+    V sum[RB];
+    node_product<RB, CB, BATCH, S, V>(a, pos, m, M, x, row, sum);
+    if (row == NONE || (uint32_t)m >= M) return;
+#pragma unroll
+    for (int p = 0; p < RB; ++p) {
+        const size_t out = ((size_t)row * RB + p) * M + m;
+        if (epi == EPI_STORE) y[out] = sum[p];
+        else if (epi == EPI_RESIDUAL) y[out] = b[out] - sum[p];
+        else y[out] = y[out] + sum[p];
+    }
+}
+
+template <int RB, int BATCH, typename S, typename V>
+__global__ __launch_bounds__(256) void cheb_step_node_kernel(Sliced<S> a, uint32_t M, const S *__restrict__ dinv,
+                                                             const V *__restrict__ d_in, const V *__restrict__ r_in, V keep,
+                                                             V step, V *__restrict__ d_out, V *__restrict__ r_out,
+                                                             V *__restrict__ x, V *__restrict__ sum_into) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const uint32_t pos = node_of_wave();
+    if (pos >= a.slices * (uint32_t)NPW) return;
+    const int m = threadIdx.x % WAVE;
+    uint32_t row;
+    V ad[RB];
+    node_product<RB, RB, BATCH, S, V>(a, pos, m, M, d_in, row, ad);
+    if (row == NONE || (uint32_t)m >= M) return;
+    V r_new[RB];
+#pragma unroll
+    for (int p = 0; p < RB; ++p) r_new[p] = r_in[((size_t)row * RB + p) * M + m] - ad[p];
+#pragma unroll
+    for (int p = 0; p < RB; ++p) {
+        const S *drow = dinv + (size_t)row * RB * RB + (size_t)p * RB;
+        V z = (V)0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * r_new[q];
+        const size_t i = ((size_t)row * RB + p) * M + m;
+        const V dn = keep * d_in[i] + step * z;
+        const V xn = x[i] + dn;
+        d_out[i] = dn;
+        r_out[i] = r_new[p];
+        x[i] = xn;
+        if (sum_into) sum_into[i] = sum_into[i] + xn;
+    }
+}
+
 // The rounding bound terms of CsrMatrix::multiply_rounding_bound (crates/solidfem-solve/src/csr.rs:142-154)
 // for M columns: per row and column, ((terms * eps) * sum |a x|)^2.
 template <int RB, int CB>

@@ -759,6 +759,27 @@ int32_t sf_pcg_is_converged(const sf_pcg *pcg) { return pcg->pcg->converged() ? 
 uint64_t sf_pcg_size(const sf_pcg *pcg) { return pcg->pcg->size(); }
 void sf_pcg_destroy(sf_pcg *pcg) { delete pcg; }
 
+sf_status sf_recover_stress(sf_context *ctx, const sf_model *model, const double *displacements, double *point_values,
+                            double *nodal_values) {
+    return guard([&] { recover_stress(model->model, displacements, point_values, nodal_values, ctx->ctx->stream); });
+}
+sf_status sf_static_study_recover(sf_static_study *study, double *point_values, double *nodal_values) {
+    return guard([&] {
+        std::vector<double> u(3 * study->model.mesh.nodes.size());
+        const sf_status st = sf_static_study_displacements(study, u.data());
+        if (st != SF_OK) throw Error(st, "the study's displacements are not available");
+        recover_stress(study->model, u.data(), point_values, nodal_values, study->ctx->stream);
+    });
+}
+uint64_t sf_model_points_per_element(const sf_model *model) { return model->model.quadrature == 14 ? 14 : 4; }
+sf_status sf_static_study_counts(const sf_static_study *study, uint64_t *nodes, uint64_t *elements, uint64_t *points) {
+    return guard([&] {
+        if (nodes) *nodes = study->model.mesh.nodes.size();
+        if (elements) *elements = study->model.mesh.tets.size();
+        if (points) *points = study->model.quadrature == 14 ? 14 : 4;
+    });
+}
+
 struct sf_server {
     std::unique_ptr<Server> server;
 };

@@ -604,7 +604,13 @@ class Hierarchy final : public DeviceMultigrid {
 
     template <int RB, int BATCH>
     static void cheb_batch_launch(LevelDev<S, V> &d, uint32_t M, const V *d_in, const V *r_in, V keep, V stp, V *d_out,
-                                  V *r_out, V *x, V *into, hipStream_t s) {
+                                  V *r_out, V *x, V *into, hipStream_t s, bool node) {
+        if (node) {
+            const size_t threads = (size_t)d.a.slices * Shape<RB>::NPW * WAVE;
+            hipLaunchKernelGGL((cheb_step_node_kernel<RB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.view(), M,
+                               d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+            return;
+        }
         const int lg = columns_log2(M);
         const size_t threads = (size_t)d.a.slices * Shape<RB>::NPW << lg;
         hipLaunchKernelGGL((cheb_step_batch_kernel<RB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.view(),
@@ -613,10 +619,11 @@ class Hierarchy final : public DeviceMultigrid {
     template <int RB>
     static void cheb_batch_rb(LevelDev<S, V> &d, int batch, uint32_t M, const V *d_in, const V *r_in, V keep, V stp,
                               V *d_out, V *r_out, V *x, V *into, hipStream_t s) {
-        switch (batch) {
-            case 1: cheb_batch_launch<RB, 1>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
-            case 2: cheb_batch_launch<RB, 2>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
-            default: cheb_batch_launch<RB, 4>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s); break;
+        const bool node = batch >= NODE_FORM;
+        switch (node ? batch - NODE_FORM : batch) {
+            case 1: cheb_batch_launch<RB, 1>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
+            case 2: cheb_batch_launch<RB, 2>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
+            default: cheb_batch_launch<RB, 4>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
         }
     }
     static void launch_cheb_batch(LevelDev<S, V> &d, int batch, uint32_t M, const V *d_in, const V *r_in, V keep, V stp,

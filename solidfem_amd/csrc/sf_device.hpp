@@ -86,6 +86,9 @@ int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initiali
 
 // Columns one batched solve carries: a group of lanes per block row, one lane per column.
 constexpr uint32_t MAX_COLUMNS = 64;
+// A batched kernel's variant code at or above NODE_FORM is the one-wavefront-per-block-row form
+// with depth (code - NODE_FORM); below it, the lane-group form with that depth.
+constexpr int NODE_FORM = 16;
 inline int columns_log2(uint32_t m) {
     int g = 0;
     while ((1u << g) < m) ++g;
@@ -277,6 +280,14 @@ class DevicePcgBatch {
     uint32_t *pinned_flags_ = nullptr;
     hipEvent_t copied_ = nullptr;
 };
+
+// ---- stress recovery --------------------------------------------------------------------------
+
+// Recovery after a solve (crates/solidfem-elastic/src/results.rs:307-576, element.rs:135-192,
+// patch.rs) on the device for nodal displacements `u` (3 per mesh node): per (element, point)
+// 10 values -- position 3, stress [xx, yy, zz, xy, yz, zx], von Mises -- and per mesh node 7 --
+// the recovered stress and its von Mises. Either output may be null.
+void recover_stress(const Model &model, const double *u, double *point_values, double *nodal_values, hipStream_t s);
 
 // ---- assembly -------------------------------------------------------------------------------
 

@@ -55,6 +55,8 @@ _SIG = {
     "ref_problem_rhs": (None, [vp, P(f64)]),
     "ref_problem_rigid_modes": (None, [vp, P(f64)]),
     "ref_problem_element_stiffness": (i32, [vp, i64, P(f64)]),
+    "ref_recover": (i32, [vp, P(f64), P(f64), P(f64), P(i64)]),
+    "ref_problem_quadrature": (i32, [vp]),
     "ref_multigrid_create": (vp, [vp, P(RefOptions), P(i32)]),
     "ref_multigrid_destroy": (None, [vp]),
     "ref_multigrid_levels": (i64, [vp]),
@@ -175,6 +177,21 @@ class Problem:
         out = np.empty(6 * self.free_dofs)
         lib().ref_problem_rigid_modes(self._p, ptr(out, f64))
         return out
+
+    @property
+    def vertex_count(self):
+        return lib().ref_problem_vertex_count(self._p)
+
+    def recover(self, u):
+        """Stress recovery of nodal displacements: (points (elements, points, 10), nodal (nodes, 7))."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        q = lib().ref_problem_quadrature(self._p)
+        points = np.empty((lib().ref_problem_tet_count(self._p), q, 10))
+        nodal = np.empty((self.node_count, 7))
+        fault = C.c_int64(-1)
+        if lib().ref_recover(self._p, ptr(u, f64), ptr(points, f64), ptr(nodal, f64), C.byref(fault)) != 0:
+            raise RuntimeError(f"element {fault.value} is inverted or has an invalid material")
+        return points, nodal
 
     def element_stiffness(self, tet):
         ke = np.empty((30, 30))
