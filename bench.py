@@ -388,6 +388,21 @@ def main():
         },
         "cpu_baseline": None,
     }
+    # stress recovery of the FP32 solution (results.rs:307-576): every element's integration-point
+    # stresses, the vertex patch fits and the nodal field, on the device, host arrays in and out
+    model = sf.Model.from_workload(workload)
+    nodal_u = np.zeros((report.total_dofs // 3, 3))
+    free_map = model.free_problem()[0]
+    live = free_map != 0xFFFFFFFF
+    nodal_u[live] = x32.reshape(-1, 3)[free_map[live]]
+    model.recover(ctx, nodal_u)  # warm
+    t = time.perf_counter()
+    points, nodal = model.recover(ctx, nodal_u)
+    recovery_ms = (time.perf_counter() - t) * 1e3
+    line["stress_recovery"] = {"ms": round(recovery_ms, 2), "elements": int(points.shape[0]),
+                               "nodes": int(nodal.shape[0]), "peak_nodal_von_mises": float(nodal[:, 6].max()),
+                               "note": "host incidence lists, uploads and downloads included"}
+    log(f"stress recovery of the plate: {recovery_ms:.1f} ms, peak nodal von Mises {nodal[:, 6].max():.4g} MPa")
     if args.columns:
         line["multi_rhs"] = multi_rhs(ctx, sf, np, args.columns, args.steps, args.warmup)
     if args.serve_requests:
@@ -396,6 +411,11 @@ def main():
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (oracle) on a bounded sample")
         line["cpu_baseline"] = cpu_baseline(workload, int(fp32["iterations"]), args.cpu_iterations)
+        import refcpu
+        problem = refcpu.Problem(workload)
+        t = time.perf_counter()
+        problem.recover(nodal_u)
+        line["stress_recovery"]["oracle_serial_ms"] = round((time.perf_counter() - t) * 1e3, 1)
     print(json.dumps(line), flush=True)
 
 
