@@ -133,8 +133,16 @@ typedef struct sf_multigrid_options {
     int32_t precision;
     int32_t threads; /* host threads for setup; 0 = all */
     int32_t coarse_solve; /* SF_COARSE_REFERENCE or SF_COARSE_INVERSE */
-    int32_t reserved;
+    int32_t setup_device; /* SF_SETUP_AUTO, SF_SETUP_HOST or SF_SETUP_DEVICE */
 } sf_multigrid_options;
+
+/* Where a level's numeric setup phases run: the power iteration, the smoothed prolongator, A P
+ * and P^T (A P) (setup.rs:390-449, :527-647). AUTO runs them on the device for levels of at
+ * least 4096 block rows when the setup has a device context, HOST never, DEVICE for every
+ * level (given a context). Aggregation, the tentative prolongator and the block inverses stay
+ * on the host. The hierarchy is the same to the bit wherever the phases run. No counterpart in
+ * the reference, whose setup runs on its host thread pool. */
+enum { SF_SETUP_AUTO = 0, SF_SETUP_HOST = 1, SF_SETUP_DEVICE = 2 };
 
 /* The coarsest level's solve. REFERENCE runs forward then back substitution with the Cholesky
  * factor in the reference's order (crates/solidfem-solve/src/multigrid/block.rs:357-373), so a

@@ -36,6 +36,15 @@ class Precision(enum.IntEnum):
     SINGLE = 2
 
 
+class SetupDevice(enum.IntEnum):
+    """Where a level's numeric setup phases run (power iteration, smoothed prolongator, A P,
+    P^T A P): AUTO on the device for levels of at least 4096 block rows when the setup has a
+    device context, HOST never, DEVICE for every level. The hierarchy's bits are the same."""
+    AUTO = 0
+    HOST = 1
+    DEVICE = 2
+
+
 class DotOrder(enum.IntEnum):
     TREE = 0
     REFERENCE = 1
@@ -114,10 +123,12 @@ class MultigridOptions:
     precision: Precision = Precision.SINGLE_STORAGE
     threads: int = 0
     coarse_solve: int = 0  # 0: reference order, 1: explicit inverse
+    setup_device: "SetupDevice" = None  # where the numeric setup phases run (default AUTO)
 
     def c(self):
+        placement = SetupDevice.AUTO if self.setup_device is None else self.setup_device
         return _capi.MultigridOptions(self.neighbour_reach, self.coarse_dofs, self.max_levels, self.smoother_degree,
-                                      self.smoother_range, int(self.precision), self.threads, self.coarse_solve, 0)
+                                      self.smoother_range, int(self.precision), self.threads, self.coarse_solve, int(placement))
 
 
 @dataclass

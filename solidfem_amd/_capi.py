@@ -58,7 +58,7 @@ class MultigridOptions(C.Structure):
         ("precision", i32),
         ("threads", i32),
         ("coarse_solve", i32),
-        ("reserved", i32),
+        ("setup_device", i32),
     ]
 
 

@@ -8,6 +8,7 @@
 
 #include "sf_batch_kernels.hpp"
 #include "sf_handles.hpp"
+#include "sf_setup_device.hpp"
 #include "sf_serve.hpp"
 
 using namespace sf;
@@ -42,6 +43,14 @@ sf_status guard(F &&f) {
     }
 }
 
+// SF_SETUP_AUTO takes levels of at least this many block rows to the device.
+constexpr size_t DEVICE_SETUP_ROWS = 4096;
+
+void attach_device_setup(MultigridSetup &setup, const std::shared_ptr<Context> &ctx, const SetupOptions &o, const double *level0_values) {
+    if (!ctx || o.setup_device == SF_SETUP_HOST) return;
+    setup.attach(std::make_unique<DeviceSetup>(ctx, o.setup_device == SF_SETUP_DEVICE ? 1 : DEVICE_SETUP_ROWS, level0_values));
+}
+
 double seconds() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
 SetupOptions setup_options(const sf_multigrid_options *o) {
@@ -54,6 +63,8 @@ SetupOptions setup_options(const sf_multigrid_options *o) {
     s.range = o->smoother_range;
     s.precision = o->precision;
     s.coarse_solve = o->coarse_solve;
+    s.setup_device = o->setup_device;
+    if (s.setup_device < SF_SETUP_AUTO || s.setup_device > SF_SETUP_DEVICE) throw Error(SF_ERR_INVALID_ARGUMENT, "unknown setup placement");
     if (s.precision < 0 || s.precision > 2) throw Error(SF_ERR_INVALID_ARGUMENT, "unknown precision");
     if (s.coarse_solve != SF_COARSE_REFERENCE && s.coarse_solve != SF_COARSE_INVERSE)
         throw Error(SF_ERR_INVALID_ARGUMENT, "unknown coarse solve");
@@ -415,6 +426,7 @@ sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_c
                     for (size_t n = 0; n < b->model.mesh.nodes.size(); ++n)
                         if (b->problem.free[n] != FIXED) positions.push_back(b->model.mesh.nodes[n]);
                     b->setup = std::make_unique<MultigridSetup>(sys.host, modes, positions, b->mg_options);
+                    attach_device_setup(*b->setup, b->ctx, b->mg_options, sys.values.ptr);
                 }
                 b->setup_finished = b->setup->step(max_elements, cancel);
                 b->preconditioner_seconds += seconds() - t1;
@@ -614,7 +626,9 @@ sf_status sf_multigrid_setup_create(sf_context *ctx, const sf_csr *matrix, const
             for (int a = 0; a < 3; ++a) pos[r][a] = positions[3 * r + a];
         auto s = std::make_unique<sf_multigrid_setup>();
         s->ctx = ctx ? ctx->ctx : nullptr;
-        s->setup = std::make_unique<MultigridSetup>(sys.host, modes, pos, setup_options(options));
+        const SetupOptions opt = setup_options(options);
+        s->setup = std::make_unique<MultigridSetup>(sys.host, modes, pos, opt);
+        attach_device_setup(*s->setup, s->ctx, opt, nullptr);
         *out = s.release();
     });
 }

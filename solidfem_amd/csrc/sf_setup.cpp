@@ -429,6 +429,19 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         scratch_.assign(dofs, 0.0);
                         scratch2_.assign(dofs, 0.0);
                     }
+                    if (power_done_ == 0 && accelerator_ && accelerator_->takes(level, current_)) {
+                        // the whole iteration at once on the accelerator, charged as the host's
+                        if (budget == 0) return false;
+                        if (cancel()) {
+                            cancelled_ = true;
+                            throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
+                        }
+                        accelerator_->begin(level, current_, inverse_diagonal_);
+                        on_device_ = true;
+                        lambda_max_ = accelerator_->spectral(power_, (int)POWER_ITERATIONS);
+                        power_done_ = POWER_ITERATIONS;
+                        budget = budget > rows * POWER_ITERATIONS ? budget - rows * POWER_ITERATIONS : 0;
+                    }
                     while (power_done_ < POWER_ITERATIONS) {
                         if (budget == 0) return false;
                         if (cancel()) {
@@ -458,6 +471,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                                         for (size_t q = 0; q < current_.cb; ++q) l.factor[(row * rb + p) * n + col * current_.cb + q] = b[p * current_.cb + q];
                                 }
                             levels_.push_back(std::move(l));
+                            release_accelerator_level();
                             cursor_ = 0;
                             phase_ = Phase::Factor;
                         } else {
@@ -518,7 +532,17 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     const std::vector<int64_t> &ns = use_filter ? filter_starts_ : strong_starts_;
                     const std::vector<u32> &nc = use_filter ? filter_columns_ : strong_columns_;
                     const size_t tb = rb * MODES;
-                    bool done = run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
+                    bool on_accelerator = false;
+                    if (on_device_ && cursor_ == 0) {
+                        if (budget == 0) return false;
+                        if (cancel()) {
+                            cancelled_ = true;
+                            throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
+                        }
+                        on_accelerator = device_p_ = accelerator_->smooth(aggregate_, aggregate_count_, ns, nc, tentative_values_, omega, prolongator_);
+                        if (on_accelerator) budget = budget > rows ? budget - rows : 0;
+                    }
+                    bool done = on_accelerator || run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         std::vector<RowOut> out(l - f);
                         parallel_for(l - f, 256, [&](size_t a, size_t b) {
                             RowAccumulator acc(aggregate_count_, tb);
@@ -564,7 +588,17 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                 case Phase::Product: {
                     // setup.rs:575-612, transfer.rs:81-95 and 155-169: A P row by row
                     const size_t tb = rb * MODES;
-                    bool done = run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
+                    bool on_accelerator = false;
+                    if (device_p_ && cursor_ == 0) {
+                        if (budget == 0) return false;
+                        if (cancel()) {
+                            cancelled_ = true;
+                            throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
+                        }
+                        on_accelerator = device_ap_ = accelerator_->product(aggregate_count_);
+                        if (on_accelerator) budget = budget > rows ? budget - rows : 0;
+                    }
+                    bool done = on_accelerator || run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         std::vector<RowOut> out(l - f);
                         parallel_for(l - f, 256, [&](size_t a, size_t b) {
                             RowAccumulator acc(aggregate_count_, tb);
@@ -606,7 +640,17 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                 case Phase::Galerkin: {
                     // setup.rs:614-647, transfer.rs:204-238: P^T (A P) by coarse rows
                     const size_t count = aggregate_count_;
-                    bool done = run_rows(count, budget, cancel, true, [&](size_t f, size_t l, bool) {
+                    bool on_accelerator = false;
+                    if (device_ap_ && cursor_ == 0) {
+                        if (budget == 0) return false;
+                        if (cancel()) {
+                            cancelled_ = true;
+                            throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
+                        }
+                        on_accelerator = accelerator_->galerkin((u32)count, pcol_starts_, pcol_entries_, coarse_);
+                        if (on_accelerator) budget = budget > count ? budget - count : 0;
+                    }
+                    bool done = on_accelerator || run_rows(count, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         std::vector<RowOut> out(l - f);
                         parallel_for(l - f, 64, [&](size_t a, size_t b) {
                             RowAccumulator acc(count, MODES * MODES);
@@ -683,7 +727,13 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
     }
 }
 
+void MultigridSetup::release_accelerator_level() {
+    if (on_device_) accelerator_->end_level();
+    on_device_ = device_p_ = device_ap_ = false;
+}
+
 void MultigridSetup::push_final_level() {
+    release_accelerator_level();
     HostLevel l;
     l.lambda_min = lambda_max_ / options_.range;
     l.lambda_max = lambda_max_;
@@ -693,6 +743,7 @@ void MultigridSetup::push_final_level() {
 }
 
 void MultigridSetup::finish_level() {
+    release_accelerator_level();
     // setup.rs:650-695: a unit diagonal on coarse values no aggregate direction reaches
     const size_t dim = coarse_.rb;
     for (size_t row = 0; row < coarse_.rows; ++row)

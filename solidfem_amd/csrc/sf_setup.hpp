@@ -4,6 +4,8 @@
 // in row order, so neither the thread count nor the slicing changes a bit.
 #pragma once
 
+#include <memory>
+
 #include "sf_host.hpp"
 
 namespace sf {
@@ -27,12 +29,33 @@ struct SetupOptions {
     double range = 30.0;
     int precision = SF_PRECISION_SINGLE_STORAGE;
     int coarse_solve = SF_COARSE_REFERENCE;
+    int setup_device = SF_SETUP_AUTO;
+};
+
+// Runs the numeric phases of a level (the power iteration, the smoothed prolongator, A P and
+// P^T A P) somewhere other than the host thread pool, with the host phases' bits. A phase that
+// returns false did not run (its rows overflow the accelerator's limits) and runs on the host.
+class SetupAccelerator {
+   public:
+    virtual ~SetupAccelerator() = default;
+    virtual bool takes(int level, const HostBsr<double> &a) const = 0;
+    virtual void begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal) = 0;
+    // `power` is the normalised start vector; returns the last iteration's norm.
+    virtual double spectral(std::vector<double> &power, int iterations) = 0;
+    virtual bool smooth(const std::vector<u32> &aggregate, u32 count, const std::vector<int64_t> &ns, const std::vector<u32> &nc,
+                        const std::vector<double> &tentative, double omega, HostBsr<double> &p) = 0;
+    virtual bool product(u32 count) = 0;
+    virtual bool galerkin(u32 count, const std::vector<int64_t> &pcol_starts, const std::vector<std::pair<u32, int64_t>> &entries,
+                          HostBsr<double> &coarse) = 0;
+    virtual void end_level() = 0;
 };
 
 class MultigridSetup {
    public:
     MultigridSetup(HostBsr<double> matrix, const std::vector<double> &near_null, const std::vector<Vec3> &positions,
                    const SetupOptions &options);
+    // Hands the numeric phases of the levels it takes to `accelerator`.
+    void attach(std::unique_ptr<SetupAccelerator> accelerator) { accelerator_ = std::move(accelerator); }
     // Up to `budget` block rows of work; true once finished. Throws Error(SF_ERR_MG_*).
     bool step(size_t budget, const Cancel &cancel);
     bool finished() const { return phase_ == Phase::Done; }
@@ -50,6 +73,8 @@ class MultigridSetup {
     void finish_level();
     void push_final_level();
     void spectral_iteration();
+    // Ends the accelerator's part in the current level, if it has one.
+    void release_accelerator_level();
 
     SetupOptions options_;
     double reach_ = 3.0;
@@ -76,6 +101,10 @@ class MultigridSetup {
     std::vector<int64_t> pcol_starts_;
     std::vector<std::pair<u32, int64_t>> pcol_entries_;
     size_t cursor_ = 0;
+    std::unique_ptr<SetupAccelerator> accelerator_;
+    bool on_device_ = false;  // this level's numeric phases run on the accelerator
+    bool device_p_ = false;   // the accelerator holds this level's prolongator
+    bool device_ap_ = false;  // and its A P
     bool is_coarsest_ = false;
     Phase phase_ = Phase::Diagonal;
     bool cancelled_ = false;

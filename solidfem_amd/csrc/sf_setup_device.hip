@@ -1,0 +1,592 @@
+// The numeric phases of a multigrid level's setup on the device: the power iteration
+// (crates/solidfem-solve/src/multigrid/setup.rs:390-449), the smoothed prolongator
+// P = T - omega D^-1 A_f T (setup.rs:527-573, transfer.rs:108-152), A P (setup.rs:575-612,
+// transfer.rs:155-169) and the Galerkin product P^T (A P) (setup.rs:614-647,
+// transfer.rs:204-238).
+//
+// Every output value is the host phase's value to the bit. The host sums each output entry in
+// a fixed order: contributions arrive row entry by row entry, and each block product adds its
+// terms in ascending inner index, one rounding per add, with no fused multiply-add (the library
+// is built with -ffp-contract=off). Here one wavefront owns one output row and one lane owns one
+// output block: every lane walks the row's contributions in the host's order and adds the ones
+// that land in its block, so each entry sees the same sequence of roundings.
+//
+// An output row's block columns are the sorted distinct coarse indices of its contributions.
+// Coarse indices are aggregate numbers, so one pass marks them in a bitmap in LDS over the
+// row's index window and a second reads the set bits in ascending order. A count launch sizes
+// the rows (host prefix sum), a value launch fills them.
+//
+// The power iteration's norm is a serial sum over the vector (setup.rs:418-420); its chain of
+// dependent additions runs on the host over a copy of the vector, the products on the device.
+#include <algorithm>
+#include <cmath>
+#include <climits>
+
+#include "sf_device.hpp"
+#include "sf_setup_device.hpp"
+
+namespace sf {
+namespace {
+
+constexpr int WAVE_SIZE = 64;
+constexpr u32 WINDOW_WORDS = 2048;  // a row's coarse indices must fit 65,536 consecutive values
+constexpr u32 MAX_SLOTS = 1024;     // blocks per output row
+constexpr int MODES6 = 6;
+
+__device__ __forceinline__ u32 wave_min_u32(u32 v) {
+    for (int o = 32; o; o >>= 1) v = min(v, (u32)__shfl_xor((int)v, o));
+    return v;
+}
+__device__ __forceinline__ u32 wave_max_u32(u32 v) {
+    for (int o = 32; o; o >>= 1) v = max(v, (u32)__shfl_xor((int)v, o));
+    return v;
+}
+// Exclusive prefix over the wave's lanes; `total` gets the sum.
+__device__ __forceinline__ u32 wave_exclusive_u32(u32 v, u32 &total) {
+    const int lane = threadIdx.x;
+    u32 inc = v;
+    for (int o = 1; o < WAVE_SIZE; o <<= 1) {
+        const u32 up = (u32)__shfl_up((int)inc, o);
+        if (lane >= o) inc += up;
+    }
+    total = (u32)__shfl((int)inc, WAVE_SIZE - 1);
+    return inc - v;
+}
+
+// The sorted distinct coarse indices one row's contributions land in. `each(lane, f)` calls
+// f(index) for every contribution a lane enumerates (lanes split the contributions among them).
+// Leaves them in `slots[0..k)` and returns k, or UINT_MAX if the row overflows the window or
+// the slot list (the caller reports the overflow and the level's phase runs on the host).
+template <typename Each>
+__device__ u32 row_columns(Each each, u32 *bitmap, u32 *slots, bool keep) {
+    const int lane = threadIdx.x;
+    u32 lo = UINT_MAX, hi = 0;
+    each(lane, [&](u32 id) {
+        lo = min(lo, id);
+        hi = max(hi, id);
+    });
+    lo = wave_min_u32(lo);
+    hi = wave_max_u32(hi);
+    if (lo > hi) return 0;
+    const u32 wlo = lo >> 5, words = (hi >> 5) - wlo + 1;
+    if (words > WINDOW_WORDS) return UINT_MAX;
+    for (u32 w = lane; w < words; w += WAVE_SIZE) bitmap[w] = 0;
+    __syncthreads();
+    each(lane, [&](u32 id) { atomicOr(&bitmap[(id >> 5) - wlo], 1u << (id & 31)); });
+    __syncthreads();
+    u32 base = 0;
+    for (u32 w0 = 0; w0 < words; w0 += WAVE_SIZE) {
+        u32 bits = w0 + lane < words ? bitmap[w0 + lane] : 0u;
+        u32 total;
+        u32 pos = base + wave_exclusive_u32((u32)__popc(bits), total);
+        if (keep) {
+            while (bits) {
+                const int b = __ffs(bits) - 1;
+                bits &= bits - 1;
+                if (pos < MAX_SLOTS) slots[pos] = ((wlo + w0 + (u32)lane) << 5) + (u32)b;
+                ++pos;
+            }
+        }
+        base += total;
+    }
+    __syncthreads();
+    return base > MAX_SLOTS ? UINT_MAX : base;
+}
+
+__device__ __forceinline__ bool sorted_contains(const u32 *first, int64_t n, u32 value) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (first[mid] < value)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < n && first[lo] == value;
+}
+
+// y = D^-1 (A x), one thread per block row, each sum in the host's order (block.rs:136-154,
+// setup.rs:409-417).
+template <int RB>
+__global__ __launch_bounds__(256) void power_kernel(size_t rows, const int64_t *__restrict__ starts, const u32 *__restrict__ cols,
+                                                    const double *__restrict__ vals, const double *__restrict__ dinv,
+                                                    const double *__restrict__ x, double *__restrict__ y) {
+    const size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    double out[RB];
+#pragma unroll
+    for (int p = 0; p < RB; ++p) out[p] = 0.0;
+    for (int64_t at = starts[row]; at < starts[row + 1]; ++at) {
+        const double *b = vals + (size_t)at * RB * RB;
+        const double *xc = x + (size_t)cols[at] * RB;
+        double xv[RB];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) xv[q] = xc[q];
+#pragma unroll
+        for (int p = 0; p < RB; ++p) {
+            double s = out[p];
+#pragma unroll
+            for (int q = 0; q < RB; ++q) s += b[p * RB + q] * xv[q];
+            out[p] = s;
+        }
+    }
+    const double *d = dinv + row * RB * RB;
+#pragma unroll
+    for (int p = 0; p < RB; ++p) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) s += d[p * RB + q] * out[q];
+        y[row * RB + p] = s;
+    }
+}
+
+__global__ void divide_kernel(size_t n, const double *__restrict__ y, double norm, double *__restrict__ x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = y[i] / norm;
+}
+
+struct SetupRows {
+    const int64_t *starts;
+    const u32 *cols;
+    const double *vals;
+};
+
+// P = T - omega D^-1 (A_f T): A's block (row, col) carries T's row col when col is the row or
+// one of its strong (fine level) or filtered (coarse levels) neighbours, else T's own row
+// (transfer.rs:108-152, aggregate.rs:80-96). COUNT launches write the row lengths.
+template <int RB, bool COUNT>
+__global__ __launch_bounds__(64) void smooth_kernel(size_t rows, SetupRows a, const u32 *__restrict__ aggregate,
+                                                    const int64_t *__restrict__ ns, const u32 *__restrict__ nc,
+                                                    const double *__restrict__ tentative, const double *__restrict__ dinv,
+                                                    double omega, u32 *__restrict__ counts, const int64_t *__restrict__ out_starts,
+                                                    u32 *__restrict__ out_cols, double *__restrict__ out_vals, u32 *overflow) {
+    __shared__ u32 bitmap[WINDOW_WORDS];
+    __shared__ u32 slots[MAX_SLOTS];
+    const size_t row = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t first = a.starts[row], last = a.starts[row + 1];
+    const u32 *nb = nc + ns[row];
+    const int64_t nn = ns[row + 1] - ns[row];
+    auto source = [&](int64_t at) -> u32 {
+        const u32 col = a.cols[at];
+        return (col == (u32)row || sorted_contains(nb, nn, col)) ? col : (u32)row;
+    };
+    const u32 k = row_columns(
+        [&](int l, auto f) {
+            for (int64_t at = first + l; at < last; at += WAVE_SIZE) f(aggregate[source(at)]);
+        },
+        bitmap, slots, !COUNT);
+    if (k == UINT_MAX) {
+        if (lane == 0) atomicOr(overflow, 1u);
+        return;
+    }
+    if (COUNT) {
+        if (lane == 0) counts[row] = k;
+        return;
+    }
+    const u32 t_col = aggregate[row];
+    const double *d = dinv + row * RB * RB;
+    const double *t_row = tentative + row * RB * MODES6;
+    const int64_t base = out_starts[row];
+    for (u32 g = 0; g < k; g += WAVE_SIZE) {
+        const u32 s = g + (u32)lane;
+        const u32 mine = s < k ? slots[s] : UINT_MAX;
+        double acc[RB * MODES6];
+#pragma unroll
+        for (int i = 0; i < RB * MODES6; ++i) acc[i] = 0.0;
+        for (int64_t at = first; at < last; ++at) {
+            const u32 src = source(at);
+            if (aggregate[src] != mine) continue;
+            const double *b = a.vals + (size_t)at * RB * RB;
+            const double *t = tentative + (size_t)src * RB * MODES6;
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+                for (int mid = 0; mid < RB; ++mid) {
+                    const double v = b[r * RB + mid];
+#pragma unroll
+                    for (int q = 0; q < MODES6; ++q) acc[r * MODES6 + q] += v * t[mid * MODES6 + q];
+                }
+        }
+        if (s >= k) continue;
+        out_cols[base + s] = mine;
+        double *out = out_vals + (size_t)(base + s) * RB * MODES6;
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int q = 0; q < MODES6; ++q) {
+                double scaled = 0.0;
+#pragma unroll
+                for (int mid = 0; mid < RB; ++mid) scaled += d[r * RB + mid] * acc[mid * MODES6 + q];
+                out[r * MODES6 + q] = (mine == t_col ? t_row[r * MODES6 + q] : 0.0) - omega * scaled;
+            }
+    }
+}
+
+// A P by fine rows: A's blocks in order, each times the prolongator row of its column, the
+// prolongator row's blocks in order (transfer.rs:155-169).
+template <int RB, bool COUNT>
+__global__ __launch_bounds__(64) void product_kernel(size_t rows, SetupRows a, SetupRows p, u32 *__restrict__ counts,
+                                                     const int64_t *__restrict__ out_starts, u32 *__restrict__ out_cols,
+                                                     double *__restrict__ out_vals, u32 *overflow) {
+    __shared__ u32 bitmap[WINDOW_WORDS];
+    __shared__ u32 slots[MAX_SLOTS];
+    const size_t row = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t first = a.starts[row], last = a.starts[row + 1];
+    const u32 k = row_columns(
+        [&](int l, auto f) {
+            for (int64_t at = first + l; at < last; at += WAVE_SIZE) {
+                const u32 col = a.cols[at];
+                for (int64_t in = p.starts[col]; in < p.starts[col + 1]; ++in) f(p.cols[in]);
+            }
+        },
+        bitmap, slots, !COUNT);
+    if (k == UINT_MAX) {
+        if (lane == 0) atomicOr(overflow, 1u);
+        return;
+    }
+    if (COUNT) {
+        if (lane == 0) counts[row] = k;
+        return;
+    }
+    const int64_t base = out_starts[row];
+    for (u32 g = 0; g < k; g += WAVE_SIZE) {
+        const u32 s = g + (u32)lane;
+        const u32 mine = s < k ? slots[s] : UINT_MAX;
+        double acc[RB * MODES6];
+#pragma unroll
+        for (int i = 0; i < RB * MODES6; ++i) acc[i] = 0.0;
+        for (int64_t at = first; at < last; ++at) {
+            const u32 col = a.cols[at];
+            const double *b = a.vals + (size_t)at * RB * RB;
+            for (int64_t in = p.starts[col]; in < p.starts[col + 1]; ++in) {
+                if (p.cols[in] != mine) continue;
+                const double *pb = p.vals + (size_t)in * RB * MODES6;
+#pragma unroll
+                for (int r = 0; r < RB; ++r)
+#pragma unroll
+                    for (int mid = 0; mid < RB; ++mid) {
+                        const double v = b[r * RB + mid];
+#pragma unroll
+                        for (int q = 0; q < MODES6; ++q) acc[r * MODES6 + q] += v * pb[mid * MODES6 + q];
+                    }
+            }
+        }
+        if (s >= k) continue;
+        out_cols[base + s] = mine;
+        double *out = out_vals + (size_t)(base + s) * RB * MODES6;
+#pragma unroll
+        for (int i = 0; i < RB * MODES6; ++i) out[i] = acc[i];
+    }
+}
+
+// P^T (A P) by coarse rows: the prolongator's entries of column c by ascending fine row, each
+// transposed block times that fine row of A P, its blocks in order (transfer.rs:204-238).
+template <int RB, bool COUNT>
+__global__ __launch_bounds__(64) void galerkin_kernel(size_t count, const int64_t *__restrict__ pcol_starts,
+                                                      const u32 *__restrict__ pcol_rows, const int64_t *__restrict__ pcol_at,
+                                                      const double *__restrict__ pvals, SetupRows ap, u32 *__restrict__ counts,
+                                                      const int64_t *__restrict__ out_starts, u32 *__restrict__ out_cols,
+                                                      double *__restrict__ out_vals, u32 *overflow) {
+    __shared__ u32 bitmap[WINDOW_WORDS];
+    __shared__ u32 slots[MAX_SLOTS];
+    const size_t c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t first = pcol_starts[c], last = pcol_starts[c + 1];
+    const u32 k = row_columns(
+        [&](int l, auto f) {
+            for (int64_t e = first + l; e < last; e += WAVE_SIZE) {
+                const u32 fine = pcol_rows[e];
+                for (int64_t in = ap.starts[fine]; in < ap.starts[fine + 1]; ++in) f(ap.cols[in]);
+            }
+        },
+        bitmap, slots, !COUNT);
+    if (k == UINT_MAX) {
+        if (lane == 0) atomicOr(overflow, 1u);
+        return;
+    }
+    if (COUNT) {
+        if (lane == 0) counts[c] = k;
+        return;
+    }
+    const int64_t base = out_starts[c];
+    for (u32 g = 0; g < k; g += WAVE_SIZE) {
+        const u32 s = g + (u32)lane;
+        const u32 mine = s < k ? slots[s] : UINT_MAX;
+        double acc[MODES6 * MODES6];
+#pragma unroll
+        for (int i = 0; i < MODES6 * MODES6; ++i) acc[i] = 0.0;
+        for (int64_t e = first; e < last; ++e) {
+            const u32 fine = pcol_rows[e];
+            const double *pb = pvals + (size_t)pcol_at[e] * RB * MODES6;
+            for (int64_t in = ap.starts[fine]; in < ap.starts[fine + 1]; ++in) {
+                if (ap.cols[in] != mine) continue;
+                const double *apb = ap.vals + (size_t)in * RB * MODES6;
+#pragma unroll
+                for (int r = 0; r < MODES6; ++r)
+#pragma unroll
+                    for (int mid = 0; mid < RB; ++mid) {
+                        const double v = pb[mid * MODES6 + r];
+#pragma unroll
+                        for (int q = 0; q < MODES6; ++q) acc[r * MODES6 + q] += v * apb[mid * MODES6 + q];
+                    }
+            }
+        }
+        if (s >= k) continue;
+        out_cols[base + s] = mine;
+        double *out = out_vals + (size_t)(base + s) * MODES6 * MODES6;
+#pragma unroll
+        for (int i = 0; i < MODES6 * MODES6; ++i) out[i] = acc[i];
+    }
+}
+
+size_t grid_of(size_t n, size_t threads) { return (n + threads - 1) / threads; }
+
+// Host prefix sum of per-row counts into row starts.
+std::vector<int64_t> starts_of(const std::vector<u32> &counts) {
+    std::vector<int64_t> starts(counts.size() + 1, 0);
+    for (size_t i = 0; i < counts.size(); ++i) starts[i + 1] = starts[i] + counts[i];
+    return starts;
+}
+
+}  // namespace
+
+DeviceSetup::DeviceSetup(std::shared_ptr<Context> ctx, size_t min_rows, const double *level0_values)
+    : ctx_(std::move(ctx)), min_rows_(min_rows), level0_values_(level0_values) {}
+
+DeviceSetup::~DeviceSetup() {
+    if (ctx_) (void)hipStreamSynchronize(ctx_->stream);
+}
+
+bool DeviceSetup::takes(int level, const HostBsr<double> &a) const {
+    return a.rows >= min_rows_ && (a.rb == 3 || a.rb == 6) && a.cb == a.rb && a.rows < (size_t)UINT_MAX;
+}
+
+void DeviceSetup::begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal) {
+    hipStream_t s = ctx_->stream;
+    level_ = level;
+    rb_ = (int)a.rb;
+    rows_ = a.rows;
+    a_starts_.upload(a.starts, s);
+    a_cols_.upload(a.columns, s);
+    if (level == 0 && level0_values_) {
+        a_vals_.release();
+        a_vals_ptr_ = level0_values_;
+    } else {
+        a_vals_.upload(a.values, s);
+        a_vals_ptr_ = a_vals_.ptr;
+    }
+    dinv_.upload(inverse_diagonal, s);
+    p_cols_.release();
+    p_vals_.release();
+    p_starts_.release();
+    ap_starts_.release();
+    ap_cols_.release();
+    ap_vals_.release();
+}
+
+void DeviceSetup::check_overflow(const char *phase) {
+    u32 flag = 0;
+    overflow_.download(&flag, 1, ctx_->stream);
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
+    if (flag) throw Overflow{};
+    (void)phase;
+}
+
+void DeviceSetup::clear_overflow() {
+    if (!overflow_.ptr) overflow_.resize(1);
+    const u32 zero = 0;
+    overflow_.upload(&zero, 1, ctx_->stream);
+}
+
+double DeviceSetup::spectral(std::vector<double> &power, int iterations) {
+    hipStream_t s = ctx_->stream;
+    const size_t n = rows_ * (size_t)rb_;
+    x_.upload(power, s);
+    y_.resize(n);
+    host_y_.resize(n);
+    double norm = 0.0;
+    for (int it = 0; it < iterations; ++it) {
+        if (rb_ == 3)
+            power_kernel<3><<<grid_of(rows_, 256), 256, 0, s>>>(rows_, a_starts_.ptr, a_cols_.ptr, a_vals_ptr_, dinv_.ptr, x_.ptr, y_.ptr);
+        else
+            power_kernel<6><<<grid_of(rows_, 256), 256, 0, s>>>(rows_, a_starts_.ptr, a_cols_.ptr, a_vals_ptr_, dinv_.ptr, x_.ptr, y_.ptr);
+        SF_HIP(hipGetLastError());
+        y_.download(host_y_.data(), n, s);
+        SF_HIP(hipStreamSynchronize(s));
+        double sum = 0.0;
+        for (double v : host_y_) sum += v * v;
+        norm = std::sqrt(sum);
+        divide_kernel<<<grid_of(n, 256), 256, 0, s>>>(n, y_.ptr, norm, x_.ptr);
+        SF_HIP(hipGetLastError());
+    }
+    SF_HIP(hipStreamSynchronize(s));
+    return norm;
+}
+
+bool DeviceSetup::smooth(const std::vector<u32> &aggregate, u32 count, const std::vector<int64_t> &ns, const std::vector<u32> &nc,
+                         const std::vector<double> &tentative, double omega, HostBsr<double> &p) {
+    hipStream_t s = ctx_->stream;
+    DeviceBuffer<u32> agg, ncols;
+    DeviceBuffer<int64_t> nstarts;
+    DeviceBuffer<double> tent;
+    agg.upload(aggregate, s);
+    nstarts.upload(ns, s);
+    if (!nc.empty()) ncols.upload(nc, s); else ncols.resize(1);
+    tent.upload(tentative, s);
+    DeviceBuffer<u32> counts(rows_);
+    clear_overflow();
+    const SetupRows a{a_starts_.ptr, a_cols_.ptr, a_vals_ptr_};
+    auto launch = [&](auto count_pass, const int64_t *out_starts, u32 *out_cols, double *out_vals) {
+        constexpr bool C = decltype(count_pass)::value;
+        if (rb_ == 3)
+            smooth_kernel<3, C><<<rows_, 64, 0, s>>>(rows_, a, agg.ptr, nstarts.ptr, ncols.ptr, tent.ptr, dinv_.ptr, omega, counts.ptr,
+                                                     out_starts, out_cols, out_vals, overflow_.ptr);
+        else
+            smooth_kernel<6, C><<<rows_, 64, 0, s>>>(rows_, a, agg.ptr, nstarts.ptr, ncols.ptr, tent.ptr, dinv_.ptr, omega, counts.ptr,
+                                                     out_starts, out_cols, out_vals, overflow_.ptr);
+        SF_HIP(hipGetLastError());
+    };
+    launch(std::true_type{}, nullptr, nullptr, nullptr);
+    try {
+        check_overflow("smooth");
+    } catch (Overflow &) {
+        return false;
+    }
+    std::vector<u32> host_counts(rows_);
+    counts.download(host_counts.data(), rows_, s);
+    SF_HIP(hipStreamSynchronize(s));
+    p.rows = rows_;
+    p.cols = count;
+    p.rb = (size_t)rb_;
+    p.cb = MODES6;
+    p.starts = starts_of(host_counts);
+    const size_t blocks = (size_t)p.starts.back();
+    p_starts_.upload(p.starts, s);
+    p_cols_.resize(std::max<size_t>(blocks, 1));
+    p_vals_.resize(std::max<size_t>(blocks * rb_ * MODES6, 1));
+    launch(std::false_type{}, p_starts_.ptr, p_cols_.ptr, p_vals_.ptr);
+    p.columns.resize(blocks);
+    p.values.resize(blocks * rb_ * MODES6);
+    p_cols_.download(p.columns.data(), blocks, s);
+    p_vals_.download(p.values.data(), p.values.size(), s);
+    SF_HIP(hipStreamSynchronize(s));
+    return true;
+}
+
+bool DeviceSetup::product(u32 count) {
+    hipStream_t s = ctx_->stream;
+    DeviceBuffer<u32> counts(rows_);
+    clear_overflow();
+    const SetupRows a{a_starts_.ptr, a_cols_.ptr, a_vals_ptr_};
+    const SetupRows p{p_starts_.ptr, p_cols_.ptr, p_vals_.ptr};
+    auto launch = [&](auto count_pass, const int64_t *out_starts, u32 *out_cols, double *out_vals) {
+        constexpr bool C = decltype(count_pass)::value;
+        if (rb_ == 3)
+            product_kernel<3, C><<<rows_, 64, 0, s>>>(rows_, a, p, counts.ptr, out_starts, out_cols, out_vals, overflow_.ptr);
+        else
+            product_kernel<6, C><<<rows_, 64, 0, s>>>(rows_, a, p, counts.ptr, out_starts, out_cols, out_vals, overflow_.ptr);
+        SF_HIP(hipGetLastError());
+    };
+    launch(std::true_type{}, nullptr, nullptr, nullptr);
+    try {
+        check_overflow("product");
+    } catch (Overflow &) {
+        return false;
+    }
+    std::vector<u32> host_counts(rows_);
+    counts.download(host_counts.data(), rows_, s);
+    SF_HIP(hipStreamSynchronize(s));
+    const std::vector<int64_t> starts = starts_of(host_counts);
+    const size_t blocks = (size_t)starts.back();
+    ap_starts_.upload(starts, s);
+    ap_cols_.resize(std::max<size_t>(blocks, 1));
+    ap_vals_.resize(std::max<size_t>(blocks * rb_ * MODES6, 1));
+    launch(std::false_type{}, ap_starts_.ptr, ap_cols_.ptr, ap_vals_.ptr);
+    SF_HIP(hipStreamSynchronize(s));
+    (void)count;
+    return true;
+}
+
+bool DeviceSetup::galerkin(u32 count, const std::vector<int64_t> &pcol_starts, const std::vector<std::pair<u32, int64_t>> &entries,
+                           HostBsr<double> &coarse) {
+    hipStream_t s = ctx_->stream;
+    std::vector<u32> rows(entries.size());
+    std::vector<int64_t> at(entries.size());
+    for (size_t i = 0; i < entries.size(); ++i) {
+        rows[i] = entries[i].first;
+        at[i] = entries[i].second;
+    }
+    DeviceBuffer<int64_t> cstarts, cat;
+    DeviceBuffer<u32> crows;
+    cstarts.upload(pcol_starts, s);
+    if (rows.empty()) {
+        rows.push_back(0);
+        at.push_back(0);
+    }
+    crows.upload(rows, s);
+    cat.upload(at, s);
+    DeviceBuffer<u32> counts(std::max<size_t>(count, 1));
+    clear_overflow();
+    const SetupRows ap{ap_starts_.ptr, ap_cols_.ptr, ap_vals_.ptr};
+    auto launch = [&](auto count_pass, const int64_t *out_starts, u32 *out_cols, double *out_vals) {
+        constexpr bool C = decltype(count_pass)::value;
+        if (count == 0) return;
+        if (rb_ == 3)
+            galerkin_kernel<3, C><<<count, 64, 0, s>>>(count, cstarts.ptr, crows.ptr, cat.ptr, p_vals_.ptr, ap, counts.ptr, out_starts,
+                                                       out_cols, out_vals, overflow_.ptr);
+        else
+            galerkin_kernel<6, C><<<count, 64, 0, s>>>(count, cstarts.ptr, crows.ptr, cat.ptr, p_vals_.ptr, ap, counts.ptr, out_starts,
+                                                       out_cols, out_vals, overflow_.ptr);
+        SF_HIP(hipGetLastError());
+    };
+    launch(std::true_type{}, nullptr, nullptr, nullptr);
+    try {
+        check_overflow("galerkin");
+    } catch (Overflow &) {
+        return false;
+    }
+    std::vector<u32> host_counts(count);
+    counts.download(host_counts.data(), count, s);
+    SF_HIP(hipStreamSynchronize(s));
+    coarse.rows = coarse.cols = count;
+    coarse.rb = coarse.cb = MODES6;
+    coarse.starts = starts_of(host_counts);
+    const size_t blocks = (size_t)coarse.starts.back();
+    DeviceBuffer<int64_t> out_starts;
+    out_starts.upload(coarse.starts, s);
+    DeviceBuffer<u32> out_cols(std::max<size_t>(blocks, 1));
+    DeviceBuffer<double> out_vals(std::max<size_t>(blocks * MODES6 * MODES6, 1));
+    launch(std::false_type{}, out_starts.ptr, out_cols.ptr, out_vals.ptr);
+    coarse.columns.resize(blocks);
+    coarse.values.resize(blocks * MODES6 * MODES6);
+    out_cols.download(coarse.columns.data(), blocks, s);
+    out_vals.download(coarse.values.data(), coarse.values.size(), s);
+    SF_HIP(hipStreamSynchronize(s));
+    // the level is done: its device copies go
+    ap_starts_.release();
+    ap_cols_.release();
+    ap_vals_.release();
+    return true;
+}
+
+void DeviceSetup::end_level() {
+    a_starts_.release();
+    a_cols_.release();
+    a_vals_.release();
+    a_vals_ptr_ = nullptr;
+    dinv_.release();
+    p_starts_.release();
+    p_cols_.release();
+    p_vals_.release();
+    ap_starts_.release();
+    ap_cols_.release();
+    ap_vals_.release();
+    x_.release();
+    y_.release();
+    host_y_.clear();
+    host_y_.shrink_to_fit();
+}
+
+}  // namespace sf

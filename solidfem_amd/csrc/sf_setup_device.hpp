@@ -1,0 +1,47 @@
+// The device side of the multigrid setup: a level's power iteration, smoothed prolongator,
+// A P and Galerkin product on the GPU, bit-identical to the host phases they replace.
+#pragma once
+
+#include <memory>
+
+#include "sf_device.hpp"
+#include "sf_setup.hpp"
+
+namespace sf {
+
+class DeviceSetup final : public SetupAccelerator {
+   public:
+    // Levels of at least `min_rows` block rows run their numeric phases here. `level0_values`,
+    // when given, is the finest matrix's values already on the device in block-row order (the
+    // study's assembled stiffness); the setup borrows it while level 0 is built.
+    DeviceSetup(std::shared_ptr<Context> ctx, size_t min_rows, const double *level0_values);
+    ~DeviceSetup() override;
+
+    bool takes(int level, const HostBsr<double> &a) const override;
+    void begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal) override;
+    double spectral(std::vector<double> &power, int iterations) override;
+    bool smooth(const std::vector<u32> &aggregate, u32 count, const std::vector<int64_t> &ns, const std::vector<u32> &nc,
+                const std::vector<double> &tentative, double omega, HostBsr<double> &p) override;
+    bool product(u32 count) override;
+    bool galerkin(u32 count, const std::vector<int64_t> &pcol_starts, const std::vector<std::pair<u32, int64_t>> &entries,
+                  HostBsr<double> &coarse) override;
+    void end_level() override;
+
+   private:
+    struct Overflow {};
+    void check_overflow(const char *phase);
+    void clear_overflow();
+
+    std::shared_ptr<Context> ctx_;
+    size_t min_rows_;
+    const double *level0_values_;
+    int level_ = -1, rb_ = 3;
+    size_t rows_ = 0;
+    DeviceBuffer<int64_t> a_starts_, p_starts_, ap_starts_;
+    DeviceBuffer<u32> a_cols_, p_cols_, ap_cols_, overflow_;
+    DeviceBuffer<double> a_vals_, dinv_, p_vals_, ap_vals_, x_, y_;
+    const double *a_vals_ptr_ = nullptr;
+    std::vector<double> host_y_;
+};
+
+}  // namespace sf

@@ -1,0 +1,116 @@
+"""The multigrid setup's numeric phases on the device against the host phases, bit for bit.
+
+The power iteration (crates/solidfem-solve/src/multigrid/setup.rs:390-449), the smoothed
+prolongator (setup.rs:527-573, transfer.rs:108-152), A P (transfer.rs:155-169) and the Galerkin
+product (transfer.rs:204-238) run on the device when the setup has a context. Every level's
+matrix, block inverse, spectrum, prolongator and aggregates, and the coarse factor, must be the
+host setup's bits, which the CPU suite pins to the oracle; here the oracle checks the device
+hierarchy directly too.
+"""
+import numpy as np
+import pytest
+
+import refcpu
+import solidfem_amd as sf
+from workloads import FINGERPRINT_CANTILEVER, FINGERPRINT_PLATE, bar_config, block_config, loaded, plate_config
+
+pytestmark = pytest.mark.gpu
+
+WORKLOADS = {
+    "cantilever": (loaded(*FINGERPRINT_CANTILEVER[:2]), 100),
+    "plate": (loaded(*FINGERPRINT_PLATE[:2]), 100),
+    "jittered-plate": (plate_config(divisions=(12, 12, 2)), 150),
+    "random-block": (block_config(3, divisions=(6, 4, 4)), 60),
+    "shell-core-torsion": (bar_config(divisions=(16, 4, 4)), 90),
+}
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def run_setup(ctx, ref, options, budget=2 ** 62):
+    rs, cols, vals = ref.csr()
+    csr = sf.CsrMatrix.new(None, rs, cols, vals)
+    setup = sf.MultigridSetup.new(ctx, csr, ref.rigid_modes(), ref.free_positions(), options)
+    while not setup.step(budget):
+        pass
+    return setup
+
+
+def assert_same_setup(a, b):
+    assert a.level_count() == b.level_count()
+    levels = a.level_count()
+    for level in range(levels):
+        ma, mb = a.level_matrix(level), b.level_matrix(level)
+        assert np.array_equal(ma["starts"], mb["starts"]) and np.array_equal(ma["columns"], mb["columns"]), level
+        assert np.array_equal(bits(ma["values"]), bits(mb["values"])), f"level {level} matrix"
+        assert np.array_equal(bits(a.level_inverse_diagonal(level)), bits(b.level_inverse_diagonal(level)))
+        assert a.smoother_spectrum(level) == b.smoother_spectrum(level), f"level {level} spectrum"
+        if level + 1 < levels:
+            pa, pb = a.level_matrix(level, prolongator=True), b.level_matrix(level, prolongator=True)
+            assert np.array_equal(pa["starts"], pb["starts"]) and np.array_equal(pa["columns"], pb["columns"])
+            assert np.array_equal(bits(pa["values"]), bits(pb["values"])), f"level {level} prolongator"
+            assert np.array_equal(a.aggregates(level), b.aggregates(level))
+    fa, fb = a.coarse_factor(), b.coarse_factor()
+    assert (fa is None) == (fb is None)
+    if fa is not None:
+        assert np.array_equal(bits(np.tril(fa)), bits(np.tril(fb)))
+
+
+def assert_matches_oracle(setup, mg):
+    assert setup.level_count() == mg.levels
+    for level in range(mg.levels):
+        starts, cols, vals = mg.matrix(level)
+        m = setup.level_matrix(level)
+        assert np.array_equal(m["columns"], cols) and np.array_equal(bits(m["values"]), bits(vals)), f"level {level}"
+        assert setup.smoother_spectrum(level) == mg.spectrum(level)
+        if level + 1 < mg.levels:
+            ps, pc, pv = mg.prolongator(level)
+            p = setup.level_matrix(level, prolongator=True)
+            assert np.array_equal(p["columns"], pc) and np.array_equal(bits(p["values"]), bits(pv)), f"level {level} P"
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+def test_device_setup_is_the_host_setup_and_the_oracle(ctx, name):
+    w, coarse = WORKLOADS[name]
+    ref = refcpu.Problem(w)
+    options = sf.MultigridOptions(coarse_dofs=coarse, precision=sf.Precision.DOUBLE, setup_device=sf.SetupDevice.DEVICE)
+    device = run_setup(ctx, ref, options)
+    host = run_setup(None, ref, options)
+    assert_same_setup(device, host)
+    assert_matches_oracle(device, refcpu.Multigrid(ref, refcpu.options(precision=0, coarse_dofs=coarse)))
+
+
+@pytest.mark.parametrize("budget", [1, 977])
+def test_a_sliced_device_setup_gives_the_same_bits(ctx, budget):
+    w, coarse = WORKLOADS["jittered-plate"]
+    ref = refcpu.Problem(w)
+    options = sf.MultigridOptions(coarse_dofs=coarse, precision=sf.Precision.DOUBLE, setup_device=sf.SetupDevice.DEVICE)
+    assert_same_setup(run_setup(ctx, ref, options, budget=budget), run_setup(None, ref, options))
+
+
+def test_automatic_placement_on_a_larger_plate(ctx):
+    # about 36,000 block rows: level 0 runs on the device, the coarse levels on the host
+    ref = refcpu.Problem(plate_config(divisions=(40, 40, 2)))
+    auto = run_setup(ctx, ref, sf.MultigridOptions(precision=sf.Precision.SINGLE))
+    host = run_setup(ctx, ref, sf.MultigridOptions(precision=sf.Precision.SINGLE, setup_device=sf.SetupDevice.HOST))
+    assert auto.level_matrix(0)["starts"].size - 1 >= 4096
+    assert_same_setup(auto, host)
+
+
+@pytest.mark.parametrize("case", [FINGERPRINT_CANTILEVER, FINGERPRINT_PLATE], ids=["cantilever", "plate"])
+def test_a_device_built_study_solves_to_the_fingerprint(ctx, case):
+    # the study builder hands its assembled values on the device to level 0's setup
+    size, div, iterations, fingerprint = case
+    b = sf.StudyBuilder.new(ctx, sf.Model.from_workload(loaded(size, div)),
+                            sf.PcgOptions(tolerance=1e-9, dot_order=sf.DotOrder.REFERENCE),
+                            sf.MultigridOptions(precision=sf.Precision.DOUBLE, coarse_dofs=100, setup_device=sf.SetupDevice.DEVICE))
+    while True:
+        p = b.step(997)
+        if p.elements_done == p.elements_total and p.setup_finished:
+            break
+    s = b.finish()
+    p = s.solve()
+    assert p.outcome == sf.StepOutcome.CONVERGED
+    assert (p.iteration, refcpu.fingerprint(s.displacements())) == (iterations, fingerprint)
