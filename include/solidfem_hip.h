@@ -70,10 +70,16 @@ enum {
     SF_ERR_MODEL_NOT_FINISHED = 48,
     SF_ERR_MODEL_AXIS_OUT_OF_RANGE = 49,   /* a=axis */
     SF_ERR_MODEL_PRESCRIBED_ON_FIXED = 50, /* a=node */
+    SF_ERR_MODEL_BODY_NOT_HELD = 51,       /* a=first tet, b=held nodes, value=SF_FREEDOM_* left */
+    SF_ERR_MODEL_DOFS_DO_NOT_HOLD = 52,    /* a=first tet, b=rigid motions held (rank < 6) */
     /* serving */
     SF_ERR_SERVE_OVER_BUDGET = 60,         /* a=bytes needed, b=budget */
     SF_ERR_SERVE_UNKNOWN_REQUEST = 61
 };
+
+/* The rigid motion an insufficiently held body keeps (Freedom,
+ * crates/solidfem-elastic/src/constraints.rs:14-23). */
+enum { SF_FREEDOM_FREE = 0, SF_FREEDOM_ROTATION_ABOUT_POINT = 1, SF_FREEDOM_ROTATION_ABOUT_LINE = 2 };
 
 /* Precision of the finished hierarchy, crates/solidfem-solve/src/multigrid/mod.rs:126-141 */
 enum { SF_PRECISION_DOUBLE = 0, SF_PRECISION_SINGLE_STORAGE = 1, SF_PRECISION_SINGLE = 2 };
@@ -209,6 +215,11 @@ typedef struct sf_workload {
     double core_youngs, core_poisson;
     double shell_depth;
     int32_t quadrature_points;
+    /* A tube when wall_cells > 0 (demo/wasm/src/part.rs:48-185): along the longest axis, the
+     * two axes across it get wall_cells cells through each wall of wall_thickness and their
+     * remaining divisions evenly over the hollow, whose cells are removed. Not jittered. */
+    int32_t wall_cells;
+    double wall_thickness;
 } sf_workload;
 
 typedef struct sf_context sf_context;

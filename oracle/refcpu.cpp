@@ -200,8 +200,27 @@ static std::vector<double> free_rigid_body_modes(const std::vector<Vec3> &nodes,
 static std::unique_ptr<Problem> build_problem(const ref_workload &w) {
     double t0 = now_seconds();
     auto p = std::make_unique<Problem>();
-    LinearMesh linear = box_tet_mesh(w.origin, w.size, w.divisions);
-    jitter_vertices(linear, w);
+    LinearMesh linear;
+    if (w.wall_cells > 0) {
+        // a bar along its longest axis with a hollow section (demo/wasm/src/part.rs:87-185)
+        int along = 0;
+        for (int a = 1; a < 3; ++a)
+            if (w.size[a] > w.size[along]) along = a;
+        const int across[2] = {(along + 1) % 3, (along + 2) % 3};
+        std::vector<double> lines[3];
+        for (int a = 0; a < 3; ++a) {
+            if (a == along) {
+                for (int i = 0; i <= w.divisions[a]; ++i) lines[a].push_back(box_coord(w.origin[a], w.size[a], (size_t)i, (size_t)w.divisions[a]));
+            } else {
+                for (double x : tube_cross_lines(w.size[a], w.wall_thickness, w.wall_cells, w.divisions[a] - 2 * w.wall_cells))
+                    lines[a].push_back(w.origin[a] + x);
+            }
+        }
+        linear = tube_tet_mesh(lines, across, w.wall_cells);
+    } else {
+        linear = box_tet_mesh(w.origin, w.size, w.divisions);
+        jitter_vertices(linear, w);
+    }
     validate_linear(linear);
     p->mesh = quadratic_from_linear(linear);
     p->boundary = boundary_faces(p->mesh);

@@ -62,6 +62,10 @@ typedef struct ref_workload {
     double core_youngs, core_poisson;   /* shell/core only */
     double shell_depth;                 /* shell/core only */
     int32_t quadrature_points;          /* 4 or 14 */
+    /* a tube about the longest axis when wall_cells > 0: wall_cells cells through each wall of
+     * wall_thickness on the two axes across, the rest of their divisions over the hollow */
+    int32_t wall_cells;
+    double wall_thickness;
 } ref_workload;
 
 typedef struct ref_mg_options {

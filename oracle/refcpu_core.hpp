@@ -78,6 +78,57 @@ inline LinearMesh box_tet_mesh(const double origin[3], const double size[3], con
     return m;
 }
 
+// demo/wasm/src/part.rs:48-82: the grid lines across a tube's wall, `walls` cells through each
+// wall and `inner` cells over the hollow's span, measured from 0.
+inline std::vector<double> tube_cross_lines(double along, double wall, int walls, int inner) {
+    const double hollow = along - 2.0 * wall, step = wall / (double)walls;
+    std::vector<double> lines;
+    for (int i = 0; i <= walls; ++i) lines.push_back(step * (double)i);
+    for (int i = 1; i < inner; ++i) lines.push_back(wall + hollow * (double)i / (double)inner);
+    for (int i = 0; i <= walls; ++i) lines.push_back(along - wall + step * (double)i);
+    return lines;
+}
+
+// demo/wasm/src/part.rs:111-185: a unit-spaced box grid whose tetrahedra are kept unless the
+// integer floor of their centroid lies in the hollow, vertices renumbered by first use and
+// moved to the grid lines at their rounded unit coordinates. `across` are the two axes of the
+// section; `lines[a]` the grid lines of axis a.
+inline LinearMesh tube_tet_mesh(const std::vector<double> lines[3], const int across[2], int walls) {
+    int cells[3];
+    double unit_size[3];
+    for (int a = 0; a < 3; ++a) {
+        cells[a] = (int)lines[a].size() - 1;
+        unit_size[a] = (double)cells[a];
+    }
+    const double zero[3] = {0.0, 0.0, 0.0};
+    const LinearMesh unit = box_tet_mesh(zero, unit_size, cells);
+    auto hollow = [&](size_t j, size_t k) {
+        return walls > 0 && j >= (size_t)walls && j < (size_t)(cells[across[0]] - walls) && k >= (size_t)walls &&
+               k < (size_t)(cells[across[1]] - walls);
+    };
+    LinearMesh m;
+    std::vector<u32> remap(unit.positions.size(), 0xffffffffu);
+    for (const auto &tet : unit.tets) {
+        double centroid[3] = {0.0, 0.0, 0.0};
+        for (u32 n : tet)
+            for (int a = 0; a < 3; ++a) centroid[a] = centroid[a] + unit.positions[n][a] * 0.25;
+        if (hollow((size_t)std::floor(centroid[across[0]]), (size_t)std::floor(centroid[across[1]]))) continue;
+        std::array<u32, 4> out;
+        for (int c = 0; c < 4; ++c) {
+            u32 &slot = remap[tet[c]];
+            if (slot == 0xffffffffu) {
+                slot = (u32)m.positions.size();
+                const Vec3 &p = unit.positions[tet[c]];
+                m.positions.push_back(Vec3{{lines[0][(size_t)std::round(p[0])], lines[1][(size_t)std::round(p[1])],
+                                            lines[2][(size_t)std::round(p[2])]}});
+            }
+            out[c] = slot;
+        }
+        m.tets.push_back(out);
+    }
+    return m;
+}
+
 // tet.rs:9-16
 inline double tet_volume(const Vec3 &a, const Vec3 &b, const Vec3 &c, const Vec3 &d) {
     double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};

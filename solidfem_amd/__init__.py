@@ -4,8 +4,9 @@ A Python mirror of the reference's Rust API (crates/solidfem-solve/src/lib.rs:28
 crates/solidfem-elastic/src/lib.rs:44-66) over the C ABI in include/solidfem_hip.h. Every
 compute call runs in libsolidfem_hip.so on the GPU: Tet10 stiffness assembly into 3x3 blocks,
 the block smoothed-aggregation multigrid V-cycle with its Chebyshev smoother, and the
-resumable, cancellable, slice-invariant conjugate gradient solve. The multigrid setup runs on
-the host threads inside the same library.
+resumable, cancellable, slice-invariant conjugate gradient solve. The multigrid setup's phase
+machine and aggregation run on host threads inside the same library, and the numeric phases of
+its large levels on the GPU.
 """
 from __future__ import annotations
 
@@ -22,6 +23,7 @@ __all__ = [
     "Context", "Precision", "DotOrder", "PcgOptions", "MultigridOptions", "Workload", "TetMesh", "QuadraticTetMesh",
     "box_tet_mesh", "Model", "StudyBuilder", "StaticStudy", "CsrMatrix", "MultigridSetup", "Multigrid", "Pcg",
     "StepOutcome", "SolidfemError", "PcgError", "MultigridError", "ModelError", "MeshError", "library",
+    "SetupDevice",
 ]
 
 
@@ -133,7 +135,7 @@ class MultigridOptions:
 
 @dataclass
 class Workload:
-    """A structured box problem (see include/solidfem_hip.h: sf_workload)."""
+    """A structured box or tube problem (see include/solidfem_hip.h: sf_workload)."""
     size: tuple = (1.0, 1.0, 1.0)
     divisions: tuple = (1, 1, 1)
     origin: tuple = (0.0, 0.0, 0.0)
@@ -157,6 +159,8 @@ class Workload:
     core_poisson: float = 0.0
     shell_depth: float = 0.0
     quadrature_points: int = 4
+    wall_cells: int = 0  # a tube when positive: cells through each wall across the longest axis
+    wall_thickness: float = 0.0
 
     def fields(self):
         return dict(origin=self.origin, size=self.size, divisions=self.divisions, jitter_fraction=self.jitter_fraction,
@@ -166,7 +170,8 @@ class Workload:
                     load_case=self.load_case, axis_u=self.axis_u, axis_v=self.axis_v, material_kind=self.material_kind,
                     youngs=self.youngs, poisson=self.poisson, core_youngs=self.core_youngs,
                     core_poisson=self.core_poisson, shell_depth=self.shell_depth,
-                    quadrature_points=self.quadrature_points)
+                    quadrature_points=self.quadrature_points, wall_cells=self.wall_cells,
+                    wall_thickness=self.wall_thickness)
 
     def c(self, struct=None):
         w = (struct or _capi.Workload)()

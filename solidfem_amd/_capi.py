@@ -111,6 +111,8 @@ class Workload(C.Structure):
         ("core_poisson", f64),
         ("shell_depth", f64),
         ("quadrature_points", i32),
+        ("wall_cells", i32),
+        ("wall_thickness", f64),
     ]
 
 

@@ -25,6 +25,8 @@ struct TetMesh {
 };
 void validate_tet_mesh(const TetMesh &mesh);  // throws Error(SF_ERR_MESH_*)
 TetMesh make_box_mesh(const double origin[3], const double size[3], const int32_t divisions[3]);
+// The tube of a workload with wall_cells > 0 (demo/wasm/src/part.rs:48-185).
+TetMesh make_tube_mesh(const sf_workload &w);
 
 struct Face {
     u32 tet;
@@ -55,6 +57,8 @@ struct QuadMesh {
     std::vector<Face> boundary_faces() const;  // quadratic.rs:89-124 order
 };
 QuadMesh upgrade_to_quadratic(const TetMesh &mesh);
+// The elements grouped into bodies joined through shared faces (quadratic.rs:129-166).
+std::vector<std::vector<u32>> face_connected_bodies(const QuadMesh &mesh);
 
 // Local node indices of each face (crates/solidfem-mesh/src/boundary.rs:8-13).
 extern const int FACE_LOCAL[4][6];

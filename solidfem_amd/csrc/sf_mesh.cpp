@@ -262,8 +262,78 @@ std::vector<Traction> workload_tractions(const QuadMesh &mesh, const std::vector
     return out;
 }
 
+namespace {
+
+int longest_axis(const double size[3]) {
+    int longest = 0;
+    for (int a = 1; a < 3; ++a)
+        if (size[a] > size[longest]) longest = a;
+    return longest;
+}
+
+}  // namespace
+
+TetMesh make_tube_mesh(const sf_workload &w) {
+    // demo/wasm/src/part.rs:48-185: the box's grid along the longest axis; across it, `wall_cells`
+    // cells through each wall of `wall_thickness` and the rest evenly over the hollow's span, whose
+    // cells are removed. Every cell is cut into the same six tetrahedra, so what remains conforms.
+    // Vertices are renumbered in order of first use by the kept tetrahedra, corner by corner.
+    const int along = longest_axis(w.size), across[2] = {(along + 1) % 3, (along + 2) % 3};
+    const int walls = w.wall_cells;
+    for (int a = 0; a < 3; ++a)
+        if (w.divisions[a] <= 0 || std::isnan(w.size[a]) || w.size[a] <= 0.0)
+            throw Error(SF_ERR_MESH_INVALID_BOX, "each axis needs at least one division and a positive length");
+    for (int a : across)
+        if (w.divisions[a] <= 2 * walls || !(w.wall_thickness > 0.0) || !(2.0 * w.wall_thickness < w.size[a]))
+            throw Error(SF_ERR_MESH_INVALID_BOX, "a tube needs a wall thinner than half its side and a hollow of at least one cell");
+    std::vector<double> lines[3];
+    for (int a = 0; a < 3; ++a) {
+        const size_t n = (size_t)w.divisions[a];
+        auto &l = lines[a];
+        if (a == along) {
+            for (size_t i = 0; i <= n; ++i) l.push_back(i == n ? w.origin[a] + w.size[a] : w.origin[a] + w.size[a] * ((double)i / (double)n));
+            continue;
+        }
+        // part.rs:58-82, the span measured from the origin
+        const double wall = w.wall_thickness, inner = w.size[a] - 2.0 * wall, step = wall / (double)walls;
+        const size_t inner_cells = n - 2 * (size_t)walls;
+        for (int i = 0; i <= walls; ++i) l.push_back(w.origin[a] + step * (double)i);
+        for (size_t i = 1; i < inner_cells; ++i) l.push_back(w.origin[a] + (wall + inner * (double)i / (double)inner_cells));
+        for (int i = 0; i <= walls; ++i) l.push_back(w.origin[a] + (w.size[a] - wall + step * (double)i));
+    }
+    const size_t n0 = (size_t)w.divisions[0], n1 = (size_t)w.divisions[1], nx = n0 + 1, ny = n1 + 1;
+    const double zero[3] = {0.0, 0.0, 0.0}, cells[3] = {(double)w.divisions[0], (double)w.divisions[1], (double)w.divisions[2]};
+    const TetMesh unit = make_box_mesh(zero, cells, w.divisions);
+    auto hollow = [&](const size_t cell[3]) {
+        const size_t j = cell[across[0]], k = cell[across[1]];
+        const size_t nj = (size_t)w.divisions[across[0]], nk = (size_t)w.divisions[across[1]];
+        return j >= (size_t)walls && j < nj - (size_t)walls && k >= (size_t)walls && k < nk - (size_t)walls;
+    };
+    TetMesh m;
+    std::vector<u32> remap(unit.positions.size(), FIXED);
+    for (size_t t = 0; t < unit.tets.size(); ++t) {
+        // the box builder emits six tetrahedra per cell, cells x-fastest
+        const size_t c = t / 6, cell[3] = {c % n0, (c / n0) % n1, c / (n0 * n1)};
+        if (hollow(cell)) continue;
+        std::array<u32, 4> tet;
+        for (int i = 0; i < 4; ++i) {
+            const u32 v = unit.tets[t][(size_t)i];
+            if (remap[v] == FIXED) {
+                remap[v] = (u32)m.positions.size();
+                const size_t at[3] = {v % nx, (v / nx) % ny, v / (nx * ny)};
+                m.positions.push_back(Vec3{{lines[0][at[0]], lines[1][at[1]], lines[2][at[2]]}});
+            }
+            tet[(size_t)i] = remap[v];
+        }
+        m.tets.push_back(tet);
+    }
+    return m;
+}
+
 Model model_from_workload(const sf_workload &w) {
-    TetMesh linear = make_box_mesh(w.origin, w.size, w.divisions);
+    if (w.wall_cells < 0) throw Error(SF_ERR_INVALID_ARGUMENT, "wall_cells must not be negative");
+    if (w.wall_cells > 0 && w.jitter_fraction > 0.0) throw Error(SF_ERR_INVALID_ARGUMENT, "a tube is not jittered");
+    TetMesh linear = w.wall_cells > 0 ? make_tube_mesh(w) : make_box_mesh(w.origin, w.size, w.divisions);
     if (w.jitter_fraction > 0.0) {
         // Interior vertices move by (2u - 1) * (fraction * h) per axis, u from (seed ^ (3 v + axis)).
         const size_t nx = (size_t)w.divisions[0] + 1, ny = (size_t)w.divisions[1] + 1;
@@ -293,9 +363,7 @@ Model model_from_workload(const sf_workload &w) {
     model.field.core_e = w.core_youngs;
     model.field.core_nu = w.core_poisson;
     model.field.shell_depth = w.shell_depth;
-    int longest = 0;
-    for (int a = 1; a < 3; ++a)
-        if (w.size[a] > w.size[longest]) longest = a;
+    const int longest = longest_axis(w.size);
     model.field.transverse[0] = (longest + 1) % 3;
     model.field.transverse[1] = (longest + 2) % 3;
     for (int a = 0; a < 3; ++a) {
@@ -335,11 +403,237 @@ static void face_load(const QuadMesh &mesh, const Face &face, const std::array<d
     }
 }
 
+// ---- held bodies (crates/solidfem-elastic/src/constraints.rs) -----------------------------
+
+std::vector<std::vector<u32>> face_connected_bodies(const QuadMesh &mesh) {
+    // quadratic.rs:129-166: elements sharing a corner triangle join; bodies come out ordered by
+    // their lowest element, each body's elements ascending. Any union order gives this partition,
+    // so the faces are matched by bucketing them on their lowest corner, then sorting each bucket.
+    const size_t n = mesh.corners.size(), vertices = mesh.vertex_count;
+    std::vector<u32> parent(n);
+    std::iota(parent.begin(), parent.end(), 0u);
+    auto find = [&](u32 x) {
+        while (parent[x] != x) {
+            parent[x] = parent[parent[x]];
+            x = parent[x];
+        }
+        return x;
+    };
+    struct Key {
+        u32 b, c, tet;
+    };
+    std::vector<u32> bucket_start(vertices + 2, 0);
+    auto face_of = [&](size_t t, int skip, u32 v[3]) {
+        u32 k = 0;
+        for (int i = 0; i < 4; ++i)
+            if (i != skip) v[k++] = mesh.corners[t][(size_t)i];
+        std::sort(v, v + 3);
+    };
+    for (size_t t = 0; t < n; ++t)
+        for (int skip = 0; skip < 4; ++skip) {
+            u32 v[3];
+            face_of(t, skip, v);
+            bucket_start[(size_t)v[0] + 2] += 1;
+        }
+    for (size_t i = 2; i < bucket_start.size(); ++i) bucket_start[i] += bucket_start[i - 1];
+    std::vector<Key> keys(4 * n);
+    for (size_t t = 0; t < n; ++t)
+        for (int skip = 0; skip < 4; ++skip) {
+            u32 v[3];
+            face_of(t, skip, v);
+            keys[bucket_start[(size_t)v[0] + 1]++] = {v[1], v[2], (u32)t};
+        }
+    for (size_t a = 0; a < vertices; ++a) {
+        Key *first = keys.data() + bucket_start[a], *last = keys.data() + bucket_start[a + 1];
+        std::sort(first, last, [](const Key &x, const Key &y) { return x.b != y.b ? x.b < y.b : x.c < y.c; });
+        for (Key *k = first + 1; k < last; ++k) {
+            if (k->b != k[-1].b || k->c != k[-1].c) continue;
+            const u32 r0 = find(k[-1].tet), r1 = find(k->tet);
+            if (r0 != r1) parent[std::max(r0, r1)] = std::min(r0, r1);
+        }
+    }
+    std::vector<std::vector<u32>> groups;
+    std::vector<u32> group_of(n, FIXED);
+    for (size_t t = 0; t < n; ++t) {
+        const u32 root = find((u32)t);
+        if (group_of[root] == FIXED) {
+            group_of[root] = (u32)groups.size();
+            groups.emplace_back();
+        }
+        groups[group_of[root]].push_back((u32)t);
+    }
+    return groups;
+}
+
+namespace {
+
+constexpr double COLLINEAR_FRACTION = 1e-9;  // constraints.rs:37-39
+
+double distance(const Vec3 &a, const Vec3 &b) {
+    const double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    return std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+// constraints.rs:256-268: the diagonal of the nodes' bounding box.
+double bounding_size(const std::vector<Vec3> &positions, const std::vector<u32> &nodes) {
+    Vec3 lo{{INFINITY, INFINITY, INFINITY}}, hi{{-INFINITY, -INFINITY, -INFINITY}};
+    for (u32 n : nodes)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::fmin(lo[a], positions[n][a]);
+            hi[a] = std::fmax(hi[a], positions[n][a]);
+        }
+    return distance(lo, hi);
+}
+
+// constraints.rs:41-72: -1 when the points hold the body, else the SF_FREEDOM_* left.
+int remaining_freedom(const std::vector<Vec3> &points, double size) {
+    if (points.empty()) return SF_FREEDOM_FREE;
+    const double limit = COLLINEAR_FRACTION * size;
+    const Vec3 first = points[0];
+    Vec3 far = first;
+    double far_distance = 0.0;
+    for (const Vec3 &p : points) {
+        const double d = distance(first, p);
+        if (d > far_distance) {
+            far = p;
+            far_distance = d;
+        }
+    }
+    if (far_distance <= limit) return SF_FREEDOM_ROTATION_ABOUT_POINT;
+    const double axis[3] = {(far[0] - first[0]) / far_distance, (far[1] - first[1]) / far_distance, (far[2] - first[2]) / far_distance};
+    for (const Vec3 &p : points) {
+        const double v[3] = {p[0] - first[0], p[1] - first[1], p[2] - first[2]};
+        const double along = v[0] * axis[0] + v[1] * axis[1] + v[2] * axis[2];
+        const double off[3] = {v[0] - along * axis[0], v[1] - along * axis[1], v[2] - along * axis[2]};
+        if (std::sqrt(off[0] * off[0] + off[1] * off[1] + off[2] * off[2]) > limit) return -1;
+    }
+    return SF_FREEDOM_ROTATION_ABOUT_LINE;
+}
+
+std::vector<Vec3> held_points(const std::vector<Vec3> &positions, const std::vector<u32> &nodes, const std::vector<char> &held) {
+    std::vector<Vec3> out;
+    for (u32 n : nodes)
+        if (held[n]) out.push_back(positions[n]);
+    return out;
+}
+
+// constraints.rs:81-135: every body needs three non-collinear held nodes; a body held so holds
+// every node it shares with the others, which may anchor them in turn.
+void check_constraints(const QuadMesh &mesh, const std::vector<char> &fixed) {
+    if (std::find(fixed.begin(), fixed.end(), 1) == fixed.end()) throw Error(SF_ERR_MODEL_NO_FIXED_NODES, "no node is fixed");
+    const auto bodies = face_connected_bodies(mesh);
+    std::vector<std::vector<u32>> body_nodes(bodies.size());
+    std::vector<double> sizes(bodies.size());
+    std::vector<u32> stamp(mesh.nodes.size(), FIXED);
+    for (size_t b = 0; b < bodies.size(); ++b) {
+        auto &nodes = body_nodes[b];
+        for (u32 t : bodies[b])
+            for (u32 node : mesh.tets[t])
+                if (stamp[node] != (u32)b) {
+                    stamp[node] = (u32)b;
+                    nodes.push_back(node);
+                }
+        std::sort(nodes.begin(), nodes.end());
+        sizes[b] = bounding_size(mesh.nodes, nodes);
+    }
+    std::vector<char> held = fixed, anchored(bodies.size(), 0);
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (size_t b = 0; b < bodies.size(); ++b) {
+            if (anchored[b] || remaining_freedom(held_points(mesh.nodes, body_nodes[b], held), sizes[b]) >= 0) continue;
+            anchored[b] = 1;
+            changed = true;
+            for (u32 n : body_nodes[b]) held[n] = 1;
+        }
+    }
+    for (size_t b = 0; b < bodies.size(); ++b) {
+        if (anchored[b]) continue;
+        const auto points = held_points(mesh.nodes, body_nodes[b], held);
+        const int freedom = remaining_freedom(points, sizes[b]);
+        throw Error(SF_ERR_MODEL_BODY_NOT_HELD, "a body is not held", bodies[b][0], (int64_t)points.size(),
+                    (double)(freedom < 0 ? SF_FREEDOM_FREE : freedom));
+    }
+}
+
+// constraints.rs:238-275: the rank of rows of six by elimination with row pivoting per column,
+// the pivot the first row of largest magnitude, counted when above COLLINEAR_FRACTION.
+size_t pivot_rank(std::vector<std::array<double, 6>> &rows) {
+    size_t rank = 0;
+    for (int column = 0; column < 6; ++column) {
+        if (rank >= rows.size()) break;
+        size_t best = rank;
+        for (size_t r = rank + 1; r < rows.size(); ++r)
+            if (std::fabs(rows[r][column]) > std::fabs(rows[best][column])) best = r;
+        if (std::fabs(rows[best][column]) <= COLLINEAR_FRACTION) continue;
+        std::swap(rows[rank], rows[best]);
+        const auto pivot = rows[rank];
+        for (size_t r = rank + 1; r < rows.size(); ++r) {
+            const double factor = rows[r][column] / pivot[column];
+            if (factor != 0.0)
+                for (int k = column; k < 6; ++k) rows[r][k] -= factor * pivot[k];
+        }
+        rank += 1;
+    }
+    return rank;
+}
+
+// constraints.rs:162-225: with prescribed components, the held components of each body must
+// span all six rigid motions (about the body's centroid, scaled by its size).
+void check_held_dofs(const QuadMesh &mesh, const std::vector<char> &fixed, const std::map<std::pair<u32, u32>, double> &prescribed) {
+    const auto &positions = mesh.nodes;
+    std::vector<char> in_body(positions.size(), 0);
+    for (const auto &body : face_connected_bodies(mesh)) {
+        std::fill(in_body.begin(), in_body.end(), 0);
+        std::vector<u32> nodes;
+        for (u32 t : body)
+            for (u32 n : mesh.tets[t])
+                if (!in_body[n]) {
+                    in_body[n] = 1;
+                    nodes.push_back(n);
+                }
+        std::sort(nodes.begin(), nodes.end());
+        const double size = std::fmax(bounding_size(positions, nodes), 2.2250738585072014e-308);
+        double center[3] = {0.0, 0.0, 0.0};
+        for (u32 n : nodes)
+            for (int a = 0; a < 3; ++a) center[a] += positions[n][a];
+        for (double &c : center) c /= (double)nodes.size();
+        std::vector<std::array<double, 6>> rows;
+        auto add = [&](u32 node, u32 axis) {
+            const Vec3 &p = positions[node];
+            const double x = (p[0] - center[0]) / size, y = (p[1] - center[1]) / size, z = (p[2] - center[2]) / size;
+            if (axis == 0)
+                rows.push_back({1.0, 0.0, 0.0, 0.0, z, -y});
+            else if (axis == 1)
+                rows.push_back({0.0, 1.0, 0.0, -z, 0.0, x});
+            else
+                rows.push_back({0.0, 0.0, 1.0, y, -x, 0.0});
+        };
+        for (u32 n : nodes)
+            if (fixed[n])
+                for (u32 a = 0; a < 3; ++a) add(n, a);
+        for (const auto &pr : prescribed) {
+            const u32 node = pr.first.first, axis = pr.first.second;
+            if (node < positions.size() && in_body[node] && axis < 3) add(node, axis);
+        }
+        const size_t rank = pivot_rank(rows);
+        if (rank < 6) throw Error(SF_ERR_MODEL_DOFS_DO_NOT_HOLD, "the held components leave a rigid motion free", body[0], (int64_t)rank);
+    }
+}
+
+}  // namespace
+
 FreeProblem number_and_load(const Model &model) {
     const size_t nn = model.mesh.nodes.size();
-    if (model.fixed.empty() && model.prescribed.empty()) throw Error(SF_ERR_MODEL_NO_FIXED_NODES, "no node is fixed");
-    for (u32 node : model.fixed)
+    // study.rs:419-432: the fixed flags, then the check that every body is held
+    std::vector<char> is_fixed(nn, 0);
+    for (u32 node : model.fixed) {
         if (node >= nn) throw Error(SF_ERR_MODEL_NODE_OUT_OF_RANGE, "fixed node out of range", node);
+        is_fixed[node] = 1;
+    }
+    if (model.prescribed.empty())
+        check_constraints(model.mesh, is_fixed);
+    else
+        check_held_dofs(model.mesh, is_fixed, model.prescribed);
     // crates/solidfem-elastic/src/study.rs:696-735: nodal forces, then tractions, in the order given
     std::vector<double> load(3 * nn, 0.0);
     for (const auto &f : model.nodal_forces) {

@@ -17,10 +17,10 @@ double now() { return std::chrono::duration<double>(std::chrono::steady_clock::n
 // Everything the matrix and the hierarchy depend on: the mesh, the constraint and the material.
 std::string system_key(const sf_workload &w) {
     char buf[640];
-    std::snprintf(buf, sizeof buf, "%a %a %a|%a %a %a|%d %d %d|%a %llu|%d %a|%d %a %a %a %a %a|%d", w.origin[0], w.origin[1],
+    std::snprintf(buf, sizeof buf, "%a %a %a|%a %a %a|%d %d %d|%a %llu|%d %a|%d %a %a %a %a %a|%d|%d %a", w.origin[0], w.origin[1],
                   w.origin[2], w.size[0], w.size[1], w.size[2], w.divisions[0], w.divisions[1], w.divisions[2], w.jitter_fraction,
                   (unsigned long long)w.jitter_seed, w.clamp_axis, w.clamp_value, w.material_kind, w.youngs, w.poisson,
-                  w.core_youngs, w.core_poisson, w.shell_depth, w.quadrature_points);
+                  w.core_youngs, w.core_poisson, w.shell_depth, w.quadrature_points, w.wall_cells, w.wall_thickness);
     return buf;
 }
 

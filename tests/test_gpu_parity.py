@@ -11,7 +11,7 @@ import pytest
 
 import refcpu
 import solidfem_amd as sf
-from workloads import FINGERPRINT_CANTILEVER, FINGERPRINT_PLATE, bar_config, block_config, loaded, plate_config
+from workloads import FINGERPRINT_CANTILEVER, FINGERPRINT_PLATE, bar_config, block_config, loaded, plate_config, tube_config
 
 pytestmark = pytest.mark.gpu
 
@@ -21,6 +21,7 @@ WORKLOADS = {
     "jittered-plate": plate_config(divisions=(12, 12, 2)),
     "random-block": block_config(3, divisions=(6, 4, 4)),
     "shell-core-torsion": bar_config(divisions=(16, 4, 4)),
+    "tube": tube_config(divisions=(20, 7, 8), wall_cells=2, poisson=0.3),
 }
 
 

@@ -52,6 +52,7 @@ def test_recovery_of_a_solved_study_is_bit_identical(ctx):
     points, nodal = study.recover()
     rp, rn = refcpu.Problem(w).recover(u)
     assert np.array_equal(bits(points), bits(rp)) and np.array_equal(bits(nodal), bits(rn))
-    # a cantilever under tip shear: the largest von Mises sits at the clamp
+    # a cantilever under tip shear: the bending moment peaks at the clamp, so the largest
+    # recovered von Mises lies in the first layer of elements (12 layers over the length)
     x = refcpu.Problem(w).nodes()
-    assert x[np.argmax(nodal[:, 6]), 0] == 0.0
+    assert x[np.argmax(nodal[:, 6]), 0] <= size[0] / div[0] + 1e-12

@@ -10,7 +10,7 @@ import pytest
 
 import refcpu
 import solidfem_amd as sf
-from workloads import FINGERPRINT_CANTILEVER, FINGERPRINT_PLATE, block_config, bar_config, loaded, plate_config
+from workloads import FINGERPRINT_CANTILEVER, FINGERPRINT_PLATE, block_config, bar_config, loaded, plate_config, tube_config
 
 WORKLOADS = {
     "cantilever": loaded(*FINGERPRINT_CANTILEVER[:2]),
@@ -18,6 +18,7 @@ WORKLOADS = {
     "jittered-plate": plate_config(divisions=(12, 12, 2)),
     "random-block": block_config(3, divisions=(6, 4, 4)),
     "shell-core-torsion": bar_config(divisions=(16, 4, 4)),
+    "tube": tube_config(divisions=(20, 7, 8), wall_cells=2, poisson=0.3),
 }
 
 
@@ -81,7 +82,7 @@ def assert_same_hierarchy(setup, mg):
 
 
 @pytest.mark.parametrize("name,coarse", [("cantilever", 100), ("plate", 100), ("jittered-plate", 150),
-                                         ("random-block", 60), ("shell-core-torsion", 90)])
+                                         ("random-block", 60), ("shell-core-torsion", 90), ("tube", 120)])
 def test_host_multigrid_setup_is_bit_identical_to_the_oracle(name, coarse):
     w = WORKLOADS[name]
     ref = refcpu.Problem(w)

@@ -42,3 +42,12 @@ def bar_config(divisions=(160, 16, 16)):
                     load_value=200.0, traction_kind=2, traction_scale=0.5, axis_u=10.0, axis_v=10.0,
                     material_kind=1, youngs=3500.0, poisson=0.36, core_youngs=700.0, core_poisson=0.3,
                     shell_depth=2.0)
+
+
+def tube_config(divisions=(50, 6, 6), wall_cells=2, poisson=0.0):
+    """A square tube (demo/wasm/src/part.rs:48-185): 100x10x10 mm with 2 mm walls, `wall_cells`
+    cells through each wall and the rest of the 6 across over the hollow, steel, clamped x = 0,
+    shear traction (0, 0, -1) MPa on the x = 100 end (64 mm^2 of wall: 64 N)."""
+    return Workload(size=(100.0, 10.0, 10.0), divisions=tuple(divisions), clamp_axis=0, clamp_value=0.0, load_axis=0,
+                    load_value=100.0, traction=(0.0, 0.0, -1.0), youngs=200000.0, poisson=poisson,
+                    wall_cells=wall_cells, wall_thickness=2.0)
