@@ -290,25 +290,29 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     break;
                 }
                 case Phase::Join: {
-                    // aggregate.rs:141-161: join the strongest seeded neighbour's aggregate
-                    bool done = run_rows(rows, budget, cancel, false, [&](size_t f, size_t l, bool) {
-                        for (size_t row = f; row < l; ++row) {
-                            if (aggregate_[row] != NONE) continue;
-                            const u32 *nb = &strong_columns_[(size_t)strong_starts_[row]];
-                            const u32 *ne = nb + (strong_starts_[row + 1] - strong_starts_[row]);
-                            double best = 0.0;
-                            u32 chosen = NONE;
-                            for (int64_t at = current_.starts[row]; at < current_.starts[row + 1]; ++at) {
-                                const u32 col = current_.columns[(size_t)at];
-                                if (!std::binary_search(nb, ne, col) || !seeded_[col]) continue;
-                                const double norm = frobenius(current_.block((size_t)at), rb * current_.cb);
-                                if (norm > best) {
-                                    best = norm;
-                                    chosen = aggregate_[col];
+                    // aggregate.rs:141-161: join the strongest seeded neighbour's aggregate. A row
+                    // reads only seeded rows' aggregates, which this pass never writes, and writes
+                    // only its own, so the rows run in parallel with the serial pass's result.
+                    bool done = run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
+                        parallel_for(l - f, 1024, [&](size_t a, size_t b) {
+                            for (size_t row = f + a; row < f + b; ++row) {
+                                if (aggregate_[row] != NONE) continue;
+                                const u32 *nb = &strong_columns_[(size_t)strong_starts_[row]];
+                                const u32 *ne = nb + (strong_starts_[row + 1] - strong_starts_[row]);
+                                double best = 0.0;
+                                u32 chosen = NONE;
+                                for (int64_t at = current_.starts[row]; at < current_.starts[row + 1]; ++at) {
+                                    const u32 col = current_.columns[(size_t)at];
+                                    if (!std::binary_search(nb, ne, col) || !seeded_[col]) continue;
+                                    const double norm = frobenius(current_.block((size_t)at), rb * current_.cb);
+                                    if (norm > best) {
+                                        best = norm;
+                                        chosen = aggregate_[col];
+                                    }
                                 }
+                                if (chosen != NONE) aggregate_[row] = chosen;
                             }
-                            if (chosen != NONE) aggregate_[row] = chosen;
-                        }
+                        });
                     });
                     if (!done) return false;
                     phase_ = Phase::Sweep;
