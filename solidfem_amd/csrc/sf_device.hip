@@ -956,6 +956,8 @@ DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, Device
     b_.upload(rhs.data(), n_, s);
     scalars_.resize(1);
     partials_.resize(DOT_BLOCKS);
+    arrivals_.resize(1);
+    SF_HIP(hipMemsetAsync(arrivals_.ptr, 0, sizeof(uint32_t), s));
     norm_.resize(1);
     SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocDefault));
     SF_HIP(hipEventCreateWithFlags(&copied_, hipEventDisableTiming));
@@ -1020,9 +1022,8 @@ void DevicePcg::dot(const double *a, const double *b, double *result, int final_
     if (dot_order_ == SF_DOT_REFERENCE) {
         hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, b, final_kind, result, scalars_.ptr);
     } else {
-        hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, b, partials_.ptr);
-        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
-                           final_kind, result, scalars_.ptr);
+        hipLaunchKernelGGL((dot_tree_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, b, partials_.ptr,
+                           arrivals_.ptr, final_kind, result, scalars_.ptr);
     }
     SF_HIP(hipGetLastError());
 }
@@ -1031,9 +1032,8 @@ void DevicePcg::sum(const double *a, double *result) {
     if (dot_order_ == SF_DOT_REFERENCE) {
         hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, a, (int)FINAL_STORE, result, scalars_.ptr);
     } else {
-        hipLaunchKernelGGL((dot_partials_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, a, partials_.ptr);
-        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
-                           (int)FINAL_STORE, result, scalars_.ptr);
+        hipLaunchKernelGGL((dot_tree_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, a, partials_.ptr,
+                           arrivals_.ptr, (int)FINAL_STORE, result, scalars_.ptr);
     }
     SF_HIP(hipGetLastError());
 }

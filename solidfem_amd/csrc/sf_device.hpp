@@ -218,6 +218,7 @@ class DevicePcg {
     DeviceBuffer<double> b_, x_, r_, z_, p_, ap_, tmp_, tmp2_, jacobi_;
     DeviceBuffer<dev::PcgScalars> scalars_;
     DeviceBuffer<double> partials_, norm_;
+    DeviceBuffer<uint32_t> arrivals_;  // the one-launch TREE reduction's arrival count, zero between launches
     dev::PcgScalars *pinned_ = nullptr;
     hipEvent_t copied_ = nullptr;
 };

@@ -585,7 +585,7 @@ struct PcgScalars {
 
 // Deterministic dot products. TREE order is a fixed two-stage reduction: DOT_BLOCKS blocks each
 // sum a fixed grid-stride pattern through a fixed shuffle tree into one partial, then one block
-// sums the partials in the same fixed pattern (a second kernel: no atomics and no fences).
+// sums the partials in the same fixed pattern (a second kernel, or the last block to finish).
 // REFERENCE order is one thread summing in index order from zero, exactly
 // crates/solidfem-solve/src/pcg.rs:285-291.
 constexpr int DOT_BLOCKS = 512;
@@ -640,6 +640,37 @@ static __global__ __launch_bounds__(DOT_THREADS) void dot_final_kernel(int parts
     for (int i = threadIdx.x; i < parts; i += blockDim.x) v = v + partials[i];
     const double total = block_sum(v);
     if (threadIdx.x == 0) dot_finish(kind, total, out, s);
+}
+
+// The same TREE reduction in one launch. Every block sums its grid-stride pattern as
+// dot_partials_kernel does and publishes the partial with an agent-scope atomic store, then counts
+// itself in; the block that arrives last reads the partials with agent-scope atomic loads and
+// finishes exactly as dot_final_kernel does (the same pattern, the same tree), then re-arms the
+// counter for the next launch. Same bits as the two launches.
+template <int MODE>
+__global__ __launch_bounds__(DOT_THREADS) void dot_tree_kernel(size_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                                                double *partials, uint32_t *arrivals, int kind, double *out,
+                                                                PcgScalars *s) {
+    double v = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v = v + (MODE == 0 ? a[i] * b[i] : a[i]);
+    const double part = block_sum(v);
+    __shared__ uint32_t last;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[blockIdx.x], part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t before = __hip_atomic_fetch_add(arrivals, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = before == gridDim.x - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!last) return;
+    double w = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
+        w = w + __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double total = block_sum(w);
+    if (threadIdx.x == 0) {
+        dot_finish(kind, total, out, s);
+        __hip_atomic_store(arrivals, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 template <int MODE>
