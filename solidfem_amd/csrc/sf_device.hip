@@ -247,30 +247,38 @@ void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s) {
 template void upload_bsr<double>(DeviceBsr<double> &, const HostBsr<double> &, hipStream_t);
 template void upload_bsr<float>(DeviceBsr<float> &, const HostBsr<double> &, hipStream_t);
 
+template <int RB, int CB, typename S, typename V, int RPW>
+static void long_rows_epi(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, Start<S, V> st) {
+    const dim3 grid((a.rows + RPW - 1) / RPW), block(WAVE);
+    if (epi == EPI_STORE) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_STORE, RPW>), grid, block, 0, s, a.view(), x, y, b, st);
+    else if (epi == EPI_RESIDUAL)
+        hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_RESIDUAL, RPW>), grid, block, 0, s, a.view(), x, y, b, st);
+    else hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_ADD, RPW>), grid, block, 0, s, a.view(), x, y, b, st);
+}
+
 template <int RB, int CB, typename S, typename V>
-static void long_rows_shape(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, Start<S, V> st) {
-    const dim3 grid(a.rows), block(WAVE);
-    if (epi == EPI_STORE) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_STORE>), grid, block, 0, s, a.view(), x, y, b, st);
-    else if (epi == EPI_RESIDUAL) hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_RESIDUAL>), grid, block, 0, s, a.view(), x, y, b, st);
-    else hipLaunchKernelGGL((long_rows_kernel<RB, CB, S, V, EPI_ADD>), grid, block, 0, s, a.view(), x, y, b, st);
+static void long_rows_shape(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, Start<S, V> st, int rpw) {
+    if (rpw == 4) long_rows_epi<RB, CB, S, V, 4>(a, x, y, b, epi, s, st);
+    else if (rpw == 2) long_rows_epi<RB, CB, S, V, 2>(a, x, y, b, epi, s, st);
+    else long_rows_epi<RB, CB, S, V, 1>(a, x, y, b, epi, s, st);
 }
 
 template <typename S, typename V>
-void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, Start<S, V> st) {
+void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s, Start<S, V> st, int rpw) {
     if (a.rows == 0) return;
-    if (a.rb == 6 && a.cb == 3) long_rows_shape<6, 3>(a, x, y, b, epi, s, st);
-    else if (a.rb == 6 && a.cb == 6) long_rows_shape<6, 6>(a, x, y, b, epi, s, st);
-    else if (a.rb == 3 && a.cb == 3) long_rows_shape<3, 3>(a, x, y, b, epi, s, st);
-    else if (a.rb == 3 && a.cb == 6) long_rows_shape<3, 6>(a, x, y, b, epi, s, st);
+    if (a.rb == 6 && a.cb == 3) long_rows_shape<6, 3>(a, x, y, b, epi, s, st, rpw);
+    else if (a.rb == 6 && a.cb == 6) long_rows_shape<6, 6>(a, x, y, b, epi, s, st, rpw);
+    else if (a.rb == 3 && a.cb == 3) long_rows_shape<3, 3>(a, x, y, b, epi, s, st, rpw);
+    else if (a.rb == 3 && a.cb == 6) long_rows_shape<3, 6>(a, x, y, b, epi, s, st, rpw);
     else throw Error(SF_ERR_INTERNAL, "unsupported block shape");
     SF_HIP(hipGetLastError());
 }
 template void launch_long_rows<double, double>(const DeviceBsr<double> &, const double *, double *, const double *, int, hipStream_t,
-                                               Start<double, double>);
+                                               Start<double, double>, int);
 template void launch_long_rows<float, double>(const DeviceBsr<float> &, const double *, double *, const double *, int, hipStream_t,
-                                              Start<float, double>);
+                                              Start<float, double>, int);
 template void launch_long_rows<float, float>(const DeviceBsr<float> &, const float *, float *, const float *, int, hipStream_t,
-                                             Start<float, float>);
+                                             Start<float, float>, int);
 
 int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates) {
     static const int forced = [] {
@@ -351,9 +359,9 @@ struct LevelDev {
     size_t rows = 0, n = 0;
     DeviceSliced<S> a, p, r;
     DeviceBsr<S> r_rows;          // the same transpose in plain block rows, for long_rows_kernel
-    bool restrict_by_rows = false;  // chosen by timing both restriction kernels
+    int restrict_rows = 0;  // 0: the sliced restriction; else long_rows_kernel with this many rows per wave (by timing)
     DeviceBsr<S> a_rows;          // coarse levels: A in plain block rows, for the row kernels
-    bool cheb_by_rows = false, residual_by_rows = false;  // chosen by timing
+    int cheb_rows = 0, residual_rows = 0;  // the same choice for the steps and residuals (by timing)
     DeviceBuffer<S> dinv;
     DeviceBuffer<double> lt;      // coarsest: transposed Cholesky factor
     DeviceBuffer<double> packed;  // coarsest: the factor packed by columns, for the LDS solve
@@ -726,15 +734,22 @@ class Hierarchy final : public DeviceMultigrid {
         smooth_batch(d, d.mres.ptr, d.mpx.ptr, x, M, s);
     }
 
-    static void launch_cheb_rows(LevelDev<S, V> &d, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out, V *x,
-                                 V *into, hipStream_t s, double *widen = nullptr) {
-        const dim3 grid(d.a_rows.rows), block(WAVE);
+    template <int RPW>
+    static void launch_cheb_rows_rpw(LevelDev<S, V> &d, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out, V *x,
+                                     V *into, hipStream_t s, double *widen) {
+        const dim3 grid((d.a_rows.rows + RPW - 1) / RPW), block(WAVE);
         if (d.rb == 3)
-            hipLaunchKernelGGL((cheb_rows_kernel<3, S, V>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out,
-                               r_out, x, into, widen);
+            hipLaunchKernelGGL((cheb_rows_kernel<3, S, V, RPW>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp,
+                               d_out, r_out, x, into, widen);
         else
-            hipLaunchKernelGGL((cheb_rows_kernel<6, S, V>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp, d_out,
-                               r_out, x, into, widen);
+            hipLaunchKernelGGL((cheb_rows_kernel<6, S, V, RPW>), grid, block, 0, s, d.a_rows.view(), d.dinv.ptr, d_in, r_in, keep, stp,
+                               d_out, r_out, x, into, widen);
+    }
+    static void launch_cheb_rows(LevelDev<S, V> &d, int rpw, const V *d_in, const V *r_in, V keep, V stp, V *d_out, V *r_out, V *x,
+                                 V *into, hipStream_t s, double *widen = nullptr) {
+        if (rpw == 4) launch_cheb_rows_rpw<4>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen);
+        else if (rpw == 2) launch_cheb_rows_rpw<2>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen);
+        else launch_cheb_rows_rpw<1>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen);
     }
 
     // Picks each level's batch depths by timing its own kernels on its own matrices.
@@ -748,26 +763,27 @@ class Hierarchy final : public DeviceMultigrid {
             });
             d.a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, batch); });
             if (d.a_rows.rows) {
-                // each kernel against its row-per-wavefront form (the same bits)
-                d.cheb_by_rows = fastest(s, [&](int sliced) {
-                    if (sliced) launch_cheb_step(d, d.a.batch_cheb, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
-                    else launch_cheb_rows(d, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
-                }, {1, 0}) == 0;
-                d.residual_by_rows = fastest(s, [&](int sliced) {
-                    if (sliced) launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s);
-                    else launch_long_rows<S, V>(d.a_rows, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s);
-                }, {1, 0}) == 0;
+                // each kernel against its row-per-wavefront forms with 1, 2 or 4 rows per wavefront
+                // (all the same bits); 0 is the sliced kernel
+                d.cheb_rows = fastest(s, [&](int rpw) {
+                    if (!rpw) launch_cheb_step(d, d.a.batch_cheb, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
+                    else launch_cheb_rows(d, rpw, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
+                }, {0, 1, 2, 4});
+                d.residual_rows = fastest(s, [&](int rpw) {
+                    if (!rpw) launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s);
+                    else launch_long_rows<S, V>(d.a_rows, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, {nullptr, (V)0, nullptr, nullptr}, rpw);
+                }, {0, 1, 2, 4});
             }
             if (d.has_p) {
                 LevelDev<S, V> &c = lv_[l + 1];
                 SF_HIP(hipMemsetAsync(c.b.ptr, 0, c.n * sizeof(V), s));
                 SF_HIP(hipMemsetAsync(c.x.ptr, 0, c.n * sizeof(V), s));
                 d.r.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, batch); });
-                // the sliced product at its best depth (1) against one wavefront per row (0)
-                d.restrict_by_rows = fastest(s, [&](int sliced) {
-                    if (sliced) launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
-                    else launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
-                }, {1, 0}) == 0;
+                // the sliced product at its best depth (0) against 1, 2 or 4 rows per wavefront
+                d.restrict_rows = fastest(s, [&](int rpw) {
+                    if (!rpw) launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
+                    else launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, {nullptr, (V)0, nullptr, nullptr}, rpw);
+                }, {0, 1, 2, 4});
                 d.p.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.p, c.x.ptr, d.px.ptr, nullptr, EPI_ADD, s, batch); });
             }
         }
@@ -802,7 +818,7 @@ class Hierarchy final : public DeviceMultigrid {
                                                                        (double)d.rows * d.rb * d.rb * sizeof(S) + 6 * vec +
                                                                        (into ? 2 * vec : 0.0) + (wide ? (double)d.n * 8 : 0.0))
                                      : -1;
-            if (d.cheb_by_rows) launch_cheb_rows(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, wide);
+            if (d.cheb_rows) launch_cheb_rows(d, d.cheb_rows, d_in, r_in, keep, stp, d_out, r_out, x, into, s, wide);
             else launch_cheb_step(d, d.a.batch_cheb, d_in, r_in, keep, stp, d_out, r_out, x, into, s, wide);
             Profiler::get().end(0, token, s);
         }
@@ -816,7 +832,7 @@ class Hierarchy final : public DeviceMultigrid {
     // res = b - A x (cycle.rs:548-564) with the level's faster kernel, optionally with the start of
     // the smoothing that follows
     static void residual(LevelDev<S, V> &d, const V *x, const V *b, hipStream_t s, Start<S, V> st = {nullptr, (V)0, nullptr, nullptr}) {
-        if (d.residual_by_rows) launch_long_rows<S, V>(d.a_rows, x, d.res.ptr, b, EPI_RESIDUAL, s, st);
+        if (d.residual_rows) launch_long_rows<S, V>(d.a_rows, x, d.res.ptr, b, EPI_RESIDUAL, s, st, d.residual_rows);
         else launch_spmv<S, V>(d.a, x, d.res.ptr, b, EPI_RESIDUAL, s, 0, st);
     }
     // Whether a level's cycle begins with a smoothing (and so with a start that can be fused).
@@ -850,7 +866,7 @@ class Hierarchy final : public DeviceMultigrid {
         // restriction, with the coarser level's first start when its cycle smooths first
         const bool fuse = smooths_first(c);
         const Start<S, V> st = fuse ? start_of(c, c.x.ptr) : Start<S, V>{nullptr, (V)0, nullptr, nullptr};
-        if (d.restrict_by_rows) launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, st);
+        if (d.restrict_rows) launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, st, d.restrict_rows);
         else launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, 0, st);
         cycle(l + 1, c.b.ptr, c.x.ptr, s, fuse);
         launch_spmv<S, V>(d.p, c.x.ptr, x, nullptr, EPI_ADD, s);
@@ -956,8 +972,6 @@ DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, Device
     b_.upload(rhs.data(), n_, s);
     scalars_.resize(1);
     partials_.resize(DOT_BLOCKS);
-    arrivals_.resize(1);
-    SF_HIP(hipMemsetAsync(arrivals_.ptr, 0, sizeof(uint32_t), s));
     norm_.resize(1);
     SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocDefault));
     SF_HIP(hipEventCreateWithFlags(&copied_, hipEventDisableTiming));
@@ -1022,8 +1036,9 @@ void DevicePcg::dot(const double *a, const double *b, double *result, int final_
     if (dot_order_ == SF_DOT_REFERENCE) {
         hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, b, final_kind, result, scalars_.ptr);
     } else {
-        hipLaunchKernelGGL((dot_tree_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, b, partials_.ptr,
-                           arrivals_.ptr, final_kind, result, scalars_.ptr);
+        hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, b, partials_.ptr);
+        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
+                           final_kind, result, scalars_.ptr);
     }
     SF_HIP(hipGetLastError());
 }
@@ -1032,8 +1047,9 @@ void DevicePcg::sum(const double *a, double *result) {
     if (dot_order_ == SF_DOT_REFERENCE) {
         hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, a, (int)FINAL_STORE, result, scalars_.ptr);
     } else {
-        hipLaunchKernelGGL((dot_tree_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, a, partials_.ptr,
-                           arrivals_.ptr, (int)FINAL_STORE, result, scalars_.ptr);
+        hipLaunchKernelGGL((dot_partials_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, a, partials_.ptr);
+        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
+                           (int)FINAL_STORE, result, scalars_.ptr);
     }
     SF_HIP(hipGetLastError());
 }

@@ -59,10 +59,11 @@ struct DeviceBsr {
 };
 template <typename T>
 void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s);
-// y = A x (epilogue per dev::Epilogue) with one wavefront per block row (long_rows_kernel).
+// y = A x (epilogue per dev::Epilogue) with `rows_per_wave` block rows per wavefront
+// (long_rows_kernel: 1, 2 or 4).
 template <typename S, typename V>
 void launch_long_rows(const DeviceBsr<S> &a, const V *x, V *y, const V *b, int epi, hipStream_t s,
-                      dev::Start<S, V> start = {nullptr, (V)0, nullptr, nullptr});
+                      dev::Start<S, V> start = {nullptr, (V)0, nullptr, nullptr}, int rows_per_wave = 1);
 
 // Builds a sliced matrix from row-major f64 blocks already on the device (`src_values`).
 template <typename T>
@@ -218,7 +219,6 @@ class DevicePcg {
     DeviceBuffer<double> b_, x_, r_, z_, p_, ap_, tmp_, tmp2_, jacobi_;
     DeviceBuffer<dev::PcgScalars> scalars_;
     DeviceBuffer<double> partials_, norm_;
-    DeviceBuffer<uint32_t> arrivals_;  // the one-launch TREE reduction's arrival count, zero between launches
     dev::PcgScalars *pinned_ = nullptr;
     hipEvent_t copied_ = nullptr;
 };

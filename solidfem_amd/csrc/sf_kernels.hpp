@@ -193,23 +193,53 @@ struct Bsr {
     uint32_t rows;
 };
 
+// The longest row among a wavefront's RPW rows (each row's length is uniform over its lanes).
+template <int RPW>
+__device__ __forceinline__ uint32_t longest_of_wave(uint32_t len) {
+#pragma unroll
+    for (int o = WAVE / RPW; o < WAVE; o <<= 1) {
+        const uint32_t other = (uint32_t)__shfl_xor((int)len, o, WAVE);
+        len = other > len ? other : len;
+    }
+    return len;
+}
+
+// Adds `n` terms in order onto `sum`, eight loads ahead.
+template <typename V>
+__device__ __forceinline__ V add_terms(V sum, const V *t, uint32_t n) {
+    uint32_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        V v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = t[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum = sum + v[u];
+    }
+    for (; i < n; ++i) sum = sum + t[i];
+    return sum;
+}
+
 // y = A x with an epilogue (as spmv_kernel) for matrices with few, long rows -- the stored
 // transposes of the prolongators, whose rows gather every fine row of an aggregate's
-// neighbourhood. One wavefront (one workgroup) per block row: its 64 lanes load 64 blocks at
-// once and form their terms A[p][q] x[q] into LDS; then lane p adds its row's terms in stored
-// block order and column order, from zero -- the summation of spmv_kernel and of the reference,
-// so the same bits, with one memory round trip per 64 blocks instead of one per few blocks.
-template <int RB, int CB, typename S, typename V, int EPI>
+// neighbourhood, and the coarse levels. One wavefront (one workgroup) takes RPW block rows,
+// 64 / RPW lanes each: the lanes load that many blocks at once and form their terms A[p][q] x[q]
+// into LDS; then lane p of the row's group adds the row's terms in stored block order and column
+// order, from zero -- the summation of spmv_kernel and of the reference, so the same bits, with
+// one memory round trip per 64 / RPW blocks instead of one per few blocks.
+template <int RB, int CB, typename S, typename V, int EPI, int RPW>
 __global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__restrict__ x, V *__restrict__ y,
                                                        const V *__restrict__ b, Start<S, V> then) {
-    __shared__ V terms[RB][WAVE * CB];
-    __shared__ V values[RB];
-    const uint32_t row = blockIdx.x;
-    const int lane = threadIdx.x;
-    const uint32_t start = a.starts[row], len = a.starts[row + 1] - start;
+    constexpr int LW = WAVE / RPW;
+    __shared__ V terms[RPW][RB][LW * CB];
+    __shared__ V values[RPW][RB];
+    const int lane = threadIdx.x, g = lane / LW, sl = lane % LW;
+    const uint32_t row = blockIdx.x * RPW + (uint32_t)g;
+    const bool live = row < a.rows;
+    const uint32_t start = live ? a.starts[row] : 0u, len = live ? a.starts[row + 1] - start : 0u;
+    const uint32_t longest = longest_of_wave<RPW>(len);
     V sum = (V)0;
-    for (uint32_t k0 = 0; k0 < len; k0 += WAVE) {
-        const uint32_t k = k0 + lane;
+    for (uint32_t k0 = 0; k0 < longest; k0 += LW) {
+        const uint32_t k = k0 + sl;
         if (k < len) {
             const uint32_t col = a.cols[start + k];
             const S *blk = a.vals + (size_t)(start + k) * RB * CB;
@@ -219,40 +249,29 @@ __global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__rest
 #pragma unroll
             for (int p = 0; p < RB; ++p)
 #pragma unroll
-                for (int q = 0; q < CB; ++q) terms[p][lane * CB + q] = widen<V>(blk[p * CB + q]) * xv[q];
+                for (int q = 0; q < CB; ++q) terms[g][p][sl * CB + q] = widen<V>(blk[p * CB + q]) * xv[q];
         }
         __syncthreads();
-        if (lane < RB) {
-            const uint32_t n = (len - k0 < (uint32_t)WAVE ? len - k0 : (uint32_t)WAVE) * CB;
-            const V *t = terms[lane];
-            uint32_t i = 0;
-            for (; i + 8 <= n; i += 8) {
-                V v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = t[i + u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) sum = sum + v[u];
-            }
-            for (; i < n; ++i) sum = sum + t[i];
-        }
+        if (sl < RB && k0 < len) sum = add_terms(sum, terms[g][sl], (len - k0 < (uint32_t)LW ? len - k0 : (uint32_t)LW) * CB);
         __syncthreads();
     }
-    const size_t out = (size_t)row * RB + (lane < RB ? lane : 0);
-    if (lane < RB) {
+    const bool owner = live && sl < RB;
+    const size_t out = (size_t)row * RB + (owner ? sl : 0);
+    if (owner) {
         V value;
         if (EPI == EPI_STORE) value = sum;
         else if (EPI == EPI_RESIDUAL) value = b[out] - sum;
         else value = y[out] + sum;
         y[out] = value;
-        values[lane] = value;
+        values[g][sl] = value;
     }
     if (then.dinv) {  // uniform
         __syncthreads();
-        if (lane < RB) {
-            const S *drow = then.dinv + (size_t)row * RB * RB + (size_t)lane * RB;
+        if (owner) {
+            const S *drow = then.dinv + (size_t)row * RB * RB + (size_t)sl * RB;
             V z = (V)0;
 #pragma unroll
-            for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * values[q];
+            for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * values[g][q];
             const V dv = z / then.theta;
             then.d[out] = dv;
             then.x[out] = dv;
@@ -260,24 +279,27 @@ __global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__rest
     }
 }
 
-// The fused Chebyshev step of cheb_step_kernel for levels with few rows, one wavefront (one
-// workgroup) per block row as long_rows_kernel: the lanes form the terms of A d_in for 64 blocks
-// at a time, lane p adds its row's terms in stored order, and the RB residuals of the block row
-// meet in LDS for the block-Jacobi product (summed over q in order from zero). The same bits as
+// The fused Chebyshev step of cheb_step_kernel for levels with few rows, RPW block rows per
+// wavefront (one workgroup) as long_rows_kernel: the lanes form the terms of A d_in, lane p of a
+// row's group adds the row's terms in stored order, and the RB residuals of the block row meet in
+// LDS for the block-Jacobi product (summed over q in order from zero). The same bits as
 // cheb_step_kernel.
-template <int RB, typename S, typename V>
+template <int RB, typename S, typename V, int RPW>
 __global__ __launch_bounds__(64) void cheb_rows_kernel(Bsr<S> a, const S *__restrict__ dinv, const V *__restrict__ d_in,
                                                        const V *__restrict__ r_in, V keep, V step, V *__restrict__ d_out,
                                                        V *__restrict__ r_out, V *__restrict__ x, V *__restrict__ sum_into,
                                                        double *__restrict__ widen_into) {
-    __shared__ V terms[RB][WAVE * RB];
-    __shared__ V r_new[RB];
-    const uint32_t row = blockIdx.x;
-    const int lane = threadIdx.x;
-    const uint32_t start = a.starts[row], len = a.starts[row + 1] - start;
+    constexpr int LW = WAVE / RPW;
+    __shared__ V terms[RPW][RB][LW * RB];
+    __shared__ V r_new[RPW][RB];
+    const int lane = threadIdx.x, g = lane / LW, sl = lane % LW;
+    const uint32_t row = blockIdx.x * RPW + (uint32_t)g;
+    const bool live = row < a.rows;
+    const uint32_t start = live ? a.starts[row] : 0u, len = live ? a.starts[row + 1] - start : 0u;
+    const uint32_t longest = longest_of_wave<RPW>(len);
     V sum = (V)0;
-    for (uint32_t k0 = 0; k0 < len; k0 += WAVE) {
-        const uint32_t k = k0 + lane;
+    for (uint32_t k0 = 0; k0 < longest; k0 += LW) {
+        const uint32_t k = k0 + sl;
         if (k < len) {
             const uint32_t col = a.cols[start + k];
             const S *blk = a.vals + (size_t)(start + k) * RB * RB;
@@ -287,36 +309,25 @@ __global__ __launch_bounds__(64) void cheb_rows_kernel(Bsr<S> a, const S *__rest
 #pragma unroll
             for (int p = 0; p < RB; ++p)
 #pragma unroll
-                for (int q = 0; q < RB; ++q) terms[p][lane * RB + q] = widen<V>(blk[p * RB + q]) * xv[q];
+                for (int q = 0; q < RB; ++q) terms[g][p][sl * RB + q] = widen<V>(blk[p * RB + q]) * xv[q];
         }
         __syncthreads();
-        if (lane < RB) {
-            const uint32_t n = (len - k0 < (uint32_t)WAVE ? len - k0 : (uint32_t)WAVE) * RB;
-            const V *t = terms[lane];
-            uint32_t i = 0;
-            for (; i + 8 <= n; i += 8) {
-                V v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = t[i + u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) sum = sum + v[u];
-            }
-            for (; i < n; ++i) sum = sum + t[i];
-        }
+        if (sl < RB && k0 < len) sum = add_terms(sum, terms[g][sl], (len - k0 < (uint32_t)LW ? len - k0 : (uint32_t)LW) * RB);
         __syncthreads();
     }
-    const size_t i = (size_t)row * RB + (lane < RB ? lane : 0);
-    if (lane < RB) r_new[lane] = r_in[i] - sum;
+    const bool owner = live && sl < RB;
+    const size_t i = (size_t)row * RB + (owner ? sl : 0);
+    if (owner) r_new[g][sl] = r_in[i] - sum;
     __syncthreads();
-    if (lane < RB) {
-        const S *drow = dinv + (size_t)row * RB * RB + (size_t)lane * RB;
+    if (owner) {
+        const S *drow = dinv + (size_t)row * RB * RB + (size_t)sl * RB;
         V z = (V)0;
 #pragma unroll
-        for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * r_new[q];
+        for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * r_new[g][q];
         const V dn = keep * d_in[i] + step * z;
         const V xn = x[i] + dn;
         d_out[i] = dn;
-        r_out[i] = r_new[lane];
+        r_out[i] = r_new[g][sl];
         x[i] = xn;
         if (sum_into) {
             const V total = sum_into[i] + xn;
@@ -585,7 +596,9 @@ struct PcgScalars {
 
 // Deterministic dot products. TREE order is a fixed two-stage reduction: DOT_BLOCKS blocks each
 // sum a fixed grid-stride pattern through a fixed shuffle tree into one partial, then one block
-// sums the partials in the same fixed pattern (a second kernel, or the last block to finish).
+// sums the partials in the same fixed pattern (a second kernel: no atomics and no fences; a
+// one-launch form whose last block finishes the sum measured slower, its per-block release fences
+// costing more than the launch they save).
 // REFERENCE order is one thread summing in index order from zero, exactly
 // crates/solidfem-solve/src/pcg.rs:285-291.
 constexpr int DOT_BLOCKS = 512;
@@ -640,37 +653,6 @@ static __global__ __launch_bounds__(DOT_THREADS) void dot_final_kernel(int parts
     for (int i = threadIdx.x; i < parts; i += blockDim.x) v = v + partials[i];
     const double total = block_sum(v);
     if (threadIdx.x == 0) dot_finish(kind, total, out, s);
-}
-
-// The same TREE reduction in one launch. Every block sums its grid-stride pattern as
-// dot_partials_kernel does and publishes the partial with an agent-scope atomic store, then counts
-// itself in; the block that arrives last reads the partials with agent-scope atomic loads and
-// finishes exactly as dot_final_kernel does (the same pattern, the same tree), then re-arms the
-// counter for the next launch. Same bits as the two launches.
-template <int MODE>
-__global__ __launch_bounds__(DOT_THREADS) void dot_tree_kernel(size_t n, const double *__restrict__ a, const double *__restrict__ b,
-                                                                double *partials, uint32_t *arrivals, int kind, double *out,
-                                                                PcgScalars *s) {
-    double v = 0.0;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v = v + (MODE == 0 ? a[i] * b[i] : a[i]);
-    const double part = block_sum(v);
-    __shared__ uint32_t last;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[blockIdx.x], part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t before = __hip_atomic_fetch_add(arrivals, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = before == gridDim.x - 1 ? 1u : 0u;
-    }
-    __syncthreads();
-    if (!last) return;
-    double w = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
-        w = w + __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double total = block_sum(w);
-    if (threadIdx.x == 0) {
-        dot_finish(kind, total, out, s);
-        __hip_atomic_store(arrivals, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 template <int MODE>
