@@ -280,17 +280,17 @@ template void launch_long_rows<float, double>(const DeviceBsr<float> &, const do
 template void launch_long_rows<float, float>(const DeviceBsr<float> &, const float *, float *, const float *, int, hipStream_t,
                                              Start<float, float>, int);
 
-int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates) {
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates, const std::string &what) {
     static const int forced = [] {
         const char *e = std::getenv("SOLIDFEM_BATCH");
         return e ? std::atoi(e) : 0;
     }();
     for (int c : candidates)
         if (forced > 0 && forced == c) return forced;
-    return fastest(s, launch, candidates);
+    return fastest(s, launch, candidates, what);
 }
 
-int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates) {
+int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates, const std::string &what) {
     hipEvent_t e0, e1;
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
@@ -305,7 +305,7 @@ int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initiali
         SF_HIP(hipEventSynchronize(e1));
         float ms = 0;
         SF_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (verbose) std::fprintf(stderr, "[tune] variant %d: %.2f us\n", batch, ms * 1e3f / 3);
+        if (verbose) std::fprintf(stderr, "[tune] %s: variant %d: %.2f us\n", what.c_str(), batch, ms * 1e3f / 3);
         if (ms < best_ms) {
             best_ms = ms;
             best = batch;
@@ -313,6 +313,7 @@ int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initiali
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (verbose) std::fprintf(stderr, "[tune] %s: chose %d\n", what.c_str(), best);
     return best;
 }
 
@@ -362,6 +363,8 @@ struct LevelDev {
     int restrict_rows = 0;  // 0: the sliced restriction; else long_rows_kernel with this many rows per wave (by timing)
     DeviceBsr<S> a_rows;          // coarse levels: A in plain block rows, for the row kernels
     int cheb_rows = 0, residual_rows = 0;  // the same choice for the steps and residuals (by timing)
+    bool fuse_restrict_start = true;       // the coarser level's first start in the restriction (by timing)
+    bool fuse_post_start = true;           // the post-smoothing's start in the second residual (by timing)
     DeviceBuffer<S> dinv;
     DeviceBuffer<double> lt;      // coarsest: transposed Cholesky factor
     DeviceBuffer<double> packed;  // coarsest: the factor packed by columns, for the LDS solve
@@ -752,42 +755,76 @@ class Hierarchy final : public DeviceMultigrid {
         else launch_cheb_rows_rpw<1>(d, d_in, r_in, keep, stp, d_out, r_out, x, into, s, widen);
     }
 
-    // Picks each level's batch depths by timing its own kernels on its own matrices.
+    // Picks each level's batch depths, kernel forms and start fusions by timing its own kernels
+    // on its own matrices. Every choice gives the same bits.
     void tune(hipStream_t s) {
         for (size_t l = 0; l < lv_.size(); ++l) {
             LevelDev<S, V> &d = lv_[l];
+            const std::string at = "L" + std::to_string(l) + (sizeof(V) == 4 ? " f32 " : " f64 ");
             V *vs[8] = {d.b.ptr, d.x.ptr, d.d0.ptr, d.d1.ptr, d.r0.ptr, d.r1.ptr, d.res.ptr, d.px.ptr};
             for (V *v : vs) SF_HIP(hipMemsetAsync(v, 0, d.n * sizeof(V), s));
             d.a.batch_cheb = tune_batch(s, [&](int batch) {
                 launch_cheb_step(d, batch, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
-            });
-            d.a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, batch); });
+            }, {1, 2, 4, 8}, at + "step depth");
+            d.a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, batch); },
+                                        {1, 2, 4, 8}, at + "residual depth");
             if (d.a_rows.rows) {
                 // each kernel against its row-per-wavefront forms with 1, 2 or 4 rows per wavefront
                 // (all the same bits); 0 is the sliced kernel
                 d.cheb_rows = fastest(s, [&](int rpw) {
                     if (!rpw) launch_cheb_step(d, d.a.batch_cheb, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
                     else launch_cheb_rows(d, rpw, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
-                }, {0, 1, 2, 4});
+                }, {0, 1, 2, 4}, at + "step rows per wave");
                 d.residual_rows = fastest(s, [&](int rpw) {
                     if (!rpw) launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s);
                     else launch_long_rows<S, V>(d.a_rows, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, {nullptr, (V)0, nullptr, nullptr}, rpw);
-                }, {0, 1, 2, 4});
+                }, {0, 1, 2, 4}, at + "residual rows per wave");
             }
             if (d.has_p) {
                 LevelDev<S, V> &c = lv_[l + 1];
                 SF_HIP(hipMemsetAsync(c.b.ptr, 0, c.n * sizeof(V), s));
                 SF_HIP(hipMemsetAsync(c.x.ptr, 0, c.n * sizeof(V), s));
-                d.r.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, batch); });
+                d.r.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, batch); },
+                                            {1, 2, 4, 8}, at + "restriction depth");
                 // the sliced product at its best depth (0) against 1, 2 or 4 rows per wavefront
                 d.restrict_rows = fastest(s, [&](int rpw) {
                     if (!rpw) launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
                     else launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, {nullptr, (V)0, nullptr, nullptr}, rpw);
-                }, {0, 1, 2, 4});
-                d.p.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.p, c.x.ptr, d.px.ptr, nullptr, EPI_ADD, s, batch); });
+                }, {0, 1, 2, 4}, at + "restriction rows per wave");
+                d.p.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.p, c.x.ptr, d.px.ptr, nullptr, EPI_ADD, s, batch); },
+                                            {1, 2, 4, 8}, at + "prolongation depth");
+                // each smoothing's start fused into the kernel before it, or launched on its own
+                if (smooths_first(c))
+                    d.fuse_restrict_start = fastest(s, [&](int fuse) {
+                        restrict_to(d, c, s, fuse != 0);
+                        if (!fuse) launch_start(c, c.b.ptr, c.x.ptr, s);
+                    }, {1, 0}, at + "restriction with the coarse start") == 1;
+                d.fuse_post_start = fastest(s, [&](int fuse) {
+                    if (fuse) residual(d, d.x.ptr, d.b.ptr, s, start_of(d, d.px.ptr));
+                    else {
+                        residual(d, d.x.ptr, d.b.ptr, s);
+                        launch_start(d, d.res.ptr, d.px.ptr, s);
+                    }
+                }, {1, 0}, at + "residual with the post-smoothing start") == 1;
             }
         }
         SF_HIP(hipStreamSynchronize(s));
+    }
+
+    // The Chebyshev start z = D^-1 b / theta, d = z, x = z (cycle.rs:578-590).
+    static void launch_start(LevelDev<S, V> &d, const V *b, V *x, hipStream_t s) {
+        const unsigned g = grid_for(d.n, 256);
+        if (d.rb == 3)
+            hipLaunchKernelGGL((cheb_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
+        else
+            hipLaunchKernelGGL((cheb_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
+    }
+
+    // c.b = P^T d.res, with the coarser level's first start in the epilogue when `fuse`.
+    static void restrict_to(LevelDev<S, V> &d, LevelDev<S, V> &c, hipStream_t s, bool fuse) {
+        const Start<S, V> st = fuse ? start_of(c, c.x.ptr) : Start<S, V>{nullptr, (V)0, nullptr, nullptr};
+        if (d.restrict_rows) launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, st, d.restrict_rows);
+        else launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, 0, st);
     }
 
     // cycle.rs:570-651: start, then degree - 1 fused steps; `sum_into` takes the final add.
@@ -795,13 +832,7 @@ class Hierarchy final : public DeviceMultigrid {
     // step also writes the sum into f64 (the V-cycle's widened output).
     void smooth(LevelDev<S, V> &d, const V *b, V *x, V *sum_into, hipStream_t s, bool finest = false, bool started = false,
                 double *widen = nullptr) {
-        const unsigned g = grid_for(d.n, 256);
-        if (!started) {
-            if (d.rb == 3)
-                hipLaunchKernelGGL((cheb_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
-            else
-                hipLaunchKernelGGL((cheb_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, d.dinv.ptr, b, (V)d.theta, d.d0.ptr, x);
-        }
+        if (!started) launch_start(d, b, x, s);
         V *dd[2] = {d.d0.ptr, d.d1.ptr};
         V *rr[2] = {d.r0.ptr, d.r1.ptr};
         const size_t steps = d.steps.size();
@@ -863,18 +894,20 @@ class Hierarchy final : public DeviceMultigrid {
         int token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
         residual(d, x, b, s);
         Profiler::get().end(2, token, s);
-        // restriction, with the coarser level's first start when its cycle smooths first
-        const bool fuse = smooths_first(c);
-        const Start<S, V> st = fuse ? start_of(c, c.x.ptr) : Start<S, V>{nullptr, (V)0, nullptr, nullptr};
-        if (d.restrict_rows) launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, st, d.restrict_rows);
-        else launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, 0, st);
+        // restriction, with the coarser level's first start when its cycle smooths first (and the
+        // fused form measured faster)
+        const bool fuse = smooths_first(c) && d.fuse_restrict_start;
+        restrict_to(d, c, s, fuse);
         cycle(l + 1, c.b.ptr, c.x.ptr, s, fuse);
         launch_spmv<S, V>(d.p, c.x.ptr, x, nullptr, EPI_ADD, s);
-        // the residual for the post-smoothing, with its start
-        token = finest ? Profiler::get().begin(2, s, res_bytes + (double)d.rows * d.rb * d.rb * sizeof(S) + 2.0 * d.n * sizeof(V)) : -1;
-        residual(d, x, b, s, start_of(d, d.px.ptr));
+        // the residual for the post-smoothing, with its start when fused
+        const bool fuse_post = d.fuse_post_start;
+        token = finest ? Profiler::get().begin(2, s, res_bytes + (fuse_post ? (double)d.rows * d.rb * d.rb * sizeof(S) + 2.0 * d.n * sizeof(V) : 0.0))
+                       : -1;
+        if (fuse_post) residual(d, x, b, s, start_of(d, d.px.ptr));
+        else residual(d, x, b, s);
         Profiler::get().end(2, token, s);
-        smooth(d, d.res.ptr, d.px.ptr, x, s, finest, true, widen);
+        smooth(d, d.res.ptr, d.px.ptr, x, s, finest, fuse_post, widen);
     }
 
     void compute_stats() {

@@ -79,9 +79,11 @@ void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi
 
 // The batch depth (blocks in flight per lane, one of `candidates`) whose launches
 // `launch(batch)` run fastest; every depth gives the same bits. SOLIDFEM_BATCH forces one.
-int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates = {1, 2, 4, 8});
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates = {1, 2, 4, 8},
+               const std::string &what = "");
 // The candidate whose launches `launch(candidate)` run fastest (timed with HIP events on `s`).
-int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates);
+int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates,
+            const std::string &what = "");
 
 // ---- several right-hand sides ------------------------------------------------------------------
 

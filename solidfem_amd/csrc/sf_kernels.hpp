@@ -562,14 +562,21 @@ __global__ __launch_bounds__(256) void coarse_inverse_kernel(uint32_t n, const d
         __syncthreads();
         const uint32_t kmax = n - k0 < 256u ? n - k0 : 256u;
         if (i < n) {
-            // sixteen loads in flight, then their terms in ascending k
+            // 32 loads in flight (the column is latency-bound), then their terms in ascending k
             uint32_t kk = 0;
-            for (; kk + 16 <= kmax; kk += 16) {
-                double a[16];
+            for (; kk + 32 <= kmax; kk += 32) {
+                double a[32];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) a[u] = inv[(size_t)(k0 + kk + u) * n + i];
+                for (int u = 0; u < 32; ++u) a[u] = inv[(size_t)(k0 + kk + u) * n + i];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) sum = sum + a[u] * bs[kk + u];
+                for (int u = 0; u < 32; ++u) sum = sum + a[u] * bs[kk + u];
+            }
+            for (; kk + 8 <= kmax; kk += 8) {
+                double a[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] = inv[(size_t)(k0 + kk + u) * n + i];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) sum = sum + a[u] * bs[kk + u];
             }
             for (; kk < kmax; ++kk) sum = sum + inv[(size_t)(k0 + kk) * n + i] * bs[kk];
         }
