@@ -484,6 +484,7 @@ class Hierarchy final : public DeviceMultigrid {
         out_.resize(in_.count);
         SF_HIP(hipStreamSynchronize(s));
         tune(s);
+        tune_fusions(s);
         compute_stats();
     }
 
@@ -533,15 +534,20 @@ class Hierarchy final : public DeviceMultigrid {
             LevelDev<S, V> &d = lv_[0];
             const size_t n = d.n;
             if (d.has_p) {
-                // the rounding in, fused with the first start; the widening out, fused with the last step
-                const unsigned g = grid_for(d.rows, 256);
-                if (d.rb == 3)
-                    hipLaunchKernelGGL((convert_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr, d.dinv.ptr,
-                                       (V)d.theta, d.d0.ptr, out_.ptr);
-                else
-                    hipLaunchKernelGGL((convert_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr, d.dinv.ptr,
-                                       (V)d.theta, d.d0.ptr, out_.ptr);
-                cycle(0, in_.ptr, out_.ptr, s, true, z);
+                // the rounding in, fused with the first start (when faster); the widening out, fused
+                // with the last step
+                if (fuse_convert_start_) {
+                    const unsigned g = grid_for(d.rows, 256);
+                    if (d.rb == 3)
+                        hipLaunchKernelGGL((convert_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr,
+                                           d.dinv.ptr, (V)d.theta, d.d0.ptr, out_.ptr);
+                    else
+                        hipLaunchKernelGGL((convert_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr,
+                                           d.dinv.ptr, (V)d.theta, d.d0.ptr, out_.ptr);
+                } else {
+                    hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, r, in_.ptr);
+                }
+                cycle(0, in_.ptr, out_.ptr, s, fuse_convert_start_, z);
             } else {
                 hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, r, in_.ptr);
                 cycle(0, in_.ptr, out_.ptr, s);
@@ -811,6 +817,51 @@ class Hierarchy final : public DeviceMultigrid {
         SF_HIP(hipStreamSynchronize(s));
     }
 
+    // The start fusions change which kernels run back to back, and what a kernel costs depends on
+    // what the caches hold when it runs, which isolated repeated launches do not reproduce. So each
+    // fusion is kept, level by level, only if whole preconditioner applications run faster with
+    // it. Every choice gives the same bits.
+    void tune_fusions(hipStream_t s) {
+        if (lv_.empty()) return;
+        static const bool verbose = std::getenv("SOLIDFEM_TUNE_LOG") != nullptr;
+        const size_t n = lv_[0].n;
+        DeviceBuffer<double> r(n), z(n);
+        SF_HIP(hipMemsetAsync(r.ptr, 0, n * sizeof(double), s));
+        hipEvent_t e0, e1;
+        SF_HIP(hipEventCreate(&e0));
+        SF_HIP(hipEventCreate(&e1));
+        auto time_applies = [&]() {
+            launch_apply(r.ptr, z.ptr, s);
+            SF_HIP(hipEventRecord(e0, s));
+            for (int k = 0; k < 8; ++k) launch_apply(r.ptr, z.ptr, s);
+            SF_HIP(hipEventRecord(e1, s));
+            SF_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return ms * 1e3f / 8;
+        };
+        float best = time_applies();
+        auto flip = [&](bool &flag, const std::string &what) {
+            flag = !flag;
+            const float t = time_applies();
+            if (verbose) std::fprintf(stderr, "[tune] %s %s: %.2f us per cycle against %.2f\n", what.c_str(), flag ? "fused" : "apart", t, best);
+            if (t < best) best = t;
+            else flag = !flag;
+        };
+        const char *prec = sizeof(V) == 4 ? " f32 " : " f64 ";
+        if (!std::is_same<V, double>::value && lv_[0].has_p) flip(fuse_convert_start_, std::string("L0") + prec + "conversion with the first start");
+        for (size_t l = 0; l < lv_.size(); ++l) {
+            LevelDev<S, V> &d = lv_[l];
+            if (!d.has_p) continue;
+            const std::string at = "L" + std::to_string(l) + prec;
+            if (smooths_first(lv_[l + 1])) flip(d.fuse_restrict_start, at + "restriction with the coarse start");
+            flip(d.fuse_post_start, at + "residual with the post-smoothing start");
+        }
+        SF_HIP(hipStreamSynchronize(s));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+
     // The Chebyshev start z = D^-1 b / theta, d = z, x = z (cycle.rs:578-590).
     static void launch_start(LevelDev<S, V> &d, const V *b, V *x, hipStream_t s) {
         const unsigned g = grid_for(d.n, 256);
@@ -957,6 +1008,7 @@ class Hierarchy final : public DeviceMultigrid {
     std::vector<Graph> graphs_;
     std::vector<LevelDev<S, V>> lv_;
     DeviceBuffer<V> in_, out_;
+    bool fuse_convert_start_ = true;  // single precision: the first start in the rounding of the residual (by timing)
     DeviceBuffer<V> min_, mout_;  // the finest level's columns in V, for the single precision
 };
 
@@ -1006,7 +1058,8 @@ DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, Device
     scalars_.resize(1);
     partials_.resize(DOT_BLOCKS);
     norm_.resize(1);
-    SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocDefault));
+    SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocCoherent));
+    SF_HIP(hipHostGetDevicePointer((void **)&mirror_, pinned_, 0));
     SF_HIP(hipEventCreateWithFlags(&copied_, hipEventDisableTiming));
     if (!mg_) {
         // Jacobi::new, preconditioner.rs:53-66
@@ -1065,24 +1118,25 @@ DevicePcg::~DevicePcg() {
     if (copied_) (void)hipEventDestroy(copied_);
 }
 
-void DevicePcg::dot(const double *a, const double *b, double *result, int final_kind) {
+void DevicePcg::dot(const double *a, const double *b, double *result, int final_kind, PcgScalars *mirror) {
     if (dot_order_ == SF_DOT_REFERENCE) {
-        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, b, final_kind, result, scalars_.ptr);
+        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, b, final_kind, result, scalars_.ptr, mirror);
     } else {
         hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, b, partials_.ptr);
         hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
-                           final_kind, result, scalars_.ptr);
+                           final_kind, result, scalars_.ptr, mirror);
     }
     SF_HIP(hipGetLastError());
 }
 
 void DevicePcg::sum(const double *a, double *result) {
     if (dot_order_ == SF_DOT_REFERENCE) {
-        hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, a, (int)FINAL_STORE, result, scalars_.ptr);
+        hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, a, (int)FINAL_STORE, result, scalars_.ptr,
+                           (PcgScalars *)nullptr);
     } else {
         hipLaunchKernelGGL((dot_partials_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, a, partials_.ptr);
         hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
-                           (int)FINAL_STORE, result, scalars_.ptr);
+                           (int)FINAL_STORE, result, scalars_.ptr, (PcgScalars *)nullptr);
     }
     SF_HIP(hipGetLastError());
 }
@@ -1118,8 +1172,8 @@ void DevicePcg::iterate() {
     Profiler::get().end(1, token, s);
     dot(p_.ptr, ap_.ptr, &sc->curvature);
     hipLaunchKernelGGL(pcg_update_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, sc, p_.ptr, ap_.ptr, x_.ptr, r_.ptr);
-    dot(r_.ptr, r_.ptr, &sc->rr);
-    SF_HIP(hipMemcpyAsync(pinned_, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    // r.r, whose final kernel also copies the scalars into host memory the host reads after the event
+    dot(r_.ptr, r_.ptr, &sc->rr, FINAL_STORE, mirror_);
     SF_HIP(hipEventRecord(copied_, s));
     precondition(r_.ptr, z_.ptr);
     dot(r_.ptr, z_.ptr, nullptr, FINAL_RZ);

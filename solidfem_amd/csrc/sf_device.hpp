@@ -201,7 +201,7 @@ class DevicePcg {
 
    private:
     void precondition(const double *r, double *z);
-    void dot(const double *a, const double *b, double *result, int final_kind = dev::FINAL_STORE);
+    void dot(const double *a, const double *b, double *result, int final_kind = dev::FINAL_STORE, dev::PcgScalars *mirror = nullptr);
     void sum(const double *a, double *result);
     void iterate();
     void reset_direction();
@@ -221,7 +221,8 @@ class DevicePcg {
     DeviceBuffer<double> b_, x_, r_, z_, p_, ap_, tmp_, tmp2_, jacobi_;
     DeviceBuffer<dev::PcgScalars> scalars_;
     DeviceBuffer<double> partials_, norm_;
-    dev::PcgScalars *pinned_ = nullptr;
+    dev::PcgScalars *pinned_ = nullptr;  // fine-grained host memory the r.r kernel copies the scalars into
+    dev::PcgScalars *mirror_ = nullptr;  // its device address
     hipEvent_t copied_ = nullptr;
 };
 

@@ -654,21 +654,27 @@ __device__ __forceinline__ void dot_finish(int kind, double total, double *out, 
     }
 }
 
+// `mirror`, when given, is host memory that receives a copy of the scalars once the sum is stored:
+// the host reads them after an event instead of a separate copy.
 static __global__ __launch_bounds__(DOT_THREADS) void dot_final_kernel(int parts, const double *__restrict__ partials, int kind,
-                                                                 double *out, PcgScalars *s) {
+                                                                 double *out, PcgScalars *s, PcgScalars *mirror) {
     double v = 0.0;
     for (int i = threadIdx.x; i < parts; i += blockDim.x) v = v + partials[i];
     const double total = block_sum(v);
-    if (threadIdx.x == 0) dot_finish(kind, total, out, s);
+    if (threadIdx.x == 0) {
+        dot_finish(kind, total, out, s);
+        if (mirror) *mirror = *s;
+    }
 }
 
 template <int MODE>
 __global__ void dot_serial_kernel(size_t n, const double *__restrict__ a, const double *__restrict__ b, int kind,
-                                  double *out, PcgScalars *s) {
+                                  double *out, PcgScalars *s, PcgScalars *mirror) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double sum = 0.0;
     for (size_t i = 0; i < n; ++i) sum = sum + (MODE == 0 ? a[i] * b[i] : a[i]);
     dot_finish(kind, sum, out, s);
+    if (mirror) *mirror = *s;
 }
 
 // The update of pcg.rs:219-230: the curvature test, then x += alpha p and r += (-alpha) A p.
