@@ -595,13 +595,13 @@ sf_status sf_csr_rounding_bound(sf_csr *c, const double *x, int32_t dot_order, d
             hipLaunchKernelGGL((dev::rounding_bound_kernel<1, 1>), grid, block, 0, s, c->system.sliced.view(), dx.ptr, rows.ptr);
         if (dot_order == SF_DOT_REFERENCE) {
             hipLaunchKernelGGL((dev::dot_serial_kernel<1>), dim3(1), dim3(64), 0, s, n, rows.ptr, rows.ptr, (int)dev::FINAL_STORE,
-                               result.ptr, (dev::PcgScalars *)nullptr, (dev::PcgScalars *)nullptr);
+                               result.ptr, (dev::PcgScalars *)nullptr, (dev::PcgScalars *)nullptr, 0u);
         } else {
             hipLaunchKernelGGL((dev::dot_partials_kernel<1>), dim3(dev::DOT_BLOCKS), dim3(dev::DOT_THREADS), 0, s, n, rows.ptr, rows.ptr,
                                partials.ptr);
             hipLaunchKernelGGL(dev::dot_final_kernel, dim3(1), dim3(dev::DOT_THREADS), 0, s, dev::DOT_BLOCKS,
                                (const double *)partials.ptr, (int)dev::FINAL_STORE, result.ptr, (dev::PcgScalars *)nullptr,
-                               (dev::PcgScalars *)nullptr);
+                               (dev::PcgScalars *)nullptr, 0u);
         }
         SF_HIP(hipGetLastError());
         double sum = 0;

@@ -1,5 +1,6 @@
 // Device objects of the solve path: sliced matrices, the V-cycle, the conjugate gradient.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1059,8 +1060,8 @@ DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, Device
     partials_.resize(DOT_BLOCKS);
     norm_.resize(1);
     SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocCoherent));
+    std::memset((void *)pinned_, 0, sizeof(PcgScalars));
     SF_HIP(hipHostGetDevicePointer((void **)&mirror_, pinned_, 0));
-    SF_HIP(hipEventCreateWithFlags(&copied_, hipEventDisableTiming));
     if (!mg_) {
         // Jacobi::new, preconditioner.rs:53-66
         std::vector<double> inv(n_);
@@ -1115,16 +1116,16 @@ void DevicePcg::restart(const double *new_rhs) {
 DevicePcg::~DevicePcg() {
     (void)hipStreamSynchronize(ctx_->stream);
     if (pinned_) (void)hipHostFree(pinned_);
-    if (copied_) (void)hipEventDestroy(copied_);
 }
 
-void DevicePcg::dot(const double *a, const double *b, double *result, int final_kind, PcgScalars *mirror) {
+void DevicePcg::dot(const double *a, const double *b, double *result, int final_kind, PcgScalars *mirror, uint32_t tag) {
     if (dot_order_ == SF_DOT_REFERENCE) {
-        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, b, final_kind, result, scalars_.ptr, mirror);
+        hipLaunchKernelGGL((dot_serial_kernel<0>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, b, final_kind, result, scalars_.ptr, mirror,
+                           tag);
     } else {
         hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, b, partials_.ptr);
         hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
-                           final_kind, result, scalars_.ptr, mirror);
+                           final_kind, result, scalars_.ptr, mirror, tag);
     }
     SF_HIP(hipGetLastError());
 }
@@ -1132,11 +1133,11 @@ void DevicePcg::dot(const double *a, const double *b, double *result, int final_
 void DevicePcg::sum(const double *a, double *result) {
     if (dot_order_ == SF_DOT_REFERENCE) {
         hipLaunchKernelGGL((dot_serial_kernel<1>), dim3(1), dim3(64), 0, ctx_->stream, n_, a, a, (int)FINAL_STORE, result, scalars_.ptr,
-                           (PcgScalars *)nullptr);
+                           (PcgScalars *)nullptr, 0u);
     } else {
         hipLaunchKernelGGL((dot_partials_kernel<1>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, ctx_->stream, n_, a, a, partials_.ptr);
         hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, ctx_->stream, DOT_BLOCKS, (const double *)partials_.ptr,
-                           (int)FINAL_STORE, result, scalars_.ptr, (PcgScalars *)nullptr);
+                           (int)FINAL_STORE, result, scalars_.ptr, (PcgScalars *)nullptr, 0u);
     }
     SF_HIP(hipGetLastError());
 }
@@ -1152,6 +1153,20 @@ void DevicePcg::reset_direction() {
     precondition(r_.ptr, z_.ptr);
     SF_HIP(hipMemcpyAsync(p_.ptr, z_.ptr, n_ * 8, hipMemcpyDeviceToDevice, ctx_->stream));
     dot(r_.ptr, z_.ptr, &scalars_.ptr->rz);
+}
+
+// Waits until the host copy of the scalars carries `tag`. The copy is written by a kernel already
+// enqueued; should it not arrive within a second of spinning, the stream is synchronised and the
+// copy checked once more, so a lost write shows as an error rather than a hang.
+void DevicePcg::wait_for_tag(uint32_t tag) {
+    const auto start = std::chrono::steady_clock::now();
+    for (uint64_t spins = 0;; ++spins) {
+        if (__atomic_load_n(&pinned_->tag, __ATOMIC_ACQUIRE) == tag) return;
+        if ((spins & 1023) == 1023 && std::chrono::steady_clock::now() - start > std::chrono::seconds(1)) break;
+    }
+    SF_HIP(hipStreamSynchronize(ctx_->stream));
+    if (__atomic_load_n(&pinned_->tag, __ATOMIC_ACQUIRE) != tag)
+        throw Error(SF_ERR_INTERNAL, "the solver's scalars did not reach the host");
 }
 
 PcgScalars DevicePcg::read_scalars() {
@@ -1172,14 +1187,14 @@ void DevicePcg::iterate() {
     Profiler::get().end(1, token, s);
     dot(p_.ptr, ap_.ptr, &sc->curvature);
     hipLaunchKernelGGL(pcg_update_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, sc, p_.ptr, ap_.ptr, x_.ptr, r_.ptr);
-    // r.r, whose final kernel also copies the scalars into host memory the host reads after the event
-    dot(r_.ptr, r_.ptr, &sc->rr, FINAL_STORE, mirror_);
-    SF_HIP(hipEventRecord(copied_, s));
+    // r.r, whose final kernel also copies the scalars into host memory, tagged for this readback
+    const uint32_t tag = ++readbacks_;
+    dot(r_.ptr, r_.ptr, &sc->rr, FINAL_STORE, mirror_, tag);
     precondition(r_.ptr, z_.ptr);
     dot(r_.ptr, z_.ptr, nullptr, FINAL_RZ);
     hipLaunchKernelGGL(pcg_direction_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, (const PcgScalars *)sc, z_.ptr, p_.ptr);
     SF_HIP(hipGetLastError());
-    SF_HIP(hipEventSynchronize(copied_));
+    wait_for_tag(tag);
     const PcgScalars h = *pinned_;
     if (!h.valid) {
         SF_HIP(hipStreamSynchronize(s));

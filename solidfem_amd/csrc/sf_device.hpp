@@ -201,7 +201,9 @@ class DevicePcg {
 
    private:
     void precondition(const double *r, double *z);
-    void dot(const double *a, const double *b, double *result, int final_kind = dev::FINAL_STORE, dev::PcgScalars *mirror = nullptr);
+    void dot(const double *a, const double *b, double *result, int final_kind = dev::FINAL_STORE, dev::PcgScalars *mirror = nullptr,
+             uint32_t tag = 0);
+    void wait_for_tag(uint32_t tag);
     void sum(const double *a, double *result);
     void iterate();
     void reset_direction();
@@ -223,7 +225,7 @@ class DevicePcg {
     DeviceBuffer<double> partials_, norm_;
     dev::PcgScalars *pinned_ = nullptr;  // fine-grained host memory the r.r kernel copies the scalars into
     dev::PcgScalars *mirror_ = nullptr;  // its device address
-    hipEvent_t copied_ = nullptr;
+    uint32_t readbacks_ = 0;             // tags of the copies so far
 };
 
 // Several load cases over one matrix and one preconditioner: every column runs the recurrence of

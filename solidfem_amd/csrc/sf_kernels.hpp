@@ -598,8 +598,24 @@ struct PcgScalars {
     double alpha;
     double beta;
     int32_t valid;     // the curvature test of pcg.rs:219-227 passed
-    int32_t pad;
+    uint32_t tag;      // host copies only: which readback the copy answers, written last
 };
+
+// Copies the scalars to host memory for the host to read without an event: every field first,
+// then, after a system-scope fence, the tag the host waits for.
+__device__ __forceinline__ void mirror_scalars(const PcgScalars *s, PcgScalars *mirror, uint32_t tag) {
+    mirror->rz = s->rz;
+    mirror->curvature = s->curvature;
+    mirror->rr = s->rr;
+    mirror->rz_next = s->rz_next;
+    mirror->true_rr = s->true_rr;
+    mirror->bound2 = s->bound2;
+    mirror->alpha = s->alpha;
+    mirror->beta = s->beta;
+    mirror->valid = s->valid;
+    __threadfence_system();
+    __hip_atomic_store(&mirror->tag, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Deterministic dot products. TREE order is a fixed two-stage reduction: DOT_BLOCKS blocks each
 // sum a fixed grid-stride pattern through a fixed shuffle tree into one partial, then one block
@@ -654,27 +670,27 @@ __device__ __forceinline__ void dot_finish(int kind, double total, double *out, 
     }
 }
 
-// `mirror`, when given, is host memory that receives a copy of the scalars once the sum is stored:
-// the host reads them after an event instead of a separate copy.
+// `mirror`, when given, is host memory that receives a copy of the scalars once the sum is stored,
+// tagged `tag`: the host waits for the tag instead of an event or a separate copy.
 static __global__ __launch_bounds__(DOT_THREADS) void dot_final_kernel(int parts, const double *__restrict__ partials, int kind,
-                                                                 double *out, PcgScalars *s, PcgScalars *mirror) {
+                                                                 double *out, PcgScalars *s, PcgScalars *mirror, uint32_t tag) {
     double v = 0.0;
     for (int i = threadIdx.x; i < parts; i += blockDim.x) v = v + partials[i];
     const double total = block_sum(v);
     if (threadIdx.x == 0) {
         dot_finish(kind, total, out, s);
-        if (mirror) *mirror = *s;
+        if (mirror) mirror_scalars(s, mirror, tag);
     }
 }
 
 template <int MODE>
 __global__ void dot_serial_kernel(size_t n, const double *__restrict__ a, const double *__restrict__ b, int kind,
-                                  double *out, PcgScalars *s, PcgScalars *mirror) {
+                                  double *out, PcgScalars *s, PcgScalars *mirror, uint32_t tag) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double sum = 0.0;
     for (size_t i = 0; i < n; ++i) sum = sum + (MODE == 0 ? a[i] * b[i] : a[i]);
     dot_finish(kind, sum, out, s);
-    if (mirror) *mirror = *s;
+    if (mirror) mirror_scalars(s, mirror, tag);
 }
 
 // The update of pcg.rs:219-230: the curvature test, then x += alpha p and r += (-alpha) A p.
