@@ -212,7 +212,7 @@ def serving(sf, np, count, rate, seed):
         f"p99 {st['p99_seconds'] * 1e3:.0f} ms; {st['builds']} builds, {st['cache_hits']} cache hits, "
         f"{st['evictions']} evictions")
     return {
-        "workload": f"{count} requests, Poisson arrivals at {rate}/s (seed {seed}); bars and plates of Tet10, "
+        "workload": f"{count} requests, Poisson arrivals at {rate}/s (seed {seed}); bars, tubes and plates of Tet10, "
                     f"log-uniform 5k-600k DOFs, E U[1,210] GPa, nu U[0.25,0.4], random clamp and loaded faces, "
                     f"random tractions N(0,1 MPa); 30% reuse an earlier system; HBM budget 200 GB; FP32 V-cycle, rtol 1e-8",
         "arrival_rate_per_s": rate,

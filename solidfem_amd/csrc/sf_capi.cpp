@@ -454,7 +454,7 @@ sf_status sf_study_builder_finish(sf_study_builder *b, sf_static_study **out) {
             const double t1 = seconds();
             st.mg.ctx = b->ctx;
             st.mg.mg = make_device_multigrid(b->setup->take_levels(), b->mg_options.degree, b->mg_options.precision,
-                                             b->mg_options.coarse_solve, s);
+                                             b->mg_options.coarse_solve, s, &st.system.system.sliced);
             pre += seconds() - t1;
         }
         st.pcg = std::make_unique<sf_pcg>();

@@ -57,7 +57,7 @@ Profiler &Profiler::get() {
     return p;
 }
 
-int Profiler::begin(int kernel, hipStream_t s, double bytes) {
+int Profiler::begin(int kernel, hipStream_t s, double bytes, u64 launches) {
     if (kernel < 0 || kernel >= KERNELS) return -1;
     std::lock_guard<std::mutex> g(lock);
     Slot &slot = slots[kernel];
@@ -72,6 +72,7 @@ int Profiler::begin(int kernel, hipStream_t s, double bytes) {
         }
     }
     p.bytes = bytes;
+    p.launches = launches;
     SF_HIP(hipEventRecord(p.start, s));
     slot.pending.push_back(p);
     return (int)slot.pending.size() - 1;
@@ -92,7 +93,7 @@ void Profiler::collect(int kernel) {
         SF_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
         slot.seconds += ms * 1e-3;
         slot.bytes += p.bytes;
-        slot.launches += 1;
+        slot.launches += p.launches;
         slot.pool.push_back(p.start);
         slot.pool.push_back(p.stop);
     }
@@ -189,6 +190,31 @@ void upload_sliced(DeviceSliced<T> &out, const HostBsr<double> &m, hipStream_t s
     DeviceBuffer<double> src;
     src.upload(m.values, s);
     build_sliced(out, plan, m.cols, src.ptr, s);
+}
+
+// The same sliced matrix in precision T from one already on the device in f64: the layout arrays
+// are copied and every value rounded to T, which is what the host path's gather would produce.
+template <typename T>
+void convert_sliced(DeviceSliced<T> &out, const DeviceSliced<double> &in, hipStream_t s) {
+    out.rb = in.rb;
+    out.cb = in.cb;
+    out.rows = in.rows;
+    out.block_cols = in.block_cols;
+    out.blocks = in.blocks;
+    out.slices = in.slices;
+    auto copy = [&](DeviceBuffer<uint32_t> &to, const DeviceBuffer<uint32_t> &from) {
+        to.resize(from.count);
+        if (from.count) SF_HIP(hipMemcpyAsync(to.ptr, from.ptr, from.count * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    };
+    copy(out.slice_start, in.slice_start);
+    copy(out.row_len, in.row_len);
+    copy(out.perm, in.perm);
+    copy(out.cols, in.cols);
+    out.vals.resize(in.vals.count);
+    if (in.vals.count)
+        hipLaunchKernelGGL((convert_kernel<T, double>), dim3(grid_for(in.vals.count, 256)), dim3(256), 0, s, in.vals.count,
+                           (const double *)in.vals.ptr, out.vals.ptr);
+    SF_HIP(hipGetLastError());
 }
 
 template void upload_sliced<double>(DeviceSliced<double> &, const HostBsr<double> &, hipStream_t);
@@ -385,7 +411,7 @@ struct LevelDev {
 template <typename S, typename V>
 class Hierarchy final : public DeviceMultigrid {
    public:
-    Hierarchy(std::vector<HostLevel> levels, size_t deg, int prec, int coarse, hipStream_t s) {
+    Hierarchy(std::vector<HostLevel> levels, size_t deg, int prec, int coarse, hipStream_t s, const DeviceSliced<double> *finest) {
         precision = prec;
         coarse_mode = coarse;
         degree = deg;
@@ -397,7 +423,11 @@ class Hierarchy final : public DeviceMultigrid {
             d.rb = (int)h.a.rb;
             d.rows = h.a.rows;
             d.n = h.a.dofs();
-            upload_sliced(d.a, h.a, s);
+            if (l == 0 && finest && finest->rows == h.a.rows && finest->blocks == h.a.blocks() && finest->rb == (int)h.a.rb &&
+                finest->cb == (int)h.a.cb)
+                convert_sliced(d.a, *finest, s);  // the study's matrix, already on the device
+            else
+                upload_sliced(d.a, h.a, s);
             if (l > 0) upload_bsr(d.a_rows, h.a, s);
             std::vector<S> dinv(h.inverse_diagonal.size());
             for (size_t i = 0; i < dinv.size(); ++i) dinv[i] = (S)h.inverse_diagonal[i];
@@ -888,6 +918,16 @@ class Hierarchy final : public DeviceMultigrid {
         V *dd[2] = {d.d0.ptr, d.d1.ptr};
         V *rr[2] = {d.r0.ptr, d.r1.ptr};
         const size_t steps = d.steps.size();
+        // the finest level's steps run back to back, timed as one bracket of `steps` launches;
+        // each step's algorithmic bytes: matrix, block inverse, six vector passes, and the final
+        // add (and widening) in the last
+        int token = -1;
+        if (finest && steps) {
+            const double vec = (double)d.n * sizeof(V);
+            const double each = (double)d.a.algorithmic_matrix_bytes() + (double)d.rows * d.rb * d.rb * sizeof(S) + 6 * vec;
+            const double last = (sum_into ? 2 * vec : 0.0) + (sum_into && widen ? (double)d.n * 8 : 0.0);
+            token = Profiler::get().begin(0, s, each * (double)steps + last, steps);
+        }
         for (size_t k = 1; k <= steps; ++k) {
             const V *d_in = dd[(k - 1) & 1];
             const V *r_in = k == 1 ? b : rr[(k - 1) & 1];
@@ -896,15 +936,10 @@ class Hierarchy final : public DeviceMultigrid {
             const V keep = (V)d.steps[k - 1].first, stp = (V)d.steps[k - 1].second;
             V *into = k == steps ? sum_into : nullptr;
             double *wide = into ? widen : nullptr;
-            const double vec = (double)d.n * sizeof(V);
-            const int token = finest ? Profiler::get().begin(0, s, (double)d.a.algorithmic_matrix_bytes() +
-                                                                       (double)d.rows * d.rb * d.rb * sizeof(S) + 6 * vec +
-                                                                       (into ? 2 * vec : 0.0) + (wide ? (double)d.n * 8 : 0.0))
-                                     : -1;
             if (d.cheb_rows) launch_cheb_rows(d, d.cheb_rows, d_in, r_in, keep, stp, d_out, r_out, x, into, s, wide);
             else launch_cheb_step(d, d.a.batch_cheb, d_in, r_in, keep, stp, d_out, r_out, x, into, s, wide);
-            Profiler::get().end(0, token, s);
         }
+        Profiler::get().end(0, token, s);
         if (steps == 0 && sum_into) {
             hipLaunchKernelGGL((add_kernel<V>), dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, sum_into, (const V *)x);
             if (widen)
@@ -1016,17 +1051,17 @@ class Hierarchy final : public DeviceMultigrid {
 }  // namespace
 
 std::unique_ptr<DeviceMultigrid> make_device_multigrid(std::vector<HostLevel> levels, size_t degree, int precision,
-                                                       int coarse_mode, hipStream_t s) {
+                                                       int coarse_mode, hipStream_t s, const DeviceSliced<double> *finest) {
     if (degree < 1) throw Error(SF_ERR_INVALID_ARGUMENT, "the smoother degree must be at least one");
     if (coarse_mode != SF_COARSE_REFERENCE && coarse_mode != SF_COARSE_INVERSE)
         throw Error(SF_ERR_INVALID_ARGUMENT, "unknown coarse solve");
     switch (precision) {
         case SF_PRECISION_DOUBLE:
-            return std::make_unique<Hierarchy<double, double>>(std::move(levels), degree, precision, coarse_mode, s);
+            return std::make_unique<Hierarchy<double, double>>(std::move(levels), degree, precision, coarse_mode, s, finest);
         case SF_PRECISION_SINGLE_STORAGE:
-            return std::make_unique<Hierarchy<float, double>>(std::move(levels), degree, precision, coarse_mode, s);
+            return std::make_unique<Hierarchy<float, double>>(std::move(levels), degree, precision, coarse_mode, s, finest);
         case SF_PRECISION_SINGLE:
-            return std::make_unique<Hierarchy<float, float>>(std::move(levels), degree, precision, coarse_mode, s);
+            return std::make_unique<Hierarchy<float, float>>(std::move(levels), degree, precision, coarse_mode, s, finest);
         default:
             throw Error(SF_ERR_INVALID_ARGUMENT, "unknown precision");
     }

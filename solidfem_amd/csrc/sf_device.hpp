@@ -116,6 +116,7 @@ struct Profiler {
     struct Pending {
         hipEvent_t start, stop;
         double bytes;
+        u64 launches;  // launches of the kernel between the two events (back to back)
     };
     struct Slot {
         bool on = false;
@@ -128,7 +129,7 @@ struct Profiler {
     std::mutex lock;
     static Profiler &get();
     // Returns an index into the pending list, or -1 when off.
-    int begin(int kernel, hipStream_t s, double bytes);
+    int begin(int kernel, hipStream_t s, double bytes, u64 launches = 1);
     void end(int kernel, int token, hipStream_t s);
     void collect(int kernel);
     void reset();
@@ -164,8 +165,11 @@ class DeviceMultigrid {
     // of launching them one by one (not while kernel timing is on). The same kernels, the same bits.
     bool use_graph = false;
 };
+// `finest`, when given, is level 0's matrix already on the device in f64 (a study's system
+// matrix, the same pattern and values): the hierarchy's finest level is converted from it instead
+// of being planned and uploaded from the host copy again.
 std::unique_ptr<DeviceMultigrid> make_device_multigrid(std::vector<HostLevel> levels, size_t degree, int precision,
-                                                       int coarse_mode, hipStream_t s);
+                                                       int coarse_mode, hipStream_t s, const DeviceSliced<double> *finest = nullptr);
 
 // ---- system matrix and conjugate gradient ---------------------------------------------------
 

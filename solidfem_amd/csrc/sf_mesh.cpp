@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <numeric>
 
 #include "sf_host.hpp"
@@ -23,32 +24,106 @@ size_t host_threads() {
     return n;
 }
 
+namespace {
+
+// A persistent pool of host_threads() - 1 workers taking chunks of parallel_for calls from one
+// queue. The caller of parallel_for runs chunks too (its own, or any other call's), so calls made
+// from several threads at once, or from inside a chunk, always progress. Chunk boundaries depend
+// only on n and the thread count, never on which thread runs a chunk.
+class HostPool {
+   public:
+    struct Job {
+        const std::function<void(size_t, size_t)> *body = nullptr;
+        size_t remaining = 0;  // chunks not yet finished, guarded by the pool's lock
+        std::exception_ptr failure;
+        std::condition_variable done;
+    };
+    struct Task {
+        Job *job;
+        size_t first, last;
+    };
+
+    static HostPool &get() {
+        // never destroyed: workers may still wait on the queue when the process exits
+        static HostPool *pool = new HostPool(host_threads() > 1 ? host_threads() - 1 : 0);
+        return *pool;
+    }
+
+    void run(Job &job, const std::vector<Task> &tasks) {
+        std::unique_lock<std::mutex> lock(m_);
+        job.remaining = tasks.size();
+        for (size_t i = 1; i < tasks.size(); ++i) queue_.push_back(tasks[i]);
+        wake_.notify_all();
+        lock.unlock();
+        execute(tasks[0]);
+        lock.lock();
+        while (job.remaining > 0) {
+            if (!queue_.empty()) {
+                const Task t = queue_.front();
+                queue_.pop_front();
+                lock.unlock();
+                execute(t);
+                lock.lock();
+            } else {
+                job.done.wait(lock);
+            }
+        }
+    }
+
+   private:
+    explicit HostPool(size_t workers) {
+        for (size_t i = 0; i < workers; ++i)
+            std::thread([this] {
+                std::unique_lock<std::mutex> lock(m_);
+                for (;;) {
+                    wake_.wait(lock, [this] { return !queue_.empty(); });
+                    const Task t = queue_.front();
+                    queue_.pop_front();
+                    lock.unlock();
+                    execute(t);
+                    lock.lock();
+                }
+            }).detach();
+    }
+
+    void execute(const Task &t) {
+        std::exception_ptr failure;
+        try {
+            (*t.job->body)(t.first, t.last);
+        } catch (...) {
+            failure = std::current_exception();
+        }
+        std::lock_guard<std::mutex> g(m_);
+        if (failure && !t.job->failure) t.job->failure = failure;
+        if (--t.job->remaining == 0) t.job->done.notify_all();
+    }
+
+    std::mutex m_;
+    std::condition_variable wake_;
+    std::deque<Task> queue_;
+};
+
+}  // namespace
+
 void parallel_for(size_t n, size_t min_chunk, const std::function<void(size_t, size_t)> &body) {
-    size_t threads = host_threads();
+    const size_t threads = host_threads();
     if (n == 0) return;
-    size_t chunks = std::min(threads, (n + min_chunk - 1) / std::max<size_t>(min_chunk, 1));
+    const size_t chunks = std::min(threads, (n + min_chunk - 1) / std::max<size_t>(min_chunk, 1));
     if (chunks <= 1) {
         body(0, n);
         return;
     }
-    std::vector<std::thread> pool;
-    std::exception_ptr failure;
-    std::mutex m;
-    size_t per = (n + chunks - 1) / chunks;
+    const size_t per = (n + chunks - 1) / chunks;
+    HostPool::Job job;
+    job.body = &body;
+    std::vector<HostPool::Task> tasks;
     for (size_t c = 0; c < chunks; ++c) {
-        size_t first = c * per, last = std::min(n, first + per);
+        const size_t first = c * per, last = std::min(n, first + per);
         if (first >= last) break;
-        pool.emplace_back([&, first, last] {
-            try {
-                body(first, last);
-            } catch (...) {
-                std::lock_guard<std::mutex> g(m);
-                if (!failure) failure = std::current_exception();
-            }
-        });
+        tasks.push_back({&job, first, last});
     }
-    for (auto &t : pool) t.join();
-    if (failure) std::rethrow_exception(failure);
+    HostPool::get().run(job, tasks);
+    if (job.failure) std::rethrow_exception(job.failure);
 }
 
 // ---- meshes -----------------------------------------------------------------------------------
