@@ -1077,6 +1077,13 @@ void DeviceSystem::upload_from_host(hipStream_t s) {
 void DeviceSystem::build_from_device_values(hipStream_t s) {
     SlicedPlan plan = plan_sliced(host.starts.data(), host.columns.data(), host.rows, (int)host.rb, (int)host.cb);
     build_sliced(sliced, plan, host.cols, values.ptr, s);
+    // SOLIDFEM_STREAM_SYSTEM=1: the conjugate gradient's product reads the system matrix with
+    // non-temporal loads (the same values, the same bits)
+    static const bool stream = [] {
+        const char *e = std::getenv("SOLIDFEM_STREAM_SYSTEM");
+        return e && std::atoi(e) != 0;
+    }();
+    sliced.streaming = stream;
     tune_spmv<double, double>(sliced, EPI_STORE, s);
 }
 
