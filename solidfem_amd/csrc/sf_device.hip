@@ -916,7 +916,8 @@ class Hierarchy final : public DeviceMultigrid {
             if (d.rb != 6 || d.n > SMALL_DOFS || !d.has_p || d.p.rb != 6 || d.p.cb != 6) break;
             --from;
         }
-        if (last.rb != 6 || last.n > SMALL_DOFS || from >= levels - 1) return;
+        // the coarsest level's right-hand side is staged in the LDS array as f64
+        if (last.rb != 6 || last.n * sizeof(double) > SMALL_DOFS * sizeof(V) || from >= levels - 1) return;
         std::vector<SmallLevel<S, V>> host_levels;
         for (int l = from; l < levels; ++l) {
             LevelDev<S, V> &d = lv_[(size_t)l];

@@ -622,13 +622,31 @@ struct SmallLevel {
 // transpose's restriction and coarse_inverse_kernel), so every value has those kernels' bits.
 template <typename S, typename V>
 __device__ __forceinline__ V small_row(const Bsr<S> &a, uint32_t i, const V *__restrict__ x) {
+    // eight blocks per memory round trip: their columns, then their values and x entries, then
+    // their terms in stored order
+    constexpr int B = 8;
     const uint32_t row = i / 6, p = i - row * 6;
+    const uint32_t first = a.starts[row], last = a.starts[row + 1];
     V sum = (V)0;
-    for (uint32_t at = a.starts[row]; at < a.starts[row + 1]; ++at) {
-        const S *blk = a.vals + (size_t)at * 36 + p * 6;
-        const V *xc = x + (size_t)a.cols[at] * 6;
+    for (uint32_t at0 = first; at0 < last; at0 += B) {
+        const uint32_t m = last - at0 < (uint32_t)B ? last - at0 : (uint32_t)B;
+        uint32_t c[B];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) sum = sum + widen<V>(blk[q]) * xc[q];
+        for (int u = 0; u < B; ++u) c[u] = (uint32_t)u < m ? a.cols[at0 + u] : 0u;
+        S v[B][6];
+        V xv[B][6];
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                v[u][q] = (uint32_t)u < m ? a.vals[(size_t)(at0 + u) * 36 + p * 6 + q] : (S)0;
+                xv[u][q] = (uint32_t)u < m ? x[(size_t)c[u] * 6 + q] : (V)0;
+            }
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if ((uint32_t)u < m)
+#pragma unroll
+                for (int q = 0; q < 6; ++q) sum = sum + widen<V>(v[u][q]) * xv[u][q];
     }
     return sum;
 }
@@ -700,9 +718,22 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_cycle_kernel(const SmallL
     for (int l = 0; l < count; ++l) {
         const SmallLevel<S, V> &d = lv[l];
         if (!d.has_p) {
+            // x = inverse b in f64, each entry summed over k ascending (coarse_inverse_kernel),
+            // b staged in LDS and 32 inverse entries in flight per thread
+            double *bs = (double *)rn;
+            for (uint32_t k = threadIdx.x; k < d.n; k += blockDim.x) bs[k] = (double)d.b[k];
+            __syncthreads();
             for (uint32_t i = threadIdx.x; i < d.n; i += blockDim.x) {
                 double sum = 0.0;
-                for (uint32_t k = 0; k < d.n; ++k) sum = sum + d.inverse[(size_t)k * d.n + i] * (double)d.b[k];
+                uint32_t k = 0;
+                for (; k + 32 <= d.n; k += 32) {
+                    double e[32];
+#pragma unroll
+                    for (int u = 0; u < 32; ++u) e[u] = d.inverse[(size_t)(k + u) * d.n + i];
+#pragma unroll
+                    for (int u = 0; u < 32; ++u) sum = sum + e[u] * bs[k + u];
+                }
+                for (; k < d.n; ++k) sum = sum + d.inverse[(size_t)k * d.n + i] * bs[k];
                 d.x[i] = (V)sum;
             }
             __syncthreads();
