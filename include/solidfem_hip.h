@@ -426,12 +426,6 @@ sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision,
  * each pair of input and output buffers, instead of launching the kernels one by one. Results are
  * the same bits. Off by default; never used while kernel timing (sf_profile_enable) is on. */
 sf_status sf_multigrid_set_graph(sf_multigrid *mg, int32_t enable);
-/* Launch plan, no reference counterpart: the first level whose whole sub-cycle runs as one
- * single-workgroup launch (levels of 6 x 6 blocks with at most 8192 unknowns each, down to a
- * coarsest level with its explicit inverse), or -1 when every level launches kernel by kernel.
- * Chosen by timing when the hierarchy is built; SOLIDFEM_SMALL_CYCLE=0 or 1 fixes it. The same
- * bits either way. */
-int32_t sf_multigrid_small_cycle(const sf_multigrid *mg);
 /* Algorithmic HBM bytes of one V-cycle of this hierarchy (every array a kernel touches, once). */
 double sf_multigrid_cycle_bytes(const sf_multigrid *mg);
 /* Restarts a solve from a zero estimate (Pcg::new semantics, pcg.rs:135-164) on the buffers it

@@ -684,10 +684,6 @@ class Multigrid(_Handle, _Levels):
         """Replay each V-cycle as one captured HIP graph (the same bits); see sf_multigrid_set_graph."""
         _raise(library().sf_multigrid_set_graph(self._ptr, 1 if enable else 0))
 
-    def small_cycle_level(self) -> int:
-        """The first level whose sub-cycle runs as one single-workgroup launch, or -1."""
-        return library().sf_multigrid_small_cycle(self._ptr)
-
     def apply_batch(self, residuals) -> np.ndarray:
         """apply() of each row of `residuals` (columns x n), all through one batched V-cycle."""
         r = np.ascontiguousarray(residuals, dtype=np.float64)

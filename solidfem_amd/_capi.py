@@ -206,7 +206,6 @@ SIGNATURES = {
     "sf_stream_read": (i32, [vp, u64, u32, P(f64)]),
     "sf_multigrid_apply_batch": (i32, [vp, P(f64), u64, P(f64)]),
     "sf_multigrid_set_graph": (i32, [vp, i32]),
-    "sf_multigrid_small_cycle": (i32, [vp]),
     "sf_recover_stress": (i32, [vp, vp, P(f64), P(f64), P(f64)]),
     "sf_static_study_recover": (i32, [vp, P(f64), P(f64)]),
     "sf_model_points_per_element": (u64, [vp]),

@@ -839,8 +839,6 @@ sf_status sf_multigrid_set_graph(sf_multigrid *mg, int32_t enable) {
     });
 }
 
-int32_t sf_multigrid_small_cycle(const sf_multigrid *mg) { return mg && mg->mg ? mg->mg->small_cycle_level() : -1; }
-
 sf_status sf_multigrid_apply_batch(sf_multigrid *mg, const double *r, uint64_t columns, double *z) {
     return guard([&] {
         if (columns == 0 || columns > MAX_COLUMNS) throw Error(SF_ERR_INVALID_ARGUMENT, "a batch carries 1 to 64 columns");

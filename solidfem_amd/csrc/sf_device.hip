@@ -389,7 +389,6 @@ struct LevelDev {
     DeviceBsr<S> r_rows;          // the same transpose in plain block rows, for long_rows_kernel
     int restrict_rows = 0;  // 0: the sliced restriction; else long_rows_kernel with this many rows per wave (by timing)
     DeviceBsr<S> a_rows;          // coarse levels: A in plain block rows, for the row kernels
-    DeviceBsr<S> p_rows;          // levels of the fused small cycle: P in plain block rows
     int cheb_rows = 0, residual_rows = 0;  // the same choice for the steps and residuals (by timing)
     bool fuse_restrict_start = true;       // the coarser level's first start in the restriction (by timing)
     bool fuse_post_start = true;           // the post-smoothing's start in the second residual (by timing)
@@ -512,7 +511,6 @@ class Hierarchy final : public DeviceMultigrid {
             }
             for (auto *buf : {&d.b, &d.x, &d.d0, &d.d1, &d.r0, &d.r1, &d.res, &d.px}) buf->resize(d.n);
         }
-        plan_small_cycle(s);
         in_.resize(host.empty() ? 0 : host[0].a.dofs());
         out_.resize(in_.count);
         SF_HIP(hipStreamSynchronize(s));
@@ -882,8 +880,6 @@ class Hierarchy final : public DeviceMultigrid {
             else flag = !flag;
         };
         const char *prec = sizeof(V) == 4 ? " f32 " : " f64 ";
-        if (small_from_ > 0 && small_forced_ < 0)
-            flip(use_small_, std::string("L") + std::to_string(small_from_) + prec + "and below in one workgroup");
         if (!std::is_same<V, double>::value && lv_[0].has_p) flip(fuse_convert_start_, std::string("L0") + prec + "conversion with the first start");
         for (size_t l = 0; l < lv_.size(); ++l) {
             LevelDev<S, V> &d = lv_[l];
@@ -895,67 +891,6 @@ class Hierarchy final : public DeviceMultigrid {
         SF_HIP(hipStreamSynchronize(s));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-    }
-
-    int small_cycle_level() const override { return use_small_ ? small_from_ : -1; }
-
-    // The levels from the first one (below the finest) whose whole sub-cycle fits one workgroup:
-    // 6 x 6 blocks, at most SMALL_DOFS unknowns each, and a coarsest level with its explicit
-    // inverse. Their V-cycle then runs as one launch of small_cycle_kernel instead of a dozen
-    // latency-bound ones, each value with the same bits (the choice is timed in tune_fusions).
-    void plan_small_cycle(hipStream_t s) {
-        small_from_ = -1;
-        use_small_ = false;
-        const int levels = (int)lv_.size();
-        if (levels < 3 || degree - 1 > (size_t)SMALL_STEPS) return;
-        const LevelDev<S, V> &last = lv_[(size_t)levels - 1];
-        if (last.has_p || !last.has_factor || !last.inverse.ptr) return;
-        int from = levels - 1;
-        while (from - 1 >= 1) {
-            const LevelDev<S, V> &d = lv_[(size_t)from - 1];
-            if (d.rb != 6 || d.n > SMALL_DOFS || !d.has_p || d.p.rb != 6 || d.p.cb != 6) break;
-            --from;
-        }
-        // the coarsest level's right-hand side is staged in the LDS array as f64
-        if (last.rb != 6 || last.n * sizeof(double) > SMALL_DOFS * sizeof(V) || from >= levels - 1) return;
-        std::vector<SmallLevel<S, V>> host_levels;
-        for (int l = from; l < levels; ++l) {
-            LevelDev<S, V> &d = lv_[(size_t)l];
-            SmallLevel<S, V> m{};
-            m.a = d.a_rows.view();
-            m.dinv = d.dinv.ptr;
-            if (d.has_p) {
-                upload_bsr(d.p_rows, host[(size_t)l].prolongator, s);
-                m.p = d.p_rows.view();
-                m.r = d.r_rows.view();
-            }
-            m.b = d.b.ptr;
-            m.x = d.x.ptr;
-            m.d0 = d.d0.ptr;
-            m.d1 = d.d1.ptr;
-            m.r0 = d.r0.ptr;
-            m.r1 = d.r1.ptr;
-            m.res = d.res.ptr;
-            m.px = d.px.ptr;
-            m.inverse = d.inverse.ptr;
-            m.theta = (V)d.theta;
-            m.n = (uint32_t)d.n;
-            m.steps = (uint32_t)d.steps.size();
-            m.has_p = d.has_p ? 1u : 0u;
-            for (size_t k = 0; k < d.steps.size(); ++k) {
-                m.keep[k] = (V)d.steps[k].first;
-                m.step[k] = (V)d.steps[k].second;
-            }
-            host_levels.push_back(m);
-        }
-        small_levels_.resize(host_levels.size() * sizeof(SmallLevel<S, V>));
-        small_levels_.upload((const char *)host_levels.data(), small_levels_.count, s);
-        small_from_ = from;
-        use_small_ = true;
-        // SOLIDFEM_SMALL_CYCLE=0 or 1 fixes the choice for hierarchies built from then on
-        if (const char *e = std::getenv("SOLIDFEM_SMALL_CYCLE")) small_forced_ = std::atoi(e) != 0 ? 1 : 0;
-        else small_forced_ = -1;
-        if (small_forced_ >= 0) use_small_ = small_forced_ == 1;
     }
 
     // The Chebyshev start z = D^-1 b / theta, d = z, x = z (cycle.rs:578-590).
@@ -1050,11 +985,7 @@ class Hierarchy final : public DeviceMultigrid {
         // fused form measured faster)
         const bool fuse = smooths_first(c) && d.fuse_restrict_start;
         restrict_to(d, c, s, fuse);
-        if (use_small_ && (int)l + 1 == small_from_)
-            hipLaunchKernelGGL((small_cycle_kernel<S, V>), dim3(1), dim3(SMALL_THREADS), 0, s, (const SmallLevel<S, V> *)small_levels_.ptr,
-                               (int)lv_.size() - small_from_, fuse ? 1 : 0);
-        else
-            cycle(l + 1, c.b.ptr, c.x.ptr, s, fuse);
+        cycle(l + 1, c.b.ptr, c.x.ptr, s, fuse);
         launch_spmv<S, V>(d.p, c.x.ptr, x, nullptr, EPI_ADD, s);
         // the residual for the post-smoothing, with its start when fused
         const bool fuse_post = d.fuse_post_start;
@@ -1114,10 +1045,6 @@ class Hierarchy final : public DeviceMultigrid {
     std::vector<LevelDev<S, V>> lv_;
     DeviceBuffer<V> in_, out_;
     bool fuse_convert_start_ = true;  // single precision: the first start in the rounding of the residual (by timing)
-    int small_from_ = -1;             // first level of the fused small cycle, or -1
-    bool use_small_ = false;          // run it (by timing)
-    int small_forced_ = -1;           // SOLIDFEM_SMALL_CYCLE: 0 or 1 fixes use_small_
-    DeviceBuffer<char> small_levels_; // its SmallLevel descriptors
     DeviceBuffer<V> min_, mout_;  // the finest level's columns in V, for the single precision
 };
 

@@ -154,8 +154,6 @@ class DeviceMultigrid {
     // Z = M^-1 R for M row-interleaved f64 columns (entry (i, m) at i * M + m); column m is bit for
     // bit apply() of column m.
     virtual void apply_batch(const double *r, double *z, uint32_t columns, hipStream_t s) = 0;
-    // The first level of the single-workgroup sub-cycle when it runs, else -1.
-    virtual int small_cycle_level() const { return -1; }
     int precision = SF_PRECISION_SINGLE_STORAGE;
     int coarse_mode = SF_COARSE_REFERENCE;
     size_t degree = 3;

@@ -592,171 +592,6 @@ __global__ __launch_bounds__(256) void coarse_inverse_kernel(uint32_t n, const d
     if (i < n) out[i] = (V)sum;
 }
 
-// ---- the cycle below a level, in one workgroup --------------------------------------------------
-
-// The Chebyshev pairs one smoothing can carry in the fused small cycle (degree up to 17).
-constexpr int SMALL_STEPS = 16;
-// The largest level (unknowns) the fused small cycle takes: its residual exchange lives in LDS.
-constexpr uint32_t SMALL_DOFS = 8192;
-constexpr int SMALL_THREADS = 1024;
-
-// A level with 6 x 6 blocks as the fused small cycle reads it: plain block rows in natural order.
-template <typename S, typename V>
-struct SmallLevel {
-    Bsr<S> a;          // the level's matrix
-    const S *dinv;     // block-Jacobi inverse, 36 per block row
-    Bsr<S> p;          // prolongator: this level's rows, the next level's columns (has_p)
-    Bsr<S> r;          // its stored transpose: the next level's rows (has_p)
-    V *b, *x, *d0, *d1, *r0, *r1, *res, *px;
-    const double *inverse;  // the coarsest level's f64 inverse by columns, n x n (no prolongator)
-    V theta;
-    uint32_t n;
-    uint32_t steps;    // degree - 1
-    uint32_t has_p;
-    V keep[SMALL_STEPS], step[SMALL_STEPS];
-};
-
-// The phases of the fused small cycle. Each loops over scalar rows (or coarse rows) with the
-// workgroup's threads and repeats, per output, the arithmetic and summation order of the kernel
-// it replaces (cheb_start_kernel, cheb_step_kernel, spmv_kernel with its epilogues, the stored
-// transpose's restriction and coarse_inverse_kernel), so every value has those kernels' bits.
-template <typename S, typename V>
-__device__ __forceinline__ V small_row(const Bsr<S> &a, uint32_t i, const V *__restrict__ x) {
-    // eight blocks per memory round trip: their columns, then their values and x entries, then
-    // their terms in stored order
-    constexpr int B = 8;
-    const uint32_t row = i / 6, p = i - row * 6;
-    const uint32_t first = a.starts[row], last = a.starts[row + 1];
-    V sum = (V)0;
-    for (uint32_t at0 = first; at0 < last; at0 += B) {
-        const uint32_t m = last - at0 < (uint32_t)B ? last - at0 : (uint32_t)B;
-        uint32_t c[B];
-#pragma unroll
-        for (int u = 0; u < B; ++u) c[u] = (uint32_t)u < m ? a.cols[at0 + u] : 0u;
-        S v[B][6];
-        V xv[B][6];
-#pragma unroll
-        for (int u = 0; u < B; ++u)
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                v[u][q] = (uint32_t)u < m ? a.vals[(size_t)(at0 + u) * 36 + p * 6 + q] : (S)0;
-                xv[u][q] = (uint32_t)u < m ? x[(size_t)c[u] * 6 + q] : (V)0;
-            }
-#pragma unroll
-        for (int u = 0; u < B; ++u)
-            if ((uint32_t)u < m)
-#pragma unroll
-                for (int q = 0; q < 6; ++q) sum = sum + widen<V>(v[u][q]) * xv[u][q];
-    }
-    return sum;
-}
-
-template <typename S, typename V>
-__device__ void small_start(const SmallLevel<S, V> &d, const V *b, V *x) {
-    for (uint32_t i = threadIdx.x; i < d.n; i += blockDim.x) {
-        const uint32_t row = i / 6, p = i - row * 6;
-        const S *dr = d.dinv + (size_t)row * 36 + p * 6;
-        V z = (V)0;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) z = z + widen<V>(dr[q]) * b[row * 6 + q];
-        const V dv = z / d.theta;
-        d.d0[i] = dv;
-        x[i] = dv;
-    }
-    __syncthreads();
-}
-
-// The degree - 1 fused steps of a smoothing whose start already ran; the last adds into
-// `sum_into` when given.
-template <typename S, typename V>
-__device__ void small_steps(const SmallLevel<S, V> &d, const V *b, V *x, V *sum_into, V *rn) {
-    V *dd[2] = {d.d0, d.d1};
-    V *rr[2] = {d.r0, d.r1};
-    for (uint32_t k = 1; k <= d.steps; ++k) {
-        const V *d_in = dd[(k - 1) & 1];
-        const V *r_in = k == 1 ? b : rr[(k - 1) & 1];
-        V *d_out = dd[k & 1];
-        V *r_out = rr[k & 1];
-        const V keep = d.keep[k - 1], stp = d.step[k - 1];
-        V *into = k == d.steps ? sum_into : nullptr;
-        for (uint32_t i = threadIdx.x; i < d.n; i += blockDim.x) rn[i] = r_in[i] - small_row(d.a, i, d_in);
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < d.n; i += blockDim.x) {
-            const uint32_t row = i / 6, p = i - row * 6;
-            const S *dr = d.dinv + (size_t)row * 36 + p * 6;
-            V z = (V)0;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) z = z + widen<V>(dr[q]) * rn[row * 6 + q];
-            const V dn = keep * d_in[i] + stp * z;
-            const V xn = x[i] + dn;
-            d_out[i] = dn;
-            r_out[i] = rn[i];
-            x[i] = xn;
-            if (into) into[i] = into[i] + xn;
-        }
-        __syncthreads();
-    }
-    if (d.steps == 0 && sum_into) {
-        for (uint32_t i = threadIdx.x; i < d.n; i += blockDim.x) sum_into[i] = sum_into[i] + x[i];
-        __syncthreads();
-    }
-}
-
-template <typename S, typename V>
-__device__ void small_residual(const SmallLevel<S, V> &d, const V *x, const V *b) {
-    for (uint32_t i = threadIdx.x; i < d.n; i += blockDim.x) d.res[i] = b[i] - small_row(d.a, i, x);
-    __syncthreads();
-}
-
-// The V-cycle (cycle.rs:461-539) over `count` levels from lv[0], whose right-hand side is lv[0].b;
-// `started`: lv[0]'s first start already ran (fused into the restriction that produced b). The
-// coarsest level applies its f64 inverse. One workgroup: the phases meet at barriers.
-template <typename S, typename V>
-__global__ __launch_bounds__(SMALL_THREADS) void small_cycle_kernel(const SmallLevel<S, V> *__restrict__ lv, int count,
-                                                                    int started) {
-    __shared__ V rn[SMALL_DOFS];
-    for (int l = 0; l < count; ++l) {
-        const SmallLevel<S, V> &d = lv[l];
-        if (!d.has_p) {
-            // x = inverse b in f64, each entry summed over k ascending (coarse_inverse_kernel),
-            // b staged in LDS and 32 inverse entries in flight per thread
-            double *bs = (double *)rn;
-            for (uint32_t k = threadIdx.x; k < d.n; k += blockDim.x) bs[k] = (double)d.b[k];
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < d.n; i += blockDim.x) {
-                double sum = 0.0;
-                uint32_t k = 0;
-                for (; k + 32 <= d.n; k += 32) {
-                    double e[32];
-#pragma unroll
-                    for (int u = 0; u < 32; ++u) e[u] = d.inverse[(size_t)(k + u) * d.n + i];
-#pragma unroll
-                    for (int u = 0; u < 32; ++u) sum = sum + e[u] * bs[k + u];
-                }
-                for (; k < d.n; ++k) sum = sum + d.inverse[(size_t)k * d.n + i] * bs[k];
-                d.x[i] = (V)sum;
-            }
-            __syncthreads();
-            break;
-        }
-        if (!(l == 0 && started)) small_start(d, d.b, d.x);
-        small_steps(d, d.b, d.x, (V *)nullptr, rn);
-        small_residual(d, d.x, d.b);
-        const SmallLevel<S, V> &c = lv[l + 1];
-        for (uint32_t j = threadIdx.x; j < c.n; j += blockDim.x) c.b[j] = small_row(d.r, j, (const V *)d.res);
-        __syncthreads();
-    }
-    for (int l = count - 2; l >= 0; --l) {
-        const SmallLevel<S, V> &d = lv[l];
-        const SmallLevel<S, V> &c = lv[l + 1];
-        for (uint32_t i = threadIdx.x; i < d.n; i += blockDim.x) d.x[i] = d.x[i] + small_row(d.p, i, (const V *)c.x);
-        __syncthreads();
-        small_residual(d, d.x, d.b);
-        small_start(d, d.res, d.px);
-        small_steps(d, d.res, d.px, d.x, rn);
-    }
-}
-
 // ---- conjugate gradient vectors -------------------------------------------------------------
 
 // Scalars of a solve, resident on the device.
