@@ -43,10 +43,19 @@ __device__ __forceinline__ void rows_product(const Sliced<S> &a, uint32_t slot0,
     const uint32_t *cp = a.cols + (size_t)slot0 * NPW + j;
     const S *vp = a.vals + ((size_t)slot0 * WAVE + (size_t)j * RB) * CB;
     const uint32_t full = len - len % BATCH;
-    for (uint32_t k = 0; k < full; k += BATCH) {
-        uint32_t c[BATCH];
+    // the next batch's block columns are loaded with this batch's values and x entries, so a
+    // batch costs one dependent memory round trip instead of two
+    uint32_t c[BATCH];
+    if (full) {
 #pragma unroll
-        for (int b = 0; b < BATCH; ++b) c[b] = cp[(size_t)(k + b) * NPW];
+        for (int b = 0; b < BATCH; ++b) c[b] = cp[(size_t)b * NPW];
+    }
+    for (uint32_t k = 0; k < full; k += BATCH) {
+        uint32_t cn[BATCH];
+        if (k + BATCH < full) {
+#pragma unroll
+            for (int b = 0; b < BATCH; ++b) cn[b] = cp[(size_t)(k + BATCH + b) * NPW];
+        }
         V xv[BATCH][CB];
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
@@ -63,6 +72,8 @@ __device__ __forceinline__ void rows_product(const Sliced<S> &a, uint32_t slot0,
             for (int p = 0; p < RB; ++p)
 #pragma unroll
                 for (int q = 0; q < CB; ++q) sum[p] = sum[p] + widen<V>(v[b][p * CB + q]) * xv[b][q];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) c[b] = cn[b];
     }
     for (uint32_t k = full; k < len; ++k) {
         const uint32_t col = cp[(size_t)k * NPW];
