@@ -108,22 +108,7 @@ DevicePcgBatch::DevicePcgBatch(std::shared_ptr<Context> ctx, const DeviceSystem 
         return true;
     }();
     (void)lds_set;
-    if (!mg_) {
-        // Jacobi::new, preconditioner.rs:53-66
-        std::vector<double> inv(n_);
-        const auto &h = a_.host;
-        for (size_t row = 0; row < n_; ++row) {
-            const size_t brow = row / h.rb, p = row % h.rb;
-            double diag = 0.0;
-            for (int64_t at = h.starts[brow]; at < h.starts[brow + 1]; ++at)
-                if (h.columns[(size_t)at] * h.cb <= row && row < (h.columns[(size_t)at] + 1) * h.cb)
-                    diag = h.values[(size_t)at * h.rb * h.cb + p * h.cb + (row - h.columns[(size_t)at] * h.cb)];
-            if (std::isnan(diag) || diag <= 0.0 || !std::isfinite(diag))
-                throw Error(SF_ERR_JACOBI_DIAGONAL, "a stiffness matrix needs a positive diagonal", (int64_t)row, 0, diag);
-            inv[row] = 1.0 / diag;
-        }
-        jacobi_.upload(inv, s);
-    }
+    if (!mg_) jacobi_.upload(jacobi_inverse(ctx_, a_), s);
     restart(rhs);
 }
 

@@ -135,6 +135,7 @@ void level_matrix_view(const std::vector<HostLevel> &h, bool single, uint64_t le
                        uint32_t *columns, double *values) {
     if (level >= h.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
     if (prolongator && !h[level].has_prolongator) throw Error(SF_ERR_INVALID_ARGUMENT, "the coarsest level has no prolongator");
+    if (values && !prolongator) h[level].fetch_values();
     const HostBsr<double> &m = prolongator ? h[level].prolongator : h[level].a;
     if (rows) *rows = m.rows;
     if (block_columns) *block_columns = m.cols;
@@ -339,8 +340,11 @@ sf_status sf_study_builder_create(sf_context *ctx, const sf_model *model, const 
         b->use_multigrid = use_multigrid != 0;
         if (b->use_multigrid) b->mg_options = setup_options(mg);
         // StudyBuilder::new, study.rs:419-474
+        StageClock clock;
         b->problem = number_and_load(b->model);
+        clock.lap("numbering, validation and load");
         b->pattern = build_pattern(b->model.mesh, b->problem.free, b->problem.free_count);
+        clock.lap("block pattern and assembly map");
         b->study = std::make_unique<sf_static_study>();
         b->study->ctx = ctx->ctx;
         auto &sys = b->study->system.system;
@@ -352,6 +356,8 @@ sf_status sf_study_builder_create(sf_context *ctx, const sf_model *model, const 
         sys.values.resize(std::max<size_t>(sys.host.blocks() * 9, 1));
         SF_HIP(hipMemsetAsync(sys.values.ptr, 0, sys.host.blocks() * 9 * sizeof(double), s));
         b->assembly.prepare(b->model, b->pattern, s);
+        SF_HIP(hipStreamSynchronize(s));
+        clock.lap("assembly upload");
         b->assembly_seconds = seconds() - t0;
         *out = b.release();
     });
@@ -377,15 +383,18 @@ sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_c
         }
         SF_HIP(hipStreamSynchronize(s));
         b->assembly_seconds += seconds() - t0;
+        StageClock clock;
+        clock.lap("element stiffness");
         if (b->next == total) {
             if (!b->prescribed_applied) {
                 b->assembly.check_fault(s);
-                // the host copy the setup reads
-                sys.host.values.resize(sys.host.blocks() * 9);
-                sys.values.download(sys.host.values.data(), sys.host.values.size(), s);
                 SF_HIP(hipStreamSynchronize(s));
+                clock.lap("element stiffness kernels");
                 if (!b->problem.prescribed.empty()) {
-                    // study.rs:567-593: symmetric elimination on the scalar entries
+                    // study.rs:567-593: symmetric elimination on the scalar entries of a host copy
+                    sys.host.values.resize(sys.host.blocks() * 9);
+                    sys.values.download(sys.host.values.data(), sys.host.values.size(), s);
+                    SF_HIP(hipStreamSynchronize(s));
                     auto &h = sys.host;
                     auto entry = [&](size_t row, size_t col) -> double * {
                         const size_t br = row / 3, bc = col / 3;
@@ -425,10 +434,29 @@ sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_c
                     positions.reserve(b->problem.free_count);
                     for (size_t n = 0; n < b->model.mesh.nodes.size(); ++n)
                         if (b->problem.free[n] != FIXED) positions.push_back(b->model.mesh.nodes[n]);
-                    b->setup = std::make_unique<MultigridSetup>(sys.host, modes, positions, b->mg_options);
+                    clock.lap("rigid modes");
+                    // The setup gets the pattern. The values stay with the system on the device
+                    // (the setup fetches what the host phases read) unless the elimination above
+                    // brought them to the host.
+                    HostBsr<double> finest;
+                    finest.rows = sys.host.rows;
+                    finest.cols = sys.host.cols;
+                    finest.rb = sys.host.rb;
+                    finest.cb = sys.host.cb;
+                    finest.starts = sys.host.starts;
+                    finest.columns = sys.host.columns;
+                    const bool held = sys.host.values.empty();
+                    finest.values = std::move(sys.host.values);
+                    sys.host.values = std::vector<double>();
+                    b->setup = std::make_unique<MultigridSetup>(std::move(finest), modes, positions, b->mg_options);
                     attach_device_setup(*b->setup, b->ctx, b->mg_options, sys.values.ptr);
+                    if (held)
+                        b->setup->hold_values_elsewhere(
+                            std::make_shared<DeviceValues>(b->ctx, sys.values.ptr, sys.host.blocks(), sys.host.block_len()));
+                    clock.lap("setup creation");
                 }
                 b->setup_finished = b->setup->step(max_elements, cancel);
+                clock.lap("multigrid setup steps");
                 b->preconditioner_seconds += seconds() - t1;
             }
         }
@@ -448,6 +476,7 @@ sf_status sf_study_builder_finish(sf_study_builder *b, sf_static_study **out) {
         const double t0 = seconds();
         hipStream_t s = b->ctx->stream;
         auto &st = *b->study;
+        StageClock clock;
         st.system.system.build_from_device_values(s);
         double pre = b->preconditioner_seconds;
         if (b->use_multigrid) {
@@ -457,8 +486,10 @@ sf_status sf_study_builder_finish(sf_study_builder *b, sf_static_study **out) {
                                              b->mg_options.coarse_solve, s, &st.system.system.sliced);
             pre += seconds() - t1;
         }
+        clock.lap("system layout and hierarchy");
         st.pcg = std::make_unique<sf_pcg>();
         st.pcg->pcg = std::make_unique<DevicePcg>(b->ctx, st.system.system, st.mg.mg.get(), b->problem.rhs, b->pcg);
+        clock.lap("conjugate gradient creation");
         st.model = std::move(b->model);
         st.problem = std::move(b->problem);
         auto &r = st.report;
@@ -628,7 +659,14 @@ sf_status sf_multigrid_setup_create(sf_context *ctx, const sf_csr *matrix, const
         auto s = std::make_unique<sf_multigrid_setup>();
         s->ctx = ctx ? ctx->ctx : nullptr;
         const SetupOptions opt = setup_options(options);
-        s->setup = std::make_unique<MultigridSetup>(sys.host, modes, pos, opt);
+        HostBsr<double> a = sys.host;
+        if (a.values.empty() && a.blocks() > 0) {
+            // a study's system keeps its values on the device: the setup takes a host copy, as it
+            // may outlive the study
+            a.values.resize(a.blocks() * a.block_len());
+            DeviceValues(matrix->ctx, sys.values.ptr, a.blocks(), a.block_len()).values(a.values.data());
+        }
+        s->setup = std::make_unique<MultigridSetup>(std::move(a), modes, pos, opt);
         attach_device_setup(*s->setup, s->ctx, opt, nullptr);
         *out = s.release();
     });
@@ -693,7 +731,7 @@ sf_status sf_multigrid_level_sizes(const sf_multigrid *mg, uint64_t *out, uint64
         for (size_t l = 0; l < h.size() && l < capacity; ++l) {
             out[3 * l] = h[l].a.rows;
             out[3 * l + 1] = h[l].a.rb;
-            out[3 * l + 2] = h[l].a.values.size();
+            out[3 * l + 2] = h[l].a.blocks() * h[l].a.block_len();
         }
     });
 }
@@ -962,7 +1000,11 @@ sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision,
     return guard([&] {
         auto m = std::make_unique<sf_multigrid>();
         m->ctx = mg->ctx;
-        m->mg = make_device_multigrid(mg->mg->host, mg->mg->degree, precision, coarse_solve < 0 ? mg->mg->coarse_mode : coarse_solve,
+        // the copy holds every value itself: it may outlive the study whose system holds the finest
+        for (const auto &h : mg->mg->host) h.fetch_values();
+        std::vector<HostLevel> levels = mg->mg->host;
+        for (auto &h : levels) h.source.reset();
+        m->mg = make_device_multigrid(std::move(levels), mg->mg->degree, precision, coarse_solve < 0 ? mg->mg->coarse_mode : coarse_solve,
                                       mg->ctx->stream);
         *out = m.release();
     });

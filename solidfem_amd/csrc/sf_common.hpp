@@ -7,8 +7,11 @@
 
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <functional>
 #include <mutex>
 #include <stdexcept>
@@ -104,6 +107,21 @@ struct Cancel {
     sf_cancel_fn fn = nullptr;
     void *user = nullptr;
     bool operator()() const { return fn ? fn(user) != 0 : false; }
+};
+
+// SOLIDFEM_SETUP_LOG: the wall time of each stage of a study's construction, on stderr.
+struct StageClock {
+    std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+    static bool on() {
+        static const bool log = std::getenv("SOLIDFEM_SETUP_LOG") != nullptr;
+        return log;
+    }
+    void lap(const char *what) {
+        if (!on()) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[build] %s %.1f ms\n", what, 1e3 * std::chrono::duration<double>(now - last).count());
+        last = now;
+    }
 };
 
 }  // namespace sf

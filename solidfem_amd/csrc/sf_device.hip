@@ -8,6 +8,7 @@
 
 #include "sf_batch_kernels.hpp"
 #include "sf_device.hpp"
+#include "sf_setup_device.hpp"
 
 namespace sf {
 
@@ -417,6 +418,7 @@ class Hierarchy final : public DeviceMultigrid {
         degree = deg;
         host = std::move(levels);
         lv_.resize(host.size());
+        StageClock clock;
         for (size_t l = 0; l < host.size(); ++l) {
             const HostLevel &h = host[l];
             LevelDev<S, V> &d = lv_[l];
@@ -426,17 +428,21 @@ class Hierarchy final : public DeviceMultigrid {
             if (l == 0 && finest && finest->rows == h.a.rows && finest->blocks == h.a.blocks() && finest->rb == (int)h.a.rb &&
                 finest->cb == (int)h.a.cb)
                 convert_sliced(d.a, *finest, s);  // the study's matrix, already on the device
-            else
+            else {
+                h.fetch_values();
                 upload_sliced(d.a, h.a, s);
+            }
             if (l > 0) upload_bsr(d.a_rows, h.a, s);
             std::vector<S> dinv(h.inverse_diagonal.size());
             for (size_t i = 0; i < dinv.size(); ++i) dinv[i] = (S)h.inverse_diagonal[i];
             d.dinv.upload(dinv, s);
             SF_HIP(hipStreamSynchronize(s));
+            clock.lap("  level matrix and block inverse");
             chebyshev(h.lambda_min, h.lambda_max, degree, d.theta, d.steps);
             if (h.has_prolongator) {
                 d.has_p = true;
                 upload_sliced(d.p, h.prolongator, s);
+                clock.lap("  prolongator layout");
                 // the stored transpose: block row c holds block column c of P in ascending fine
                 // row order, each block transposed (crates/solidfem-solve/src/gpu/layout.rs:254-291)
                 HostBsr<double> t;
@@ -459,8 +465,11 @@ class Hierarchy final : public DeviceMultigrid {
                         for (size_t pp = 0; pp < P.rb; ++pp)
                             for (size_t q = 0; q < P.cb; ++q) t.values[to * bl + q * P.rb + pp] = P.values[(size_t)at * bl + pp * P.cb + q];
                     }
+                clock.lap("  restriction transpose");
                 upload_sliced(d.r, t, s);
+                clock.lap("  restriction layout");
                 upload_bsr(d.r_rows, t, s);
+                clock.lap("  restriction rows");
             }
             if (h.has_factor) {
                 d.has_factor = true;
@@ -510,12 +519,16 @@ class Hierarchy final : public DeviceMultigrid {
                 }
             }
             for (auto *buf : {&d.b, &d.x, &d.d0, &d.d1, &d.r0, &d.r1, &d.res, &d.px}) buf->resize(d.n);
+            SF_HIP(hipStreamSynchronize(s));
+            clock.lap(l == 0 ? "hierarchy level 0 upload" : "hierarchy coarse level upload");
         }
         in_.resize(host.empty() ? 0 : host[0].a.dofs());
         out_.resize(in_.count);
         SF_HIP(hipStreamSynchronize(s));
         tune(s);
+        clock.lap("hierarchy kernel tuning");
         tune_fusions(s);
+        clock.lap("hierarchy fusion tuning");
         compute_stats();
     }
 
@@ -999,11 +1012,12 @@ class Hierarchy final : public DeviceMultigrid {
 
     void compute_stats() {
         // MultigridStats::of_levels, cycle.rs:676-699, with stored values in S
-        const double finest = host.empty() ? 1.0 : (double)host[0].a.values.size();
+        const double finest = host.empty() ? 1.0 : (double)(host[0].a.blocks() * host[0].a.block_len());
         size_t stored = 0, memory = 0;
         for (const auto &h : host) {
-            stored += h.a.values.size();
-            memory += h.a.starts.size() * 8 + h.a.columns.size() * 4 + h.a.values.size() * sizeof(S);
+            const size_t values = h.a.blocks() * h.a.block_len();
+            stored += values;
+            memory += h.a.starts.size() * 8 + h.a.columns.size() * 4 + values * sizeof(S);
             memory += h.inverse_diagonal.size() * sizeof(S);
             if (h.has_prolongator)
                 memory += h.prolongator.starts.size() * 8 + h.prolongator.columns.size() * 4 + h.prolongator.values.size() * sizeof(S);
@@ -1075,8 +1089,12 @@ void DeviceSystem::upload_from_host(hipStream_t s) {
 }
 
 void DeviceSystem::build_from_device_values(hipStream_t s) {
+    StageClock clock;
     SlicedPlan plan = plan_sliced(host.starts.data(), host.columns.data(), host.rows, (int)host.rb, (int)host.cb);
+    clock.lap("system layout plan");
     build_sliced(sliced, plan, host.cols, values.ptr, s);
+    SF_HIP(hipStreamSynchronize(s));
+    clock.lap("system layout build");
     // SOLIDFEM_STREAM_SYSTEM=1: the conjugate gradient's product reads the system matrix with
     // non-temporal loads (the same values, the same bits)
     static const bool stream = [] {
@@ -1085,9 +1103,50 @@ void DeviceSystem::build_from_device_values(hipStream_t s) {
     }();
     sliced.streaming = stream;
     tune_spmv<double, double>(sliced, EPI_STORE, s);
+    clock.lap("system product tuning");
 }
 
 // ---- the conjugate gradient -------------------------------------------------------------------
+
+std::vector<double> jacobi_inverse(const std::shared_ptr<Context> &ctx, const DeviceSystem &a) {
+    // Jacobi::new, preconditioner.rs:53-66
+    const auto &h = a.host;
+    const size_t n = a.size;
+    std::vector<double> inv(n);
+    // A study keeps its values on the device: its diagonal blocks come to the host.
+    std::vector<double> diagonal;
+    std::vector<char> has_diagonal;
+    if (h.values.empty() && h.blocks() > 0) {
+        if (h.rb != h.cb) throw Error(SF_ERR_INTERNAL, "a system matrix of non-square blocks");
+        std::vector<int64_t> slots(h.rows, 0);
+        has_diagonal.assign(h.rows, 0);
+        for (size_t br = 0; br < h.rows; ++br) {
+            auto first = h.columns.begin() + h.starts[br], last = h.columns.begin() + h.starts[br + 1];
+            auto it = std::lower_bound(first, last, (u32)br);
+            if (it != last && *it == (u32)br) {
+                slots[br] = it - h.columns.begin();
+                has_diagonal[br] = 1;
+            }
+        }
+        diagonal.resize(h.rows * h.block_len());
+        DeviceValues(ctx, a.values.ptr, h.blocks(), h.block_len()).blocks(slots, diagonal.data());
+    }
+    for (size_t row = 0; row < n; ++row) {
+        const size_t brow = row / h.rb, p = row % h.rb;
+        double diag = 0.0;
+        if (!diagonal.empty()) {
+            if (has_diagonal[brow]) diag = diagonal[brow * h.block_len() + p * h.cb + p];
+        } else {
+            for (int64_t at = h.starts[brow]; at < h.starts[brow + 1]; ++at)
+                if (h.columns[(size_t)at] * h.cb <= row && row < (h.columns[(size_t)at] + 1) * h.cb)
+                    diag = h.values[(size_t)at * h.rb * h.cb + p * h.cb + (row - h.columns[(size_t)at] * h.cb)];
+        }
+        if (std::isnan(diag) || diag <= 0.0 || !std::isfinite(diag))
+            throw Error(SF_ERR_JACOBI_DIAGONAL, "a stiffness matrix needs a positive diagonal", (int64_t)row, 0, diag);
+        inv[row] = 1.0 / diag;
+    }
+    return inv;
+}
 
 DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
                      const sf_pcg_options &options)
@@ -1104,22 +1163,7 @@ DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, Device
     SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocCoherent));
     std::memset((void *)pinned_, 0, sizeof(PcgScalars));
     SF_HIP(hipHostGetDevicePointer((void **)&mirror_, pinned_, 0));
-    if (!mg_) {
-        // Jacobi::new, preconditioner.rs:53-66
-        std::vector<double> inv(n_);
-        const auto &h = a_.host;
-        for (size_t row = 0; row < n_; ++row) {
-            const size_t brow = row / h.rb, p = row % h.rb;
-            double diag = 0.0;
-            for (int64_t at = h.starts[brow]; at < h.starts[brow + 1]; ++at)
-                if (h.columns[(size_t)at] * h.cb <= row && row < (h.columns[(size_t)at] + 1) * h.cb)
-                    diag = h.values[(size_t)at * h.rb * h.cb + p * h.cb + (row - h.columns[(size_t)at] * h.cb)];
-            if (std::isnan(diag) || diag <= 0.0 || !std::isfinite(diag))
-                throw Error(SF_ERR_JACOBI_DIAGONAL, "a stiffness matrix needs a positive diagonal", (int64_t)row, 0, diag);
-            inv[row] = 1.0 / diag;
-        }
-        jacobi_.upload(inv, s);
-    }
+    if (!mg_) jacobi_.upload(jacobi_inverse(ctx_, a_), s);
     initialise();
 }
 

@@ -186,6 +186,10 @@ struct DeviceSystem {
     void build_from_device_values(hipStream_t s);
 };
 
+// The Jacobi preconditioner's inverse diagonal of a system (Jacobi::new,
+// crates/solidfem-solve/src/preconditioner.rs:53-66); throws SF_ERR_JACOBI_DIAGONAL.
+std::vector<double> jacobi_inverse(const std::shared_ptr<Context> &ctx, const DeviceSystem &a);
+
 enum class PcgStatus { Active, Converged, Cancelled, NotPositiveDefinite, NotConverged };
 
 class DevicePcg {

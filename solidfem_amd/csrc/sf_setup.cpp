@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 
 namespace sf {
@@ -191,19 +193,63 @@ void MultigridSetup::spectral_iteration() {
     });
 }
 
+void MultigridSetup::charge_phase(int level) {
+    static const bool log = std::getenv("SOLIDFEM_SETUP_LOG") != nullptr;
+    if (!log) return;
+    const auto now = std::chrono::steady_clock::now();
+    if (phase_seconds_.size() <= (size_t)timed_level_) phase_seconds_.resize((size_t)timed_level_ + 1, {});
+    phase_seconds_[(size_t)timed_level_][(size_t)timed_phase_] += std::chrono::duration<double>(now - timed_since_).count();
+    timed_since_ = now;
+    timed_level_ = level;
+    timed_phase_ = phase_;
+    if (phase_ != Phase::Done) return;
+    static const char *names[] = {"diagonal", "strength", "seed", "join", "sweep", "group", "tentative",
+                                  "spectral", "filter", "smooth", "product", "galerkin", "factor"};
+    for (size_t l = 0; l < phase_seconds_.size(); ++l) {
+        std::fprintf(stderr, "[setup] level %zu:", l);
+        for (size_t p = 0; p < 13; ++p)
+            if (phase_seconds_[l][p] > 0) std::fprintf(stderr, " %s %.1f ms", names[p], 1e3 * phase_seconds_[l][p]);
+        std::fprintf(stderr, "\n");
+    }
+}
+
 bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
     if (failed_) throw failure_;
     if (cancelled_) throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
+    timed_since_ = std::chrono::steady_clock::now();
+    struct Charge {
+        MultigridSetup *s;
+        ~Charge() { if (s->phase_ != Phase::Done) s->charge_phase((int)s->levels_.size()); }
+    } charge_on_exit{this};
     try {
         for (;;) {
             const size_t rows = current_.rows, rb = current_.rb;
             const int level = (int)levels_.size();
+            if (phase_ != timed_phase_ || level != timed_level_) charge_phase(level);
             switch (phase_) {
                 case Phase::Done:
                     return true;
                 case Phase::Diagonal: {
                     // setup.rs:269-321
-                    if (cursor_ == 0) inverse_diagonal_.assign(rows * rb * rb, 0.0);
+                    const bool held = source_ && current_.values.empty() && current_.blocks() > 0;
+                    if (cursor_ == 0) {
+                        inverse_diagonal_.assign(rows * rb * rb, 0.0);
+                        if (held) {
+                            // only the diagonal blocks come to the host (a missing one reads slot
+                            // 0 and is refused below before use)
+                            std::vector<int64_t> slots(rows, 0);
+                            parallel_for(rows, 4096, [&](size_t a, size_t b) {
+                                for (size_t row = a; row < b; ++row) {
+                                    auto first = current_.columns.begin() + current_.starts[row];
+                                    auto last = current_.columns.begin() + current_.starts[row + 1];
+                                    auto it = std::lower_bound(first, last, (u32)row);
+                                    if (it != last && *it == (u32)row) slots[row] = it - current_.columns.begin();
+                                }
+                            });
+                            diagonal_blocks_.resize(rows * rb * rb);
+                            source_->blocks(slots, diagonal_blocks_.data());
+                        }
+                    }
                     bool done = run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         parallel_for(l - f, 1024, [&](size_t a, size_t b) {
                             for (size_t row = f + a; row < f + b; ++row) {
@@ -213,14 +259,16 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                                 if (it == last || *it != (u32)row)
                                     throw Error(SF_ERR_MG_NOT_POSITIVE_DEFINITE, "a diagonal block is missing", level, (int64_t)row, 0.0);
                                 double pivot = 0;
-                                if (!invert_spd(current_.block((size_t)(it - current_.columns.begin())), rb,
-                                                &inverse_diagonal_[row * rb * rb], pivot))
+                                const double *block = held ? &diagonal_blocks_[row * rb * rb]
+                                                           : current_.block((size_t)(it - current_.columns.begin()));
+                                if (!invert_spd(block, rb, &inverse_diagonal_[row * rb * rb], pivot))
                                     throw Error(SF_ERR_MG_NOT_POSITIVE_DEFINITE, "a diagonal block is not positive definite", level,
                                                 (int64_t)row, pivot);
                             }
                         });
                     });
                     if (!done) return false;
+                    diagonal_blocks_ = std::vector<double>();
                     if (current_.dofs() <= options_.coarse_dofs || (size_t)level + 1 >= options_.max_levels) {
                         is_coarsest_ = true;
                         phase_ = Phase::Spectral;
@@ -293,6 +341,15 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     // aggregate.rs:141-161: join the strongest seeded neighbour's aggregate. A row
                     // reads only seeded rows' aggregates, which this pass never writes, and writes
                     // only its own, so the rows run in parallel with the serial pass's result.
+                    const bool held = source_ && current_.values.empty() && current_.blocks() > 0;
+                    if (held && cursor_ == 0) {
+                        // the same norms from the sums of squares summed on the device in order
+                        block_norms_.resize(current_.blocks());
+                        source_->block_squares(block_norms_.data());
+                        parallel_for(block_norms_.size(), 65536, [&](size_t a, size_t b) {
+                            for (size_t i = a; i < b; ++i) block_norms_[i] = std::sqrt(block_norms_[i]);
+                        });
+                    }
                     bool done = run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         parallel_for(l - f, 1024, [&](size_t a, size_t b) {
                             for (size_t row = f + a; row < f + b; ++row) {
@@ -304,7 +361,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                                 for (int64_t at = current_.starts[row]; at < current_.starts[row + 1]; ++at) {
                                     const u32 col = current_.columns[(size_t)at];
                                     if (!std::binary_search(nb, ne, col) || !seeded_[col]) continue;
-                                    const double norm = frobenius(current_.block((size_t)at), rb * current_.cb);
+                                    const double norm = held ? block_norms_[(size_t)at] : frobenius(current_.block((size_t)at), rb * current_.cb);
                                     if (norm > best) {
                                         best = norm;
                                         chosen = aggregate_[col];
@@ -315,6 +372,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         });
                     });
                     if (!done) return false;
+                    block_norms_ = std::vector<double>();
                     phase_ = Phase::Sweep;
                     break;
                 }
@@ -446,6 +504,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         power_done_ = POWER_ITERATIONS;
                         budget = budget > rows * POWER_ITERATIONS ? budget - rows * POWER_ITERATIONS : 0;
                     }
+                    if (power_done_ < POWER_ITERATIONS) ensure_values();
                     while (power_done_ < POWER_ITERATIONS) {
                         if (budget == 0) return false;
                         if (cancel()) {
@@ -464,6 +523,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     if (is_coarsest_) {
                         if (current_.dofs() <= DENSE_LIMIT) {
                             // setup.rs:451-468
+                            ensure_values();
                             HostLevel l;
                             const size_t n = current_.dofs();
                             l.factor.assign(n * n, 0.0);
@@ -546,6 +606,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         on_accelerator = device_p_ = accelerator_->smooth(aggregate_, aggregate_count_, ns, nc, tentative_values_, omega, prolongator_);
                         if (on_accelerator) budget = budget > rows ? budget - rows : 0;
                     }
+                    if (!on_accelerator) ensure_values();
                     bool done = on_accelerator || run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         std::vector<RowOut> out(l - f);
                         parallel_for(l - f, 256, [&](size_t a, size_t b) {
@@ -602,6 +663,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         on_accelerator = device_ap_ = accelerator_->product(aggregate_count_);
                         if (on_accelerator) budget = budget > rows ? budget - rows : 0;
                     }
+                    if (!on_accelerator) ensure_values();
                     bool done = on_accelerator || run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         std::vector<RowOut> out(l - f);
                         parallel_for(l - f, 256, [&](size_t a, size_t b) {
@@ -736,9 +798,23 @@ void MultigridSetup::release_accelerator_level() {
     on_device_ = device_p_ = device_ap_ = false;
 }
 
+void MultigridSetup::ensure_values() {
+    if (!source_ || !current_.values.empty() || current_.blocks() == 0) return;
+    current_.values.resize(current_.blocks() * current_.block_len());
+    source_->values(current_.values.data());
+}
+
+void HostLevel::fetch_values() const {
+    if (!source || !a.values.empty() || a.blocks() == 0) return;
+    a.values.resize(a.blocks() * a.block_len());
+    source->values(a.values.data());
+}
+
 void MultigridSetup::push_final_level() {
     release_accelerator_level();
     HostLevel l;
+    if (current_.values.empty()) l.source = source_;
+    source_.reset();
     l.lambda_min = lambda_max_ / options_.range;
     l.lambda_max = lambda_max_;
     l.a = std::move(current_);
@@ -759,6 +835,8 @@ void MultigridSetup::finish_level() {
             }
         }
     HostLevel l;
+    if (current_.values.empty()) l.source = source_;
+    source_.reset();
     l.lambda_min = lambda_max_ / options_.range;
     l.lambda_max = lambda_max_;
     l.a = std::move(current_);

@@ -4,14 +4,33 @@
 // in row order, so neither the thread count nor the slicing changes a bit.
 #pragma once
 
+#include <array>
+#include <chrono>
 #include <memory>
 
 #include "sf_host.hpp"
 
 namespace sf {
 
+// The values of a matrix kept elsewhere (a study's assembled system on the device), fetched
+// whole or in part when the host needs them.
+class ValueSource {
+   public:
+    virtual ~ValueSource() = default;
+    // Every value: row-major blocks in storage order.
+    virtual void values(double *out) const = 0;
+    // The blocks at the given storage slots, in that order.
+    virtual void blocks(const std::vector<int64_t> &slots, double *out) const = 0;
+    // Per stored block, the sum of its squared values in storage order, from zero.
+    virtual void block_squares(double *out) const = 0;
+};
+
 struct HostLevel {
-    HostBsr<double> a;
+    // The level's matrix. The finest level of a study leaves its values with the study's system
+    // on the device (`source`); fetch_values() brings them to the host on first use.
+    mutable HostBsr<double> a;
+    std::shared_ptr<const ValueSource> source;
+    void fetch_values() const;
     std::vector<double> inverse_diagonal;
     double lambda_min = 0, lambda_max = 0;
     bool has_prolongator = false;
@@ -56,6 +75,9 @@ class MultigridSetup {
                    const SetupOptions &options);
     // Hands the numeric phases of the levels it takes to `accelerator`.
     void attach(std::unique_ptr<SetupAccelerator> accelerator) { accelerator_ = std::move(accelerator); }
+    // The finest matrix was given without values: `source` holds them. The host then reads only
+    // the diagonal blocks and the block norms unless a phase falls back to the host.
+    void hold_values_elsewhere(std::shared_ptr<const ValueSource> source) { source_ = std::move(source); }
     // Up to `budget` block rows of work; true once finished. Throws Error(SF_ERR_MG_*).
     bool step(size_t budget, const Cancel &cancel);
     bool finished() const { return phase_ == Phase::Done; }
@@ -73,8 +95,12 @@ class MultigridSetup {
     void finish_level();
     void push_final_level();
     void spectral_iteration();
+    // Brings the current level's values to the host when they are held elsewhere.
+    void ensure_values();
     // Ends the accelerator's part in the current level, if it has one.
     void release_accelerator_level();
+    // SOLIDFEM_SETUP_LOG: wall time per level and phase, printed when the setup finishes.
+    void charge_phase(int level);
 
     SetupOptions options_;
     double reach_ = 3.0;
@@ -102,11 +128,18 @@ class MultigridSetup {
     std::vector<std::pair<u32, int64_t>> pcol_entries_;
     size_t cursor_ = 0;
     std::unique_ptr<SetupAccelerator> accelerator_;
+    std::shared_ptr<const ValueSource> source_;  // the current level's values, when not on the host
+    std::vector<double> diagonal_blocks_;        // fetched from source_ for the block inverse
+    std::vector<double> block_norms_;            // fetched from source_ for the join
     bool on_device_ = false;  // this level's numeric phases run on the accelerator
     bool device_p_ = false;   // the accelerator holds this level's prolongator
     bool device_ap_ = false;  // and its A P
     bool is_coarsest_ = false;
     Phase phase_ = Phase::Diagonal;
+    Phase timed_phase_ = Phase::Diagonal;
+    int timed_level_ = 0;
+    std::chrono::steady_clock::time_point timed_since_;
+    std::vector<std::array<double, 14>> phase_seconds_;
     bool cancelled_ = false;
     bool failed_ = false;
     Error failure_{SF_OK, ""};

@@ -145,6 +145,25 @@ __global__ void divide_kernel(size_t n, const double *__restrict__ y, double nor
     if (i < n) x[i] = y[i] / norm;
 }
 
+// The blocks at `slots`, in that order.
+__global__ void gather_blocks_kernel(size_t count, int len, const int64_t *__restrict__ slots, const double *__restrict__ values,
+                                     double *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count * (size_t)len) return;
+    const size_t b = i / (size_t)len, k = i % (size_t)len;
+    out[i] = values[(size_t)slots[b] * (size_t)len + k];
+}
+
+// Per block, the sum of its squared values in storage order from zero (the host's order).
+__global__ void block_squares_kernel(size_t blocks, int len, const double *__restrict__ values, double *__restrict__ out) {
+    const size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= blocks) return;
+    const double *v = values + b * (size_t)len;
+    double s = 0.0;
+    for (int k = 0; k < len; ++k) s += v[k] * v[k];
+    out[b] = s;
+}
+
 struct SetupRows {
     const int64_t *starts;
     const u32 *cols;
@@ -587,6 +606,40 @@ void DeviceSetup::end_level() {
     y_.release();
     host_y_.clear();
     host_y_.shrink_to_fit();
+}
+
+DeviceValues::DeviceValues(std::shared_ptr<Context> ctx, const double *values, size_t blocks, size_t block_len)
+    : ctx_(std::move(ctx)), values_(values), blocks_(blocks), len_(block_len) {}
+
+void DeviceValues::values(double *out) const {
+    hipStream_t s = ctx_->stream;
+    if (blocks_) SF_HIP(hipMemcpyAsync(out, values_, blocks_ * len_ * sizeof(double), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
+}
+
+void DeviceValues::blocks(const std::vector<int64_t> &slots, double *out) const {
+    if (slots.empty()) return;
+    for (int64_t at : slots)
+        if (at < 0 || (size_t)at >= blocks_) throw Error(SF_ERR_INTERNAL, "a block slot outside the matrix");
+    hipStream_t s = ctx_->stream;
+    DeviceBuffer<int64_t> d_slots;
+    d_slots.upload(slots, s);
+    const size_t n = slots.size() * len_;
+    DeviceBuffer<double> d_out(n);
+    gather_blocks_kernel<<<grid_of(n, 256), 256, 0, s>>>(slots.size(), (int)len_, d_slots.ptr, values_, d_out.ptr);
+    SF_HIP(hipGetLastError());
+    d_out.download(out, n, s);
+    SF_HIP(hipStreamSynchronize(s));
+}
+
+void DeviceValues::block_squares(double *out) const {
+    if (!blocks_) return;
+    hipStream_t s = ctx_->stream;
+    DeviceBuffer<double> d_out(blocks_);
+    block_squares_kernel<<<grid_of(blocks_, 256), 256, 0, s>>>(blocks_, (int)len_, values_, d_out.ptr);
+    SF_HIP(hipGetLastError());
+    d_out.download(out, blocks_, s);
+    SF_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace sf

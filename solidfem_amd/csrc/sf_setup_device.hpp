@@ -44,4 +44,19 @@ class DeviceSetup final : public SetupAccelerator {
     std::vector<double> host_y_;
 };
 
+// A study's assembled values on the device as the finest level's value source. The study owns
+// the values and outlives the hierarchy that holds this source.
+class DeviceValues final : public ValueSource {
+   public:
+    DeviceValues(std::shared_ptr<Context> ctx, const double *values, size_t blocks, size_t block_len);
+    void values(double *out) const override;
+    void blocks(const std::vector<int64_t> &slots, double *out) const override;
+    void block_squares(double *out) const override;
+
+   private:
+    std::shared_ptr<Context> ctx_;
+    const double *values_;
+    size_t blocks_, len_;
+};
+
 }  // namespace sf
