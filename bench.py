@@ -214,7 +214,7 @@ def serving(sf, np, count, rate, seed):
     return {
         "workload": f"{count} requests, Poisson arrivals at {rate}/s (seed {seed}); bars, tubes and plates of Tet10, "
                     f"log-uniform 5k-600k DOFs, E U[1,210] GPa, nu U[0.25,0.4], random clamp and loaded faces, "
-                    f"random tractions N(0,1 MPa); 30% reuse an earlier system; HBM budget 200 GB; FP32 V-cycle, rtol 1e-8",
+                    f"random tractions N(0,1 MPa); 30% reuse an earlier system; HBM budget 200 GiB; FP32 V-cycle, rtol 1e-8",
         "arrival_rate_per_s": rate,
         "p50_ms": round(st["p50_seconds"] * 1e3, 1),
         "p99_ms": round(st["p99_seconds"] * 1e3, 1),
@@ -225,7 +225,7 @@ def serving(sf, np, count, rate, seed):
         "builds": int(st["builds"]),
         "cache_hits": int(st["cache_hits"]),
         "evictions": int(st["evictions"]),
-        "hbm_peak_reserved_gb": round(st["hbm_peak"] / 1e9, 2),
+        "hbm_peak_reserved_gib": round(st["hbm_peak"] / 2 ** 30, 2),
         "wall_s": round(wall, 1),
         "served_dofs_per_s": round(sum(dofs) / wall, 1),
         "median_request_dofs": int(np.median(dofs)) if dofs else 0,

@@ -122,45 +122,54 @@ SlicedPlan plan_sliced(const int64_t *starts, const u32 *columns, size_t rows, i
     const size_t npw = (size_t)p.npw;
     p.slices = (uint32_t)((rows + npw - 1) / npw);
     // Rows sorted by block count (descending, stable) inside windows of 32 slices: a slice's
-    // lanes then run loops of nearly equal length while rows stay near their neighbours.
+    // lanes then run loops of nearly equal length while rows stay near their neighbours. Windows
+    // and slices are independent, so they are laid out on the host pool.
     std::vector<uint32_t> order(rows);
-    std::iota(order.begin(), order.end(), 0u);
     const size_t window = 32 * npw;
-    for (size_t w0 = 0; w0 < rows; w0 += window) {
-        const size_t w1 = std::min(rows, w0 + window);
-        std::stable_sort(order.begin() + w0, order.begin() + w1, [&](uint32_t a, uint32_t b) {
-            return (starts[a + 1] - starts[a]) > (starts[b + 1] - starts[b]);
-        });
-    }
+    const size_t windows = (rows + window - 1) / window;
+    parallel_for(windows, 8, [&](size_t first, size_t last) {
+        for (size_t w = first; w < last; ++w) {
+            const size_t w0 = w * window, w1 = std::min(rows, w0 + window);
+            std::iota(order.begin() + w0, order.begin() + w1, (uint32_t)w0);
+            std::stable_sort(order.begin() + w0, order.begin() + w1, [&](uint32_t a, uint32_t b) {
+                return (starts[a + 1] - starts[a]) > (starts[b + 1] - starts[b]);
+            });
+        }
+    });
     p.perm.assign((size_t)p.slices * npw, NONE);
     p.row_len.assign((size_t)p.slices * npw, 0);
     p.slice_start.assign((size_t)p.slices + 1, 0);
-    for (uint32_t s = 0; s < p.slices; ++s) {
-        uint32_t width = 0;
-        for (size_t j = 0; j < npw; ++j) {
-            const size_t at = (size_t)s * npw + j;
-            if (at >= rows) break;
-            const uint32_t row = order[at];
-            p.perm[at] = row;
-            p.row_len[at] = (uint32_t)(starts[row + 1] - starts[row]);
-            width = std::max(width, p.row_len[at]);
+    parallel_for(p.slices, 1024, [&](size_t first, size_t last) {
+        for (size_t s = first; s < last; ++s) {
+            uint32_t width = 0;
+            for (size_t j = 0; j < npw; ++j) {
+                const size_t at = s * npw + j;
+                if (at >= rows) break;
+                const uint32_t row = order[at];
+                p.perm[at] = row;
+                p.row_len[at] = (uint32_t)(starts[row + 1] - starts[row]);
+                width = std::max(width, p.row_len[at]);
+            }
+            p.slice_start[s + 1] = width;
         }
-        p.slice_start[s + 1] = p.slice_start[s] + width;
-    }
+    });
+    for (uint32_t s = 0; s < p.slices; ++s) p.slice_start[s + 1] += p.slice_start[s];
     const size_t slots = p.slice_start[p.slices];
     p.cols.assign(slots * npw, 0);
     p.block_of_slot.assign(slots * npw, NONE);
-    for (uint32_t s = 0; s < p.slices; ++s)
-        for (size_t j = 0; j < npw; ++j) {
-            const size_t at = (size_t)s * npw + j;
-            if (p.perm[at] == NONE) continue;
-            const uint32_t row = p.perm[at];
-            for (uint32_t k = 0; k < p.row_len[at]; ++k) {
-                const size_t slot = (size_t)(p.slice_start[s] + k) * npw + j;
-                p.cols[slot] = columns[(size_t)starts[row] + k];
-                p.block_of_slot[slot] = (uint32_t)(starts[row] + k);
+    parallel_for(p.slices, 1024, [&](size_t first, size_t last) {
+        for (size_t s = first; s < last; ++s)
+            for (size_t j = 0; j < npw; ++j) {
+                const size_t at = s * npw + j;
+                if (p.perm[at] == NONE) continue;
+                const uint32_t row = p.perm[at];
+                for (uint32_t k = 0; k < p.row_len[at]; ++k) {
+                    const size_t slot = (size_t)(p.slice_start[s] + k) * npw + j;
+                    p.cols[slot] = columns[(size_t)starts[row] + k];
+                    p.block_of_slot[slot] = (uint32_t)(starts[row] + k);
+                }
             }
-        }
+    });
     return p;
 }
 
@@ -267,7 +276,9 @@ void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s) {
     out.rows = (uint32_t)m.rows;
     std::vector<uint32_t> starts(m.starts.begin(), m.starts.end());
     std::vector<T> vals(m.values.size());
-    for (size_t i = 0; i < vals.size(); ++i) vals[i] = (T)m.values[i];
+    parallel_for(vals.size(), 65536, [&](size_t first, size_t last) {
+        for (size_t i = first; i < last; ++i) vals[i] = (T)m.values[i];
+    });
     out.starts.upload(starts, s);
     out.cols.upload(m.columns.empty() ? std::vector<uint32_t>{0} : m.columns, s);
     out.vals.upload(vals.empty() ? std::vector<T>{T(0)} : vals, s);
@@ -391,8 +402,8 @@ struct LevelDev {
     int restrict_rows = 0;  // 0: the sliced restriction; else long_rows_kernel with this many rows per wave (by timing)
     DeviceBsr<S> a_rows;          // coarse levels: A in plain block rows, for the row kernels
     int cheb_rows = 0, residual_rows = 0;  // the same choice for the steps and residuals (by timing)
-    bool fuse_restrict_start = true;       // the coarser level's first start in the restriction (by timing)
-    bool fuse_post_start = true;           // the post-smoothing's start in the second residual (by timing)
+    bool fuse_restrict_start = true;       // the coarser level's first start in the restriction
+    bool fuse_post_start = true;           // the post-smoothing's start in the second residual
     DeviceBuffer<S> dinv;
     DeviceBuffer<double> lt;      // coarsest: transposed Cholesky factor
     DeviceBuffer<double> packed;  // coarsest: the factor packed by columns, for the LDS solve
@@ -454,17 +465,21 @@ class Hierarchy final : public DeviceMultigrid {
                 t.starts.assign(t.rows + 1, 0);
                 for (u32 c : P.columns) t.starts[(size_t)c + 1] += 1;
                 for (size_t i = 0; i < t.rows; ++i) t.starts[i + 1] += t.starts[i];
-                std::vector<int64_t> fill(t.starts.begin(), t.starts.end() - 1);
+                std::vector<int64_t> fill(t.starts.begin(), t.starts.end() - 1), to(P.columns.size());
                 t.columns.assign(P.columns.size(), 0);
-                t.values.assign(P.values.size(), 0.0);
-                const size_t bl = P.rb * P.cb;
                 for (size_t row = 0; row < P.rows; ++row)
                     for (int64_t at = P.starts[row]; at < P.starts[row + 1]; ++at) {
-                        const size_t to = (size_t)fill[P.columns[(size_t)at]]++;
-                        t.columns[to] = (u32)row;
-                        for (size_t pp = 0; pp < P.rb; ++pp)
-                            for (size_t q = 0; q < P.cb; ++q) t.values[to * bl + q * P.rb + pp] = P.values[(size_t)at * bl + pp * P.cb + q];
+                        to[(size_t)at] = fill[P.columns[(size_t)at]]++;
+                        t.columns[(size_t)to[(size_t)at]] = (u32)row;
                     }
+                t.values.resize(P.values.size());
+                const size_t bl = P.rb * P.cb;
+                parallel_for(P.columns.size(), 16384, [&](size_t first, size_t last) {
+                    for (size_t at = first; at < last; ++at)
+                        for (size_t pp = 0; pp < P.rb; ++pp)
+                            for (size_t q = 0; q < P.cb; ++q)
+                                t.values[(size_t)to[at] * bl + q * P.rb + pp] = P.values[at * bl + pp * P.cb + q];
+                });
                 clock.lap("  restriction transpose");
                 upload_sliced(d.r, t, s);
                 clock.lap("  restriction layout");
@@ -527,8 +542,6 @@ class Hierarchy final : public DeviceMultigrid {
         SF_HIP(hipStreamSynchronize(s));
         tune(s);
         clock.lap("hierarchy kernel tuning");
-        tune_fusions(s);
-        clock.lap("hierarchy fusion tuning");
         compute_stats();
     }
 
@@ -843,67 +856,9 @@ class Hierarchy final : public DeviceMultigrid {
                 }, {0, 1, 2, 4}, at + "restriction rows per wave");
                 d.p.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.p, c.x.ptr, d.px.ptr, nullptr, EPI_ADD, s, batch); },
                                             {1, 2, 4, 8}, at + "prolongation depth");
-                // each smoothing's start fused into the kernel before it, or launched on its own
-                if (smooths_first(c))
-                    d.fuse_restrict_start = fastest(s, [&](int fuse) {
-                        restrict_to(d, c, s, fuse != 0);
-                        if (!fuse) launch_start(c, c.b.ptr, c.x.ptr, s);
-                    }, {1, 0}, at + "restriction with the coarse start") == 1;
-                d.fuse_post_start = fastest(s, [&](int fuse) {
-                    if (fuse) residual(d, d.x.ptr, d.b.ptr, s, start_of(d, d.px.ptr));
-                    else {
-                        residual(d, d.x.ptr, d.b.ptr, s);
-                        launch_start(d, d.res.ptr, d.px.ptr, s);
-                    }
-                }, {1, 0}, at + "residual with the post-smoothing start") == 1;
             }
         }
         SF_HIP(hipStreamSynchronize(s));
-    }
-
-    // The start fusions change which kernels run back to back, and what a kernel costs depends on
-    // what the caches hold when it runs, which isolated repeated launches do not reproduce. So each
-    // fusion is kept, level by level, only if whole preconditioner applications run faster with
-    // it. Every choice gives the same bits.
-    void tune_fusions(hipStream_t s) {
-        if (lv_.empty()) return;
-        static const bool verbose = std::getenv("SOLIDFEM_TUNE_LOG") != nullptr;
-        const size_t n = lv_[0].n;
-        DeviceBuffer<double> r(n), z(n);
-        SF_HIP(hipMemsetAsync(r.ptr, 0, n * sizeof(double), s));
-        hipEvent_t e0, e1;
-        SF_HIP(hipEventCreate(&e0));
-        SF_HIP(hipEventCreate(&e1));
-        auto time_applies = [&]() {
-            launch_apply(r.ptr, z.ptr, s);
-            SF_HIP(hipEventRecord(e0, s));
-            for (int k = 0; k < 8; ++k) launch_apply(r.ptr, z.ptr, s);
-            SF_HIP(hipEventRecord(e1, s));
-            SF_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            SF_HIP(hipEventElapsedTime(&ms, e0, e1));
-            return ms * 1e3f / 8;
-        };
-        float best = time_applies();
-        auto flip = [&](bool &flag, const std::string &what) {
-            flag = !flag;
-            const float t = time_applies();
-            if (verbose) std::fprintf(stderr, "[tune] %s %s: %.2f us per cycle against %.2f\n", what.c_str(), flag ? "fused" : "apart", t, best);
-            if (t < best) best = t;
-            else flag = !flag;
-        };
-        const char *prec = sizeof(V) == 4 ? " f32 " : " f64 ";
-        if (!std::is_same<V, double>::value && lv_[0].has_p) flip(fuse_convert_start_, std::string("L0") + prec + "conversion with the first start");
-        for (size_t l = 0; l < lv_.size(); ++l) {
-            LevelDev<S, V> &d = lv_[l];
-            if (!d.has_p) continue;
-            const std::string at = "L" + std::to_string(l) + prec;
-            if (smooths_first(lv_[l + 1])) flip(d.fuse_restrict_start, at + "restriction with the coarse start");
-            flip(d.fuse_post_start, at + "residual with the post-smoothing start");
-        }
-        SF_HIP(hipStreamSynchronize(s));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
     }
 
     // The Chebyshev start z = D^-1 b / theta, d = z, x = z (cycle.rs:578-590).
@@ -1058,7 +1013,7 @@ class Hierarchy final : public DeviceMultigrid {
     std::vector<Graph> graphs_;
     std::vector<LevelDev<S, V>> lv_;
     DeviceBuffer<V> in_, out_;
-    bool fuse_convert_start_ = true;  // single precision: the first start in the rounding of the residual (by timing)
+    bool fuse_convert_start_ = true;  // single precision: the first start in the rounding of the residual
     DeviceBuffer<V> min_, mout_;  // the finest level's columns in V, for the single precision
 };
 

@@ -54,7 +54,9 @@ struct QuadMesh {
     std::vector<std::array<u32, 10>> tets;
     std::vector<std::array<u32, 4>> corners;
     size_t vertex_count = 0;
-    std::vector<Face> boundary_faces() const;  // quadratic.rs:89-124 order
+    std::vector<Face> boundary;  // faces met once, in quadratic.rs:89-124 order
+    std::vector<std::array<u32, 2>> shared_faces;  // tetrahedra meeting at a corner triangle
+    const std::vector<Face> &boundary_faces() const { return boundary; }
 };
 QuadMesh upgrade_to_quadratic(const TetMesh &mesh);
 // The elements grouped into bodies joined through shared faces (quadratic.rs:129-166).

@@ -146,6 +146,25 @@ static std::array<u32, 3> sorted_corners(const std::array<u32, 4> &t, int face) 
     return k;
 }
 
+// Sorts `items` by `less` where `lead(item)`, in [0, range), orders items first: a counting sort
+// by lead value, then each bucket sorted on the host pool. Wherever items that compare equal are
+// identical, the result is std::sort's.
+template <typename T, typename Lead, typename Less>
+static void lead_sort(std::vector<T> &items, size_t range, Lead lead, Less less) {
+    std::vector<size_t> start(range + 1, 0);
+    for (const T &x : items) start[(size_t)lead(x) + 1] += 1;
+    for (size_t i = 0; i < range; ++i) start[i + 1] += start[i];
+    std::vector<T> out(items.size());
+    {
+        std::vector<size_t> fill(start.begin(), start.end() - 1);
+        for (const T &x : items) out[fill[(size_t)lead(x)]++] = x;
+    }
+    parallel_for(range, 4096, [&](size_t first, size_t last) {
+        for (size_t r = first; r < last; ++r) std::sort(out.begin() + (int64_t)start[r], out.begin() + (int64_t)start[r + 1], less);
+    });
+    items.swap(out);
+}
+
 void validate_tet_mesh(const TetMesh &m) {
     // crates/solidfem-mesh/src/tet.rs:59-109
     if (m.tets.empty()) throw Error(SF_ERR_MESH_EMPTY, "the mesh has no tetrahedra");
@@ -176,11 +195,13 @@ void validate_tet_mesh(const TetMesh &m) {
     }
     for (size_t n = 0; n < used.size(); ++n)
         if (!used[n]) throw Error(SF_ERR_MESH_UNUSED_NODE, "node belongs to no tetrahedron", (int64_t)n);
-    std::vector<std::array<u32, 3>> faces;
-    faces.reserve(4 * m.tets.size());
-    for (const auto &t : m.tets)
-        for (int f = 0; f < 4; ++f) faces.push_back(sorted_corners(t, f));
-    std::sort(faces.begin(), faces.end());
+    std::vector<std::array<u32, 3>> faces(4 * m.tets.size());
+    parallel_for(m.tets.size(), 8192, [&](size_t first, size_t last) {
+        for (size_t t = first; t < last; ++t)
+            for (int f = 0; f < 4; ++f) faces[4 * t + f] = sorted_corners(m.tets[t], f);
+    });
+    lead_sort(faces, m.positions.size(), [](const std::array<u32, 3> &k) { return k[0]; },
+              [](const std::array<u32, 3> &a, const std::array<u32, 3> &b) { return a < b; });
     for (size_t i = 2; i < faces.size(); ++i)
         if (faces[i] == faces[i - 2]) throw Error(SF_ERR_MESH_NON_MANIFOLD_FACE, "a face is shared by more than two tetrahedra");
 }
@@ -220,6 +241,8 @@ TetMesh make_box_mesh(const double origin[3], const double size[3], const int32_
     return m;
 }
 
+static void find_faces(QuadMesh &mesh);
+
 QuadMesh upgrade_to_quadratic(const TetMesh &mesh) {
     // crates/solidfem-mesh/src/quadratic.rs:24-71: one midpoint per distinct edge, numbered
     // after the corners in ascending (min, max) corner order. Edges are packed into u64 keys.
@@ -234,7 +257,7 @@ QuadMesh upgrade_to_quadratic(const TetMesh &mesh) {
             }
     });
     std::vector<u64> unique = edges;
-    std::sort(unique.begin(), unique.end());
+    lead_sort(unique, mesh.positions.size(), [](u64 e) { return e >> 32; }, [](u64 a, u64 b) { return a < b; });
     unique.erase(std::unique(unique.begin(), unique.end()), unique.end());
     q.nodes.resize(q.vertex_count + unique.size());
     std::copy(mesh.positions.begin(), mesh.positions.end(), q.nodes.begin());
@@ -255,12 +278,13 @@ QuadMesh upgrade_to_quadratic(const TetMesh &mesh) {
         }
     });
     q.corners = mesh.tets;
+    find_faces(q);
     return q;
 }
 
-std::vector<Face> QuadMesh::boundary_faces() const {
+static void find_faces(QuadMesh &mesh) {
     // crates/solidfem-mesh/src/quadratic.rs:89-124: faces met once, ascending by sorted corners,
-    // then by (tet, face).
+    // then by (tet, face); and the pairs of tetrahedra that meet at a face (quadratic.rs:129-166).
     struct Key {
         std::array<u32, 3> k;
         u32 tet, face;
@@ -270,11 +294,15 @@ std::vector<Face> QuadMesh::boundary_faces() const {
             return face < o.face;
         }
     };
-    std::vector<Key> keys(corners.size() * 4);
-    for (size_t t = 0; t < corners.size(); ++t)
-        for (int f = 0; f < 4; ++f) keys[4 * t + f] = Key{sorted_corners(corners[t], f), (u32)t, (u32)f};
-    std::sort(keys.begin(), keys.end());
-    std::vector<Face> out;
+    std::vector<Key> keys(mesh.corners.size() * 4);
+    parallel_for(mesh.corners.size(), 8192, [&](size_t first, size_t last) {
+        for (size_t t = first; t < last; ++t)
+            for (int f = 0; f < 4; ++f) keys[4 * t + f] = Key{sorted_corners(mesh.corners[t], f), (u32)t, (u32)f};
+    });
+    lead_sort(keys, mesh.nodes.size(), [](const Key &k) { return k.k[0]; }, [](const Key &a, const Key &b) { return a < b; });
+    std::vector<Face> &out = mesh.boundary;
+    out.clear();
+    mesh.shared_faces.clear();
     for (size_t at = 0; at < keys.size();) {
         size_t end = at + 1;
         while (end < keys.size() && keys[end].k == keys[at].k) ++end;
@@ -282,12 +310,12 @@ std::vector<Face> QuadMesh::boundary_faces() const {
             Face f;
             f.tet = keys[at].tet;
             f.face = keys[at].face;
-            for (int s = 0; s < 6; ++s) f.nodes[s] = tets[f.tet][FACE_LOCAL[f.face][s]];
+            for (int s = 0; s < 6; ++s) f.nodes[s] = mesh.tets[f.tet][FACE_LOCAL[f.face][s]];
             out.push_back(f);
         }
+        for (size_t k = at + 1; k < end; ++k) mesh.shared_faces.push_back({keys[k - 1].tet, keys[k].tet});
         at = end;
     }
-    return out;
 }
 
 // ---- workload inputs ------------------------------------------------------------------------
@@ -408,7 +436,9 @@ TetMesh make_tube_mesh(const sf_workload &w) {
 Model model_from_workload(const sf_workload &w) {
     if (w.wall_cells < 0) throw Error(SF_ERR_INVALID_ARGUMENT, "wall_cells must not be negative");
     if (w.wall_cells > 0 && w.jitter_fraction > 0.0) throw Error(SF_ERR_INVALID_ARGUMENT, "a tube is not jittered");
+    StageClock clock;
     TetMesh linear = w.wall_cells > 0 ? make_tube_mesh(w) : make_box_mesh(w.origin, w.size, w.divisions);
+    clock.lap("linear mesh");
     if (w.jitter_fraction > 0.0) {
         // Interior vertices move by (2u - 1) * (fraction * h) per axis, u from (seed ^ (3 v + axis)).
         const size_t nx = (size_t)w.divisions[0] + 1, ny = (size_t)w.divisions[1] + 1;
@@ -425,13 +455,19 @@ Model model_from_workload(const sf_workload &w) {
             }
         }
     }
+    clock.lap("jitter");
     validate_tet_mesh(linear);
+    clock.lap("mesh validation");
     Model model;
     model.mesh = upgrade_to_quadratic(linear);
+    clock.lap("quadratic upgrade");
     const auto &nodes = model.mesh.nodes;
     for (size_t n = 0; n < nodes.size(); ++n)
         if (std::fabs(nodes[n][w.clamp_axis] - w.clamp_value) < 1e-9) model.fixed.insert((u32)n);
-    model.tractions = workload_tractions(model.mesh, model.mesh.boundary_faces(), w);
+    const auto boundary = model.mesh.boundary_faces();
+    clock.lap("boundary faces");
+    model.tractions = workload_tractions(model.mesh, boundary, w);
+    clock.lap("tractions");
     model.field.kind = w.material_kind;
     model.field.e = w.youngs;
     model.field.nu = w.poisson;
@@ -483,8 +519,8 @@ static void face_load(const QuadMesh &mesh, const Face &face, const std::array<d
 std::vector<std::vector<u32>> face_connected_bodies(const QuadMesh &mesh) {
     // quadratic.rs:129-166: elements sharing a corner triangle join; bodies come out ordered by
     // their lowest element, each body's elements ascending. Any union order gives this partition,
-    // so the faces are matched by bucketing them on their lowest corner, then sorting each bucket.
-    const size_t n = mesh.corners.size(), vertices = mesh.vertex_count;
+    // so the pairs the face matching of the mesh found are joined in the order found.
+    const size_t n = mesh.corners.size();
     std::vector<u32> parent(n);
     std::iota(parent.begin(), parent.end(), 0u);
     auto find = [&](u32 x) {
@@ -494,38 +530,9 @@ std::vector<std::vector<u32>> face_connected_bodies(const QuadMesh &mesh) {
         }
         return x;
     };
-    struct Key {
-        u32 b, c, tet;
-    };
-    std::vector<u32> bucket_start(vertices + 2, 0);
-    auto face_of = [&](size_t t, int skip, u32 v[3]) {
-        u32 k = 0;
-        for (int i = 0; i < 4; ++i)
-            if (i != skip) v[k++] = mesh.corners[t][(size_t)i];
-        std::sort(v, v + 3);
-    };
-    for (size_t t = 0; t < n; ++t)
-        for (int skip = 0; skip < 4; ++skip) {
-            u32 v[3];
-            face_of(t, skip, v);
-            bucket_start[(size_t)v[0] + 2] += 1;
-        }
-    for (size_t i = 2; i < bucket_start.size(); ++i) bucket_start[i] += bucket_start[i - 1];
-    std::vector<Key> keys(4 * n);
-    for (size_t t = 0; t < n; ++t)
-        for (int skip = 0; skip < 4; ++skip) {
-            u32 v[3];
-            face_of(t, skip, v);
-            keys[bucket_start[(size_t)v[0] + 1]++] = {v[1], v[2], (u32)t};
-        }
-    for (size_t a = 0; a < vertices; ++a) {
-        Key *first = keys.data() + bucket_start[a], *last = keys.data() + bucket_start[a + 1];
-        std::sort(first, last, [](const Key &x, const Key &y) { return x.b != y.b ? x.b < y.b : x.c < y.c; });
-        for (Key *k = first + 1; k < last; ++k) {
-            if (k->b != k[-1].b || k->c != k[-1].c) continue;
-            const u32 r0 = find(k[-1].tet), r1 = find(k->tet);
-            if (r0 != r1) parent[std::max(r0, r1)] = std::min(r0, r1);
-        }
+    for (const auto &pair : mesh.shared_faces) {
+        const u32 r0 = find(pair[0]), r1 = find(pair[1]);
+        if (r0 != r1) parent[std::max(r0, r1)] = std::min(r0, r1);
     }
     std::vector<std::vector<u32>> groups;
     std::vector<u32> group_of(n, FIXED);
@@ -705,10 +712,12 @@ FreeProblem number_and_load(const Model &model) {
         if (node >= nn) throw Error(SF_ERR_MODEL_NODE_OUT_OF_RANGE, "fixed node out of range", node);
         is_fixed[node] = 1;
     }
+    StageClock clock;
     if (model.prescribed.empty())
         check_constraints(model.mesh, is_fixed);
     else
         check_held_dofs(model.mesh, is_fixed, model.prescribed);
+    clock.lap("  held-body validation");
     // crates/solidfem-elastic/src/study.rs:696-735: nodal forces, then tractions, in the order given
     std::vector<double> load(3 * nn, 0.0);
     for (const auto &f : model.nodal_forces) {
@@ -728,6 +737,7 @@ FreeProblem number_and_load(const Model &model) {
             face_load(model.mesh, t.first, t.second, load);
         }
     }
+    clock.lap("  loads");
     FreeProblem p;
     p.free.assign(nn, FIXED);
     for (size_t n = 0; n < nn; ++n)
@@ -750,6 +760,7 @@ FreeProblem number_and_load(const Model &model) {
     for (double r : p.rhs) any = any || r != 0.0;
     for (auto &pr : p.prescribed) any = any || pr.second != 0.0;
     if (!any) throw Error(SF_ERR_MODEL_NO_EFFECTIVE_LOAD, "the model has no effective load");
+    clock.lap("  numbering");
     return p;
 }
 
