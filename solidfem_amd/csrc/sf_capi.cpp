@@ -135,7 +135,7 @@ void level_matrix_view(const std::vector<HostLevel> &h, bool single, uint64_t le
                        uint32_t *columns, double *values) {
     if (level >= h.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
     if (prolongator && !h[level].has_prolongator) throw Error(SF_ERR_INVALID_ARGUMENT, "the coarsest level has no prolongator");
-    if (values && !prolongator) h[level].fetch_values();
+    if (values) h[level].fetch_values();
     const HostBsr<double> &m = prolongator ? h[level].prolongator : h[level].a;
     if (rows) *rows = m.rows;
     if (block_columns) *block_columns = m.cols;
@@ -1003,7 +1003,10 @@ sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision,
         // the copy holds every value itself: it may outlive the study whose system holds the finest
         for (const auto &h : mg->mg->host) h.fetch_values();
         std::vector<HostLevel> levels = mg->mg->host;
-        for (auto &h : levels) h.source.reset();
+        for (auto &h : levels) {
+            h.source.reset();
+            h.prolongator_source.reset();
+        }
         m->mg = make_device_multigrid(std::move(levels), mg->mg->degree, precision, coarse_solve < 0 ? mg->mg->coarse_mode : coarse_solve,
                                       mg->ctx->stream);
         *out = m.release();

@@ -283,6 +283,34 @@ void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s) {
     out.cols.upload(m.columns.empty() ? std::vector<uint32_t>{0} : m.columns, s);
     out.vals.upload(vals.empty() ? std::vector<T>{T(0)} : vals, s);
 }
+// Block k of `out` (cb x rb, row-major) is the transpose of block from[k] of `in` (rb x cb).
+static __global__ void transpose_blocks_kernel(size_t blocks, int rb, int cb, const uint32_t *__restrict__ from,
+                                        const double *__restrict__ in, double *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t len = (size_t)rb * (size_t)cb;
+    if (i >= blocks * len) return;
+    const size_t k = i / len, e = i % len;
+    const size_t q = e / (size_t)rb, p = e % (size_t)rb;
+    out[i] = in[(size_t)from[k] * len + p * (size_t)cb + q];
+}
+
+template <typename T>
+void build_bsr(DeviceBsr<T> &out, const HostBsr<double> &pattern, const double *values, hipStream_t s) {
+    if (pattern.starts.back() > (int64_t)UINT32_MAX) throw Error(SF_ERR_INTERNAL, "a block matrix too large for 32-bit offsets");
+    out.rb = (int)pattern.rb;
+    out.cb = (int)pattern.cb;
+    out.rows = (uint32_t)pattern.rows;
+    std::vector<uint32_t> starts(pattern.starts.begin(), pattern.starts.end());
+    out.starts.upload(starts, s);
+    out.cols.upload(pattern.columns.empty() ? std::vector<uint32_t>{0} : pattern.columns, s);
+    const size_t n = pattern.blocks() * pattern.block_len();
+    out.vals.resize(std::max<size_t>(n, 1));
+    if (n) hipLaunchKernelGGL((convert_kernel<T, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, values, out.vals.ptr);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipStreamSynchronize(s));
+}
+template void build_bsr<double>(DeviceBsr<double> &, const HostBsr<double> &, const double *, hipStream_t);
+template void build_bsr<float>(DeviceBsr<float> &, const HostBsr<double> &, const double *, hipStream_t);
 template void upload_bsr<double>(DeviceBsr<double> &, const HostBsr<double> &, hipStream_t);
 template void upload_bsr<float>(DeviceBsr<float> &, const HostBsr<double> &, hipStream_t);
 
@@ -452,12 +480,22 @@ class Hierarchy final : public DeviceMultigrid {
             chebyshev(h.lambda_min, h.lambda_max, degree, d.theta, d.steps);
             if (h.has_prolongator) {
                 d.has_p = true;
-                upload_sliced(d.p, h.prolongator, s);
+                const auto &P = h.prolongator;
+                // P's values: on the device where the setup built them, else uploaded
+                const double *pdev = P.values.empty() && h.prolongator_source ? h.prolongator_source->device() : nullptr;
+                DeviceBuffer<double> uploaded;
+                if (!pdev) {
+                    h.fetch_values();
+                    uploaded.upload(P.values, s);
+                    pdev = uploaded.ptr;
+                }
+                build_sliced(d.p, plan_sliced(P.starts.data(), P.columns.data(), P.rows, (int)P.rb, (int)P.cb), P.cols, pdev, s);
                 clock.lap("  prolongator layout");
                 // the stored transpose: block row c holds block column c of P in ascending fine
-                // row order, each block transposed (crates/solidfem-solve/src/gpu/layout.rs:254-291)
+                // row order, each block transposed (crates/solidfem-solve/src/gpu/layout.rs:254-291);
+                // its pattern on the host, its values gathered on the device
+                if (P.blocks() > (size_t)UINT32_MAX) throw Error(SF_ERR_INTERNAL, "a prolongator too large for 32-bit block indices");
                 HostBsr<double> t;
-                const auto &P = h.prolongator;
                 t.rows = P.cols;
                 t.cols = P.rows;
                 t.rb = P.cb;
@@ -465,25 +503,30 @@ class Hierarchy final : public DeviceMultigrid {
                 t.starts.assign(t.rows + 1, 0);
                 for (u32 c : P.columns) t.starts[(size_t)c + 1] += 1;
                 for (size_t i = 0; i < t.rows; ++i) t.starts[i + 1] += t.starts[i];
-                std::vector<int64_t> fill(t.starts.begin(), t.starts.end() - 1), to(P.columns.size());
-                t.columns.assign(P.columns.size(), 0);
+                std::vector<int64_t> fill(t.starts.begin(), t.starts.end() - 1);
+                std::vector<uint32_t> from(P.blocks());
+                t.columns.assign(P.blocks(), 0);
                 for (size_t row = 0; row < P.rows; ++row)
                     for (int64_t at = P.starts[row]; at < P.starts[row + 1]; ++at) {
-                        to[(size_t)at] = fill[P.columns[(size_t)at]]++;
-                        t.columns[(size_t)to[(size_t)at]] = (u32)row;
+                        const size_t to = (size_t)fill[P.columns[(size_t)at]]++;
+                        t.columns[to] = (u32)row;
+                        from[to] = (uint32_t)at;
                     }
-                t.values.resize(P.values.size());
-                const size_t bl = P.rb * P.cb;
-                parallel_for(P.columns.size(), 16384, [&](size_t first, size_t last) {
-                    for (size_t at = first; at < last; ++at)
-                        for (size_t pp = 0; pp < P.rb; ++pp)
-                            for (size_t q = 0; q < P.cb; ++q)
-                                t.values[(size_t)to[at] * bl + q * P.rb + pp] = P.values[at * bl + pp * P.cb + q];
-                });
+                const size_t tn = P.blocks() * P.block_len();
+                DeviceBuffer<double> tvals(std::max<size_t>(tn, 1));
+                {
+                    DeviceBuffer<uint32_t> dfrom;
+                    dfrom.upload(from, s);
+                    if (tn)
+                        hipLaunchKernelGGL(transpose_blocks_kernel, dim3(grid_for(tn, 256)), dim3(256), 0, s, P.blocks(), (int)P.rb,
+                                           (int)P.cb, dfrom.ptr, pdev, tvals.ptr);
+                    SF_HIP(hipGetLastError());
+                    SF_HIP(hipStreamSynchronize(s));
+                }
                 clock.lap("  restriction transpose");
-                upload_sliced(d.r, t, s);
+                build_sliced(d.r, plan_sliced(t.starts.data(), t.columns.data(), t.rows, (int)t.rb, (int)t.cb), t.cols, tvals.ptr, s);
                 clock.lap("  restriction layout");
-                upload_bsr(d.r_rows, t, s);
+                build_bsr(d.r_rows, t, tvals.ptr, s);
                 clock.lap("  restriction rows");
             }
             if (h.has_factor) {
@@ -975,7 +1018,8 @@ class Hierarchy final : public DeviceMultigrid {
             memory += h.a.starts.size() * 8 + h.a.columns.size() * 4 + values * sizeof(S);
             memory += h.inverse_diagonal.size() * sizeof(S);
             if (h.has_prolongator)
-                memory += h.prolongator.starts.size() * 8 + h.prolongator.columns.size() * 4 + h.prolongator.values.size() * sizeof(S);
+                memory += h.prolongator.starts.size() * 8 + h.prolongator.columns.size() * 4 +
+                          h.prolongator.blocks() * h.prolongator.block_len() * sizeof(S);
             if (h.has_factor) memory += h.factor.size() * sizeof(double);
         }
         stats.levels = host.size();

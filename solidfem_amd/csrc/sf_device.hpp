@@ -60,6 +60,9 @@ struct DeviceBsr {
 };
 template <typename T>
 void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s);
+// The same from `pattern` with its values already on the device (f64, row-major blocks).
+template <typename T>
+void build_bsr(DeviceBsr<T> &out, const HostBsr<double> &pattern, const double *values, hipStream_t s);
 // y = A x (epilogue per dev::Epilogue) with `rows_per_wave` block rows per wavefront
 // (long_rows_kernel: 1, 2 or 4).
 template <typename S, typename V>

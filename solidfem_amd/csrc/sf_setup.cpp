@@ -604,7 +604,10 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                             throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
                         }
                         on_accelerator = device_p_ = accelerator_->smooth(aggregate_, aggregate_count_, ns, nc, tentative_values_, omega, prolongator_);
-                        if (on_accelerator) budget = budget > rows ? budget - rows : 0;
+                        if (on_accelerator) {
+                            p_source_ = accelerator_->prolongator_values();
+                            budget = budget > rows ? budget - rows : 0;
+                        }
                     }
                     if (!on_accelerator) ensure_values();
                     bool done = on_accelerator || run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
@@ -663,7 +666,10 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         on_accelerator = device_ap_ = accelerator_->product(aggregate_count_);
                         if (on_accelerator) budget = budget > rows ? budget - rows : 0;
                     }
-                    if (!on_accelerator) ensure_values();
+                    if (!on_accelerator) {
+                        ensure_values();
+                        ensure_prolongator_values();
+                    }
                     bool done = on_accelerator || run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         std::vector<RowOut> out(l - f);
                         parallel_for(l - f, 256, [&](size_t a, size_t b) {
@@ -716,6 +722,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         on_accelerator = accelerator_->galerkin((u32)count, pcol_starts_, pcol_entries_, coarse_);
                         if (on_accelerator) budget = budget > count ? budget - count : 0;
                     }
+                    if (!on_accelerator) ensure_prolongator_values();
                     bool done = on_accelerator || run_rows(count, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         std::vector<RowOut> out(l - f);
                         parallel_for(l - f, 64, [&](size_t a, size_t b) {
@@ -804,10 +811,21 @@ void MultigridSetup::ensure_values() {
     source_->values(current_.values.data());
 }
 
+void MultigridSetup::ensure_prolongator_values() {
+    if (!p_source_ || !prolongator_.values.empty() || prolongator_.blocks() == 0) return;
+    prolongator_.values.resize(prolongator_.blocks() * prolongator_.block_len());
+    p_source_->values(prolongator_.values.data());
+}
+
 void HostLevel::fetch_values() const {
-    if (!source || !a.values.empty() || a.blocks() == 0) return;
-    a.values.resize(a.blocks() * a.block_len());
-    source->values(a.values.data());
+    if (source && a.values.empty() && a.blocks() > 0) {
+        a.values.resize(a.blocks() * a.block_len());
+        source->values(a.values.data());
+    }
+    if (prolongator_source && prolongator.values.empty() && prolongator.blocks() > 0) {
+        prolongator.values.resize(prolongator.blocks() * prolongator.block_len());
+        prolongator_source->values(prolongator.values.data());
+    }
 }
 
 void MultigridSetup::push_final_level() {
@@ -823,6 +841,8 @@ void MultigridSetup::push_final_level() {
 }
 
 void MultigridSetup::finish_level() {
+    std::shared_ptr<const ValueSource> p_source = prolongator_.values.empty() ? p_source_ : nullptr;
+    p_source_.reset();
     release_accelerator_level();
     // setup.rs:650-695: a unit diagonal on coarse values no aggregate direction reaches
     const size_t dim = coarse_.rb;
@@ -843,6 +863,7 @@ void MultigridSetup::finish_level() {
     l.inverse_diagonal = std::move(inverse_diagonal_);
     l.has_prolongator = true;
     l.prolongator = std::move(prolongator_);
+    l.prolongator_source = std::move(p_source);
     l.aggregates = std::move(aggregate_);
     levels_.push_back(std::move(l));
     current_ = std::move(coarse_);

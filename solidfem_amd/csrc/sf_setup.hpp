@@ -23,18 +23,22 @@ class ValueSource {
     virtual void blocks(const std::vector<int64_t> &slots, double *out) const = 0;
     // Per stored block, the sum of its squared values in storage order, from zero.
     virtual void block_squares(double *out) const = 0;
+    // The values on the device, when they are there (on the device of the setup's context).
+    virtual const double *device() const { return nullptr; }
 };
 
 struct HostLevel {
     // The level's matrix. The finest level of a study leaves its values with the study's system
-    // on the device (`source`); fetch_values() brings them to the host on first use.
+    // on the device (`source`), and a prolongator built on the device keeps its values there
+    // (`prolongator_source`); fetch_values() brings both to the host on first use.
     mutable HostBsr<double> a;
     std::shared_ptr<const ValueSource> source;
     void fetch_values() const;
     std::vector<double> inverse_diagonal;
     double lambda_min = 0, lambda_max = 0;
     bool has_prolongator = false;
-    HostBsr<double> prolongator;  // fine rows x coarse blocks (3x6 or 6x6)
+    mutable HostBsr<double> prolongator;  // fine rows x coarse blocks (3x6 or 6x6)
+    std::shared_ptr<const ValueSource> prolongator_source;
     bool has_factor = false;
     std::vector<double> factor;  // n x n, lower triangle is the Cholesky factor
     std::vector<u32> aggregates;
@@ -66,6 +70,9 @@ class SetupAccelerator {
     virtual bool product(u32 count) = 0;
     virtual bool galerkin(u32 count, const std::vector<int64_t> &pcol_starts, const std::vector<std::pair<u32, int64_t>> &entries,
                           HostBsr<double> &coarse) = 0;
+    // The prolongator's values after a successful smooth(), which leaves only its pattern on the
+    // host; the source keeps them after end_level().
+    virtual std::shared_ptr<const ValueSource> prolongator_values() = 0;
     virtual void end_level() = 0;
 };
 
@@ -95,8 +102,10 @@ class MultigridSetup {
     void finish_level();
     void push_final_level();
     void spectral_iteration();
-    // Brings the current level's values to the host when they are held elsewhere.
+    // Brings the current level's values, or its prolongator's, to the host when they are held
+    // elsewhere.
     void ensure_values();
+    void ensure_prolongator_values();
     // Ends the accelerator's part in the current level, if it has one.
     void release_accelerator_level();
     // SOLIDFEM_SETUP_LOG: wall time per level and phase, printed when the setup finishes.
@@ -129,6 +138,7 @@ class MultigridSetup {
     size_t cursor_ = 0;
     std::unique_ptr<SetupAccelerator> accelerator_;
     std::shared_ptr<const ValueSource> source_;  // the current level's values, when not on the host
+    std::shared_ptr<const ValueSource> p_source_;  // its prolongator's values, when not on the host
     std::vector<double> diagonal_blocks_;        // fetched from source_ for the block inverse
     std::vector<double> block_norms_;            // fetched from source_ for the join
     bool on_device_ = false;  // this level's numeric phases run on the accelerator

@@ -398,7 +398,7 @@ void DeviceSetup::begin(int level, const HostBsr<double> &a, const std::vector<d
     }
     dinv_.upload(inverse_diagonal, s);
     p_cols_.release();
-    p_vals_.release();
+    p_vals_.reset();
     p_starts_.release();
     ap_starts_.release();
     ap_cols_.release();
@@ -484,14 +484,20 @@ bool DeviceSetup::smooth(const std::vector<u32> &aggregate, u32 count, const std
     const size_t blocks = (size_t)p.starts.back();
     p_starts_.upload(p.starts, s);
     p_cols_.resize(std::max<size_t>(blocks, 1));
-    p_vals_.resize(std::max<size_t>(blocks * rb_ * MODES6, 1));
-    launch(std::false_type{}, p_starts_.ptr, p_cols_.ptr, p_vals_.ptr);
+    p_vals_ = std::make_shared<DeviceBuffer<double>>(std::max<size_t>(blocks * rb_ * MODES6, 1));
+    p_blocks_ = blocks;
+    launch(std::false_type{}, p_starts_.ptr, p_cols_.ptr, p_vals_->ptr);
+    // the pattern comes to the host; the values stay here (prolongator_values())
     p.columns.resize(blocks);
-    p.values.resize(blocks * rb_ * MODES6);
+    p.values.clear();
     p_cols_.download(p.columns.data(), blocks, s);
-    p_vals_.download(p.values.data(), p.values.size(), s);
     SF_HIP(hipStreamSynchronize(s));
     return true;
+}
+
+std::shared_ptr<const ValueSource> DeviceSetup::prolongator_values() {
+    if (!p_vals_) throw Error(SF_ERR_INTERNAL, "no prolongator on the device");
+    return std::make_shared<DeviceValues>(ctx_, p_vals_, p_blocks_, (size_t)rb_ * MODES6);
 }
 
 bool DeviceSetup::product(u32 count) {
@@ -499,7 +505,7 @@ bool DeviceSetup::product(u32 count) {
     DeviceBuffer<u32> counts(rows_);
     clear_overflow();
     const SetupRows a{a_starts_.ptr, a_cols_.ptr, a_vals_ptr_};
-    const SetupRows p{p_starts_.ptr, p_cols_.ptr, p_vals_.ptr};
+    const SetupRows p{p_starts_.ptr, p_cols_.ptr, p_vals_->ptr};
     auto launch = [&](auto count_pass, const int64_t *out_starts, u32 *out_cols, double *out_vals) {
         constexpr bool C = decltype(count_pass)::value;
         if (rb_ == 3)
@@ -553,10 +559,10 @@ bool DeviceSetup::galerkin(u32 count, const std::vector<int64_t> &pcol_starts, c
         constexpr bool C = decltype(count_pass)::value;
         if (count == 0) return;
         if (rb_ == 3)
-            galerkin_kernel<3, C><<<count, 64, 0, s>>>(count, cstarts.ptr, crows.ptr, cat.ptr, p_vals_.ptr, ap, counts.ptr, out_starts,
+            galerkin_kernel<3, C><<<count, 64, 0, s>>>(count, cstarts.ptr, crows.ptr, cat.ptr, p_vals_->ptr, ap, counts.ptr, out_starts,
                                                        out_cols, out_vals, overflow_.ptr);
         else
-            galerkin_kernel<6, C><<<count, 64, 0, s>>>(count, cstarts.ptr, crows.ptr, cat.ptr, p_vals_.ptr, ap, counts.ptr, out_starts,
+            galerkin_kernel<6, C><<<count, 64, 0, s>>>(count, cstarts.ptr, crows.ptr, cat.ptr, p_vals_->ptr, ap, counts.ptr, out_starts,
                                                        out_cols, out_vals, overflow_.ptr);
         SF_HIP(hipGetLastError());
     };
@@ -598,7 +604,7 @@ void DeviceSetup::end_level() {
     dinv_.release();
     p_starts_.release();
     p_cols_.release();
-    p_vals_.release();
+    p_vals_.reset();
     ap_starts_.release();
     ap_cols_.release();
     ap_vals_.release();
@@ -610,6 +616,9 @@ void DeviceSetup::end_level() {
 
 DeviceValues::DeviceValues(std::shared_ptr<Context> ctx, const double *values, size_t blocks, size_t block_len)
     : ctx_(std::move(ctx)), values_(values), blocks_(blocks), len_(block_len) {}
+
+DeviceValues::DeviceValues(std::shared_ptr<Context> ctx, std::shared_ptr<DeviceBuffer<double>> owned, size_t blocks, size_t block_len)
+    : ctx_(std::move(ctx)), owned_(std::move(owned)), values_(owned_->ptr), blocks_(blocks), len_(block_len) {}
 
 void DeviceValues::values(double *out) const {
     hipStream_t s = ctx_->stream;

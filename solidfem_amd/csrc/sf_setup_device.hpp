@@ -25,6 +25,7 @@ class DeviceSetup final : public SetupAccelerator {
     bool product(u32 count) override;
     bool galerkin(u32 count, const std::vector<int64_t> &pcol_starts, const std::vector<std::pair<u32, int64_t>> &entries,
                   HostBsr<double> &coarse) override;
+    std::shared_ptr<const ValueSource> prolongator_values() override;
     void end_level() override;
 
    private:
@@ -39,22 +40,28 @@ class DeviceSetup final : public SetupAccelerator {
     size_t rows_ = 0;
     DeviceBuffer<int64_t> a_starts_, p_starts_, ap_starts_;
     DeviceBuffer<u32> a_cols_, p_cols_, ap_cols_, overflow_;
-    DeviceBuffer<double> a_vals_, dinv_, p_vals_, ap_vals_, x_, y_;
+    DeviceBuffer<double> a_vals_, dinv_, ap_vals_, x_, y_;
+    std::shared_ptr<DeviceBuffer<double>> p_vals_;  // shared with the level's prolongator source
+    size_t p_blocks_ = 0;
     const double *a_vals_ptr_ = nullptr;
     std::vector<double> host_y_;
 };
 
-// A study's assembled values on the device as the finest level's value source. The study owns
-// the values and outlives the hierarchy that holds this source.
+// Block values on the device as a value source: borrowed (a study's assembled values, which the
+// study keeps while the hierarchy holding this source lives) or shared (a prolongator built on
+// the device).
 class DeviceValues final : public ValueSource {
    public:
     DeviceValues(std::shared_ptr<Context> ctx, const double *values, size_t blocks, size_t block_len);
+    DeviceValues(std::shared_ptr<Context> ctx, std::shared_ptr<DeviceBuffer<double>> owned, size_t blocks, size_t block_len);
     void values(double *out) const override;
     void blocks(const std::vector<int64_t> &slots, double *out) const override;
     void block_squares(double *out) const override;
+    const double *device() const override { return values_; }
 
    private:
     std::shared_ptr<Context> ctx_;
+    std::shared_ptr<DeviceBuffer<double>> owned_;
     const double *values_;
     size_t blocks_, len_;
 };
