@@ -56,9 +56,10 @@ __device__ __forceinline__ u32 wave_exclusive_u32(u32 v, u32 &total) {
 // The sorted distinct coarse indices one row's contributions land in. `each(lane, f)` calls
 // f(index) for every contribution a lane enumerates (lanes split the contributions among them).
 // Leaves them in `slots[0..k)` and returns k, or UINT_MAX if the row overflows the window or
-// the slot list (the caller reports the overflow and the level's phase runs on the host).
+// holds more than `limit` (at most MAX_SLOTS) columns: the caller reports the overflow and the
+// level's phase runs on the host.
 template <typename Each>
-__device__ u32 row_columns(Each each, u32 *bitmap, u32 *slots, bool keep) {
+__device__ u32 row_columns(Each each, u32 *bitmap, u32 *slots, bool keep, u32 limit) {
     const int lane = threadIdx.x;
     u32 lo = UINT_MAX, hi = 0;
     each(lane, [&](u32 id) {
@@ -90,7 +91,7 @@ __device__ u32 row_columns(Each each, u32 *bitmap, u32 *slots, bool keep) {
         base += total;
     }
     __syncthreads();
-    return base > MAX_SLOTS ? UINT_MAX : base;
+    return base > limit ? UINT_MAX : base;
 }
 
 __device__ __forceinline__ bool sorted_contains(const u32 *first, int64_t n, u32 value) {
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(64) void smooth_kernel(size_t rows, SetupRows a, co
                                                     const int64_t *__restrict__ ns, const u32 *__restrict__ nc,
                                                     const double *__restrict__ tentative, const double *__restrict__ dinv,
                                                     double omega, u32 *__restrict__ counts, const int64_t *__restrict__ out_starts,
-                                                    u32 *__restrict__ out_cols, double *__restrict__ out_vals, u32 *overflow) {
+                                                    u32 *__restrict__ out_cols, double *__restrict__ out_vals, u32 *overflow, u32 slot_limit) {
     __shared__ u32 bitmap[WINDOW_WORDS];
     __shared__ u32 slots[MAX_SLOTS];
     const size_t row = blockIdx.x;
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(64) void smooth_kernel(size_t rows, SetupRows a, co
         [&](int l, auto f) {
             for (int64_t at = first + l; at < last; at += WAVE_SIZE) f(aggregate[source(at)]);
         },
-        bitmap, slots, !COUNT);
+        bitmap, slots, !COUNT, slot_limit);
     if (k == UINT_MAX) {
         if (lane == 0) atomicOr(overflow, 1u);
         return;
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(64) void smooth_kernel(size_t rows, SetupRows a, co
 template <int RB, bool COUNT>
 __global__ __launch_bounds__(64) void product_kernel(size_t rows, SetupRows a, SetupRows p, u32 *__restrict__ counts,
                                                      const int64_t *__restrict__ out_starts, u32 *__restrict__ out_cols,
-                                                     double *__restrict__ out_vals, u32 *overflow) {
+                                                     double *__restrict__ out_vals, u32 *overflow, u32 slot_limit) {
     __shared__ u32 bitmap[WINDOW_WORDS];
     __shared__ u32 slots[MAX_SLOTS];
     const size_t row = blockIdx.x;
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(64) void product_kernel(size_t rows, SetupRows a, S
                 for (int64_t in = p.starts[col]; in < p.starts[col + 1]; ++in) f(p.cols[in]);
             }
         },
-        bitmap, slots, !COUNT);
+        bitmap, slots, !COUNT, slot_limit);
     if (k == UINT_MAX) {
         if (lane == 0) atomicOr(overflow, 1u);
         return;
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(64) void galerkin_kernel(size_t count, const int64_
                                                       const u32 *__restrict__ pcol_rows, const int64_t *__restrict__ pcol_at,
                                                       const double *__restrict__ pvals, SetupRows ap, u32 *__restrict__ counts,
                                                       const int64_t *__restrict__ out_starts, u32 *__restrict__ out_cols,
-                                                      double *__restrict__ out_vals, u32 *overflow) {
+                                                      double *__restrict__ out_vals, u32 *overflow, u32 slot_limit) {
     __shared__ u32 bitmap[WINDOW_WORDS];
     __shared__ u32 slots[MAX_SLOTS];
     const size_t c = blockIdx.x;
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(64) void galerkin_kernel(size_t count, const int64_
                 for (int64_t in = ap.starts[fine]; in < ap.starts[fine + 1]; ++in) f(ap.cols[in]);
             }
         },
-        bitmap, slots, !COUNT);
+        bitmap, slots, !COUNT, slot_limit);
     if (k == UINT_MAX) {
         if (lane == 0) atomicOr(overflow, 1u);
         return;
@@ -372,7 +373,12 @@ std::vector<int64_t> starts_of(const std::vector<u32> &counts) {
 }  // namespace
 
 DeviceSetup::DeviceSetup(std::shared_ptr<Context> ctx, size_t min_rows, const double *level0_values)
-    : ctx_(std::move(ctx)), min_rows_(min_rows), level0_values_(level0_values) {}
+    : ctx_(std::move(ctx)), min_rows_(min_rows), level0_values_(level0_values) {
+    // SOLIDFEM_SETUP_SLOT_LIMIT: fewer output blocks per row than the kernels hold, so that rows
+    // above it take the host path (the fallback's own bits)
+    if (const char *e = std::getenv("SOLIDFEM_SETUP_SLOT_LIMIT"))
+        slot_limit_ = (u32)std::min<long>(MAX_SLOTS, std::max<long>(1, std::atol(e)));
+}
 
 DeviceSetup::~DeviceSetup() {
     if (ctx_) (void)hipStreamSynchronize(ctx_->stream);
@@ -461,10 +467,10 @@ bool DeviceSetup::smooth(const std::vector<u32> &aggregate, u32 count, const std
         constexpr bool C = decltype(count_pass)::value;
         if (rb_ == 3)
             smooth_kernel<3, C><<<rows_, 64, 0, s>>>(rows_, a, agg.ptr, nstarts.ptr, ncols.ptr, tent.ptr, dinv_.ptr, omega, counts.ptr,
-                                                     out_starts, out_cols, out_vals, overflow_.ptr);
+                                                     out_starts, out_cols, out_vals, overflow_.ptr, slot_limit_);
         else
             smooth_kernel<6, C><<<rows_, 64, 0, s>>>(rows_, a, agg.ptr, nstarts.ptr, ncols.ptr, tent.ptr, dinv_.ptr, omega, counts.ptr,
-                                                     out_starts, out_cols, out_vals, overflow_.ptr);
+                                                     out_starts, out_cols, out_vals, overflow_.ptr, slot_limit_);
         SF_HIP(hipGetLastError());
     };
     launch(std::true_type{}, nullptr, nullptr, nullptr);
@@ -509,9 +515,9 @@ bool DeviceSetup::product(u32 count) {
     auto launch = [&](auto count_pass, const int64_t *out_starts, u32 *out_cols, double *out_vals) {
         constexpr bool C = decltype(count_pass)::value;
         if (rb_ == 3)
-            product_kernel<3, C><<<rows_, 64, 0, s>>>(rows_, a, p, counts.ptr, out_starts, out_cols, out_vals, overflow_.ptr);
+            product_kernel<3, C><<<rows_, 64, 0, s>>>(rows_, a, p, counts.ptr, out_starts, out_cols, out_vals, overflow_.ptr, slot_limit_);
         else
-            product_kernel<6, C><<<rows_, 64, 0, s>>>(rows_, a, p, counts.ptr, out_starts, out_cols, out_vals, overflow_.ptr);
+            product_kernel<6, C><<<rows_, 64, 0, s>>>(rows_, a, p, counts.ptr, out_starts, out_cols, out_vals, overflow_.ptr, slot_limit_);
         SF_HIP(hipGetLastError());
     };
     launch(std::true_type{}, nullptr, nullptr, nullptr);
@@ -560,10 +566,10 @@ bool DeviceSetup::galerkin(u32 count, const std::vector<int64_t> &pcol_starts, c
         if (count == 0) return;
         if (rb_ == 3)
             galerkin_kernel<3, C><<<count, 64, 0, s>>>(count, cstarts.ptr, crows.ptr, cat.ptr, p_vals_->ptr, ap, counts.ptr, out_starts,
-                                                       out_cols, out_vals, overflow_.ptr);
+                                                       out_cols, out_vals, overflow_.ptr, slot_limit_);
         else
             galerkin_kernel<6, C><<<count, 64, 0, s>>>(count, cstarts.ptr, crows.ptr, cat.ptr, p_vals_->ptr, ap, counts.ptr, out_starts,
-                                                       out_cols, out_vals, overflow_.ptr);
+                                                       out_cols, out_vals, overflow_.ptr, slot_limit_);
         SF_HIP(hipGetLastError());
     };
     launch(std::true_type{}, nullptr, nullptr, nullptr);

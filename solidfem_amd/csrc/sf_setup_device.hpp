@@ -42,6 +42,7 @@ class DeviceSetup final : public SetupAccelerator {
     DeviceBuffer<u32> a_cols_, p_cols_, ap_cols_, overflow_;
     DeviceBuffer<double> a_vals_, dinv_, ap_vals_, x_, y_;
     std::shared_ptr<DeviceBuffer<double>> p_vals_;  // shared with the level's prolongator source
+    u32 slot_limit_ = 1024;                          // output blocks per row the kernels take
     size_t p_blocks_ = 0;
     const double *a_vals_ptr_ = nullptr;
     std::vector<double> host_y_;

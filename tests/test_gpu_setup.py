@@ -114,3 +114,28 @@ def test_a_device_built_study_solves_to_the_fingerprint(ctx, case):
     p = s.solve()
     assert p.outcome == sf.StepOutcome.CONVERGED
     assert (p.iteration, refcpu.fingerprint(s.displacements())) == (iterations, fingerprint)
+
+
+@pytest.mark.parametrize("limit", [4, 24])
+def test_device_phases_that_overflow_take_the_host_path_with_the_same_bits(ctx, monkeypatch, limit):
+    # rows with more output blocks than the limit send their phase to the host: with 4, the
+    # smoothed prolongator; with 24, A P and the Galerkin product. The host then fetches the
+    # values the device kept (the study's matrix, the device prolongator), and the study must
+    # still solve to the reference's fingerprint with the host setup's hierarchy
+    monkeypatch.setenv("SOLIDFEM_SETUP_SLOT_LIMIT", str(limit))
+    size, div, iterations, fingerprint = FINGERPRINT_CANTILEVER
+    w = loaded(size, div)
+    options = sf.MultigridOptions(precision=sf.Precision.DOUBLE, coarse_dofs=100, setup_device=sf.SetupDevice.DEVICE)
+    b = sf.StudyBuilder.new(ctx, sf.Model.from_workload(w), sf.PcgOptions(tolerance=1e-9, dot_order=sf.DotOrder.REFERENCE), options)
+    b.step(2 ** 62)
+    s = b.finish()
+    p = s.solve()
+    assert p.outcome == sf.StepOutcome.CONVERGED
+    assert (p.iteration, refcpu.fingerprint(s.displacements())) == (iterations, fingerprint)
+    mg, host = s.multigrid(), run_setup(None, refcpu.Problem(w), options)
+    assert len(mg.level_sizes()) == host.level_count()
+    for level in range(host.level_count()):
+        for prolongator in ([False, True] if level + 1 < host.level_count() else [False]):
+            a, b = mg.level_matrix(level, prolongator), host.level_matrix(level, prolongator)
+            assert np.array_equal(a["columns"], b["columns"]) and np.array_equal(bits(a["values"]), bits(b["values"]))
+        assert np.array_equal(bits(mg.level_inverse_diagonal(level)), bits(host.level_inverse_diagonal(level)))
