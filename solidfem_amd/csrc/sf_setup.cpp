@@ -607,6 +607,8 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         if (on_accelerator) {
                             p_source_ = accelerator_->prolongator_values();
                             budget = budget > rows ? budget - rows : 0;
+                        } else if (StageClock::on()) {
+                            std::fprintf(stderr, "[setup] level %d: the smoothing falls back to the host\n", level);
                         }
                     }
                     if (!on_accelerator) ensure_values();
@@ -665,6 +667,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         }
                         on_accelerator = device_ap_ = accelerator_->product(aggregate_count_);
                         if (on_accelerator) budget = budget > rows ? budget - rows : 0;
+                        else if (StageClock::on()) std::fprintf(stderr, "[setup] level %d: A P falls back to the host\n", level);
                     }
                     if (!on_accelerator) {
                         ensure_values();
@@ -721,8 +724,13 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         }
                         on_accelerator = accelerator_->galerkin((u32)count, pcol_starts_, pcol_entries_, coarse_);
                         if (on_accelerator) budget = budget > count ? budget - count : 0;
+                        else if (StageClock::on()) std::fprintf(stderr, "[setup] level %d: the Galerkin product falls back to the host\n", level);
                     }
-                    if (!on_accelerator) ensure_prolongator_values();
+                    if (!on_accelerator) {
+                        ensure_prolongator_values();
+                        if (device_ap_ && a_times_p_.columns.empty() && a_times_p_.starts.size() != a_times_p_.rows + 1)
+                            accelerator_->product_values(a_times_p_);
+                    }
                     bool done = on_accelerator || run_rows(count, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         std::vector<RowOut> out(l - f);
                         parallel_for(l - f, 64, [&](size_t a, size_t b) {

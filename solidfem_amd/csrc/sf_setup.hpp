@@ -73,6 +73,9 @@ class SetupAccelerator {
     // The prolongator's values after a successful smooth(), which leaves only its pattern on the
     // host; the source keeps them after end_level().
     virtual std::shared_ptr<const ValueSource> prolongator_values() = 0;
+    // A P from the last successful product(), which keeps it on the device, for a Galerkin
+    // product that falls back to the host.
+    virtual void product_values(HostBsr<double> &ap) = 0;
     virtual void end_level() = 0;
 };
 

@@ -409,6 +409,7 @@ void DeviceSetup::begin(int level, const HostBsr<double> &a, const std::vector<d
     ap_starts_.release();
     ap_cols_.release();
     ap_vals_.release();
+    ap_host_starts_.clear();
 }
 
 void DeviceSetup::check_overflow(const char *phase) {
@@ -529,15 +530,27 @@ bool DeviceSetup::product(u32 count) {
     std::vector<u32> host_counts(rows_);
     counts.download(host_counts.data(), rows_, s);
     SF_HIP(hipStreamSynchronize(s));
-    const std::vector<int64_t> starts = starts_of(host_counts);
-    const size_t blocks = (size_t)starts.back();
-    ap_starts_.upload(starts, s);
+    ap_host_starts_ = starts_of(host_counts);
+    const size_t blocks = (size_t)ap_host_starts_.back();
+    ap_starts_.upload(ap_host_starts_, s);
     ap_cols_.resize(std::max<size_t>(blocks, 1));
     ap_vals_.resize(std::max<size_t>(blocks * rb_ * MODES6, 1));
     launch(std::false_type{}, ap_starts_.ptr, ap_cols_.ptr, ap_vals_.ptr);
     SF_HIP(hipStreamSynchronize(s));
     (void)count;
     return true;
+}
+
+void DeviceSetup::product_values(HostBsr<double> &ap) {
+    hipStream_t s = ctx_->stream;
+    if (ap_host_starts_.size() != rows_ + 1) throw Error(SF_ERR_INTERNAL, "no A P on the device");
+    const size_t blocks = (size_t)ap_host_starts_.back();
+    ap.starts = ap_host_starts_;
+    ap.columns.resize(blocks);
+    ap.values.resize(blocks * rb_ * MODES6);
+    ap_cols_.download(ap.columns.data(), blocks, s);
+    ap_vals_.download(ap.values.data(), ap.values.size(), s);
+    SF_HIP(hipStreamSynchronize(s));
 }
 
 bool DeviceSetup::galerkin(u32 count, const std::vector<int64_t> &pcol_starts, const std::vector<std::pair<u32, int64_t>> &entries,
@@ -614,6 +627,7 @@ void DeviceSetup::end_level() {
     ap_starts_.release();
     ap_cols_.release();
     ap_vals_.release();
+    ap_host_starts_.clear();
     x_.release();
     y_.release();
     host_y_.clear();

@@ -26,6 +26,7 @@ class DeviceSetup final : public SetupAccelerator {
     bool galerkin(u32 count, const std::vector<int64_t> &pcol_starts, const std::vector<std::pair<u32, int64_t>> &entries,
                   HostBsr<double> &coarse) override;
     std::shared_ptr<const ValueSource> prolongator_values() override;
+    void product_values(HostBsr<double> &ap) override;
     void end_level() override;
 
    private:
@@ -44,6 +45,7 @@ class DeviceSetup final : public SetupAccelerator {
     std::shared_ptr<DeviceBuffer<double>> p_vals_;  // shared with the level's prolongator source
     u32 slot_limit_ = 1024;                          // output blocks per row the kernels take
     size_t p_blocks_ = 0;
+    std::vector<int64_t> ap_host_starts_;  // the pattern of the last A P
     const double *a_vals_ptr_ = nullptr;
     std::vector<double> host_y_;
 };
