@@ -644,8 +644,8 @@ void DeviceAssembly::prepare(const Model &model, const AssemblyPattern &pattern,
     for (size_t e = 0; e < mesh.tets.size(); ++e)
         for (int a = 0; a < 10; ++a) t[10 * e + a] = mesh.tets[e][a];
     tets.upload(t, s);
-    contribution_starts.upload(pattern.contribution_starts, s);
-    contributions.upload(pattern.contributions, s);
+    contribution_starts.upload(pattern.contribution_starts.data(), pattern.contribution_starts.size(), s);
+    contributions.upload(pattern.contributions.data(), pattern.contributions.size(), s);
     elements = mesh.tets.size();
     if (elements >= (1u << 25)) throw Error(SF_ERR_INVALID_ARGUMENT, "too many elements for the assembly map");
     field = model.field;

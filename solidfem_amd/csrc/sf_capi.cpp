@@ -320,7 +320,8 @@ sf_status sf_model_element_stiffness(sf_context *ctx, const sf_model *model, uin
         if (first + count > model->model.mesh.tets.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "element range out of bounds");
         DeviceAssembly a;
         AssemblyPattern empty;
-        empty.contribution_starts.assign(1, 0);
+        empty.contribution_starts = HostArray<int64_t>(1);
+        empty.contribution_starts[0] = 0;
         a.prepare(model->model, empty, ctx->ctx->stream);
         a.element_matrices(first, count, ke, ctx->ctx->stream);
         a.check_fault(ctx->ctx->stream);

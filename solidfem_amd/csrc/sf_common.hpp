@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -95,6 +96,21 @@ struct DeviceBuffer {
     void download(T *host, size_t n, hipStream_t s = nullptr) const {
         if (n) SF_HIP(hipMemcpyAsync(host, ptr, n * sizeof(T), hipMemcpyDeviceToHost, s));
     }
+};
+
+// A host array of a trivial type whose elements start uninitialised, so that large arrays are
+// first touched by the (parallel) code that fills them rather than zeroed serially.
+template <typename T>
+struct HostArray {
+    std::unique_ptr<T[]> data_;
+    size_t size_ = 0;
+    HostArray() = default;
+    explicit HostArray(size_t n) : data_(n ? new T[n] : nullptr), size_(n) {}
+    size_t size() const { return size_; }
+    T *data() { return data_.get(); }
+    const T *data() const { return data_.get(); }
+    T &operator[](size_t i) { return data_[i]; }
+    const T &operator[](size_t i) const { return data_[i]; }
 };
 
 // Runs `body(first, last)` over [0, n) in contiguous chunks on up to `threads` threads. Each

@@ -131,8 +131,8 @@ struct HostBsr {
 // contributions in ascending element order, packed as (element << 7) | (10 a + b).
 struct AssemblyPattern {
     HostBsr<double> pattern;               // values empty
-    std::vector<int64_t> contribution_starts;  // blocks + 1
-    std::vector<u32> contributions;
+    HostArray<int64_t> contribution_starts;  // blocks + 1
+    HostArray<u32> contributions;
 };
 AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count);
 

@@ -812,6 +812,7 @@ std::vector<double> rigid_body_modes(const QuadMesh &mesh, const std::vector<u32
 // ---- pattern ----------------------------------------------------------------------------
 
 AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count) {
+    StageClock clock;
     // Node -> elements adjacency (ascending elements), by counting sort.
     const size_t ne = mesh.tets.size();
     std::vector<int64_t> adj_start(free_count + 1, 0);
@@ -828,24 +829,32 @@ AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free
                 if (f != FIXED) adj[(size_t)fill[f]++] = ((u32)e << 4) | (u32)a;
             }
     }
+    clock.lap("  adjacency");
     AssemblyPattern out;
     auto &pat = out.pattern;
     pat.rows = pat.cols = free_count;
     pat.rb = pat.cb = 3;
-    // Pass 1: per row, its sorted unique free neighbour set and contribution count.
+    // A row's distinct free neighbours are marked in a per-task array indexed by free node,
+    // stamped with the row so that nothing is cleared between rows; a second array gives each
+    // neighbour's place in the row's sorted columns.
+    // Pass 1: per row, the number of distinct free neighbours and of contributions.
     std::vector<int64_t> row_len(free_count, 0), row_contrib(free_count, 0);
     parallel_for(free_count, 2048, [&](size_t first, size_t last) {
-        std::vector<u32> cols;
+        std::vector<u32> stamp(free_count, FIXED);
         for (size_t r = first; r < last; ++r) {
-            cols.clear();
-            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at) {
-                const auto &t = mesh.tets[adj[(size_t)at] >> 4];
-                for (u32 node : t)
-                    if (free[node] != FIXED) cols.push_back(free[node]);
-            }
-            row_contrib[r] = (int64_t)cols.size();
-            std::sort(cols.begin(), cols.end());
-            row_len[r] = std::unique(cols.begin(), cols.end()) - cols.begin();
+            int64_t distinct = 0, all = 0;
+            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at)
+                for (u32 node : mesh.tets[adj[(size_t)at] >> 4]) {
+                    const u32 f = free[node];
+                    if (f == FIXED) continue;
+                    ++all;
+                    if (stamp[f] != (u32)r) {
+                        stamp[f] = (u32)r;
+                        ++distinct;
+                    }
+                }
+            row_len[r] = distinct;
+            row_contrib[r] = all;
         }
     });
     pat.starts.assign(free_count + 1, 0);
@@ -854,51 +863,56 @@ AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free
         pat.starts[r + 1] = pat.starts[r] + row_len[r];
         contrib_row[r + 1] = contrib_row[r] + row_contrib[r];
     }
+    clock.lap("  pattern pass 1");
     const size_t nnzb = (size_t)pat.starts[free_count];
-    pat.columns.assign(nnzb, 0);
-    out.contribution_starts.assign(nnzb + 1, 0);
-    out.contributions.assign((size_t)contrib_row[free_count], 0);
-    // Pass 2: columns, then each block's contributions in ascending element order.
+    pat.columns.resize(nnzb);
+    out.contribution_starts = HostArray<int64_t>(nnzb + 1);
+    out.contributions = HostArray<u32>((size_t)contrib_row[free_count]);
+    // Pass 2: the sorted columns, then each block's contributions in ascending element order.
     parallel_for(free_count, 2048, [&](size_t first, size_t last) {
-        std::vector<u32> cols;
-        std::vector<int64_t> count;
+        std::vector<u32> stamp(free_count, FIXED), place(free_count, 0), cols;
+        std::vector<int64_t> fill;
         for (size_t r = first; r < last; ++r) {
             cols.clear();
-            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at) {
-                const auto &t = mesh.tets[adj[(size_t)at] >> 4];
-                for (u32 node : t)
-                    if (free[node] != FIXED) cols.push_back(free[node]);
-            }
+            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at)
+                for (u32 node : mesh.tets[adj[(size_t)at] >> 4]) {
+                    const u32 f = free[node];
+                    if (f != FIXED && stamp[f] != (u32)r) {
+                        stamp[f] = (u32)r;
+                        cols.push_back(f);
+                    }
+                }
             std::sort(cols.begin(), cols.end());
-            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-            std::copy(cols.begin(), cols.end(), pat.columns.begin() + pat.starts[r]);
-            count.assign(cols.size(), 0);
-            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at) {
-                const auto &t = mesh.tets[adj[(size_t)at] >> 4];
-                for (u32 node : t)
-                    if (free[node] != FIXED) count[std::lower_bound(cols.begin(), cols.end(), free[node]) - cols.begin()] += 1;
+            const size_t row0 = (size_t)pat.starts[r];
+            for (size_t k = 0; k < cols.size(); ++k) {
+                pat.columns[row0 + k] = cols[k];
+                place[cols[k]] = (u32)k;
             }
+            fill.assign(cols.size(), 0);
+            for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at)
+                for (u32 node : mesh.tets[adj[(size_t)at] >> 4])
+                    if (free[node] != FIXED) fill[place[free[node]]] += 1;
             // block-local offsets within this row's contribution range
             int64_t base = contrib_row[r];
             for (size_t k = 0; k < cols.size(); ++k) {
-                out.contribution_starts[(size_t)pat.starts[r] + k] = base;
-                base += count[k];
+                out.contribution_starts[row0 + k] = base;
+                const int64_t c = fill[k];
+                fill[k] = base;
+                base += c;
             }
-            std::vector<int64_t> fill(cols.size());
-            for (size_t k = 0; k < cols.size(); ++k) fill[k] = out.contribution_starts[(size_t)pat.starts[r] + k];
             for (int64_t at = adj_start[r]; at < adj_start[r + 1]; ++at) {
                 const u32 e = adj[(size_t)at] >> 4, a = adj[(size_t)at] & 15u;
                 const auto &t = mesh.tets[e];
                 for (u32 b = 0; b < 10; ++b) {
-                    u32 f = free[t[b]];
+                    const u32 f = free[t[b]];
                     if (f == FIXED) continue;
-                    size_t k = std::lower_bound(cols.begin(), cols.end(), f) - cols.begin();
-                    out.contributions[(size_t)fill[k]++] = (e << 7) | (10 * a + b);
+                    out.contributions[(size_t)fill[place[f]]++] = (e << 7) | (10 * a + b);
                 }
             }
         }
     });
     out.contribution_starts[nnzb] = contrib_row[free_count];
+    clock.lap("  pattern pass 2");
     return out;
 }
 
