@@ -269,6 +269,16 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     });
                     if (!done) return false;
                     diagonal_blocks_ = std::vector<double>();
+                    if (accelerator_ && accelerator_->takes(level, current_)) {
+                        // the power iteration needs only A and the block inverses: it runs on the
+                        // accelerator while the host aggregates
+                        init_power();
+                        on_device_ = true;
+                        spectral_job_ = std::async(std::launch::async, [this, level] {
+                            accelerator_->begin(level, current_, inverse_diagonal_);
+                            return accelerator_->spectral(power_, (int)POWER_ITERATIONS);
+                        });
+                    }
                     if (current_.dofs() <= options_.coarse_dofs || (size_t)level + 1 >= options_.max_levels) {
                         is_coarsest_ = true;
                         phase_ = Phase::Spectral;
@@ -478,29 +488,15 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                 case Phase::Spectral: {
                     // setup.rs:390-449
                     const size_t dofs = current_.dofs();
-                    if (power_done_ == 0 && power_.size() != dofs) {
-                        power_.resize(dofs);
-                        for (size_t i = 0; i < dofs; ++i) {
-                            const u64 mixed = ((u64)i + 1) * 0x9E3779B97F4A7C15ull;  // block.rs:379-382
-                            power_[i] = 0.5 + (double)(mixed >> 11) / (double)(1ull << 53);
-                        }
-                        double s = 0.0;
-                        for (double v : power_) s += v * v;
-                        const double norm = std::sqrt(s);
-                        for (double &v : power_) v /= norm;
-                        scratch_.assign(dofs, 0.0);
-                        scratch2_.assign(dofs, 0.0);
-                    }
-                    if (power_done_ == 0 && accelerator_ && accelerator_->takes(level, current_)) {
-                        // the whole iteration at once on the accelerator, charged as the host's
+                    if (power_done_ == 0 && power_.size() != dofs) init_power();
+                    if (power_done_ == 0 && spectral_job_.valid()) {
+                        // the iteration started beside the aggregation, charged as the host's
                         if (budget == 0) return false;
                         if (cancel()) {
                             cancelled_ = true;
                             throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
                         }
-                        accelerator_->begin(level, current_, inverse_diagonal_);
-                        on_device_ = true;
-                        lambda_max_ = accelerator_->spectral(power_, (int)POWER_ITERATIONS);
+                        lambda_max_ = spectral_job_.get();
                         power_done_ = POWER_ITERATIONS;
                         budget = budget > rows * POWER_ITERATIONS ? budget - rows * POWER_ITERATIONS : 0;
                     }
@@ -811,6 +807,21 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
 void MultigridSetup::release_accelerator_level() {
     if (on_device_) accelerator_->end_level();
     on_device_ = device_p_ = device_ap_ = false;
+}
+
+void MultigridSetup::init_power() {
+    const size_t dofs = current_.dofs();
+    power_.resize(dofs);
+    for (size_t i = 0; i < dofs; ++i) {
+        const u64 mixed = ((u64)i + 1) * 0x9E3779B97F4A7C15ull;  // block.rs:379-382
+        power_[i] = 0.5 + (double)(mixed >> 11) / (double)(1ull << 53);
+    }
+    double s = 0.0;
+    for (double v : power_) s += v * v;
+    const double norm = std::sqrt(s);
+    for (double &v : power_) v /= norm;
+    scratch_.assign(dofs, 0.0);
+    scratch2_.assign(dofs, 0.0);
 }
 
 void MultigridSetup::ensure_values() {

@@ -6,6 +6,7 @@
 
 #include <array>
 #include <chrono>
+#include <future>
 #include <memory>
 
 #include "sf_host.hpp"
@@ -105,6 +106,7 @@ class MultigridSetup {
     void finish_level();
     void push_final_level();
     void spectral_iteration();
+    void init_power();  // the normalised start vector (block.rs:379-382)
     // Brings the current level's values, or its prolongator's, to the host when they are held
     // elsewhere.
     void ensure_values();
@@ -156,6 +158,10 @@ class MultigridSetup {
     bool cancelled_ = false;
     bool failed_ = false;
     Error failure_{SF_OK, ""};
+    // The power iteration of a level the accelerator takes, started once the block inverses are
+    // known and run beside the aggregation; the Spectral phase collects it. Declared last, so
+    // that it is waited for before anything it reads goes.
+    std::future<double> spectral_job_;
 };
 
 bool invert_spd(const double *input, size_t n, double *output, double &pivot);

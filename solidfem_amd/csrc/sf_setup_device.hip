@@ -389,6 +389,7 @@ bool DeviceSetup::takes(int level, const HostBsr<double> &a) const {
 }
 
 void DeviceSetup::begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal) {
+    SF_HIP(hipSetDevice(ctx_->device));  // may run on a thread of its own
     hipStream_t s = ctx_->stream;
     level_ = level;
     rb_ = (int)a.rb;
