@@ -464,14 +464,22 @@ class Hierarchy final : public DeviceMultigrid {
             d.rb = (int)h.a.rb;
             d.rows = h.a.rows;
             d.n = h.a.dofs();
+            // the matrix's values on the device where the setup left them (a coarse matrix built
+            // there), else uploaded
+            const double *adev = h.a.values.empty() && h.source ? h.source->device() : nullptr;
             if (l == 0 && finest && finest->rows == h.a.rows && finest->blocks == h.a.blocks() && finest->rb == (int)h.a.rb &&
                 finest->cb == (int)h.a.cb)
                 convert_sliced(d.a, *finest, s);  // the study's matrix, already on the device
+            else if (adev)
+                build_sliced(d.a, plan_sliced(h.a.starts.data(), h.a.columns.data(), h.a.rows, (int)h.a.rb, (int)h.a.cb), h.a.cols, adev, s);
             else {
                 h.fetch_values();
                 upload_sliced(d.a, h.a, s);
             }
-            if (l > 0) upload_bsr(d.a_rows, h.a, s);
+            if (l > 0) {
+                if (adev) build_bsr(d.a_rows, h.a, adev, s);
+                else upload_bsr(d.a_rows, h.a, s);
+            }
             std::vector<S> dinv(h.inverse_diagonal.size());
             for (size_t i = 0; i < dinv.size(); ++i) dinv[i] = (S)h.inverse_diagonal[i];
             d.dinv.upload(dinv, s);

@@ -274,8 +274,9 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                         // accelerator while the host aggregates
                         init_power();
                         on_device_ = true;
-                        spectral_job_ = std::async(std::launch::async, [this, level] {
-                            accelerator_->begin(level, current_, inverse_diagonal_);
+                        const double *held = current_.values.empty() && source_ ? source_->device() : nullptr;
+                        spectral_job_ = std::async(std::launch::async, [this, level, held] {
+                            accelerator_->begin(level, current_, inverse_diagonal_, held);
                             return accelerator_->spectral(power_, (int)POWER_ITERATIONS);
                         });
                     }
@@ -719,7 +720,10 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                             throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
                         }
                         on_accelerator = accelerator_->galerkin((u32)count, pcol_starts_, pcol_entries_, coarse_);
-                        if (on_accelerator) budget = budget > count ? budget - count : 0;
+                        if (on_accelerator) {
+                            c_source_ = accelerator_->coarse_values();
+                            budget = budget > count ? budget - count : 0;
+                        }
                         else if (StageClock::on()) std::fprintf(stderr, "[setup] level %d: the Galerkin product falls back to the host\n", level);
                     }
                     if (!on_accelerator) {
@@ -863,9 +867,10 @@ void MultigridSetup::finish_level() {
     std::shared_ptr<const ValueSource> p_source = prolongator_.values.empty() ? p_source_ : nullptr;
     p_source_.reset();
     release_accelerator_level();
-    // setup.rs:650-695: a unit diagonal on coarse values no aggregate direction reaches
+    // setup.rs:650-695: a unit diagonal on coarse values no aggregate direction reaches (the
+    // device's Galerkin product applies it as it writes)
     const size_t dim = coarse_.rb;
-    for (size_t row = 0; row < coarse_.rows; ++row)
+    for (size_t row = 0; row < (coarse_.values.empty() ? 0 : coarse_.rows); ++row)
         for (int64_t at = coarse_.starts[row]; at < coarse_.starts[row + 1]; ++at) {
             if (coarse_.columns[(size_t)at] != row) continue;
             for (size_t d = 0; d < dim; ++d) {
@@ -887,6 +892,8 @@ void MultigridSetup::finish_level() {
     levels_.push_back(std::move(l));
     current_ = std::move(coarse_);
     coarse_ = HostBsr<double>();
+    source_ = current_.values.empty() ? std::move(c_source_) : nullptr;
+    c_source_.reset();
     a_times_p_ = HostBsr<double>();
     pcol_starts_.clear();
     pcol_entries_.clear();

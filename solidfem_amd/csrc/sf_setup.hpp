@@ -63,7 +63,8 @@ class SetupAccelerator {
    public:
     virtual ~SetupAccelerator() = default;
     virtual bool takes(int level, const HostBsr<double> &a) const = 0;
-    virtual void begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal) = 0;
+    // `device_values`, when given, are a's values already on the device; else a's host values.
+    virtual void begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal, const double *device_values) = 0;
     // `power` is the normalised start vector; returns the last iteration's norm.
     virtual double spectral(std::vector<double> &power, int iterations) = 0;
     virtual bool smooth(const std::vector<u32> &aggregate, u32 count, const std::vector<int64_t> &ns, const std::vector<u32> &nc,
@@ -77,6 +78,9 @@ class SetupAccelerator {
     // A P from the last successful product(), which keeps it on the device, for a Galerkin
     // product that falls back to the host.
     virtual void product_values(HostBsr<double> &ap) = 0;
+    // The coarse matrix's values after a successful galerkin(), which leaves only its pattern
+    // on the host (with the unit diagonal where no aggregate direction reaches already applied).
+    virtual std::shared_ptr<const ValueSource> coarse_values() = 0;
     virtual void end_level() = 0;
 };
 
@@ -144,6 +148,7 @@ class MultigridSetup {
     std::unique_ptr<SetupAccelerator> accelerator_;
     std::shared_ptr<const ValueSource> source_;  // the current level's values, when not on the host
     std::shared_ptr<const ValueSource> p_source_;  // its prolongator's values, when not on the host
+    std::shared_ptr<const ValueSource> c_source_;  // the coarse matrix's values, when not on the host
     std::vector<double> diagonal_blocks_;        // fetched from source_ for the block inverse
     std::vector<double> block_norms_;            // fetched from source_ for the join
     bool on_device_ = false;  // this level's numeric phases run on the accelerator

@@ -354,6 +354,12 @@ __global__ __launch_bounds__(64) void galerkin_kernel(size_t count, const int64_
             }
         }
         if (s >= k) continue;
+        // setup.rs:650-695: a unit diagonal where no aggregate direction reaches (the host applies
+        // the same rule to the finished matrix; it leaves these values as they are)
+        if (mine == (u32)c)
+#pragma unroll
+            for (int d = 0; d < MODES6; ++d)
+                if (acc[d * MODES6 + d] == 0.0) acc[d * MODES6 + d] = 1.0;
         out_cols[base + s] = mine;
         double *out = out_vals + (size_t)(base + s) * MODES6 * MODES6;
 #pragma unroll
@@ -388,7 +394,7 @@ bool DeviceSetup::takes(int level, const HostBsr<double> &a) const {
     return a.rows >= min_rows_ && (a.rb == 3 || a.rb == 6) && a.cb == a.rb && a.rows < (size_t)UINT_MAX;
 }
 
-void DeviceSetup::begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal) {
+void DeviceSetup::begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal, const double *device_values) {
     SF_HIP(hipSetDevice(ctx_->device));  // may run on a thread of its own
     hipStream_t s = ctx_->stream;
     level_ = level;
@@ -396,7 +402,10 @@ void DeviceSetup::begin(int level, const HostBsr<double> &a, const std::vector<d
     rows_ = a.rows;
     a_starts_.upload(a.starts, s);
     a_cols_.upload(a.columns, s);
-    if (level == 0 && level0_values_) {
+    if (device_values) {
+        a_vals_.release();
+        a_vals_ptr_ = device_values;
+    } else if (level == 0 && level0_values_) {
         a_vals_.release();
         a_vals_ptr_ = level0_values_;
     } else {
@@ -503,6 +512,11 @@ bool DeviceSetup::smooth(const std::vector<u32> &aggregate, u32 count, const std
     return true;
 }
 
+std::shared_ptr<const ValueSource> DeviceSetup::coarse_values() {
+    if (!coarse_vals_) throw Error(SF_ERR_INTERNAL, "no coarse matrix on the device");
+    return std::move(coarse_vals_);
+}
+
 std::shared_ptr<const ValueSource> DeviceSetup::prolongator_values() {
     if (!p_vals_) throw Error(SF_ERR_INTERNAL, "no prolongator on the device");
     return std::make_shared<DeviceValues>(ctx_, p_vals_, p_blocks_, (size_t)rb_ * MODES6);
@@ -602,13 +616,14 @@ bool DeviceSetup::galerkin(u32 count, const std::vector<int64_t> &pcol_starts, c
     DeviceBuffer<int64_t> out_starts;
     out_starts.upload(coarse.starts, s);
     DeviceBuffer<u32> out_cols(std::max<size_t>(blocks, 1));
-    DeviceBuffer<double> out_vals(std::max<size_t>(blocks * MODES6 * MODES6, 1));
-    launch(std::false_type{}, out_starts.ptr, out_cols.ptr, out_vals.ptr);
+    auto out_vals = std::make_shared<DeviceBuffer<double>>(std::max<size_t>(blocks * MODES6 * MODES6, 1));
+    launch(std::false_type{}, out_starts.ptr, out_cols.ptr, out_vals->ptr);
+    // the pattern comes to the host; the values stay here (coarse_values())
     coarse.columns.resize(blocks);
-    coarse.values.resize(blocks * MODES6 * MODES6);
+    coarse.values.clear();
     out_cols.download(coarse.columns.data(), blocks, s);
-    out_vals.download(coarse.values.data(), coarse.values.size(), s);
     SF_HIP(hipStreamSynchronize(s));
+    coarse_vals_ = std::make_shared<DeviceValues>(ctx_, out_vals, blocks, (size_t)(MODES6 * MODES6));
     // the level is done: its device copies go
     ap_starts_.release();
     ap_cols_.release();

@@ -18,7 +18,7 @@ class DeviceSetup final : public SetupAccelerator {
     ~DeviceSetup() override;
 
     bool takes(int level, const HostBsr<double> &a) const override;
-    void begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal) override;
+    void begin(int level, const HostBsr<double> &a, const std::vector<double> &inverse_diagonal, const double *device_values) override;
     double spectral(std::vector<double> &power, int iterations) override;
     bool smooth(const std::vector<u32> &aggregate, u32 count, const std::vector<int64_t> &ns, const std::vector<u32> &nc,
                 const std::vector<double> &tentative, double omega, HostBsr<double> &p) override;
@@ -26,6 +26,7 @@ class DeviceSetup final : public SetupAccelerator {
     bool galerkin(u32 count, const std::vector<int64_t> &pcol_starts, const std::vector<std::pair<u32, int64_t>> &entries,
                   HostBsr<double> &coarse) override;
     std::shared_ptr<const ValueSource> prolongator_values() override;
+    std::shared_ptr<const ValueSource> coarse_values() override;
     void product_values(HostBsr<double> &ap) override;
     void end_level() override;
 
@@ -46,6 +47,7 @@ class DeviceSetup final : public SetupAccelerator {
     u32 slot_limit_ = 1024;                          // output blocks per row the kernels take
     size_t p_blocks_ = 0;
     std::vector<int64_t> ap_host_starts_;  // the pattern of the last A P
+    std::shared_ptr<const ValueSource> coarse_vals_;  // the last Galerkin product's values
     const double *a_vals_ptr_ = nullptr;
     std::vector<double> host_y_;
 };
