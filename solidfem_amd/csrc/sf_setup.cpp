@@ -269,13 +269,16 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     });
                     if (!done) return false;
                     diagonal_blocks_ = std::vector<double>();
+                    if (held && !(current_.dofs() <= options_.coarse_dofs || (size_t)level + 1 >= options_.max_levels))
+                        // the join's norms come over while the host finds the strong neighbours
+                        norms_job_ = std::async(std::launch::async, [this] { return held_block_norms(); });
                     if (accelerator_ && accelerator_->takes(level, current_)) {
                         // the power iteration needs only A and the block inverses: it runs on the
                         // accelerator while the host aggregates
-                        init_power();
                         on_device_ = true;
                         const double *held = current_.values.empty() && source_ ? source_->device() : nullptr;
                         spectral_job_ = std::async(std::launch::async, [this, level, held] {
+                            init_power(false);
                             accelerator_->begin(level, current_, inverse_diagonal_, held);
                             return accelerator_->spectral(power_, (int)POWER_ITERATIONS);
                         });
@@ -294,8 +297,11 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     // setup_aggregation.rs:12-43, aggregate.rs:53-69
                     if (cursor_ == 0) closest_.clear();
                     bool done = run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
-                        std::vector<RowOut> out(l - f);
-                        std::vector<double> close(l - f);
+                        // each row's closest neighbour and strong count, then the strong columns
+                        // written in place at their row's offset
+                        const size_t first_row = closest_.size();
+                        closest_.resize(first_row + (l - f));
+                        strong_starts_.resize(first_row + 1 + (l - f));
                         parallel_for(l - f, 512, [&](size_t a, size_t b) {
                             for (size_t i = a; i < b; ++i) {
                                 const size_t row = f + i;
@@ -305,18 +311,28 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                                     if (col != row) c = std::min(c, dist2(positions_, row, col));
                                 }
                                 const double limit = reach_ * reach_ * c;
+                                int64_t strong = 0;
                                 for (int64_t at = current_.starts[row]; at < current_.starts[row + 1]; ++at) {
                                     size_t col = current_.columns[(size_t)at];
-                                    if (col != row && dist2(positions_, row, col) <= limit) out[i].cols.push_back((u32)col);
+                                    if (col != row && dist2(positions_, row, col) <= limit) ++strong;
                                 }
-                                close[i] = c;
+                                closest_[first_row + i] = c;
+                                strong_starts_[first_row + 1 + i] = strong;
                             }
                         });
-                        for (size_t i = 0; i < out.size(); ++i) {
-                            strong_columns_.insert(strong_columns_.end(), out[i].cols.begin(), out[i].cols.end());
-                            strong_starts_.push_back((int64_t)strong_columns_.size());
-                            closest_.push_back(close[i]);
-                        }
+                        for (size_t i = 0; i < l - f; ++i) strong_starts_[first_row + 1 + i] += strong_starts_[first_row + i];
+                        strong_columns_.resize((size_t)strong_starts_[first_row + (l - f)]);
+                        parallel_for(l - f, 512, [&](size_t a, size_t b) {
+                            for (size_t i = a; i < b; ++i) {
+                                const size_t row = f + i;
+                                const double limit = reach_ * reach_ * closest_[first_row + i];
+                                size_t to = (size_t)strong_starts_[first_row + i];
+                                for (int64_t at = current_.starts[row]; at < current_.starts[row + 1]; ++at) {
+                                    size_t col = current_.columns[(size_t)at];
+                                    if (col != row && dist2(positions_, row, col) <= limit) strong_columns_[to++] = (u32)col;
+                                }
+                            }
+                        });
                     });
                     if (!done) return false;
                     aggregate_.assign(rows, NONE);
@@ -353,14 +369,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     // reads only seeded rows' aggregates, which this pass never writes, and writes
                     // only its own, so the rows run in parallel with the serial pass's result.
                     const bool held = source_ && current_.values.empty() && current_.blocks() > 0;
-                    if (held && cursor_ == 0) {
-                        // the same norms from the sums of squares summed on the device in order
-                        block_norms_.resize(current_.blocks());
-                        source_->block_squares(block_norms_.data());
-                        parallel_for(block_norms_.size(), 65536, [&](size_t a, size_t b) {
-                            for (size_t i = a; i < b; ++i) block_norms_[i] = std::sqrt(block_norms_[i]);
-                        });
-                    }
+                    if (held && cursor_ == 0) block_norms_ = norms_job_.valid() ? norms_job_.get() : held_block_norms();
                     bool done = run_rows(rows, budget, cancel, true, [&](size_t f, size_t l, bool) {
                         parallel_for(l - f, 1024, [&](size_t a, size_t b) {
                             for (size_t row = f + a; row < f + b; ++row) {
@@ -489,7 +498,7 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                 case Phase::Spectral: {
                     // setup.rs:390-449
                     const size_t dofs = current_.dofs();
-                    if (power_done_ == 0 && power_.size() != dofs) init_power();
+                    if (power_done_ == 0 && !spectral_job_.valid() && power_.size() != dofs) init_power(true);
                     if (power_done_ == 0 && spectral_job_.valid()) {
                         // the iteration started beside the aggregation, charged as the host's
                         if (budget == 0) return false;
@@ -813,7 +822,17 @@ void MultigridSetup::release_accelerator_level() {
     on_device_ = device_p_ = device_ap_ = false;
 }
 
-void MultigridSetup::init_power() {
+std::vector<double> MultigridSetup::held_block_norms() const {
+    // the host's Frobenius norms from the sums of squares summed on the device in its order
+    std::vector<double> norms(current_.blocks());
+    source_->block_squares(norms.data());
+    parallel_for(norms.size(), 65536, [&](size_t a, size_t b) {
+        for (size_t i = a; i < b; ++i) norms[i] = std::sqrt(norms[i]);
+    });
+    return norms;
+}
+
+void MultigridSetup::init_power(bool host_iteration) {
     const size_t dofs = current_.dofs();
     power_.resize(dofs);
     for (size_t i = 0; i < dofs; ++i) {
@@ -824,8 +843,10 @@ void MultigridSetup::init_power() {
     for (double v : power_) s += v * v;
     const double norm = std::sqrt(s);
     for (double &v : power_) v /= norm;
-    scratch_.assign(dofs, 0.0);
-    scratch2_.assign(dofs, 0.0);
+    if (host_iteration) {
+        scratch_.assign(dofs, 0.0);
+        scratch2_.assign(dofs, 0.0);
+    }
 }
 
 void MultigridSetup::ensure_values() {

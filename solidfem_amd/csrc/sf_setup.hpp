@@ -110,7 +110,10 @@ class MultigridSetup {
     void finish_level();
     void push_final_level();
     void spectral_iteration();
-    void init_power();  // the normalised start vector (block.rs:379-382)
+    // The normalised start vector (block.rs:379-382), and the host iteration's work vectors.
+    void init_power(bool host_iteration);
+    // Every block's Frobenius norm, for the join, when the values are held elsewhere.
+    std::vector<double> held_block_norms() const;
     // Brings the current level's values, or its prolongator's, to the host when they are held
     // elsewhere.
     void ensure_values();
@@ -167,6 +170,7 @@ class MultigridSetup {
     // known and run beside the aggregation; the Spectral phase collects it. Declared last, so
     // that it is waited for before anything it reads goes.
     std::future<double> spectral_job_;
+    std::future<std::vector<double>> norms_job_;  // the join's block norms, fetched early
 };
 
 bool invert_spd(const double *input, size_t n, double *output, double &pivot);
