@@ -13,15 +13,25 @@ using namespace dev;
 
 static inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
-std::initializer_list<int> spmm_candidates(int rb, int cb, size_t value_size) {
-    // Depths 1, 2, 4 of the lane-group kernels, and NODE_FORM + depth for one wavefront per block
-    // row. A lane holds BATCH blocks of rb * cb values: keep that within registers.
-    static const std::initializer_list<int> small = {1, 2, 4, NODE_FORM + 1, NODE_FORM + 2, NODE_FORM + 4},
-                                            large = {1, 2, NODE_FORM + 1, NODE_FORM + 2};
-    return (size_t)rb * cb * value_size > 144 ? large : small;
+std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M) {
+    // Depths 1, 2, 4 of the lane-group kernels, NODE_FORM + depth for one wavefront per block
+    // row, and the forms with two or four columns per lane where M allows. A lane holds BATCH
+    // blocks of rb * cb values and BATCH * cb * CPL entries of x: keep that within registers.
+    const bool large = (size_t)rb * cb * value_size > 144;
+    std::vector<int> c = large ? std::vector<int>{1, 2, NODE_FORM + 1, NODE_FORM + 2}
+                               : std::vector<int>{1, 2, 4, NODE_FORM + 1, NODE_FORM + 2, NODE_FORM + 4};
+    if (M % 2 == 0 && M >= 4) {
+        c.push_back(PAIR_FORM + 1);
+        c.push_back(PAIR_FORM + 2);
+    }
+    if (M % 4 == 0 && M >= 8) {
+        c.push_back(QUAD_FORM + 1);
+        if (!large) c.push_back(QUAD_FORM + 2);
+    }
+    return c;
 }
 
-template <int RB, int CB, int BATCH, typename S, typename V>
+template <int RB, int CB, int BATCH, int CPL, typename S, typename V>
 static void spmm_launch(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, bool node) {
     if (node) {
         const size_t threads = (size_t)a.slices * Shape<RB>::NPW * WAVE;
@@ -29,20 +39,29 @@ static void spmm_launch(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, 
                            x, y, b);
         return;
     }
-    const int lg = columns_log2(M);
+    const int lg = columns_log2((M + CPL - 1) / CPL);
     const size_t threads = (size_t)a.slices * Shape<RB>::NPW << lg;
-    hipLaunchKernelGGL((spmm_kernel<RB, CB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a.view(), lg, M, epi,
+    hipLaunchKernelGGL((spmm_kernel<RB, CB, BATCH, S, V, CPL>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a.view(), lg, M, epi,
                        x, y, b);
+}
+
+template <int RB, int CB, int CPL, typename S, typename V>
+static void spmm_depth(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, int depth, bool node) {
+    switch (depth) {
+        case 1: spmm_launch<RB, CB, 1, CPL>(a, M, x, y, b, epi, s, node); break;
+        case 2: spmm_launch<RB, CB, 2, CPL>(a, M, x, y, b, epi, s, node); break;
+        default: spmm_launch<RB, CB, 4, CPL>(a, M, x, y, b, epi, s, node); break;
+    }
 }
 
 template <int RB, int CB, typename S, typename V>
 static void spmm_shape(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch) {
-    const bool node = batch >= NODE_FORM;
-    switch (node ? batch - NODE_FORM : batch) {
-        case 1: spmm_launch<RB, CB, 1>(a, M, x, y, b, epi, s, node); break;
-        case 2: spmm_launch<RB, CB, 2>(a, M, x, y, b, epi, s, node); break;
-        default: spmm_launch<RB, CB, 4>(a, M, x, y, b, epi, s, node); break;
-    }
+    const int depth = form_depth(batch);
+    int cpl = form_columns(batch);
+    if (M % (uint32_t)cpl) cpl = 1;
+    if (cpl == 4) spmm_depth<RB, CB, 4>(a, M, x, y, b, epi, s, depth, false);
+    else if (cpl == 2) spmm_depth<RB, CB, 2>(a, M, x, y, b, epi, s, depth, false);
+    else spmm_depth<RB, CB, 1>(a, M, x, y, b, epi, s, depth, node_form(batch));
 }
 
 template <typename S, typename V>
@@ -57,7 +76,7 @@ void launch_spmm(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V
             SF_HIP(hipMemsetAsync(tx.ptr, 0, tx.bytes(), s));
             SF_HIP(hipMemsetAsync(ty.ptr, 0, ty.bytes(), s));
             tuned = tune_batch(s, [&](int depth) { launch_spmm<S, V>(a, M, tx.ptr, ty.ptr, ty.ptr, EPI_STORE, s, depth); },
-                               spmm_candidates(a.rb, a.cb, sizeof(S)));
+                               spmm_candidates(a.rb, a.cb, sizeof(S), M));
             SF_HIP(hipStreamSynchronize(s));
         }
         batch = tuned;

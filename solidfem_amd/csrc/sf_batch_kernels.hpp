@@ -32,35 +32,83 @@ __device__ __forceinline__ GroupLane group_lane(int log2g) {
     return {t >> log2g, (int)(t & ((1u << log2g) - 1u))};
 }
 
-// Row sums of one block row for column m: sum[p] = sum over stored blocks k and block columns q
-// of A[p][q] x[(col * CB + q) * M + m], each from zero in the reference order.
-template <int RB, int CB, int BATCH, typename S, typename V>
-__device__ __forceinline__ void rows_product(const Sliced<S> &a, uint32_t slot0, uint32_t len, int j, int m, uint32_t M,
-                                             const V *__restrict__ x, V (&sum)[RB]) {
+// CPL consecutive columns of one row-interleaved vector at p (p aligned to CPL values).
+template <int CPL, typename V>
+__device__ __forceinline__ void load_columns(const V *__restrict__ p, V (&out)[CPL]) {
+    if constexpr (CPL == 1) {
+        out[0] = p[0];
+    } else if constexpr (CPL == 2) {
+        using T = typename std::conditional<std::is_same<V, float>::value, float2, double2>::type;
+        const T t = *reinterpret_cast<const T *>(p);
+        out[0] = t.x;
+        out[1] = t.y;
+    } else {
+        static_assert(CPL == 4, "two or four columns per lane");
+        using T = typename std::conditional<std::is_same<V, float>::value, float4, double2>::type;
+        if constexpr (std::is_same<V, float>::value) {
+            const T t = *reinterpret_cast<const T *>(p);
+            out[0] = t.x;
+            out[1] = t.y;
+            out[2] = t.z;
+            out[3] = t.w;
+        } else {
+            const T t0 = reinterpret_cast<const T *>(p)[0], t1 = reinterpret_cast<const T *>(p)[1];
+            out[0] = t0.x;
+            out[1] = t0.y;
+            out[2] = t1.x;
+            out[3] = t1.y;
+        }
+    }
+}
+
+// Stores CPL consecutive columns at p (aligned as for load_columns).
+template <int CPL, typename V>
+__device__ __forceinline__ void store_columns(V *__restrict__ p, const V (&in)[CPL]) {
+    if constexpr (CPL == 1) {
+        p[0] = in[0];
+    } else if constexpr (CPL == 2) {
+        using T = typename std::conditional<std::is_same<V, float>::value, float2, double2>::type;
+        *reinterpret_cast<T *>(p) = T{in[0], in[1]};
+    } else if constexpr (std::is_same<V, float>::value) {
+        *reinterpret_cast<float4 *>(p) = float4{in[0], in[1], in[2], in[3]};
+    } else {
+        reinterpret_cast<double2 *>(p)[0] = double2{in[0], in[1]};
+        reinterpret_cast<double2 *>(p)[1] = double2{in[2], in[3]};
+    }
+}
+
+// Row sums of one block row for the CPL columns m0 .. m0 + CPL - 1: sum[p][c] = sum over stored
+// blocks k and block columns q of A[p][q] x[(col * CB + q) * M + m0 + c], each from zero in the
+// reference order. A lane reads a block once for all its columns.
+template <int RB, int CB, int BATCH, typename S, typename V, int CPL>
+__device__ __forceinline__ void rows_product_columns(const Sliced<S> &a, uint32_t slot0, uint32_t len, int j, int m0, uint32_t M,
+                                                     const V *__restrict__ x, V (&sum)[RB][CPL]) {
     constexpr int NPW = Shape<RB>::NPW;
 #pragma unroll
-    for (int p = 0; p < RB; ++p) sum[p] = (V)0;
+    for (int p = 0; p < RB; ++p)
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) sum[p][c] = (V)0;
     const uint32_t *cp = a.cols + (size_t)slot0 * NPW + j;
     const S *vp = a.vals + ((size_t)slot0 * WAVE + (size_t)j * RB) * CB;
     const uint32_t full = len - len % BATCH;
     // the next batch's block columns are loaded with this batch's values and x entries, so a
     // batch costs one dependent memory round trip instead of two
-    uint32_t c[BATCH];
+    uint32_t col[BATCH];
     if (full) {
 #pragma unroll
-        for (int b = 0; b < BATCH; ++b) c[b] = cp[(size_t)b * NPW];
+        for (int b = 0; b < BATCH; ++b) col[b] = cp[(size_t)b * NPW];
     }
     for (uint32_t k = 0; k < full; k += BATCH) {
-        uint32_t cn[BATCH];
+        uint32_t next[BATCH];
         if (k + BATCH < full) {
 #pragma unroll
-            for (int b = 0; b < BATCH; ++b) cn[b] = cp[(size_t)(k + BATCH + b) * NPW];
+            for (int b = 0; b < BATCH; ++b) next[b] = cp[(size_t)(k + BATCH + b) * NPW];
         }
-        V xv[BATCH][CB];
+        V xv[BATCH][CB][CPL];
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
 #pragma unroll
-            for (int q = 0; q < CB; ++q) xv[b][q] = x[((size_t)c[b] * CB + q) * M + m];
+            for (int q = 0; q < CB; ++q) load_columns<CPL, V>(x + ((size_t)col[b] * CB + q) * M + m0, xv[b][q]);
         S v[BATCH][RB * CB];
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
@@ -71,43 +119,66 @@ __device__ __forceinline__ void rows_product(const Sliced<S> &a, uint32_t slot0,
 #pragma unroll
             for (int p = 0; p < RB; ++p)
 #pragma unroll
-                for (int q = 0; q < CB; ++q) sum[p] = sum[p] + widen<V>(v[b][p * CB + q]) * xv[b][q];
+                for (int q = 0; q < CB; ++q)
 #pragma unroll
-        for (int b = 0; b < BATCH; ++b) c[b] = cn[b];
+                    for (int c = 0; c < CPL; ++c) sum[p][c] = sum[p][c] + widen<V>(v[b][p * CB + q]) * xv[b][q][c];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) col[b] = next[b];
     }
     for (uint32_t k = full; k < len; ++k) {
-        const uint32_t col = cp[(size_t)k * NPW];
-        V xv[CB];
+        const uint32_t cl = cp[(size_t)k * NPW];
+        V xv[CB][CPL];
 #pragma unroll
-        for (int q = 0; q < CB; ++q) xv[q] = x[((size_t)col * CB + q) * M + m];
+        for (int q = 0; q < CB; ++q) load_columns<CPL, V>(x + ((size_t)cl * CB + q) * M + m0, xv[q]);
         const S *blk = vp + (size_t)k * WAVE * CB;
 #pragma unroll
         for (int p = 0; p < RB; ++p)
 #pragma unroll
-            for (int q = 0; q < CB; ++q) sum[p] = sum[p] + widen<V>(blk[p * CB + q]) * xv[q];
+            for (int q = 0; q < CB; ++q)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) sum[p][c] = sum[p][c] + widen<V>(blk[p * CB + q]) * xv[q][c];
     }
+}
+
+// The same for one column m.
+template <int RB, int CB, int BATCH, typename S, typename V>
+__device__ __forceinline__ void rows_product(const Sliced<S> &a, uint32_t slot0, uint32_t len, int j, int m, uint32_t M,
+                                             const V *__restrict__ x, V (&sum)[RB]) {
+    V s[RB][1];
+    rows_product_columns<RB, CB, BATCH, S, V, 1>(a, slot0, len, j, m, M, x, s);
+#pragma unroll
+    for (int p = 0; p < RB; ++p) sum[p] = s[p][0];
 }
 
 // Y = A X (STORE), Y = B - A X (RESIDUAL) or Y = Y + A X (ADD) for M columns: the batched form
 // of spmv_kernel (products, residuals, restriction over the stored transpose, prolongation).
-template <int RB, int CB, int BATCH, typename S, typename V>
+template <int RB, int CB, int BATCH, typename S, typename V, int CPL>
 __global__ __launch_bounds__(256) void spmm_kernel(Sliced<S> a, int log2g, uint32_t M, int epi, const V *__restrict__ x,
                                                    V *__restrict__ y, const V *__restrict__ b) {
+    // CPL > 1: lane m of the group takes the CPL columns from m * CPL (M a multiple of CPL)
     constexpr int NPW = Shape<RB>::NPW;
     const GroupLane g = group_lane(log2g);
-    if (g.pos >= a.slices * (uint32_t)NPW || (uint32_t)g.m >= M) return;
+    const uint32_t m0 = (uint32_t)g.m * CPL;
+    if (g.pos >= a.slices * (uint32_t)NPW || m0 >= M) return;
     const uint32_t row = a.perm[g.pos];
     if (row == NONE) return;
     const uint32_t slice = g.pos / NPW;
     const int j = (int)(g.pos - slice * NPW);
-    V sum[RB];
-    rows_product<RB, CB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[g.pos], j, g.m, M, x, sum);
+    V sum[RB][CPL];
+    rows_product_columns<RB, CB, BATCH, S, V, CPL>(a, a.slice_start[slice], a.row_len[g.pos], j, (int)m0, M, x, sum);
 #pragma unroll
     for (int p = 0; p < RB; ++p) {
-        const size_t out = ((size_t)row * RB + p) * M + g.m;
-        if (epi == EPI_STORE) y[out] = sum[p];
-        else if (epi == EPI_RESIDUAL) y[out] = b[out] - sum[p];
-        else y[out] = y[out] + sum[p];
+        const size_t out = ((size_t)row * RB + p) * M + m0;
+        V v[CPL];
+        if (epi == EPI_STORE) {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) v[c] = sum[p][c];
+        } else {
+            load_columns<CPL, V>((epi == EPI_RESIDUAL ? b : y) + out, v);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) v[c] = epi == EPI_RESIDUAL ? v[c] - sum[p][c] : v[c] + sum[p][c];
+        }
+        store_columns<CPL, V>(y + out, v);
     }
 }
 
@@ -273,36 +344,55 @@ __global__ __launch_bounds__(256) void cheb_start_batch_kernel(uint32_t rows, ui
 // One later Chebyshev step for M columns, fused as cheb_step_kernel: ad = A d_in;
 // r_out = r_in - ad; z = D^-1 r_out (the lane holds its block row's residuals); d_out = keep d_in
 // + step z; x += d_out; with `sum_into`, sum_into += x (cycle.rs:611-650, :534-536, :655-674).
-template <int RB, int BATCH, typename S, typename V>
+template <int RB, int BATCH, typename S, typename V, int CPL>
 __global__ __launch_bounds__(256) void cheb_step_batch_kernel(Sliced<S> a, int log2g, uint32_t M, const S *__restrict__ dinv,
                                                               const V *__restrict__ d_in, const V *__restrict__ r_in, V keep,
                                                               V step, V *__restrict__ d_out, V *__restrict__ r_out,
                                                               V *__restrict__ x, V *__restrict__ sum_into) {
     constexpr int NPW = Shape<RB>::NPW;
     const GroupLane g = group_lane(log2g);
-    if (g.pos >= a.slices * (uint32_t)NPW || (uint32_t)g.m >= M) return;
+    const uint32_t m0 = (uint32_t)g.m * CPL;
+    if (g.pos >= a.slices * (uint32_t)NPW || m0 >= M) return;
     const uint32_t row = a.perm[g.pos];
     if (row == NONE) return;
     const uint32_t slice = g.pos / NPW;
     const int j = (int)(g.pos - slice * NPW);
-    V ad[RB];
-    rows_product<RB, RB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[g.pos], j, g.m, M, d_in, ad);
-    V r_new[RB];
+    V ad[RB][CPL];
+    rows_product_columns<RB, RB, BATCH, S, V, CPL>(a, a.slice_start[slice], a.row_len[g.pos], j, (int)m0, M, d_in, ad);
+    // the CPL columns of each row entry move as one vector
+    V r_new[RB][CPL];
 #pragma unroll
-    for (int p = 0; p < RB; ++p) r_new[p] = r_in[((size_t)row * RB + p) * M + g.m] - ad[p];
+    for (int p = 0; p < RB; ++p) {
+        V r[CPL];
+        load_columns<CPL, V>(r_in + ((size_t)row * RB + p) * M + m0, r);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) r_new[p][c] = r[c] - ad[p][c];
+    }
 #pragma unroll
     for (int p = 0; p < RB; ++p) {
         const S *drow = dinv + (size_t)row * RB * RB + (size_t)p * RB;
-        V z = (V)0;
+        const size_t i = ((size_t)row * RB + p) * M + m0;
+        V dold[CPL], xold[CPL], dn[CPL], xn[CPL];
+        load_columns<CPL, V>(d_in + i, dold);
+        load_columns<CPL, V>(x + i, xold);
 #pragma unroll
-        for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * r_new[q];
-        const size_t i = ((size_t)row * RB + p) * M + g.m;
-        const V dn = keep * d_in[i] + step * z;
-        const V xn = x[i] + dn;
-        d_out[i] = dn;
-        r_out[i] = r_new[p];
-        x[i] = xn;
-        if (sum_into) sum_into[i] = sum_into[i] + xn;
+        for (int c = 0; c < CPL; ++c) {
+            V z = (V)0;
+#pragma unroll
+            for (int q = 0; q < RB; ++q) z = z + widen<V>(drow[q]) * r_new[q][c];
+            dn[c] = keep * dold[c] + step * z;
+            xn[c] = xold[c] + dn[c];
+        }
+        store_columns<CPL, V>(d_out + i, dn);
+        store_columns<CPL, V>(r_out + i, r_new[p]);
+        store_columns<CPL, V>(x + i, xn);
+        if (sum_into) {
+            V acc[CPL];
+            load_columns<CPL, V>(sum_into + i, acc);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) acc[c] = acc[c] + xn[c];
+            store_columns<CPL, V>(sum_into + i, acc);
+        }
     }
 }
 

@@ -347,17 +347,17 @@ template void launch_long_rows<float, double>(const DeviceBsr<float> &, const do
 template void launch_long_rows<float, float>(const DeviceBsr<float> &, const float *, float *, const float *, int, hipStream_t,
                                              Start<float, float>, int);
 
-int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates, const std::string &what) {
-    static const int forced = [] {
-        const char *e = std::getenv("SOLIDFEM_BATCH");
-        return e ? std::atoi(e) : 0;
-    }();
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates, const std::string &what) {
+    // SOLIDFEM_BATCH=code forces a variant wherever it is a candidate (read at each tuning, so
+    // that the tests can force each batched form in turn)
+    const char *e = std::getenv("SOLIDFEM_BATCH");
+    const int forced = e ? std::atoi(e) : 0;
     for (int c : candidates)
         if (forced > 0 && forced == c) return forced;
     return fastest(s, launch, candidates, what);
 }
 
-int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates, const std::string &what) {
+int fastest(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates, const std::string &what) {
     hipEvent_t e0, e1;
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
@@ -727,7 +727,7 @@ class Hierarchy final : public DeviceMultigrid {
 
     // ---- several columns ----
 
-    template <int RB, int BATCH>
+    template <int RB, int BATCH, int CPL>
     static void cheb_batch_launch(LevelDev<S, V> &d, uint32_t M, const V *d_in, const V *r_in, V keep, V stp, V *d_out,
                                   V *r_out, V *x, V *into, hipStream_t s, bool node) {
         if (node) {
@@ -736,20 +736,29 @@ class Hierarchy final : public DeviceMultigrid {
                                d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
             return;
         }
-        const int lg = columns_log2(M);
+        const int lg = columns_log2((M + CPL - 1) / CPL);
         const size_t threads = (size_t)d.a.slices * Shape<RB>::NPW << lg;
-        hipLaunchKernelGGL((cheb_step_batch_kernel<RB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.view(),
+        hipLaunchKernelGGL((cheb_step_batch_kernel<RB, BATCH, S, V, CPL>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.view(),
                            lg, M, d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
+    }
+    template <int RB, int CPL>
+    static void cheb_batch_depth(LevelDev<S, V> &d, int depth, uint32_t M, const V *d_in, const V *r_in, V keep, V stp,
+                                 V *d_out, V *r_out, V *x, V *into, hipStream_t s, bool node) {
+        switch (depth) {
+            case 1: cheb_batch_launch<RB, 1, CPL>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
+            case 2: cheb_batch_launch<RB, 2, CPL>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
+            default: cheb_batch_launch<RB, 4, CPL>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
+        }
     }
     template <int RB>
     static void cheb_batch_rb(LevelDev<S, V> &d, int batch, uint32_t M, const V *d_in, const V *r_in, V keep, V stp,
                               V *d_out, V *r_out, V *x, V *into, hipStream_t s) {
-        const bool node = batch >= NODE_FORM;
-        switch (node ? batch - NODE_FORM : batch) {
-            case 1: cheb_batch_launch<RB, 1>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
-            case 2: cheb_batch_launch<RB, 2>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
-            default: cheb_batch_launch<RB, 4>(d, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node); break;
-        }
+        const int depth = form_depth(batch);
+        int cpl = form_columns(batch);
+        if (M % (uint32_t)cpl) cpl = 1;
+        if (cpl == 4) cheb_batch_depth<RB, 4>(d, depth, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, false);
+        else if (cpl == 2) cheb_batch_depth<RB, 2>(d, depth, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, false);
+        else cheb_batch_depth<RB, 1>(d, depth, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node_form(batch));
     }
     static void launch_cheb_batch(LevelDev<S, V> &d, int batch, uint32_t M, const V *d_in, const V *r_in, V keep, V stp,
                                   V *d_out, V *r_out, V *x, V *into, hipStream_t s) {
@@ -781,10 +790,10 @@ class Hierarchy final : public DeviceMultigrid {
             for (V *v : vs) SF_HIP(hipMemsetAsync(v, 0, need * sizeof(V), s));
             d.mm_depth[lg][0] = tune_batch(s, [&](int batch) {
                 launch_cheb_batch(d, batch, M, d.md0.ptr, d.mb.ptr, (V)0, (V)0, d.md1.ptr, d.mr0.ptr, d.mx.ptr, nullptr, s);
-            }, spmm_candidates(d.rb, d.rb, sizeof(S)));
+            }, spmm_candidates(d.rb, d.rb, sizeof(S), M));
             d.mm_depth[lg][1] = tune_batch(s, [&](int batch) {
                 launch_spmm<S, V>(d.a, M, d.mx.ptr, d.mres.ptr, d.mb.ptr, EPI_RESIDUAL, s, batch);
-            }, spmm_candidates(d.rb, d.rb, sizeof(S)));
+            }, spmm_candidates(d.rb, d.rb, sizeof(S), M));
             if (d.has_p) {
                 LevelDev<S, V> &c = lv_[l + 1];
                 const size_t cneed = c.n * (size_t)M;
@@ -792,10 +801,10 @@ class Hierarchy final : public DeviceMultigrid {
                 SF_HIP(hipMemsetAsync(c.mx.ptr, 0, cneed * sizeof(V), s));
                 d.mm_depth[lg][2] = tune_batch(s, [&](int batch) {
                     launch_spmm<S, V>(d.r, M, d.mres.ptr, c.mb.ptr, nullptr, EPI_STORE, s, batch);
-                }, spmm_candidates(d.r.rb, d.r.cb, sizeof(S)));
+                }, spmm_candidates(d.r.rb, d.r.cb, sizeof(S), M));
                 d.mm_depth[lg][3] = tune_batch(s, [&](int batch) {
                     launch_spmm<S, V>(d.p, M, c.mx.ptr, d.mpx.ptr, nullptr, EPI_ADD, s, batch);
-                }, spmm_candidates(d.p.rb, d.p.cb, sizeof(S)));
+                }, spmm_candidates(d.p.rb, d.p.cb, sizeof(S), M));
             }
         }
         SF_HIP(hipStreamSynchronize(s));

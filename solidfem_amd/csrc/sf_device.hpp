@@ -83,19 +83,26 @@ void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi
 
 // The batch depth (blocks in flight per lane, one of `candidates`) whose launches
 // `launch(batch)` run fastest; every depth gives the same bits. SOLIDFEM_BATCH forces one.
-int tune_batch(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates = {1, 2, 4, 8},
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates = {1, 2, 4, 8},
                const std::string &what = "");
 // The candidate whose launches `launch(candidate)` run fastest (timed with HIP events on `s`).
-int fastest(hipStream_t s, const std::function<void(int)> &launch, std::initializer_list<int> candidates,
+int fastest(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates,
             const std::string &what = "");
 
 // ---- several right-hand sides ------------------------------------------------------------------
 
 // Columns one batched solve carries: a group of lanes per block row, one lane per column.
 constexpr uint32_t MAX_COLUMNS = 64;
-// A batched kernel's variant code at or above NODE_FORM is the one-wavefront-per-block-row form
-// with depth (code - NODE_FORM); below it, the lane-group form with that depth.
+// A batched kernel's variant code: below NODE_FORM, the lane-group form with that depth (blocks in
+// flight); from NODE_FORM, the one-wavefront-per-block-row form with depth (code - NODE_FORM);
+// from PAIR_FORM and QUAD_FORM, the lane-group form whose lanes take two or four consecutive
+// columns each (for M a multiple of 2 or 4; otherwise the one-column form of the same depth).
 constexpr int NODE_FORM = 16;
+constexpr int PAIR_FORM = 32;
+constexpr int QUAD_FORM = 48;
+inline int form_columns(int code) { return code >= QUAD_FORM ? 4 : code >= PAIR_FORM ? 2 : 1; }
+inline int form_depth(int code) { return code >= QUAD_FORM ? code - QUAD_FORM : code >= PAIR_FORM ? code - PAIR_FORM : code >= NODE_FORM ? code - NODE_FORM : code; }
+inline bool node_form(int code) { return code >= NODE_FORM && code < PAIR_FORM; }
 inline int columns_log2(uint32_t m) {
     int g = 0;
     while ((1u << g) < m) ++g;
@@ -106,7 +113,7 @@ inline int columns_log2(uint32_t m) {
 template <typename S, typename V>
 void launch_spmm(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch = 0);
 // The batch depths worth trying for a matrix's batched product.
-std::initializer_list<int> spmm_candidates(int rb, int cb, size_t value_size);
+std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M);
 // Tunes a matrix's product on scratch vectors of its shape.
 template <typename S, typename V>
 void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s);
