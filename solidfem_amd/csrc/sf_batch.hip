@@ -28,6 +28,10 @@ std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M) 
         c.push_back(QUAD_FORM + 1);
         if (!large) c.push_back(QUAD_FORM + 2);
     }
+    if (M % 8 == 0 && M >= 16 && !large) {
+        c.push_back(OCT_FORM + 1);
+        c.push_back(OCT_FORM + 2);
+    }
     return c;
 }
 
@@ -59,7 +63,8 @@ static void spmm_shape(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, c
     const int depth = form_depth(batch);
     int cpl = form_columns(batch);
     if (M % (uint32_t)cpl) cpl = 1;
-    if (cpl == 4) spmm_depth<RB, CB, 4>(a, M, x, y, b, epi, s, depth, false);
+    if (cpl == 8) spmm_depth<RB, CB, 8>(a, M, x, y, b, epi, s, depth, false);
+    else if (cpl == 4) spmm_depth<RB, CB, 4>(a, M, x, y, b, epi, s, depth, false);
     else if (cpl == 2) spmm_depth<RB, CB, 2>(a, M, x, y, b, epi, s, depth, false);
     else spmm_depth<RB, CB, 1>(a, M, x, y, b, epi, s, depth, node_form(batch));
 }

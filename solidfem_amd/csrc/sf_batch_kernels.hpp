@@ -42,21 +42,23 @@ __device__ __forceinline__ void load_columns(const V *__restrict__ p, V (&out)[C
         const T t = *reinterpret_cast<const T *>(p);
         out[0] = t.x;
         out[1] = t.y;
+    } else if constexpr (std::is_same<V, float>::value) {
+        static_assert(CPL % 4 == 0, "two, four or eight columns per lane");
+#pragma unroll
+        for (int h = 0; h < CPL / 4; ++h) {
+            const float4 t = reinterpret_cast<const float4 *>(p)[h];
+            out[4 * h] = t.x;
+            out[4 * h + 1] = t.y;
+            out[4 * h + 2] = t.z;
+            out[4 * h + 3] = t.w;
+        }
     } else {
-        static_assert(CPL == 4, "two or four columns per lane");
-        using T = typename std::conditional<std::is_same<V, float>::value, float4, double2>::type;
-        if constexpr (std::is_same<V, float>::value) {
-            const T t = *reinterpret_cast<const T *>(p);
-            out[0] = t.x;
-            out[1] = t.y;
-            out[2] = t.z;
-            out[3] = t.w;
-        } else {
-            const T t0 = reinterpret_cast<const T *>(p)[0], t1 = reinterpret_cast<const T *>(p)[1];
-            out[0] = t0.x;
-            out[1] = t0.y;
-            out[2] = t1.x;
-            out[3] = t1.y;
+        static_assert(CPL % 4 == 0, "two, four or eight columns per lane");
+#pragma unroll
+        for (int h = 0; h < CPL / 2; ++h) {
+            const double2 t = reinterpret_cast<const double2 *>(p)[h];
+            out[2 * h] = t.x;
+            out[2 * h + 1] = t.y;
         }
     }
 }
@@ -70,10 +72,11 @@ __device__ __forceinline__ void store_columns(V *__restrict__ p, const V (&in)[C
         using T = typename std::conditional<std::is_same<V, float>::value, float2, double2>::type;
         *reinterpret_cast<T *>(p) = T{in[0], in[1]};
     } else if constexpr (std::is_same<V, float>::value) {
-        *reinterpret_cast<float4 *>(p) = float4{in[0], in[1], in[2], in[3]};
+#pragma unroll
+        for (int h = 0; h < CPL / 4; ++h) reinterpret_cast<float4 *>(p)[h] = float4{in[4 * h], in[4 * h + 1], in[4 * h + 2], in[4 * h + 3]};
     } else {
-        reinterpret_cast<double2 *>(p)[0] = double2{in[0], in[1]};
-        reinterpret_cast<double2 *>(p)[1] = double2{in[2], in[3]};
+#pragma unroll
+        for (int h = 0; h < CPL / 2; ++h) reinterpret_cast<double2 *>(p)[h] = double2{in[2 * h], in[2 * h + 1]};
     }
 }
 

@@ -756,7 +756,8 @@ class Hierarchy final : public DeviceMultigrid {
         const int depth = form_depth(batch);
         int cpl = form_columns(batch);
         if (M % (uint32_t)cpl) cpl = 1;
-        if (cpl == 4) cheb_batch_depth<RB, 4>(d, depth, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, false);
+        if (cpl == 8) cheb_batch_depth<RB, 8>(d, depth, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, false);
+        else if (cpl == 4) cheb_batch_depth<RB, 4>(d, depth, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, false);
         else if (cpl == 2) cheb_batch_depth<RB, 2>(d, depth, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, false);
         else cheb_batch_depth<RB, 1>(d, depth, M, d_in, r_in, keep, stp, d_out, r_out, x, into, s, node_form(batch));
     }

@@ -95,13 +95,17 @@ int fastest(hipStream_t s, const std::function<void(int)> &launch, const std::ve
 constexpr uint32_t MAX_COLUMNS = 64;
 // A batched kernel's variant code: below NODE_FORM, the lane-group form with that depth (blocks in
 // flight); from NODE_FORM, the one-wavefront-per-block-row form with depth (code - NODE_FORM);
-// from PAIR_FORM and QUAD_FORM, the lane-group form whose lanes take two or four consecutive
-// columns each (for M a multiple of 2 or 4; otherwise the one-column form of the same depth).
+// from PAIR_FORM, QUAD_FORM and OCT_FORM, the lane-group form whose lanes take two, four or
+// eight consecutive columns each (for M a multiple of that; otherwise the one-column form of the
+// same depth).
 constexpr int NODE_FORM = 16;
 constexpr int PAIR_FORM = 32;
 constexpr int QUAD_FORM = 48;
-inline int form_columns(int code) { return code >= QUAD_FORM ? 4 : code >= PAIR_FORM ? 2 : 1; }
-inline int form_depth(int code) { return code >= QUAD_FORM ? code - QUAD_FORM : code >= PAIR_FORM ? code - PAIR_FORM : code >= NODE_FORM ? code - NODE_FORM : code; }
+constexpr int OCT_FORM = 64;  // eight columns per lane
+inline int form_columns(int code) { return code >= OCT_FORM ? 8 : code >= QUAD_FORM ? 4 : code >= PAIR_FORM ? 2 : 1; }
+inline int form_depth(int code) {
+    return code >= OCT_FORM ? code - OCT_FORM : code >= QUAD_FORM ? code - QUAD_FORM : code >= PAIR_FORM ? code - PAIR_FORM : code >= NODE_FORM ? code - NODE_FORM : code;
+}
 inline bool node_form(int code) { return code >= NODE_FORM && code < PAIR_FORM; }
 inline int columns_log2(uint32_t m) {
     int g = 0;

@@ -199,20 +199,21 @@ def test_solves_outlive_their_context_handle():
 
 # the batched product forms: lane groups with 1, 2 or 4 blocks in flight, one wavefront per block
 # row, and lanes that take two or four consecutive columns each
-FORMS = {"group-1": 1, "group-4": 4, "node-2": 16 + 2, "pair-1": 32 + 1, "pair-2": 32 + 2, "quad-1": 48 + 1, "quad-2": 48 + 2}
+FORMS = {"group-1": 1, "group-4": 4, "node-2": 16 + 2, "pair-1": 32 + 1, "pair-2": 32 + 2, "quad-1": 48 + 1, "quad-2": 48 + 2,
+         "oct-1": 64 + 1, "oct-2": 64 + 2}
 
 
 @pytest.mark.parametrize("form", list(FORMS))
 @pytest.mark.parametrize("precision", [0, 2], ids=["double", "single"])
 def test_each_batched_form_gives_each_column_its_single_bits(ctx, monkeypatch, form, precision):
     # every product and Chebyshev step of the batched V-cycle forced to one form (where it is a
-    # candidate for the level's block shape): 12 and 8 columns, multiples of two and four, then
-    # 6, which reuses the choices made for 8 (the same group size), so the four-column form
-    # falls back to one column per lane
+    # candidate for the level's block shape): 16 and 8 columns, multiples of two, four and (16)
+    # eight, then 12 and 6, which reuse the choices made for 16 and 8 (the same group sizes), so
+    # the eight- and four-column forms fall back to one column per lane
     monkeypatch.setenv("SOLIDFEM_BATCH", str(FORMS[form]))
     ref = refcpu.Problem(loaded(*FINGERPRINT_PLATE[:2]))
     _, mg = device_multigrid(ctx, ref, precision, 100)
-    for columns in (12, 8, 6):
+    for columns in (16, 8, 12, 6):
         r = np.random.default_rng(columns).standard_normal((columns, ref.free_dofs))
         z = mg.apply_batch(r)
         for m in range(columns):
