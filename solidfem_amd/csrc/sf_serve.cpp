@@ -448,6 +448,63 @@ void Server::serve(ServeSystem &system, const std::shared_ptr<Context> &ctx) {
     }
     take(width, ids, rhs);
     if (ids.empty()) return;
+    // the free-DOF solution of a request as its nodal displacements
+    auto nodal_of = [&](const std::vector<double> &x) {
+        std::vector<double> nodal(3 * mesh.nodes.size(), 0.0);
+        for (size_t node = 0; node < mesh.nodes.size(); ++node)
+            if (numbering.free[node] != FIXED)
+                for (int a = 0; a < 3; ++a) nodal[3 * node + a] = x[3 * (size_t)numbering.free[node] + a];
+        return nodal;
+    };
+    if (ids.size() == 1) {
+        // One request: the single-column solve, whose kernels are several times faster than a
+        // one-column batch's (the same bits); requests that arrive meanwhile wait for the next
+        // round, where they batch together.
+        const sf_pcg_options pcg{options_.tolerance, 0, options_.dot_order, 0};
+        sf_column_progress done{};
+        std::vector<double> x;
+        try {
+            DevicePcg single(ctx, st.system.system, st.mg.mg.get(), rhs[0], pcg);
+            for (;;) {
+                const int outcome = single.step(options_.slice_iterations, Cancel{nullptr, nullptr});
+                if (outcome != SF_RUNNING) break;
+                std::lock_guard<std::mutex> g(mu_);
+                if (stop_) return;
+            }
+            const sf_progress p = single.progress();
+            done.iteration = p.iteration;
+            done.relative_residual = p.relative_residual;
+            done.at_rounding_floor = p.at_rounding_floor;
+            done.outcome = p.outcome;
+            x.resize(n);
+            single.estimate(x.data());
+        } catch (const Error &e) {
+            if (e.status == SF_ERR_HIP) throw;  // the device, not the request: the system's failure path
+            done.outcome = SF_FAILED;
+            done.error = e.status;
+            done.error_iteration = (uint64_t)e.a;
+            done.error_value = e.value;
+        }
+        std::vector<double> nodal;
+        if (done.outcome == SF_CONVERGED) nodal = nodal_of(x);
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = requests_.find(ids[0]);
+        if (it != requests_.end()) {
+            ServeRequest &r = *it->second;
+            r.progress = done;
+            r.free_dofs = n;
+            if (done.outcome == SF_CONVERGED) {
+                r.displacements = std::move(nodal);
+                finish_request(r, SF_SERVE_DONE);
+            } else {
+                r.error = Error(done.error ? done.error : SF_ERR_INTERNAL, "the solve failed", (int64_t)done.error_iteration, 0,
+                                done.error_value);
+                finish_request(r, SF_SERVE_FAILED);
+            }
+        }
+        cv_.notify_all();
+        return;
+    }
     const uint32_t M = (uint32_t)width;
     std::vector<double> columns(n * M, 0.0);
     std::vector<u64> slot(M, 0);
@@ -472,10 +529,7 @@ void Server::serve(ServeSystem &system, const std::shared_ptr<Context> &ctx) {
             std::vector<double> nodal;
             if (progress[m].outcome == SF_CONVERGED) {
                 batch.estimate_column(m, x.data());
-                nodal.assign(3 * mesh.nodes.size(), 0.0);
-                for (size_t node = 0; node < mesh.nodes.size(); ++node)
-                    if (numbering.free[node] != FIXED)
-                        for (int a = 0; a < 3; ++a) nodal[3 * node + a] = x[3 * (size_t)numbering.free[node] + a];
+                nodal = nodal_of(x);
             }
             std::lock_guard<std::mutex> g(mu_);
             auto it = requests_.find(slot[m]);
