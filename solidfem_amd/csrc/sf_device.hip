@@ -430,8 +430,6 @@ struct LevelDev {
     int restrict_rows = 0;  // 0: the sliced restriction; else long_rows_kernel with this many rows per wave (by timing)
     DeviceBsr<S> a_rows;          // coarse levels: A in plain block rows, for the row kernels
     int cheb_rows = 0, residual_rows = 0;  // the same choice for the steps and residuals (by timing)
-    bool fuse_restrict_start = true;       // the coarser level's first start in the restriction
-    bool fuse_post_start = true;           // the post-smoothing's start in the second residual
     DeviceBuffer<S> dinv;
     DeviceBuffer<double> lt;      // coarsest: transposed Cholesky factor
     DeviceBuffer<double> packed;  // coarsest: the factor packed by columns, for the LDS solve
@@ -642,20 +640,16 @@ class Hierarchy final : public DeviceMultigrid {
             LevelDev<S, V> &d = lv_[0];
             const size_t n = d.n;
             if (d.has_p) {
-                // the rounding in, fused with the first start (when faster); the widening out, fused
-                // with the last step
-                if (fuse_convert_start_) {
-                    const unsigned g = grid_for(d.rows, 256);
-                    if (d.rb == 3)
-                        hipLaunchKernelGGL((convert_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr,
-                                           d.dinv.ptr, (V)d.theta, d.d0.ptr, out_.ptr);
-                    else
-                        hipLaunchKernelGGL((convert_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr,
-                                           d.dinv.ptr, (V)d.theta, d.d0.ptr, out_.ptr);
-                } else {
-                    hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, r, in_.ptr);
-                }
-                cycle(0, in_.ptr, out_.ptr, s, fuse_convert_start_, z);
+                // the rounding in, fused with the first start; the widening out, fused with the
+                // last step
+                const unsigned g = grid_for(d.rows, 256);
+                if (d.rb == 3)
+                    hipLaunchKernelGGL((convert_start_kernel<3, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr,
+                                       d.dinv.ptr, (V)d.theta, d.d0.ptr, out_.ptr);
+                else
+                    hipLaunchKernelGGL((convert_start_kernel<6, S, V>), dim3(g), dim3(256), 0, s, (uint32_t)d.rows, r, in_.ptr,
+                                       d.dinv.ptr, (V)d.theta, d.d0.ptr, out_.ptr);
+                cycle(0, in_.ptr, out_.ptr, s, true, z);
             } else {
                 hipLaunchKernelGGL((convert_kernel<V, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, r, in_.ptr);
                 cycle(0, in_.ptr, out_.ptr, s);
@@ -1010,20 +1004,16 @@ class Hierarchy final : public DeviceMultigrid {
         int token = finest ? Profiler::get().begin(2, s, res_bytes) : -1;
         residual(d, x, b, s);
         Profiler::get().end(2, token, s);
-        // restriction, with the coarser level's first start when its cycle smooths first (and the
-        // fused form measured faster)
-        const bool fuse = smooths_first(c) && d.fuse_restrict_start;
+        // restriction, with the coarser level's first start when its cycle smooths first
+        const bool fuse = smooths_first(c);
         restrict_to(d, c, s, fuse);
         cycle(l + 1, c.b.ptr, c.x.ptr, s, fuse);
         launch_spmv<S, V>(d.p, c.x.ptr, x, nullptr, EPI_ADD, s);
-        // the residual for the post-smoothing, with its start when fused
-        const bool fuse_post = d.fuse_post_start;
-        token = finest ? Profiler::get().begin(2, s, res_bytes + (fuse_post ? (double)d.rows * d.rb * d.rb * sizeof(S) + 2.0 * d.n * sizeof(V) : 0.0))
-                       : -1;
-        if (fuse_post) residual(d, x, b, s, start_of(d, d.px.ptr));
-        else residual(d, x, b, s);
+        // the residual for the post-smoothing, with its start
+        token = finest ? Profiler::get().begin(2, s, res_bytes + (double)d.rows * d.rb * d.rb * sizeof(S) + 2.0 * d.n * sizeof(V)) : -1;
+        residual(d, x, b, s, start_of(d, d.px.ptr));
         Profiler::get().end(2, token, s);
-        smooth(d, d.res.ptr, d.px.ptr, x, s, finest, fuse_post, widen);
+        smooth(d, d.res.ptr, d.px.ptr, x, s, finest, true, widen);
     }
 
     void compute_stats() {
@@ -1075,7 +1065,6 @@ class Hierarchy final : public DeviceMultigrid {
     std::vector<Graph> graphs_;
     std::vector<LevelDev<S, V>> lv_;
     DeviceBuffer<V> in_, out_;
-    bool fuse_convert_start_ = true;  // single precision: the first start in the rounding of the residual
     DeviceBuffer<V> min_, mout_;  // the finest level's columns in V, for the single precision
 };
 
