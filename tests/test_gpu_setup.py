@@ -141,3 +141,30 @@ def test_device_phases_that_overflow_take_the_host_path_with_the_same_bits(ctx, 
             a, b = mg.level_matrix(level, prolongator), host.level_matrix(level, prolongator)
             assert np.array_equal(a["columns"], b["columns"]) and np.array_equal(bits(a["values"]), bits(b["values"]))
         assert np.array_equal(bits(mg.level_inverse_diagonal(level)), bits(host.level_inverse_diagonal(level)))
+
+
+@pytest.mark.parametrize("polls", [1, 3, 40])
+def test_a_device_setup_cancelled_midway_refuses_further_steps_and_a_new_one_matches(ctx, polls):
+    # setup.rs:278-284: the cancel closure is polled before every setup chunk. Cancelled after a
+    # few polls, while the level's power iteration may still be running beside the aggregation,
+    # the setup refuses further steps by name and can be dropped; a new setup gives the bits.
+    w, coarse = WORKLOADS["jittered-plate"]
+    ref = refcpu.Problem(w)
+    rs, cols, vals = ref.csr()
+    csr = sf.CsrMatrix.new(None, rs, cols, vals)
+    options = sf.MultigridOptions(coarse_dofs=coarse, precision=sf.Precision.DOUBLE, setup_device=sf.SetupDevice.DEVICE)
+    setup = sf.MultigridSetup.new(ctx, csr, ref.rigid_modes(), ref.free_positions(), options)
+    count = [0]
+
+    def cancel():
+        count[0] += 1
+        return count[0] > polls
+
+    with pytest.raises(sf.MultigridError) as e:
+        while not setup.step(97, cancel=cancel):
+            pass
+    assert e.value.status == 23  # SF_ERR_MG_CANCELLED
+    with pytest.raises(sf.MultigridError):
+        setup.step(2 ** 62)
+    setup.close()
+    assert_same_setup(run_setup(ctx, ref, options), run_setup(None, ref, options))
