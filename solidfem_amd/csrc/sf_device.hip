@@ -1109,13 +1109,6 @@ void DeviceSystem::build_from_device_values(hipStream_t s) {
     build_sliced(sliced, plan, host.cols, values.ptr, s);
     SF_HIP(hipStreamSynchronize(s));
     clock.lap("system layout build");
-    // SOLIDFEM_STREAM_SYSTEM=1: the conjugate gradient's product reads the system matrix with
-    // non-temporal loads (the same values, the same bits)
-    static const bool stream = [] {
-        const char *e = std::getenv("SOLIDFEM_STREAM_SYSTEM");
-        return e && std::atoi(e) != 0;
-    }();
-    sliced.streaming = stream;
     tune_spmv<double, double>(sliced, EPI_STORE, s);
     clock.lap("system product tuning");
 }

@@ -35,12 +35,11 @@ struct DeviceSliced {
     int batch_cheb = 4;
     mutable int batch_mm[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // batched product, per log2 of the column group
     mutable uint32_t passes = 0;  // launches so far, for the alternating traversal
-    bool streaming = false;       // read the values non-temporally (SOLIDFEM_STREAM_SYSTEM for the system matrix)
     // The view one launch takes; each call is one pass (see dev::slice_of for the orders).
     dev::Sliced<T> view() const {
         uint32_t order = xcd_order();
         if (ping_pong() && (passes++ & 1u)) order |= dev::ORDER_REVERSE;
-        return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices, order, streaming ? 1u : 0u};
+        return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices, order};
     }
     static uint32_t xcd_order();  // SOLIDFEM_XCD_ORDER: 0 none, 1 contiguous runs, 2 whole windows
     static bool ping_pong();      // SOLIDFEM_PING_PONG: alternate passes walk the slices backwards

@@ -45,8 +45,6 @@ struct Sliced {
     const T *vals;                // per block slot, block column and lane
     uint32_t slices;
     uint32_t order;               // workgroup -> slice order: ORDER_* bits (slice_of)
-    uint32_t stream;              // values read with non-temporal loads (a matrix read once per pass
-                                  // that should not displace the hierarchy's from the caches)
 };
 
 // Workgroup-to-slice orders; none changes a bit of any result, only which workgroups run
@@ -88,7 +86,7 @@ __device__ __forceinline__ uint32_t slice_of(uint32_t order) {
 // Sum of one lane's row of A x, from zero in stored order: the row product of the reference.
 // The column indices of the next batch are loaded with the current batch's values, so each
 // batch costs one dependent memory round trip.
-template <int RB, int CB, int BATCH, typename S, typename V, bool NT = false>
+template <int RB, int CB, int BATCH, typename S, typename V>
 __device__ __forceinline__ V row_product(const Sliced<S> &a, uint32_t slot0, uint32_t len, int j, int lane,
                                          const V *__restrict__ x) {
     constexpr int NPW = Shape<RB>::NPW;
@@ -108,11 +106,7 @@ __device__ __forceinline__ V row_product(const Sliced<S> &a, uint32_t slot0, uin
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
 #pragma unroll
-            for (int q = 0; q < CB; ++q) {
-                const S *at = &vp[(size_t)(k + b) * WAVE * CB + q];
-                if constexpr (NT) v[b][q] = __builtin_nontemporal_load(at);
-                else v[b][q] = *at;
-            }
+            for (int q = 0; q < CB; ++q) v[b][q] = vp[(size_t)(k + b) * WAVE * CB + q];
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
 #pragma unroll
@@ -169,8 +163,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(Sliced<S> a, const V *__restr
     const size_t out = live ? (size_t)row * RB + p : 0;
     V value = (V)0;
     if (live) {
-        const V sum = a.stream ? row_product<RB, CB, BATCH, S, V, true>(a, a.slice_start[slice], a.row_len[pos], j, lane, x)
-                               : row_product<RB, CB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, x);
+        const V sum = row_product<RB, CB, BATCH, S, V>(a, a.slice_start[slice], a.row_len[pos], j, lane, x);
         if (EPI == EPI_STORE) value = sum;
         else if (EPI == EPI_RESIDUAL) value = b[out] - sum;
         else value = y[out] + sum;
