@@ -255,8 +255,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate of the serving measurement, per s")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
-              "independent solves shard one per GPU, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU: there is no multi-GPU path (asked for {args.gpus}); "
+              "independent solves and load cases shard one per GPU with no communication", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
