@@ -22,7 +22,8 @@ constexpr size_t POWER_ITERATIONS = 20;     // setup.rs:41
 constexpr double SPECTRAL_MARGIN = 1.15;    // setup.rs:45
 constexpr double RANK_TOLERANCE = 1e-10;    // transfer.rs:12
 constexpr u32 NONE = 0xffffffffu;
-constexpr size_t CHUNK = 4096;              // rows between cancel polls
+constexpr size_t CHUNK = 4096;              // rows between cancel polls in the serial phases
+constexpr size_t PARALLEL_CHUNK = 32768;    // and in the row-parallel ones (fewer pool dispatches)
 
 // C += A B for row-major dense blocks (block.rs:229-245).
 inline void gemm_acc(double *c, const double *a, size_t m, size_t k, const double *b, size_t n) {
@@ -161,7 +162,7 @@ bool MultigridSetup::run_rows(size_t rows, size_t &budget, const Cancel &cancel,
             cancelled_ = true;
             throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
         }
-        size_t n = std::min({rows - cursor_, budget, CHUNK});
+        size_t n = std::min({rows - cursor_, budget, parallel ? PARALLEL_CHUNK : CHUNK});
         chunk(cursor_, cursor_ + n, parallel);
         cursor_ += n;
         budget -= n;
