@@ -371,6 +371,7 @@ sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_c
         hipStream_t s = b->ctx->stream;
         const size_t total = b->model.mesh.tets.size();
         const double t0 = seconds();
+        StageClock clock;
         auto &sys = b->study->system.system;
         // study.rs:483-500: element stiffness in element order; the cancel check is read before
         // each chunk of elements
@@ -384,13 +385,10 @@ sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_c
         }
         SF_HIP(hipStreamSynchronize(s));
         b->assembly_seconds += seconds() - t0;
-        StageClock clock;
-        clock.lap("element stiffness");
+        clock.lap("element stiffness and scatter (device)");
         if (b->next == total) {
             if (!b->prescribed_applied) {
                 b->assembly.check_fault(s);
-                SF_HIP(hipStreamSynchronize(s));
-                clock.lap("element stiffness kernels");
                 if (!b->problem.prescribed.empty()) {
                     // study.rs:567-593: symmetric elimination on the scalar entries of a host copy
                     sys.host.values.resize(sys.host.blocks() * 9);
