@@ -35,6 +35,7 @@ enum {
     SF_ERR_HIP = 2,
     SF_ERR_OUT_OF_MEMORY = 3,
     SF_ERR_INTERNAL = 4,
+    SF_ERR_GRAPH_UNDER_TRACER = 5,       /* sf_multigrid_set_graph under rocprofv3 */
     /* PcgError, crates/solidfem-solve/src/pcg.rs:59-77 */
     SF_ERR_PCG_NOT_POSITIVE_DEFINITE = 10, /* payload: a=iteration, value=curvature */
     SF_ERR_PCG_NOT_CONVERGED = 11,         /* payload: a=iterations, value=relative residual */
@@ -424,7 +425,8 @@ sf_status sf_multigrid_with_precision(const sf_multigrid *mg, int32_t precision,
 /* Launch option, no reference counterpart: with enable set, sf_multigrid_apply and the solves
  * using this hierarchy replay a HIP graph of one V-cycle's launches, captured on first use for
  * each pair of input and output buffers, instead of launching the kernels one by one. Results are
- * the same bits. Off by default; never used while kernel timing (sf_profile_enable) is on. */
+ * the same bits. Off by default; never used while kernel timing (sf_profile_enable) is on.
+ * Enabling it is refused with SF_ERR_GRAPH_UNDER_TRACER while rocprofv3 traces the process. */
 sf_status sf_multigrid_set_graph(sf_multigrid *mg, int32_t enable);
 /* Algorithmic HBM bytes of one V-cycle of this hierarchy (every array a kernel touches, once). */
 double sf_multigrid_cycle_bytes(const sf_multigrid *mg);
@@ -529,6 +531,18 @@ sf_status sf_server_release(sf_server *server, uint64_t id);
 sf_status sf_server_stats_get(sf_server *server, sf_server_stats *out);
 /* Stops the threads; requests still open fail. */
 void sf_server_destroy(sf_server *server);
+
+/* Tuning and fallback hooks, no reference counterpart (for tests and measurement). Both replace
+ * environment variables, which a caller could only change with setenv while library threads may
+ * read the environment (undefined behaviour in glibc); the environment is read once, as the
+ * starting value.
+ * sf_force_variant: every later tuning of a batch depth or batched form that has `code` among its
+ *   candidates takes it (0: time the candidates, the default; SOLIDFEM_BATCH). Bits never depend
+ *   on the variant.
+ * sf_setup_slot_limit: device setup rows of more than `blocks` output blocks take the host path
+ *   (0: the kernels' own limit of 1024; SOLIDFEM_SETUP_SLOT_LIMIT). Read when a setup begins. */
+sf_status sf_force_variant(int32_t code);
+sf_status sf_setup_slot_limit(uint32_t blocks);
 
 /* Kernel launch timing with HIP events on the launching stream. Kernels: 0 = the fused
  * Chebyshev step on the finest level, 1 = the conjugate gradient's system product A p, 2 = the

@@ -918,6 +918,16 @@ class Kernel(enum.IntEnum):
     FINE_RESIDUAL = 2
 
 
+def force_variant(code: int = 0):
+    """Every later tuning that has `code` among its candidates takes it (0: time them)."""
+    _raise(library().sf_force_variant(int(code)))
+
+
+def setup_slot_limit(blocks: int = 0):
+    """Device setup rows of more than `blocks` output blocks take the host path (0: no extra limit)."""
+    _raise(library().sf_setup_slot_limit(int(blocks)))
+
+
 def profile_enable(kernel: Kernel, enable: bool = True):
     _raise(library().sf_profile_enable(int(kernel), 1 if enable else 0))
 

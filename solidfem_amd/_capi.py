@@ -199,6 +199,8 @@ SIGNATURES = {
     "sf_multigrid_with_precision": (i32, [vp, i32, i32, P(vp)]),
     "sf_multigrid_cycle_bytes": (f64, [vp]),
     "sf_pcg_restart": (i32, [vp, P(f64)]),
+    "sf_force_variant": (i32, [i32]),
+    "sf_setup_slot_limit": (i32, [u32]),
     "sf_profile_enable": (i32, [i32, i32]),
     "sf_profile_reset": (i32, []),
     "sf_profile_read": (i32, [i32, P(u64), P(f64), P(f64)]),

@@ -871,6 +871,11 @@ void sf_server_destroy(sf_server *server) { delete server; }
 
 sf_status sf_multigrid_set_graph(sf_multigrid *mg, int32_t enable) {
     return guard([&] {
+        // rocprofv3 sets ROCPROF_OUTPUT_PATH in the environment of the program it traces. A bench
+        // run replaying captured V-cycles under its kernel trace once died inside the HIP
+        // runtime's graph launch; replays are refused under the tracer rather than risked.
+        if (enable && std::getenv("ROCPROF_OUTPUT_PATH"))
+            throw Error(SF_ERR_GRAPH_UNDER_TRACER, "graph replay is refused while a kernel tracer (rocprofv3) is attached");
         SF_HIP(hipStreamSynchronize(mg->ctx->stream));
         mg->mg->use_graph = enable != 0;
     });
@@ -1016,6 +1021,16 @@ double sf_multigrid_cycle_bytes(const sf_multigrid *mg) { return mg->mg->cycle_b
 
 sf_status sf_pcg_restart(sf_pcg *pcg, const double *new_rhs) {
     return guard([&] { pcg->pcg->restart(new_rhs); });
+}
+
+sf_status sf_force_variant(int32_t code) {
+    return guard([&] {
+        if (code < 0) throw Error(SF_ERR_INVALID_ARGUMENT, "a variant code is not negative");
+        forced_variant().store(code);
+    });
+}
+sf_status sf_setup_slot_limit(uint32_t blocks) {
+    return guard([&] { setup_slot_limit().store(blocks); });
 }
 
 sf_status sf_profile_enable(int32_t kernel, int32_t enable) {

@@ -347,11 +347,19 @@ template void launch_long_rows<float, double>(const DeviceBsr<float> &, const do
 template void launch_long_rows<float, float>(const DeviceBsr<float> &, const float *, float *, const float *, int, hipStream_t,
                                              Start<float, float>, int);
 
+std::atomic<int> &forced_variant() {
+    // SOLIDFEM_BATCH=code gives the starting value; sf_force_variant changes it. The environment
+    // is read once: a setenv by another thread while this thread reads it is undefined behaviour.
+    static std::atomic<int> code{[] {
+        const char *e = std::getenv("SOLIDFEM_BATCH");
+        return e ? std::atoi(e) : 0;
+    }()};
+    return code;
+}
+
 int tune_batch(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates, const std::string &what) {
-    // SOLIDFEM_BATCH=code forces a variant wherever it is a candidate (read at each tuning, so
-    // that the tests can force each batched form in turn)
-    const char *e = std::getenv("SOLIDFEM_BATCH");
-    const int forced = e ? std::atoi(e) : 0;
+    // a forced code is taken wherever it is a candidate
+    const int forced = forced_variant().load(std::memory_order_relaxed);
     for (int c : candidates)
         if (forced > 0 && forced == c) return forced;
     return fastest(s, launch, candidates, what);
@@ -457,6 +465,14 @@ struct LevelDev {
 template <typename S, typename V>
 class Hierarchy final : public DeviceMultigrid {
    public:
+    struct Graph {
+        const double *r;
+        double *z;
+        hipGraphExec_t exec;
+        hipEvent_t done;  // recorded after every replay of this graph
+    };
+    std::vector<Graph> graphs_;
+
     Hierarchy(std::vector<HostLevel> levels, size_t deg, int prec, int coarse, hipStream_t s, const DeviceSliced<double> *finest) {
         precision = prec;
         coarse_mode = coarse;
@@ -603,42 +619,63 @@ class Hierarchy final : public DeviceMultigrid {
     }
 
     ~Hierarchy() override {
-        if (!graphs_.empty()) (void)hipDeviceSynchronize();  // a replay may still run on any stream
-        for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
+        for (auto &g : graphs_) drop_graph(g);
     }
 
     void apply(const double *r, double *z, hipStream_t s) override {
         if (lv_.empty()) return;
         if (use_graph && !Profiler::get().any_on()) {
-            for (const auto &g : graphs_)
-                if (g.r == r && g.z == z && g.stream == s) {
+            // The captured launches hold the hierarchy's own buffers, which never move after
+            // construction, and r and z, hence the key. An executable graph may be launched on
+            // any stream; each replay records the entry's own event, so eviction and teardown
+            // wait on that event and never on a caller's stream, which may be gone by then.
+            for (auto &g : graphs_)
+                if (g.r == r && g.z == z) {
                     SF_HIP(hipGraphLaunch(g.exec, s));
+                    SF_HIP(hipEventRecord(g.done, s));
                     return;
                 }
-            if (graphs_.size() >= 8) {  // a bounded cache: the oldest goes once its stream is idle
-                SF_HIP(hipStreamSynchronize(graphs_.front().stream));
-                SF_HIP(hipGraphExecDestroy(graphs_.front().exec));
+            if (graphs_.size() >= 8) {  // a bounded cache: the oldest goes once its last replay ended
+                drop_graph(graphs_.front());
                 graphs_.erase(graphs_.begin());
             }
+            Graph g{r, z, nullptr, nullptr};
+            SF_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
             hipGraph_t graph = nullptr;
-            SF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) {
+                (void)hipEventDestroy(g.done);
+                SF_HIP(e);
+            }
             try {
                 launch_apply(r, z, s);
             } catch (...) {
                 (void)hipStreamEndCapture(s, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
+                (void)hipEventDestroy(g.done);
                 throw;
             }
-            SF_HIP(hipStreamEndCapture(s, &graph));
-            hipGraphExec_t exec = nullptr;
-            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            SF_HIP(e);
-            graphs_.push_back({r, z, s, exec});
-            SF_HIP(hipGraphLaunch(exec, s));
+            e = hipStreamEndCapture(s, &graph);
+            if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                (void)hipEventDestroy(g.done);
+                SF_HIP(e);
+            }
+            graphs_.push_back(g);
+            SF_HIP(hipGraphLaunch(g.exec, s));
+            SF_HIP(hipEventRecord(g.done, s));
             return;
         }
         launch_apply(r, z, s);
+    }
+
+    static void drop_graph(Graph &g) {
+        if (g.done) (void)hipEventSynchronize(g.done);
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.done) (void)hipEventDestroy(g.done);
+        g.exec = nullptr;
+        g.done = nullptr;
     }
 
     void launch_apply(const double *r, double *z, hipStream_t s) {
@@ -1064,13 +1101,6 @@ class Hierarchy final : public DeviceMultigrid {
         cycle_bytes = bytes;
     }
 
-    struct Graph {
-        const double *r;
-        double *z;
-        hipStream_t stream;
-        hipGraphExec_t exec;
-    };
-    std::vector<Graph> graphs_;
     std::vector<LevelDev<S, V>> lv_;
     DeviceBuffer<V> in_, out_;
     DeviceBuffer<V> min_, mout_;  // the finest level's columns in V, for the single precision

@@ -82,6 +82,8 @@ void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi
 
 // The batch depth (blocks in flight per lane, one of `candidates`) whose launches
 // `launch(batch)` run fastest; every depth gives the same bits. SOLIDFEM_BATCH forces one.
+// The tuning variant forced on every tuning that has it among its candidates (0: none).
+std::atomic<int> &forced_variant();
 int tune_batch(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates = {1, 2, 4, 8},
                const std::string &what = "");
 // The candidate whose launches `launch(candidate)` run fastest (timed with HIP events on `s`).

@@ -370,10 +370,10 @@ void Server::solver_loop() {
                 cv_.wait(g);
             }
         }
-        try {
-            serve(*system, ctx);
-        } catch (const Error &e) {
-            // a device or host failure inside a batch fails the requests it held
+        // a device or host failure inside a batch fails the requests it held; anything else
+        // thrown (std::bad_alloc from a host vector, say) does the same as an internal error
+        // rather than leaving the thread and ending the process
+        auto fail_solving = [&](const Error &e) {
             std::lock_guard<std::mutex> g(mu_);
             for (auto &kv : requests_) {
                 ServeRequest &r = *kv.second;
@@ -382,6 +382,17 @@ void Server::solver_loop() {
                     finish_request(r, SF_SERVE_FAILED);
                 }
             }
+        };
+        try {
+            serve(*system, ctx);
+        } catch (const Error &e) {
+            fail_solving(e);
+        } catch (const std::bad_alloc &) {
+            fail_solving(Error(SF_ERR_OUT_OF_MEMORY, "host memory ran out while solving a batch"));
+        } catch (const std::exception &e) {
+            fail_solving(Error(SF_ERR_INTERNAL, std::string("a batch failed: ") + e.what()));
+        } catch (...) {
+            fail_solving(Error(SF_ERR_INTERNAL, "a batch failed"));
         }
         std::lock_guard<std::mutex> g(mu_);
         system->busy = false;

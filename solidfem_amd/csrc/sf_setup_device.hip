@@ -378,12 +378,21 @@ std::vector<int64_t> starts_of(const std::vector<u32> &counts) {
 
 }  // namespace
 
+std::atomic<u32> &setup_slot_limit() {
+    // SOLIDFEM_SETUP_SLOT_LIMIT gives the starting value (read once), sf_setup_slot_limit changes
+    // it; 0 means the kernels' own limit
+    static std::atomic<u32> limit{[] {
+        const char *e = std::getenv("SOLIDFEM_SETUP_SLOT_LIMIT");
+        return e ? (u32)std::max<long>(0, std::atol(e)) : 0u;
+    }()};
+    return limit;
+}
+
 DeviceSetup::DeviceSetup(std::shared_ptr<Context> ctx, size_t min_rows, const double *level0_values)
     : ctx_(std::move(ctx)), min_rows_(min_rows), level0_values_(level0_values) {
-    // SOLIDFEM_SETUP_SLOT_LIMIT: fewer output blocks per row than the kernels hold, so that rows
-    // above it take the host path (the fallback's own bits)
-    if (const char *e = std::getenv("SOLIDFEM_SETUP_SLOT_LIMIT"))
-        slot_limit_ = (u32)std::min<long>(MAX_SLOTS, std::max<long>(1, std::atol(e)));
+    // fewer output blocks per row than the kernels hold, so that rows above it take the host
+    // path (the fallback's own bits)
+    if (const u32 limit = setup_slot_limit().load(std::memory_order_relaxed)) slot_limit_ = std::min<u32>(MAX_SLOTS, limit);
 }
 
 DeviceSetup::~DeviceSetup() {

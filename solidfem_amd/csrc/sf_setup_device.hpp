@@ -55,6 +55,9 @@ class DeviceSetup final : public SetupAccelerator {
 // Block values on the device as a value source: borrowed (a study's assembled values, which the
 // study keeps while the hierarchy holding this source lives) or shared (a prolongator built on
 // the device).
+// Output blocks per row above which a device setup row takes the host path (0: the kernels' limit).
+std::atomic<u32> &setup_slot_limit();
+
 class DeviceValues final : public ValueSource {
    public:
     DeviceValues(std::shared_ptr<Context> ctx, const double *values, size_t blocks, size_t block_len);

@@ -17,3 +17,19 @@ def pytest_configure(config):
 def ctx():
     import solidfem_amd as sf
     return sf.Context(0)
+
+
+@pytest.fixture
+def forced_variant():
+    """Forces a tuning variant through the library (no setenv while library threads run); undone after the test."""
+    import solidfem_amd as sf
+    yield sf.force_variant
+    sf.force_variant(0)
+
+
+@pytest.fixture
+def slot_limit():
+    """Lowers the device setup's per-row block limit through the library; undone after the test."""
+    import solidfem_amd as sf
+    yield sf.setup_slot_limit
+    sf.setup_slot_limit(0)

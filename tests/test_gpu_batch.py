@@ -205,12 +205,12 @@ FORMS = {"group-1": 1, "group-4": 4, "node-2": 16 + 2, "pair-1": 32 + 1, "pair-2
 
 @pytest.mark.parametrize("form", list(FORMS))
 @pytest.mark.parametrize("precision", [0, 2], ids=["double", "single"])
-def test_each_batched_form_gives_each_column_its_single_bits(ctx, monkeypatch, form, precision):
+def test_each_batched_form_gives_each_column_its_single_bits(ctx, forced_variant, form, precision):
     # every product and Chebyshev step of the batched V-cycle forced to one form (where it is a
     # candidate for the level's block shape): 16 and 8 columns, multiples of two, four and (16)
     # eight, then 12 and 6, which reuse the choices made for 16 and 8 (the same group sizes), so
     # the eight- and four-column forms fall back to one column per lane
-    monkeypatch.setenv("SOLIDFEM_BATCH", str(FORMS[form]))
+    forced_variant(FORMS[form])
     ref = refcpu.Problem(loaded(*FINGERPRINT_PLATE[:2]))
     _, mg = device_multigrid(ctx, ref, precision, 100)
     for columns in (16, 8, 12, 6):

@@ -117,14 +117,14 @@ def test_a_device_built_study_solves_to_the_fingerprint(ctx, case):
 
 
 @pytest.mark.parametrize("limit", [4, 8, 16])
-def test_device_phases_that_overflow_take_the_host_path_with_the_same_bits(ctx, monkeypatch, limit):
+def test_device_phases_that_overflow_take_the_host_path_with_the_same_bits(ctx, slot_limit, limit):
     # rows with more output blocks than the limit send their phase to the host. On this
     # cantilever's finest level the prolongator rows hold at most 7 blocks, A P rows 12 and
     # Galerkin rows 19: 4 sends the smoothing (and so A P and Galerkin) to the host, 8 sends A P
     # and Galerkin, 16 the Galerkin product alone. The host then fetches what the device kept (the
     # study's matrix, the prolongator, A P), and the study must still solve to the reference's
     # fingerprint with the host setup's hierarchy
-    monkeypatch.setenv("SOLIDFEM_SETUP_SLOT_LIMIT", str(limit))
+    slot_limit(limit)
     size, div, iterations, fingerprint = FINGERPRINT_CANTILEVER
     w = loaded(size, div)
     options = sf.MultigridOptions(precision=sf.Precision.DOUBLE, coarse_dofs=100, setup_device=sf.SetupDevice.DEVICE)
