@@ -192,7 +192,10 @@ typedef struct sf_build_progress {
 
 /* The structured box problems the benchmark configurations are built from: a box mesh
  * (crates/solidfem-mesh/src/box_mesh.rs:22-80) with optional seeded interior jitter, upgraded to
- * Tet10, clamped on a plane, loaded by tractions on the boundary faces of a plane. */
+ * Tet10, clamped on a plane, loaded by tractions on the boundary faces of a plane. clamp_axis -1
+ * clamps no plane and load_axis -1 loads none (the model may then be held and loaded by
+ * sf_model_fix_node, sf_model_prescribe_displacement and sf_model_add_traction); any other axis
+ * outside 0..2 is refused with SF_ERR_MODEL_AXIS_OUT_OF_RANGE. */
 enum { SF_TRACTION_UNIFORM = 0, SF_TRACTION_RANDOM = 1, SF_TRACTION_TORSION = 2 };
 enum { SF_MATERIAL_UNIFORM = 0, SF_MATERIAL_SHELL_CORE = 1 };
 typedef struct sf_workload {

@@ -92,7 +92,50 @@ struct Problem {
     std::vector<double> rhs;
     CsrMatrix matrix;
     double assembly_seconds = 0;
+    // Model::prescribe_displacement (study.rs:90-99): the map keyed by (node, axis), the last call
+    // winning, then study.rs:661-683's (free dof, value) list sorted by dof
+    std::map<std::pair<u32, u32>, double> prescribed_nodes;
+    std::vector<std::pair<size_t, double>> prescribed;
 };
+
+// study.rs:661-683: each prescribed component as (free dof, value), sorted by dof; refuses an axis
+// above 2, a non-finite value, a node out of range or a fixed node
+static std::vector<std::pair<size_t, double>> prescribed_dofs(const Problem &p) {
+    std::vector<std::pair<size_t, double>> dofs;
+    for (const auto &kv : p.prescribed_nodes) {
+        const u32 node = kv.first.first, axis = kv.first.second;
+        if (axis > 2) throw std::runtime_error("axis out of range");
+        if (!std::isfinite(kv.second)) throw std::runtime_error("non-finite prescribed value");
+        if ((size_t)node >= p.free.size()) throw std::runtime_error("node out of range");
+        if (p.free[node] == FIXED) throw std::runtime_error("prescribed on a fixed node");
+        dofs.push_back({3 * (size_t)p.free[node] + axis, kv.second});
+    }
+    std::stable_sort(dofs.begin(), dofs.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    return dofs;
+}
+
+// study.rs:567-593: symmetric elimination. Each prescribed row's off-diagonal entries (read before
+// this row's own updates) move `entry * value` off the right-hand side of the unconstrained rows
+// and are cancelled in both the row and the column; then the prescribed rows' right-hand sides
+// become diagonal * value.
+static void apply_prescribed(Problem &p) {
+    std::vector<char> constrained(p.rhs.size(), 0);
+    for (const auto &pr : p.prescribed) constrained[pr.first] = 1;
+    for (const auto &pr : p.prescribed) {
+        const size_t dof = pr.first;
+        const double value = pr.second;
+        std::vector<std::pair<u32, double>> entries;
+        for (size_t at = p.matrix.row_starts[dof]; at < p.matrix.row_starts[dof + 1]; ++at)
+            if ((size_t)p.matrix.columns[at] != dof) entries.push_back({p.matrix.columns[at], p.matrix.values[at]});
+        for (const auto &e : entries) {
+            const size_t c = e.first;
+            if (!constrained[c]) p.rhs[c] -= e.second * value;
+            p.matrix.add(c, (u32)dof, -e.second);
+            p.matrix.add(dof, e.first, -e.second);
+        }
+    }
+    for (const auto &pr : p.prescribed) p.rhs[pr.first] = p.matrix.get(pr.first, (u32)pr.first) * pr.second;
+}
 
 // element.rs:98-133
 static bool element_stiffness(const Vec3 nodes[10], const MaterialField &field, int rule, double ke[900]) {
@@ -197,9 +240,11 @@ static std::vector<double> free_rigid_body_modes(const std::vector<Vec3> &nodes,
     return modes;
 }
 
-static std::unique_ptr<Problem> build_problem(const ref_workload &w) {
+static std::unique_ptr<Problem> build_problem(const ref_workload &w, const uint32_t *pnodes = nullptr, const uint32_t *paxes = nullptr,
+                                              const double *pvalues = nullptr, int64_t pcount = 0) {
     double t0 = now_seconds();
     auto p = std::make_unique<Problem>();
+    for (int64_t k = 0; k < pcount; ++k) p->prescribed_nodes[{pnodes[k], paxes[k]}] = pvalues[k];
     LinearMesh linear;
     if (w.wall_cells > 0) {
         // a bar along its longest axis with a hollow section (demo/wasm/src/part.rs:87-185)
@@ -226,12 +271,16 @@ static std::unique_ptr<Problem> build_problem(const ref_workload &w) {
     p->boundary = boundary_faces(p->mesh);
     const auto &nodes = p->mesh.nodes;
     p->is_fixed.assign(nodes.size(), 0);
-    for (size_t n = 0; n < nodes.size(); ++n)
-        if (std::fabs(nodes[n][w.clamp_axis] - w.clamp_value) < 1e-9) p->is_fixed[n] = 1;
-    // The loaded faces: boundary faces whose corners lie on the plane (support.rs:80-89).
+    if (w.clamp_axis >= 0)
+        for (size_t n = 0; n < nodes.size(); ++n)
+            if (std::fabs(nodes[n][w.clamp_axis] - w.clamp_value) < 1e-9) p->is_fixed[n] = 1;
+    // The loaded faces: boundary faces whose corners lie on the plane (support.rs:80-89); none
+    // when load_axis < 0.
     size_t face_index = 0;
+    const bool loads = w.load_axis >= 0;
     int u = (w.load_axis + 1) % 3, v = (w.load_axis + 2) % 3;
     for (const auto &f : p->boundary) {
+        if (!loads) break;
         bool on = true;
         for (int c = 0; c < 3; ++c)
             if (!(std::fabs(nodes[f.nodes[c]][w.load_axis] - w.load_value) < 1e-9)) on = false;
@@ -283,6 +332,7 @@ static std::unique_ptr<Problem> build_problem(const ref_workload &w) {
     for (size_t n = 0; n < nodes.size(); ++n)
         if (p->free[n] != FIXED)
             for (int a = 0; a < 3; ++a) p->rhs.push_back(load[3 * n + a]);
+    p->prescribed = prescribed_dofs(*p);
     // assembly.rs:55-81
     std::vector<std::vector<u32>> mapped;
     mapped.reserve(p->mesh.tets.size());
@@ -312,8 +362,28 @@ static std::unique_ptr<Problem> build_problem(const ref_workload &w) {
             }
         }
     }
+    // study.rs:499-502: the elimination once every element is in
+    if (!p->prescribed.empty()) apply_prescribed(*p);
     p->assembly_seconds = now_seconds() - t0;
     return p;
+}
+
+// study.rs:218-237: every mesh node's displacement, fixed nodes zero, prescribed components exact
+static std::vector<double> nodal_displacements(const Problem &p, const std::vector<double> &x) {
+    const size_t nn = p.mesh.nodes.size();
+    std::vector<double> disp(3 * nn, 0.0);
+    for (size_t n = 0; n < nn; ++n)
+        if (p.free[n] != FIXED)
+            for (int a = 0; a < 3; ++a) disp[3 * n + a] = x[3 * (size_t)p.free[n] + a];
+    for (const auto &kv : p.prescribed_nodes) disp[3 * (size_t)kv.first.first + kv.first.second] = kv.second;
+    return disp;
+}
+
+// study.rs:529-549: the rigid modes as near null space, zero on prescribed components
+static std::vector<double> setup_modes(const Problem &p) {
+    auto modes = free_rigid_body_modes(p.mesh.nodes, p.free, p.free_count);
+    for (const auto &pr : p.prescribed) std::fill(&modes[pr.first * 6], &modes[pr.first * 6] + 6, 0.0);
+    return modes;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -480,6 +550,16 @@ ref_problem *ref_problem_create(const ref_workload *w) {
         return nullptr;
     }
 }
+ref_problem *ref_problem_create_prescribed(const ref_workload *w, const uint32_t *nodes, const uint32_t *axes, const double *values,
+                                           int64_t count) {
+    try {
+        auto rp = std::make_unique<ref_problem>();
+        rp->p = build_problem(*w, nodes, axes, values, count);
+        return rp.release();
+    } catch (...) {
+        return nullptr;
+    }
+}
 void ref_problem_destroy(ref_problem *p) { delete p; }
 
 int64_t ref_problem_node_count(const ref_problem *p) { return (int64_t)p->p->mesh.nodes.size(); }
@@ -525,7 +605,7 @@ void ref_problem_csr(const ref_problem *p, int64_t *row_starts, uint32_t *column
 }
 void ref_problem_rhs(const ref_problem *p, double *out) { std::copy(p->p->rhs.begin(), p->p->rhs.end(), out); }
 void ref_problem_rigid_modes(const ref_problem *p, double *out) {
-    auto modes = free_rigid_body_modes(p->p->mesh.nodes, p->p->free, p->p->free_count);
+    auto modes = setup_modes(*p->p);
     std::copy(modes.begin(), modes.end(), out);
 }
 // Stress recovery of nodal displacements `u` (3 per node), results.rs:307-576: per (element,
@@ -567,7 +647,7 @@ ref_multigrid *ref_multigrid_create(const ref_problem *rp, const ref_mg_options 
     double t0 = now_seconds();
     try {
         // study.rs:530-549
-        auto modes = free_rigid_body_modes(p.mesh.nodes, p.free, p.free_count);
+        auto modes = setup_modes(p);
         std::vector<Vec3> positions;
         positions.reserve(p.free_count);
         for (size_t n = 0; n < p.mesh.nodes.size(); ++n)
@@ -664,12 +744,8 @@ int ref_solve_with(const ref_problem *rp, const ref_multigrid *rm, double tolera
     Pcg<decltype(apply)> pcg(p.matrix, apply, p.rhs, tolerance, 0);
     Status st = pcg.step(iteration_limit ? iteration_limit : ~(u64)0);
     double t1 = now_seconds();
-    // study.rs:218-237: fixed nodes are zero
     size_t nn = p.mesh.nodes.size();
-    std::vector<double> disp(3 * nn, 0.0);
-    for (size_t n = 0; n < nn; ++n)
-        if (p.free[n] != FIXED)
-            for (int a = 0; a < 3; ++a) disp[3 * n + a] = pcg.x[3 * (size_t)p.free[n] + a];
+    std::vector<double> disp = nodal_displacements(p, pcg.x);
     if (displacements) std::copy(disp.begin(), disp.end(), displacements);
     if (result) {
         std::memset(result, 0, sizeof(*result));

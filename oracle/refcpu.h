@@ -98,6 +98,13 @@ void ref_default_mg_options(ref_mg_options *o);
 typedef struct ref_problem ref_problem;
 
 ref_problem *ref_problem_create(const ref_workload *w);
+/* The same with prescribed displacement components (Model::prescribe_displacement,
+ * crates/solidfem-elastic/src/study.rs:90-99; the last call for a node and axis wins), eliminated
+ * symmetrically once assembled (study.rs:567-593); NULL when a component is refused
+ * (study.rs:661-683). A workload with clamp_axis < 0 fixes no node; with load_axis < 0 it loads no
+ * face. */
+ref_problem *ref_problem_create_prescribed(const ref_workload *w, const uint32_t *nodes, const uint32_t *axes,
+                                           const double *values, int64_t count);
 void ref_problem_destroy(ref_problem *p);
 
 int64_t ref_problem_node_count(const ref_problem *p);

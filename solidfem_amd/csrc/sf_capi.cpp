@@ -390,35 +390,11 @@ sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_c
             if (!b->prescribed_applied) {
                 b->assembly.check_fault(s);
                 if (!b->problem.prescribed.empty()) {
-                    // study.rs:567-593: symmetric elimination on the scalar entries of a host copy
+                    // study.rs:567-593 on a host copy of the assembled blocks
                     sys.host.values.resize(sys.host.blocks() * 9);
                     sys.values.download(sys.host.values.data(), sys.host.values.size(), s);
                     SF_HIP(hipStreamSynchronize(s));
-                    auto &h = sys.host;
-                    auto entry = [&](size_t row, size_t col) -> double * {
-                        const size_t br = row / 3, bc = col / 3;
-                        auto first = h.columns.begin() + h.starts[br], last = h.columns.begin() + h.starts[br + 1];
-                        auto it = std::lower_bound(first, last, (u32)bc);
-                        if (it == last || *it != bc) return nullptr;
-                        return &h.values[(size_t)(it - h.columns.begin()) * 9 + (row % 3) * 3 + col % 3];
-                    };
-                    std::vector<char> constrained(sys.size, 0);
-                    for (auto &pr : b->problem.prescribed) constrained[pr.first] = 1;
-                    for (auto &pr : b->problem.prescribed) {
-                        const size_t dof = pr.first, br = dof / 3;
-                        std::vector<std::pair<size_t, double>> entries;
-                        for (int64_t at = h.starts[br]; at < h.starts[br + 1]; ++at)
-                            for (size_t q = 0; q < 3; ++q) {
-                                const size_t col = h.columns[(size_t)at] * 3 + q;
-                                if (col != dof) entries.push_back({col, h.values[(size_t)at * 9 + (dof % 3) * 3 + q]});
-                            }
-                        for (auto &e : entries) {
-                            if (!constrained[e.first]) b->problem.rhs[e.first] -= e.second * pr.second;
-                            *entry(e.first, dof) += -e.second;
-                            *entry(dof, e.first) += -e.second;
-                        }
-                    }
-                    for (auto &pr : b->problem.prescribed) b->problem.rhs[pr.first] = *entry(pr.first, pr.first) * pr.second;
+                    apply_prescribed(sys.host, b->problem.rhs, b->problem.prescribed);
                     sys.values.upload(sys.host.values.data(), sys.host.values.size(), s);
                 }
                 b->prescribed_applied = true;

@@ -136,4 +136,11 @@ struct AssemblyPattern {
 };
 AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free, size_t free_count);
 
+// study.rs:567-593: each prescribed row's off-diagonal entries, read before that row's own updates,
+// move `entry * value` off the right-hand side of every unconstrained row and are cancelled in
+// both the row and its mirror column; then each prescribed row's right-hand side becomes its
+// diagonal times the value. The system stays symmetric positive definite with the prescribed
+// unknowns decoupled.
+void apply_prescribed(HostBsr<double> &a, std::vector<double> &rhs, const std::vector<std::pair<size_t, double>> &prescribed);
+
 }  // namespace sf

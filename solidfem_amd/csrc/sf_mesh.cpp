@@ -337,6 +337,8 @@ static double gaussian(u64 seed, u64 counter) {
 std::vector<Traction> workload_tractions(const QuadMesh &mesh, const std::vector<Face> &boundary, const sf_workload &w) {
     const auto &nodes = mesh.nodes;
     std::vector<Traction> out;
+    if (w.load_axis < 0) return out;  // no loaded plane
+    if (w.load_axis > 2) throw Error(SF_ERR_MODEL_AXIS_OUT_OF_RANGE, "load axis out of range", w.load_axis);
     int u = (w.load_axis + 1) % 3, v = (w.load_axis + 2) % 3;
     u64 index = 0;
     for (const Face &f : boundary) {
@@ -435,6 +437,8 @@ TetMesh make_tube_mesh(const sf_workload &w) {
 
 Model model_from_workload(const sf_workload &w) {
     if (w.wall_cells < 0) throw Error(SF_ERR_INVALID_ARGUMENT, "wall_cells must not be negative");
+    if (w.clamp_axis < -1 || w.clamp_axis > 2) throw Error(SF_ERR_MODEL_AXIS_OUT_OF_RANGE, "clamp axis out of range", w.clamp_axis);
+    if (w.load_axis < -1 || w.load_axis > 2) throw Error(SF_ERR_MODEL_AXIS_OUT_OF_RANGE, "load axis out of range", w.load_axis);
     if (w.wall_cells > 0 && w.jitter_fraction > 0.0) throw Error(SF_ERR_INVALID_ARGUMENT, "a tube is not jittered");
     StageClock clock;
     TetMesh linear = w.wall_cells > 0 ? make_tube_mesh(w) : make_box_mesh(w.origin, w.size, w.divisions);
@@ -462,8 +466,9 @@ Model model_from_workload(const sf_workload &w) {
     model.mesh = upgrade_to_quadratic(linear);
     clock.lap("quadratic upgrade");
     const auto &nodes = model.mesh.nodes;
-    for (size_t n = 0; n < nodes.size(); ++n)
-        if (std::fabs(nodes[n][w.clamp_axis] - w.clamp_value) < 1e-9) model.fixed.insert((u32)n);
+    if (w.clamp_axis >= 0)
+        for (size_t n = 0; n < nodes.size(); ++n)
+            if (std::fabs(nodes[n][w.clamp_axis] - w.clamp_value) < 1e-9) model.fixed.insert((u32)n);
     const auto boundary = model.mesh.boundary_faces();
     clock.lap("boundary faces");
     model.tractions = workload_tractions(model.mesh, boundary, w);
@@ -762,6 +767,41 @@ FreeProblem number_and_load(const Model &model) {
     if (!any) throw Error(SF_ERR_MODEL_NO_EFFECTIVE_LOAD, "the model has no effective load");
     clock.lap("  numbering");
     return p;
+}
+
+void apply_prescribed(HostBsr<double> &a, std::vector<double> &rhs, const std::vector<std::pair<size_t, double>> &prescribed) {
+    if (a.rb != 3 || a.cb != 3 || a.values.size() != a.blocks() * 9 || rhs.size() != a.dofs())
+        throw Error(SF_ERR_INTERNAL, "prescribed elimination needs the assembled 3x3 blocks and a matching right-hand side");
+    // the slot of block (row, col), which the symmetric pattern holds for every stored mirror
+    auto slot = [&](size_t brow, size_t bcol) -> size_t {
+        const u32 *first = a.columns.data() + a.starts[brow], *last = a.columns.data() + a.starts[brow + 1];
+        const u32 *it = std::lower_bound(first, last, (u32)bcol);
+        if (it == last || *it != bcol) throw Error(SF_ERR_INTERNAL, "the stiffness pattern is not symmetric");
+        return (size_t)(it - a.columns.data());
+    };
+    std::vector<char> constrained(rhs.size(), 0);
+    for (const auto &pr : prescribed) constrained[pr.first] = 1;
+    std::vector<std::pair<size_t, double>> row;
+    for (const auto &pr : prescribed) {
+        const size_t dof = pr.first, brow = dof / 3, p = dof % 3;
+        // the row's off-diagonal entries in column order, as they stand before this row's updates
+        row.clear();
+        for (int64_t at = a.starts[brow]; at < a.starts[brow + 1]; ++at)
+            for (size_t q = 0; q < 3; ++q) {
+                const size_t col = (size_t)a.columns[(size_t)at] * 3 + q;
+                if (col != dof) row.push_back({col, a.values[(size_t)at * 9 + p * 3 + q]});
+            }
+        for (const auto &e : row) {
+            const size_t col = e.first;
+            if (!constrained[col]) rhs[col] -= e.second * pr.second;
+            a.values[slot(col / 3, brow) * 9 + (col % 3) * 3 + p] += -e.second;
+            a.values[slot(brow, col / 3) * 9 + p * 3 + col % 3] += -e.second;
+        }
+    }
+    for (const auto &pr : prescribed) {
+        const size_t dof = pr.first;
+        rhs[dof] = a.values[slot(dof / 3, dof / 3) * 9 + (dof % 3) * 4] * pr.second;
+    }
 }
 
 std::vector<double> traction_rhs(const QuadMesh &mesh, const std::vector<Traction> &tractions, const std::vector<u32> &free,

@@ -39,6 +39,7 @@ _SIG = {
     "ref_default_mg_options": (None, [P(RefOptions)]),
     "ref_problem_create": (vp, [P(RefWorkload)]),
     "ref_problem_destroy": (None, [vp]),
+    "ref_problem_create_prescribed": (vp, [P(RefWorkload), P(u32), P(u32), P(f64), i64]),
     "ref_problem_node_count": (i64, [vp]),
     "ref_problem_vertex_count": (i64, [vp]),
     "ref_problem_tet_count": (i64, [vp]),
@@ -113,9 +114,18 @@ def options(precision=1, coarse_dofs=1500, degree=3, reach=3.0, max_levels=12, s
 
 
 class Problem:
-    def __init__(self, w):
+    def __init__(self, w, prescribed=None):
+        """`prescribed`: (node, axis, value) components (Model::prescribe_displacement), eliminated
+        once assembled as crates/solidfem-elastic/src/study.rs:567-593 does."""
         self.w = workload(w)
-        self._p = lib().ref_problem_create(C.byref(self.w))
+        if prescribed:
+            nodes = np.array([c[0] for c in prescribed], dtype=np.uint32)
+            axes = np.array([c[1] for c in prescribed], dtype=np.uint32)
+            values = np.array([c[2] for c in prescribed], dtype=np.float64)
+            self._p = lib().ref_problem_create_prescribed(C.byref(self.w), ptr(nodes, u32), ptr(axes, u32), ptr(values, f64),
+                                                          len(nodes))
+        else:
+            self._p = lib().ref_problem_create(C.byref(self.w))
         if not self._p:
             raise RuntimeError("the oracle refused the workload")
 

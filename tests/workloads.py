@@ -51,3 +51,29 @@ def tube_config(divisions=(50, 6, 6), wall_cells=2, poisson=0.0):
     return Workload(size=(100.0, 10.0, 10.0), divisions=tuple(divisions), clamp_axis=0, clamp_value=0.0, load_axis=0,
                     load_value=100.0, traction=(0.0, 0.0, -1.0), youngs=200000.0, poisson=poisson,
                     wall_cells=wall_cells, wall_thickness=2.0)
+
+
+STRETCH = 0.001
+
+
+def stretched_cube(divisions=(1, 1, 1)):
+    """crates/solidfem/tests/validation/prescribed.rs:16-63: a unit steel cube (E 200 GPa, nu 0.3,
+    4-point rule) of box tetrahedra, no clamp and no load; its held components come from
+    `stretch_components`."""
+    return Workload(size=(1.0, 1.0, 1.0), divisions=tuple(divisions), clamp_axis=-1, load_axis=-1, youngs=200000.0,
+                    poisson=0.3)
+
+
+def stretch_components(nodes):
+    """prescribed.rs:40-63: every node of x = 0 held in x, x = 1 stretched by STRETCH in x, the corner
+    (0, 0, 0) held in y and z and (0, 1, 0) in z, as (node, axis, value) in the reference's call
+    order."""
+    out = []
+    for n, p in enumerate(nodes):
+        if p[0] == 0.0:
+            out.append((n, 0, 0.0))
+        if p[0] == 1.0:
+            out.append((n, 0, STRETCH))
+    corner = lambda target: next(n for n, p in enumerate(nodes) if tuple(p) == target)
+    out += [(corner((0.0, 0.0, 0.0)), 1, 0.0), (corner((0.0, 0.0, 0.0)), 2, 0.0), (corner((0.0, 1.0, 0.0)), 2, 0.0)]
+    return out
