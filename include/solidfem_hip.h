@@ -352,6 +352,19 @@ sf_status sf_csr_multiply(sf_csr *csr, const double *x, double *y);
 /* CsrMatrix::multiply_rounding_bound, csr.rs:142-154, on the device */
 sf_status sf_csr_rounding_bound(sf_csr *csr, const double *x, int32_t dot_order, double *bound);
 
+/* The rounding bound of sf_csr_rounding_bound, allocating nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). x (size values) and bound (one value) are
+ * device pointers; the call enqueues its kernels on `stream` (a hipStream_t; NULL = the matrix's
+ * context stream) and returns without waiting; *bound then holds the same bits as
+ * sf_csr_rounding_bound gives for the same x and dot order. Its scratch is one 16 MiB device array
+ * per GPU, set up when a context is created on that GPU and kept until the process ends, so a
+ * matrix of more than sf_noalloc_max_rows() scalar rows is refused (SF_ERR_INVALID_ARGUMENT,
+ * a = rows, b = the limit). Every call on a GPU shares that array: calls that overlap on one GPU
+ * (on different streams, or from different threads) race and must be serialised by the caller. */
+sf_status sf_csr_rounding_bound_noalloc(const sf_csr *csr, const double *x, int32_t dot_order, double *bound, void *stream);
+uint64_t sf_noalloc_max_rows(void);
+
 /* MultigridSetup::new, crates/solidfem-solve/src/multigrid/setup.rs:129-189: near_null holds
  * near_null_len values (6 per scalar row), positions 3 per block row; wrong lengths are refused. */
 sf_status sf_multigrid_setup_create(sf_context *ctx, const sf_csr *matrix, const double *near_null,

@@ -200,6 +200,8 @@ SIGNATURES = {
     "sf_multigrid_cycle_bytes": (f64, [vp]),
     "sf_pcg_restart": (i32, [vp, P(f64)]),
     "sf_force_variant": (i32, [i32]),
+    "sf_csr_rounding_bound_noalloc": (i32, [vp, vp, i32, vp, vp]),
+    "sf_noalloc_max_rows": (u64, []),
     "sf_setup_slot_limit": (i32, [u32]),
     "sf_profile_enable": (i32, [i32, i32]),
     "sf_profile_reset": (i32, []),

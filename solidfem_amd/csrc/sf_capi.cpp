@@ -13,6 +13,13 @@
 
 using namespace sf;
 
+namespace sf {
+// sf_noalloc.hip: the allocation-free rounding bound and its per-device scratch
+extern const size_t NOALLOC_MAX_ROWS;
+void register_scratch(int device);
+void rounding_bound_noalloc(const sf_csr &c, const double *x, int32_t dot_order, double *bound, hipStream_t s);
+}  // namespace sf
+
 namespace {
 
 thread_local std::string g_message;
@@ -177,7 +184,8 @@ sf_status sf_device_count(int32_t *count) {
 sf_status sf_context_create(int32_t device, sf_context **out) {
     return guard([&] {
         auto c = std::make_unique<sf_context>();
-        c->ctx = std::make_shared<Context>(device);
+        c->ctx = std::make_shared<Context>(device);  // makes `device` current
+        register_scratch(device);
         *out = c.release();
     });
 }
@@ -616,6 +624,14 @@ sf_status sf_csr_rounding_bound(sf_csr *c, const double *x, int32_t dot_order, d
         *bound = std::sqrt(sum);
     });
 }
+
+sf_status sf_csr_rounding_bound_noalloc(const sf_csr *c, const double *x, int32_t dot_order, double *bound, void *stream) {
+    return guard([&] {
+        if (!c) throw Error(SF_ERR_INVALID_ARGUMENT, "no matrix");
+        rounding_bound_noalloc(*c, x, dot_order, bound, stream ? (hipStream_t)stream : c->ctx ? c->ctx->stream : nullptr);
+    });
+}
+uint64_t sf_noalloc_max_rows(void) { return (uint64_t)NOALLOC_MAX_ROWS; }
 
 sf_status sf_multigrid_setup_create(sf_context *ctx, const sf_csr *matrix, const double *near_null, uint64_t near_null_len,
                                     const double *positions, uint64_t position_count, const sf_multigrid_options *options,
