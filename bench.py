@@ -10,8 +10,15 @@ conjugate gradient solve from a zero estimate to a true relative residual of 1e-
 The headline value is ms per solve with the FP32 V-cycle inside the FP64 conjugate gradient
 (the reference's Precision::Single); the plain all-FP64 form (Precision::Double) is timed beside
 it. The dominant kernel -- the fused Chebyshev step on the finest level -- is timed with HIP
-events on the library's stream during one more FP32 solve after the timed ones, for the roofline. The CPU baseline is the
-oracle's serial restatement of the reference CPU path timed on a bounded sample of iterations.
+events on the library's stream during one more FP32 solve after the timed ones, for the roofline,
+and so is the conjugate gradient's FP64 product A p.
+
+Both GPU forms are checked against the oracle's FP64 solve of the same plate in the reference's
+own order (serial dot products, reference coarse solve): energy-norm and displacement errors.
+The CPU baseline is the oracle (the C++ restatement of the reference CPU path, with its
+row-chunk thread pool): assembly and setup timed whole, the FP32-V-cycle solve on a bounded
+sample of iterations at the reference's default thread count, scaled to the GPU's iteration
+count; the FP64 oracle solve, run whole for the error check, is timed at all the host's cores.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -36,38 +43,91 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, if any."""
-    f = ROOT / "profiles" / "traffic_fine_chebyshev_plate606k.json"
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` (its full name, template arguments included) from the
+    committed rocprofv3 PMC summary, or None when that variant was not profiled."""
+    f = ROOT / "profiles" / "traffic_plate606k.json"
     if not f.exists():
         return None
     try:
-        return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+        return json.loads(f.read_text())["kernels"].get(kernel, {}).get("hbm_bytes_per_launch")
     except Exception:
         return None
 
 
-def cpu_baseline(workload, gpu_iterations, sample_iterations):
-    """The oracle (serial C++ restatement of the reference CPU path) on the same plate: setup once,
-    then `sample_iterations` PCG iterations with the f32 V-cycle, scaled to a full solve."""
+def cpu_info():
+    """The host's CPU model, logical processors and this process's CPU share."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else logical
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        share = min(share, int(os.environ["OMP_NUM_THREADS"]))
+    return model, logical, max(1, share)
+
+
+def oracle_leg(np, workload, gpu_iterations, sample_iterations, x_gpu):
+    """The oracle on the same plate. Assembly and the multigrid setup are timed whole; the
+    FP32-V-cycle solve (Precision::Single) is timed on `sample_iterations` iterations at the
+    reference's default thread count, min(logical / 2, 8) (parallel.rs:43-49), and scaled to the
+    GPU's iteration count; the FP64 solve (Precision::Double) in the reference's order runs whole
+    at all the host's cores and is the answer the GPU forms are checked against."""
+    import errors
     import refcpu
 
+    model, logical, share = cpu_info()
+    default = int(refcpu.lib().ref_default_threads())
     t0 = time.perf_counter()
     problem = refcpu.Problem(workload)
-    mg = refcpu.Multigrid(problem, refcpu.options(precision=2))
-    prep = time.perf_counter() - t0
-    _, r = mg.solve(1e-8, iteration_limit=sample_iterations)
+    assembly = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mg32 = refcpu.Multigrid(problem, refcpu.options(precision=2, threads=default))
+    setup = time.perf_counter() - t0
+    _, r = mg32.solve(1e-8, iteration_limit=sample_iterations)
     per_iteration = r.solve_seconds / max(1, r.iterations)
-    return {
+    threads32 = int(mg32.threads)
+    log(f"oracle: assembly {assembly:.2f} s, setup {setup:.2f} s, FP32 V-cycle {per_iteration * 1e3:.1f} ms/iteration "
+        f"on {threads32} threads")
+    del mg32
+    t0 = time.perf_counter()
+    mg64 = refcpu.Multigrid(problem, refcpu.options(precision=0, threads=share))
+    setup64 = time.perf_counter() - t0
+    disp, r64 = mg64.solve(1e-8)
+    log(f"oracle: FP64 solve {r64.solve_seconds:.1f} s, {r64.iterations} iterations on {mg64.threads} threads")
+    fm = problem.free_map()
+    x_o = disp.reshape(-1, 3)[fm != 0xFFFFFFFF].reshape(-1)
+    rs, cols, vals = problem.csr()
+    norm_o = errors.energy(rs, cols, vals, x_o)
+    check = {"oracle_iterations": int(r64.iterations), "oracle_relative_residual": r64.relative_residual,
+             "oracle_at_rounding_floor": bool(r64.at_rounding_floor)}
+    for name, x in x_gpu.items():
+        check[f"{name}_vs_oracle_energy_rel"] = errors.energy(rs, cols, vals, x - x_o) / norm_o
+        check[f"{name}_vs_oracle_displacement_max_rel"] = float(np.abs(x - x_o).max() / np.abs(x_o).max())
+    baseline = {
         "value": per_iteration * gpu_iterations * 1e3,
         "unit": "ms/solve",
-        "cores": 1,
+        "cores": threads32,
         "kind": "port",
-        "sample": (f"oracle (serial C++ restatement of the reference CPU path, Precision::Single V-cycle) on the same "
-                   f"606k-DOF plate: {r.iterations} PCG iterations timed ({r.solve_seconds:.2f} s, "
-                   f"{per_iteration * 1e3:.1f} ms/iteration) x {gpu_iterations} iterations of the GPU solve; "
-                   f"assembly+setup {prep:.1f} s not included"),
+        "sample": (f"oracle (C++ restatement of the reference CPU path, its row-chunk pool) on the same 606k-DOF plate, "
+                   f"Precision::Single: {r.iterations} PCG iterations timed ({r.solve_seconds:.2f} s, "
+                   f"{per_iteration * 1e3:.1f} ms/iteration on {threads32} threads, the reference's default) x "
+                   f"{gpu_iterations} iterations of the GPU solve; assembly and setup timed whole, apart"),
+        "cpu_model": model,
+        "logical_processors": logical,
+        "cpu_share": share,
+        "assembly_s": round(assembly, 3),
+        "multigrid_setup_s": round(setup, 3),
+        "solve_ms_per_iteration": round(per_iteration * 1e3, 3),
+        "fp64_full_solve": {"threads": int(mg64.threads), "setup_s": round(setup64, 3),
+                            "solve_s": round(r64.solve_seconds, 3), "iterations": int(r64.iterations)},
     }
+    return baseline, check
 
 
 def multi_rhs(ctx, sf, np, columns, steps, warmup):
@@ -131,58 +191,11 @@ def multi_rhs(ctx, sf, np, columns, steps, warmup):
     }
 
 
-def serving_requests(sf, np, count, rate, seed):
-    """Synthetic serving traffic: `count` requests with Poisson arrivals at `rate` per second. A
-    request is a bar (10:1:1), a square tube (10:1:1, walls a quarter of the side) or a plate
-    (1:1:0.02) of Tet10, one kind in three each, with a log-uniform size between 5k and 600k DOFs,
-    E ~ U[1, 210] GPa, nu ~ U[0.25, 0.4], clamped on a random face and loaded on another by random
-    tractions N(0, 1 MPa) per component and face; 30% of requests reuse the mesh, material and
-    clamp of an earlier request (its cached hierarchy) with new tractions. A tube's clamp and load
-    planes are its ends or its outer walls."""
-    rng = np.random.default_rng(seed)
-    out, systems = [], []
-    t = 0.0
-    for i in range(count):
-        t += rng.exponential(1.0 / rate)
-        if systems and rng.random() < 0.3:
-            base = systems[rng.integers(len(systems))]
-        else:
-            dofs = float(np.exp(rng.uniform(np.log(5e3), np.log(6e5))))
-            kind = rng.integers(3)
-            tube = {}
-            if kind == 0:  # bar: 3 (20k + 1)(2k + 1)^2 DOFs
-                k = max(1, int(round((dofs / 240.0) ** (1.0 / 3.0))))
-                while k > 1 and 3 * (20 * k + 1) * (2 * k + 1) ** 2 > 6e5:
-                    k -= 1
-                size, div = (10.0 * k, 1.0 * k, 1.0 * k), (10 * k, k, k)
-            elif kind == 1:  # tube of side k cells, k/4 through each wall: 3 (20k + 1)((2k + 1)^2 - (k - 1)^2) DOFs
-                k = max(4, 4 * int(round((dofs / 180.0) ** (1.0 / 3.0) / 4.0)))
-                while k > 4 and 3 * (20 * k + 1) * ((2 * k + 1) ** 2 - (k - 1) ** 2) > 6e5:
-                    k -= 4
-                size, div = (10.0 * k, 1.0 * k, 1.0 * k), (10 * k, k, k)
-                tube = dict(wall_cells=k // 4, wall_thickness=float(k // 4))
-            else:  # plate: 3 (2k + 1)^2 5 DOFs
-                k = max(2, int(round(np.sqrt(dofs / 60.0))))
-                while k > 2 and 15 * (2 * k + 1) ** 2 > 6e5:
-                    k -= 1
-                size, div = (1.0 * k, 1.0 * k, 0.4), (k, k, 2)
-            clamp = int(rng.integers(6))
-            base = dict(size=size, divisions=div, clamp_axis=clamp // 2,
-                        clamp_value=0.0 if clamp % 2 == 0 else size[clamp // 2],
-                        youngs=float(rng.uniform(1e3, 2.1e5)), poisson=float(rng.uniform(0.25, 0.4)), **tube)
-            systems.append(base)
-        faces = [f for f in range(6) if f != 2 * base["clamp_axis"] + (0 if base["clamp_value"] == 0.0 else 1)]
-        load = faces[rng.integers(len(faces))]
-        axis = load // 2
-        w = sf.Workload(**base, load_axis=axis, load_value=0.0 if load % 2 == 0 else base["size"][axis],
-                        traction_kind=1, traction_scale=1.0, traction_seed=seed, load_case=i)
-        out.append((t, w))
-    return out
-
-
 def serving(sf, np, count, rate, seed):
     """Serves the synthetic traffic on one GPU and reports latency percentiles."""
-    traffic = serving_requests(sf, np, count, rate, seed)
+    from workloads import serving_requests
+
+    traffic = serving_requests(count, rate, seed)
     opts = sf.ServerOptions(hbm_budget=200 << 30, max_columns=32, slice_iterations=8, builders=6, solvers=8)
     server = sf.Server.new(0, opts)
     ids = []
@@ -241,18 +254,42 @@ def serving(sf, np, count, rate, seed):
     }
 
 
+def profile_variants(sf, ctx, workload):
+    """Every batch depth of the finest level's kernels and of the conjugate gradient's product,
+    forced in turn, each through one FP32-V-cycle and one FP64 solve of the plate, then the plain
+    1 GiB copy that calibrates the byte counters. The bits never depend on the depth."""
+    pcg_options = sf.PcgOptions(tolerance=1e-8, dot_order=sf.DotOrder.TREE)
+    for depth in (1, 2, 4, 8):
+        sf.force_variant(depth)
+        builder = sf.StudyBuilder.new(ctx, sf.Model.from_workload(workload), pcg_options,
+                                      sf.MultigridOptions(precision=sf.Precision.DOUBLE))
+        builder.step(2 ** 62)
+        study = builder.finish()
+        mg64 = study.multigrid().with_precision(sf.Precision.DOUBLE, coarse_solve=1)
+        for mg in (mg64.with_precision(sf.Precision.SINGLE), mg64):
+            p = sf.Pcg.new(ctx, study.system(), mg, study.rhs(), pcg_options).step(2 ** 62)
+            log(f"depth {depth} ({mg.precision().name}): {p.iteration} iterations")
+        del study, builder
+    sf.force_variant(0)
+    ctx.stream_copy(1 << 30, 10)
+    log("calibration copy done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--divisions", type=str, default="100,100,2")
-    ap.add_argument("--cpu-iterations", type=int, default=8)
+    ap.add_argument("--cpu-iterations", type=int, default=60, help="FP32 oracle iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--columns", type=int, default=32, help="load cases of the multi-RHS measurement (0 = skip)")
     ap.add_argument("--graph", type=int, default=0, help="replay each V-cycle as a captured HIP graph (1) or launch eagerly (0)")
     ap.add_argument("--serve-requests", type=int, default=512, help="requests of the serving measurement (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate of the serving measurement, per s")
+    ap.add_argument("--profile-variants", action="store_true",
+                    help="for a PMC profiler: one FP32 and one FP64 solve with every batch depth forced in turn, then "
+                         "the 1 GiB calibration copy; prints no line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: there is no multi-GPU path (asked for {args.gpus}); "
@@ -267,6 +304,9 @@ def main():
     divisions = tuple(int(v) for v in args.divisions.split(","))
     workload = plate_config(divisions=divisions)
     ctx = sf.Context(0)
+    if args.profile_variants:
+        profile_variants(sf, ctx, workload)
+        return
 
     log(f"building the plate {divisions}")
     t0 = time.perf_counter()
@@ -344,6 +384,13 @@ def main():
     achieved = per_launch / (seconds / launches) / 1e9
     spmv_n, spmv_s, spmv_b = fp32["kernels"]["SYSTEM_PRODUCT"]
     res_n, res_s, res_b = fp32["kernels"]["FINE_RESIDUAL"]
+    # the variants the tuning chose, by their full kernel names (the batch depth is a template
+    # argument), so the traffic read from the PMC summary is that of the timed variant
+    step_depth, res_depth, spmv_depth = mg32.tuned_depth(0, 0), mg32.tuned_depth(0, 1), system.tuned_depth()
+    step_kernel = f"void cheb_step_kernel<3, {step_depth}, float, float>"
+    spmv_kernel = f"void spmv_kernel<3, 3, {spmv_depth}, double, double, 0>"
+    spmv_per_launch = spmv_b / max(1, spmv_n)
+    spmv_achieved = spmv_per_launch / (spmv_s / max(1, spmv_n)) / 1e9
 
     line = {
         "metric": METRIC,
@@ -361,7 +408,8 @@ def main():
         "config": {
             "workload": f"plate 100x100x0.8 mm, {divisions[0]}x{divisions[1]}x{divisions[2]} hex x 6 Tet10, "
                         f"{3 * (2 * divisions[0] + 1) * (2 * divisions[1] + 1) * (2 * divisions[2] + 1)} DOFs "
-                        f"({n_free} free), jitter 5% seed 20261202, rtol 1e-8",
+                        f"({n_free} free), jitter 5% seed 20261202; to rtol 1e-8 or the reference's rounding-floor "
+                        f"stop (pcg.rs:234-255), whichever comes first",
             "levels": int(report.multigrid.levels),
             "coarse_solve": "explicit f64 inverse (fixed-order product)",
             "operator_complexity": round(report.multigrid.operator_complexity, 4),
@@ -370,6 +418,7 @@ def main():
         "fp32_over_fp64_speedup": round(results["fp64"]["ms"] / fp32["ms"], 3),
         "iterations": {"fp32": int(fp32["iterations"]), "fp64": int(results["fp64"]["iterations"])},
         "relative_residual": {"fp32": fp32["residual"], "fp64": results["fp64"]["residual"]},
+        "at_rounding_floor": {"fp32": bool(fp32["floor"]), "fp64": bool(results["fp64"]["floor"])},
         "fp32_vs_fp64_displacement_max_rel_diff": disp_err,
         "dof_iterations_per_s": {"fp32": n_free * fp32["iterations"] / (fp32["ms"] * 1e-3),
                                  "fp64": n_free * results["fp64"]["iterations"] / (results["fp64"]["ms"] * 1e-3)},
@@ -377,23 +426,34 @@ def main():
         "build_seconds": {"assembly": round(report.assembly_seconds, 3),
                           "multigrid_setup": round(report.preconditioner_seconds, 3)},
         "roofline": {
-            "kernel": "cheb_step_kernel<3, float, float> (fused Chebyshev step, finest level)",
+            "kernel": f"{step_kernel} (fused Chebyshev step, finest level, batch depth {step_depth})",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_traffic(),
+            "traffic": read_traffic(step_kernel),
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_us": round(seconds / launches * 1e6, 2),
             "launches": launches,
             "achievable_copy_GBs": round(copy_gbs, 1),
             "achievable_read_GBs": round(read_gbs, 1),
         },
+        "roofline_fp64_system_product": {
+            "kernel": f"{spmv_kernel} (the conjugate gradient's A p, f64, batch depth {spmv_depth})",
+            "bound": "hbm",
+            "achieved": round(spmv_achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(spmv_achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(spmv_kernel),
+            "algorithmic_bytes_per_launch": spmv_per_launch,
+            "avg_launch_us": round(spmv_s / max(1, spmv_n) * 1e6, 2),
+            "launches": spmv_n,
+        },
         "other_kernels": {
-            "system_product_f64": {"avg_launch_us": round(spmv_s / max(1, spmv_n) * 1e6, 2),
-                                   "achieved_GBs": round(spmv_b / max(spmv_s, 1e-30) / 1e9, 1), "launches": spmv_n},
-            "fine_residual_f32": {"avg_launch_us": round(res_s / max(1, res_n) * 1e6, 2),
+            "fine_residual_f32": {"kernel": f"void spmv_kernel<3, 3, {res_depth}, float, float, 1>",
+                                  "avg_launch_us": round(res_s / max(1, res_n) * 1e6, 2),
                                   "achieved_GBs": round(res_b / max(res_s, 1e-30) / 1e9, 1), "launches": res_n},
         },
         "cpu_baseline": None,
@@ -419,8 +479,11 @@ def main():
         line["serving"] = serving(sf, np, args.serve_requests, args.serve_rate, 20261202)
         line["served_p99_ms"] = line["serving"]["p99_ms"]
     if not args.no_cpu_baseline:
-        log("timing the CPU baseline (oracle) on a bounded sample")
-        line["cpu_baseline"] = cpu_baseline(workload, int(fp32["iterations"]), args.cpu_iterations)
+        log("the oracle: CPU baseline and the FP64 answer the GPU forms are checked against")
+        line["cpu_baseline"], line["oracle_check"] = oracle_leg(np, workload, int(fp32["iterations"]), args.cpu_iterations,
+                                                                {"fp32": x32, "fp64": x64})
+        line["fp32_vs_oracle_energy_rel"] = line["oracle_check"]["fp32_vs_oracle_energy_rel"]
+        line["fp64_vs_oracle_energy_rel"] = line["oracle_check"]["fp64_vs_oracle_energy_rel"]
         import refcpu
         problem = refcpu.Problem(workload)
         t = time.perf_counter()

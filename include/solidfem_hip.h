@@ -548,6 +548,12 @@ sf_status sf_server_stats_get(sf_server *server, sf_server_stats *out);
 /* Stops the threads; requests still open fail. */
 void sf_server_destroy(sf_server *server);
 
+/* The batch depth (blocks in flight per lane, 1 to 8) the tuning chose for a level's fused
+ * Chebyshev step (kind 0) or residual product (kind 1), and for a matrix's conjugate-gradient
+ * product A p. No reference counterpart: measurement reports which variant it timed. */
+sf_status sf_multigrid_tuned_depth(const sf_multigrid *mg, uint64_t level, int32_t kind, int32_t *depth);
+sf_status sf_csr_tuned_depth(const sf_csr *csr, int32_t *depth);
+
 /* Tuning and fallback hooks, no reference counterpart (for tests and measurement). Both replace
  * environment variables, which a caller could only change with setenv while library threads may
  * read the environment (undefined behaviour in glibc); the environment is read once, as the

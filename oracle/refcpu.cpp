@@ -5,6 +5,7 @@
 #include "refcpu.h"
 
 #include <chrono>
+#include <functional>
 #include <map>
 #include <memory>
 
@@ -406,6 +407,9 @@ template <typename Apply>
 struct Pcg {
     const CsrMatrix &matrix;
     Apply precondition;
+    // Preconditioner::system_multiply (preconditioner.rs:10-20): the matrix's own product unless
+    // the preconditioner supplies one (the multigrid's, on its pool: cycle.rs:447-457)
+    std::function<void(const double *, double *)> system_multiply;
     std::vector<double> rhs, x, r, z, p, ap;
     double rz = 0, rhs_norm = 0, relative_residual = 0, tolerance;
     u64 iterations = 0, max_iterations;
@@ -415,6 +419,7 @@ struct Pcg {
     // pcg.rs:135-164
     Pcg(const CsrMatrix &m, Apply apply, std::vector<double> b, double tol, u64 max_it)
         : matrix(m), precondition(apply), rhs(std::move(b)), tolerance(tol) {
+        system_multiply = [this](const double *x, double *y) { matrix.multiply(x, y); };
         size_t n = matrix.size;
         rhs_norm = norm(rhs);
         max_iterations = max_it ? max_it : 10 * (u64)n + 1000;
@@ -446,7 +451,7 @@ struct Pcg {
     }
     // pcg.rs:216-275
     void iterate() {
-        matrix.multiply(p.data(), ap.data());
+        system_multiply(p.data(), ap.data());
         double curvature = dot(p, ap);
         if (std::isnan(curvature) || curvature <= 0.0 || !std::isfinite(rz)) {
             status = Status::NotPositiveDefinite;
@@ -459,7 +464,7 @@ struct Pcg {
         relative_residual = norm(r) / rhs_norm;
         if (relative_residual <= tolerance) {
             std::vector<double> residual(rhs.size(), 0.0);
-            matrix.multiply(x.data(), residual.data());
+            system_multiply(x.data(), residual.data());
             for (size_t i = 0; i < residual.size(); ++i) residual[i] = rhs[i] - residual[i];
             relative_residual = norm(residual) / rhs_norm;
             double floor = matrix.multiply_rounding_bound(x.data()) / rhs_norm;
@@ -502,6 +507,7 @@ static MultigridOptions to_options(const ref_mg_options *o) {
     m.smoother_degree = (size_t)o->smoother_degree;
     m.smoother_range = o->smoother_range;
     m.precision = (Precision)o->precision;
+    m.threads = o->threads > 0 ? (size_t)o->threads : default_threads();
     return m;
 }
 
@@ -539,6 +545,7 @@ void ref_default_mg_options(ref_mg_options *o) {
     o->smoother_degree = 3;
     o->smoother_range = 30.0;
     o->precision = 1;
+    o->threads = 0;
 }
 
 ref_problem *ref_problem_create(const ref_workload *w) {
@@ -655,7 +662,7 @@ ref_multigrid *ref_multigrid_create(const ref_problem *rp, const ref_mg_options 
         MultigridOptions opt = to_options(o);
         auto levels = build_hierarchy(p.matrix, modes, positions, opt);
         auto *m = new ref_multigrid;
-        m->m = std::make_unique<Multigrid>(std::move(levels), opt.smoother_degree, opt.precision);
+        m->m = std::make_unique<Multigrid>(std::move(levels), opt.smoother_degree, opt.precision, opt.threads);
         m->setup_seconds = now_seconds() - t0;
         if (status) *status = 0;
         return m;
@@ -666,6 +673,8 @@ ref_multigrid *ref_multigrid_create(const ref_problem *rp, const ref_mg_options 
 }
 void ref_multigrid_destroy(ref_multigrid *m) { delete m; }
 int64_t ref_multigrid_levels(const ref_multigrid *m) { return (int64_t)m->m->levels64.size(); }
+int64_t ref_multigrid_threads(const ref_multigrid *m) { return (int64_t)m->m->threads(); }
+int64_t ref_default_threads(void) { return (int64_t)default_threads(); }
 double ref_multigrid_setup_seconds(const ref_multigrid *m) { return m->setup_seconds; }
 void ref_multigrid_level_info(const ref_multigrid *m, int64_t level, int64_t *info) {
     const auto &l = m->m->levels64[(size_t)level];
@@ -742,6 +751,7 @@ int ref_solve_with(const ref_problem *rp, const ref_multigrid *rm, double tolera
     auto apply = [&mg](const double *r, double *z) { mg.apply(r, z); };
     double t0 = now_seconds();
     Pcg<decltype(apply)> pcg(p.matrix, apply, p.rhs, tolerance, 0);
+    pcg.system_multiply = [&](const double *x, double *y) { mg.system_multiply(p.matrix, x, y); };
     Status st = pcg.step(iteration_limit ? iteration_limit : ~(u64)0);
     double t1 = now_seconds();
     size_t nn = p.mesh.nodes.size();
@@ -767,6 +777,25 @@ int ref_solve_with(const ref_problem *rp, const ref_multigrid *rm, double tolera
         result->assembly_seconds = p.assembly_seconds;
         result->setup_seconds = rm->setup_seconds;
         result->solve_seconds = t1 - t0;
+    }
+    return result ? result->status : 0;
+}
+
+int ref_solve_rhs(const ref_problem *rp, const ref_multigrid *rm, const double *rhs, double tolerance, double *x, ref_result *result) {
+    // Pcg::new + step (pcg.rs:135-189) on the problem's matrix with another right-hand side
+    const Problem &p = *rp->p;
+    const Multigrid &mg = *rm->m;
+    auto apply = [&mg](const double *r, double *z) { mg.apply(r, z); };
+    Pcg<decltype(apply)> pcg(p.matrix, apply, std::vector<double>(rhs, rhs + p.matrix.size), tolerance, 0);
+    pcg.system_multiply = [&](const double *v, double *y) { mg.system_multiply(p.matrix, v, y); };
+    Status st = pcg.step(~(u64)0);
+    std::copy(pcg.x.begin(), pcg.x.end(), x);
+    if (result) {
+        std::memset(result, 0, sizeof(*result));
+        result->status = st == Status::Converged ? 0 : st == Status::NotConverged ? 1 : 2;
+        result->iterations = pcg.iterations;
+        result->relative_residual = pcg.relative_residual;
+        result->at_rounding_floor = pcg.floor_reached ? 1 : 0;
     }
     return result ? result->status : 0;
 }

@@ -75,6 +75,8 @@ typedef struct ref_mg_options {
     int64_t smoother_degree;  /* 3 */
     double smoother_range;    /* 30.0 */
     int32_t precision;        /* 0 Double, 1 SingleStorage, 2 Single */
+    int32_t reserved;
+    int64_t threads;          /* 0: min(logical processors / 2, 8), parallel.rs:43-49 */
 } ref_mg_options;
 
 typedef struct ref_result {
@@ -137,6 +139,10 @@ typedef struct ref_multigrid ref_multigrid;
 ref_multigrid *ref_multigrid_create(const ref_problem *p, const ref_mg_options *o, int32_t *status);
 void ref_multigrid_destroy(ref_multigrid *m);
 int64_t ref_multigrid_levels(const ref_multigrid *m);
+/* The threads the cycle's row products run on, the caller included (Multigrid::threads,
+ * cycle.rs:310-316), and the default count. */
+int64_t ref_multigrid_threads(const ref_multigrid *m);
+int64_t ref_default_threads(void);
 double ref_multigrid_setup_seconds(const ref_multigrid *m);
 /* level info: rows, row_block, nnzb, prolongator nnzb (0 on coarsest), has_factor */
 void ref_multigrid_level_info(const ref_multigrid *m, int64_t level, int64_t *info5);
@@ -153,6 +159,10 @@ void ref_multigrid_apply(const ref_multigrid *m, const double *r, double *z);
 void ref_multigrid_stats(const ref_multigrid *m, int64_t *levels, int64_t *coarse_dofs, double *operator_complexity, int64_t *memory_bytes);
 
 /* Full solve: assembly, multigrid setup, PCG to `tolerance`. displacements: 3*node_count. */
+/* The conjugate gradient of the problem's matrix, preconditioned by `mg`, for another right-hand
+ * side (free unknowns in, free unknowns out). */
+int ref_solve_rhs(const ref_problem *p, const ref_multigrid *mg, const double *rhs, double tolerance, double *x,
+                  ref_result *result);
 int ref_solve(const ref_workload *w, const ref_mg_options *o, double tolerance,
               double *displacements, ref_result *result);
 /* Solve an existing problem + hierarchy; max_iterations 0 = the default cap.

@@ -411,14 +411,15 @@ struct CsrMatrix {
         return values[it - columns.begin()];
     }
 
-    // csr.rs:124-132
-    void multiply(const double *x, double *y) const {
-        for (size_t row = 0; row < size; ++row) {
+    // csr.rs:112-132: rows [first, last) of A x, written at their own indices of y
+    void multiply_rows(size_t first, size_t last, const double *x, double *y) const {
+        for (size_t row = first; row < last; ++row) {
             double sum = 0.0;
             for (size_t at = row_starts[row]; at < row_starts[row + 1]; ++at) sum += values[at] * x[columns[at]];
             y[row] = sum;
         }
     }
+    void multiply(const double *x, double *y) const { multiply_rows(0, size, x, y); }
 
     // csr.rs:142-154
     double multiply_rounding_bound(const double *x) const {

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "refcpu_core.hpp"
+#include "refcpu_parallel.hpp"
 
 namespace refcpu {
 
@@ -30,6 +31,7 @@ struct MultigridOptions {
     size_t smoother_degree = 3;
     double smoother_range = 30.0;
     Precision precision = Precision::SingleStorage;
+    size_t threads = 1;  // MultigridOptions::threads (mod.rs:162-167): threads of the cycle's row products
 };
 
 struct SetupError : std::runtime_error {
@@ -783,6 +785,8 @@ template <typename V>
 struct LevelWork {
     std::vector<V> residual, scaled, direction, product, post_rhs, post_x, coarse_rhs, coarse_x;
     std::vector<double> dense;
+    // cycle.rs:84-106: this level's row chunks, one when it runs on one thread
+    std::vector<size_t> bounds;
 };
 
 template <typename S, typename V>
@@ -799,43 +803,54 @@ inline void apply_inverse_diagonal(const Level<S> &level, size_t first, size_t l
     }
 }
 
-// cycle.rs:570-651
+// cycle.rs:570-651, each region over the level's row chunks
 template <typename S, typename V>
-inline void smooth(const Level<S> &level, size_t degree, LevelWork<V> &w, const V *b, V *x) {
+inline void smooth(const Level<S> &level, size_t degree, WorkerPool *pool, LevelWork<V> &w, const V *b, V *x) {
     double theta = 0.5 * (level.lambda_max + level.lambda_min);
     double delta = 0.5 * (level.lambda_max - level.lambda_min);
     double sigma = theta / delta;
     double rho = 1.0 / sigma;
-    size_t n = level.a.dofs();
+    const size_t block = level.a.row_block;
     V first = (V)theta;
-    for (size_t i = 0; i < n; ++i) w.residual[i] = b[i];
-    apply_inverse_diagonal(level, 0, level.a.rows, w.residual.data(), w.scaled.data());
-    for (size_t i = 0; i < n; ++i) w.direction[i] = w.scaled[i] / first;
-    for (size_t i = 0; i < n; ++i) x[i] = w.direction[i];
+    for_rows(pool, w.bounds, [&](size_t r0, size_t r1) {
+        const size_t i0 = r0 * block, i1 = r1 * block;
+        for (size_t i = i0; i < i1; ++i) w.residual[i] = b[i];
+        apply_inverse_diagonal(level, r0, r1, w.residual.data() + i0, w.scaled.data() + i0);
+        for (size_t i = i0; i < i1; ++i) w.direction[i] = w.scaled[i] / first;
+        for (size_t i = i0; i < i1; ++i) x[i] = w.direction[i];
+    });
     for (size_t step = 1; step < degree; ++step) {
-        level.a.multiply(w.direction.data(), w.product.data());
+        // the product reads every row of the direction: a region of its own
+        for_rows(pool, w.bounds, [&](size_t r0, size_t r1) {
+            level.a.multiply_rows(r0, r1, w.direction.data(), w.product.data() + r0 * block);
+        });
         double next_rho = 1.0 / (2.0 * sigma - rho);
         V keep = (V)(next_rho * rho);
         V stp = (V)(2.0 * next_rho / delta);
-        for (size_t i = 0; i < n; ++i) w.residual[i] -= w.product[i];
-        apply_inverse_diagonal(level, 0, level.a.rows, w.residual.data(), w.scaled.data());
-        for (size_t i = 0; i < n; ++i) w.direction[i] = keep * w.direction[i] + stp * w.scaled[i];
-        for (size_t i = 0; i < n; ++i) x[i] += w.direction[i];
+        for_rows(pool, w.bounds, [&](size_t r0, size_t r1) {
+            const size_t i0 = r0 * block, i1 = r1 * block;
+            for (size_t i = i0; i < i1; ++i) w.residual[i] -= w.product[i];
+            apply_inverse_diagonal(level, r0, r1, w.residual.data() + i0, w.scaled.data() + i0);
+            for (size_t i = i0; i < i1; ++i) w.direction[i] = keep * w.direction[i] + stp * w.scaled[i];
+            for (size_t i = i0; i < i1; ++i) x[i] += w.direction[i];
+        });
         rho = next_rho;
     }
 }
 
 // cycle.rs:548-564
 template <typename S, typename V>
-inline void residual(const Level<S> &level, const V *b, const V *x, V *r) {
-    level.a.multiply(x, r);
-    size_t n = level.a.dofs();
-    for (size_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+inline void residual(const Level<S> &level, WorkerPool *pool, const std::vector<size_t> &bounds, const V *b, const V *x, V *r) {
+    const size_t block = level.a.row_block;
+    for_rows(pool, bounds, [&](size_t r0, size_t r1) {
+        level.a.multiply_rows(r0, r1, x, r + r0 * block);
+        for (size_t i = r0 * block; i < r1 * block; ++i) r[i] = b[i] - r[i];
+    });
 }
 
 // cycle.rs:461-539
 template <typename S, typename V>
-inline void cycle(const std::vector<Level<S>> &levels, size_t index, size_t degree, std::vector<LevelWork<V>> &work,
+inline void cycle(const std::vector<Level<S>> &levels, size_t index, size_t degree, WorkerPool *pool, std::vector<LevelWork<V>> &work,
                   const V *rhs, V *x) {
     const Level<S> &level = levels[index];
     LevelWork<V> &lw = work[index];
@@ -846,29 +861,33 @@ inline void cycle(const std::vector<Level<S>> &levels, size_t index, size_t degr
             level.coarse.solve(lw.dense.data());
             for (size_t i = 0; i < n; ++i) x[i] = (V)lw.dense[i];
         } else {
-            smooth(level, degree, lw, rhs, x);
+            smooth(level, degree, pool, lw, rhs, x);
         }
         return;
     }
-    size_t n = level.a.dofs();
-    smooth(level, degree, lw, rhs, x);
-    residual(level, rhs, x, lw.residual.data());
+    const size_t block = level.a.row_block;
+    smooth(level, degree, pool, lw, rhs, x);
+    residual(level, pool, lw.bounds, rhs, x, lw.residual.data());
     level.prolongator.multiply_transpose(lw.residual.data(), lw.coarse_rhs.data(), lw.coarse_rhs.size());
     std::fill(lw.coarse_x.begin(), lw.coarse_x.end(), (V)0);
-    cycle(levels, index + 1, degree, work, lw.coarse_rhs.data(), lw.coarse_x.data());
-    level.prolongator.multiply(lw.coarse_x.data(), lw.product.data());
-    for (size_t i = 0; i < n; ++i) x[i] += lw.product[i];
-    residual(level, rhs, x, lw.post_rhs.data());
+    cycle(levels, index + 1, degree, pool, work, lw.coarse_rhs.data(), lw.coarse_x.data());
+    for_rows(pool, lw.bounds, [&](size_t r0, size_t r1) {
+        level.prolongator.multiply_rows(r0, r1, lw.coarse_x.data(), lw.product.data() + r0 * block);
+        for (size_t i = r0 * block; i < r1 * block; ++i) x[i] += lw.product[i];
+    });
+    residual(level, pool, lw.bounds, rhs, x, lw.post_rhs.data());
     std::vector<V> post_rhs = std::move(lw.post_rhs);
     std::vector<V> post_x = std::move(lw.post_x);
-    smooth(level, degree, lw, post_rhs.data(), post_x.data());
-    for (size_t i = 0; i < n; ++i) x[i] += post_x[i];
+    smooth(level, degree, pool, lw, post_rhs.data(), post_x.data());
+    for_rows(pool, lw.bounds, [&](size_t r0, size_t r1) {
+        for (size_t i = r0 * block; i < r1 * block; ++i) x[i] += post_x[i];
+    });
     lw.post_rhs = std::move(post_rhs);
     lw.post_x = std::move(post_x);
 }
 
 template <typename S, typename V>
-inline std::vector<LevelWork<V>> make_work(const std::vector<Level<S>> &levels) {
+inline std::vector<LevelWork<V>> make_work(const std::vector<Level<S>> &levels, size_t threads) {
     std::vector<LevelWork<V>> work(levels.size());
     for (size_t i = 0; i < levels.size(); ++i) {
         size_t dofs = levels[i].a.dofs();
@@ -883,6 +902,8 @@ inline std::vector<LevelWork<V>> make_work(const std::vector<Level<S>> &levels) 
         w.coarse_rhs.assign(cd, 0);
         w.coarse_x.assign(cd, 0);
         if (levels[i].has_coarse) w.dense.assign(dofs, 0.0);
+        const size_t chunks = levels[i].a.values.size() >= MIN_PARALLEL_VALUES ? threads : 1;
+        w.bounds = balanced_bounds(levels[i].a.starts, chunks);
     }
     return work;
 }
@@ -896,16 +917,24 @@ struct Multigrid {
     mutable std::vector<LevelWork<double>> work64;
     mutable std::vector<LevelWork<float>> work32;
     mutable std::vector<float> rhs32, x32;
+    // cycle.rs:265-308: a pool when more than one thread is asked for and some level is large
+    // enough to use it
+    std::unique_ptr<WorkerPool> pool;
 
-    Multigrid(std::vector<Level<double>> levels, size_t deg, Precision p) : precision(p), degree(deg), levels64(std::move(levels)) {
+    Multigrid(std::vector<Level<double>> levels, size_t deg, Precision p, size_t threads = 1)
+        : precision(p), degree(deg), levels64(std::move(levels)) {
+        bool large = false;
+        for (const auto &l : levels64) large = large || l.a.values.size() >= MIN_PARALLEL_VALUES;
+        if (large) pool = make_pool(threads);
+        const size_t used = pool ? pool->threads() : 1;
         if (precision == Precision::Double) {
-            work64 = make_work<double, double>(levels64);
+            work64 = make_work<double, double>(levels64, used);
         } else {
             for (auto &l : levels64) levels32.push_back(l.into_precision<float>());
             if (precision == Precision::SingleStorage) {
-                work64 = make_work<float, double>(levels32);
+                work64 = make_work<float, double>(levels32, used);
             } else {
-                work32 = make_work<float, float>(levels32);
+                work32 = make_work<float, float>(levels32, used);
                 rhs32.assign(levels32[0].a.dofs(), 0.0f);
                 x32.assign(levels32[0].a.dofs(), 0.0f);
             }
@@ -916,15 +945,15 @@ struct Multigrid {
     void apply(const double *r, double *z) const {
         switch (precision) {
             case Precision::Double:
-                cycle<double, double>(levels64, 0, degree, work64, r, z);
+                cycle<double, double>(levels64, 0, degree, pool.get(), work64, r, z);
                 break;
             case Precision::SingleStorage:
-                cycle<float, double>(levels32, 0, degree, work64, r, z);
+                cycle<float, double>(levels32, 0, degree, pool.get(), work64, r, z);
                 break;
             case Precision::Single: {
                 size_t n = rhs32.size();
                 for (size_t i = 0; i < n; ++i) rhs32[i] = (float)r[i];
-                cycle<float, float>(levels32, 0, degree, work32, rhs32.data(), x32.data());
+                cycle<float, float>(levels32, 0, degree, pool.get(), work32, rhs32.data(), x32.data());
                 for (size_t i = 0; i < n; ++i) z[i] = (double)x32[i];
                 break;
             }
@@ -935,19 +964,32 @@ struct Multigrid {
     void smooth_level(size_t level, const double *b, double *x) const {
         switch (precision) {
             case Precision::Double:
-                smooth<double, double>(levels64[level], degree, work64[level], b, x);
+                smooth<double, double>(levels64[level], degree, pool.get(), work64[level], b, x);
                 break;
             case Precision::SingleStorage:
-                smooth<float, double>(levels32[level], degree, work64[level], b, x);
+                smooth<float, double>(levels32[level], degree, pool.get(), work64[level], b, x);
                 break;
             case Precision::Single: {
                 size_t n = levels32[level].a.dofs();
                 std::vector<float> bb(n), xx(n);
                 for (size_t i = 0; i < n; ++i) bb[i] = (float)b[i];
-                smooth<float, float>(levels32[level], degree, work32[level], bb.data(), xx.data());
+                smooth<float, float>(levels32[level], degree, pool.get(), work32[level], bb.data(), xx.data());
                 for (size_t i = 0; i < n; ++i) x[i] = (double)xx[i];
                 break;
             }
+        }
+    }
+
+    size_t threads() const { return pool ? pool->threads() : 1; }
+
+    // Multigrid::system_multiply, cycle.rs:447-457: the conjugate gradient's A p on the pool
+    // once the matrix holds enough values
+    void system_multiply(const CsrMatrix &matrix, const double *x, double *y) const {
+        if (pool && matrix.values.size() >= MIN_PARALLEL_VALUES) {
+            const auto bounds = balanced_bounds(matrix.row_starts, pool->threads());
+            for_rows(pool.get(), bounds, [&](size_t r0, size_t r1) { matrix.multiply_rows(r0, r1, x, y); });
+        } else {
+            matrix.multiply(x, y);
         }
     }
 

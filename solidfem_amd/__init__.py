@@ -542,6 +542,12 @@ class CsrMatrix(_Handle):
         _raise(library().sf_csr_multiply(self._ptr, _ptr(xx, C.c_double), _ptr(y, C.c_double)))
         return y
 
+    def tuned_depth(self) -> int:
+        """The batch depth the tuning chose for this matrix's conjugate-gradient product."""
+        d = C.c_int32()
+        _raise(library().sf_csr_tuned_depth(self._ptr, C.byref(d)))
+        return d.value
+
     def multiply_rounding_bound(self, x, dot_order=DotOrder.REFERENCE) -> float:
         xx = np.ascontiguousarray(x, dtype=np.float64)
         b = C.c_double()
@@ -647,6 +653,12 @@ class Multigrid(_Handle, _Levels):
     def close(self):
         if getattr(self, "_owned", True):
             super().close()
+
+    def tuned_depth(self, level: int, kind: int = 0) -> int:
+        """The batch depth of `level`'s fused Chebyshev step (kind 0) or residual (kind 1)."""
+        d = C.c_int32()
+        _raise(library().sf_multigrid_tuned_depth(self._ptr, level, kind, C.byref(d)))
+        return d.value
 
     def stats(self) -> _capi.MultigridStats:
         s = _capi.MultigridStats()

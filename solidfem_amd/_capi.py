@@ -200,6 +200,8 @@ SIGNATURES = {
     "sf_multigrid_cycle_bytes": (f64, [vp]),
     "sf_pcg_restart": (i32, [vp, P(f64)]),
     "sf_force_variant": (i32, [i32]),
+    "sf_multigrid_tuned_depth": (i32, [vp, u64, i32, P(i32)]),
+    "sf_csr_tuned_depth": (i32, [vp, P(i32)]),
     "sf_csr_rounding_bound_noalloc": (i32, [vp, vp, i32, vp, vp]),
     "sf_noalloc_max_rows": (u64, []),
     "sf_setup_slot_limit": (i32, [u32]),

@@ -1015,6 +1015,20 @@ sf_status sf_pcg_restart(sf_pcg *pcg, const double *new_rhs) {
     return guard([&] { pcg->pcg->restart(new_rhs); });
 }
 
+sf_status sf_multigrid_tuned_depth(const sf_multigrid *mg, uint64_t level, int32_t kind, int32_t *depth) {
+    return guard([&] {
+        if (level >= mg->mg->host.size()) throw Error(SF_ERR_INVALID_ARGUMENT, "no such level");
+        if (kind != 0 && kind != 1) throw Error(SF_ERR_INVALID_ARGUMENT, "kind is 0 (Chebyshev step) or 1 (residual)");
+        *depth = mg->mg->tuned_depth((size_t)level, kind);
+    });
+}
+sf_status sf_csr_tuned_depth(const sf_csr *csr, int32_t *depth) {
+    return guard([&] {
+        if (!csr->ctx) throw Error(SF_ERR_INVALID_ARGUMENT, "the matrix is not on a device");
+        *depth = csr->system.sliced.batch_spmv;
+    });
+}
+
 sf_status sf_force_variant(int32_t code) {
     return guard([&] {
         if (code < 0) throw Error(SF_ERR_INVALID_ARGUMENT, "a variant code is not negative");

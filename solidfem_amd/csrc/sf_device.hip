@@ -719,6 +719,11 @@ class Hierarchy final : public DeviceMultigrid {
         SF_HIP(hipGetLastError());
     }
 
+    int tuned_depth(size_t level, int kind) const override {
+        const LevelDev<S, V> &d = lv_.at(level);
+        return kind == 0 ? d.a.batch_cheb : d.a.batch_spmv;
+    }
+
     void smooth_level(size_t level, const double *b, double *x, hipStream_t s) override {
         LevelDev<S, V> &d = lv_.at(level);
         if constexpr (std::is_same<V, double>::value) {

@@ -169,6 +169,8 @@ class DeviceMultigrid {
     // Z = M^-1 R for M row-interleaved f64 columns (entry (i, m) at i * M + m); column m is bit for
     // bit apply() of column m.
     virtual void apply_batch(const double *r, double *z, uint32_t columns, hipStream_t s) = 0;
+    // The tuned batch depth of a level's fused Chebyshev step (kind 0) or residual product (kind 1).
+    virtual int tuned_depth(size_t level, int kind) const = 0;
     int precision = SF_PRECISION_SINGLE_STORAGE;
     int coarse_mode = SF_COARSE_REFERENCE;
     size_t degree = 3;

@@ -24,7 +24,7 @@ class RefWorkload(C.Structure):
 
 class RefOptions(C.Structure):
     _fields_ = [("neighbour_reach", f64), ("coarse_dofs", i64), ("max_levels", i64), ("smoother_degree", i64),
-                ("smoother_range", f64), ("precision", i32)]
+                ("smoother_range", f64), ("precision", i32), ("reserved", i32), ("threads", i64)]
 
 
 class RefResult(C.Structure):
@@ -61,6 +61,8 @@ _SIG = {
     "ref_multigrid_create": (vp, [vp, P(RefOptions), P(i32)]),
     "ref_multigrid_destroy": (None, [vp]),
     "ref_multigrid_levels": (i64, [vp]),
+    "ref_multigrid_threads": (i64, [vp]),
+    "ref_default_threads": (i64, []),
     "ref_multigrid_setup_seconds": (f64, [vp]),
     "ref_multigrid_level_info": (None, [vp, i64, P(i64)]),
     "ref_multigrid_level_matrix": (None, [vp, i64, P(i64), P(u32), P(f64)]),
@@ -74,6 +76,7 @@ _SIG = {
     "ref_multigrid_stats": (None, [vp, P(i64), P(i64), P(f64), P(i64)]),
     "ref_solve": (i32, [P(RefWorkload), P(RefOptions), f64, P(f64), P(RefResult)]),
     "ref_solve_with": (i32, [vp, vp, f64, u64, P(f64), P(RefResult)]),
+    "ref_solve_rhs": (i32, [vp, vp, P(f64), f64, P(f64), P(RefResult)]),
     "ref_fingerprint": (u64, [P(f64), i64]),
     "ref_pcg_jacobi": (i32, [i64, P(i64), P(u32), P(f64), P(f64), f64, u64, P(f64), P(RefResult)]),
     "ref_csr_multiply": (None, [i64, P(i64), P(u32), P(f64), P(f64), P(f64)]),
@@ -105,10 +108,11 @@ def workload(w) -> RefWorkload:
     return w.c(RefWorkload)
 
 
-def options(precision=1, coarse_dofs=1500, degree=3, reach=3.0, max_levels=12, smoother_range=30.0) -> RefOptions:
+def options(precision=1, coarse_dofs=1500, degree=3, reach=3.0, max_levels=12, smoother_range=30.0, threads=1) -> RefOptions:
+    """MultigridOptions; `threads` 0 takes the reference's default, min(logical / 2, 8)."""
     o = RefOptions()
     lib().ref_default_mg_options(C.byref(o))
-    o.precision, o.coarse_dofs, o.smoother_degree = precision, coarse_dofs, degree
+    o.precision, o.coarse_dofs, o.smoother_degree, o.threads = precision, coarse_dofs, degree, threads
     o.neighbour_reach, o.max_levels, o.smoother_range = reach, max_levels, smoother_range
     return o
 
@@ -231,6 +235,10 @@ class Multigrid:
     def levels(self):
         return lib().ref_multigrid_levels(self._m)
 
+    @property
+    def threads(self):
+        return lib().ref_multigrid_threads(self._m)
+
     def info(self, level):
         out = np.empty(5, dtype=np.int64)
         lib().ref_multigrid_level_info(self._m, level, ptr(out, i64))
@@ -286,6 +294,14 @@ class Multigrid:
         lib().ref_multigrid_smooth(self._m, level, ptr(b, f64), ptr(x, f64))
         return x
 
+    def solve_rhs(self, rhs, tolerance):
+        """The problem's matrix solved for another right-hand side over the free unknowns."""
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.empty_like(b)
+        r = RefResult()
+        lib().ref_solve_rhs(self.problem._p, self._m, ptr(b, f64), tolerance, ptr(x, f64), C.byref(r))
+        return x, r
+
     def solve(self, tolerance, iteration_limit=0):
         disp = np.empty(3 * self.problem.node_count)
         r = RefResult()
@@ -293,12 +309,13 @@ class Multigrid:
         return disp.reshape(-1, 3), r
 
 
-def solve(w, precision=0, coarse_dofs=1500, tolerance=1e-9, degree=3):
+def solve(w, precision=0, coarse_dofs=1500, tolerance=1e-9, degree=3, threads=1):
     disp = None
     p = Problem(w)
     disp = np.empty(3 * p.node_count)
     r = RefResult()
-    lib().ref_solve(C.byref(p.w), C.byref(options(precision, coarse_dofs, degree)), tolerance, ptr(disp, f64), C.byref(r))
+    lib().ref_solve(C.byref(p.w), C.byref(options(precision, coarse_dofs, degree, threads=threads)), tolerance, ptr(disp, f64),
+                    C.byref(r))
     return disp.reshape(-1, 3), r
 
 

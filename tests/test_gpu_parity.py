@@ -93,15 +93,30 @@ def test_device_v_cycle_is_bit_identical(ctx, name, coarse, precision):
 
 @pytest.mark.parametrize("precision", [0, 2], ids=["double", "single"])
 def test_inverse_coarse_solve_is_deterministic_and_within_rounding(ctx, precision):
+    # The explicit inverse and the substitutions compute A_c^-1 b with different roundings; each
+    # is within 2 n eps kappa(A_c) of the exact solution relative to it (the standard forward-error
+    # bound of a backward-stable solve), so they differ by at most twice that. The cycle carries
+    # the difference through one prolongation and one post-smoothing, linear maps whose gain is
+    # allowed a factor 10 here. In the FP32 cycle the coarse correction is rounded to f32: a
+    # difference below half an f32 ulp changes at most the rounding of the values it reaches,
+    # which 8 f32 ulps of the output's scale bound.
     ref = refcpu.Problem(WORKLOADS["plate"])
+    mg_ref = refcpu.Multigrid(ref, refcpu.options(precision=0, coarse_dofs=100))
+    factor = np.tril(mg_ref.coarse_factor())
+    n = factor.shape[0]
+    kappa = np.linalg.cond(factor @ factor.T)
+    coarse_bound = 2 * 2 * n * np.finfo(np.float64).eps * kappa
     _, mg = device_multigrid(ctx, ref, precision, 100)
     fast = mg.with_precision(sf.Precision(precision), coarse_solve=1)
     r = np.random.default_rng(8).standard_normal(ref.free_dofs)
     z_ref, z_fast = mg.apply(r), fast.apply(r)
     assert np.array_equal(bits(fast.apply(r)), bits(z_fast))
     scale = np.abs(z_ref).max()
-    tol = 1e-9 if precision == 0 else 1e-3  # an f64 coarse solve's rounding, carried through f64 or f32 cycles
-    assert np.abs(z_fast - z_ref).max() <= tol * scale
+    diff = np.abs(z_fast - z_ref).max() / scale
+    tol = 10 * coarse_bound if precision == 0 else max(10 * coarse_bound, 8 * 2.0 ** -24)
+    print(f"inverse coarse solve, precision {precision}: n {n}, kappa {kappa:.3e}, difference {diff:.3e} of the scale, "
+          f"bound {tol:.3e}")
+    assert diff <= tol
 
 
 def test_device_coarse_solve_matches_the_factor(ctx):

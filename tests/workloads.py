@@ -1,4 +1,6 @@
 """The workloads the tests and the benchmark share (sizes and seeds stated here)."""
+import numpy as np
+
 from solidfem_amd import Workload
 
 
@@ -76,4 +78,53 @@ def stretch_components(nodes):
             out.append((n, 0, STRETCH))
     corner = lambda target: next(n for n, p in enumerate(nodes) if tuple(p) == target)
     out += [(corner((0.0, 0.0, 0.0)), 1, 0.0), (corner((0.0, 0.0, 0.0)), 2, 0.0), (corner((0.0, 1.0, 0.0)), 2, 0.0)]
+    return out
+
+
+def serving_requests(count, rate, seed, min_dofs=5e3, max_dofs=6e5):
+    """Synthetic serving traffic: `count` requests with Poisson arrivals at `rate` per second. A
+    request is a bar (10:1:1), a square tube (10:1:1, walls a quarter of the side) or a plate
+    (1:1:0.02) of Tet10, one kind in three each, with a log-uniform size between min_dofs and max_dofs (5k and 600k DOFs by default),
+    E ~ U[1, 210] GPa, nu ~ U[0.25, 0.4], clamped on a random face and loaded on another by random
+    tractions N(0, 1 MPa) per component and face; 30% of requests reuse the mesh, material and
+    clamp of an earlier request (its cached hierarchy) with new tractions. A tube's clamp and load
+    planes are its ends or its outer walls."""
+    rng = np.random.default_rng(seed)
+    out, systems = [], []
+    t = 0.0
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        if systems and rng.random() < 0.3:
+            base = systems[rng.integers(len(systems))]
+        else:
+            dofs = float(np.exp(rng.uniform(np.log(min_dofs), np.log(max_dofs))))
+            kind = rng.integers(3)
+            tube = {}
+            if kind == 0:  # bar: 3 (20k + 1)(2k + 1)^2 DOFs
+                k = max(1, int(round((dofs / 240.0) ** (1.0 / 3.0))))
+                while k > 1 and 3 * (20 * k + 1) * (2 * k + 1) ** 2 > max_dofs:
+                    k -= 1
+                size, div = (10.0 * k, 1.0 * k, 1.0 * k), (10 * k, k, k)
+            elif kind == 1:  # tube of side k cells, k/4 through each wall: 3 (20k + 1)((2k + 1)^2 - (k - 1)^2) DOFs
+                k = max(4, 4 * int(round((dofs / 180.0) ** (1.0 / 3.0) / 4.0)))
+                while k > 4 and 3 * (20 * k + 1) * ((2 * k + 1) ** 2 - (k - 1) ** 2) > max_dofs:
+                    k -= 4
+                size, div = (10.0 * k, 1.0 * k, 1.0 * k), (10 * k, k, k)
+                tube = dict(wall_cells=k // 4, wall_thickness=float(k // 4))
+            else:  # plate: 3 (2k + 1)^2 5 DOFs
+                k = max(2, int(round(np.sqrt(dofs / 60.0))))
+                while k > 2 and 15 * (2 * k + 1) ** 2 > max_dofs:
+                    k -= 1
+                size, div = (1.0 * k, 1.0 * k, 0.4), (k, k, 2)
+            clamp = int(rng.integers(6))
+            base = dict(size=size, divisions=div, clamp_axis=clamp // 2,
+                        clamp_value=0.0 if clamp % 2 == 0 else size[clamp // 2],
+                        youngs=float(rng.uniform(1e3, 2.1e5)), poisson=float(rng.uniform(0.25, 0.4)), **tube)
+            systems.append(base)
+        faces = [f for f in range(6) if f != 2 * base["clamp_axis"] + (0 if base["clamp_value"] == 0.0 else 1)]
+        load = faces[rng.integers(len(faces))]
+        axis = load // 2
+        w = Workload(**base, load_axis=axis, load_value=0.0 if load % 2 == 0 else base["size"][axis],
+                        traction_kind=1, traction_scale=1.0, traction_seed=seed, load_case=i)
+        out.append((t, w))
     return out
