@@ -1201,6 +1201,7 @@ DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, Device
     b_.upload(rhs.data(), n_, s);
     scalars_.resize(1);
     partials_.resize(DOT_BLOCKS);
+    partials2_.resize(DOT_BLOCKS);
     norm_.resize(1);
     SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocCoherent));
     std::memset((void *)pinned_, 0, sizeof(PcgScalars));
@@ -1220,6 +1221,7 @@ void DevicePcg::initialise() {
     iterations_ = 0;
     relative_ = 0;
     floor_ = false;
+    rz_slot_ = 0;
     status_ = PcgStatus::Active;
     dot(b_.ptr, b_.ptr, norm_.ptr);
     double bb = 0;
@@ -1280,7 +1282,7 @@ void DevicePcg::reset_direction() {
     // pcg.rs:277-282
     precondition(r_.ptr, z_.ptr);
     SF_HIP(hipMemcpyAsync(p_.ptr, z_.ptr, n_ * 8, hipMemcpyDeviceToDevice, ctx_->stream));
-    dot(r_.ptr, z_.ptr, &scalars_.ptr->rz);
+    dot(r_.ptr, z_.ptr, rz_slot_ ? &scalars_.ptr->rz_next : &scalars_.ptr->rz);
 }
 
 // Waits until the host copy of the scalars carries `tag`. The copy is written by a kernel already
@@ -1313,14 +1315,34 @@ void DevicePcg::iterate() {
     const int token = Profiler::get().begin(1, s, (double)a_.sliced.algorithmic_matrix_bytes() + 2.0 * n_ * 8);
     launch_spmv<double, double>(a_.sliced, p_.ptr, ap_.ptr, nullptr, EPI_STORE, s);
     Profiler::get().end(1, token, s);
-    dot(p_.ptr, ap_.ptr, &sc->curvature);
-    hipLaunchKernelGGL(pcg_update_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, sc, p_.ptr, ap_.ptr, x_.ptr, r_.ptr);
-    // r.r, whose final kernel also copies the scalars into host memory, tagged for this readback
     const uint32_t tag = ++readbacks_;
-    dot(r_.ptr, r_.ptr, &sc->rr, FINAL_STORE, mirror_, tag);
-    precondition(r_.ptr, z_.ptr);
-    dot(r_.ptr, z_.ptr, nullptr, FINAL_RZ);
-    hipLaunchKernelGGL(pcg_direction_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, (const PcgScalars *)sc, z_.ptr, p_.ptr);
+    if (dot_order_ == SF_DOT_TREE) {
+        // the same reductions in the same order, their final sums formed by the kernels that use
+        // them: p.Ap's by the update, which also forms r.r's partials, and r.z's by the direction
+        hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n_, (const double *)p_.ptr,
+                           (const double *)ap_.ptr, partials_.ptr);
+        hipLaunchKernelGGL(pcg_update_fused_kernel, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n_, sc, rz_slot_,
+                           (const double *)partials_.ptr, (const double *)p_.ptr, (const double *)ap_.ptr, x_.ptr, r_.ptr,
+                           partials2_.ptr);
+        // r.r's final sum also copies the scalars into host memory, tagged for this readback
+        hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, s, DOT_BLOCKS, (const double *)partials2_.ptr,
+                           (int)FINAL_STORE, &sc->rr, sc, mirror_, tag);
+        precondition(r_.ptr, z_.ptr);
+        hipLaunchKernelGGL((dot_partials_kernel<0>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n_, (const double *)r_.ptr,
+                           (const double *)z_.ptr, partials_.ptr);
+        hipLaunchKernelGGL(pcg_direction_fused_kernel, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n_, sc, rz_slot_,
+                           (const double *)partials_.ptr, (const double *)z_.ptr, p_.ptr);
+        rz_slot_ ^= 1u;
+    } else {
+        dot(p_.ptr, ap_.ptr, &sc->curvature);
+        hipLaunchKernelGGL(pcg_update_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, sc, p_.ptr, ap_.ptr, x_.ptr, r_.ptr);
+        // r.r, whose final kernel also copies the scalars into host memory, tagged for this readback
+        dot(r_.ptr, r_.ptr, &sc->rr, FINAL_STORE, mirror_, tag);
+        precondition(r_.ptr, z_.ptr);
+        dot(r_.ptr, z_.ptr, nullptr, FINAL_RZ);
+        hipLaunchKernelGGL(pcg_direction_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, (const PcgScalars *)sc, z_.ptr,
+                           p_.ptr);
+    }
     SF_HIP(hipGetLastError());
     wait_for_tag(tag);
     const PcgScalars h = *pinned_;

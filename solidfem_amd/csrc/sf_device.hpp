@@ -248,7 +248,8 @@ class DevicePcg {
     Error failure_{SF_OK, ""};
     DeviceBuffer<double> b_, x_, r_, z_, p_, ap_, tmp_, tmp2_, jacobi_;
     DeviceBuffer<dev::PcgScalars> scalars_;
-    DeviceBuffer<double> partials_, norm_;
+    DeviceBuffer<double> partials_, partials2_, norm_;
+    uint32_t rz_slot_ = 0;  // TREE order: the PcgScalars slot holding the current r . z (rz or rz_next)
     dev::PcgScalars *pinned_ = nullptr;  // fine-grained host memory the r.r kernel copies the scalars into
     dev::PcgScalars *mirror_ = nullptr;  // its device address
     uint32_t readbacks_ = 0;             // tags of the copies so far

@@ -711,6 +711,75 @@ static __global__ void pcg_update_kernel(size_t n, PcgScalars *__restrict__ s, c
     r[i] = r[i] + (-alpha) * ap[i];
 }
 
+// The total of a TREE dot product formed by every block of a consumer kernel of DOT_THREADS
+// threads, exactly as dot_final_kernel forms it (the same partials summed in the same order):
+// the consumer needs no separate final launch and the bits do not change.
+__device__ __forceinline__ double tree_total(const double *__restrict__ partials) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < DOT_BLOCKS; i += blockDim.x) v = v + partials[i];
+    const double total = block_sum(v);
+    __shared__ double shared_total;
+    if (threadIdx.x == 0) shared_total = total;
+    __syncthreads();
+    return shared_total;
+}
+
+// The slot of PcgScalars that holds the current r.z in the fused TREE iteration: the direction
+// kernel reads it in every block while block 0 stores the next one, so the two alternate.
+__device__ __forceinline__ double &rz_in(PcgScalars *s, uint32_t slot) { return slot ? s->rz_next : s->rz; }
+
+// pcg.rs:216-230 with both reductions around it fused, for TREE order, on DOT_BLOCKS blocks of
+// DOT_THREADS: p . A p summed from its partials as dot_final_kernel does, the curvature test,
+// x += alpha p and r += (-alpha) A p, and the partials of the new r . r over the same grid-stride
+// rows as dot_partials_kernel -- the same bits as the separate launches.
+static __global__ __launch_bounds__(DOT_THREADS) void pcg_update_fused_kernel(size_t n, PcgScalars *__restrict__ s, uint32_t slot,
+                                                                              const double *__restrict__ pap_partials,
+                                                                              const double *__restrict__ p,
+                                                                              const double *__restrict__ ap, double *__restrict__ x,
+                                                                              double *__restrict__ r, double *__restrict__ rr_partials) {
+    const double curvature = tree_total(pap_partials);
+    const double rz = rz_in(s, slot);
+    const bool valid = !(isnan(curvature) || curvature <= 0.0 || !isfinite(rz));
+    const double alpha = rz / curvature;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->curvature = curvature;
+        s->valid = valid ? 1 : 0;
+        s->alpha = alpha;
+    }
+    double v = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double ri = r[i];
+        if (valid) {
+            x[i] = x[i] + alpha * p[i];
+            ri = ri + (-alpha) * ap[i];
+            r[i] = ri;
+        }
+        v = v + ri * ri;
+    }
+    __syncthreads();  // block_sum's shared words were read by tree_total
+    const double part = block_sum(v);
+    if (threadIdx.x == 0) rr_partials[blockIdx.x] = part;
+}
+
+// pcg.rs:256-263 fused for TREE order, on DOT_BLOCKS blocks of DOT_THREADS: r . z summed from its
+// partials as dot_final_kernel does, beta = rz_next / rz, then p = z + beta p. Block 0 stores the
+// next r . z in the other slot.
+static __global__ __launch_bounds__(DOT_THREADS) void pcg_direction_fused_kernel(size_t n, PcgScalars *__restrict__ s,
+                                                                                 uint32_t slot,
+                                                                                 const double *__restrict__ rz_partials,
+                                                                                 const double *__restrict__ z,
+                                                                                 double *__restrict__ p) {
+    const double total = tree_total(rz_partials);
+    const double beta = total / rz_in(s, slot);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rz_in(s, slot ^ 1u) = total;
+        s->beta = beta;
+    }
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = z[i] + beta * p[i];
+}
+
 // p = z + beta p (pcg.rs:261-263), beta formed by the r.z reduction.
 static __global__ void pcg_direction_kernel(size_t n, const PcgScalars *__restrict__ s, const double *__restrict__ z,
                                             double *__restrict__ p) {
