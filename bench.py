@@ -165,6 +165,24 @@ def multi_rhs(ctx, sf, np, columns, steps, warmup):
     ctx.synchronize()
     batch_ms = (time.perf_counter() - t) / steps * 1e3
     xb = batch.estimate()
+    # one more batch with the batched kernels bracketed by HIP events: their rooflines
+    kernels = (sf.Kernel.BATCH_FINE_CHEBYSHEV, sf.Kernel.BATCH_SYSTEM_PRODUCT)
+    for k in kernels:
+        sf.profile_enable(k, True)
+    sf.profile_reset()
+    run_batch()
+    ctx.synchronize()
+    timed = {k: sf.profile_read(k) for k in kernels}
+    for k in kernels:
+        sf.profile_enable(k, False)
+
+    def roofline(k, what):
+        n, sec, nbytes = timed[k]
+        per = nbytes / max(1, n)
+        gbs = per / (sec / max(1, n)) / 1e9 if n else 0.0
+        return {"kernel": what, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": per,
+                "avg_launch_us": round(sec / max(1, n) * 1e6, 2), "launches": n}
     singles = [sf.Pcg.new(ctx, system, mg, rhs[m], pcg_options) for m in range(columns)]
     for p in singles[:2]:
         p.step(2 ** 62)
@@ -188,6 +206,10 @@ def multi_rhs(ctx, sf, np, columns, steps, warmup):
         "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
         "iterations": {"min": int(min(p.iteration for p in progress)), "max": int(max(p.iteration for p in progress))},
         "columns_bit_identical_to_single_solves": bool(identical),
+        "roofline_batched_fine_chebyshev": roofline(sf.Kernel.BATCH_FINE_CHEBYSHEV,
+                                                    f"batched fused Chebyshev step, finest level, f32, {columns} columns"),
+        "roofline_batched_system_product": roofline(sf.Kernel.BATCH_SYSTEM_PRODUCT,
+                                                    f"batched system product A P, f64, {columns} columns"),
     }
 
 

@@ -568,9 +568,11 @@ sf_status sf_setup_slot_limit(uint32_t blocks);
 
 /* Kernel launch timing with HIP events on the launching stream. Kernels: 0 = the fused
  * Chebyshev step on the finest level, 1 = the conjugate gradient's system product A p, 2 = the
- * residual on the finest level. read returns launches, summed seconds and summed algorithmic
- * bytes since the last reset. */
-enum { SF_KERNEL_FINE_CHEBYSHEV = 0, SF_KERNEL_SYSTEM_PRODUCT = 1, SF_KERNEL_FINE_RESIDUAL = 2 };
+ * residual on the finest level; for several columns, 3 = the finest level's batched Chebyshev
+ * steps and 4 = the batched system product. read returns launches, summed seconds and summed
+ * algorithmic bytes since the last reset. */
+enum { SF_KERNEL_FINE_CHEBYSHEV = 0, SF_KERNEL_SYSTEM_PRODUCT = 1, SF_KERNEL_FINE_RESIDUAL = 2,
+       SF_KERNEL_BATCH_FINE_CHEBYSHEV = 3, SF_KERNEL_BATCH_SYSTEM_PRODUCT = 4 };
 sf_status sf_profile_enable(int32_t kernel, int32_t enable);
 sf_status sf_profile_reset(void);
 sf_status sf_profile_read(int32_t kernel, uint64_t *launches, double *seconds, double *bytes);

@@ -928,6 +928,8 @@ class Kernel(enum.IntEnum):
     FINE_CHEBYSHEV = 0
     SYSTEM_PRODUCT = 1
     FINE_RESIDUAL = 2
+    BATCH_FINE_CHEBYSHEV = 3
+    BATCH_SYSTEM_PRODUCT = 4
 
 
 def force_variant(code: int = 0):

@@ -39,13 +39,13 @@ template <int RB, int CB, int BATCH, int CPL, typename S, typename V>
 static void spmm_launch(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, bool node) {
     if (node) {
         const size_t threads = (size_t)a.slices * Shape<RB>::NPW * WAVE;
-        hipLaunchKernelGGL((spmm_node_kernel<RB, CB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a.view(), M, epi,
+        hipLaunchKernelGGL((spmm_node_kernel<RB, CB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a.batch_view(), M, epi,
                            x, y, b);
         return;
     }
     const int lg = columns_log2((M + CPL - 1) / CPL);
     const size_t threads = (size_t)a.slices * Shape<RB>::NPW << lg;
-    hipLaunchKernelGGL((spmm_kernel<RB, CB, BATCH, S, V, CPL>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a.view(), lg, M, epi,
+    hipLaunchKernelGGL((spmm_kernel<RB, CB, BATCH, S, V, CPL>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a.batch_view(), lg, M, epi,
                        x, y, b);
 }
 
@@ -241,7 +241,9 @@ void DevicePcgBatch::iterate() {
     hipStream_t s = ctx_->stream;
     const size_t total = n_ * m_;
     const unsigned g = grid_for(total, 256);
+    const int token = Profiler::get().begin(4, s, (double)a_.sliced.algorithmic_matrix_bytes() + 2.0 * total * 8);
     launch_spmm<double, double>(a_.sliced, m_, p_.ptr, ap_.ptr, nullptr, EPI_STORE, s);
+    Profiler::get().end(4, token, s);
     dot(p_.ptr, ap_.ptr, offsetof(PcgScalars, curvature), FINAL_STORE, nullptr);
     hipLaunchKernelGGL(pcg_update_batch_kernel, dim3(g), dim3(256), 0, s, n_, m_, scalars_.ptr, (const uint32_t *)active_.ptr,
                        (const double *)p_.ptr, (const double *)ap_.ptr, x_.ptr, r_.ptr);
@@ -282,10 +284,10 @@ void DevicePcgBatch::iterate() {
         const int lg = columns_log2(m_);
         const size_t threads = (size_t)a_.sliced.slices * (a_.rb == 3 ? Shape<3>::NPW : Shape<1>::NPW) << lg;
         if (a_.rb == 3)
-            hipLaunchKernelGGL((rounding_bound_batch_kernel<3, 3>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a_.sliced.view(), lg, m_,
+            hipLaunchKernelGGL((rounding_bound_batch_kernel<3, 3>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a_.sliced.batch_view(), lg, m_,
                                (const double *)x_.ptr, tmp2_.ptr);
         else
-            hipLaunchKernelGGL((rounding_bound_batch_kernel<1, 1>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a_.sliced.view(), lg, m_,
+            hipLaunchKernelGGL((rounding_bound_batch_kernel<1, 1>), dim3(grid_for(threads, 256)), dim3(256), 0, s, a_.sliced.batch_view(), lg, m_,
                                (const double *)x_.ptr, tmp2_.ptr);
         dot(tmp2_.ptr, tmp2_.ptr, offsetof(PcgScalars, bound2), FINAL_STORE, mask_.ptr, 1);
         const PcgScalars *t = read_scalars();

@@ -27,8 +27,16 @@ struct GroupLane {
     int m;         // column of this lane
 };
 
-__device__ __forceinline__ GroupLane group_lane(int log2g) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+// With ORDER_GROUP_RUNS in the matrix view's order, each XCD takes one contiguous run of the
+// workgroups (as xcd_runs_block): the rows an XCD works on at a time are then neighbours, and so
+// are the rows of x they gather, which its L2 can hold for all their columns. Which workgroup
+// takes which rows changes no bit.
+__device__ __forceinline__ uint32_t group_block(uint32_t order) {
+    return (order & ORDER_GROUP_RUNS) ? xcd_runs_block() : blockIdx.x;
+}
+
+__device__ __forceinline__ GroupLane group_lane(int log2g, uint32_t order) {
+    const uint32_t t = group_block(order) * blockDim.x + threadIdx.x;
     return {t >> log2g, (int)(t & ((1u << log2g) - 1u))};
 }
 
@@ -160,7 +168,7 @@ __global__ __launch_bounds__(256) void spmm_kernel(Sliced<S> a, int log2g, uint3
                                                    V *__restrict__ y, const V *__restrict__ b) {
     // CPL > 1: lane m of the group takes the CPL columns from m * CPL (M a multiple of CPL)
     constexpr int NPW = Shape<RB>::NPW;
-    const GroupLane g = group_lane(log2g);
+    const GroupLane g = group_lane(log2g, a.order);
     const uint32_t m0 = (uint32_t)g.m * CPL;
     if (g.pos >= a.slices * (uint32_t)NPW || m0 >= M) return;
     const uint32_t row = a.perm[g.pos];
@@ -233,15 +241,15 @@ __device__ __forceinline__ void node_product(const Sliced<S> &a, uint32_t pos, i
 }
 
 // Position of this wavefront's block row, made uniform for the compiler.
-__device__ __forceinline__ uint32_t node_of_wave() {
-    return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+__device__ __forceinline__ uint32_t node_of_wave(uint32_t order) {
+    return __builtin_amdgcn_readfirstlane(group_block(order) * (blockDim.x / WAVE) + threadIdx.x / WAVE);
 }
 
 template <int RB, int CB, int BATCH, typename S, typename V>
 __global__ __launch_bounds__(256) void spmm_node_kernel(Sliced<S> a, uint32_t M, int epi, const V *__restrict__ x,
                                                         V *__restrict__ y, const V *__restrict__ b) {
     constexpr int NPW = Shape<RB>::NPW;
-    const uint32_t pos = node_of_wave();
+    const uint32_t pos = node_of_wave(a.order);
     if (pos >= a.slices * (uint32_t)NPW) return;
     const int m = threadIdx.x % WAVE;
     uint32_t row;
@@ -263,7 +271,7 @@ __global__ __launch_bounds__(256) void cheb_step_node_kernel(Sliced<S> a, uint32
                                                              V step, V *__restrict__ d_out, V *__restrict__ r_out,
                                                              V *__restrict__ x, V *__restrict__ sum_into) {
     constexpr int NPW = Shape<RB>::NPW;
-    const uint32_t pos = node_of_wave();
+    const uint32_t pos = node_of_wave(a.order);
     if (pos >= a.slices * (uint32_t)NPW) return;
     const int m = threadIdx.x % WAVE;
     uint32_t row;
@@ -295,7 +303,7 @@ template <int RB, int CB>
 __global__ __launch_bounds__(256) void rounding_bound_batch_kernel(Sliced<double> a, int log2g, uint32_t M,
                                                                    const double *__restrict__ x, double *__restrict__ out) {
     constexpr int NPW = Shape<RB>::NPW;
-    const GroupLane g = group_lane(log2g);
+    const GroupLane g = group_lane(log2g, a.order);
     if (g.pos >= a.slices * (uint32_t)NPW || (uint32_t)g.m >= M) return;
     const uint32_t row = a.perm[g.pos];
     if (row == NONE) return;
@@ -353,7 +361,7 @@ __global__ __launch_bounds__(256) void cheb_step_batch_kernel(Sliced<S> a, int l
                                                               V step, V *__restrict__ d_out, V *__restrict__ r_out,
                                                               V *__restrict__ x, V *__restrict__ sum_into) {
     constexpr int NPW = Shape<RB>::NPW;
-    const GroupLane g = group_lane(log2g);
+    const GroupLane g = group_lane(log2g, a.order);
     const uint32_t m0 = (uint32_t)g.m * CPL;
     if (g.pos >= a.slices * (uint32_t)NPW || m0 >= M) return;
     const uint32_t row = a.perm[g.pos];

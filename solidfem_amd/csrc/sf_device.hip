@@ -41,6 +41,16 @@ uint32_t DeviceSliced<T>::xcd_order() {
     return order;
 }
 template <typename T>
+bool DeviceSliced<T>::group_runs() {
+    // off unless SOLIDFEM_GROUP_RUNS=1: measured slower on the 32-column block (81 against 59 ms
+    // per batch)
+    static const bool on = [] {
+        const char *e = std::getenv("SOLIDFEM_GROUP_RUNS");
+        return e && std::atoi(e) != 0;
+    }();
+    return on;
+}
+template <typename T>
 bool DeviceSliced<T>::ping_pong() {
     static const bool on = [] {
         const char *e = std::getenv("SOLIDFEM_PING_PONG");
@@ -776,13 +786,13 @@ class Hierarchy final : public DeviceMultigrid {
                                   V *r_out, V *x, V *into, hipStream_t s, bool node) {
         if (node) {
             const size_t threads = (size_t)d.a.slices * Shape<RB>::NPW * WAVE;
-            hipLaunchKernelGGL((cheb_step_node_kernel<RB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.view(), M,
+            hipLaunchKernelGGL((cheb_step_node_kernel<RB, BATCH, S, V>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.batch_view(), M,
                                d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
             return;
         }
         const int lg = columns_log2((M + CPL - 1) / CPL);
         const size_t threads = (size_t)d.a.slices * Shape<RB>::NPW << lg;
-        hipLaunchKernelGGL((cheb_step_batch_kernel<RB, BATCH, S, V, CPL>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.view(),
+        hipLaunchKernelGGL((cheb_step_batch_kernel<RB, BATCH, S, V, CPL>), dim3(grid_for(threads, 256)), dim3(256), 0, s, d.a.batch_view(),
                            lg, M, d.dinv.ptr, d_in, r_in, keep, stp, d_out, r_out, x, into);
     }
     template <int RB, int CPL>
@@ -865,12 +875,22 @@ class Hierarchy final : public DeviceMultigrid {
         V *dd[2] = {d.md0.ptr, d.md1.ptr};
         V *rr[2] = {d.mr0.ptr, d.mr1.ptr};
         const size_t steps = d.steps.size();
+        // the finest level's steps are timed for the batched roofline: the matrix and block inverses
+        // once per launch, d_in, r_in, d_out, r_out and x read or written for every column (x both),
+        // and the cycle's x read and written by the last step of a post-smoothing
+        int token = -1;
+        if (&d == &lv_[0] && steps) {
+            const double vec = (double)d.n * M * sizeof(V);
+            const double each = (double)d.a.algorithmic_matrix_bytes() + (double)d.rows * d.rb * d.rb * sizeof(S) + 6 * vec;
+            token = Profiler::get().begin(3, s, each * (double)steps + (sum_into ? 2 * vec : 0.0), steps);
+        }
         for (size_t k = 1; k <= steps; ++k) {
             const V *d_in = dd[(k - 1) & 1];
             const V *r_in = k == 1 ? b : rr[(k - 1) & 1];
             const V keep = (V)d.steps[k - 1].first, stp = (V)d.steps[k - 1].second;
             launch_cheb_batch(d, d.mm_depth[lg][0], M, d_in, r_in, keep, stp, dd[k & 1], rr[k & 1], x, k == steps ? sum_into : nullptr, s);
         }
+        Profiler::get().end(3, token, s);
         if (steps == 0 && sum_into) {
             const size_t n = d.n * (size_t)M;
             hipLaunchKernelGGL((add_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, sum_into, (const V *)x);

@@ -41,6 +41,14 @@ struct DeviceSliced {
         if (ping_pong() && (passes++ & 1u)) order |= dev::ORDER_REVERSE;
         return {slice_start.ptr, row_len.ptr, perm.ptr, cols.ptr, vals.ptr, slices, order};
     }
+    // The view the batched (several-column) kernels take: with SOLIDFEM_GROUP_RUNS=1 their
+    // workgroups run in contiguous runs per XCD.
+    dev::Sliced<T> batch_view() const {
+        dev::Sliced<T> v = view();
+        if (group_runs()) v.order |= dev::ORDER_GROUP_RUNS;
+        return v;
+    }
+    static bool group_runs();
     static uint32_t xcd_order();  // SOLIDFEM_XCD_ORDER: 0 none, 1 contiguous runs, 2 whole windows
     static bool ping_pong();      // SOLIDFEM_PING_PONG: alternate passes walk the slices backwards
     size_t dofs() const { return rows * rb; }
@@ -128,7 +136,7 @@ void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s);
 // Brackets launches of a chosen kernel with HIP events on the launching stream and accumulates
 // their durations and algorithmic bytes (read back on request).
 struct Profiler {
-    static constexpr int KERNELS = 3;
+    static constexpr int KERNELS = 5;
     struct Pending {
         hipEvent_t start, stop;
         double bytes;

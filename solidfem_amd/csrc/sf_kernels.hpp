@@ -52,7 +52,7 @@ struct Sliced {
 // each XCD whole sort windows (8 workgroups of 4 slices), so the rows a window permutes share one
 // L2; REVERSE walks the slices from the end (alternate passes over a matrix then start where the
 // last one ended, in the cache).
-enum : uint32_t { ORDER_XCD_RUNS = 1, ORDER_XCD_WINDOWS = 2, ORDER_REVERSE = 4 };
+enum : uint32_t { ORDER_XCD_RUNS = 1, ORDER_XCD_WINDOWS = 2, ORDER_REVERSE = 4, ORDER_GROUP_RUNS = 8 };
 
 template <typename V, typename S>
 __device__ __forceinline__ V widen(S s) {
