@@ -254,12 +254,22 @@ sf_status sf_context_synchronize(sf_context *ctx);
 /* TetMesh::new, crates/solidfem-mesh/src/tet.rs:59-109 */
 sf_status sf_tet_mesh_create(const double *positions, uint64_t node_count, const uint32_t *tets,
                              uint64_t tet_count, sf_tet_mesh **out);
+/* A mesh from its text form, the on-wire form of a mesh request: `vertex-count tet-count`, then
+ * `x y z` for every vertex, then `a b c d` (vertex indices from 0) for every tetrahedron, separated
+ * by any white space -- the triangle-surface text of crates/solidfem-bench/src/bin/gate.rs:9-14
+ * with tetrahedra in place of triangles (this path has no mesher to fill a surface). Validated as
+ * TetMesh::new (tet.rs:59-109, SF_ERR_MESH_*); malformed text is refused with
+ * SF_ERR_INVALID_ARGUMENT, a = the line. */
+sf_status sf_tet_mesh_read_text(const char *text, uint64_t length, sf_tet_mesh **out);
 /* box_tet_mesh, crates/solidfem-mesh/src/box_mesh.rs:22-80 */
 sf_status sf_box_tet_mesh(const double origin[3], const double size[3], const int32_t divisions[3],
                           sf_tet_mesh **out);
 void sf_tet_mesh_destroy(sf_tet_mesh *mesh);
 uint64_t sf_tet_mesh_node_count(const sf_tet_mesh *mesh);
 uint64_t sf_tet_mesh_tet_count(const sf_tet_mesh *mesh);
+/* TetMesh::positions / tets, tet.rs:111-121: 3 per vertex, 4 per tetrahedron */
+void sf_tet_mesh_positions(const sf_tet_mesh *mesh, double *out);
+void sf_tet_mesh_tets(const sf_tet_mesh *mesh, uint32_t *out);
 /* QuadraticTetMesh::from_linear, crates/solidfem-mesh/src/quadratic.rs:24-71 */
 sf_status sf_quad_mesh_from_linear(const sf_tet_mesh *mesh, sf_quad_mesh **out);
 void sf_quad_mesh_destroy(sf_quad_mesh *mesh);
@@ -537,6 +547,11 @@ typedef struct sf_server_stats {
 } sf_server_stats;
 sf_status sf_server_create(int32_t device, const sf_server_options *options, sf_server **out);
 sf_status sf_server_submit(sf_server *server, const sf_workload *workload, uint64_t *id);
+/* A model of any mesh (copied): requests whose models have the same mesh, fixed nodes, prescribed
+ * components (with their values), material and rule share one system; each brings its own nodal
+ * forces and tractions, and its answer is that model's StaticStudy answer with the server's
+ * options, bit for bit. */
+sf_status sf_server_submit_model(sf_server *server, const sf_model *model, uint64_t *id);
 sf_status sf_server_status(sf_server *server, uint64_t id, sf_serve_status *out);
 /* Waits until the request is done or failed, or timeout_seconds pass (negative: no limit). */
 sf_status sf_server_wait(sf_server *server, uint64_t id, double timeout_seconds, sf_serve_status *out);

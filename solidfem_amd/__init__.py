@@ -240,6 +240,35 @@ class TetMesh(_Handle):
         _raise(library().sf_tet_mesh_create(_ptr(pos, C.c_double), len(pos), _ptr(tt, C.c_uint32), len(tt), C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def read_text(cls, text):
+        """A mesh from its text form (`vertex-count tet-count`, x y z per vertex, a b c d per
+        tetrahedron): the on-wire form of a mesh request."""
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        h = C.c_void_p()
+        _raise(library().sf_tet_mesh_read_text(data, len(data), C.byref(h)))
+        return cls(h)
+
+    def positions(self) -> np.ndarray:
+        out = np.empty((library().sf_tet_mesh_node_count(self._ptr), 3))
+        library().sf_tet_mesh_positions(self._ptr, _ptr(out, C.c_double))
+        return out
+
+    def tets(self) -> np.ndarray:
+        out = np.empty((library().sf_tet_mesh_tet_count(self._ptr), 4), dtype=np.uint32)
+        library().sf_tet_mesh_tets(self._ptr, _ptr(out, C.c_uint32))
+        return out
+
+    @staticmethod
+    def text_of(positions, tets) -> str:
+        """The text form of a mesh, with every coordinate written exactly (17 significant digits)."""
+        pos = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+        tt = np.asarray(tets, dtype=np.uint32).reshape(-1, 4)
+        lines = [f"{len(pos)} {len(tt)}"]
+        lines += [f"{x:.17g} {y:.17g} {z:.17g}" for x, y, z in pos]
+        lines += [f"{a} {b} {c} {d}" for a, b, c, d in tt]
+        return "\n".join(lines) + "\n"
+
 
 def box_tet_mesh(origin, size, divisions) -> TetMesh:
     """crates/solidfem-mesh/src/box_mesh.rs:22-80"""
@@ -896,6 +925,13 @@ class Server(_Handle):
         w = workload.c(_capi.Workload)
         rid = C.c_uint64()
         _raise(library().sf_server_submit(self._ptr, C.byref(w), C.byref(rid)))
+        return rid.value
+
+    def submit_model(self, model: "Model") -> int:
+        """A model of any mesh (copied); models with the same mesh, constraints and material share
+        one system."""
+        rid = C.c_uint64()
+        _raise(library().sf_server_submit_model(self._ptr, model._ptr, C.byref(rid)))
         return rid.value
 
     def status(self, rid: int) -> ServeStatus:

@@ -128,6 +128,8 @@ SIGNATURES = {
     "sf_tet_mesh_create": (i32, [P(f64), u64, P(u32), u64, P(vp)]),
     "sf_box_tet_mesh": (i32, [P(f64), P(f64), P(i32), P(vp)]),
     "sf_tet_mesh_destroy": (None, [vp]),
+    "sf_tet_mesh_positions": (None, [vp, P(f64)]),
+    "sf_tet_mesh_tets": (None, [vp, P(u32)]),
     "sf_tet_mesh_node_count": (u64, [vp]),
     "sf_tet_mesh_tet_count": (u64, [vp]),
     "sf_quad_mesh_from_linear": (i32, [vp, P(vp)]),
@@ -225,6 +227,8 @@ SIGNATURES = {
     "sf_pcg_batch_destroy": (None, [vp]),
     "sf_server_create": (i32, [i32, P(ServerOptions), P(vp)]),
     "sf_server_submit": (i32, [vp, P(Workload), P(u64)]),
+    "sf_server_submit_model": (i32, [vp, vp, P(u64)]),
+    "sf_tet_mesh_read_text": (i32, [C.c_char_p, u64, P(vp)]),
     "sf_server_status": (i32, [vp, u64, P(ServeStatus)]),
     "sf_server_wait": (i32, [vp, u64, f64, P(ServeStatus)]),
     "sf_server_result": (i32, [vp, u64, P(f64), u64]),

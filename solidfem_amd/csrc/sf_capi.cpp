@@ -210,6 +210,22 @@ sf_status sf_tet_mesh_create(const double *positions, uint64_t node_count, const
         *out = m.release();
     });
 }
+void sf_tet_mesh_positions(const sf_tet_mesh *m, double *out) {
+    for (size_t n = 0; n < m->mesh.positions.size(); ++n)
+        for (int a = 0; a < 3; ++a) out[3 * n + a] = m->mesh.positions[n][a];
+}
+void sf_tet_mesh_tets(const sf_tet_mesh *m, uint32_t *out) {
+    for (size_t t = 0; t < m->mesh.tets.size(); ++t)
+        for (int a = 0; a < 4; ++a) out[4 * t + a] = m->mesh.tets[t][a];
+}
+sf_status sf_tet_mesh_read_text(const char *text, uint64_t length, sf_tet_mesh **out) {
+    return guard([&] {
+        if (!text && length) throw Error(SF_ERR_INVALID_ARGUMENT, "no text");
+        auto m = std::make_unique<sf_tet_mesh>();
+        m->mesh = read_tet_mesh_text(text, (size_t)length);
+        *out = m.release();
+    });
+}
 sf_status sf_box_tet_mesh(const double origin[3], const double size[3], const int32_t divisions[3], sf_tet_mesh **out) {
     return guard([&] {
         auto m = std::make_unique<sf_tet_mesh>();
@@ -402,6 +418,7 @@ sf_status sf_study_builder_step(sf_study_builder *b, uint64_t max_elements, sf_c
                     sys.host.values.resize(sys.host.blocks() * 9);
                     sys.values.download(sys.host.values.data(), sys.host.values.size(), s);
                     SF_HIP(hipStreamSynchronize(s));
+                    b->study->prescribed_rhs = prescribed_rhs_program(sys.host, b->problem.prescribed);
                     apply_prescribed(sys.host, b->problem.rhs, b->problem.prescribed);
                     sys.values.upload(sys.host.values.data(), sys.host.values.size(), s);
                 }
@@ -840,6 +857,12 @@ sf_status sf_server_create(int32_t device, const sf_server_options *options, sf_
 }
 sf_status sf_server_submit(sf_server *server, const sf_workload *workload, uint64_t *id) {
     return guard([&] { *id = server->server->submit(*workload); });
+}
+sf_status sf_server_submit_model(sf_server *server, const sf_model *model, uint64_t *id) {
+    return guard([&] {
+        if (!model) throw Error(SF_ERR_INVALID_ARGUMENT, "no model");
+        *id = server->server->submit_model(model->model);
+    });
 }
 sf_status sf_server_status(sf_server *server, uint64_t id, sf_serve_status *out) {
     return guard([&] { *out = server->server->status(id); });

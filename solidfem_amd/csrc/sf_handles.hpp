@@ -45,6 +45,7 @@ struct sf_static_study {
     std::shared_ptr<Context> ctx;
     Model model;
     FreeProblem problem;
+    PrescribedRhs prescribed_rhs;  // what eliminating the prescribed components does to a load
     sf_csr system;
     sf_multigrid mg;
     std::unique_ptr<sf_pcg> pcg;

@@ -24,6 +24,9 @@ struct TetMesh {
     std::vector<std::array<u32, 4>> tets;
 };
 void validate_tet_mesh(const TetMesh &mesh);  // throws Error(SF_ERR_MESH_*)
+// A mesh from its text form: `vertex-count tet-count`, x y z per vertex, a b c d per tetrahedron,
+// validated; malformed text throws Error(SF_ERR_INVALID_ARGUMENT, a = line).
+TetMesh read_tet_mesh_text(const char *text, size_t length);
 TetMesh make_box_mesh(const double origin[3], const double size[3], const int32_t divisions[3]);
 // The tube of a workload with wall_cells > 0 (demo/wasm/src/part.rs:48-185).
 TetMesh make_tube_mesh(const sf_workload &w);
@@ -101,6 +104,9 @@ struct FreeProblem {
     std::vector<std::pair<size_t, double>> prescribed;  // (free dof, value), sorted
 };
 FreeProblem number_and_load(const Model &model);  // throws Error(SF_ERR_MODEL_*)
+// The nodal load of a model's forces and tractions (study.rs:696-735), 3 per mesh node; the
+// mesh's boundary faces, sorted, may be given to save finding them again.
+std::vector<double> model_load(const Model &model, const std::vector<Face> *sorted_boundary = nullptr);
 // The right-hand side over the free unknowns of tractions alone: number_and_load's load for a
 // model with no nodal forces and no prescribed displacements, the same bits.
 std::vector<double> traction_rhs(const QuadMesh &mesh, const std::vector<Traction> &tractions, const std::vector<u32> &free,
@@ -142,5 +148,17 @@ AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free
 // diagonal times the value. The system stays symmetric positive definite with the prescribed
 // unknowns decoupled.
 void apply_prescribed(HostBsr<double> &a, std::vector<double> &rhs, const std::vector<std::pair<size_t, double>> &prescribed);
+// The right-hand side part of that elimination, recorded from the matrix before it, so that another
+// load on the same system (the same mesh, constraints and prescribed values) gets the same bits:
+// for each prescribed unknown in order, rhs[c] -= K[dof][c] * value over its row's unconstrained
+// columns c, then every prescribed rhs[dof] = K[dof][dof] * value.
+struct PrescribedRhs {
+    std::vector<std::pair<size_t, double>> prescribed;  // (free dof, value), sorted
+    std::vector<int64_t> starts;                        // entries of prescribed unknown k
+    std::vector<std::pair<size_t, double>> entries;     // (column, K[dof][column]) before elimination
+    std::vector<double> diagonal;                        // K[dof][dof]
+};
+PrescribedRhs prescribed_rhs_program(const HostBsr<double> &a, const std::vector<std::pair<size_t, double>> &prescribed);
+void apply_prescribed_rhs(const PrescribedRhs &program, std::vector<double> &rhs);
 
 }  // namespace sf

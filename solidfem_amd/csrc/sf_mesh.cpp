@@ -2,6 +2,9 @@
 // and rigid-body modes (crates/solidfem-elastic/src/{study,loads,rigid_modes}.rs), the block
 // pattern with its assembly map, and the deterministic host thread pool.
 #include <algorithm>
+#include <string>
+#include <cerrno>
+#include <cctype>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -435,6 +438,53 @@ TetMesh make_tube_mesh(const sf_workload &w) {
     return m;
 }
 
+TetMesh read_tet_mesh_text(const char *text, size_t length) {
+    // tokens separated by white space; the line of each token kept for the refusal
+    size_t at = 0, line = 1;
+    auto next = [&](std::string &token) -> bool {
+        while (at < length && std::isspace((unsigned char)text[at])) {
+            if (text[at] == '\n') ++line;
+            ++at;
+        }
+        if (at >= length) return false;
+        const size_t from = at;
+        while (at < length && !std::isspace((unsigned char)text[at])) ++at;
+        token.assign(text + from, at - from);
+        return true;
+    };
+    auto refuse = [&](const char *what) { throw Error(SF_ERR_INVALID_ARGUMENT, std::string("mesh text: ") + what, (int64_t)line); };
+    auto count = [&](const char *what) -> uint64_t {
+        std::string token;
+        if (!next(token)) refuse(what);
+        char *end = nullptr;
+        errno = 0;
+        const unsigned long long v = std::strtoull(token.c_str(), &end, 10);
+        if (errno || end == token.c_str() || *end || token[0] == '-' || v >= (1ull << 32)) refuse(what);
+        return v;
+    };
+    auto real = [&](const char *what) -> double {
+        std::string token;
+        if (!next(token)) refuse(what);
+        char *end = nullptr;
+        const double v = std::strtod(token.c_str(), &end);
+        if (end == token.c_str() || *end) refuse(what);
+        return v;
+    };
+    TetMesh mesh;
+    const uint64_t vertices = count("the vertex count is missing or not a count");
+    const uint64_t tets = count("the tetrahedron count is missing or not a count");
+    mesh.positions.resize(vertices);
+    for (auto &p : mesh.positions)
+        for (int a = 0; a < 3; ++a) p[a] = real("a vertex coordinate is missing or not a number");
+    mesh.tets.resize(tets);
+    for (auto &t : mesh.tets)
+        for (int a = 0; a < 4; ++a) t[a] = (u32)count("a tetrahedron's vertex index is missing or not an index");
+    std::string extra;
+    if (next(extra)) refuse("text follows the last tetrahedron");
+    validate_tet_mesh(mesh);
+    return mesh;
+}
+
 Model model_from_workload(const sf_workload &w) {
     if (w.wall_cells < 0) throw Error(SF_ERR_INVALID_ARGUMENT, "wall_cells must not be negative");
     if (w.clamp_axis < -1 || w.clamp_axis > 2) throw Error(SF_ERR_MODEL_AXIS_OUT_OF_RANGE, "clamp axis out of range", w.clamp_axis);
@@ -709,6 +759,65 @@ void check_held_dofs(const QuadMesh &mesh, const std::vector<char> &fixed, const
 
 }  // namespace
 
+std::vector<double> model_load(const Model &model, const std::vector<Face> *sorted_boundary) {
+    // crates/solidfem-elastic/src/study.rs:696-735: nodal forces, then tractions, in the order given
+    const size_t nn = model.mesh.nodes.size();
+    std::vector<double> load(3 * nn, 0.0);
+    for (const auto &f : model.nodal_forces) {
+        if (f.first >= nn) throw Error(SF_ERR_MODEL_NODE_OUT_OF_RANGE, "force node out of range", f.first);
+        for (double c : f.second)
+            if (!std::isfinite(c)) throw Error(SF_ERR_MODEL_NON_FINITE_LOAD, "a load is not finite");
+        for (int a = 0; a < 3; ++a) load[3 * (size_t)f.first + a] += f.second[a];
+    }
+    if (!model.tractions.empty()) {
+        std::vector<Face> own;
+        if (!sorted_boundary) {
+            own = model.mesh.boundary_faces();
+            std::sort(own.begin(), own.end());
+            sorted_boundary = &own;
+        }
+        for (const auto &t : model.tractions) {
+            if (!std::binary_search(sorted_boundary->begin(), sorted_boundary->end(), t.first))
+                throw Error(SF_ERR_MODEL_FACE_NOT_ON_BOUNDARY, "the face is not on the boundary", t.first.tet, t.first.face);
+            for (double c : t.second)
+                if (!std::isfinite(c)) throw Error(SF_ERR_MODEL_NON_FINITE_LOAD, "a traction is not finite");
+            face_load(model.mesh, t.first, t.second, load);
+        }
+    }
+    return load;
+}
+
+PrescribedRhs prescribed_rhs_program(const HostBsr<double> &a, const std::vector<std::pair<size_t, double>> &prescribed) {
+    // what study.rs:567-593 does to the right-hand side, read from the matrix before elimination:
+    // a row's off-diagonal entry in an unconstrained column is never changed before that row is
+    // eliminated (only eliminating its own row, or its column's, changes it, and a constrained
+    // column's term does not reach the right-hand side), and no diagonal entry is ever changed
+    PrescribedRhs p;
+    p.prescribed = prescribed;
+    std::vector<char> constrained(a.dofs(), 0);
+    for (const auto &pr : prescribed) constrained[pr.first] = 1;
+    p.starts.push_back(0);
+    for (const auto &pr : prescribed) {
+        const size_t dof = pr.first, brow = dof / 3, row = dof % 3;
+        for (int64_t at = a.starts[brow]; at < a.starts[brow + 1]; ++at)
+            for (size_t q = 0; q < 3; ++q) {
+                const size_t col = (size_t)a.columns[(size_t)at] * 3 + q;
+                const double v = a.values[(size_t)at * 9 + row * 3 + q];
+                if (col == dof) p.diagonal.push_back(v);
+                else if (!constrained[col]) p.entries.push_back({col, v});
+            }
+        p.starts.push_back((int64_t)p.entries.size());
+    }
+    return p;
+}
+
+void apply_prescribed_rhs(const PrescribedRhs &p, std::vector<double> &rhs) {
+    for (size_t k = 0; k < p.prescribed.size(); ++k)
+        for (int64_t at = p.starts[k]; at < p.starts[k + 1]; ++at)
+            rhs[p.entries[(size_t)at].first] -= p.entries[(size_t)at].second * p.prescribed[k].second;
+    for (size_t k = 0; k < p.prescribed.size(); ++k) rhs[p.prescribed[k].first] = p.diagonal[k] * p.prescribed[k].second;
+}
+
 FreeProblem number_and_load(const Model &model) {
     const size_t nn = model.mesh.nodes.size();
     // study.rs:419-432: the fixed flags, then the check that every body is held
@@ -723,25 +832,7 @@ FreeProblem number_and_load(const Model &model) {
     else
         check_held_dofs(model.mesh, is_fixed, model.prescribed);
     clock.lap("  held-body validation");
-    // crates/solidfem-elastic/src/study.rs:696-735: nodal forces, then tractions, in the order given
-    std::vector<double> load(3 * nn, 0.0);
-    for (const auto &f : model.nodal_forces) {
-        if (f.first >= nn) throw Error(SF_ERR_MODEL_NODE_OUT_OF_RANGE, "force node out of range", f.first);
-        for (double c : f.second)
-            if (!std::isfinite(c)) throw Error(SF_ERR_MODEL_NON_FINITE_LOAD, "a load is not finite");
-        for (int a = 0; a < 3; ++a) load[3 * (size_t)f.first + a] += f.second[a];
-    }
-    if (!model.tractions.empty()) {
-        std::vector<Face> boundary = model.mesh.boundary_faces();
-        std::sort(boundary.begin(), boundary.end());
-        for (const auto &t : model.tractions) {
-            if (!std::binary_search(boundary.begin(), boundary.end(), t.first))
-                throw Error(SF_ERR_MODEL_FACE_NOT_ON_BOUNDARY, "the face is not on the boundary", t.first.tet, t.first.face);
-            for (double c : t.second)
-                if (!std::isfinite(c)) throw Error(SF_ERR_MODEL_NON_FINITE_LOAD, "a traction is not finite");
-            face_load(model.mesh, t.first, t.second, load);
-        }
-    }
+    const std::vector<double> load = model_load(model);
     clock.lap("  loads");
     FreeProblem p;
     p.free.assign(nn, FIXED);
@@ -772,6 +863,7 @@ FreeProblem number_and_load(const Model &model) {
 void apply_prescribed(HostBsr<double> &a, std::vector<double> &rhs, const std::vector<std::pair<size_t, double>> &prescribed) {
     if (a.rb != 3 || a.cb != 3 || a.values.size() != a.blocks() * 9 || rhs.size() != a.dofs())
         throw Error(SF_ERR_INTERNAL, "prescribed elimination needs the assembled 3x3 blocks and a matching right-hand side");
+    const PrescribedRhs program = prescribed_rhs_program(a, prescribed);
     // the slot of block (row, col), which the symmetric pattern holds for every stored mirror
     auto slot = [&](size_t brow, size_t bcol) -> size_t {
         const u32 *first = a.columns.data() + a.starts[brow], *last = a.columns.data() + a.starts[brow + 1];
@@ -779,12 +871,11 @@ void apply_prescribed(HostBsr<double> &a, std::vector<double> &rhs, const std::v
         if (it == last || *it != bcol) throw Error(SF_ERR_INTERNAL, "the stiffness pattern is not symmetric");
         return (size_t)(it - a.columns.data());
     };
-    std::vector<char> constrained(rhs.size(), 0);
-    for (const auto &pr : prescribed) constrained[pr.first] = 1;
     std::vector<std::pair<size_t, double>> row;
     for (const auto &pr : prescribed) {
         const size_t dof = pr.first, brow = dof / 3, p = dof % 3;
-        // the row's off-diagonal entries in column order, as they stand before this row's updates
+        // the row's off-diagonal entries in column order, as they stand before this row's updates,
+        // cancelled in the row and in its mirror column
         row.clear();
         for (int64_t at = a.starts[brow]; at < a.starts[brow + 1]; ++at)
             for (size_t q = 0; q < 3; ++q) {
@@ -793,15 +884,11 @@ void apply_prescribed(HostBsr<double> &a, std::vector<double> &rhs, const std::v
             }
         for (const auto &e : row) {
             const size_t col = e.first;
-            if (!constrained[col]) rhs[col] -= e.second * pr.second;
             a.values[slot(col / 3, brow) * 9 + (col % 3) * 3 + p] += -e.second;
             a.values[slot(brow, col / 3) * 9 + p * 3 + col % 3] += -e.second;
         }
     }
-    for (const auto &pr : prescribed) {
-        const size_t dof = pr.first;
-        rhs[dof] = a.values[slot(dof / 3, dof / 3) * 9 + (dof % 3) * 4] * pr.second;
-    }
+    apply_prescribed_rhs(program, rhs);
 }
 
 std::vector<double> traction_rhs(const QuadMesh &mesh, const std::vector<Traction> &tractions, const std::vector<u32> &free,

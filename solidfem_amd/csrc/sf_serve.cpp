@@ -24,6 +24,36 @@ std::string system_key(const sf_workload &w) {
     return buf;
 }
 
+// A model request's system: its mesh, fixed nodes, prescribed components, material and rule, as
+// the bytes of an FNV-1a hash (with the counts, against collisions of unequal sizes).
+std::string model_key(const Model &m) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](const void *p, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; ++i) {
+            h ^= b[i];
+            h *= 0x00000100000001b3ull;
+        }
+    };
+    mix(m.mesh.nodes.data(), m.mesh.nodes.size() * sizeof(Vec3));
+    mix(m.mesh.tets.data(), m.mesh.tets.size() * sizeof(m.mesh.tets[0]));
+    for (u32 f : m.fixed) mix(&f, sizeof f);
+    for (const auto &p : m.prescribed) {
+        mix(&p.first.first, sizeof(u32));
+        mix(&p.first.second, sizeof(u32));
+        mix(&p.second, sizeof(double));
+    }
+    const MaterialField &f = m.field;
+    mix(&f.kind, sizeof f.kind);
+    for (double v : {f.e, f.nu, f.core_e, f.core_nu, f.shell_depth, f.lo[0], f.lo[1], f.lo[2], f.hi[0], f.hi[1], f.hi[2]}) mix(&v, sizeof v);
+    mix(f.transverse, sizeof f.transverse);
+    mix(&m.quadrature, sizeof m.quadrature);
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "model %016llx %zu %zu %zu", (unsigned long long)h, m.mesh.nodes.size(), m.mesh.tets.size(),
+                  m.prescribed.size());
+    return buf;
+}
+
 void check(sf_status s) {
     if (s == SF_OK) return;
     int64_t a = 0, b = 0;
@@ -67,12 +97,21 @@ Server::~Server() {
     contexts_.clear();
 }
 
-u64 Server::estimate(const sf_workload &w) const {
+u64 Server::estimate(const ServeRequest &r) const {
     // A structured box of Tet10: (2 nx + 1)(2 ny + 1)(2 nz + 1) nodes with at most 27 blocks per
-    // block row. The system holds the assembled f64 blocks twice (row-major and sliced), the
-    // hierarchy about 0.6 of that again in f64, and a full batch 14 vectors of every column.
-    const double nodes = (2.0 * w.divisions[0] + 1) * (2.0 * w.divisions[1] + 1) * (2.0 * w.divisions[2] + 1);
-    const double dofs = 3 * nodes, blocks = 27 * nodes;
+    // block row; a model's mesh: its nodes, and at most its 100 block pairs per element. The system
+    // holds the assembled f64 blocks twice (row-major and sliced), the hierarchy about 0.6 of that
+    // again in f64, and a full batch 14 vectors of every column.
+    const sf_workload &w = r.workload;
+    double nodes, blocks;
+    if (r.model) {
+        nodes = (double)r.model->mesh.nodes.size();
+        blocks = std::min(100.0 * (double)r.model->mesh.tets.size(), nodes * nodes);
+    } else {
+        nodes = (2.0 * w.divisions[0] + 1) * (2.0 * w.divisions[1] + 1) * (2.0 * w.divisions[2] + 1);
+        blocks = 27 * nodes;
+    }
+    const double dofs = 3 * nodes;
     const double system = blocks * (72 + 80) * 1.6 + dofs * 8 * 24;
     const double batch = dofs * options_.max_columns * (10 * 8 + 1.4 * 8 * vector_size(options_.precision));
     return (u64)(system + batch);
@@ -82,6 +121,17 @@ u64 Server::submit(const sf_workload &w) {
     auto r = std::make_unique<ServeRequest>();
     r->workload = w;
     r->key = system_key(w);
+    return enqueue(std::move(r));
+}
+
+u64 Server::submit_model(const Model &model) {
+    auto r = std::make_unique<ServeRequest>();
+    r->model = std::make_shared<const Model>(model);
+    r->key = model_key(model);
+    return enqueue(std::move(r));
+}
+
+u64 Server::enqueue(std::unique_ptr<ServeRequest> r) {
     r->submitted = now();
     std::lock_guard<std::mutex> g(mu_);
     if (stop_) throw Error(SF_ERR_INVALID_ARGUMENT, "the server is stopping");
@@ -214,9 +264,16 @@ bool Server::evict_for(u64 bytes, std::vector<sf_static_study *> &doomed) {
     return true;
 }
 
-void Server::build(ServeSystem &system, const sf_workload &w, sf_context &ctx) {
+void Server::build(ServeSystem &system, const sf_workload &w, const std::shared_ptr<const Model> &from, sf_context &ctx) {
     sf_model *model = nullptr;
-    check(sf_model_from_workload(&w, &model));
+    if (from) {
+        auto m = std::make_unique<sf_model>();
+        m->model = *from;
+        m->view.mesh = m->model.mesh;
+        model = m.release();
+    } else {
+        check(sf_model_from_workload(&w, &model));
+    }
     std::unique_ptr<sf_model, void (*)(sf_model *)> hold(model, sf_model_destroy);
     const sf_pcg_options pcg{options_.tolerance, 0, options_.dot_order, 0};
     sf_multigrid_options mg{3.0, 1500, 12, 3, 30.0, options_.precision, 0, options_.coarse_solve, 0};
@@ -236,6 +293,8 @@ void Server::build(ServeSystem &system, const sf_workload &w, sf_context &ctx) {
     }
     system.study = study;
     system.boundary = study->model.mesh.boundary_faces();
+    system.sorted_boundary = system.boundary;
+    std::sort(system.sorted_boundary.begin(), system.sorted_boundary.end());
 }
 
 void Server::builder_loop() {
@@ -254,6 +313,7 @@ void Server::builder_loop() {
     for (;;) {
         ServeSystem *system = nullptr;
         sf_workload w{};
+        std::shared_ptr<const Model> m;
         u64 first = 0;
         std::vector<sf_static_study *> doomed;
         {
@@ -281,7 +341,7 @@ void Server::builder_loop() {
                 }
                 if (!unbuilt_.empty()) {
                     ServeRequest &r = *requests_.at(unbuilt_.front());
-                    const u64 need = estimate(r.workload);
+                    const u64 need = estimate(r);
                     if (need > options_.hbm_budget) {
                         unbuilt_.pop_front();
                         r.error = Error(SF_ERR_SERVE_OVER_BUDGET, "the request needs more HBM than the whole budget", (int64_t)need,
@@ -302,6 +362,7 @@ void Server::builder_loop() {
                         in_use_ += need;
                         peak_ = std::max(peak_, in_use_);
                         w = r.workload;
+                        m = r.model;
                         first = r.id;
                         break;
                     }
@@ -315,7 +376,7 @@ void Server::builder_loop() {
         const double t0 = now();
         Error failure{SF_OK, ""};
         try {
-            build(*system, w, *raw);
+            build(*system, w, m, *raw);
         } catch (const Error &e) {
             failure = e;
         } catch (const std::exception &e) {
@@ -406,12 +467,26 @@ void Server::serve(ServeSystem &system, const std::shared_ptr<Context> &ctx) {
     const QuadMesh &mesh = st.model.mesh;
     const FreeProblem &numbering = st.problem;
     const size_t n = st.system.system.size;
-    // the right-hand side of a request, or an error that fails it alone
-    auto load_of = [&](const sf_workload &w, std::vector<double> &rhs) -> Error {
+    // the right-hand side of a request, or an error that fails it alone: a workload's tractions,
+    // or a model's forces and tractions with the system's prescribed components eliminated as
+    // the study eliminated them (study.rs:419-474, :567-593)
+    auto load_of = [&](const sf_workload &w, const std::shared_ptr<const Model> &m, std::vector<double> &rhs) -> Error {
         try {
-            rhs = traction_rhs(mesh, workload_tractions(mesh, system.boundary, w), numbering.free, numbering.free_count);
             bool any = false;
-            for (double v : rhs) any = any || v != 0.0;
+            if (m) {
+                const std::vector<double> load = model_load(*m, &system.sorted_boundary);
+                rhs.clear();
+                rhs.reserve(3 * numbering.free_count);
+                for (size_t node = 0; node < mesh.nodes.size(); ++node)
+                    if (numbering.free[node] != FIXED)
+                        for (int a = 0; a < 3; ++a) rhs.push_back(load[3 * node + a]);
+                for (double v : rhs) any = any || v != 0.0;
+                for (const auto &pr : st.prescribed_rhs.prescribed) any = any || pr.second != 0.0;
+                apply_prescribed_rhs(st.prescribed_rhs, rhs);
+            } else {
+                rhs = traction_rhs(mesh, workload_tractions(mesh, system.boundary, w), numbering.free, numbering.free_count);
+                for (double v : rhs) any = any || v != 0.0;
+            }
             if (!any) return Error(SF_ERR_MODEL_NO_EFFECTIVE_LOAD, "the model has no effective load");
             return Error(SF_OK, "");
         } catch (const Error &e) {
@@ -420,7 +495,12 @@ void Server::serve(ServeSystem &system, const std::shared_ptr<Context> &ctx) {
     };
     // takes up to `room` waiting requests, failing those whose loads are refused
     auto take = [&](size_t room, std::vector<u64> &ids, std::vector<std::vector<double>> &rhs) {
-        std::vector<std::pair<u64, sf_workload>> picked;
+        struct Picked {
+            u64 id;
+            sf_workload w;
+            std::shared_ptr<const Model> m;
+        };
+        std::vector<Picked> picked;
         {
             std::lock_guard<std::mutex> g(mu_);
             while (picked.size() < room && !system.waiting.empty()) {
@@ -430,21 +510,21 @@ void Server::serve(ServeSystem &system, const std::shared_ptr<Context> &ctx) {
                 if (it == requests_.end()) continue;
                 it->second->state = SF_SERVE_SOLVING;
                 it->second->solve_started = now();
-                picked.push_back({id, it->second->workload});
+                picked.push_back({id, it->second->workload, it->second->model});
             }
         }
         for (auto &p : picked) {
             std::vector<double> b;
-            Error e = load_of(p.second, b);
+            Error e = load_of(p.w, p.m, b);
             if (e.status != SF_OK) {
                 std::lock_guard<std::mutex> g(mu_);
-                ServeRequest &r = *requests_.at(p.first);
+                ServeRequest &r = *requests_.at(p.id);
                 r.error = e;
                 finish_request(r, SF_SERVE_FAILED);
                 cv_.notify_all();
                 continue;
             }
-            ids.push_back(p.first);
+            ids.push_back(p.id);
             rhs.push_back(std::move(b));
         }
     };
@@ -465,6 +545,8 @@ void Server::serve(ServeSystem &system, const std::shared_ptr<Context> &ctx) {
         for (size_t node = 0; node < mesh.nodes.size(); ++node)
             if (numbering.free[node] != FIXED)
                 for (int a = 0; a < 3; ++a) nodal[3 * node + a] = x[3 * (size_t)numbering.free[node] + a];
+        // a prescribed component is exact (study.rs:233-235); the system's requests share them
+        for (const auto &pr : st.model.prescribed) nodal[3 * (size_t)pr.first.first + pr.first.second] = pr.second;
         return nodal;
     };
     // Fewer than four requests: each runs the single-column solve in turn (the same bits as a

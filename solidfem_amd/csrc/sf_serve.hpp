@@ -21,6 +21,7 @@ namespace sf {
 struct ServeRequest {
     u64 id = 0;
     sf_workload workload{};
+    std::shared_ptr<const Model> model;  // a model request (any mesh); null for a box workload
     std::string key;
     int state = SF_SERVE_QUEUED;
     Error error{SF_OK, ""};
@@ -35,6 +36,7 @@ struct ServeSystem {
     std::string key;
     sf_static_study *study = nullptr;   // the assembled matrix and the device hierarchy
     std::vector<Face> boundary;         // the mesh's boundary faces in order
+    std::vector<Face> sorted_boundary;  // the same, sorted (a model request's traction faces are checked against it)
     std::deque<u64> waiting;            // requests of this system not yet in a batch
     bool ready = false, busy = false;
     u64 bytes = 0;                      // reserved HBM: the system and one full batch
@@ -46,6 +48,9 @@ class Server {
     Server(int device, const sf_server_options &options);
     ~Server();
     u64 submit(const sf_workload &w);
+    // A model of any mesh: requests whose models share mesh, fixed nodes, prescribed components and
+    // material share one system; their forces and tractions are their own.
+    u64 submit_model(const Model &model);
     sf_serve_status status(u64 id);
     // Waits up to `timeout` seconds (negative: without limit) for the request to end.
     sf_serve_status wait(u64 id, double timeout);
@@ -56,11 +61,12 @@ class Server {
    private:
     void builder_loop();
     void solver_loop();
-    void build(ServeSystem &system, const sf_workload &w, sf_context &ctx);
+    void build(ServeSystem &system, const sf_workload &w, const std::shared_ptr<const Model> &model, sf_context &ctx);
+    u64 enqueue(std::unique_ptr<ServeRequest> r);
     void serve(ServeSystem &system, const std::shared_ptr<Context> &ctx);
     void finish_request(ServeRequest &r, int state);
     sf_serve_status describe(const ServeRequest &r) const;
-    u64 estimate(const sf_workload &w) const;
+    u64 estimate(const ServeRequest &r) const;
     bool evict_for(u64 bytes, std::vector<sf_static_study *> &doomed);
     ServeRequest &request(u64 id);
 

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import solidfem_amd as sf
-from workloads import block_config
+from workloads import block_config, stretch_components, stretched_cube
 
 pytestmark = pytest.mark.gpu
 
@@ -110,3 +110,54 @@ def test_a_request_without_load_fails_by_name(ctx):
     srv.release(ok)
     with pytest.raises(sf.SolidfemError):
         srv.result(ok)
+
+
+def text_model(text, youngs=70000.0, poisson=0.33, traction=(0.0, 0.0, -1.0)):
+    """A model of a mesh sent as text: fixed at x = 0, loaded on the faces at x = max."""
+    mesh = sf.QuadraticTetMesh.from_linear(sf.TetMesh.read_text(text))
+    model = sf.Model.new(mesh, youngs, poisson)
+    nodes = model.mesh().nodes()
+    for n in np.flatnonzero(np.abs(nodes[:, 0]) < 1e-9):
+        model.fix_node(int(n))
+    xmax = nodes[:, 0].max()
+    for face in model.mesh().boundary_faces():
+        if np.all(np.abs(nodes[face[2:5], 0] - xmax) < 1e-9):
+            model.add_traction(face, traction)
+    return model
+
+
+def model_alone(ctx, model):
+    b = sf.StudyBuilder.new(ctx, model, sf.PcgOptions(tolerance=OPTIONS["tolerance"], dot_order=OPTIONS["dot_order"]),
+                            sf.MultigridOptions(precision=OPTIONS["precision"], coarse_solve=OPTIONS["coarse_solve"]))
+    b.step(2 ** 62)
+    s = b.finish()
+    p = s.solve()
+    assert p.outcome == sf.StepOutcome.CONVERGED
+    return s.displacements(), p.iteration
+
+
+def test_models_of_meshes_sent_as_text_equal_themselves_alone(ctx):
+    # an unstructured-looking mesh (a box with its interior vertices moved) sent as text: three
+    # load cases share its system, another material makes a second, and the stretched cube with
+    # prescribed components a third, all in flight together
+    box = sf.box_tet_mesh((0.0, 0.0, 0.0), (24.0, 4.0, 4.0), (12, 3, 3))
+    pos, tets = box.positions(), box.tets()
+    inside = np.all((pos > 1e-9) & (pos < np.array([24.0, 4.0, 4.0]) - 1e-9), axis=1)
+    pos[inside] += np.random.default_rng(9).uniform(-0.2, 0.2, (inside.sum(), 3))
+    text = sf.TetMesh.text_of(pos, tets)
+    models = [text_model(text, traction=t) for t in ((0.0, 0.0, -1.0), (0.0, 0.5, 0.0), (0.2, -0.3, 0.7))]
+    models.append(text_model(text, youngs=3500.0, poisson=0.36))
+    cube = sf.Model.from_workload(stretched_cube((2, 2, 2)))
+    for n, a, v in stretch_components(cube.mesh().nodes()):
+        cube.prescribe_displacement(n, a, v)
+    models.append(cube)
+    srv = server(max_columns=4, slice_iterations=2)
+    ids = [srv.submit_model(m) for m in models]
+    states = [srv.wait(i, 300.0) for i in ids]
+    for k, (m, rid, st) in enumerate(zip(models, ids, states)):
+        assert st.state == sf.ServeState.DONE, (k, st)
+        ref, iterations = model_alone(ctx, m)
+        assert st.iterations == iterations, k
+        assert np.array_equal(bits(srv.result(rid)), bits(ref)), k
+    assert sum(s.cache_hit for s in states) == 2  # the second and third load cases of the shared mesh
+    assert srv.stats()["builds"] == 3
