@@ -180,6 +180,7 @@ typedef struct sf_report {
     double preconditioner_seconds;
     sf_multigrid_stats multigrid;
     double solve_seconds;
+    double recovery_seconds; /* the study's recoveries (Report::recovery_seconds, study.rs:175) */
 } sf_report;
 
 /* BuildProgress, crates/solidfem-elastic/src/study.rs:377-387 */
@@ -473,6 +474,65 @@ sf_status sf_recover_stress(sf_context *ctx, const sf_model *model, const double
                             double *nodal_values);
 /* The same for a study's current displacements (Recovery as StaticStudy drives it, study.rs). */
 sf_status sf_static_study_recover(sf_static_study *study, double *point_values, double *nodal_values);
+/* The resumable form (Recovery, results.rs:103-124 and :440-536): a snapshot of displacements
+ * recovered in slices -- every element in order, then one patch fit per mesh vertex in order --
+ * whose finished field has the bits of sf_recover_stress whatever the slicing. The recovery
+ * holds its own device copies of what it reads and outlives the model or study it came from. */
+typedef struct sf_recovery sf_recovery;
+typedef struct sf_recovery_state {
+    uint64_t elements, elements_done; /* Recovery::elements_done */
+    uint64_t vertices, fits_done;     /* patch fits, one per mesh vertex */
+    uint64_t nodes, points_per_element;
+    int32_t converged;                /* the displacements are a converged solve's */
+    int32_t finished;
+} sf_recovery_state;
+/* A recovery of `displacements` (3 per mesh node) over `model`, none of it done. */
+sf_status sf_recovery_create(sf_context *ctx, const sf_model *model, const double *displacements, int32_t converged,
+                             sf_recovery **out);
+/* StaticStudy::begin_recovery (study.rs:263-273): the study's current displacements, the converged
+ * answer or a preview of a solve in progress. */
+sf_status sf_static_study_begin_recovery(sf_static_study *study, sf_recovery **out);
+/* Recovery::advance (results.rs:477-536): up to max_work more units, an element, then a patch fit;
+ * fits start only once every element is done. `cancel` (may be NULL) is read before each unit:
+ * SF_ERR_MODEL_CANCELLED, with the units before it done. A faulting element fails with
+ * SF_ERR_MODEL_INVERTED_JACOBIAN (a = the element), which stays the next to do. `finished`
+ * (may be NULL) is set to whether every unit is done. */
+sf_status sf_recovery_advance(sf_recovery *recovery, uint64_t max_work, sf_cancel_fn cancel, void *user, int32_t *finished);
+/* StaticStudy::continue_recovery (study.rs:275-292): sf_recovery_advance, its time added to the
+ * study's recovery_seconds. */
+sf_status sf_static_study_continue_recovery(sf_static_study *study, sf_recovery *recovery, uint64_t max_work,
+                                            sf_cancel_fn cancel, void *user, int32_t *finished);
+sf_status sf_recovery_state_of(const sf_recovery *recovery, sf_recovery_state *out);
+/* The displacements being recovered, 3 per mesh node (Recovery::displacements). */
+sf_status sf_recovery_displacements(const sf_recovery *recovery, double *out);
+/* The finished field in sf_recover_stress's layout (either output may be NULL);
+ * SF_ERR_MODEL_NOT_FINISHED (a = elements done, b = fits done) while units are left
+ * (StaticStudy::finish_recovery, study.rs:294-300). */
+sf_status sf_recovery_field(sf_recovery *recovery, double *point_values, double *nodal_values);
+void sf_recovery_destroy(sf_recovery *recovery);
+
+/* ---- peak stress (results.rs:149-226) ---------------------------------------------------------
+ * The peak von Mises over the positions `inside` accepts (NULL: everywhere): the mesh nodes in
+ * order, then the integration points in order, each classified by its own physical position. A
+ * later candidate replaces the peak only when strictly larger, so the first of equal values holds
+ * it; the first NaN met becomes the peak and stays it. node_positions: 3 per node; the nodal von
+ * Mises of node n at nodal_von_mises[n * nodal_stride] (7 with sf_recover_stress's nodal values,
+ * reading from their von Mises column); point_values: 10 per point as sf_recover_stress writes
+ * them. peak_von_mises is this peak's value. */
+typedef int32_t (*sf_region_fn)(const double *position, void *user);
+typedef struct sf_peak {
+    int32_t found;     /* 0: no node and no point is inside */
+    int32_t at_point;  /* held by an integration point (else a mesh node): PeakHolder */
+    uint64_t index;    /* of that node or point */
+    double von_mises;
+    double position[3];
+} sf_peak;
+sf_status sf_peak_stress(const double *node_positions, uint64_t nodes, const double *nodal_von_mises, uint64_t nodal_stride,
+                         const double *point_values, uint64_t points, sf_region_fn inside, void *user, sf_peak *out);
+/* nodal_rotations (results.rs:262-305): the local rotation (half the curl of the displacement), in
+ * radians, at every mesh node -- each incident element's value at the node weighted by its
+ * Jacobian determinant there, summed in element order. rotations: 3 per node. */
+sf_status sf_nodal_rotations(sf_context *ctx, const sf_model *model, const double *displacements, double *rotations);
 /* A study's mesh nodes, elements and integration points per element. */
 sf_status sf_static_study_counts(const sf_static_study *study, uint64_t *nodes, uint64_t *elements, uint64_t *points);
 /* Integration points per element of a model's rule (4 or 14). */

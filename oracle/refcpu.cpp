@@ -641,6 +641,32 @@ int ref_recover(const ref_problem *p, const double *u, double *point_values, dou
     return 0;
 }
 int ref_problem_quadrature(const ref_problem *p) { return p->p->quadrature; }
+// peak_with_holder (results.rs:184-226) over the candidates the masks keep (NULL: all). Returns 1
+// with the peak, 0 when no candidate is inside.
+int ref_peak(const double *positions, uint64_t nodes, const double *nodal_vm, uint64_t stride, const double *points, uint64_t count,
+             const uint8_t *node_inside, const uint8_t *point_inside, int32_t *at_point, uint64_t *index, double *von_mises,
+             double *position) {
+    const Peak peak = peak_of(positions, (size_t)nodes, nodal_vm, (size_t)stride, points, (size_t)count, node_inside, point_inside);
+    if (!peak.found) return 0;
+    *at_point = peak.at_point ? 1 : 0;
+    *index = peak.index;
+    *von_mises = peak.von_mises;
+    for (int a = 0; a < 3; ++a) position[a] = peak.position[a];
+    return 1;
+}
+// nodal_rotations (results.rs:262-305) of nodal displacements `u` (3 per node): 3 per node. Returns
+// 0, or -1 with the failing element in *fault_tet.
+int ref_nodal_rotations(const ref_problem *p, const double *u, double *out, int64_t *fault_tet) {
+    const Problem &pr = *p->p;
+    std::vector<double> disp(u, u + 3 * pr.mesh.nodes.size()), rotations;
+    const int64_t fault = nodal_rotations(pr.mesh, disp, rotations);
+    if (fault >= 0) {
+        if (fault_tet) *fault_tet = fault;
+        return -1;
+    }
+    std::copy(rotations.begin(), rotations.end(), out);
+    return 0;
+}
 
 int ref_problem_element_stiffness(const ref_problem *p, int64_t tet, double *ke) {
     const auto &t = p->p->mesh.tets[(size_t)tet];

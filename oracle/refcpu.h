@@ -132,6 +132,13 @@ void ref_problem_rigid_modes(const ref_problem *p, double *out); /* 6 per free d
 int ref_problem_element_stiffness(const ref_problem *p, int64_t tet, double *ke);
 int ref_recover(const ref_problem *p, const double *u, double *point_values, double *nodal_values, int64_t *fault_tet);
 int ref_problem_quadrature(const ref_problem *p);
+/* peak_with_holder over the nodes, then the points, the masks keep (NULL: all); 1 with the peak,
+ * 0 when none is inside */
+int ref_peak(const double *positions, uint64_t nodes, const double *nodal_vm, uint64_t stride, const double *points, uint64_t count,
+             const uint8_t *node_inside, const uint8_t *point_inside, int32_t *at_point, uint64_t *index, double *von_mises,
+             double *position);
+/* nodal_rotations: 3 per node; 0, or -1 with the failing element */
+int ref_nodal_rotations(const ref_problem *p, const double *u, double *out, int64_t *fault_tet);
 double ref_problem_assembly_seconds(const ref_problem *p);
 
 /* Multigrid hierarchy built over the problem's matrix. */

@@ -274,4 +274,66 @@ inline RecoveredField recover(const QuadMesh &mesh, const Field &field, int rule
     return out;
 }
 
+// Peak (peak_with_holder, results.rs:184-226): the candidates are the nodes, then the points, that
+// `node_inside` / `point_inside` keep (the caller's region test, evaluated by the caller); the
+// fold keeps the peak when it is NaN, else takes a candidate that is NaN or strictly larger.
+struct Peak {
+    bool found = false, at_point = false;
+    size_t index = 0;
+    double von_mises = 0.0;
+    double position[3] = {0.0, 0.0, 0.0};
+};
+
+inline Peak peak_of(const double *positions, size_t nodes, const double *nodal_vm, size_t stride, const double *points,
+                    size_t count, const unsigned char *node_inside, const unsigned char *point_inside) {
+    Peak peak;
+    auto fold = [&](bool at_point, size_t index, double value, const double *position) {
+        if (peak.found) {
+            if (std::isnan(peak.von_mises)) return;
+            if (!(std::isnan(value) || value > peak.von_mises)) return;
+        }
+        peak.found = true;
+        peak.at_point = at_point;
+        peak.index = index;
+        peak.von_mises = value;
+        for (int a = 0; a < 3; ++a) peak.position[a] = position[a];
+    };
+    for (size_t n = 0; n < nodes; ++n)
+        if (!node_inside || node_inside[n]) fold(false, n, nodal_vm[n * stride], positions + 3 * n);
+    for (size_t k = 0; k < count; ++k)
+        if (!point_inside || point_inside[k]) fold(true, k, points[10 * k + 9], points + 10 * k);
+    return peak;
+}
+
+// nodal_rotations (results.rs:262-305): per element in order, per node in order, the node's
+// geometry, rotation_at (element.rs:149-156) and the determinant-weighted sums; then each node's
+// sum over its weight. Returns the first element whose geometry fails at a node, else -1.
+inline int64_t nodal_rotations(const QuadMesh &mesh, const std::vector<double> &u, std::vector<double> &out) {
+    const size_t nodes = mesh.nodes.size();
+    std::vector<double> weighted(3 * nodes, 0.0), weights(nodes, 0.0);
+    for (size_t t = 0; t < mesh.tets.size(); ++t) {
+        const auto &tet = mesh.tets[t];
+        Vec3 en[10];
+        double ue[10][3];
+        for (int a = 0; a < 10; ++a) {
+            en[a] = mesh.nodes[tet[a]];
+            for (int i = 0; i < 3; ++i) ue[a][i] = u[3 * (size_t)tet[a] + i];
+        }
+        for (int n = 0; n < 10; ++n) {
+            PointGeometry g;
+            if (!point_geometry(en, NODE_NATURAL[n], g)) return (int64_t)t;
+            double h[3][3];
+            displacement_gradient(g, ue, h);
+            const double rotation[3] = {0.5 * (h[2][1] - h[1][2]), 0.5 * (h[0][2] - h[2][0]), 0.5 * (h[1][0] - h[0][1])};
+            const size_t global = tet[n];
+            for (int axis = 0; axis < 3; ++axis) weighted[3 * global + axis] += g.det_j * rotation[axis];
+            weights[global] += g.det_j;
+        }
+    }
+    out.resize(3 * nodes);
+    for (size_t n = 0; n < nodes; ++n)
+        for (int axis = 0; axis < 3; ++axis) out[3 * n + axis] = weighted[3 * n + axis] / weights[n];
+    return -1;
+}
+
 }  // namespace refcpu

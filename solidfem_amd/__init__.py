@@ -23,7 +23,7 @@ __all__ = [
     "Context", "Precision", "DotOrder", "PcgOptions", "MultigridOptions", "Workload", "TetMesh", "QuadraticTetMesh",
     "box_tet_mesh", "Model", "StudyBuilder", "StaticStudy", "CsrMatrix", "MultigridSetup", "Multigrid", "Pcg",
     "StepOutcome", "SolidfemError", "PcgError", "MultigridError", "ModelError", "MeshError", "library",
-    "SetupDevice",
+    "SetupDevice", "Recovery", "PeakStress", "peak_stress", "peak_von_mises",
 ]
 
 
@@ -369,6 +369,29 @@ class Model(_Handle):
     def points_per_element(self) -> int:
         return library().sf_model_points_per_element(self._ptr)
 
+    def begin_recovery(self, ctx: "Context", displacements, converged: bool = False) -> "Recovery":
+        """A resumable recovery of `displacements` (3 per mesh node) over this model, none of it done
+        (Recovery::new, results.rs:440-467)."""
+        u = self._displacements(displacements)
+        h = C.c_void_p()
+        _raise(library().sf_recovery_create(ctx._ptr, self._ptr, _ptr(u, C.c_double), 1 if converged else 0, C.byref(h)))
+        return Recovery(h)
+
+    def nodal_rotations(self, ctx: "Context", displacements) -> np.ndarray:
+        """nodal_rotations (results.rs:262-305): the local rotation, half the curl of the
+        displacement, in radians at every mesh node, (nodes, 3)."""
+        u = self._displacements(displacements)
+        out = np.empty((u.size // 3, 3))
+        _raise(library().sf_nodal_rotations(ctx._ptr, self._ptr, _ptr(u, C.c_double), _ptr(out, C.c_double)))
+        return out
+
+    def _displacements(self, displacements) -> np.ndarray:
+        u = np.ascontiguousarray(displacements, dtype=np.float64).reshape(-1)
+        if u.size != 3 * self.mesh().nodes().shape[0]:
+            # results.rs:268-273 (ModelError::DisplacementFieldLength)
+            raise ValueError("displacements must hold 3 values per mesh node")
+        return u
+
     def recover(self, ctx: "Context", displacements):
         """Stress recovery on the device (results.rs:307-576): (point_values (elements, points, 10):
         position, stress [xx, yy, zz, xy, yz, zx], von Mises; nodal_values (nodes, 7): recovered
@@ -492,6 +515,26 @@ class StaticStudy(_Handle):
         nodal = np.empty((nodes.value, 7))
         _raise(library().sf_static_study_recover(self._ptr, _ptr(points, C.c_double), _ptr(nodal, C.c_double)))
         return points, nodal
+
+    def begin_recovery(self) -> "Recovery":
+        """StaticStudy::begin_recovery (study.rs:263-273): a resumable recovery of the current
+        displacements, the converged answer or a preview of a solve in progress."""
+        h = C.c_void_p()
+        _raise(library().sf_static_study_begin_recovery(self._ptr, C.byref(h)))
+        return Recovery(h)
+
+    def continue_recovery(self, recovery: "Recovery", max_work: int, cancel=None) -> bool:
+        """StaticStudy::continue_recovery (study.rs:275-292): up to max_work more units; True once
+        all are done. The time joins the report's recovery_seconds."""
+        done = C.c_int32()
+        _raise(library().sf_static_study_continue_recovery(self._ptr, recovery._ptr, int(max_work), _cancel(cancel), None,
+                                                           C.byref(done)))
+        return bool(done.value)
+
+    def finish_recovery(self, recovery: "Recovery"):
+        """StaticStudy::finish_recovery (study.rs:294-300): the finished field; refuses one with
+        units left (ModelError NOT_FINISHED)."""
+        return recovery.field()
 
     def rhs(self) -> np.ndarray:
         out = np.empty(self.report().free_dofs)
@@ -991,3 +1034,89 @@ def profile_read(kernel: Kernel):
     n, t, b = C.c_uint64(), C.c_double(), C.c_double()
     _raise(library().sf_profile_read(int(kernel), C.byref(n), C.byref(t), C.byref(b)))
     return n.value, t.value, b.value
+
+
+class Recovery(_Handle):
+    """A resumable stress recovery on the device (Recovery, results.rs:103-124 and :440-536): every
+    element in order, then one patch fit per mesh vertex in order, in slices of any size; the
+    finished field has the bits of a recovery in one call."""
+    _destroy = "sf_recovery_destroy"
+
+    def advance(self, max_work: int, cancel=None) -> bool:
+        """Up to max_work more units (an element, then a patch fit), `cancel` read before each;
+        True once all are done."""
+        done = C.c_int32()
+        _raise(library().sf_recovery_advance(self._ptr, int(max_work), _cancel(cancel), None, C.byref(done)))
+        return bool(done.value)
+
+    def state(self) -> _capi.RecoveryState:
+        st = _capi.RecoveryState()
+        _raise(library().sf_recovery_state_of(self._ptr, C.byref(st)))
+        return st
+
+    def elements_done(self) -> int:
+        return self.state().elements_done
+
+    def is_finished(self) -> bool:
+        return bool(self.state().finished)
+
+    def is_converged(self) -> bool:
+        return bool(self.state().converged)
+
+    def displacements(self) -> np.ndarray:
+        out = np.empty((self.state().nodes, 3))
+        _raise(library().sf_recovery_displacements(self._ptr, _ptr(out, C.c_double)))
+        return out
+
+    def field(self):
+        """(point_values (elements, points, 10), nodal_values (nodes, 7)), as Model.recover; refuses
+        while units are left (ModelError NOT_FINISHED)."""
+        st = self.state()
+        points = np.empty((st.elements, st.points_per_element, 10))
+        nodal = np.empty((st.nodes, 7))
+        _raise(library().sf_recovery_field(self._ptr, _ptr(points, C.c_double), _ptr(nodal, C.c_double)))
+        return points, nodal
+
+
+@dataclass
+class PeakStress:
+    """PeakStress (results.rs:141-147), with what holds it (PeakHolder, results.rs:178-182)."""
+    von_mises: float
+    position: np.ndarray
+    at_point: bool
+    index: int
+
+
+def peak_stress(positions, nodal_von_mises, point_values, inside: Optional[Callable[[np.ndarray], bool]] = None):
+    """peak_stress (results.rs:164-176): the peak von Mises over the mesh nodes (positions (nodes, 3),
+    nodal_von_mises (nodes,)), then the integration points (point_values (..., 10) as recovered),
+    whose positions `inside` accepts (None: everywhere); None when none is inside."""
+    x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    vm = np.ascontiguousarray(nodal_von_mises, dtype=np.float64).reshape(-1)
+    pts = np.ascontiguousarray(point_values, dtype=np.float64).reshape(-1, 10)
+    if vm.size != x.shape[0]:
+        raise ValueError("one nodal von Mises per node")
+    failure = []
+
+    def region(p, _user):
+        try:
+            return 1 if inside(np.array(p[:3])) else 0
+        except BaseException as e:  # the callback cannot raise through C
+            failure.append(e)
+            return 0
+
+    fn = _capi.REGION_FN(region) if inside is not None else _capi.REGION_FN()
+    out = _capi.Peak()
+    _raise(library().sf_peak_stress(_ptr(x, C.c_double), x.shape[0], _ptr(vm, C.c_double), 1, _ptr(pts, C.c_double),
+                                    pts.shape[0], fn, None, C.byref(out)))
+    if failure:
+        raise failure[0]
+    if not out.found:
+        return None
+    return PeakStress(out.von_mises, np.array(out.position[:]), bool(out.at_point), out.index)
+
+
+def peak_von_mises(positions, nodal_von_mises, point_values, inside=None) -> Optional[float]:
+    """peak_von_mises (results.rs:155-162): peak_stress's value."""
+    peak = peak_stress(positions, nodal_von_mises, point_values, inside)
+    return None if peak is None else peak.von_mises

@@ -16,6 +16,7 @@ P = C.POINTER
 vp = C.c_void_p
 
 CANCEL_FN = C.CFUNCTYPE(i32, vp)
+REGION_FN = C.CFUNCTYPE(i32, P(f64), vp)
 
 
 class Progress(C.Structure):
@@ -79,7 +80,17 @@ class Report(C.Structure):
         ("preconditioner_seconds", f64),
         ("multigrid", MultigridStats),
         ("solve_seconds", f64),
+        ("recovery_seconds", f64),
     ]
+
+
+class RecoveryState(C.Structure):
+    _fields_ = [("elements", u64), ("elements_done", u64), ("vertices", u64), ("fits_done", u64), ("nodes", u64),
+                ("points_per_element", u64), ("converged", i32), ("finished", i32)]
+
+
+class Peak(C.Structure):
+    _fields_ = [("found", i32), ("at_point", i32), ("index", u64), ("von_mises", f64), ("position", f64 * 3)]
 
 
 class BuildProgress(C.Structure):
@@ -216,6 +227,16 @@ SIGNATURES = {
     "sf_multigrid_set_graph": (i32, [vp, i32]),
     "sf_recover_stress": (i32, [vp, vp, P(f64), P(f64), P(f64)]),
     "sf_static_study_recover": (i32, [vp, P(f64), P(f64)]),
+    "sf_recovery_create": (i32, [vp, vp, P(f64), i32, P(vp)]),
+    "sf_static_study_begin_recovery": (i32, [vp, P(vp)]),
+    "sf_recovery_advance": (i32, [vp, u64, CANCEL_FN, vp, P(i32)]),
+    "sf_static_study_continue_recovery": (i32, [vp, vp, u64, CANCEL_FN, vp, P(i32)]),
+    "sf_recovery_state_of": (i32, [vp, P(RecoveryState)]),
+    "sf_recovery_displacements": (i32, [vp, P(f64)]),
+    "sf_recovery_field": (i32, [vp, P(f64), P(f64)]),
+    "sf_recovery_destroy": (None, [vp]),
+    "sf_peak_stress": (i32, [P(f64), u64, P(f64), u64, P(f64), u64, REGION_FN, vp, P(Peak)]),
+    "sf_nodal_rotations": (i32, [vp, vp, P(f64), P(f64)]),
     "sf_model_points_per_element": (u64, [vp]),
     "sf_static_study_counts": (i32, [vp, P(u64), P(u64), P(u64)]),
     "sf_pcg_batch_create": (i32, [vp, vp, vp, P(f64), u64, P(PcgOptions), P(vp)]),

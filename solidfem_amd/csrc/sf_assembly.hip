@@ -13,6 +13,8 @@
 // chunked assembly continues each running sum, so chunks change no bit.
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <limits>
 
 #include "sf_device.hpp"
 
@@ -229,7 +231,8 @@ __constant__ double NODE_NATURAL[10][3] = {{0.0, 0.0, 0.0}, {1.0, 0.0, 0.0}, {0.
 
 // point_geometry (element.rs:25-80) for one thread: the element's gradients and position at a
 // natural point, the same operations in the same order as element_kernel. False when inverted.
-__device__ bool thread_geometry(const double xe[10][3], const double nat[3], double grad[10][3], double pos[3]) {
+__device__ bool thread_geometry(const double xe[10][3], const double nat[3], double grad[10][3], double pos[3],
+                                double *det_out = nullptr) {
     double l[4] = {1.0 - nat[0] - nat[1] - nat[2], nat[0], nat[1], nat[2]};
     double j[9];
     for (int i = 0; i < 3; ++i)
@@ -261,6 +264,7 @@ __device__ bool thread_geometry(const double xe[10][3], const double nat[3], dou
                                cof[k * 3 + 2] * shape_derivative(a, 2, l);
             grad[a][k] = sum / det;
         }
+    if (det_out) *det_out = det;
     return true;
 }
 
@@ -318,12 +322,13 @@ __device__ void load_element(const double *__restrict__ nodes, const uint32_t *_
 // point). `material` keeps each point's (E, nu) for the node values and the patch test of
 // uniformity.
 template <int NQ>
-__global__ __launch_bounds__(256) void recovery_points_kernel(size_t elements, const double *__restrict__ nodes,
+__global__ __launch_bounds__(256) void recovery_points_kernel(size_t first, size_t count, const double *__restrict__ nodes,
                                                               const uint32_t *__restrict__ tets, const double *__restrict__ u,
                                                               FieldArgs field, double *__restrict__ out, double *__restrict__ material,
                                                               int32_t *__restrict__ fault) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= elements * NQ) return;
+    const size_t local = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= count * NQ) return;
+    const size_t t = first * NQ + local;
     const size_t e = t / NQ;
     const int q = (int)(t - e * NQ);
     double xe[10][3], ue[10][3], grad[10][3], pos[3];
@@ -353,12 +358,13 @@ __global__ __launch_bounds__(256) void recovery_points_kernel(size_t elements, c
 // The element's stress at each of its nodes with the material of its nearest integration point
 // (Recovery::advance's node loop): one thread per (element, node).
 template <int NQ>
-__global__ __launch_bounds__(256) void recovery_nodes_kernel(size_t elements, const double *__restrict__ nodes,
+__global__ __launch_bounds__(256) void recovery_nodes_kernel(size_t first, size_t count, const double *__restrict__ nodes,
                                                              const uint32_t *__restrict__ tets, const double *__restrict__ u,
                                                              const double *__restrict__ material, const int8_t *__restrict__ nearest,
                                                              double *__restrict__ out, int32_t *__restrict__ fault) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= elements * 10) return;
+    const size_t local = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= count * 10) return;
+    const size_t t = first * 10 + local;
     const size_t e = t / 10;
     const int n = (int)(t - e * 10);
     double xe[10][3], ue[10][3], grad[10][3], pos[3];
@@ -397,11 +403,12 @@ __global__ void recovery_worst_kernel(size_t nodes, const uint32_t *__restrict__
 // order; a patch of more than one material is left unfitted. fit: flag, centre 3, scale,
 // coefficients 4 x 6 (29 values).
 template <int NQ>
-__global__ void recovery_fit_kernel(size_t vertices, const double *__restrict__ nodes, const uint32_t *__restrict__ starts,
+__global__ void recovery_fit_kernel(size_t first, size_t count, const double *__restrict__ nodes, const uint32_t *__restrict__ starts,
                                     const uint32_t *__restrict__ elems, const double *__restrict__ points,
                                     const double *__restrict__ material, double *__restrict__ fit) {
-    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= vertices) return;
+    const size_t local = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= count) return;
+    const size_t v = first + local;
     double *f = fit + v * 29;
     f[0] = 0.0;
     const double c[3] = {nodes[v * 3], nodes[v * 3 + 1], nodes[v * 3 + 2]};
@@ -518,16 +525,55 @@ FieldArgs field_args(const MaterialField &f) {
     return a;
 }
 
+// Each element's determinant-weighted rotation at its nodes (nodal_rotations, results.rs:262-299):
+// one thread per (element, node); out: det_j and det_j * rotation (4 values).
+__global__ __launch_bounds__(256) void rotation_nodes_kernel(size_t elements, const double *__restrict__ nodes,
+                                                             const uint32_t *__restrict__ tets, const double *__restrict__ u,
+                                                             double *__restrict__ out, int32_t *__restrict__ fault) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= elements * 10) return;
+    const size_t e = t / 10;
+    const int n = (int)(t - e * 10);
+    double xe[10][3], ue[10][3], grad[10][3], pos[3], det = 0.0;
+    load_element(nodes, tets, u, e, xe, ue);
+    const double nat[3] = {NODE_NATURAL[n][0], NODE_NATURAL[n][1], NODE_NATURAL[n][2]};
+    if (!thread_geometry(xe, nat, grad, pos, &det)) {
+        atomicMin(&fault[0], (int32_t)e);
+        return;
+    }
+    // rotation_at (element.rs:149-156) from displacement_gradient (element.rs:135-146)
+    double h[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 10; ++a)
+        for (int i = 0; i < 3; ++i)
+            for (int jj = 0; jj < 3; ++jj) h[i][jj] += ue[a][i] * grad[a][jj];
+    const double r[3] = {0.5 * (h[2][1] - h[1][2]), 0.5 * (h[0][2] - h[2][0]), 0.5 * (h[1][0] - h[0][1])};
+    double *o = out + t * 4;
+    o[0] = det;
+    for (int axis = 0; axis < 3; ++axis) o[1 + axis] = det * r[axis];
+}
+
+// A node's weighted mean over its incident elements, summed in element order from zero
+__global__ void rotation_gather_kernel(size_t nodes, const uint32_t *__restrict__ starts, const uint32_t *__restrict__ at,
+                                       const double *__restrict__ values, double *__restrict__ out) {
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nodes) return;
+    double sum[3] = {0.0, 0.0, 0.0}, weight = 0.0;
+    for (uint32_t k = starts[n]; k < starts[n + 1]; ++k) {
+        const double *v = values + (size_t)at[k] * 4;
+        for (int axis = 0; axis < 3; ++axis) sum[axis] += v[1 + axis];
+        weight += v[0];
+    }
+    for (int axis = 0; axis < 3; ++axis) out[n * 3 + axis] = sum[axis] / weight;
+}
+
 constexpr size_t CHUNK_ELEMENTS = 1u << 20;
+constexpr int32_t NO_FAULT = 0x7fffffff;
 
-}  // namespace
+dim3 grid_of(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-void recover_stress(const Model &model, const double *u, double *point_values, double *nodal_values, hipStream_t s) {
-    const auto &mesh = model.mesh;
-    const size_t elements = mesh.tets.size(), nodes = mesh.nodes.size(), vertices = mesh.vertex_count;
-    const int nq = model.quadrature == 14 ? 14 : 4;
-    if (elements == 0) return;
-    // the integration point nearest each node in natural coordinates, first on a tie (results.rs:555-571)
+// The integration point nearest each node in natural coordinates, the first on a tie
+// (results.rs:555-571).
+std::vector<int8_t> nearest_points(int nq) {
     const double r4[4][3] = {{A4, A4, A4}, {B4, A4, A4}, {A4, B4, A4}, {A4, A4, B4}};
     const double r14[14][3] = {{A14, A14, A14}, {B14, A14, A14}, {A14, B14, A14}, {A14, A14, B14}, {C14, C14, C14},
                                {D14, C14, C14}, {C14, D14, C14}, {C14, C14, D14}, {E14, E14, F14}, {E14, F14, E14},
@@ -549,101 +595,233 @@ void recover_stress(const Model &model, const double *u, double *point_values, d
         }
         nearest[n] = (int8_t)index;
     }
-    // a node's incident (element, node) values in element order; a vertex's incident elements
-    // (build_patches, results.rs:414-436); a midside's edge ends from the first element reaching it
-    std::vector<uint32_t> node_starts(nodes + 1, 0), vert_starts(vertices + 1, 0);
-    for (const auto &t : mesh.tets) {
-        for (int a = 0; a < 10; ++a) node_starts[t[a] + 1] += 1;
-        for (int a = 0; a < 4; ++a) vert_starts[t[a] + 1] += 1;
-    }
-    for (size_t n = 0; n < nodes; ++n) node_starts[n + 1] += node_starts[n];
-    for (size_t v = 0; v < vertices; ++v) vert_starts[v + 1] += vert_starts[v];
-    std::vector<uint32_t> node_at(node_starts[nodes]), vert_at(vert_starts[vertices]);
-    std::vector<uint32_t> nfill(node_starts.begin(), node_starts.end() - 1), vfill(vert_starts.begin(), vert_starts.end() - 1);
-    std::vector<uint32_t> ends(2 * (nodes - vertices), 0);
+    return nearest;
+}
+
+// A node's incident (element, node) slots in element order, then a vertex's incident elements
+// (build_patches, results.rs:414-436) and a midside's edge ends from the first element reaching
+// it (into_field, results.rs:386-401).
+struct Incidence {
+    std::vector<uint32_t> node_starts, node_at, vert_starts, vert_at, ends;
+};
+
+Incidence incidence_of(const QuadMesh &mesh, bool with_vertices) {
+    const size_t elements = mesh.tets.size(), nodes = mesh.nodes.size(), vertices = mesh.vertex_count;
+    Incidence in;
+    in.node_starts.assign(nodes + 1, 0);
+    for (const auto &t : mesh.tets)
+        for (int a = 0; a < 10; ++a) in.node_starts[t[a] + 1] += 1;
+    for (size_t n = 0; n < nodes; ++n) in.node_starts[n + 1] += in.node_starts[n];
+    in.node_at.resize(in.node_starts[nodes]);
+    std::vector<uint32_t> nfill(in.node_starts.begin(), in.node_starts.end() - 1);
+    for (size_t e = 0; e < elements; ++e)
+        for (int a = 0; a < 10; ++a) in.node_at[nfill[mesh.tets[e][a]]++] = (uint32_t)(e * 10 + a);
+    if (!with_vertices) return in;
+    in.vert_starts.assign(vertices + 1, 0);
+    for (const auto &t : mesh.tets)
+        for (int a = 0; a < 4; ++a) in.vert_starts[t[a] + 1] += 1;
+    for (size_t v = 0; v < vertices; ++v) in.vert_starts[v + 1] += in.vert_starts[v];
+    in.vert_at.resize(in.vert_starts[vertices]);
+    std::vector<uint32_t> vfill(in.vert_starts.begin(), in.vert_starts.end() - 1);
+    in.ends.assign(2 * (nodes - vertices), 0);
     std::vector<char> seen(nodes - vertices, 0);
     static const int EA[6] = {0, 1, 2, 0, 1, 2}, EB[6] = {1, 2, 0, 3, 3, 3};
     for (size_t e = 0; e < elements; ++e) {
         const auto &t = mesh.tets[e];
-        for (int a = 0; a < 10; ++a) node_at[nfill[t[a]]++] = (uint32_t)(e * 10 + a);
-        for (int a = 0; a < 4; ++a) vert_at[vfill[t[a]]++] = (uint32_t)e;
+        for (int a = 0; a < 4; ++a) in.vert_at[vfill[t[a]]++] = (uint32_t)e;
         for (int slot = 0; slot < 6; ++slot) {
             const size_t m = t[4 + slot] - vertices;
             if (seen[m]) continue;
             seen[m] = 1;
-            ends[2 * m] = t[EA[slot]];
-            ends[2 * m + 1] = t[EB[slot]];
+            in.ends[2 * m] = t[EA[slot]];
+            in.ends[2 * m + 1] = t[EB[slot]];
         }
     }
-    std::vector<double> xyz(nodes * 3);
-    for (size_t n = 0; n < nodes; ++n)
+    return in;
+}
+
+std::vector<double> positions_of(const QuadMesh &mesh) {
+    std::vector<double> xyz(mesh.nodes.size() * 3);
+    for (size_t n = 0; n < mesh.nodes.size(); ++n)
         for (int a = 0; a < 3; ++a) xyz[3 * n + a] = mesh.nodes[n][a];
-    std::vector<uint32_t> tets(elements * 10);
-    for (size_t e = 0; e < elements; ++e)
+    return xyz;
+}
+
+std::vector<uint32_t> tets_of(const QuadMesh &mesh) {
+    std::vector<uint32_t> tets(mesh.tets.size() * 10);
+    for (size_t e = 0; e < mesh.tets.size(); ++e)
         for (int a = 0; a < 10; ++a) tets[10 * e + a] = mesh.tets[e][a];
-    DeviceBuffer<double> d_xyz, d_u, d_points(elements * nq * 10), d_material(elements * nq * 2), d_nodevals(elements * 10 * 7),
-        d_worst(nodes * 6), d_fit(std::max<size_t>(vertices, 1) * 29), d_nodal(nodes * 7);
-    DeviceBuffer<uint32_t> d_tets, d_nstart, d_nat, d_vstart, d_vat, d_ends;
-    DeviceBuffer<int8_t> d_nearest;
-    DeviceBuffer<int32_t> d_fault;
-    d_xyz.upload(xyz, s);
-    d_u.upload(u, nodes * 3, s);
-    d_tets.upload(tets, s);
-    d_nstart.upload(node_starts, s);
-    d_nat.upload(node_at, s);
-    d_vstart.upload(vert_starts, s);
-    d_vat.upload(vert_at, s);
-    d_ends.upload(ends.empty() ? std::vector<uint32_t>{0, 0} : ends, s);
-    d_nearest.upload(nearest, s);
-    const int32_t none = 0x7fffffff;
-    d_fault.upload(&none, 1, s);
-    const FieldArgs fa = field_args(model.field);
-    auto grid = [](size_t n) { return dim3((unsigned)((n + 255) / 256)); };
-    if (nq == 14) {
-        hipLaunchKernelGGL(recovery_points_kernel<14>, grid(elements * 14), dim3(256), 0, s, elements, d_xyz.ptr, d_tets.ptr, d_u.ptr, fa,
-                           d_points.ptr, d_material.ptr, d_fault.ptr);
-        hipLaunchKernelGGL(recovery_nodes_kernel<14>, grid(elements * 10), dim3(256), 0, s, elements, d_xyz.ptr, d_tets.ptr, d_u.ptr,
-                           (const double *)d_material.ptr, (const int8_t *)d_nearest.ptr, d_nodevals.ptr, d_fault.ptr);
+    return tets;
+}
+
+}  // namespace
+
+DeviceRecovery::DeviceRecovery(const Model &model, const double *u, bool converged, hipStream_t s)
+    : s_(s),
+      elements_(model.mesh.tets.size()),
+      nodes_(model.mesh.nodes.size()),
+      vertices_(model.mesh.vertex_count),
+      nq_(model.quadrature == 14 ? 14 : 4),
+      field_(model.field),
+      converged_(converged),
+      u_(u, u + 3 * model.mesh.nodes.size()) {
+    const Incidence in = incidence_of(model.mesh, true);
+    xyz_.upload(positions_of(model.mesh), s);
+    u_d_.upload(u_, s);
+    tets_.upload(tets_of(model.mesh), s);
+    nstart_.upload(in.node_starts, s);
+    nat_.upload(in.node_at, s);
+    vstart_.upload(in.vert_starts, s);
+    vat_.upload(in.vert_at, s);
+    ends_.upload(in.ends.empty() ? std::vector<uint32_t>{0, 0} : in.ends, s);
+    nearest_.upload(nearest_points(nq_), s);
+    points_.resize(elements_ * nq_ * 10);
+    material_.resize(elements_ * nq_ * 2);
+    nodevals_.resize(elements_ * 10 * 7);
+    worst_.resize(nodes_ * 6);
+    fit_.resize(std::max<size_t>(vertices_, 1) * 29);
+    nodal_.resize(nodes_ * 7);
+    fault_.resize(1);
+}
+
+void DeviceRecovery::run_elements(size_t count) {
+    const int32_t none = NO_FAULT;
+    fault_.upload(&none, 1, s_);
+    const FieldArgs fa = field_args(field_);
+    if (nq_ == 14) {
+        hipLaunchKernelGGL(recovery_points_kernel<14>, grid_of(count * 14), dim3(256), 0, s_, next_, count, xyz_.ptr, tets_.ptr,
+                           u_d_.ptr, fa, points_.ptr, material_.ptr, fault_.ptr);
+        hipLaunchKernelGGL(recovery_nodes_kernel<14>, grid_of(count * 10), dim3(256), 0, s_, next_, count, xyz_.ptr, tets_.ptr,
+                           u_d_.ptr, (const double *)material_.ptr, (const int8_t *)nearest_.ptr, nodevals_.ptr, fault_.ptr);
     } else {
-        hipLaunchKernelGGL(recovery_points_kernel<4>, grid(elements * 4), dim3(256), 0, s, elements, d_xyz.ptr, d_tets.ptr, d_u.ptr, fa,
-                           d_points.ptr, d_material.ptr, d_fault.ptr);
-        hipLaunchKernelGGL(recovery_nodes_kernel<4>, grid(elements * 10), dim3(256), 0, s, elements, d_xyz.ptr, d_tets.ptr, d_u.ptr,
-                           (const double *)d_material.ptr, (const int8_t *)d_nearest.ptr, d_nodevals.ptr, d_fault.ptr);
+        hipLaunchKernelGGL(recovery_points_kernel<4>, grid_of(count * 4), dim3(256), 0, s_, next_, count, xyz_.ptr, tets_.ptr,
+                           u_d_.ptr, fa, points_.ptr, material_.ptr, fault_.ptr);
+        hipLaunchKernelGGL(recovery_nodes_kernel<4>, grid_of(count * 10), dim3(256), 0, s_, next_, count, xyz_.ptr, tets_.ptr,
+                           u_d_.ptr, (const double *)material_.ptr, (const int8_t *)nearest_.ptr, nodevals_.ptr, fault_.ptr);
     }
     SF_HIP(hipGetLastError());
-    int32_t fault = 0;
+    int32_t fault = NO_FAULT;
+    fault_.download(&fault, 1, s_);
+    SF_HIP(hipStreamSynchronize(s_));
+    if (fault != NO_FAULT) {
+        // the elements before the first faulting one are done; it stays the next to do
+        next_ = (size_t)fault;
+        throw Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element has an inverted point or an invalid material", fault);
+    }
+    next_ += count;
+}
+
+void DeviceRecovery::ensure_worst() {
+    if (worst_done_) return;
+    hipLaunchKernelGGL(recovery_worst_kernel, grid_of(nodes_), dim3(256), 0, s_, nodes_, (const uint32_t *)nstart_.ptr,
+                       (const uint32_t *)nat_.ptr, (const double *)nodevals_.ptr, worst_.ptr);
+    SF_HIP(hipGetLastError());
+    worst_done_ = true;
+}
+
+void DeviceRecovery::run_fits(size_t count) {
+    if (nq_ == 14)
+        hipLaunchKernelGGL(recovery_fit_kernel<14>, grid_of(count), dim3(256), 0, s_, next_vertex_, count, (const double *)xyz_.ptr,
+                           (const uint32_t *)vstart_.ptr, (const uint32_t *)vat_.ptr, (const double *)points_.ptr,
+                           (const double *)material_.ptr, fit_.ptr);
+    else
+        hipLaunchKernelGGL(recovery_fit_kernel<4>, grid_of(count), dim3(256), 0, s_, next_vertex_, count, (const double *)xyz_.ptr,
+                           (const uint32_t *)vstart_.ptr, (const uint32_t *)vat_.ptr, (const double *)points_.ptr,
+                           (const double *)material_.ptr, fit_.ptr);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipStreamSynchronize(s_));
+    next_vertex_ += count;
+}
+
+bool DeviceRecovery::advance(size_t max_work, const std::function<bool()> &cancel) {
+    // Recovery::advance (results.rs:477-536): the budget is spent on elements first, and patch
+    // fits start only once every element is done; the cancel check is read once per unit.
+    size_t budget = max_work, take = 0;
+    bool cancelled = false;
+    while (next_ + take < elements_ && budget > 0) {
+        budget -= 1;
+        if (cancel && cancel()) {
+            cancelled = true;
+            break;
+        }
+        take += 1;
+    }
+    if (take) run_elements(take);
+    if (cancelled) throw Error(SF_ERR_MODEL_CANCELLED, "recovery cancelled");
+    if (next_ == elements_ && next_vertex_ < vertices_ && budget > 0) {
+        // the worst incident value at a node reads every element's node values, all done now
+        ensure_worst();
+        take = 0;
+        while (next_vertex_ + take < vertices_ && budget > 0) {
+            budget -= 1;
+            if (cancel && cancel()) {
+                cancelled = true;
+                break;
+            }
+            take += 1;
+        }
+        if (take) run_fits(take);
+        if (cancelled) throw Error(SF_ERR_MODEL_CANCELLED, "recovery cancelled");
+    }
+    return finished();
+}
+
+void DeviceRecovery::field(double *point_values, double *nodal_values) {
+    if (!finished()) throw Error(SF_ERR_MODEL_NOT_FINISHED, "the recovery has elements or patch fits left to do",
+                                 (int64_t)next_, (int64_t)next_vertex_);
+    if (elements_ == 0) return;
+    ensure_worst();
+    if (!nodal_done_) {
+        hipLaunchKernelGGL(recovery_nodal_kernel, grid_of(nodes_), dim3(256), 0, s_, nodes_, vertices_, (const double *)xyz_.ptr,
+                           (const uint32_t *)ends_.ptr, (const double *)fit_.ptr, (const double *)worst_.ptr, nodal_.ptr);
+        SF_HIP(hipGetLastError());
+        nodal_done_ = true;
+    }
+    if (point_values) points_.download(point_values, elements_ * nq_ * 10, s_);
+    if (nodal_values) nodal_.download(nodal_values, nodes_ * 7, s_);
+    SF_HIP(hipStreamSynchronize(s_));
+}
+
+void recover_stress(const Model &model, const double *u, double *point_values, double *nodal_values, hipStream_t s) {
+    if (model.mesh.tets.empty()) return;
+    DeviceRecovery r(model, u, true, s);
+    r.advance(std::numeric_limits<size_t>::max(), nullptr);
+    r.field(point_values, nodal_values);
+}
+
+void nodal_rotations(const Model &model, const double *u, double *rotations, hipStream_t s) {
+    const auto &mesh = model.mesh;
+    const size_t elements = mesh.tets.size(), nodes = mesh.nodes.size();
+    if (elements == 0) return;
+    const Incidence in = incidence_of(mesh, false);
+    DeviceBuffer<double> d_xyz, d_u, d_values(elements * 10 * 4), d_out(nodes * 3);
+    DeviceBuffer<uint32_t> d_tets, d_starts, d_at;
+    DeviceBuffer<int32_t> d_fault;
+    d_xyz.upload(positions_of(mesh), s);
+    d_u.upload(u, nodes * 3, s);
+    d_tets.upload(tets_of(mesh), s);
+    d_starts.upload(in.node_starts, s);
+    d_at.upload(in.node_at, s);
+    const int32_t none = NO_FAULT;
+    d_fault.upload(&none, 1, s);
+    hipLaunchKernelGGL(rotation_nodes_kernel, grid_of(elements * 10), dim3(256), 0, s, elements, (const double *)d_xyz.ptr,
+                       (const uint32_t *)d_tets.ptr, (const double *)d_u.ptr, d_values.ptr, d_fault.ptr);
+    SF_HIP(hipGetLastError());
+    int32_t fault = NO_FAULT;
     d_fault.download(&fault, 1, s);
     SF_HIP(hipStreamSynchronize(s));
-    if (fault != 0x7fffffff)
-        throw Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element has an inverted point or an invalid material", fault);
-    hipLaunchKernelGGL(recovery_worst_kernel, grid(nodes), dim3(256), 0, s, nodes, (const uint32_t *)d_nstart.ptr,
-                       (const uint32_t *)d_nat.ptr, (const double *)d_nodevals.ptr, d_worst.ptr);
-    if (nq == 14)
-        hipLaunchKernelGGL(recovery_fit_kernel<14>, grid(vertices), dim3(256), 0, s, vertices, (const double *)d_xyz.ptr,
-                           (const uint32_t *)d_vstart.ptr, (const uint32_t *)d_vat.ptr, (const double *)d_points.ptr,
-                           (const double *)d_material.ptr, d_fit.ptr);
-    else
-        hipLaunchKernelGGL(recovery_fit_kernel<4>, grid(vertices), dim3(256), 0, s, vertices, (const double *)d_xyz.ptr,
-                           (const uint32_t *)d_vstart.ptr, (const uint32_t *)d_vat.ptr, (const double *)d_points.ptr,
-                           (const double *)d_material.ptr, d_fit.ptr);
-    hipLaunchKernelGGL(recovery_nodal_kernel, grid(nodes), dim3(256), 0, s, nodes, vertices, (const double *)d_xyz.ptr,
-                       (const uint32_t *)d_ends.ptr, (const double *)d_fit.ptr, (const double *)d_worst.ptr, d_nodal.ptr);
+    if (fault != NO_FAULT) throw Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element is inverted at a node", fault);
+    hipLaunchKernelGGL(rotation_gather_kernel, grid_of(nodes), dim3(256), 0, s, nodes, (const uint32_t *)d_starts.ptr,
+                       (const uint32_t *)d_at.ptr, (const double *)d_values.ptr, d_out.ptr);
     SF_HIP(hipGetLastError());
-    if (point_values) d_points.download(point_values, elements * nq * 10, s);
-    if (nodal_values) d_nodal.download(nodal_values, nodes * 7, s);
+    d_out.download(rotations, nodes * 3, s);
     SF_HIP(hipStreamSynchronize(s));
 }
 
 void DeviceAssembly::prepare(const Model &model, const AssemblyPattern &pattern, hipStream_t s) {
     const auto &mesh = model.mesh;
-    std::vector<double> xyz(mesh.nodes.size() * 3);
-    for (size_t n = 0; n < mesh.nodes.size(); ++n)
-        for (int a = 0; a < 3; ++a) xyz[3 * n + a] = mesh.nodes[n][a];
-    nodes.upload(xyz, s);
-    std::vector<uint32_t> t(mesh.tets.size() * 10);
-    for (size_t e = 0; e < mesh.tets.size(); ++e)
-        for (int a = 0; a < 10; ++a) t[10 * e + a] = mesh.tets[e][a];
-    tets.upload(t, s);
+    nodes.upload(positions_of(mesh), s);
+    tets.upload(tets_of(mesh), s);
     contribution_starts.upload(pattern.contribution_starts.data(), pattern.contribution_starts.size(), s);
     contributions.upload(pattern.contributions.data(), pattern.contributions.size(), s);
     elements = mesh.tets.size();

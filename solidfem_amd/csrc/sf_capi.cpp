@@ -826,11 +826,96 @@ sf_status sf_recover_stress(sf_context *ctx, const sf_model *model, const double
 }
 sf_status sf_static_study_recover(sf_static_study *study, double *point_values, double *nodal_values) {
     return guard([&] {
+        const double t0 = seconds();
         std::vector<double> u(3 * study->model.mesh.nodes.size());
         const sf_status st = sf_static_study_displacements(study, u.data());
         if (st != SF_OK) throw Error(st, "the study's displacements are not available");
         recover_stress(study->model, u.data(), point_values, nodal_values, study->ctx->stream);
+        study->report.recovery_seconds += seconds() - t0;
     });
+}
+
+sf_status sf_recovery_create(sf_context *ctx, const sf_model *model, const double *displacements, int32_t converged,
+                             sf_recovery **out) {
+    return guard([&] {
+        if (!ctx || !model || !out || (!displacements && !model->model.mesh.nodes.empty()))
+            throw Error(SF_ERR_INVALID_ARGUMENT, "a recovery needs a context, a model and displacements");
+        auto r = std::make_unique<sf_recovery>();
+        r->ctx = ctx->ctx;
+        r->recovery = std::make_unique<DeviceRecovery>(model->model, displacements, converged != 0, ctx->ctx->stream);
+        *out = r.release();
+    });
+}
+sf_status sf_static_study_begin_recovery(sf_static_study *study, sf_recovery **out) {
+    return guard([&] {
+        if (!out) throw Error(SF_ERR_INVALID_ARGUMENT, "no place for the recovery");
+        std::vector<double> u(3 * study->model.mesh.nodes.size());
+        const sf_status st = sf_static_study_displacements(study, u.data());
+        if (st != SF_OK) throw Error(st, "the study's displacements are not available");
+        auto r = std::make_unique<sf_recovery>();
+        r->ctx = study->ctx;
+        r->recovery = std::make_unique<DeviceRecovery>(study->model, u.data(), study->pcg->pcg->converged(), study->ctx->stream);
+        *out = r.release();
+    });
+}
+sf_status sf_recovery_advance(sf_recovery *recovery, uint64_t max_work, sf_cancel_fn cancel, void *user, int32_t *finished) {
+    return guard([&] {
+        const Cancel c{cancel, user};
+        const bool done = recovery->recovery->advance((size_t)max_work, [&] { return c(); });
+        if (finished) *finished = done ? 1 : 0;
+    });
+}
+sf_status sf_static_study_continue_recovery(sf_static_study *study, sf_recovery *recovery, uint64_t max_work,
+                                            sf_cancel_fn cancel, void *user, int32_t *finished) {
+    const double t0 = seconds();
+    const sf_status st = sf_recovery_advance(recovery, max_work, cancel, user, finished);
+    study->report.recovery_seconds += seconds() - t0;
+    return st;
+}
+sf_status sf_recovery_state_of(const sf_recovery *recovery, sf_recovery_state *out) {
+    return guard([&] {
+        const DeviceRecovery &r = *recovery->recovery;
+        out->elements = r.elements();
+        out->elements_done = r.elements_done();
+        out->vertices = r.vertices();
+        out->fits_done = r.fits_done();
+        out->nodes = r.nodes();
+        out->points_per_element = (uint64_t)r.points_per_element();
+        out->converged = r.converged() ? 1 : 0;
+        out->finished = r.finished() ? 1 : 0;
+    });
+}
+sf_status sf_recovery_displacements(const sf_recovery *recovery, double *out) {
+    return guard([&] {
+        const auto &u = recovery->recovery->displacements();
+        std::copy(u.begin(), u.end(), out);
+    });
+}
+sf_status sf_recovery_field(sf_recovery *recovery, double *point_values, double *nodal_values) {
+    return guard([&] { recovery->recovery->field(point_values, nodal_values); });
+}
+void sf_recovery_destroy(sf_recovery *recovery) { delete recovery; }
+
+sf_status sf_peak_stress(const double *node_positions, uint64_t nodes, const double *nodal_von_mises, uint64_t nodal_stride,
+                         const double *point_values, uint64_t points, sf_region_fn inside, void *user, sf_peak *out) {
+    return guard([&] {
+        if (!out) throw Error(SF_ERR_INVALID_ARGUMENT, "no place for the peak");
+        if ((nodes && (!node_positions || !nodal_von_mises || nodal_stride == 0)) || (points && !point_values))
+            throw Error(SF_ERR_INVALID_ARGUMENT, "a candidate array is missing");
+        std::function<bool(const double *)> region;
+        if (inside) region = [&](const double *p) { return inside(p, user) != 0; };
+        const PeakStress peak =
+            peak_stress(node_positions, (size_t)nodes, nodal_von_mises, (size_t)nodal_stride, point_values, (size_t)points, region);
+        *out = sf_peak{};
+        out->found = peak.found ? 1 : 0;
+        out->at_point = peak.at_point ? 1 : 0;
+        out->index = peak.index;
+        out->von_mises = peak.von_mises;
+        for (int a = 0; a < 3; ++a) out->position[a] = peak.position[a];
+    });
+}
+sf_status sf_nodal_rotations(sf_context *ctx, const sf_model *model, const double *displacements, double *rotations) {
+    return guard([&] { nodal_rotations(model->model, displacements, rotations, ctx->ctx->stream); });
 }
 uint64_t sf_model_points_per_element(const sf_model *model) { return model->model.quadrature == 14 ? 14 : 4; }
 sf_status sf_static_study_counts(const sf_static_study *study, uint64_t *nodes, uint64_t *elements, uint64_t *points) {

@@ -2,6 +2,8 @@
 // V-cycle, the conjugate gradient solve and the stiffness assembly, all resident in HBM.
 #pragma once
 
+#include <functional>
+
 #include "sf_host.hpp"
 #include "sf_kernels.hpp"
 #include "sf_setup.hpp"
@@ -324,11 +326,59 @@ class DevicePcgBatch {
 
 // ---- stress recovery --------------------------------------------------------------------------
 
-// Recovery after a solve (crates/solidfem-elastic/src/results.rs:307-576, element.rs:135-192,
-// patch.rs) on the device for nodal displacements `u` (3 per mesh node): per (element, point)
-// 10 values -- position 3, stress [xx, yy, zz, xy, yz, zx], von Mises -- and per mesh node 7 --
-// the recovered stress and its von Mises. Either output may be null.
+// A resumable stress recovery (Recovery, crates/solidfem-elastic/src/results.rs:103-124 and
+// :307-576, with element.rs:135-192 and patch.rs) of one snapshot of nodal displacements (3 per
+// mesh node), on the device: every element in order, then one patch fit per mesh vertex in order.
+// Slices of any size give the bits of one run: an element's values depend on that element alone,
+// a fit on its vertex's patch once every element is done, and a node's worst incident value is
+// taken over its elements in element order, once, before the first fit. The field: per
+// (element, point) 10 values -- position 3, stress [xx, yy, zz, xy, yz, zx], von Mises -- and per
+// mesh node 7 -- the recovered stress and its von Mises.
+class DeviceRecovery {
+   public:
+    DeviceRecovery(const Model &model, const double *u, bool converged, hipStream_t s);
+    // Up to `max_work` more units (an element, then a vertex's patch fit); `cancel` (may be empty)
+    // is read before each. Returns whether every unit is done. A cancel throws
+    // SF_ERR_MODEL_CANCELLED with the units before it done; a faulting element throws with its
+    // index, the elements before it done and it the next to do.
+    bool advance(size_t max_work, const std::function<bool()> &cancel);
+    bool finished() const { return next_ == elements_ && next_vertex_ == vertices_; }
+    size_t elements_done() const { return next_; }
+    size_t fits_done() const { return next_vertex_; }
+    size_t elements() const { return elements_; }
+    size_t vertices() const { return vertices_; }
+    size_t nodes() const { return nodes_; }
+    int points_per_element() const { return nq_; }
+    bool converged() const { return converged_; }
+    const std::vector<double> &displacements() const { return u_; }
+    // The finished field (either output may be null); SF_ERR_MODEL_NOT_FINISHED before then.
+    void field(double *point_values, double *nodal_values);
+
+   private:
+    void run_elements(size_t count);
+    void run_fits(size_t count);
+    void ensure_worst();
+
+    hipStream_t s_;
+    size_t elements_, nodes_, vertices_;
+    int nq_;
+    MaterialField field_;
+    bool converged_;
+    std::vector<double> u_;
+    size_t next_ = 0, next_vertex_ = 0;
+    bool worst_done_ = false, nodal_done_ = false;
+    DeviceBuffer<double> xyz_, u_d_, points_, material_, nodevals_, worst_, fit_, nodal_;
+    DeviceBuffer<uint32_t> tets_, nstart_, nat_, vstart_, vat_, ends_;
+    DeviceBuffer<int8_t> nearest_;
+    DeviceBuffer<int32_t> fault_;
+};
+
+// The whole recovery in one call; either output may be null.
 void recover_stress(const Model &model, const double *u, double *point_values, double *nodal_values, hipStream_t s);
+
+// nodal_rotations (results.rs:262-305): half the curl of the displacement at every mesh node, the
+// element values at the node weighted by their Jacobian determinants and summed in element order.
+void nodal_rotations(const Model &model, const double *u, double *rotations, hipStream_t s);
 
 // ---- assembly -------------------------------------------------------------------------------
 

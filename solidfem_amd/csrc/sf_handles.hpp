@@ -52,6 +52,11 @@ struct sf_static_study {
     sf_report report{};
 };
 
+struct sf_recovery {
+    std::shared_ptr<Context> ctx;
+    std::unique_ptr<DeviceRecovery> recovery;
+};
+
 struct sf_study_builder {
     std::shared_ptr<Context> ctx;
     Model model;

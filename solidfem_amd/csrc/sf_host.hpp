@@ -3,6 +3,7 @@
 // (assembly, multigrid cycle, conjugate gradient) and the host multigrid setup.
 #pragma once
 
+#include <functional>
 #include <map>
 #include <set>
 
@@ -160,5 +161,20 @@ struct PrescribedRhs {
 };
 PrescribedRhs prescribed_rhs_program(const HostBsr<double> &a, const std::vector<std::pair<size_t, double>> &prescribed);
 void apply_prescribed_rhs(const PrescribedRhs &program, std::vector<double> &rhs);
+
+// peak_with_holder (crates/solidfem-elastic/src/results.rs:184-226): the largest von Mises over
+// the nodes, then the integration points, whose positions `inside` accepts. A later candidate
+// replaces the peak only when strictly larger, so the first of equal values holds it; the first
+// NaN met becomes the peak and stays it. Nodal values are read at nodal_vm[stride * node]; a point
+// is 10 values (position 3, stress 6, von Mises).
+struct PeakStress {
+    bool found = false;
+    bool at_point = false;  // held by an integration point, else by a mesh node
+    size_t index = 0;       // of that node or point
+    double von_mises = 0.0;
+    double position[3] = {0.0, 0.0, 0.0};
+};
+PeakStress peak_stress(const double *positions, size_t nodes, const double *nodal_vm, size_t stride, const double *point_values,
+                       size_t points, const std::function<bool(const double *)> &inside);
 
 }  // namespace sf

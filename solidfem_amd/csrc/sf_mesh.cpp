@@ -1043,4 +1043,21 @@ AssemblyPattern build_pattern(const QuadMesh &mesh, const std::vector<u32> &free
     return out;
 }
 
+PeakStress peak_stress(const double *positions, size_t nodes, const double *nodal_vm, size_t stride, const double *point_values,
+                       size_t points, const std::function<bool(const double *)> &inside) {
+    PeakStress peak;
+    auto offer = [&](bool at_point, size_t index, double value, const double *position) {
+        if (inside && !inside(position)) return;
+        if (peak.found && (std::isnan(peak.von_mises) || !(std::isnan(value) || value > peak.von_mises))) return;
+        peak.found = true;
+        peak.at_point = at_point;
+        peak.index = index;
+        peak.von_mises = value;
+        for (int a = 0; a < 3; ++a) peak.position[a] = position[a];
+    };
+    for (size_t n = 0; n < nodes; ++n) offer(false, n, nodal_vm[n * stride], positions + 3 * n);
+    for (size_t p = 0; p < points; ++p) offer(true, p, point_values[p * 10 + 9], point_values + p * 10);
+    return peak;
+}
+
 }  // namespace sf

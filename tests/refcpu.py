@@ -58,6 +58,8 @@ _SIG = {
     "ref_problem_element_stiffness": (i32, [vp, i64, P(f64)]),
     "ref_recover": (i32, [vp, P(f64), P(f64), P(f64), P(i64)]),
     "ref_problem_quadrature": (i32, [vp]),
+    "ref_peak": (i32, [P(f64), u64, P(f64), u64, P(f64), u64, P(C.c_uint8), P(C.c_uint8), P(i32), P(u64), P(f64), P(f64)]),
+    "ref_nodal_rotations": (i32, [vp, P(f64), P(f64), P(i64)]),
     "ref_multigrid_create": (vp, [vp, P(RefOptions), P(i32)]),
     "ref_multigrid_destroy": (None, [vp]),
     "ref_multigrid_levels": (i64, [vp]),
@@ -207,6 +209,15 @@ class Problem:
             raise RuntimeError(f"element {fault.value} is inverted or has an invalid material")
         return points, nodal
 
+    def nodal_rotations(self, u):
+        """nodal_rotations (results.rs:262-305): (nodes, 3)."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        out = np.empty((self.node_count, 3))
+        fault = C.c_int64(-1)
+        if lib().ref_nodal_rotations(self._p, ptr(u, f64), ptr(out, f64), C.byref(fault)) != 0:
+            raise RuntimeError(f"element {fault.value} is inverted at a node")
+        return out
+
     def element_stiffness(self, tet):
         ke = np.empty((30, 30))
         if lib().ref_problem_element_stiffness(self._p, tet, ptr(ke, f64)) != 0:
@@ -344,3 +355,22 @@ def pcg_jacobi(rs, cols, vals, rhs, tolerance=1e-9, slice_iterations=0):
     lib().ref_pcg_jacobi(n, ptr(rs, i64), ptr(cols, u32), ptr(vals, f64), ptr(rhs, f64), tolerance, slice_iterations,
                          ptr(x, f64), C.byref(r))
     return x, r
+
+
+def peak(positions, nodal_von_mises, point_values, node_inside=None, point_inside=None):
+    """peak_with_holder (results.rs:184-226) over the candidates the boolean masks keep (None: all):
+    None, or (von_mises, position (3,), at_point, index)."""
+    x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    vm = np.ascontiguousarray(nodal_von_mises, dtype=np.float64).reshape(-1)
+    pts = np.ascontiguousarray(point_values, dtype=np.float64).reshape(-1, 10)
+    masks = []
+    for m, n in ((node_inside, x.shape[0]), (point_inside, pts.shape[0])):
+        masks.append(None if m is None else np.ascontiguousarray(np.broadcast_to(m, (n,)), dtype=np.uint8))
+    at_point, index, value, position = C.c_int32(), C.c_uint64(), C.c_double(), np.empty(3)
+    found = lib().ref_peak(ptr(x, f64), x.shape[0], ptr(vm, f64), 1, ptr(pts, f64), pts.shape[0],
+                           None if masks[0] is None else ptr(masks[0], C.c_uint8),
+                           None if masks[1] is None else ptr(masks[1], C.c_uint8),
+                           C.byref(at_point), C.byref(index), C.byref(value), ptr(position, f64))
+    if not found:
+        return None
+    return value.value, position, bool(at_point.value), index.value

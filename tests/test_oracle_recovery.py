@@ -4,7 +4,10 @@ Closed forms: an affine displacement field has constant strain, so every integra
 every recovered node must read the constitutive stress of that strain; a quadratic field has a
 linear stress, which Tet10 represents exactly and the linear patch fit must reproduce at every
 node. An independent numpy evaluation of the Tet10 gradients checks the point stresses and the
-worst-value fallback where a patch holds two materials.
+worst-value fallback where a patch holds two materials. The nodal rotations (results.rs:262-305)
+are pinned by a rigid rotation and a linear field (their skew part, everywhere), a translation
+(zero), and an independent numpy evaluation of the determinant-weighted mean on a cubic field, where
+the elements disagree at shared nodes and an unweighted mean differs.
 """
 import dataclasses
 
@@ -131,3 +134,52 @@ def test_points_and_the_two_material_fallback_match_an_independent_evaluation():
             mixed += 1
             assert np.abs(nodal[v, :6] - worst[v][1]).max() <= 1e-9 * max(1.0, np.abs(worst[v][1]).max()), v
     assert mixed > 0
+
+
+# ---- nodal rotations (results.rs:262-305) ----
+
+
+def rotation_of(grad_u):
+    """Half the curl of u from grad_u[i][j] = du_i/dx_j."""
+    return 0.5 * np.array([grad_u[2, 1] - grad_u[1, 2], grad_u[0, 2] - grad_u[2, 0], grad_u[1, 0] - grad_u[0, 1]])
+
+
+def test_rigid_and_linear_fields_rotate_by_their_skew_part():
+    w = plate_config(divisions=(4, 4, 2))
+    p = refcpu.Problem(w)
+    x = p.nodes()
+    omega = np.array([2e-3, -5e-4, 1e-3])
+    rigid = np.cross(omega, x) + np.array([0.3, -0.1, 0.2])
+    r = p.nodal_rotations(rigid)
+    assert np.abs(r - omega).max() <= 1e-13 * np.abs(omega).max() * 100
+    a = np.array([[1e-3, 2e-4, -3e-4], [5e-4, -2e-3, 1e-4], [-4e-4, 3e-4, 7e-4]])
+    r = p.nodal_rotations(x @ a.T)
+    assert np.abs(r - rotation_of(a)).max() <= 1e-11 * np.abs(a).max()
+    # a translation: the gradients sum to zero up to rounding, of order eps |u| / h
+    assert np.abs(p.nodal_rotations(np.tile([1.0, 2.0, 3.0], (len(x), 1)))).max() <= 1e-13
+
+
+def test_rotations_are_the_volume_weighted_mean_of_the_elements_values():
+    # a cubic field is outside the Tet10 space: each element's gradient at a shared node differs,
+    # so the determinant weighting (six times the volume of a straight-edged element) shows
+    w = plate_config(divisions=(3, 3, 2))
+    p = refcpu.Problem(w)
+    x, tets = p.nodes(), p.tets()
+    u = np.stack([x[:, 1] ** 3 * 1e-4, x[:, 0] ** 3 * 2e-5 + x[:, 2] ** 2 * 1e-3, x[:, 0] * x[:, 1] * x[:, 2] * 1e-4], axis=1)
+    weighted, weights = np.zeros((len(x), 3)), np.zeros(len(x))
+    for t in tets:
+        xe, ue = x[t], u[t]
+        det = np.linalg.det(np.array([xe[1] - xe[0], xe[2] - xe[0], xe[3] - xe[0]]))
+        for n in range(10):
+            weighted[t[n]] += det * rotation_of(ue.T @ tet10_gradients(xe, NATURAL[n]))
+            weights[t[n]] += det
+    expected = weighted / weights[:, None]
+    r = p.nodal_rotations(u)
+    assert np.abs(r - expected).max() <= 1e-12 * np.abs(expected).max()
+    # the unweighted mean is a different answer on this jittered mesh
+    plain, counts = np.zeros((len(x), 3)), np.zeros(len(x))
+    for t in tets:
+        for n in range(10):
+            plain[t[n]] += rotation_of(u[t].T @ tet10_gradients(x[t], NATURAL[n]))
+            counts[t[n]] += 1
+    assert np.abs(plain / counts[:, None] - expected).max() > 1e-6 * np.abs(expected).max()
