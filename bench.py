@@ -130,7 +130,7 @@ def oracle_leg(np, workload, gpu_iterations, sample_iterations, x_gpu):
     return baseline, check
 
 
-def multi_rhs(ctx, sf, np, columns, steps, warmup):
+def multi_rhs(ctx, sf, np, columns, steps, warmup, sequential=True):
     """Many load cases over one hierarchy: the 60 x 40 x 40 mm block (30 x 20 x 20 hex x 6 Tet10,
     E 70 GPa, nu 0.33, clamped z = 0) under `columns` random-traction load cases (N(0, 1 MPa) per
     component on the z = 40 faces, seed 7, case c), solved as one lockstep batch with the FP32
@@ -183,6 +183,20 @@ def multi_rhs(ctx, sf, np, columns, steps, warmup):
         return {"kernel": what, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": per,
                 "avg_launch_us": round(sec / max(1, n) * 1e6, 2), "launches": n}
+    out = {
+        "workload": f"block 60x40x40 mm, 30x20x20 hex x 6 Tet10, {rhs.shape[1]} free DOFs, {columns} random-traction "
+                    f"load cases (seed 7), FP32 V-cycle in FP64 PCG, rtol 1e-8",
+        "columns": columns,
+        "ms_per_batch": round(batch_ms, 3),
+        "ms_per_load_case": round(batch_ms / columns, 3),
+        "iterations": {"min": int(min(p.iteration for p in progress)), "max": int(max(p.iteration for p in progress))},
+        "roofline_batched_fine_chebyshev": roofline(sf.Kernel.BATCH_FINE_CHEBYSHEV,
+                                                    f"batched fused Chebyshev step, finest level, f32, {columns} columns"),
+        "roofline_batched_system_product": roofline(sf.Kernel.BATCH_SYSTEM_PRODUCT,
+                                                    f"batched system product A P, f64, {columns} columns"),
+    }
+    if not sequential:
+        return out
     singles = [sf.Pcg.new(ctx, system, mg, rhs[m], pcg_options) for m in range(columns)]
     for p in singles[:2]:
         p.step(2 ** 62)
@@ -196,21 +210,10 @@ def multi_rhs(ctx, sf, np, columns, steps, warmup):
     seq_ms = (time.perf_counter() - t) * 1e3
     identical = all(np.array_equal(singles[m].estimate().view(np.uint64), xb[m].view(np.uint64)) for m in range(columns))
     log(f"multi-rhs: batch {batch_ms:.1f} ms for {columns} cases, one at a time {seq_ms:.1f} ms; bit-identical {identical}")
-    return {
-        "workload": f"block 60x40x40 mm, 30x20x20 hex x 6 Tet10, {rhs.shape[1]} free DOFs, {columns} random-traction "
-                    f"load cases (seed 7), FP32 V-cycle in FP64 PCG, rtol 1e-8",
-        "columns": columns,
-        "ms_per_batch": round(batch_ms, 3),
-        "ms_per_load_case": round(batch_ms / columns, 3),
-        "sequential_ms_per_load_case": round(seq_ms / columns, 3),
-        "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
-        "iterations": {"min": int(min(p.iteration for p in progress)), "max": int(max(p.iteration for p in progress))},
-        "columns_bit_identical_to_single_solves": bool(identical),
-        "roofline_batched_fine_chebyshev": roofline(sf.Kernel.BATCH_FINE_CHEBYSHEV,
-                                                    f"batched fused Chebyshev step, finest level, f32, {columns} columns"),
-        "roofline_batched_system_product": roofline(sf.Kernel.BATCH_SYSTEM_PRODUCT,
-                                                    f"batched system product A P, f64, {columns} columns"),
-    }
+    out["sequential_ms_per_load_case"] = round(seq_ms / columns, 3)
+    out["speedup_vs_sequential"] = round(seq_ms / batch_ms, 3)
+    out["columns_bit_identical_to_single_solves"] = bool(identical)
+    return out
 
 
 def serving(sf, np, count, rate, seed):
@@ -276,10 +279,11 @@ def serving(sf, np, count, rate, seed):
     }
 
 
-def profile_variants(sf, ctx, workload):
+def profile_variants(sf, np, ctx, workload):
     """Every batch depth of the finest level's kernels and of the conjugate gradient's product,
-    forced in turn, each through one FP32-V-cycle and one FP64 solve of the plate, then the plain
-    1 GiB copy that calibrates the byte counters. The bits never depend on the depth."""
+    forced in turn, each through one FP32-V-cycle and one FP64 solve of the plate, then two batched
+    solves of the 32-load-case block, then the plain 1 GiB copy that calibrates the byte counters.
+    The bits never depend on the depth."""
     pcg_options = sf.PcgOptions(tolerance=1e-8, dot_order=sf.DotOrder.TREE)
     for depth in (1, 2, 4, 8):
         sf.force_variant(depth)
@@ -293,6 +297,7 @@ def profile_variants(sf, ctx, workload):
             log(f"depth {depth} ({mg.precision().name}): {p.iteration} iterations")
         del study, builder
     sf.force_variant(0)
+    multi_rhs(ctx, sf, np, 32, 1, 0, sequential=False)
     ctx.stream_copy(1 << 30, 10)
     log("calibration copy done")
 
@@ -327,7 +332,7 @@ def main():
     workload = plate_config(divisions=divisions)
     ctx = sf.Context(0)
     if args.profile_variants:
-        profile_variants(sf, ctx, workload)
+        profile_variants(sf, np, ctx, workload)
         return
 
     log(f"building the plate {divisions}")

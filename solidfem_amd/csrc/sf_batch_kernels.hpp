@@ -32,7 +32,9 @@ struct GroupLane {
 // are the rows of x they gather, which its L2 can hold for all their columns. Which workgroup
 // takes which rows changes no bit.
 __device__ __forceinline__ uint32_t group_block(uint32_t order) {
-    return (order & ORDER_GROUP_RUNS) ? xcd_runs_block() : blockIdx.x;
+    if (order & ORDER_GROUP_RUNS) return xcd_runs_block();
+    if (order & ORDER_GROUP_CHUNKS) return xcd_chunks_block(order >> ORDER_CHUNK_SHIFT);
+    return blockIdx.x;
 }
 
 __device__ __forceinline__ GroupLane group_lane(int log2g, uint32_t order) {

@@ -41,12 +41,13 @@ uint32_t DeviceSliced<T>::xcd_order() {
     return order;
 }
 template <typename T>
-bool DeviceSliced<T>::group_runs() {
-    // off unless SOLIDFEM_GROUP_RUNS=1: measured slower on the 32-column block (81 against 59 ms
-    // per batch)
-    static const bool on = [] {
+int DeviceSliced<T>::group_runs() {
+    // SOLIDFEM_GROUP_RUNS: 0 (default) dispatch order, 1 one contiguous run per XCD, a power of two
+    // W > 1 chunks of W workgroups per XCD in turn
+    static const int on = [] {
         const char *e = std::getenv("SOLIDFEM_GROUP_RUNS");
-        return e && std::atoi(e) != 0;
+        const int v = e ? std::atoi(e) : 0;
+        return (v > 1 && (v & (v - 1)) == 0 && v <= 1024) || v == 1 ? v : 0;
     }();
     return on;
 }

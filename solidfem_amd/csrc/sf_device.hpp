@@ -47,10 +47,12 @@ struct DeviceSliced {
     // workgroups run in contiguous runs per XCD.
     dev::Sliced<T> batch_view() const {
         dev::Sliced<T> v = view();
-        if (group_runs()) v.order |= dev::ORDER_GROUP_RUNS;
+        const int g = group_runs();
+        if (g == 1) v.order |= dev::ORDER_GROUP_RUNS;
+        else if (g > 1) v.order |= dev::ORDER_GROUP_CHUNKS | ((uint32_t)__builtin_ctz((unsigned)g) << dev::ORDER_CHUNK_SHIFT);
         return v;
     }
-    static bool group_runs();
+    static int group_runs();
     static uint32_t xcd_order();  // SOLIDFEM_XCD_ORDER: 0 none, 1 contiguous runs, 2 whole windows
     static bool ping_pong();      // SOLIDFEM_PING_PONG: alternate passes walk the slices backwards
     size_t dofs() const { return rows * rb; }

@@ -52,7 +52,9 @@ struct Sliced {
 // each XCD whole sort windows (8 workgroups of 4 slices), so the rows a window permutes share one
 // L2; REVERSE walks the slices from the end (alternate passes over a matrix then start where the
 // last one ended, in the cache).
-enum : uint32_t { ORDER_XCD_RUNS = 1, ORDER_XCD_WINDOWS = 2, ORDER_REVERSE = 4, ORDER_GROUP_RUNS = 8 };
+enum : uint32_t { ORDER_XCD_RUNS = 1, ORDER_XCD_WINDOWS = 2, ORDER_REVERSE = 4, ORDER_GROUP_RUNS = 8, ORDER_GROUP_CHUNKS = 16 };
+// With ORDER_GROUP_CHUNKS, bits 24-31 of an order hold log2 of the chunk (workgroups per XCD).
+constexpr uint32_t ORDER_CHUNK_SHIFT = 24;
 
 template <typename V, typename S>
 __device__ __forceinline__ V widen(S s) {
@@ -73,6 +75,14 @@ __device__ __forceinline__ uint32_t xcd_windows_block() {
     if (b >= nb - nb % 64) return b;
     const uint32_t r = b % 64;
     return b - r + (r % 8) * 8 + r / 8;
+}
+// Within each group of 8 W workgroups, XCD x takes the W consecutive logical workgroups
+// x W .. x W + W - 1 (W = 2^lw); the tail past the last whole group keeps its order.
+__device__ __forceinline__ uint32_t xcd_chunks_block(uint32_t lw) {
+    const uint32_t nb = gridDim.x, b = blockIdx.x, group = 8u << lw;
+    if (b >= nb - nb % group) return b;
+    const uint32_t r = b & (group - 1u);
+    return b - r + (r & 7u) * (1u << lw) + (r >> 3);
 }
 // The slice a wavefront works on under `order` (4 wavefronts of 64 lanes per workgroup).
 __device__ __forceinline__ uint32_t slice_of(uint32_t order) {
