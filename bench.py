@@ -314,6 +314,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay each V-cycle as a captured HIP graph (1) or launch eagerly (0)")
     ap.add_argument("--serve-requests", type=int, default=512, help="requests of the serving measurement (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate of the serving measurement, per s")
+    ap.add_argument("--multi-rhs-only", action="store_true",
+                    help="only the 32-load-case block (for profiling its batched kernels); prints its multi_rhs object")
     ap.add_argument("--profile-variants", action="store_true",
                     help="for a PMC profiler: one FP32 and one FP64 solve with every batch depth forced in turn, then "
                          "the 1 GiB calibration copy; prints no line")
@@ -333,6 +335,9 @@ def main():
     ctx = sf.Context(0)
     if args.profile_variants:
         profile_variants(sf, np, ctx, workload)
+        return
+    if args.multi_rhs_only:
+        print(json.dumps(multi_rhs(ctx, sf, np, args.columns or 32, args.steps, args.warmup, sequential=False)), flush=True)
         return
 
     log(f"building the plate {divisions}")
