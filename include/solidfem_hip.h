@@ -50,6 +50,7 @@ enum {
     SF_ERR_MG_COARSE_LENGTH = 27,          /* a=expected, b=found */
     /* JacobiError, crates/solidfem-solve/src/preconditioner.rs:22-31 */
     SF_ERR_JACOBI_DIAGONAL = 28,           /* a=row, value=diagonal */
+    SF_ERR_PRECONDITIONER = 29,            /* a caller's preconditioner failed: a=its return value */
     /* MeshError, crates/solidfem-mesh/src/error.rs:7-27 */
     SF_ERR_MESH_EMPTY = 30,
     SF_ERR_MESH_NON_FINITE_POSITION = 31,  /* a=node */
@@ -430,6 +431,25 @@ sf_status sf_multigrid_setup_coarse_factor(const sf_multigrid_setup *setup, uint
  * and the multigrid must outlive it. */
 sf_status sf_pcg_create(sf_context *ctx, const sf_csr *matrix, sf_multigrid *mg, const double *rhs,
                         const sf_pcg_options *options, sf_pcg **out);
+/* The Preconditioner seam (preconditioner.rs:9-20): a caller's M^-1, applied by a callback that
+ * writes z = M^-1 r for the n unknowns and returns 0 (non-zero fails the step with
+ * SF_ERR_PRECONDITIONER, a = the value). It must be linear, symmetric and positive definite.
+ *   SF_APPLY_HOST: r and z are host arrays (Preconditioner::apply's slices); the library copies
+ *     r out of HBM before the call and z back after it, 2 n doubles over PCIe per iteration.
+ *   SF_APPLY_DEVICE: r and z are device pointers; the callback enqueues its work on `stream`
+ *     (the solve's hipStream_t) and returns; the solve's next launch follows it on that stream.
+ * The callback runs on the thread that steps the solve. */
+typedef int32_t (*sf_apply_fn)(const double *r, double *z, uint64_t n, void *stream, void *user);
+enum { SF_APPLY_HOST = 0, SF_APPLY_DEVICE = 1 };
+typedef struct sf_preconditioner {
+    sf_apply_fn apply;
+    void *user;
+    int32_t where; /* SF_APPLY_HOST or SF_APPLY_DEVICE */
+    int32_t reserved;
+} sf_preconditioner;
+/* Pcg::new with a caller's preconditioner; the callback and its user data must outlive the solve. */
+sf_status sf_pcg_create_with(sf_context *ctx, const sf_csr *matrix, const sf_preconditioner *preconditioner,
+                             const double *rhs, const sf_pcg_options *options, sf_pcg **out);
 /* Pcg::step, pcg.rs:168-189 */
 sf_status sf_pcg_step(sf_pcg *pcg, uint64_t max_iterations, sf_cancel_fn cancel, void *user,
                       sf_progress *progress);

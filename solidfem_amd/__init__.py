@@ -82,7 +82,7 @@ def _raise(status: int):
     lib.sf_last_error_payload(C.byref(a), C.byref(b), C.byref(v))
     msg = lib.sf_last_error().decode()
     cls = SolidfemError
-    if 10 <= status < 20:
+    if 10 <= status < 20 or status == 29:  # 29: a caller's preconditioner failed inside a step
         cls = PcgError
     elif 20 <= status < 30:
         cls = MultigridError
@@ -806,10 +806,50 @@ class Pcg(_Handle):
         p._keep = (ctx, matrix, preconditioner)
         return p
 
+    @classmethod
+    def with_preconditioner(cls, ctx: Context, matrix: CsrMatrix, apply, rhs, options: PcgOptions = None,
+                            device: bool = False):
+        """Pcg::new with a caller's Preconditioner (preconditioner.rs:9-20). Host form: apply(r, z)
+        gets two numpy arrays of n values and writes M^-1 r into z. Device form (device=True):
+        apply(r_ptr, z_ptr, n, stream) gets device addresses and the solve's stream, and enqueues its
+        work there. An exception in apply fails the step (PcgError) and is raised after it."""
+        options = options or PcgOptions()
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        o = options.c()
+        failures = []
+
+        def trampoline(r, z, n, stream, _user):
+            try:
+                if device:
+                    apply(C.cast(r, C.c_void_p).value, C.cast(z, C.c_void_p).value, int(n), stream)
+                else:
+                    apply(np.ctypeslib.as_array(r, shape=(int(n),)), np.ctypeslib.as_array(z, shape=(int(n),)))
+                return 0
+            except BaseException as e:  # the callback cannot raise through C
+                failures.append(e)
+                return 1
+
+        fn = _capi.APPLY_FN(trampoline)
+        pre = _capi.Preconditioner(fn, None, 1 if device else 0, 0)
+        h = C.c_void_p()
+        status = library().sf_pcg_create_with(ctx._ptr, matrix._ptr, C.byref(pre), _ptr(b, C.c_double), C.byref(o),
+                                              C.byref(h))
+        if failures:
+            raise failures[0]
+        _raise(status)
+        p = cls(h)
+        p._keep = (ctx, matrix, fn, pre)
+        p._failures = failures
+        return p
+
     def step(self, max_iterations: int, cancel=None) -> Progress:
         p = _capi.Progress()
         cb = _cancel(cancel)
-        _raise(library().sf_pcg_step(self._ptr, max_iterations, cb, None, C.byref(p)))
+        status = library().sf_pcg_step(self._ptr, max_iterations, cb, None, C.byref(p))
+        failures = getattr(self, "_failures", None)
+        if failures:
+            raise failures.pop(0)
+        _raise(status)
         return Progress.of(p)
 
     def restart(self, rhs=None):

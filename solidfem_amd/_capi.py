@@ -17,6 +17,11 @@ vp = C.c_void_p
 
 CANCEL_FN = C.CFUNCTYPE(i32, vp)
 REGION_FN = C.CFUNCTYPE(i32, P(f64), vp)
+APPLY_FN = C.CFUNCTYPE(i32, P(f64), P(f64), u64, vp, vp)
+
+
+class Preconditioner(C.Structure):
+    _fields_ = [("apply", APPLY_FN), ("user", vp), ("where", i32), ("reserved", i32)]
 
 
 class Progress(C.Structure):
@@ -202,6 +207,7 @@ SIGNATURES = {
     "sf_multigrid_aggregates": (i32, [vp, u64, P(u32), P(u64)]),
     "sf_multigrid_solve_coarsest": (i32, [vp, P(f64), u64]),
     "sf_pcg_create": (i32, [vp, vp, vp, P(f64), P(PcgOptions), P(vp)]),
+    "sf_pcg_create_with": (i32, [vp, vp, P(Preconditioner), P(f64), P(PcgOptions), P(vp)]),
     "sf_pcg_step": (i32, [vp, u64, CANCEL_FN, vp, P(Progress)]),
     "sf_pcg_estimate": (i32, [vp, P(f64)]),
     "sf_pcg_progress": (i32, [vp, P(Progress)]),

@@ -801,6 +801,17 @@ sf_status sf_pcg_create(sf_context *ctx, const sf_csr *matrix, sf_multigrid *mg,
         *out = p.release();
     });
 }
+sf_status sf_pcg_create_with(sf_context *ctx, const sf_csr *matrix, const sf_preconditioner *preconditioner, const double *rhs,
+                             const sf_pcg_options *options, sf_pcg **out) {
+    return guard([&] {
+        if (!preconditioner) throw Error(SF_ERR_INVALID_ARGUMENT, "no preconditioner");
+        auto p = std::make_unique<sf_pcg>();
+        const sf_pcg_options o = options ? *options : sf_pcg_options{1e-9, 0, SF_DOT_TREE, 0};
+        std::vector<double> b(rhs, rhs + matrix->system.size);
+        p->pcg = std::make_unique<DevicePcg>(ctx->ctx, matrix->system, nullptr, b, o, preconditioner);
+        *out = p.release();
+    });
+}
 sf_status sf_pcg_step(sf_pcg *pcg, uint64_t max_iterations, sf_cancel_fn cancel, void *user, sf_progress *progress) {
     int outcome = SF_RUNNING;
     sf_status st = guard([&] { outcome = pcg->pcg->step(max_iterations, Cancel{cancel, user}); });

@@ -1212,8 +1212,15 @@ std::vector<double> jacobi_inverse(const std::shared_ptr<Context> &ctx, const De
 }
 
 DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
-                     const sf_pcg_options &options)
+                     const sf_pcg_options &options, const sf_preconditioner *user)
     : ctx_(std::move(ctx)), a_(a), mg_(mg), n_(a.size), dot_order_(options.dot_order), tolerance_(options.tolerance) {
+    if (user) {
+        if (!user->apply) throw Error(SF_ERR_INVALID_ARGUMENT, "the preconditioner has no apply callback");
+        if (user->where != SF_APPLY_HOST && user->where != SF_APPLY_DEVICE)
+            throw Error(SF_ERR_INVALID_ARGUMENT, "unknown preconditioner placement", user->where);
+        if (mg_) throw Error(SF_ERR_INVALID_ARGUMENT, "a solve takes one preconditioner");
+        user_ = *user;
+    }
     if (rhs.size() != n_) throw Error(SF_ERR_INVALID_ARGUMENT, "the right-hand side does not match the matrix");
     if (mg_ && mg_->dofs() != n_) throw Error(SF_ERR_INVALID_ARGUMENT, "the multigrid does not match the matrix");
     hipStream_t s = ctx_->stream;
@@ -1227,7 +1234,11 @@ DevicePcg::DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, Device
     SF_HIP(hipHostMalloc((void **)&pinned_, sizeof(PcgScalars), hipHostMallocCoherent));
     std::memset((void *)pinned_, 0, sizeof(PcgScalars));
     SF_HIP(hipHostGetDevicePointer((void **)&mirror_, pinned_, 0));
-    if (!mg_) jacobi_.upload(jacobi_inverse(ctx_, a_), s);
+    if (user_.apply && user_.where == SF_APPLY_HOST) {
+        SF_HIP(hipHostMalloc((void **)&host_r_, std::max<size_t>(n_, 1) * 8, hipHostMallocDefault));
+        SF_HIP(hipHostMalloc((void **)&host_z_, std::max<size_t>(n_, 1) * 8, hipHostMallocDefault));
+    }
+    if (!mg_ && !user_.apply) jacobi_.upload(jacobi_inverse(ctx_, a_), s);
     initialise();
 }
 
@@ -1267,6 +1278,8 @@ void DevicePcg::restart(const double *new_rhs) {
 DevicePcg::~DevicePcg() {
     (void)hipStreamSynchronize(ctx_->stream);
     if (pinned_) (void)hipHostFree(pinned_);
+    if (host_r_) (void)hipHostFree(host_r_);
+    if (host_z_) (void)hipHostFree(host_z_);
 }
 
 void DevicePcg::dot(const double *a, const double *b, double *result, int final_kind, PcgScalars *mirror, uint32_t tag) {
@@ -1294,8 +1307,24 @@ void DevicePcg::sum(const double *a, double *result) {
 }
 
 void DevicePcg::precondition(const double *r, double *z) {
-    if (mg_) mg_->apply(r, z, ctx_->stream);
-    else hipLaunchKernelGGL(jacobi_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, ctx_->stream, n_, jacobi_.ptr, r, z);
+    hipStream_t s = ctx_->stream;
+    if (user_.apply) {
+        int32_t rc = 0;
+        if (user_.where == SF_APPLY_DEVICE) {
+            rc = user_.apply(r, z, (uint64_t)n_, (void *)s, user_.user);
+        } else {
+            // the previous z's copy to the device is ordered before this copy on the stream, so
+            // the staging is free once this synchronisation returns
+            SF_HIP(hipMemcpyAsync(host_r_, r, n_ * 8, hipMemcpyDeviceToHost, s));
+            SF_HIP(hipStreamSynchronize(s));
+            rc = user_.apply(host_r_, host_z_, (uint64_t)n_, (void *)s, user_.user);
+            if (rc == 0) SF_HIP(hipMemcpyAsync(z, host_z_, n_ * 8, hipMemcpyHostToDevice, s));
+        }
+        if (rc != 0) throw Error(SF_ERR_PRECONDITIONER, "the caller's preconditioner failed", rc);
+        return;
+    }
+    if (mg_) mg_->apply(r, z, s);
+    else hipLaunchKernelGGL(jacobi_kernel, dim3(grid_for(n_, 256)), dim3(256), 0, s, n_, jacobi_.ptr, r, z);
     SF_HIP(hipGetLastError());
 }
 

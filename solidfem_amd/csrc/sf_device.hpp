@@ -224,7 +224,7 @@ enum class PcgStatus { Active, Converged, Cancelled, NotPositiveDefinite, NotCon
 class DevicePcg {
    public:
     DevicePcg(std::shared_ptr<Context> ctx, const DeviceSystem &a, DeviceMultigrid *mg, const std::vector<double> &rhs,
-              const sf_pcg_options &options);
+              const sf_pcg_options &options, const sf_preconditioner *user = nullptr);
     // Pcg::new again on the buffers already held, optionally with a new right-hand side.
     void restart(const double *new_rhs);
     ~DevicePcg();
@@ -265,6 +265,8 @@ class DevicePcg {
     dev::PcgScalars *pinned_ = nullptr;  // fine-grained host memory the r.r kernel copies the scalars into
     dev::PcgScalars *mirror_ = nullptr;  // its device address
     uint32_t readbacks_ = 0;             // tags of the copies so far
+    sf_preconditioner user_{};           // a caller's preconditioner (apply null: none)
+    double *host_r_ = nullptr, *host_z_ = nullptr;  // pinned staging of a host-side preconditioner
 };
 
 // Several load cases over one matrix and one preconditioner: every column runs the recurrence of
