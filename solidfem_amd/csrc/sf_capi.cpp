@@ -55,7 +55,8 @@ constexpr size_t DEVICE_SETUP_ROWS = 4096;
 
 void attach_device_setup(MultigridSetup &setup, const std::shared_ptr<Context> &ctx, const SetupOptions &o, const double *level0_values) {
     if (!ctx || o.setup_device == SF_SETUP_HOST) return;
-    setup.attach(std::make_unique<DeviceSetup>(ctx, o.setup_device == SF_SETUP_DEVICE ? 1 : DEVICE_SETUP_ROWS, level0_values));
+    setup.attach(std::make_unique<DeviceSetup>(ctx, o.setup_device == SF_SETUP_DEVICE ? 1 : DEVICE_SETUP_ROWS, level0_values,
+                                               o.setup_device == SF_SETUP_DEVICE));
 }
 
 double seconds() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }

@@ -344,6 +344,19 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                 }
                 case Phase::Seed: {
                     // aggregate.rs:119-136: a row seeds when it and all its strong neighbours are free
+                    if (cursor_ == 0 && accelerator_ && accelerator_->takes(level, current_)) {
+                        // on the accelerator the pass is one unit of the rows' budget
+                        if (budget == 0) return false;
+                        if (cancel()) {
+                            cancelled_ = true;
+                            throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
+                        }
+                        if (accelerator_->seed(level, strong_starts_, strong_columns_, aggregate_, seeded_, aggregate_count_)) {
+                            budget -= std::min(budget, rows);
+                            phase_ = Phase::Join;
+                            break;
+                        }
+                    }
                     bool done = run_rows(rows, budget, cancel, false, [&](size_t f, size_t l, bool) {
                         for (size_t row = f; row < l; ++row) {
                             const u32 *nb = &strong_columns_[(size_t)strong_starts_[row]];

@@ -82,6 +82,14 @@ class SetupAccelerator {
     // on the host (with the unit diagonal where no aggregate direction reaches already applied).
     virtual std::shared_ptr<const ValueSource> coarse_values() = 0;
     virtual void end_level() = 0;
+    // The seed pass of aggregation (aggregate.rs:119-136) over the strong neighbour lists, with
+    // the serial pass's result: aggregate numbers for the seeds' rows, their seeded flags and the
+    // count. False leaves it to the host. It may run while spectral() runs on another thread.
+    virtual bool seed(int level, const std::vector<int64_t> &starts, const std::vector<u32> &columns, std::vector<u32> &aggregate,
+                      std::vector<char> &seeded, u32 &count) {
+        (void)level, (void)starts, (void)columns, (void)aggregate, (void)seeded, (void)count;
+        return false;
+    }
 };
 
 class MultigridSetup {

@@ -376,6 +376,56 @@ std::vector<int64_t> starts_of(const std::vector<u32> &counts) {
     return starts;
 }
 
+// ---- the seed pass of aggregation, in rounds ----
+//
+// The serial pass (aggregate.rs:119-136) visits the rows in index order, and a row seeds when it
+// and its strong neighbours (its closed neighbourhood) are all still free. That is the greedy
+// independent set, in index order, of the graph in which two rows meet when their closed
+// neighbourhoods share a row. The rounds below build that same set: in each, every undecided row
+// claims the rows of its closed neighbourhood with its index (the least claim wins); a row that
+// wins all of its own seeds, since no undecided row of lower index shares a row with it, and the
+// serial pass would find its neighbourhood free; then every undecided row whose neighbourhood
+// meets a seed's is out. The least undecided row seeds in every round, so the rounds end.
+constexpr uint8_t SEED_UNDECIDED = 0, SEED_IN = 1, SEED_OUT = 2, SEED_NEVER = 3;
+
+__global__ void seed_claim_kernel(u32 rows, const u32 *__restrict__ starts, const u32 *__restrict__ cols,
+                                  const uint8_t *__restrict__ state, u32 *__restrict__ bid) {
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows || state[r] != SEED_UNDECIDED) return;
+    atomicMin(&bid[r], r);
+    for (u32 k = starts[r]; k < starts[r + 1]; ++k) atomicMin(&bid[cols[k]], r);
+}
+
+__global__ void seed_decide_kernel(u32 rows, const u32 *__restrict__ starts, const u32 *__restrict__ cols,
+                                   uint8_t *__restrict__ state, const u32 *__restrict__ bid, u32 *__restrict__ owner) {
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows || state[r] != SEED_UNDECIDED || bid[r] != r) return;
+    for (u32 k = starts[r]; k < starts[r + 1]; ++k)
+        if (bid[cols[k]] != r) return;
+    // two seeds of one round never share a row: each won all of its own
+    state[r] = SEED_IN;
+    owner[r] = r;
+    for (u32 k = starts[r]; k < starts[r + 1]; ++k) owner[cols[k]] = r;
+}
+
+__global__ void seed_settle_kernel(u32 rows, const u32 *__restrict__ starts, const u32 *__restrict__ cols,
+                                   uint8_t *__restrict__ state, const u32 *__restrict__ owner, u32 *__restrict__ bid,
+                                   u32 *__restrict__ undecided) {
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    bid[r] = UINT_MAX;  // the next round's claims start from none
+    if (state[r] != SEED_UNDECIDED) return;
+    bool out = owner[r] != UINT_MAX;
+    for (u32 k = starts[r]; k < starts[r + 1] && !out; ++k) out = owner[cols[k]] != UINT_MAX;
+    if (out) state[r] = SEED_OUT;
+    else if (undecided) *undecided = 1u;
+}
+
+__global__ void fill_u32_kernel(size_t n, u32 value, u32 *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = value;
+}
+
 }  // namespace
 
 std::atomic<u32> &setup_slot_limit() {
@@ -388,8 +438,8 @@ std::atomic<u32> &setup_slot_limit() {
     return limit;
 }
 
-DeviceSetup::DeviceSetup(std::shared_ptr<Context> ctx, size_t min_rows, const double *level0_values)
-    : ctx_(std::move(ctx)), min_rows_(min_rows), level0_values_(level0_values) {
+DeviceSetup::DeviceSetup(std::shared_ptr<Context> ctx, size_t min_rows, const double *level0_values, bool aggregate_on_device)
+    : ctx_(std::move(ctx)), min_rows_(min_rows), level0_values_(level0_values), aggregate_on_device_(aggregate_on_device) {
     // fewer output blocks per row than the kernels hold, so that rows above it take the host
     // path (the fallback's own bits)
     if (const u32 limit = setup_slot_limit().load(std::memory_order_relaxed)) slot_limit_ = std::min<u32>(MAX_SLOTS, limit);
@@ -637,6 +687,67 @@ bool DeviceSetup::galerkin(u32 count, const std::vector<int64_t> &pcol_starts, c
     ap_starts_.release();
     ap_cols_.release();
     ap_vals_.release();
+    return true;
+}
+
+bool DeviceSetup::seed(int level, const std::vector<int64_t> &starts, const std::vector<u32> &columns, std::vector<u32> &aggregate,
+                       std::vector<char> &seeded, u32 &count) {
+    (void)level;
+    if (!aggregate_on_device_) return false;
+    const size_t rows = starts.size() - 1;
+    if (rows == 0 || rows >= (size_t)UINT_MAX || columns.size() >= (size_t)UINT_MAX) return false;
+    // its own stream: the power iteration may be running on the context's stream from another thread
+    SF_HIP(hipSetDevice(ctx_->device));
+    hipStream_t s = nullptr;
+    SF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s); }
+    } guard{s};
+    std::vector<u32> st(rows + 1);
+    std::vector<uint8_t> state(rows);
+    for (size_t r = 0; r <= rows; ++r) st[r] = (u32)starts[r];
+    for (size_t r = 0; r < rows; ++r) state[r] = starts[r + 1] > starts[r] ? SEED_UNDECIDED : SEED_NEVER;
+    DeviceBuffer<u32> d_starts, d_cols, d_bid(rows), d_owner(rows), d_flag(1);
+    DeviceBuffer<uint8_t> d_state;
+    d_starts.upload(st, s);
+    d_cols.upload(columns.empty() ? std::vector<u32>{0} : columns, s);
+    d_state.upload(state, s);
+    const unsigned grid = (unsigned)((rows + 255) / 256);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, s, rows, (u32)UINT_MAX, d_bid.ptr);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, s, rows, (u32)UINT_MAX, d_owner.ptr);
+    constexpr int ROUNDS_PER_CHECK = 8;
+    for (size_t round = 0;; round += ROUNDS_PER_CHECK) {
+        if (round > rows + ROUNDS_PER_CHECK) throw Error(SF_ERR_INTERNAL, "the seed rounds did not end");
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, (size_t)1, 0u, d_flag.ptr);
+        for (int i = 0; i < ROUNDS_PER_CHECK; ++i) {
+            hipLaunchKernelGGL(seed_claim_kernel, dim3(grid), dim3(256), 0, s, (u32)rows, (const u32 *)d_starts.ptr,
+                               (const u32 *)d_cols.ptr, (const uint8_t *)d_state.ptr, d_bid.ptr);
+            hipLaunchKernelGGL(seed_decide_kernel, dim3(grid), dim3(256), 0, s, (u32)rows, (const u32 *)d_starts.ptr,
+                               (const u32 *)d_cols.ptr, d_state.ptr, (const u32 *)d_bid.ptr, d_owner.ptr);
+            hipLaunchKernelGGL(seed_settle_kernel, dim3(grid), dim3(256), 0, s, (u32)rows, (const u32 *)d_starts.ptr,
+                               (const u32 *)d_cols.ptr, d_state.ptr, (const u32 *)d_owner.ptr, d_bid.ptr,
+                               i == ROUNDS_PER_CHECK - 1 ? d_flag.ptr : nullptr);
+        }
+        SF_HIP(hipGetLastError());
+        u32 left = 0;
+        d_flag.download(&left, 1, s);
+        SF_HIP(hipStreamSynchronize(s));
+        if (!left) break;
+    }
+    std::vector<u32> owner(rows);
+    d_owner.download(owner.data(), rows, s);
+    d_state.download(state.data(), rows, s);
+    SF_HIP(hipStreamSynchronize(s));
+    // aggregate numbers in seed order, which is index order
+    std::vector<u32> id(rows, UINT_MAX);
+    for (size_t r = 0; r < rows; ++r)
+        if (state[r] == SEED_IN) id[r] = count++;
+    for (size_t r = 0; r < rows; ++r)
+        if (owner[r] != UINT_MAX) {
+            aggregate[r] = id[owner[r]];
+            seeded[r] = 1;
+        }
     return true;
 }
 

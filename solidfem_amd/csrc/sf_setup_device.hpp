@@ -14,7 +14,9 @@ class DeviceSetup final : public SetupAccelerator {
     // Levels of at least `min_rows` block rows run their numeric phases here. `level0_values`,
     // when given, is the finest matrix's values already on the device in block-row order (the
     // study's assembled stiffness); the setup borrows it while level 0 is built.
-    DeviceSetup(std::shared_ptr<Context> ctx, size_t min_rows, const double *level0_values);
+    // With `aggregate_on_device`, the seed pass of aggregation runs here too (in rounds, with the
+    // serial pass's result).
+    DeviceSetup(std::shared_ptr<Context> ctx, size_t min_rows, const double *level0_values, bool aggregate_on_device = false);
     ~DeviceSetup() override;
 
     bool takes(int level, const HostBsr<double> &a) const override;
@@ -29,6 +31,8 @@ class DeviceSetup final : public SetupAccelerator {
     std::shared_ptr<const ValueSource> coarse_values() override;
     void product_values(HostBsr<double> &ap) override;
     void end_level() override;
+    bool seed(int level, const std::vector<int64_t> &starts, const std::vector<u32> &columns, std::vector<u32> &aggregate,
+              std::vector<char> &seeded, u32 &count) override;
 
    private:
     struct Overflow {};
@@ -38,6 +42,7 @@ class DeviceSetup final : public SetupAccelerator {
     std::shared_ptr<Context> ctx_;
     size_t min_rows_;
     const double *level0_values_;
+    bool aggregate_on_device_;
     int level_ = -1, rb_ = 3;
     size_t rows_ = 0;
     DeviceBuffer<int64_t> a_starts_, p_starts_, ap_starts_;

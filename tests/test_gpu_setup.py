@@ -5,7 +5,9 @@ prolongator (setup.rs:527-573, transfer.rs:108-152), A P (transfer.rs:155-169) a
 product (transfer.rs:204-238) run on the device when the setup has a context. Every level's
 matrix, block inverse, spectrum, prolongator and aggregates, and the coarse factor, must be the
 host setup's bits, which the CPU suite pins to the oracle; here the oracle checks the device
-hierarchy directly too.
+hierarchy directly too. With every level on the device (SetupDevice.DEVICE) the seed pass of the
+aggregation (aggregate.rs:119-136) runs there as well, in parallel rounds that build the serial
+pass's greedy set, so the aggregates, and with them every level, keep their bits.
 """
 import numpy as np
 import pytest
@@ -88,6 +90,14 @@ def test_a_sliced_device_setup_gives_the_same_bits(ctx, budget):
     ref = refcpu.Problem(w)
     options = sf.MultigridOptions(coarse_dofs=coarse, precision=sf.Precision.DOUBLE, setup_device=sf.SetupDevice.DEVICE)
     assert_same_setup(run_setup(ctx, ref, options, budget=budget), run_setup(None, ref, options))
+
+
+def test_the_device_seed_pass_on_a_larger_plate_is_the_serial_pass(ctx):
+    # about 36,000 block rows on level 0: many rounds of the parallel seed pass
+    ref = refcpu.Problem(plate_config(divisions=(40, 40, 2)))
+    device = run_setup(ctx, ref, sf.MultigridOptions(precision=sf.Precision.SINGLE, setup_device=sf.SetupDevice.DEVICE))
+    host = run_setup(ctx, ref, sf.MultigridOptions(precision=sf.Precision.SINGLE, setup_device=sf.SetupDevice.HOST))
+    assert_same_setup(device, host)
 
 
 def test_automatic_placement_on_a_larger_plate(ctx):
