@@ -13,13 +13,14 @@ using namespace dev;
 
 static inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
-std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M) {
+std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M, bool product) {
     // Depths 1, 2, 4 of the lane-group kernels, NODE_FORM + depth for one wavefront per block
     // row, and the forms with two or four columns per lane where M allows. A lane holds BATCH
     // blocks of rb * cb values and BATCH * cb * CPL entries of x: keep that within registers.
     const bool large = (size_t)rb * cb * value_size > 144;
     std::vector<int> c = large ? std::vector<int>{1, 2, NODE_FORM + 1, NODE_FORM + 2}
                                : std::vector<int>{1, 2, 4, NODE_FORM + 1, NODE_FORM + 2, NODE_FORM + 4};
+    if (!large && product) c.push_back(8);  // products only: the Chebyshev step has no depth 8
     if (M % 2 == 0 && M >= 4) {
         c.push_back(PAIR_FORM + 1);
         c.push_back(PAIR_FORM + 2);
@@ -54,6 +55,13 @@ static void spmm_depth(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, c
     switch (depth) {
         case 1: spmm_launch<RB, CB, 1, CPL>(a, M, x, y, b, epi, s, node); break;
         case 2: spmm_launch<RB, CB, 2, CPL>(a, M, x, y, b, epi, s, node); break;
+        case 8:
+            // eight blocks in flight: for one column per lane (long rows, few of them)
+            if constexpr (CPL == 1) {
+                spmm_launch<RB, CB, 8, CPL>(a, M, x, y, b, epi, s, node);
+                break;
+            }
+            [[fallthrough]];
         default: spmm_launch<RB, CB, 4, CPL>(a, M, x, y, b, epi, s, node); break;
     }
 }

@@ -846,7 +846,7 @@ class Hierarchy final : public DeviceMultigrid {
             for (V *v : vs) SF_HIP(hipMemsetAsync(v, 0, need * sizeof(V), s));
             d.mm_depth[lg][0] = tune_batch(s, [&](int batch) {
                 launch_cheb_batch(d, batch, M, d.md0.ptr, d.mb.ptr, (V)0, (V)0, d.md1.ptr, d.mr0.ptr, d.mx.ptr, nullptr, s);
-            }, spmm_candidates(d.rb, d.rb, sizeof(S), M));
+            }, spmm_candidates(d.rb, d.rb, sizeof(S), M, false));
             d.mm_depth[lg][1] = tune_batch(s, [&](int batch) {
                 launch_spmm<S, V>(d.a, M, d.mx.ptr, d.mres.ptr, d.mb.ptr, EPI_RESIDUAL, s, batch);
             }, spmm_candidates(d.rb, d.rb, sizeof(S), M));

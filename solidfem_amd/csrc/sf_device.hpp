@@ -130,7 +130,8 @@ inline int columns_log2(uint32_t m) {
 template <typename S, typename V>
 void launch_spmm(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch = 0);
 // The batch depths worth trying for a matrix's batched product.
-std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M);
+// The variant codes a batched product (`product`) or Chebyshev step is timed over for M columns.
+std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M, bool product = true);
 // Tunes a matrix's product on scratch vectors of its shape.
 template <typename S, typename V>
 void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s);
