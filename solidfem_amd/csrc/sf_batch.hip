@@ -89,7 +89,8 @@ void launch_spmm(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V
             SF_HIP(hipMemsetAsync(tx.ptr, 0, tx.bytes(), s));
             SF_HIP(hipMemsetAsync(ty.ptr, 0, ty.bytes(), s));
             tuned = tune_batch(s, [&](int depth) { launch_spmm<S, V>(a, M, tx.ptr, ty.ptr, ty.ptr, EPI_STORE, s, depth); },
-                               spmm_candidates(a.rb, a.cb, sizeof(S), M));
+                               spmm_candidates(a.rb, a.cb, sizeof(S), M), "",
+                               "batch product " + std::to_string(columns_log2(M)) + " " + shape_class(a) + "|" + std::to_string(sizeof(V)));
             SF_HIP(hipStreamSynchronize(s));
         }
         batch = tuned;

@@ -4,6 +4,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 
 #include "sf_batch_kernels.hpp"
@@ -368,15 +370,50 @@ std::atomic<int> &forced_variant() {
     return code;
 }
 
-int tune_batch(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates, const std::string &what) {
+int tune_batch(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates, const std::string &what,
+               const std::string &memo) {
     // a forced code is taken wherever it is a candidate
     const int forced = forced_variant().load(std::memory_order_relaxed);
     for (int c : candidates)
         if (forced > 0 && forced == c) return forced;
-    return fastest(s, launch, candidates, what);
+    return fastest(s, launch, candidates, what, memo);
 }
 
-int fastest(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates, const std::string &what) {
+static int fastest_timed(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates,
+                         const std::string &what);
+
+namespace {
+std::mutex g_memo_lock;
+std::map<std::string, int> &tuning_memo() {
+    static std::map<std::string, int> memo;
+    return memo;
+}
+}  // namespace
+
+int fastest(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates, const std::string &what,
+            const std::string &memo) {
+    static const bool verbose = std::getenv("SOLIDFEM_TUNE_LOG") != nullptr;
+    std::string key;
+    if (!memo.empty()) {
+        key = memo;
+        for (int c : candidates) key += "," + std::to_string(c);
+        std::lock_guard<std::mutex> g(g_memo_lock);
+        auto it = tuning_memo().find(key);
+        if (it != tuning_memo().end()) {
+            if (verbose) std::fprintf(stderr, "[tune] %s: kept %d for %s\n", what.c_str(), it->second, memo.c_str());
+            return it->second;
+        }
+    }
+    const int best = fastest_timed(s, launch, candidates, what);
+    if (!key.empty()) {
+        std::lock_guard<std::mutex> g(g_memo_lock);
+        tuning_memo().emplace(key, best);
+    }
+    return best;
+}
+
+static int fastest_timed(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates,
+                         const std::string &what) {
     hipEvent_t e0, e1;
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
@@ -417,7 +454,8 @@ void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s) {
     DeviceBuffer<V> x(std::max<size_t>(a.block_cols * a.cb, 1)), y(std::max<size_t>(a.rows * a.rb, 1));
     SF_HIP(hipMemsetAsync(x.ptr, 0, x.bytes(), s));
     SF_HIP(hipMemsetAsync(y.ptr, 0, y.bytes(), s));
-    a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(a, x.ptr, y.ptr, y.ptr, epi, s, batch); });
+    a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(a, x.ptr, y.ptr, y.ptr, epi, s, batch); }, {1, 2, 4, 8}, "",
+                              "product " + std::to_string(epi) + " " + shape_class(a) + "|" + std::to_string(sizeof(V)));
     SF_HIP(hipStreamSynchronize(s));
 }
 
@@ -838,6 +876,8 @@ class Hierarchy final : public DeviceMultigrid {
             min_.resize(top);
             mout_.resize(top);
         }
+        // the memo's class: the column count's group, the matrix's shape class, the vector precision
+        const std::string group = std::to_string(lg) + " ", vsize = "|" + std::to_string(sizeof(V));
         for (size_t l = 0; l < lv_.size(); ++l) {
             LevelDev<S, V> &d = lv_[l];
             if (d.mm_depth[lg][0]) continue;
@@ -846,10 +886,10 @@ class Hierarchy final : public DeviceMultigrid {
             for (V *v : vs) SF_HIP(hipMemsetAsync(v, 0, need * sizeof(V), s));
             d.mm_depth[lg][0] = tune_batch(s, [&](int batch) {
                 launch_cheb_batch(d, batch, M, d.md0.ptr, d.mb.ptr, (V)0, (V)0, d.md1.ptr, d.mr0.ptr, d.mx.ptr, nullptr, s);
-            }, spmm_candidates(d.rb, d.rb, sizeof(S), M, false));
+            }, spmm_candidates(d.rb, d.rb, sizeof(S), M, false), "", "batch step " + group + shape_class(d.a) + vsize);
             d.mm_depth[lg][1] = tune_batch(s, [&](int batch) {
                 launch_spmm<S, V>(d.a, M, d.mx.ptr, d.mres.ptr, d.mb.ptr, EPI_RESIDUAL, s, batch);
-            }, spmm_candidates(d.rb, d.rb, sizeof(S), M));
+            }, spmm_candidates(d.rb, d.rb, sizeof(S), M), "", "batch residual " + group + shape_class(d.a) + vsize);
             if (d.has_p) {
                 LevelDev<S, V> &c = lv_[l + 1];
                 const size_t cneed = c.n * (size_t)M;
@@ -857,10 +897,10 @@ class Hierarchy final : public DeviceMultigrid {
                 SF_HIP(hipMemsetAsync(c.mx.ptr, 0, cneed * sizeof(V), s));
                 d.mm_depth[lg][2] = tune_batch(s, [&](int batch) {
                     launch_spmm<S, V>(d.r, M, d.mres.ptr, c.mb.ptr, nullptr, EPI_STORE, s, batch);
-                }, spmm_candidates(d.r.rb, d.r.cb, sizeof(S), M));
+                }, spmm_candidates(d.r.rb, d.r.cb, sizeof(S), M), "", "batch restriction " + group + shape_class(d.r) + vsize);
                 d.mm_depth[lg][3] = tune_batch(s, [&](int batch) {
                     launch_spmm<S, V>(d.p, M, c.mx.ptr, d.mpx.ptr, nullptr, EPI_ADD, s, batch);
-                }, spmm_candidates(d.p.rb, d.p.cb, sizeof(S), M));
+                }, spmm_candidates(d.p.rb, d.p.cb, sizeof(S), M), "", "batch prolongation " + group + shape_class(d.p) + vsize);
             }
         }
         SF_HIP(hipStreamSynchronize(s));
@@ -950,38 +990,39 @@ class Hierarchy final : public DeviceMultigrid {
         for (size_t l = 0; l < lv_.size(); ++l) {
             LevelDev<S, V> &d = lv_[l];
             const std::string at = "L" + std::to_string(l) + (sizeof(V) == 4 ? " f32 " : " f64 ");
+            const std::string vs_ = "|" + std::to_string(sizeof(V)), a_class = shape_class(d.a) + vs_;
             V *vs[8] = {d.b.ptr, d.x.ptr, d.d0.ptr, d.d1.ptr, d.r0.ptr, d.r1.ptr, d.res.ptr, d.px.ptr};
             for (V *v : vs) SF_HIP(hipMemsetAsync(v, 0, d.n * sizeof(V), s));
             d.a.batch_cheb = tune_batch(s, [&](int batch) {
                 launch_cheb_step(d, batch, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
-            }, {1, 2, 4, 8}, at + "step depth");
+            }, {1, 2, 4, 8}, at + "step depth", "step " + a_class);
             d.a.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, batch); },
-                                        {1, 2, 4, 8}, at + "residual depth");
+                                        {1, 2, 4, 8}, at + "residual depth", "residual " + a_class);
             if (d.a_rows.rows) {
                 // each kernel against its row-per-wavefront forms with 1, 2 or 4 rows per wavefront
                 // (all the same bits); 0 is the sliced kernel
                 d.cheb_rows = fastest(s, [&](int rpw) {
                     if (!rpw) launch_cheb_step(d, d.a.batch_cheb, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
                     else launch_cheb_rows(d, rpw, d.d0.ptr, d.b.ptr, (V)0, (V)0, d.d1.ptr, d.r0.ptr, d.x.ptr, nullptr, s);
-                }, {0, 1, 2, 4}, at + "step rows per wave");
+                }, {0, 1, 2, 4}, at + "step rows per wave", "step rows " + a_class);
                 d.residual_rows = fastest(s, [&](int rpw) {
                     if (!rpw) launch_spmv<S, V>(d.a, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s);
                     else launch_long_rows<S, V>(d.a_rows, d.x.ptr, d.res.ptr, d.b.ptr, EPI_RESIDUAL, s, {nullptr, (V)0, nullptr, nullptr}, rpw);
-                }, {0, 1, 2, 4}, at + "residual rows per wave");
+                }, {0, 1, 2, 4}, at + "residual rows per wave", "residual rows " + a_class);
             }
             if (d.has_p) {
                 LevelDev<S, V> &c = lv_[l + 1];
                 SF_HIP(hipMemsetAsync(c.b.ptr, 0, c.n * sizeof(V), s));
                 SF_HIP(hipMemsetAsync(c.x.ptr, 0, c.n * sizeof(V), s));
                 d.r.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, batch); },
-                                            {1, 2, 4, 8}, at + "restriction depth");
+                                            {1, 2, 4, 8}, at + "restriction depth", "restriction " + shape_class(d.r) + vs_);
                 // the sliced product at its best depth (0) against 1, 2 or 4 rows per wavefront
                 d.restrict_rows = fastest(s, [&](int rpw) {
                     if (!rpw) launch_spmv<S, V>(d.r, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s);
                     else launch_long_rows<S, V>(d.r_rows, d.res.ptr, c.b.ptr, nullptr, EPI_STORE, s, {nullptr, (V)0, nullptr, nullptr}, rpw);
-                }, {0, 1, 2, 4}, at + "restriction rows per wave");
+                }, {0, 1, 2, 4}, at + "restriction rows per wave", "restriction rows " + shape_class(d.r) + vs_);
                 d.p.batch_spmv = tune_batch(s, [&](int batch) { launch_spmv<S, V>(d.p, c.x.ptr, d.px.ptr, nullptr, EPI_ADD, s, batch); },
-                                            {1, 2, 4, 8}, at + "prolongation depth");
+                                            {1, 2, 4, 8}, at + "prolongation depth", "prolongation " + shape_class(d.p) + vs_);
             }
         }
         SF_HIP(hipStreamSynchronize(s));

@@ -2,7 +2,9 @@
 // V-cycle, the conjugate gradient solve and the stiffness assembly, all resident in HBM.
 #pragma once
 
+#include <cmath>
 #include <functional>
+#include <string>
 
 #include "sf_host.hpp"
 #include "sf_kernels.hpp"
@@ -97,10 +99,22 @@ void launch_spmv(const DeviceSliced<S> &a, const V *x, V *y, const V *b, int epi
 // The tuning variant forced on every tuning that has it among its candidates (0: none).
 std::atomic<int> &forced_variant();
 int tune_batch(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates = {1, 2, 4, 8},
-               const std::string &what = "");
+               const std::string &what = "", const std::string &memo = "");
 // The candidate whose launches `launch(candidate)` run fastest (timed with HIP events on `s`).
+// With a `memo` key (a kernel and a matrix shape class), the first choice made for that key and
+// candidate list is kept for the process and returned again without timing: matrices of one
+// class (a new system in a server, a study rebuilt) skip the timing. Every choice gives the
+// same bits.
 int fastest(hipStream_t s, const std::function<void(int)> &launch, const std::vector<int> &candidates,
-            const std::string &what = "");
+            const std::string &what = "", const std::string &memo = "");
+// A matrix's shape class for the tuning memo: block shape, value size, and the binary orders of
+// its block rows and of its mean blocks per row.
+template <typename T>
+std::string shape_class(const DeviceSliced<T> &a) {
+    auto order = [](double v) { return v < 1 ? 0 : (int)std::floor(std::log2(v)); };
+    return std::to_string(a.rb) + "x" + std::to_string(a.cb) + "/" + std::to_string(sizeof(T)) + "/" +
+           std::to_string(order((double)a.rows)) + "/" + std::to_string(order(2.0 * (double)a.blocks / std::max<double>(1.0, (double)a.rows)));
+}
 
 // ---- several right-hand sides ------------------------------------------------------------------
 

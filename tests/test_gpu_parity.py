@@ -302,3 +302,23 @@ def test_graph_replays_equal_eager_launches(ctx, precision):
     b.restart()
     assert b.step(10 ** 6).iteration == pa.iteration
     assert np.array_equal(bits(a.estimate()), bits(b.estimate()))
+
+
+def test_a_rebuilt_study_keeps_its_kernel_choices_and_bits(ctx):
+    # the second build of a matrix class takes the first's tuned kernel forms from the tuning
+    # memo (no timing); the choices never change a bit
+    w = plate_config(divisions=(24, 24, 2))
+
+    def build():
+        b = sf.StudyBuilder.new(ctx, sf.Model.from_workload(w), sf.PcgOptions(tolerance=1e-8),
+                                sf.MultigridOptions(precision=sf.Precision.SINGLE))
+        b.step(2 ** 62)
+        s = b.finish()
+        assert s.solve().outcome == sf.StepOutcome.CONVERGED
+        mg = s.multigrid()
+        depths = [(mg.tuned_depth(level, 0), mg.tuned_depth(level, 1)) for level in range(mg.stats().levels)]
+        return depths, s.system().tuned_depth(), s.displacements()
+
+    first, second = build(), build()
+    assert first[0] == second[0] and first[1] == second[1]
+    assert np.array_equal(bits(first[2]), bits(second[2]))
