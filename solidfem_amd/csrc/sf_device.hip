@@ -617,26 +617,24 @@ class Hierarchy final : public DeviceMultigrid {
                     for (size_t k = 0; k <= i; ++k) lt[k * n + i] = h.factor[i * n + k];
                 d.lt.upload(lt, s);
                 if (coarse_mode == SF_COARSE_INVERSE) {
-                    // inverse = L^-T L^-1 in f64: the columns of L^-1 by forward substitution, then
-                    // entry (i, j) = sum_k Linv[k][i] Linv[k][j], symmetric by construction
-                    std::vector<double> linv(n * n, 0.0), inv(n * n, 0.0);
-                    parallel_for(n, 8, [&](size_t c0, size_t c1) {
-                        for (size_t j = c0; j < c1; ++j)
-                            for (size_t i = j; i < n; ++i) {
-                                double v = i == j ? 1.0 : 0.0;
-                                for (size_t k = j; k < i; ++k) v -= h.factor[i * n + k] * linv[k * n + j];
-                                linv[i * n + j] = v / h.factor[i * n + i];
-                            }
-                    });
-                    parallel_for(n, 8, [&](size_t r0, size_t r1) {
-                        for (size_t i = r0; i < r1; ++i)
-                            for (size_t j = 0; j < n; ++j) {
-                                double v = 0.0;
-                                for (size_t k = std::max(i, j); k < n; ++k) v += linv[k * n + i] * linv[k * n + j];
-                                inv[j * n + i] = v;
-                            }
-                    });
-                    d.inverse.upload(inv, s);
+                    // inverse = L^-T L^-1 in f64 on the device (symmetric by construction): the
+                    // columns of L^-1, then every entry, each in a fixed order
+                    DeviceBuffer<double> linv;
+                    std::vector<double> identity(n * n, 0.0);
+                    for (size_t i = 0; i < n; ++i) identity[i * n + i] = 1.0;
+                    linv.upload(identity, s);
+                    d.inverse.resize(n * n);
+                    for (size_t k = 0; k < n; ++k) {
+                        hipLaunchKernelGGL(factor_inverse_finish_row_kernel, dim3(grid_for(k + 1, 256)), dim3(256), 0, s, (uint32_t)n,
+                                           (uint32_t)k, (const double *)d.lt.ptr, linv.ptr);
+                        if (k + 1 < n)
+                            hipLaunchKernelGGL(factor_inverse_update_kernel, dim3(grid_for(k + 1, 256), (unsigned)(n - k - 1)), dim3(256), 0,
+                                               s, (uint32_t)n, (uint32_t)k, (const double *)d.lt.ptr, linv.ptr);
+                    }
+                    hipLaunchKernelGGL(factor_inverse_entries_kernel, dim3(grid_for(n * n, 256)), dim3(256), 0, s, (uint32_t)n,
+                                       (const double *)linv.ptr, d.inverse.ptr);
+                    SF_HIP(hipGetLastError());
+                    SF_HIP(hipStreamSynchronize(s));  // linv is freed on return
                 }
                 const size_t need = (n * (n + 1) / 2 + 2 * n) * sizeof(double);
                 int dev = 0, limit = 0;

@@ -563,6 +563,46 @@ __global__ __launch_bounds__(64) void coarse_solve_lds_kernel(uint32_t n, const 
 // The coarsest level's solve as a product with the f64 inverse (stored by columns: inv[k n + i]
 // is entry (i, k)), the right-hand side widened from V and staged through LDS, each row summed
 // in ascending k from zero: a fixed order, so the same bits on every run.
+// The explicit inverse of a coarse factor, L^-T L^-1. First linv = L^-1 by forward substitution
+// for all columns at once, one step per row k: row k is finished (divided by L[k][k]), then every
+// later row i takes the term L[i][k] linv[k][j] off its entries j <= k. Entry (i, j) thus computes
+// ((i == j) - sum over k from j to i - 1 of L[i][k] linv[k][j], subtracted in ascending k) /
+// L[i][i]; `lt` holds L by columns (lt[k n + i] = L[i][k]), linv starts as the identity. Then
+// entry (i, j) of the inverse = sum over k from max(i, j) of linv[k][i] linv[k][j], added in
+// ascending k from zero, stored at inv[j n + i] (the loads of each chunk of 16 terms are issued
+// before the terms are taken, in order).
+static __global__ void factor_inverse_finish_row_kernel(uint32_t n, uint32_t k, const double *__restrict__ lt, double *__restrict__ linv) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > k) return;
+    linv[(size_t)k * n + j] = linv[(size_t)k * n + j] / lt[(size_t)k * n + k];
+}
+static __global__ void factor_inverse_update_kernel(uint32_t n, uint32_t k, const double *__restrict__ lt, double *__restrict__ linv) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x, i = k + 1 + blockIdx.y;
+    if (j > k || i >= n) return;
+    linv[(size_t)i * n + j] -= lt[(size_t)k * n + i] * linv[(size_t)k * n + j];
+}
+static __global__ __launch_bounds__(256) void factor_inverse_entries_kernel(uint32_t n, const double *__restrict__ linv,
+                                                                           double *__restrict__ inv) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * n) return;
+    const uint32_t i = (uint32_t)(t % n), j = (uint32_t)(t / n);
+    constexpr uint32_t CHUNK = 16;
+    double v = 0.0;
+    uint32_t k = i > j ? i : j;
+    for (; k + CHUNK <= n; k += CHUNK) {
+        double p[CHUNK], q[CHUNK];
+#pragma unroll
+        for (uint32_t u = 0; u < CHUNK; ++u) {
+            p[u] = linv[(size_t)(k + u) * n + i];
+            q[u] = linv[(size_t)(k + u) * n + j];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < CHUNK; ++u) v += p[u] * q[u];
+    }
+    for (; k < n; ++k) v += linv[(size_t)k * n + i] * linv[(size_t)k * n + j];
+    inv[(size_t)j * n + i] = v;
+}
+
 template <typename V>
 __global__ __launch_bounds__(256) void coarse_inverse_kernel(uint32_t n, const double *__restrict__ inv,
                                                              const V *__restrict__ rhs, V *__restrict__ out) {

@@ -788,6 +788,18 @@ bool MultigridSetup::step(size_t budget, const Cancel &cancel) {
                     HostLevel &l = levels_.back();
                     const size_t n = current_.dofs();
                     double *a = l.factor.data();
+                    if (cursor_ == 0 && accelerator_) {
+                        // on the accelerator the factor is one unit of the budget
+                        if (budget == 0) return false;
+                        if (cancel()) {
+                            cancelled_ = true;
+                            throw Error(SF_ERR_MG_CANCELLED, "the multigrid setup was cancelled");
+                        }
+                        if (accelerator_->factor(level, n, a)) {
+                            budget -= std::min(budget, n);
+                            cursor_ = n;
+                        }
+                    }
                     while (cursor_ < n) {
                         if (budget == 0) return false;
                         if (cancel()) {

@@ -90,6 +90,13 @@ class SetupAccelerator {
         (void)level, (void)starts, (void)columns, (void)aggregate, (void)seeded, (void)count;
         return false;
     }
+    // The coarsest level's dense Cholesky factor (block.rs:327-348) of the n x n row-major matrix
+    // `a`, in place, with the host loop's result to the bit. False leaves it to the host; a matrix
+    // that is not positive definite throws the host's error. It may run while spectral() runs.
+    virtual bool factor(int level, size_t n, double *a) {
+        (void)level, (void)n, (void)a;
+        return false;
+    }
 };
 
 class MultigridSetup {

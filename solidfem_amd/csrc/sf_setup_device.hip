@@ -421,6 +421,32 @@ __global__ void seed_settle_kernel(u32 rows, const u32 *__restrict__ starts, con
     else if (undecided) *undecided = 1u;
 }
 
+// The dense Cholesky factor (block.rs:327-348) in place, right-looking: at step k, column k is
+// finished (its entries below the diagonal divided by the root of the diagonal), then every entry
+// of the trailing lower triangle takes its term a[i][k] a[j][k]. An entry thus receives its
+// subtractions one per step, in ascending k, and its division last: the operations, in the order,
+// of the host's column loop, whose bits it has. The diagonal a[k][k] is read by step k's division
+// and by nothing after, so the roots are stored at the end (the value under each root is kept for
+// the host's refusal of a matrix that is not positive definite).
+__global__ void cholesky_finish_column_kernel(u32 n, u32 k, double *__restrict__ a, double *__restrict__ pivots) {
+    const u32 i = k + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = a[(size_t)k * n + k];
+    if (i == k) pivots[k] = s;
+    else a[(size_t)i * n + k] = a[(size_t)i * n + k] / sqrt(s);
+}
+
+__global__ void cholesky_update_kernel(u32 n, u32 k, double *__restrict__ a) {
+    const u32 j = k + 1 + blockIdx.x * blockDim.x + threadIdx.x, i = k + 1 + blockIdx.y;
+    if (i >= n || j > i) return;
+    a[(size_t)i * n + j] -= a[(size_t)i * n + k] * a[(size_t)j * n + k];
+}
+
+__global__ void cholesky_diagonal_kernel(u32 n, const double *__restrict__ pivots, double *__restrict__ a) {
+    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) a[(size_t)j * n + j] = sqrt(pivots[j]);
+}
+
 __global__ void fill_u32_kernel(size_t n, u32 value, u32 *__restrict__ out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = value;
@@ -748,6 +774,39 @@ bool DeviceSetup::seed(int level, const std::vector<int64_t> &starts, const std:
             aggregate[r] = id[owner[r]];
             seeded[r] = 1;
         }
+    return true;
+}
+
+bool DeviceSetup::factor(int level, size_t n, double *a) {
+    // below this size the host's column loop is as quick as the launches
+    if (n < 128 || n > 65535) return false;
+    SF_HIP(hipSetDevice(ctx_->device));
+    hipStream_t s = nullptr;  // its own: the power iteration may be running on the context's stream
+    SF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s); }
+    } guard{s};
+    DeviceBuffer<double> d_a, d_pivots(n);
+    d_a.upload(a, n * n, s);
+    for (size_t k = 0; k < n; ++k) {
+        const size_t rest = n - k;  // rows k .. n - 1
+        hipLaunchKernelGGL(cholesky_finish_column_kernel, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, s, (u32)n, (u32)k, d_a.ptr,
+                           d_pivots.ptr);
+        if (rest > 1)
+            hipLaunchKernelGGL(cholesky_update_kernel, dim3((unsigned)((rest - 1 + 255) / 256), (unsigned)(rest - 1)), dim3(256), 0, s,
+                               (u32)n, (u32)k, d_a.ptr);
+    }
+    hipLaunchKernelGGL(cholesky_diagonal_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (u32)n, (const double *)d_pivots.ptr,
+                       d_a.ptr);
+    SF_HIP(hipGetLastError());
+    std::vector<double> pivots(n);
+    d_pivots.download(pivots.data(), n, s);
+    d_a.download(a, n * n, s);
+    SF_HIP(hipStreamSynchronize(s));
+    for (size_t j = 0; j < n; ++j)
+        if (!std::isfinite(pivots[j]) || pivots[j] <= 0.0)
+            throw Error(SF_ERR_MG_NOT_POSITIVE_DEFINITE, "the coarse matrix is not positive definite", level, (int64_t)j, pivots[j]);
     return true;
 }
 

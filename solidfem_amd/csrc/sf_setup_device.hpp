@@ -33,6 +33,7 @@ class DeviceSetup final : public SetupAccelerator {
     void end_level() override;
     bool seed(int level, const std::vector<int64_t> &starts, const std::vector<u32> &columns, std::vector<u32> &aggregate,
               std::vector<char> &seeded, u32 &count) override;
+    bool factor(int level, size_t n, double *a) override;
 
    private:
     struct Overflow {};

@@ -120,6 +120,43 @@ def test_inverse_coarse_solve_is_deterministic_and_within_rounding(ctx, precisio
     assert diff <= tol
 
 
+def test_the_explicit_coarse_inverse_is_the_fixed_order_inverse_of_the_factor(ctx):
+    # a one-level hierarchy (the whole matrix is the coarsest level): its cycle is the coarse solve
+    # alone, so with the explicit inverse it returns sum over k of inv[k][i] r[k] in ascending k,
+    # where linv (the columns of L^-1 by forward substitution, subtractions in ascending k) and
+    # inv[i][j] = sum over k >= max(i, j) of linv[k][i] linv[k][j] follow the device's order
+    ref = refcpu.Problem(loaded([2.0, 1.0, 1.0], [2, 1, 1]))
+    mg_ref = refcpu.Multigrid(ref, refcpu.options(precision=0, coarse_dofs=100000))
+    assert mg_ref.levels == 1
+    L = np.tril(mg_ref.coarse_factor())
+    n = L.shape[0]
+    linv = np.zeros((n, n))
+    for j in range(n):
+        for i in range(j, n):
+            v = 1.0 if i == j else 0.0
+            for k in range(j, i):
+                v -= L[i, k] * linv[k, j]
+            linv[i, j] = v / L[i, i]
+    inv = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            v = 0.0
+            for k in range(max(i, j), n):
+                v += linv[k, i] * linv[k, j]
+            inv[j, i] = v
+    r = np.random.default_rng(12).standard_normal(n)
+    expected = np.zeros(n)
+    for i in range(n):
+        v = 0.0
+        for k in range(n):
+            v += inv[k, i] * r[k]
+        expected[i] = v
+    _, mg = device_multigrid(ctx, ref, 0, 100000)
+    assert np.array_equal(bits(mg.apply(r)), bits(mg.solve_coarsest(r)))  # one level: the cycle is the solve
+    fast = mg.with_precision(sf.Precision.DOUBLE, coarse_solve=1)
+    assert np.array_equal(bits(fast.apply(r)), bits(expected))
+
+
 def test_device_coarse_solve_matches_the_factor(ctx):
     ref = refcpu.Problem(WORKLOADS["plate"])
     mg_ref = refcpu.Multigrid(ref, refcpu.options(precision=0, coarse_dofs=100))

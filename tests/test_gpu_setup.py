@@ -100,6 +100,25 @@ def test_the_device_seed_pass_on_a_larger_plate_is_the_serial_pass(ctx):
     assert_same_setup(device, host)
 
 
+def test_a_coarse_factor_of_hundreds_of_unknowns_is_factored_on_the_device_to_the_hosts_bits(ctx):
+    # a 10:1:1 bar of 36,300 free DOFs coarsens to a dense level of about 950 unknowns, whose
+    # Cholesky factor runs on the device column by column (the host's order): the factor, the
+    # whole hierarchy and the oracle's factor agree to the bit, also with the setup sliced
+    from solidfem_amd import Workload
+    w = Workload(size=(50.0, 5.0, 5.0), divisions=(50, 5, 5), clamp_axis=0, clamp_value=0.0, load_axis=0, load_value=50.0,
+                 traction_kind=1, traction_scale=1.0, traction_seed=1, youngs=70000.0, poisson=0.3)
+    ref = refcpu.Problem(w)
+    options = sf.MultigridOptions(precision=sf.Precision.SINGLE, coarse_dofs=1500)
+    auto = run_setup(ctx, ref, options)
+    n = auto.coarse_factor().shape[0]
+    assert 128 <= n <= 1500, n
+    host = run_setup(None, ref, options)
+    assert_same_setup(auto, host)
+    assert_same_setup(run_setup(ctx, ref, options, budget=977), host)
+    factor = refcpu.Multigrid(ref, refcpu.options(precision=2, coarse_dofs=1500)).coarse_factor()
+    assert np.array_equal(bits(np.tril(auto.coarse_factor())), bits(np.tril(factor)))
+
+
 def test_automatic_placement_on_a_larger_plate(ctx):
     # about 36,000 block rows: level 0 runs on the device, the coarse levels on the host
     ref = refcpu.Problem(plate_config(divisions=(40, 40, 2)))
