@@ -2,6 +2,7 @@
 // objects. No exception crosses this boundary; every failure becomes an sf_status with its
 // message and payload kept per thread.
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -50,12 +51,20 @@ sf_status guard(F &&f) {
     }
 }
 
-// SF_SETUP_AUTO takes levels of at least this many block rows to the device.
-constexpr size_t DEVICE_SETUP_ROWS = 4096;
+// SF_SETUP_AUTO takes levels of at least this many block rows to the device
+// (SOLIDFEM_SETUP_MIN_ROWS overrides it, read once).
+size_t device_setup_rows() {
+    static const size_t rows = [] {
+        const char *e = std::getenv("SOLIDFEM_SETUP_MIN_ROWS");
+        const long v = e ? std::atol(e) : 0;
+        return v > 0 ? (size_t)v : (size_t)4096;
+    }();
+    return rows;
+}
 
 void attach_device_setup(MultigridSetup &setup, const std::shared_ptr<Context> &ctx, const SetupOptions &o, const double *level0_values) {
     if (!ctx || o.setup_device == SF_SETUP_HOST) return;
-    setup.attach(std::make_unique<DeviceSetup>(ctx, o.setup_device == SF_SETUP_DEVICE ? 1 : DEVICE_SETUP_ROWS, level0_values,
+    setup.attach(std::make_unique<DeviceSetup>(ctx, o.setup_device == SF_SETUP_DEVICE ? 1 : device_setup_rows(), level0_values,
                                                o.setup_device == SF_SETUP_DEVICE));
 }
 
