@@ -54,6 +54,18 @@ int DeviceSliced<T>::group_runs() {
     return on;
 }
 template <typename T>
+bool DeviceBsr<T>::rows_ping_pong() {
+    // on unless SOLIDFEM_ROWS_PING_PONG=0: measured 0.6-1.1 ms faster per plate FP32 solve
+    static const bool on = [] {
+        const char *e = std::getenv("SOLIDFEM_ROWS_PING_PONG");
+        return !e || std::atoi(e) != 0;
+    }();
+    return on;
+}
+template struct DeviceBsr<double>;
+template struct DeviceBsr<float>;
+
+template <typename T>
 bool DeviceSliced<T>::ping_pong() {
     static const bool on = [] {
         const char *e = std::getenv("SOLIDFEM_PING_PONG");

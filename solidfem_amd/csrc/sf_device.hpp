@@ -69,7 +69,13 @@ struct DeviceBsr {
     uint32_t rows = 0;
     DeviceBuffer<uint32_t> starts, cols;
     DeviceBuffer<T> vals;
-    dev::Bsr<T> view() const { return {starts.ptr, cols.ptr, vals.ptr, rows}; }
+    mutable uint32_t passes = 0;  // launches so far, for the alternating traversal
+    // Every other pass takes each XCD's rows in reverse order (SOLIDFEM_ROWS_PING_PONG=0: never).
+    dev::Bsr<T> view() const {
+        const uint32_t reverse = rows_ping_pong() ? (passes++ & 1u) : 0u;
+        return {starts.ptr, cols.ptr, vals.ptr, rows, reverse};
+    }
+    static bool rows_ping_pong();
 };
 template <typename T>
 void upload_bsr(DeviceBsr<T> &out, const HostBsr<double> &m, hipStream_t s);

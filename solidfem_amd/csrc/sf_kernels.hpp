@@ -203,7 +203,17 @@ struct Bsr {
     const uint32_t *cols;
     const T *vals;
     uint32_t rows;
+    uint32_t reverse;  // nonzero: each XCD takes its workgroups' rows in reverse order (xcd_reversed_block)
 };
+
+// Workgroup b runs on XCD b mod 8. Each XCD takes the same rows as in dispatch order, in reverse:
+// block x + 8 l becomes x + 8 (c - 1 - l), c the XCD's count of blocks. A pass that follows a
+// forward pass over the same matrix then starts with the rows that XCD's L2 holds last.
+__device__ __forceinline__ uint32_t xcd_reversed_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x, x = b % 8u, l = b / 8u;
+    const uint32_t count = (nb - x + 7u) / 8u;
+    return x + 8u * (count - 1u - l);
+}
 
 // The longest row among a wavefront's RPW rows (each row's length is uniform over its lanes).
 template <int RPW>
@@ -245,7 +255,7 @@ __global__ __launch_bounds__(64) void long_rows_kernel(Bsr<S> a, const V *__rest
     __shared__ V terms[RPW][RB][LW * CB];
     __shared__ V values[RPW][RB];
     const int lane = threadIdx.x, g = lane / LW, sl = lane % LW;
-    const uint32_t row = blockIdx.x * RPW + (uint32_t)g;
+    const uint32_t row = (a.reverse ? xcd_reversed_block() : blockIdx.x) * RPW + (uint32_t)g;
     const bool live = row < a.rows;
     const uint32_t start = live ? a.starts[row] : 0u, len = live ? a.starts[row + 1] - start : 0u;
     const uint32_t longest = longest_of_wave<RPW>(len);
@@ -305,7 +315,7 @@ __global__ __launch_bounds__(64) void cheb_rows_kernel(Bsr<S> a, const S *__rest
     __shared__ V terms[RPW][RB][LW * RB];
     __shared__ V r_new[RPW][RB];
     const int lane = threadIdx.x, g = lane / LW, sl = lane % LW;
-    const uint32_t row = blockIdx.x * RPW + (uint32_t)g;
+    const uint32_t row = (a.reverse ? xcd_reversed_block() : blockIdx.x) * RPW + (uint32_t)g;
     const bool live = row < a.rows;
     const uint32_t start = live ? a.starts[row] : 0u, len = live ? a.starts[row + 1] - start : 0u;
     const uint32_t longest = longest_of_wave<RPW>(len);
