@@ -310,6 +310,14 @@ const sf_quad_mesh *sf_model_mesh(const sf_model *model);
  * elements [first, first+count): 900 values each, row-major. */
 sf_status sf_model_element_stiffness(sf_context *ctx, const sf_model *model, uint64_t first,
                                      uint64_t count, double *ke);
+/* assemble_stiffness, crates/solidfem-elastic/src/assembly.rs:17-27: the full stiffness with no
+ * node eliminated, three unknowns per node in node order, assembled on the device (the study's
+ * element kernels and scatter with every node free). It has the model's six rigid-body null
+ * vectors. The model's constraints and loads are not read. The cancel check is read before each
+ * chunk of 65536 elements (SF_ERR_MODEL_CANCELLED); an inverted element gives
+ * SF_ERR_MODEL_INVERTED_JACOBIAN. The matrix stays on the device: sf_csr_read reads it back. */
+sf_status sf_assemble_stiffness(sf_context *ctx, const sf_model *model, sf_cancel_fn cancel, void *user,
+                                sf_csr **out);
 
 /* The numbered problem of StudyBuilder::new (study.rs:436-449, assembly.rs:55-81) on the host:
  * free map (node -> free index or UINT32_MAX), restricted right-hand side, the free stiffness's
@@ -359,6 +367,11 @@ void sf_static_study_destroy(sf_static_study *study);
 sf_status sf_csr_create(sf_context *ctx, uint64_t size, const int64_t *row_starts,
                         const uint32_t *columns, const double *values, sf_csr **out);
 void sf_csr_destroy(sf_csr *csr);
+/* The matrix as scalar CSR (CsrMatrix's row_starts, columns and values, csr.rs:6-13): size,
+ * nonzeros, row_starts (size+1), columns and values (nonzeros); any pointer may be NULL. A matrix
+ * on the device is read from there. */
+sf_status sf_csr_read(const sf_csr *csr, uint64_t *size, uint64_t *nonzeros, int64_t *row_starts,
+                      uint32_t *columns, double *values);
 /* CsrMatrix::multiply, csr.rs:112-132, on the device: host x and y of `size` */
 sf_status sf_csr_multiply(sf_csr *csr, const double *x, double *y);
 /* CsrMatrix::multiply_rounding_bound, csr.rs:142-154, on the device */

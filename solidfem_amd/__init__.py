@@ -366,6 +366,17 @@ class Model(_Handle):
         _raise(library().sf_model_element_stiffness(ctx._ptr, self._ptr, first, count, _ptr(out, C.c_double)))
         return out
 
+    def assemble_stiffness(self, ctx: Context, cancel: Optional[Callable[[], bool]] = None) -> "CsrMatrix":
+        """assemble_stiffness (assembly.rs:17-27) on the device: the full stiffness with no node
+        eliminated, three unknowns per node in node order, as a device CsrMatrix."""
+        h = C.c_void_p()
+        _raise(library().sf_assemble_stiffness(ctx._ptr, self._ptr, _cancel(cancel), None, C.byref(h)))
+        m = CsrMatrix(h)
+        n = C.c_uint64()
+        _raise(library().sf_csr_read(h, C.byref(n), None, None, None, None))
+        m._ctx, m.size = ctx, n.value
+        return m
+
     def points_per_element(self) -> int:
         return library().sf_model_points_per_element(self._ptr)
 
@@ -607,6 +618,18 @@ class CsrMatrix(_Handle):
         m = cls(h)
         m._ctx, m.size = ctx, len(rs) - 1
         return m
+
+    def read(self):
+        """The matrix as scalar CSR (row_starts, columns, values), read back from the device."""
+        lib = library()
+        n, nnz = C.c_uint64(), C.c_uint64()
+        _raise(lib.sf_csr_read(self._ptr, C.byref(n), C.byref(nnz), None, None, None))
+        rs = np.empty(n.value + 1, dtype=np.int64)
+        cols = np.empty(nnz.value, dtype=np.uint32)
+        vals = np.empty(nnz.value)
+        _raise(lib.sf_csr_read(self._ptr, None, None, _ptr(rs, C.c_int64), _ptr(cols, C.c_uint32),
+                               _ptr(vals, C.c_double)))
+        return rs, cols, vals
 
     def multiply(self, x) -> np.ndarray:
         xx = np.ascontiguousarray(x, dtype=np.float64)

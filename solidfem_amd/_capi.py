@@ -183,6 +183,8 @@ SIGNATURES = {
     "sf_static_study_destroy": (None, [vp]),
     "sf_csr_create": (i32, [vp, u64, P(i64), P(u32), P(f64), P(vp)]),
     "sf_csr_destroy": (None, [vp]),
+    "sf_csr_read": (i32, [vp, P(u64), P(u64), P(i64), P(u32), P(f64)]),
+    "sf_assemble_stiffness": (i32, [vp, vp, CANCEL_FN, vp, P(vp)]),
     "sf_csr_multiply": (i32, [vp, P(f64), P(f64)]),
     "sf_csr_rounding_bound": (i32, [vp, P(f64), i32, P(f64)]),
     "sf_multigrid_setup_create": (i32, [vp, vp, P(f64), u64, P(f64), u64, P(MultigridOptions), P(vp)]),

@@ -362,6 +362,44 @@ sf_status sf_model_element_stiffness(sf_context *ctx, const sf_model *model, uin
     });
 }
 
+sf_status sf_assemble_stiffness(sf_context *ctx, const sf_model *model, sf_cancel_fn cancel_fn, void *user, sf_csr **out) {
+    return guard([&] {
+        // assemble_stiffness, assembly.rs:17-27: assemble_free with the identity free map
+        const Cancel cancel{cancel_fn, user};
+        const QuadMesh &mesh = model->model.mesh;
+        const size_t nodes = mesh.nodes.size(), total = mesh.tets.size();
+        std::vector<u32> free(nodes);
+        for (size_t n = 0; n < nodes; ++n) free[n] = (u32)n;
+        const AssemblyPattern pattern = build_pattern(mesh, free, nodes);
+        auto c = std::make_unique<sf_csr>();
+        c->ctx = ctx->ctx;
+        auto &sys = c->system;
+        sys.size = 3 * nodes;
+        sys.rb = 3;
+        sys.host = pattern.pattern;
+        hipStream_t s = ctx->ctx->stream;
+        const size_t blocks = sys.host.blocks();
+        sys.values.resize(std::max<size_t>(blocks * 9, 1));
+        SF_HIP(hipMemsetAsync(sys.values.ptr, 0, blocks * 9 * sizeof(double), s));
+        DeviceAssembly assembly;
+        assembly.prepare(model->model, pattern, s);
+        constexpr size_t POLL = 1u << 16;
+        for (size_t next = 0; next < total;) {
+            if (cancel()) {
+                SF_HIP(hipStreamSynchronize(s));
+                throw Error(SF_ERR_MODEL_CANCELLED, "the stiffness assembly was cancelled");
+            }
+            const size_t stop = std::min(total, next + POLL);
+            assembly.add_elements(next, stop, sys.values.ptr, blocks, s);
+            next = stop;
+        }
+        SF_HIP(hipStreamSynchronize(s));
+        assembly.check_fault(s);
+        if (nodes) sys.build_from_device_values(s);
+        *out = c.release();
+    });
+}
+
 // ---- study -----------------------------------------------------------------------------------
 
 sf_status sf_study_builder_create(sf_context *ctx, const sf_model *model, const sf_pcg_options *pcg, int32_t use_multigrid,
@@ -577,6 +615,25 @@ sf_status sf_static_study_matrix(const sf_static_study *study, uint64_t *size, u
         SF_HIP(hipStreamSynchronize(study->ctx->stream));
         if (size) *size = sys.size;
         if (nonzeros) *nonzeros = h.blocks() * 9;
+        to_scalar_csr(h, row_starts, columns, values);
+    });
+}
+sf_status sf_csr_read(const sf_csr *csr, uint64_t *size, uint64_t *nonzeros, int64_t *row_starts, uint32_t *columns,
+                      double *values) {
+    return guard([&] {
+        const auto &sys = csr->system;
+        const size_t entries = sys.host.blocks() * sys.host.rb * sys.host.cb;
+        if (size) *size = sys.size;
+        if (nonzeros) *nonzeros = entries;
+        if (!row_starts && !columns && !values) return;
+        if (!values || !csr->ctx) {
+            to_scalar_csr(sys.host, row_starts, columns, values);
+            return;
+        }
+        HostBsr<double> h = sys.host;
+        h.values.resize(entries);
+        if (entries) sys.values.download(h.values.data(), entries, csr->ctx->stream);
+        SF_HIP(hipStreamSynchronize(csr->ctx->stream));
         to_scalar_csr(h, row_starts, columns, values);
     });
 }

@@ -55,6 +55,63 @@ def test_device_element_stiffness_is_bit_identical(ctx, points):
         assert np.array_equal(bits(ke[e]), bits(ref.element_stiffness(e))), e
 
 
+@pytest.mark.parametrize("name", ["jittered-plate", "random-block", "shell-core-torsion", "tube"])
+def test_full_stiffness_is_the_oracles_with_no_node_fixed(ctx, name):
+    """assemble_stiffness (assembly.rs:17-27): the model's clamp and loads are not read, so the
+    oracle's assembly of the same mesh with no clamp is the answer, bit for bit."""
+    w = WORKLOADS[name]
+    ref = refcpu.Problem(sf.Workload(**{**w.__dict__, "clamp_axis": -1}))
+    assert ref.free_dofs == 3 * ref.node_count
+    m = sf.Model.from_workload(w).assemble_stiffness(ctx)
+    assert m.size == 3 * ref.node_count
+    rs, cols, vals = m.read()
+    r_rs, r_cols, r_vals = ref.csr()
+    assert np.array_equal(rs, r_rs) and np.array_equal(cols, r_cols)
+    assert np.array_equal(bits(vals), bits(r_vals))
+    x = np.random.default_rng(3).standard_normal(m.size)
+    assert np.array_equal(bits(m.multiply(x)), bits(refcpu.csr_multiply(r_rs, r_cols, r_vals, x)))
+
+
+def test_full_stiffness_passes_the_reference_patch_test(ctx):
+    """crates/solidfem-elastic/tests/element/patch.rs:17-118 on the device assembly: a 3^3 box of
+    side 3 with its interior vertices moved by 0.15 h sin(...), E 2100, nu 0.3; a linear field is
+    balanced at every interior node to 1e-12 of max|diag| * max|u|, and the six rigid-body modes
+    are null vectors to the same bound."""
+    box = sf.box_tet_mesh((0.0, 0.0, 0.0), (3.0, 3.0, 3.0), (3, 3, 3))
+    p = box.positions().copy()
+    jitter = 0.15 * 1.0
+    inner = ~np.any((p == 0.0) | (p == 3.0), axis=1)
+    q = p[inner]
+    p[inner] = q + jitter * np.sin(np.stack([13 * q[:, 0] + 7 * q[:, 1] + 3 * q[:, 2],
+                                             5 * q[:, 0] + 11 * q[:, 1] + 2 * q[:, 2],
+                                             3 * q[:, 0] + 4 * q[:, 1] + 17 * q[:, 2]], axis=1))
+    mesh = sf.QuadraticTetMesh.from_linear(sf.TetMesh.new(p, box.tets()))
+    x = mesh.nodes()
+    m = sf.Model.new(mesh, 2100.0, 0.3).assemble_stiffness(ctx)
+    rs, cols, vals = m.read()
+    diag = max(abs(vals[rs[i]:rs[i + 1]][cols[rs[i]:rs[i + 1]] == i][0]) for i in range(m.size))
+    gradient = np.array([[0.01, 0.002, -0.003], [0.004, -0.005, 0.006], [0.007, 0.001, 0.008]])
+    u = np.array([0.3, -0.2, 0.1]) + x @ gradient.T
+    f = m.multiply(u.ravel()).reshape(-1, 3)
+    interior = np.all((x > 1e-9) & (x < 3.0 - 1e-9), axis=1)
+    assert interior.sum() > 0
+    assert np.abs(f[interior]).max() < 1e-12 * diag * np.abs(u).max()
+    for k in range(6):
+        r = np.zeros_like(x)
+        if k < 3:
+            r[:, k] = 1.0
+        else:
+            r = np.cross(np.eye(3)[k - 3], x)
+        assert np.abs(m.multiply(r.ravel())).max() < 1e-12 * diag * np.abs(r).max(), k
+
+
+def test_full_stiffness_assembly_is_cancelled_by_name(ctx):
+    model = sf.Model.from_workload(WORKLOADS["random-block"])
+    with pytest.raises(sf.ModelError) as e:
+        model.assemble_stiffness(ctx, cancel=lambda: True)
+    assert e.value.status == 47  # SF_ERR_MODEL_CANCELLED
+
+
 def test_device_product_and_rounding_bound_are_bit_identical(ctx):
     ref = refcpu.Problem(plate_config(divisions=(12, 12, 2)))
     rs, cols, vals = ref.csr()
