@@ -505,6 +505,19 @@ def main():
                                "nodes": int(nodal.shape[0]), "peak_nodal_von_mises": float(nodal[:, 6].max()),
                                "note": "host incidence lists, uploads and downloads included"}
     log(f"stress recovery of the plate: {recovery_ms:.1f} ms, peak nodal von Mises {nodal[:, 6].max():.4g} MPa")
+    # the error estimate of that recovery (error_estimate.rs): 14-point integrals per element on the
+    # device, the sums in element order on the host
+    recovery = model.begin_recovery(ctx, nodal_u, converged=True)
+    recovery.advance(2 ** 62)
+    model.begin_error_estimate(recovery).advance(2 ** 62)  # warm
+    t = time.perf_counter()
+    estimation = model.begin_error_estimate(recovery)
+    estimation.advance(2 ** 62)
+    estimate = estimation.estimate()
+    estimate_ms = (time.perf_counter() - t) * 1e3
+    line["error_estimate"] = {"ms": round(estimate_ms, 2), "relative_error": estimate.relative_error,
+                              "note": "a finished recovery in; the mesh upload and the element errors' download included"}
+    log(f"error estimate of the plate: {estimate_ms:.1f} ms, eta {estimate.relative_error:.4g}")
     if args.columns:
         line["multi_rhs"] = multi_rhs(ctx, sf, np, args.columns, args.steps, args.warmup)
     if args.serve_requests:
@@ -521,6 +534,13 @@ def main():
         t = time.perf_counter()
         problem.recover(nodal_u)
         line["stress_recovery"]["oracle_serial_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        t = time.perf_counter()
+        oracle_estimate = problem.error_estimate(nodal_u)
+        line["error_estimate"]["oracle_serial_ms_with_recovery"] = round((time.perf_counter() - t) * 1e3, 1)
+        line["error_estimate"]["oracle_bits_equal"] = bool(
+            oracle_estimate["status"] == 0
+            and np.array_equal(oracle_estimate["element_errors"].view(np.uint64), estimate.element_errors.view(np.uint64))
+            and np.float64(oracle_estimate["relative_error"]).view(np.uint64) == np.float64(estimate.relative_error).view(np.uint64))
     print(json.dumps(line), flush=True)
 
 

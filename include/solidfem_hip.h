@@ -74,6 +74,11 @@ enum {
     SF_ERR_MODEL_PRESCRIBED_ON_FIXED = 50, /* a=node */
     SF_ERR_MODEL_BODY_NOT_HELD = 51,       /* a=first tet, b=held nodes, value=SF_FREEDOM_* left */
     SF_ERR_MODEL_DOFS_DO_NOT_HOLD = 52,    /* a=first tet, b=rigid motions held (rank < 6) */
+    SF_ERR_MODEL_MIXED_MATERIAL_PATCH = 53,/* a=vertex */
+    SF_ERR_MODEL_COPLANAR_PATCH = 54,      /* a=vertex */
+    SF_ERR_MODEL_ZERO_STRAIN_ENERGY = 55,
+    SF_ERR_MODEL_NON_FINITE_ENERGY = 56,
+    SF_ERR_MODEL_RECOVERY_MESH_MISMATCH = 57,
     /* serving */
     SF_ERR_SERVE_OVER_BUDGET = 60,         /* a=bytes needed, b=budget */
     SF_ERR_SERVE_UNKNOWN_REQUEST = 61
@@ -543,6 +548,49 @@ sf_status sf_recovery_displacements(const sf_recovery *recovery, double *out);
  * (StaticStudy::finish_recovery, study.rs:294-300). */
 sf_status sf_recovery_field(sf_recovery *recovery, double *point_values, double *nodal_values);
 void sf_recovery_destroy(sf_recovery *recovery);
+
+/* ---- error estimate (crates/solidfem-elastic/src/error_estimate.rs) --------------------------
+ * ErrorEstimation: how converged a finished recovery's solution is -- per element the energy norm
+ * e_K of the recovered minus the element stress, integrated with the 14-point rule whatever rule
+ * the solve used, and over the model ||e||^2 = sum e_K^2, ||u_h||^2 (twice the strain energy) and
+ * eta = ||e|| / sqrt(||u_h||^2 + ||e||^2). The element integrals run on the device; the sums are
+ * kept in element order, so any slicing gives one run's bits. The estimation borrows the recovery
+ * (ErrorEstimation holds it, error_estimate.rs:80-89): destroy the estimation first. */
+typedef struct sf_error_estimation sf_error_estimation;
+typedef struct sf_error_estimate {
+    double error_norm_squared;    /* ||e||^2 */
+    double solution_norm_squared; /* ||u_h||^2 */
+    double relative_error;        /* eta, in [0, 1) */
+    uint64_t elements;            /* element_errors' length */
+} sf_error_estimate;
+/* StaticStudy::begin_error_estimate (study.rs:329-333): an estimate of `recovery` over the study's
+ * mesh and material, none of it done. */
+sf_status sf_static_study_begin_error_estimate(const sf_static_study *study, sf_recovery *recovery,
+                                               sf_error_estimation **out);
+/* The same over a model's mesh and material. */
+sf_status sf_error_estimation_create(const sf_model *model, sf_recovery *recovery, sf_error_estimation **out);
+/* ErrorEstimation::advance (error_estimate.rs:121-151): up to max_elements more elements in order,
+ * `cancel` (may be NULL) read before each (SF_ERR_MODEL_CANCELLED, the elements before it done).
+ * Refuses a recovery of another mesh (SF_ERR_MODEL_RECOVERY_MESH_MISMATCH) and an unfinished or
+ * preview recovery (SF_ERR_MODEL_NOT_FINISHED). An element whose corner has no patch fit gives
+ * SF_ERR_MODEL_MIXED_MATERIAL_PATCH or SF_ERR_MODEL_COPLANAR_PATCH (a = vertex); an inverted point
+ * SF_ERR_MODEL_INVERTED_JACOBIAN, an invalid material SF_ERR_MODEL_INVALID_MATERIAL (a = element,
+ * b = point of the 14-point rule); the elements before it are done. `finished` may be NULL. */
+sf_status sf_error_estimation_advance(sf_error_estimation *estimation, uint64_t max_elements, sf_cancel_fn cancel,
+                                      void *user, int32_t *finished);
+/* StaticStudy::continue_error_estimate (study.rs:335-352): sf_error_estimation_advance, its time
+ * added to the study's recovery_seconds. */
+sf_status sf_static_study_continue_error_estimate(sf_static_study *study, sf_error_estimation *estimation,
+                                                  uint64_t max_elements, sf_cancel_fn cancel, void *user,
+                                                  int32_t *finished);
+/* ErrorEstimation::elements_done */
+uint64_t sf_error_estimation_elements_done(const sf_error_estimation *estimation);
+/* ErrorEstimation::estimate (error_estimate.rs:153-171), StaticStudy::finish_error_estimate:
+ * element_errors (one per element, may be NULL) and the sums. SF_ERR_MODEL_NOT_FINISHED with
+ * elements left, SF_ERR_MODEL_NON_FINITE_ENERGY, SF_ERR_MODEL_ZERO_STRAIN_ENERGY. */
+sf_status sf_error_estimation_finish(const sf_error_estimation *estimation, sf_error_estimate *out,
+                                     double *element_errors);
+void sf_error_estimation_destroy(sf_error_estimation *estimation);
 
 /* ---- peak stress (results.rs:149-226) ---------------------------------------------------------
  * The peak von Mises over the positions `inside` accepts (NULL: everywhere): the mesh nodes in

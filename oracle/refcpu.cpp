@@ -640,6 +640,26 @@ int ref_recover(const ref_problem *p, const double *u, double *point_values, dou
         }
     return 0;
 }
+int ref_error_estimate(const ref_problem *p, const double *u, double *element_errors, double *sums, int64_t *payload) {
+    const Problem &pr = *p->p;
+    std::vector<double> disp(u, u + 3 * pr.mesh.nodes.size());
+    const RecoveredField f = recover(pr.mesh, pr.material, pr.quadrature, disp);
+    if (f.fault_tet >= 0) {
+        payload[0] = f.fault_tet;
+        payload[1] = f.fault_point;
+        payload[2] = 0;
+        return ESTIMATE_INVERTED;
+    }
+    const ErrorEstimate e = estimate_error(pr.mesh, pr.material, disp, f);
+    std::copy(e.element_errors.begin(), e.element_errors.end(), element_errors);
+    sums[0] = e.error_sum;
+    sums[1] = e.energy_sum;
+    sums[2] = e.relative_error;
+    payload[0] = e.a;
+    payload[1] = e.b;
+    payload[2] = (int64_t)e.element_errors.size();
+    return e.status;
+}
 int ref_problem_quadrature(const ref_problem *p) { return p->p->quadrature; }
 // peak_with_holder (results.rs:184-226) over the candidates the masks keep (NULL: all). Returns 1
 // with the peak, 0 when no candidate is inside.

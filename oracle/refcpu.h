@@ -132,6 +132,12 @@ void ref_problem_rigid_modes(const ref_problem *p, double *out); /* 6 per free d
 int ref_problem_element_stiffness(const ref_problem *p, int64_t tet, double *ke);
 int ref_recover(const ref_problem *p, const double *u, double *point_values, double *nodal_values, int64_t *fault_tet);
 int ref_problem_quadrature(const ref_problem *p);
+/* The error estimate of a converged solve's displacements u (3 per node): recovery, then
+ * ErrorEstimation over every element (error_estimate.rs:121-171). element_errors (one per element
+ * done), sums = {error_norm_squared, solution_norm_squared, relative_error}, payload = {a, b,
+ * elements done}. Returns 0, or the refusal as the product's status code (45 inverted, 46 material,
+ * 53 mixed-material patch, 54 coplanar patch, 55 zero strain energy, 56 non-finite). */
+int ref_error_estimate(const ref_problem *p, const double *u, double *element_errors, double *sums, int64_t *payload);
 /* peak_with_holder over the nodes, then the points, the masks keep (NULL: all); 1 with the peak,
  * 0 when none is inside */
 int ref_peak(const double *positions, uint64_t nodes, const double *nodal_vm, uint64_t stride, const double *points, uint64_t count,

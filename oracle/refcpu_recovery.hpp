@@ -153,7 +153,10 @@ struct RecoveredField {
     std::vector<double> nodal_stress;    // 6 per node
     std::vector<double> nodal_von_mises;
     int64_t fault_tet = -1, fault_point = -1;  // an inverted element, when the recovery failed
+    std::vector<PatchFit> fits;                // per vertex, meaningful where fit_state is FITTED
+    std::vector<char> fit_state;               // per vertex: UNFITTED (flat), FITTED or MIXED
 };
+enum : char { UNFITTED = 0, FITTED = 1, MIXED = 2 };
 
 // Recovery::advance and Recovery::into_field (results.rs:376-536) for nodal displacements `u`
 // (3 per node); the material comes from `field` at each point.
@@ -240,12 +243,16 @@ inline RecoveredField recover(const QuadMesh &mesh, const Field &field, int rule
             }
         }
         if (uniform) fitted[v] = PatchFit::fit(mesh.nodes[v].v, samples, fits[v]) ? 1 : 0;
+        else fitted[v] = 2;
     }
+    out.fits = fits;
+    out.fit_state.resize(vertices);
+    for (size_t v = 0; v < vertices; ++v) out.fit_state[v] = fitted[v] == 1 ? FITTED : fitted[v] == 2 ? MIXED : UNFITTED;
     // into_field, results.rs:376-411
     out.nodal_stress.assign(6 * nodes, 0.0);
     for (size_t v = 0; v < vertices; ++v) {
         double s[6];
-        if (fitted[v]) fits[v].at(mesh.nodes[v].v, s);
+        if (fitted[v] == 1) fits[v].at(mesh.nodes[v].v, s);
         else
             for (int c = 0; c < 6; ++c) s[c] = worst_stress[v][c];
         for (int c = 0; c < 6; ++c) out.nodal_stress[6 * v + c] = s[c];
@@ -258,7 +265,7 @@ inline RecoveredField recover(const QuadMesh &mesh, const Field &field, int rule
             done[midside - vertices] = 1;
             const u32 a = tet[EDGE_CORNERS[slot][0]], b = tet[EDGE_CORNERS[slot][1]];
             double s[6];
-            if (fitted[a] && fitted[b]) {
+            if (fitted[a] == 1 && fitted[b] == 1) {
                 // results.rs:544-552
                 double p[6], q[6];
                 fits[a].at(mesh.nodes[midside].v, p);
@@ -271,6 +278,111 @@ inline RecoveredField recover(const QuadMesh &mesh, const Field &field, int rule
         }
     out.nodal_von_mises.resize(nodes);
     for (size_t n = 0; n < nodes; ++n) out.nodal_von_mises[n] = von_mises(&out.nodal_stress[6 * n]);
+    return out;
+}
+
+// The refusals of an error estimate, numbered as the product's status codes.
+enum EstimateStatus {
+    ESTIMATE_OK = 0,
+    ESTIMATE_INVERTED = 45,     // a = element, b = point of the 14-point rule
+    ESTIMATE_MATERIAL = 46,     // a = element, b = point
+    ESTIMATE_MIXED = 53,        // a = vertex
+    ESTIMATE_COPLANAR = 54,     // a = vertex
+    ESTIMATE_ZERO_ENERGY = 55,
+    ESTIMATE_NON_FINITE = 56,
+};
+
+struct ErrorEstimate {
+    std::vector<double> element_errors;  // e_K, in element order
+    double error_sum = 0.0, energy_sum = 0.0, relative_error = 0.0;
+    int status = ESTIMATE_OK;
+    int64_t a = -1, b = -1;
+};
+
+// compliance_energy, error_estimate.rs:240-248: s : C^-1 : s for an isotropic material
+inline double compliance_energy(const Material &m, const double s[6]) {
+    const double nu = m.nu;
+    const double normal = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const double coupling = s[0] * s[1] + s[1] * s[2] + s[2] * s[0];
+    const double shear = s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
+    return (normal - 2.0 * nu * coupling + 2.0 * (1.0 + nu) * shear) / m.e;
+}
+
+// ErrorEstimation::advance over every element, then ErrorEstimation::estimate
+// (error_estimate.rs:121-171): per element, element_integrals (:174-205) with the 14-point rule
+// whatever rule the solve used, the recovered stress at the element's nodes from the patch fits
+// (recovered_at_nodes, :207-231: a corner's fit at the corner, a midside the mean of its edge's
+// two fits at the midside), and the running sums in element order.
+template <typename Field>
+inline ErrorEstimate estimate_error(const QuadMesh &mesh, const Field &field, const std::vector<double> &u,
+                                    const RecoveredField &r) {
+    ErrorEstimate out;
+    for (size_t t = 0; t < mesh.tets.size(); ++t) {
+        const auto &tet = mesh.tets[t];
+        for (int corner = 0; corner < 4; ++corner) {
+            const u32 v = tet[corner];
+            if (r.fit_state[v] != FITTED) {
+                out.status = r.fit_state[v] == MIXED ? ESTIMATE_MIXED : ESTIMATE_COPLANAR;
+                out.a = v;
+                return out;
+            }
+        }
+        double recovered[10][6];
+        for (int corner = 0; corner < 4; ++corner) r.fits[tet[corner]].at(mesh.nodes[tet[corner]].v, recovered[corner]);
+        for (int slot = 0; slot < 6; ++slot) {
+            const double *at = mesh.nodes[tet[4 + slot]].v;
+            double p[6], q[6];
+            r.fits[tet[EDGE_CORNERS[slot][0]]].at(at, p);
+            r.fits[tet[EDGE_CORNERS[slot][1]]].at(at, q);
+            for (int k = 0; k < 6; ++k) recovered[4 + slot][k] = 0.5 * (p[k] + q[k]);
+        }
+        Vec3 en[10];
+        double ue[10][3];
+        for (int a = 0; a < 10; ++a) {
+            en[a] = mesh.nodes[tet[a]];
+            for (int i = 0; i < 3; ++i) ue[a][i] = u[3 * (size_t)tet[a] + i];
+        }
+        double error = 0.0, energy = 0.0;
+        for (int q = 0; q < 14; ++q) {
+            const QPoint &point = FOURTEEN_POINT[q];
+            PointGeometry g;
+            if (!point_geometry(en, point.natural, g)) {
+                out.status = ESTIMATE_INVERTED;
+                out.a = (int64_t)t;
+                out.b = q;
+                return out;
+            }
+            const Material m = field.at(g.position);
+            if (!m.valid()) {
+                out.status = ESTIMATE_MATERIAL;
+                out.a = (int64_t)t;
+                out.b = q;
+                return out;
+            }
+            double e[6], stress[6], shape[10], difference[6] = {0, 0, 0, 0, 0, 0};
+            strain_at(g, ue, e);
+            stress_at(m, e, stress);
+            shape_values(point.natural, shape);
+            for (int a = 0; a < 10; ++a)
+                for (int k = 0; k < 6; ++k) difference[k] += shape[a] * recovered[a][k];
+            for (int k = 0; k < 6; ++k) difference[k] -= stress[k];
+            const double measure = point.weight * g.det_j / 6.0;
+            error += measure * compliance_energy(m, difference);
+            energy += measure * compliance_energy(m, stress);
+        }
+        out.element_errors.push_back(std::sqrt(error));
+        out.error_sum += error;
+        out.energy_sum += energy;
+    }
+    if (!std::isfinite(out.energy_sum) || !std::isfinite(out.error_sum)) {
+        out.status = ESTIMATE_NON_FINITE;
+        return out;
+    }
+    if (out.energy_sum <= 0.0) {
+        out.status = ESTIMATE_ZERO_ENERGY;
+        return out;
+    }
+    out.relative_error = std::sqrt(out.error_sum / (out.energy_sum + out.error_sum));
     return out;
 }
 

@@ -388,6 +388,14 @@ class Model(_Handle):
         _raise(library().sf_recovery_create(ctx._ptr, self._ptr, _ptr(u, C.c_double), 1 if converged else 0, C.byref(h)))
         return Recovery(h)
 
+    def begin_error_estimate(self, recovery: "Recovery") -> "ErrorEstimation":
+        """An error estimate of `recovery` over this model's mesh and material, none of it done."""
+        h = C.c_void_p()
+        _raise(library().sf_error_estimation_create(self._ptr, recovery._ptr, C.byref(h)))
+        e = ErrorEstimation(h)
+        e._recovery = recovery
+        return e
+
     def nodal_rotations(self, ctx: "Context", displacements) -> np.ndarray:
         """nodal_rotations (results.rs:262-305): the local rotation, half the curl of the
         displacement, in radians at every mesh node, (nodes, 3)."""
@@ -546,6 +554,37 @@ class StaticStudy(_Handle):
         """StaticStudy::finish_recovery (study.rs:294-300): the finished field; refuses one with
         units left (ModelError NOT_FINISHED)."""
         return recovery.field()
+
+    def begin_error_estimate(self, recovery: "Recovery") -> "ErrorEstimation":
+        """StaticStudy::begin_error_estimate (study.rs:329-333)."""
+        h = C.c_void_p()
+        _raise(library().sf_static_study_begin_error_estimate(self._ptr, recovery._ptr, C.byref(h)))
+        e = ErrorEstimation(h)
+        e._recovery = recovery
+        return e
+
+    def continue_error_estimate(self, estimation: "ErrorEstimation", max_elements: int, cancel=None) -> bool:
+        """StaticStudy::continue_error_estimate (study.rs:335-352): True once every element is done;
+        the time joins the report's recovery_seconds."""
+        done = C.c_int32()
+        _raise(library().sf_static_study_continue_error_estimate(self._ptr, estimation._ptr, int(max_elements),
+                                                                 _cancel(cancel), None, C.byref(done)))
+        return bool(done.value)
+
+    def finish_error_estimate(self, estimation: "ErrorEstimation") -> ErrorEstimate:
+        """StaticStudy::finish_error_estimate (study.rs:354-360)."""
+        return estimation.estimate()
+
+    def solution_with_error_estimate(self, cancel=None):
+        """StaticStudy::solution_with_error_estimate (study.rs:304-327): refuses until the solve has
+        converged (a preview's estimate is refused NOT_FINISHED); ((point_values, nodal_values),
+        ErrorEstimate)."""
+        recovery = self.begin_recovery()
+        self.continue_recovery(recovery, 2 ** 63 - 1, cancel)
+        estimation = self.begin_error_estimate(recovery)
+        self.continue_error_estimate(estimation, 2 ** 63 - 1, cancel)
+        estimate = self.finish_error_estimate(estimation)
+        return self.finish_recovery(estimation.into_recovery()), estimate
 
     def rhs(self) -> np.ndarray:
         out = np.empty(self.report().free_dofs)
@@ -1139,6 +1178,41 @@ class Recovery(_Handle):
         nodal = np.empty((st.nodes, 7))
         _raise(library().sf_recovery_field(self._ptr, _ptr(points, C.c_double), _ptr(nodal, C.c_double)))
         return points, nodal
+
+
+@dataclass
+class ErrorEstimate:
+    """ErrorEstimate (error_estimate.rs:62-78)."""
+    element_errors: np.ndarray
+    error_norm_squared: float
+    solution_norm_squared: float
+    relative_error: float
+
+
+class ErrorEstimation(_Handle):
+    """A resumable error estimate of a finished recovery, on the device (ErrorEstimation,
+    error_estimate.rs:80-238): one element per unit in element order; any slicing gives one run's
+    bits. It borrows the recovery, which it keeps alive."""
+    _destroy = "sf_error_estimation_destroy"
+
+    def advance(self, max_elements: int, cancel=None) -> bool:
+        done = C.c_int32()
+        _raise(library().sf_error_estimation_advance(self._ptr, int(max_elements), _cancel(cancel), None, C.byref(done)))
+        return bool(done.value)
+
+    def elements_done(self) -> int:
+        return library().sf_error_estimation_elements_done(self._ptr)
+
+    def estimate(self) -> ErrorEstimate:
+        """ErrorEstimation::estimate; refuses one with elements left (ModelError NOT_FINISHED)."""
+        out = _capi.ErrorEstimate()
+        _raise(library().sf_error_estimation_finish(self._ptr, C.byref(out), None))
+        errors = np.empty(out.elements)
+        _raise(library().sf_error_estimation_finish(self._ptr, None, _ptr(errors, C.c_double)))
+        return ErrorEstimate(errors, out.error_norm_squared, out.solution_norm_squared, out.relative_error)
+
+    def into_recovery(self) -> "Recovery":
+        return self._recovery
 
 
 @dataclass

@@ -89,6 +89,10 @@ class Report(C.Structure):
     ]
 
 
+class ErrorEstimate(C.Structure):
+    _fields_ = [("error_norm_squared", f64), ("solution_norm_squared", f64), ("relative_error", f64), ("elements", u64)]
+
+
 class RecoveryState(C.Structure):
     _fields_ = [("elements", u64), ("elements_done", u64), ("vertices", u64), ("fits_done", u64), ("nodes", u64),
                 ("points_per_element", u64), ("converged", i32), ("finished", i32)]
@@ -238,6 +242,13 @@ SIGNATURES = {
     "sf_recovery_create": (i32, [vp, vp, P(f64), i32, P(vp)]),
     "sf_static_study_begin_recovery": (i32, [vp, P(vp)]),
     "sf_recovery_advance": (i32, [vp, u64, CANCEL_FN, vp, P(i32)]),
+    "sf_error_estimation_create": (i32, [vp, vp, P(vp)]),
+    "sf_static_study_begin_error_estimate": (i32, [vp, vp, P(vp)]),
+    "sf_error_estimation_advance": (i32, [vp, u64, CANCEL_FN, vp, P(i32)]),
+    "sf_static_study_continue_error_estimate": (i32, [vp, vp, u64, CANCEL_FN, vp, P(i32)]),
+    "sf_error_estimation_elements_done": (u64, [vp]),
+    "sf_error_estimation_finish": (i32, [vp, P(ErrorEstimate), P(f64)]),
+    "sf_error_estimation_destroy": (None, [vp]),
     "sf_static_study_continue_recovery": (i32, [vp, vp, u64, CANCEL_FN, vp, P(i32)]),
     "sf_recovery_state_of": (i32, [vp, P(RecoveryState)]),
     "sf_recovery_displacements": (i32, [vp, P(f64)]),

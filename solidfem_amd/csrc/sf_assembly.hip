@@ -257,6 +257,7 @@ __device__ bool thread_geometry(const double xe[10][3], const double nat[3], dou
     cof[7] = j[1] * j[6] - j[0] * j[7];
     cof[8] = j[0] * j[4] - j[1] * j[3];
     const double det = j[0] * cof[0] + j[1] * cof[3] + j[2] * cof[6];
+    if (det_out) *det_out = det;
     if (isnan(det) || det <= 0.0) return false;
     for (int a = 0; a < 10; ++a)
         for (int k = 0; k < 3; ++k) {
@@ -264,7 +265,6 @@ __device__ bool thread_geometry(const double xe[10][3], const double nat[3], dou
                                cof[k * 3 + 2] * shape_derivative(a, 2, l);
             grad[a][k] = sum / det;
         }
-    if (det_out) *det_out = det;
     return true;
 }
 
@@ -400,8 +400,8 @@ __global__ void recovery_worst_kernel(size_t nodes, const uint32_t *__restrict__
 }
 
 // PatchFit::fit (patch.rs:35-119) at each vertex over its incident elements' points in element
-// order; a patch of more than one material is left unfitted. fit: flag, centre 3, scale,
-// coefficients 4 x 6 (29 values).
+// order; a patch of more than one material is left unfitted. fit: flag (1 fitted, 0 flat, -1 more
+// than one material), centre 3, scale, coefficients 4 x 6 (29 values).
 template <int NQ>
 __global__ void recovery_fit_kernel(size_t first, size_t count, const double *__restrict__ nodes, const uint32_t *__restrict__ starts,
                                     const uint32_t *__restrict__ elems, const double *__restrict__ points,
@@ -420,8 +420,10 @@ __global__ void recovery_fit_kernel(size_t first, size_t count, const double *__
         for (int q = 0; q < NQ; ++q) {
             const size_t p = (size_t)elems[k] * NQ + q;
             if (__double_as_longlong(material[p * 2]) != __double_as_longlong(e0) ||
-                __double_as_longlong(material[p * 2 + 1]) != __double_as_longlong(nu0))
-                return;  // mixed materials
+                __double_as_longlong(material[p * 2 + 1]) != __double_as_longlong(nu0)) {
+                f[0] = -1.0;  // mixed materials
+                return;
+            }
             for (int a = 0; a < 3; ++a) scale = fmax(scale, fabs(points[p * 10 + a] - c[a]));
         }
     if (scale == 0.0) return;
@@ -488,12 +490,12 @@ __global__ void recovery_nodal_kernel(size_t nodes, size_t vertices, const doubl
     double s[6];
     if (n < vertices) {
         const double *f = fit + n * 29;
-        if (f[0] != 0.0) fit_at(f, p, s);
+        if (f[0] > 0.0) fit_at(f, p, s);
         else
             for (int c = 0; c < 6; ++c) s[c] = worst[n * 6 + c];
     } else {
         const double *fa = fit + (size_t)ends[(n - vertices) * 2] * 29, *fb = fit + (size_t)ends[(n - vertices) * 2 + 1] * 29;
-        if (fa[0] != 0.0 && fb[0] != 0.0) {
+        if (fa[0] > 0.0 && fb[0] > 0.0) {
             double a[6], b[6];
             fit_at(fa, p, a);
             fit_at(fb, p, b);
@@ -506,6 +508,110 @@ __global__ void recovery_nodal_kernel(size_t nodes, size_t vertices, const doubl
     double *o = out + n * 7;
     for (int k = 0; k < 6; ++k) o[k] = s[k];
     o[6] = sqrt(0.5 * (a * a + b * b + c * c) + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+}
+
+// ---- error estimate (crates/solidfem-elastic/src/error_estimate.rs) ----------------------------
+
+// An element's first refusal is reported as element * 32 + what: 0-3 the corner whose patch has no
+// fit, 4 + 2q an inverted point q, 5 + 2q an invalid material at point q -- the order of
+// element_integrals (error_estimate.rs:174-205): the fits first, then the points in rule order,
+// the geometry before the material. The smallest report is the first refusal in element order.
+constexpr unsigned long long NO_ESTIMATE_FAULT = ~0ull;
+
+// recovered_at_nodes (error_estimate.rs:207-231): one thread per (element, node); a corner takes
+// its vertex's fit at the corner, a midside the mean of its edge's two fits at the midside
+// (midside_mean, results.rs:544-552). recovered: 6 per (element, node) of the slice.
+__global__ __launch_bounds__(256) void estimate_nodes_kernel(size_t first, size_t count, const double *__restrict__ nodes,
+                                                             const uint32_t *__restrict__ tets, const double *__restrict__ fit,
+                                                             double *__restrict__ recovered, unsigned long long *__restrict__ fault) {
+    const size_t local = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= count * 10) return;
+    const size_t e = first + local / 10;
+    const int a = (int)(local % 10);
+    const uint32_t *t = tets + e * 10;
+    const double p[3] = {nodes[(size_t)t[a] * 3], nodes[(size_t)t[a] * 3 + 1], nodes[(size_t)t[a] * 3 + 2]};
+    double s[6];
+    if (a < 4) {
+        const double *f = fit + (size_t)t[a] * 29;
+        if (!(f[0] > 0.0)) {
+            atomicMin(fault, (unsigned long long)e * 32 + (unsigned long long)a);
+            return;
+        }
+        fit_at(f, p, s);
+    } else {
+        const double *fa = fit + (size_t)t[EDGE_A[a - 4]] * 29, *fb = fit + (size_t)t[EDGE_B[a - 4]] * 29;
+        if (!(fa[0] > 0.0) || !(fb[0] > 0.0)) return;  // the corner threads report it
+        double x[6], y[6];
+        fit_at(fa, p, x);
+        fit_at(fb, p, y);
+        for (int c = 0; c < 6; ++c) s[c] = 0.5 * (x[c] + y[c]);
+    }
+    double *o = recovered + local * 6;
+    for (int c = 0; c < 6; ++c) o[c] = s[c];
+}
+
+// compliance_energy (error_estimate.rs:240-248): s : C^-1 : s, one fixed expression
+__device__ double compliance_energy(double E, double nu, const double s[6]) {
+    const double normal = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const double coupling = s[0] * s[1] + s[1] * s[2] + s[2] * s[0];
+    const double shear = s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
+    return (normal - 2.0 * nu * coupling + 2.0 * (1.0 + nu) * shear) / E;
+}
+
+// element_integrals' terms at each point of the 14-point rule (error_estimate.rs:183-203): one
+// thread per (element, point); terms: measure * the difference's energy, measure * the element
+// stress's energy. An inverted point leaves its determinant in its first term for the report.
+__global__ __launch_bounds__(256) void estimate_points_kernel(size_t first, size_t count, const double *__restrict__ nodes,
+                                                              const uint32_t *__restrict__ tets, const double *__restrict__ u,
+                                                              const double *__restrict__ recovered, FieldArgs field,
+                                                              double *__restrict__ terms, unsigned long long *__restrict__ fault) {
+    const size_t local = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= count * 14) return;
+    const size_t slot = local / 14;
+    const int q = (int)(local - slot * 14);
+    const size_t e = first + slot;
+    double xe[10][3], ue[10][3], grad[10][3], pos[3], det = 0.0;
+    load_element(nodes, tets, u, e, xe, ue);
+    const QPoint qp = RULE14[q];
+    const double nat[3] = {qp.r, qp.s, qp.t};
+    if (!thread_geometry(xe, nat, grad, pos, &det)) {
+        terms[local * 2] = det;
+        atomicMin(fault, (unsigned long long)e * 32 + 4 + 2 * (unsigned long long)q);
+        return;
+    }
+    double E, nu;
+    material_at(field, pos, E, nu);
+    if (!(isfinite(E) && E > 0.0 && nu > -1.0 && nu < 0.5)) {
+        atomicMin(fault, (unsigned long long)e * 32 + 5 + 2 * (unsigned long long)q);
+        return;
+    }
+    double stress[6];
+    point_stress(grad, ue, E, nu, stress);
+    const double l[4] = {1.0 - nat[0] - nat[1] - nat[2], nat[0], nat[1], nat[2]};
+    const double *rec = recovered + slot * 60;
+    double difference[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int a = 0; a < 10; ++a) {
+        const double w = shape_value(a, l);
+        for (int c = 0; c < 6; ++c) difference[c] += w * rec[a * 6 + c];
+    }
+    for (int c = 0; c < 6; ++c) difference[c] -= stress[c];
+    const double measure = qp.w * det / 6.0;
+    terms[local * 2] = measure * compliance_energy(E, nu, difference);
+    terms[local * 2 + 1] = measure * compliance_energy(E, nu, stress);
+}
+
+// Each element's two integrals, its point terms added from zero in rule order: one thread per
+// element of the slice.
+__global__ void estimate_sums_kernel(size_t count, const double *__restrict__ terms, double *__restrict__ sums) {
+    const size_t slot = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= count) return;
+    double error = 0.0, energy = 0.0;
+    for (int q = 0; q < 14; ++q) {
+        error += terms[(slot * 14 + q) * 2];
+        energy += terms[(slot * 14 + q) * 2 + 1];
+    }
+    sums[slot * 2] = error;
+    sums[slot * 2 + 1] = energy;
 }
 
 FieldArgs field_args(const MaterialField &f) {
@@ -787,6 +893,128 @@ void recover_stress(const Model &model, const double *u, double *point_values, d
     DeviceRecovery r(model, u, true, s);
     r.advance(std::numeric_limits<size_t>::max(), nullptr);
     r.field(point_values, nodal_values);
+}
+
+DeviceErrorEstimation::DeviceErrorEstimation(const Model &model, DeviceRecovery &recovery, hipStream_t s)
+    : recovery_(recovery),
+      s_(s),
+      elements_(model.mesh.tets.size()),
+      nodes_(model.mesh.nodes.size()),
+      vertices_(model.mesh.vertex_count),
+      field_(model.field) {
+    // ErrorEstimation::new (error_estimate.rs:94-103) refuses nothing; a recovery of another mesh
+    // is refused by advance, so nothing of this mesh is needed then.
+    if (!matches()) return;
+    xyz_.upload(positions_of(model.mesh), s);
+    tets_.upload(tets_of(model.mesh), s);
+    element_errors_.reserve(elements_);
+}
+
+bool DeviceErrorEstimation::matches() const {
+    // the counts of error_estimate.rs:128-134: displacements, fits and elements
+    return recovery_.nodes() == nodes_ && recovery_.vertices() == vertices_ && recovery_.elements() == elements_;
+}
+
+void DeviceErrorEstimation::run(size_t count) {
+    constexpr size_t SLICE = size_t(1) << 16;  // elements per launch: 31 MB of recovered values
+    if (!recovered_.count) {
+        recovered_.resize(std::min(SLICE, elements_) * 60);
+        terms_.resize(std::min(SLICE, elements_) * 28);
+        sums_.resize(std::min(SLICE, elements_) * 2);
+        fault_.resize(1);
+    }
+    const FieldArgs fa = field_args(field_);
+    std::vector<double> sums;
+    for (size_t done = 0; done < count;) {
+        const size_t n = std::min(SLICE, count - done), first = next_;
+        const unsigned long long none = NO_ESTIMATE_FAULT;
+        SF_HIP(hipMemcpyAsync(fault_.ptr, &none, sizeof(none), hipMemcpyHostToDevice, s_));
+        hipLaunchKernelGGL(estimate_nodes_kernel, grid_of(n * 10), dim3(256), 0, s_, first, n, (const double *)xyz_.ptr,
+                           (const uint32_t *)tets_.ptr, recovery_.device_fits(), recovered_.ptr, fault_.ptr);
+        hipLaunchKernelGGL(estimate_points_kernel, grid_of(n * 14), dim3(256), 0, s_, first, n, (const double *)xyz_.ptr,
+                           (const uint32_t *)tets_.ptr, recovery_.device_displacements(), (const double *)recovered_.ptr, fa,
+                           terms_.ptr, fault_.ptr);
+        hipLaunchKernelGGL(estimate_sums_kernel, grid_of(n), dim3(256), 0, s_, n, (const double *)terms_.ptr, sums_.ptr);
+        SF_HIP(hipGetLastError());
+        unsigned long long fault = NO_ESTIMATE_FAULT;
+        sums.resize(n * 2);
+        fault_.download(&fault, 1, s_);
+        sums_.download(sums.data(), n * 2, s_);
+        SF_HIP(hipStreamSynchronize(s_));
+        // ErrorEstimation::advance's running sums (error_estimate.rs:143-148), in element order,
+        // up to the first refusing element
+        const size_t good = fault == NO_ESTIMATE_FAULT ? n : (size_t)(fault / 32) - first;
+        for (size_t k = 0; k < good; ++k) {
+            element_errors_.push_back(std::sqrt(sums[2 * k]));
+            error_sum_ += sums[2 * k];
+            energy_sum_ += sums[2 * k + 1];
+        }
+        next_ += good;
+        if (fault != NO_ESTIMATE_FAULT) throw refusal(fault, first);
+        done += n;
+    }
+}
+
+Error DeviceErrorEstimation::refusal(unsigned long long fault, size_t first) const {
+    const size_t e = (size_t)(fault / 32);
+    const int what = (int)(fault % 32);
+    if (what < 4) {
+        // the vertex of the first corner with no fit: a patch of two materials, or a flat one
+        uint32_t vertex = 0;
+        double flag = 0.0;
+        SF_HIP(hipMemcpyAsync(&vertex, tets_.ptr + e * 10 + what, sizeof(vertex), hipMemcpyDeviceToHost, s_));
+        SF_HIP(hipStreamSynchronize(s_));
+        SF_HIP(hipMemcpyAsync(&flag, recovery_.device_fits() + (size_t)vertex * 29, sizeof(flag), hipMemcpyDeviceToHost, s_));
+        SF_HIP(hipStreamSynchronize(s_));
+        if (flag < 0.0)
+            return Error(SF_ERR_MODEL_MIXED_MATERIAL_PATCH, "the elements at a vertex are of more than one material", vertex);
+        return Error(SF_ERR_MODEL_COPLANAR_PATCH, "the elements at a vertex are flat: no smoothed stress was fitted", vertex);
+    }
+    const int point = (what - 4) / 2;
+    if ((what - 4) % 2 == 0) {
+        double det = 0.0;
+        const size_t slot = e - first;  // the slice's terms start at its first element
+        SF_HIP(hipMemcpyAsync(&det, terms_.ptr + (slot * 14 + (size_t)point) * 2, sizeof(det), hipMemcpyDeviceToHost, s_));
+        SF_HIP(hipStreamSynchronize(s_));
+        return Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element is inverted at a point of the 14-point rule", (int64_t)e, point,
+                     det);
+    }
+    return Error(SF_ERR_MODEL_INVALID_MATERIAL, "an element has an invalid material at a point of the 14-point rule", (int64_t)e,
+                 point);
+}
+
+bool DeviceErrorEstimation::advance(size_t max_elements, const std::function<bool()> &cancel) {
+    // ErrorEstimation::advance (error_estimate.rs:121-151)
+    if (!matches()) throw Error(SF_ERR_MODEL_RECOVERY_MESH_MISMATCH, "the recovery was begun for another mesh");
+    if (!recovery_.finished() || !recovery_.converged())
+        throw Error(SF_ERR_MODEL_NOT_FINISHED, "the recovery is unfinished or of a solve that had not converged");
+    size_t budget = max_elements, take = 0;
+    bool cancelled = false;
+    while (next_ + take < elements_ && budget > 0) {
+        budget -= 1;
+        if (cancel && cancel()) {
+            cancelled = true;
+            break;
+        }
+        take += 1;
+    }
+    if (take) run(take);
+    if (cancelled) throw Error(SF_ERR_MODEL_CANCELLED, "error estimate cancelled");
+    return next_ == elements_;
+}
+
+void DeviceErrorEstimation::estimate(double *element_errors, double *error_norm_squared, double *solution_norm_squared,
+                                     double *relative_error) const {
+    // ErrorEstimation::estimate (error_estimate.rs:153-171)
+    if (next_ != elements_ || !recovery_.converged())
+        throw Error(SF_ERR_MODEL_NOT_FINISHED, "the error estimate has elements left to do", (int64_t)next_);
+    if (!std::isfinite(energy_sum_) || !std::isfinite(error_sum_))
+        throw Error(SF_ERR_MODEL_NON_FINITE_ENERGY, "the solution's energy is not a finite number");
+    if (energy_sum_ <= 0.0) throw Error(SF_ERR_MODEL_ZERO_STRAIN_ENERGY, "the solution stores no strain energy");
+    if (element_errors) std::copy(element_errors_.begin(), element_errors_.end(), element_errors);
+    if (error_norm_squared) *error_norm_squared = error_sum_;
+    if (solution_norm_squared) *solution_norm_squared = energy_sum_;
+    if (relative_error) *relative_error = std::sqrt(error_sum_ / (energy_sum_ + error_sum_));
 }
 
 void nodal_rotations(const Model &model, const double *u, double *rotations, hipStream_t s) {

@@ -974,6 +974,53 @@ sf_status sf_recovery_field(sf_recovery *recovery, double *point_values, double 
 }
 void sf_recovery_destroy(sf_recovery *recovery) { delete recovery; }
 
+sf_status sf_error_estimation_create(const sf_model *model, sf_recovery *recovery, sf_error_estimation **out) {
+    return guard([&] {
+        if (!model || !recovery || !out) throw Error(SF_ERR_INVALID_ARGUMENT, "an error estimate needs a model and a recovery");
+        auto e = std::make_unique<sf_error_estimation>();
+        e->ctx = recovery->ctx;
+        e->estimation = std::make_unique<DeviceErrorEstimation>(model->model, *recovery->recovery, recovery->ctx->stream);
+        *out = e.release();
+    });
+}
+sf_status sf_static_study_begin_error_estimate(const sf_static_study *study, sf_recovery *recovery, sf_error_estimation **out) {
+    return guard([&] {
+        if (!study || !recovery || !out) throw Error(SF_ERR_INVALID_ARGUMENT, "an error estimate needs a study and a recovery");
+        auto e = std::make_unique<sf_error_estimation>();
+        e->ctx = recovery->ctx;
+        e->estimation = std::make_unique<DeviceErrorEstimation>(study->model, *recovery->recovery, recovery->ctx->stream);
+        *out = e.release();
+    });
+}
+sf_status sf_error_estimation_advance(sf_error_estimation *estimation, uint64_t max_elements, sf_cancel_fn cancel, void *user,
+                                      int32_t *finished) {
+    return guard([&] {
+        const Cancel c{cancel, user};
+        const bool done = estimation->estimation->advance((size_t)max_elements, [&] { return c(); });
+        if (finished) *finished = done ? 1 : 0;
+    });
+}
+sf_status sf_static_study_continue_error_estimate(sf_static_study *study, sf_error_estimation *estimation, uint64_t max_elements,
+                                                  sf_cancel_fn cancel, void *user, int32_t *finished) {
+    const double t0 = seconds();
+    const sf_status st = sf_error_estimation_advance(estimation, max_elements, cancel, user, finished);
+    study->report.recovery_seconds += seconds() - t0;
+    return st;
+}
+uint64_t sf_error_estimation_elements_done(const sf_error_estimation *estimation) {
+    return estimation ? estimation->estimation->elements_done() : 0;
+}
+sf_status sf_error_estimation_finish(const sf_error_estimation *estimation, sf_error_estimate *out, double *element_errors) {
+    return guard([&] {
+        const DeviceErrorEstimation &e = *estimation->estimation;
+        sf_error_estimate r{};
+        e.estimate(element_errors, &r.error_norm_squared, &r.solution_norm_squared, &r.relative_error);
+        r.elements = e.elements();
+        if (out) *out = r;
+    });
+}
+void sf_error_estimation_destroy(sf_error_estimation *estimation) { delete estimation; }
+
 sf_status sf_peak_stress(const double *node_positions, uint64_t nodes, const double *nodal_von_mises, uint64_t nodal_stride,
                          const double *point_values, uint64_t points, sf_region_fn inside, void *user, sf_peak *out) {
     return guard([&] {

@@ -378,6 +378,10 @@ class DeviceRecovery {
     const std::vector<double> &displacements() const { return u_; }
     // The finished field (either output may be null); SF_ERR_MODEL_NOT_FINISHED before then.
     void field(double *point_values, double *nodal_values);
+    // On the device: the displacements (3 per node) and the patch fits (29 per vertex: a flag --
+    // 1 fitted, 0 flat, -1 more than one material -- the centre, the scale and 4 x 6 coefficients).
+    const double *device_displacements() const { return u_d_.ptr; }
+    const double *device_fits() const { return fit_.ptr; }
 
    private:
     void run_elements(size_t count);
@@ -396,6 +400,41 @@ class DeviceRecovery {
     DeviceBuffer<uint32_t> tets_, nstart_, nat_, vstart_, vat_, ends_;
     DeviceBuffer<int8_t> nearest_;
     DeviceBuffer<int32_t> fault_;
+};
+
+// ErrorEstimation (crates/solidfem-elastic/src/error_estimate.rs:80-238) of a recovery it borrows
+// (which must outlive it), over `model`'s mesh and material: per element, the 14-point integrals of
+// the recovered minus the element stress and of the element stress in the compliance norm on the
+// device; the running sums in element order on the host, so any slicing gives one run's bits.
+class DeviceErrorEstimation {
+   public:
+    DeviceErrorEstimation(const Model &model, DeviceRecovery &recovery, hipStream_t s);
+    // Up to `max_elements` more elements, `cancel` (may be empty) read before each; whether every
+    // element is done. Refuses a recovery of another mesh (SF_ERR_MODEL_RECOVERY_MESH_MISMATCH),
+    // an unfinished or preview one (SF_ERR_MODEL_NOT_FINISHED); a cancel throws
+    // SF_ERR_MODEL_CANCELLED and an element's refusal its error, the elements before it done.
+    bool advance(size_t max_elements, const std::function<bool()> &cancel);
+    size_t elements_done() const { return next_; }
+    size_t elements() const { return elements_; }
+    // ErrorEstimation::estimate: element errors (one per element; may be null) and the sums.
+    void estimate(double *element_errors, double *error_norm_squared, double *solution_norm_squared,
+                  double *relative_error) const;
+
+   private:
+    bool matches() const;
+    void run(size_t count);
+    Error refusal(unsigned long long fault, size_t first) const;
+
+    DeviceRecovery &recovery_;
+    hipStream_t s_;
+    size_t elements_, nodes_, vertices_;
+    MaterialField field_;
+    DeviceBuffer<double> xyz_, recovered_, terms_, sums_;
+    DeviceBuffer<uint32_t> tets_;
+    DeviceBuffer<unsigned long long> fault_;
+    size_t next_ = 0;
+    std::vector<double> element_errors_;
+    double error_sum_ = 0.0, energy_sum_ = 0.0;
 };
 
 // The whole recovery in one call; either output may be null.

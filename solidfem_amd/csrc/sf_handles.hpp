@@ -52,6 +52,10 @@ struct sf_static_study {
     sf_report report{};
 };
 
+struct sf_error_estimation {
+    std::shared_ptr<Context> ctx;
+    std::unique_ptr<DeviceErrorEstimation> estimation;
+};
 struct sf_recovery {
     std::shared_ptr<Context> ctx;
     std::unique_ptr<DeviceRecovery> recovery;

@@ -57,6 +57,7 @@ _SIG = {
     "ref_problem_rigid_modes": (None, [vp, P(f64)]),
     "ref_problem_element_stiffness": (i32, [vp, i64, P(f64)]),
     "ref_recover": (i32, [vp, P(f64), P(f64), P(f64), P(i64)]),
+    "ref_error_estimate": (i32, [vp, P(f64), P(f64), P(f64), P(i64)]),
     "ref_problem_quadrature": (i32, [vp]),
     "ref_peak": (i32, [P(f64), u64, P(f64), u64, P(f64), u64, P(C.c_uint8), P(C.c_uint8), P(i32), P(u64), P(f64), P(f64)]),
     "ref_nodal_rotations": (i32, [vp, P(f64), P(f64), P(i64)]),
@@ -208,6 +209,17 @@ class Problem:
         if lib().ref_recover(self._p, ptr(u, f64), ptr(points, f64), ptr(nodal, f64), C.byref(fault)) != 0:
             raise RuntimeError(f"element {fault.value} is inverted or has an invalid material")
         return points, nodal
+
+    def error_estimate(self, u):
+        """The error estimate of displacements u (error_estimate.rs:121-171): a dict with status (0,
+        or the refusal's status code), a, b, the per-element errors done and the three sums."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        errors = np.empty(lib().ref_problem_tet_count(self._p))
+        sums = np.zeros(3)
+        payload = np.zeros(3, dtype=np.int64)
+        status = lib().ref_error_estimate(self._p, ptr(u, f64), ptr(errors, f64), ptr(sums, f64), ptr(payload, i64))
+        return dict(status=status, a=int(payload[0]), b=int(payload[1]), element_errors=errors[:payload[2]].copy(),
+                    error_norm_squared=sums[0], solution_norm_squared=sums[1], relative_error=sums[2])
 
     def nodal_rotations(self, u):
         """nodal_rotations (results.rs:262-305): (nodes, 3)."""
