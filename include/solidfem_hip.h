@@ -79,10 +79,20 @@ enum {
     SF_ERR_MODEL_ZERO_STRAIN_ENERGY = 55,
     SF_ERR_MODEL_NON_FINITE_ENERGY = 56,
     SF_ERR_MODEL_RECOVERY_MESH_MISMATCH = 57,
+    SF_ERR_MODEL_SOLUTION_MESH_MISMATCH = 58, /* a=SF_COMPARED_* */
+    SF_ERR_MODEL_NOTHING_INSIDE_REGION = 59,  /* a=SF_COMPARED_* */
     /* serving */
     SF_ERR_SERVE_OVER_BUDGET = 60,         /* a=bytes needed, b=budget */
-    SF_ERR_SERVE_UNKNOWN_REQUEST = 61
+    SF_ERR_SERVE_UNKNOWN_REQUEST = 61,
+    /* ModelError's mesh-comparison refusals, continued (crates/solidfem-elastic/src/error.rs:95-119) */
+    SF_ERR_MODEL_NON_FINITE_PEAK = 70,        /* a=SF_COMPARED_*, value=the peak */
+    SF_ERR_MODEL_ZERO_PEAK = 71,              /* a=SF_COMPARED_* */
+    SF_ERR_MODEL_FINE_MESH_NOT_FINER = 72,    /* a=coarse elements, b=fine elements in the region */
+    SF_ERR_MODEL_INVALID_YIELD_STRENGTH = 73  /* value=the yield strength */
 };
+
+/* Which solve of a mesh comparison a refusal names (ComparedMesh, error.rs). */
+enum { SF_COMPARED_COARSE = 0, SF_COMPARED_FINE = 1 };
 
 /* The rigid motion an insufficiently held body keeps (Freedom,
  * crates/solidfem-elastic/src/constraints.rs:14-23). */
@@ -610,6 +620,53 @@ typedef struct sf_peak {
 } sf_peak;
 sf_status sf_peak_stress(const double *node_positions, uint64_t nodes, const double *nodal_von_mises, uint64_t nodal_stride,
                          const double *point_values, uint64_t points, sf_region_fn inside, void *user, sf_peak *out);
+
+/* ---- mesh comparison (crates/solidfem-elastic/src/comparison.rs) ------------------------------
+ * How much the peak von Mises moved between a coarser and a finer solve of one part and load, read
+ * over a region (the region callback, NULL: everywhere). A solve is given as its mesh, its nodal
+ * von Mises (`nodes` values, `nodal_stride` apart: 7 with sf_recover_stress's nodal values, from
+ * its von Mises column) and its point values (10 per point as sf_recover_stress writes them, the
+ * same number per element in element order). Host work over finished solutions. */
+typedef struct sf_compared_solve {
+    const sf_quad_mesh *mesh;
+    const double *nodal_von_mises;
+    uint64_t nodes, nodal_stride;
+    const double *point_values;
+    uint64_t points;
+} sf_compared_solve;
+/* ComparedPeak (comparison.rs:55-66) */
+typedef struct sf_compared_peak {
+    sf_peak peak;
+    uint64_t elements;          /* elements with an integration point inside the region */
+    double peak_element_volume; /* the peak's element, or the mean of the elements holding its node */
+} sf_compared_peak;
+/* MeshComparison (comparison.rs:83-92) */
+typedef struct sf_mesh_comparison {
+    sf_compared_peak coarse, fine;
+    double peak_change;   /* (fine - coarse) / fine of the peak */
+    double peak_distance; /* between the two peaks' positions */
+} sf_mesh_comparison;
+/* Verdict (comparison.rs:68-80) */
+enum { SF_VERDICT_SETTLED = 0, SF_VERDICT_STILL_MOVING = 1, SF_VERDICT_MESHES_TOO_ALIKE = 2 };
+/* MIN_ELEMENT_RATIO (comparison.rs:53) */
+#define SF_MIN_ELEMENT_RATIO 4.0
+/* MeshComparison::between (comparison.rs:101-141): SF_ERR_MODEL_SOLUTION_MESH_MISMATCH,
+ * SF_ERR_MODEL_NOTHING_INSIDE_REGION, SF_ERR_MODEL_NON_FINITE_PEAK, SF_ERR_MODEL_ZERO_PEAK (a = the
+ * solve), SF_ERR_MODEL_FINE_MESH_NOT_FINER (a, b = the element counts). The region
+ * callback returns nonzero for a position inside. */
+sf_status sf_mesh_comparison_between(const sf_compared_solve *coarse, const sf_compared_solve *fine, sf_region_fn inside,
+                                     void *user, sf_mesh_comparison *out);
+/* element_ratio, peak_element_ratio (comparison.rs:154-169) */
+double sf_mesh_comparison_element_ratio(const sf_mesh_comparison *comparison);
+double sf_mesh_comparison_peak_element_ratio(const sf_mesh_comparison *comparison);
+/* safety_factors and safety_factor_change (comparison.rs:186-205): SF_ERR_MODEL_INVALID_YIELD_STRENGTH
+ * for a yield strength that is not finite and positive */
+sf_status sf_mesh_comparison_safety_factors(const sf_mesh_comparison *comparison, double yield_strength, double *coarse,
+                                            double *fine);
+sf_status sf_mesh_comparison_safety_factor_change(const sf_mesh_comparison *comparison, double yield_strength,
+                                                  double *change);
+/* verdict (comparison.rs:207-222): SF_VERDICT_* */
+int32_t sf_mesh_comparison_verdict(const sf_mesh_comparison *comparison, double tolerance);
 /* nodal_rotations (results.rs:262-305): the local rotation (half the curl of the displacement), in
  * radians, at every mesh node -- each incident element's value at the node weighted by its
  * Jacobian determinant there, summed in element order. rotations: 3 per node. */

@@ -674,6 +674,31 @@ int ref_peak(const double *positions, uint64_t nodes, const double *nodal_vm, ui
     for (int a = 0; a < 3; ++a) position[a] = peak.position[a];
     return 1;
 }
+int ref_compare(const ref_problem *coarse, const double *coarse_nodal_vm, uint64_t coarse_stride, const double *coarse_points,
+                uint64_t coarse_count, const uint8_t *coarse_node_inside, const uint8_t *coarse_point_inside, const ref_problem *fine,
+                const double *fine_nodal_vm, uint64_t fine_stride, const double *fine_points, uint64_t fine_count,
+                const uint8_t *fine_node_inside, const uint8_t *fine_point_inside, double *values, int64_t *integers) {
+    const ComparedSide c{&coarse->p->mesh, coarse_nodal_vm, coarse->p->mesh.nodes.size(), (size_t)coarse_stride, coarse_points,
+                         (size_t)coarse_count, coarse_node_inside, coarse_point_inside};
+    const ComparedSide f{&fine->p->mesh, fine_nodal_vm, fine->p->mesh.nodes.size(), (size_t)fine_stride, fine_points,
+                         (size_t)fine_count, fine_node_inside, fine_point_inside};
+    const Comparison r = compare_meshes(c, f);
+    const ComparedPeak *sides[2] = {&r.coarse, &r.fine};
+    for (int w = 0; w < 2; ++w) {
+        values[5 * w] = sides[w]->peak.von_mises;
+        for (int a = 0; a < 3; ++a) values[5 * w + 1 + a] = sides[w]->peak.position[a];
+        values[5 * w + 4] = sides[w]->volume;
+        integers[3 * w] = (int64_t)sides[w]->elements;
+        integers[3 * w + 1] = sides[w]->peak.at_point ? 1 : 0;
+        integers[3 * w + 2] = (int64_t)sides[w]->peak.index;
+    }
+    values[10] = r.change;
+    values[11] = r.distance;
+    values[12] = r.value;
+    integers[6] = r.a;
+    integers[7] = r.b;
+    return r.status;
+}
 // nodal_rotations (results.rs:262-305) of nodal displacements `u` (3 per node): 3 per node. Returns
 // 0, or -1 with the failing element in *fault_tet.
 int ref_nodal_rotations(const ref_problem *p, const double *u, double *out, int64_t *fault_tet) {

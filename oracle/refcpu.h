@@ -143,6 +143,16 @@ int ref_error_estimate(const ref_problem *p, const double *u, double *element_er
 int ref_peak(const double *positions, uint64_t nodes, const double *nodal_vm, uint64_t stride, const double *points, uint64_t count,
              const uint8_t *node_inside, const uint8_t *point_inside, int32_t *at_point, uint64_t *index, double *von_mises,
              double *position);
+/* MeshComparison::between (comparison.rs:101-141) of a coarse and a fine solve of two problems:
+ * nodal von Mises (`stride` apart, one per node), point values (10 per point, element-major) and
+ * the masks of the candidates inside the region (NULL: all). values (13): per side (coarse, fine)
+ * the peak's von Mises, position (3) and element volume, then the change, the distance and a
+ * refusal's value; integers (8): per side the elements inside, at_point and index, then a
+ * refusal's a and b. Returns 0 or the refusal as the product's status code. */
+int ref_compare(const ref_problem *coarse, const double *coarse_nodal_vm, uint64_t coarse_stride, const double *coarse_points,
+                uint64_t coarse_count, const uint8_t *coarse_node_inside, const uint8_t *coarse_point_inside, const ref_problem *fine,
+                const double *fine_nodal_vm, uint64_t fine_stride, const double *fine_points, uint64_t fine_count,
+                const uint8_t *fine_node_inside, const uint8_t *fine_point_inside, double *values, int64_t *integers);
 /* nodal_rotations: 3 per node; 0, or -1 with the failing element */
 int ref_nodal_rotations(const ref_problem *p, const double *u, double *out, int64_t *fault_tet);
 double ref_problem_assembly_seconds(const ref_problem *p);

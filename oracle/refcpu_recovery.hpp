@@ -448,4 +448,105 @@ inline int64_t nodal_rotations(const QuadMesh &mesh, const std::vector<double> &
     return -1;
 }
 
+// MeshComparison::between (comparison.rs:101-141) of a coarse and a fine solve, each given as its
+// mesh, nodal von Mises (stride apart), point values (10 per point, element-major) and the masks of
+// the candidates inside the region (evaluated by the caller; NULL: all).
+struct ComparedSide {
+    const QuadMesh *mesh;
+    const double *nodal_vm;
+    size_t nodes, stride;
+    const double *points;
+    size_t count;
+    const unsigned char *node_inside, *point_inside;
+};
+struct ComparedPeak {
+    Peak peak;
+    size_t elements = 0;
+    double volume = 0.0;
+};
+struct Comparison {
+    int status = 0;  // 0, or the refusal as the product's status code
+    int64_t a = -1, b = -1;
+    double value = 0.0;
+    ComparedPeak coarse, fine;
+    double change = 0.0, distance = 0.0;
+};
+
+inline Comparison compare_meshes(const ComparedSide &coarse, const ComparedSide &fine) {
+    Comparison out;
+    auto refuse = [&](int status, int64_t a, int64_t b = -1, double value = 0.0) {
+        out.status = status;
+        out.a = a;
+        out.b = b;
+        out.value = value;
+        return out;
+    };
+    const ComparedSide *sides[2] = {&coarse, &fine};
+    // check_matches, comparison.rs:199-224
+    for (int w = 0; w < 2; ++w) {
+        const ComparedSide &s = *sides[w];
+        const size_t tets = s.mesh->tets.size();
+        if (s.nodes != s.mesh->nodes.size() || tets == 0 || s.count == 0 || s.count % tets != 0) return refuse(58, w);
+    }
+    // region_peak, comparison.rs:271-305
+    ComparedPeak *peaks[2] = {&out.coarse, &out.fine};
+    for (int w = 0; w < 2; ++w) {
+        const ComparedSide &s = *sides[w];
+        bool any = false;
+        for (size_t k = 0; k < s.count && !any; ++k) any = !s.point_inside || s.point_inside[k];
+        if (!any) return refuse(59, w);
+        std::vector<double> positions(3 * s.mesh->nodes.size());
+        for (size_t n = 0; n < s.mesh->nodes.size(); ++n)
+            for (int a = 0; a < 3; ++a) positions[3 * n + a] = s.mesh->nodes[n][a];
+        const Peak peak = peak_of(positions.data(), s.nodes, s.nodal_vm, s.stride, s.points, s.count, s.node_inside, s.point_inside);
+        if (!peak.found) return refuse(59, w);
+        if (!std::isfinite(peak.von_mises)) return refuse(70, w, -1, peak.von_mises);
+        if (peak.von_mises <= 0.0) return refuse(71, w);
+        peaks[w]->peak = peak;
+    }
+    // elements_inside, comparison.rs:226-239
+    for (int w = 0; w < 2; ++w) {
+        const ComparedSide &s = *sides[w];
+        const size_t per = s.count / s.mesh->tets.size();
+        int64_t last = -1;
+        for (size_t k = 0; k < s.count; ++k) {
+            const int64_t tet = (int64_t)(k / per);
+            if (last != tet && (!s.point_inside || s.point_inside[k])) {
+                peaks[w]->elements += 1;
+                last = tet;
+            }
+        }
+    }
+    if (out.fine.elements <= out.coarse.elements) return refuse(72, (int64_t)out.coarse.elements, (int64_t)out.fine.elements);
+    out.change = (out.fine.peak.von_mises - out.coarse.peak.von_mises) / out.fine.peak.von_mises;
+    double squared = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double d = out.fine.peak.position[a] - out.coarse.peak.position[a];
+        squared += d * d;
+    }
+    out.distance = std::sqrt(squared);
+    // holder_volume, comparison.rs:241-269: the corner tetrahedron's volume (tet.rs:9-16)
+    for (int w = 0; w < 2; ++w) {
+        const ComparedSide &s = *sides[w];
+        const QuadMesh &m = *s.mesh;
+        auto volume = [&](const std::array<u32, 10> &t) { return tet_volume(m.nodes[t[0]], m.nodes[t[1]], m.nodes[t[2]], m.nodes[t[3]]); };
+        const Peak &peak = peaks[w]->peak;
+        if (peak.at_point) {
+            peaks[w]->volume = volume(m.tets[peak.index / (s.count / m.tets.size())]);
+        } else {
+            double sum = 0.0;
+            size_t n = 0;
+            for (const auto &t : m.tets)
+                for (int a = 0; a < 10; ++a)
+                    if (t[a] == (u32)peak.index) {
+                        sum += volume(t);
+                        n += 1;
+                        break;
+                    }
+            peaks[w]->volume = sum / (double)n;
+        }
+    }
+    return out;
+}
+
 }  // namespace refcpu

@@ -88,7 +88,7 @@ def _raise(status: int):
         cls = MultigridError
     elif 30 <= status < 40:
         cls = MeshError
-    elif 40 <= status < 60:
+    elif 40 <= status < 60 or 70 <= status < 80:
         cls = ModelError
     raise cls(status, msg, a.value, b.value, v.value)
 
@@ -339,7 +339,10 @@ class Model(_Handle):
         return cls(h)
 
     def mesh(self) -> QuadraticTetMesh:
-        return QuadraticTetMesh(library().sf_model_mesh(self._ptr), owned=False)
+        """The model's mesh, a view that keeps the model alive."""
+        m = QuadraticTetMesh(library().sf_model_mesh(self._ptr), owned=False)
+        m._parent = self
+        return m
 
     def set_quadrature(self, points: int):
         _raise(library().sf_model_set_quadrature(self._ptr, points))
@@ -1251,6 +1254,104 @@ def peak_stress(positions, nodal_von_mises, point_values, inside: Optional[Calla
     if not out.found:
         return None
     return PeakStress(out.von_mises, np.array(out.position[:]), bool(out.at_point), out.index)
+
+
+class Verdict(enum.IntEnum):
+    """Verdict (comparison.rs:68-80)."""
+    SETTLED = 0
+    STILL_MOVING = 1
+    MESHES_TOO_ALIKE = 2
+
+
+MIN_ELEMENT_RATIO = 4.0  # comparison.rs:53
+
+
+@dataclass
+class ComparedPeak:
+    """ComparedPeak (comparison.rs:55-66)."""
+    peak: PeakStress
+    elements: int
+    peak_element_volume: float
+
+
+class MeshComparison:
+    """MeshComparison (comparison.rs:83-223): how much the peak von Mises moved between a coarser and
+    a finer solve of one part and load, over a region. A solve is (mesh, nodal_values (nodes, 7) or
+    the nodal von Mises (nodes,), point_values (elements, points, 10)) as a recovery gives them."""
+
+    def __init__(self, raw):
+        self._raw = raw
+
+    @classmethod
+    def between(cls, coarse, fine, inside: Optional[Callable[[np.ndarray], bool]] = None) -> "MeshComparison":
+        keep = []
+
+        def solve(side):
+            mesh, nodal, points = side
+            nodal = np.ascontiguousarray(nodal, dtype=np.float64)
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 10)
+            keep.extend([nodal, pts])
+            if nodal.ndim == 1:
+                vm, stride = nodal.ctypes.data, 1
+            else:  # recovered nodal values: the von Mises is the last of 7 columns
+                if nodal.shape[1] != 7:
+                    raise ValueError("nodal values are 7 per node")
+                vm, stride = nodal.ctypes.data + 6 * 8, 7
+            return _capi.ComparedSolve(mesh._ptr, C.cast(vm, C.POINTER(C.c_double)), nodal.shape[0], stride,
+                                       _ptr(pts, C.c_double), pts.shape[0])
+
+        failure = []
+
+        def region(p, _user):
+            try:
+                return 1 if inside(np.array(p[:3])) else 0
+            except BaseException as e:  # the callback cannot raise through C
+                failure.append(e)
+                return 0
+
+        fn = _capi.REGION_FN(region) if inside is not None else _capi.REGION_FN()
+        c, f = solve(coarse), solve(fine)
+        out = _capi.MeshComparison()
+        status = library().sf_mesh_comparison_between(C.byref(c), C.byref(f), fn, None, C.byref(out))
+        if failure:
+            raise failure[0]
+        _raise(status)
+        return cls(out)
+
+    def _peak(self, p) -> ComparedPeak:
+        return ComparedPeak(PeakStress(p.peak.von_mises, np.array(p.peak.position[:]), bool(p.peak.at_point), p.peak.index),
+                            p.elements, p.peak_element_volume)
+
+    def coarse(self) -> ComparedPeak:
+        return self._peak(self._raw.coarse)
+
+    def fine(self) -> ComparedPeak:
+        return self._peak(self._raw.fine)
+
+    def peak_change(self) -> float:
+        return self._raw.peak_change
+
+    def peak_distance(self) -> float:
+        return self._raw.peak_distance
+
+    def element_ratio(self) -> float:
+        return library().sf_mesh_comparison_element_ratio(C.byref(self._raw))
+
+    def peak_element_ratio(self) -> float:
+        return library().sf_mesh_comparison_peak_element_ratio(C.byref(self._raw))
+
+    def safety_factors(self, yield_strength: float):
+        a, b = C.c_double(), C.c_double()
+        _raise(library().sf_mesh_comparison_safety_factors(C.byref(self._raw), yield_strength, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def safety_factor_change(self, yield_strength: float) -> float:
+        out = C.c_double()
+        _raise(library().sf_mesh_comparison_safety_factor_change(C.byref(self._raw), yield_strength, C.byref(out)))
+        return out.value
+
+    def verdict(self, tolerance: float) -> Verdict:
+        return Verdict(library().sf_mesh_comparison_verdict(C.byref(self._raw), tolerance))
 
 
 def peak_von_mises(positions, nodal_von_mises, point_values, inside=None) -> Optional[float]:

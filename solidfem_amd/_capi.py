@@ -102,6 +102,19 @@ class Peak(C.Structure):
     _fields_ = [("found", i32), ("at_point", i32), ("index", u64), ("von_mises", f64), ("position", f64 * 3)]
 
 
+class ComparedSolve(C.Structure):
+    _fields_ = [("mesh", vp), ("nodal_von_mises", P(f64)), ("nodes", u64), ("nodal_stride", u64),
+                ("point_values", P(f64)), ("points", u64)]
+
+
+class ComparedPeak(C.Structure):
+    _fields_ = [("peak", Peak), ("elements", u64), ("peak_element_volume", f64)]
+
+
+class MeshComparison(C.Structure):
+    _fields_ = [("coarse", ComparedPeak), ("fine", ComparedPeak), ("peak_change", f64), ("peak_distance", f64)]
+
+
 class BuildProgress(C.Structure):
     _fields_ = [("elements_done", u64), ("elements_total", u64), ("setup_finished", i32), ("reserved", i32)]
 
@@ -249,6 +262,12 @@ SIGNATURES = {
     "sf_error_estimation_elements_done": (u64, [vp]),
     "sf_error_estimation_finish": (i32, [vp, P(ErrorEstimate), P(f64)]),
     "sf_error_estimation_destroy": (None, [vp]),
+    "sf_mesh_comparison_between": (i32, [P(ComparedSolve), P(ComparedSolve), REGION_FN, vp, P(MeshComparison)]),
+    "sf_mesh_comparison_element_ratio": (f64, [P(MeshComparison)]),
+    "sf_mesh_comparison_peak_element_ratio": (f64, [P(MeshComparison)]),
+    "sf_mesh_comparison_safety_factors": (i32, [P(MeshComparison), f64, P(f64), P(f64)]),
+    "sf_mesh_comparison_safety_factor_change": (i32, [P(MeshComparison), f64, P(f64)]),
+    "sf_mesh_comparison_verdict": (i32, [P(MeshComparison), f64]),
     "sf_static_study_continue_recovery": (i32, [vp, vp, u64, CANCEL_FN, vp, P(i32)]),
     "sf_recovery_state_of": (i32, [vp, P(RecoveryState)]),
     "sf_recovery_displacements": (i32, [vp, P(f64)]),

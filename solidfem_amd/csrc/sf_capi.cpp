@@ -1039,6 +1039,82 @@ sf_status sf_peak_stress(const double *node_positions, uint64_t nodes, const dou
         for (int a = 0; a < 3; ++a) out->position[a] = peak.position[a];
     });
 }
+namespace {
+ComparedSolve compared_solve(const sf_compared_solve *s) {
+    if (!s || !s->mesh) throw Error(SF_ERR_INVALID_ARGUMENT, "a compared solve needs its mesh");
+    if ((s->nodes && (!s->nodal_von_mises || s->nodal_stride == 0)) || (s->points && !s->point_values))
+        throw Error(SF_ERR_INVALID_ARGUMENT, "a compared solve's array is missing");
+    ComparedSolve c;
+    c.mesh = &s->mesh->mesh;
+    c.nodal_vm = s->nodal_von_mises;
+    c.nodes = (size_t)s->nodes;
+    c.stride = (size_t)s->nodal_stride;
+    c.points = s->point_values;
+    c.point_count = (size_t)s->points;
+    return c;
+}
+sf_compared_peak compared_peak(const ComparedPeak &p) {
+    sf_compared_peak out{};
+    out.peak.found = p.peak.found ? 1 : 0;
+    out.peak.at_point = p.peak.at_point ? 1 : 0;
+    out.peak.index = p.peak.index;
+    out.peak.von_mises = p.peak.von_mises;
+    for (int a = 0; a < 3; ++a) out.peak.position[a] = p.peak.position[a];
+    out.elements = p.elements;
+    out.peak_element_volume = p.peak_element_volume;
+    return out;
+}
+void check_yield(double yield_strength) {
+    // comparison.rs:186-193
+    if (!(std::isfinite(yield_strength) && yield_strength > 0.0))
+        throw Error(SF_ERR_MODEL_INVALID_YIELD_STRENGTH, "the yield strength is not finite and positive", 0, 0, yield_strength);
+}
+}  // namespace
+
+sf_status sf_mesh_comparison_between(const sf_compared_solve *coarse, const sf_compared_solve *fine, sf_region_fn inside,
+                                     void *user, sf_mesh_comparison *out) {
+    return guard([&] {
+        if (!out) throw Error(SF_ERR_INVALID_ARGUMENT, "no place for the comparison");
+        std::function<bool(const double *)> region;
+        if (inside) region = [&](const double *p) { return inside(p, user) != 0; };
+        const MeshComparison c = compare_meshes(compared_solve(coarse), compared_solve(fine), region);
+        *out = sf_mesh_comparison{};
+        out->coarse = compared_peak(c.coarse);
+        out->fine = compared_peak(c.fine);
+        out->peak_change = c.peak_change;
+        out->peak_distance = c.peak_distance;
+    });
+}
+double sf_mesh_comparison_element_ratio(const sf_mesh_comparison *c) {
+    // element counts are far below 2^53, so both convert exactly (comparison.rs:154-160)
+    return (double)c->fine.elements / (double)c->coarse.elements;
+}
+double sf_mesh_comparison_peak_element_ratio(const sf_mesh_comparison *c) {
+    return c->coarse.peak_element_volume / c->fine.peak_element_volume;
+}
+sf_status sf_mesh_comparison_safety_factors(const sf_mesh_comparison *c, double yield_strength, double *coarse, double *fine) {
+    return guard([&] {
+        check_yield(yield_strength);
+        if (coarse) *coarse = yield_strength / c->coarse.peak.von_mises;
+        if (fine) *fine = yield_strength / c->fine.peak.von_mises;
+    });
+}
+sf_status sf_mesh_comparison_safety_factor_change(const sf_mesh_comparison *c, double yield_strength, double *change) {
+    return guard([&] {
+        check_yield(yield_strength);
+        const double coarse = yield_strength / c->coarse.peak.von_mises, fine = yield_strength / c->fine.peak.von_mises;
+        if (change) *change = (fine - coarse) / fine;
+    });
+}
+int32_t sf_mesh_comparison_verdict(const sf_mesh_comparison *c, double tolerance) {
+    // comparison.rs:207-222: too alike under the minimum ratio (or NaN), whatever the change; a
+    // NaN tolerance is never met
+    const double ratio = sf_mesh_comparison_peak_element_ratio(c);
+    if (std::isnan(ratio) || ratio < SF_MIN_ELEMENT_RATIO) return SF_VERDICT_MESHES_TOO_ALIKE;
+    if (std::fabs(c->peak_change) <= tolerance) return SF_VERDICT_SETTLED;
+    return SF_VERDICT_STILL_MOVING;
+}
+
 sf_status sf_nodal_rotations(sf_context *ctx, const sf_model *model, const double *displacements, double *rotations) {
     return guard([&] { nodal_rotations(model->model, displacements, rotations, ctx->ctx->stream); });
 }

@@ -177,4 +177,29 @@ struct PeakStress {
 PeakStress peak_stress(const double *positions, size_t nodes, const double *nodal_vm, size_t stride, const double *point_values,
                        size_t points, const std::function<bool(const double *)> &inside);
 
+// One finished solve of a mesh comparison (crates/solidfem-elastic/src/comparison.rs): its mesh,
+// its nodal von Mises (`nodes` values, `stride` apart) and its integration-point values (10 per
+// point, the same number per element, in element order).
+struct ComparedSolve {
+    const QuadMesh *mesh = nullptr;
+    const double *nodal_vm = nullptr;
+    size_t nodes = 0, stride = 1;
+    const double *points = nullptr;
+    size_t point_count = 0;
+};
+struct ComparedPeak {
+    PeakStress peak;
+    size_t elements = 0;             // elements with an integration point inside the region
+    double peak_element_volume = 0;  // the volume of the element(s) holding the peak
+};
+struct MeshComparison {
+    ComparedPeak coarse, fine;
+    double peak_change = 0, peak_distance = 0;
+};
+// MeshComparison::between (comparison.rs:101-141); throws the refusals
+// (SF_ERR_MODEL_SOLUTION_MESH_MISMATCH, _NOTHING_INSIDE_REGION, _NON_FINITE_PEAK, _ZERO_PEAK,
+// _FINE_MESH_NOT_FINER).
+MeshComparison compare_meshes(const ComparedSolve &coarse, const ComparedSolve &fine,
+                              const std::function<bool(const double *)> &inside);
+
 }  // namespace sf

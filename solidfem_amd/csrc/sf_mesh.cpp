@@ -1060,4 +1060,90 @@ PeakStress peak_stress(const double *positions, size_t nodes, const double *noda
     return peak;
 }
 
+namespace {
+
+// check_matches (comparison.rs:199-224): one value per node, and the same whole number of
+// integration points for every element (the point layout is element-major by construction)
+void check_matches(const ComparedSolve &s, int which) {
+    const size_t tets = s.mesh->tets.size();
+    const bool per_node = s.nodes == s.mesh->nodes.size();
+    const bool covers = tets > 0 && s.point_count > 0 && s.point_count % tets == 0;
+    if (!per_node || !covers)
+        throw Error(SF_ERR_MODEL_SOLUTION_MESH_MISMATCH, "a solution's arrays do not match its mesh", which);
+}
+
+// region_peak (comparison.rs:271-305): refused when no integration point is inside, or the peak
+// is not finite or is zero
+PeakStress region_peak(const ComparedSolve &s, const std::function<bool(const double *)> &inside, int which) {
+    bool any = false;
+    for (size_t p = 0; p < s.point_count && !any; ++p) any = !inside || inside(s.points + p * 10);
+    if (!any) throw Error(SF_ERR_MODEL_NOTHING_INSIDE_REGION, "no integration point of a mesh is inside the region", which);
+    std::vector<double> positions(s.mesh->nodes.size() * 3);
+    for (size_t n = 0; n < s.mesh->nodes.size(); ++n)
+        for (int a = 0; a < 3; ++a) positions[3 * n + a] = s.mesh->nodes[n][a];
+    const PeakStress peak = peak_stress(positions.data(), s.nodes, s.nodal_vm, s.stride, s.points, s.point_count, inside);
+    if (!peak.found) throw Error(SF_ERR_MODEL_NOTHING_INSIDE_REGION, "no integration point of a mesh is inside the region", which);
+    if (!std::isfinite(peak.von_mises))
+        throw Error(SF_ERR_MODEL_NON_FINITE_PEAK, "a peak stress is not a finite number", which, 0, peak.von_mises);
+    if (peak.von_mises <= 0.0) throw Error(SF_ERR_MODEL_ZERO_PEAK, "a peak stress is zero", which);
+    return peak;
+}
+
+// elements_inside (comparison.rs:226-239): elements with at least one point inside, counted once
+size_t elements_inside(const ComparedSolve &s, const std::function<bool(const double *)> &inside) {
+    const size_t per = s.point_count / s.mesh->tets.size();
+    size_t count = 0, last = SIZE_MAX;
+    for (size_t p = 0; p < s.point_count; ++p) {
+        const size_t tet = p / per;
+        if (last != tet && (!inside || inside(s.points + p * 10))) {
+            count += 1;
+            last = tet;
+        }
+    }
+    return count;
+}
+
+// holder_volume (comparison.rs:241-269): a point's own element, or the mean volume of the
+// elements holding a node, summed in element order
+double holder_volume(const ComparedSolve &s, const PeakStress &peak) {
+    const QuadMesh &m = *s.mesh;
+    auto volume = [&](const std::array<u32, 10> &t) { return signed_volume(m.nodes[t[0]], m.nodes[t[1]], m.nodes[t[2]], m.nodes[t[3]]); };
+    if (peak.at_point) return volume(m.tets[peak.index / (s.point_count / m.tets.size())]);
+    double sum = 0.0;
+    size_t count = 0;
+    for (const auto &t : m.tets)
+        if (std::find(t.begin(), t.end(), (u32)peak.index) != t.end()) {
+            sum += volume(t);
+            count += 1;
+        }
+    return sum / (double)count;
+}
+
+}  // namespace
+
+MeshComparison compare_meshes(const ComparedSolve &coarse, const ComparedSolve &fine,
+                              const std::function<bool(const double *)> &inside) {
+    // MeshComparison::between, comparison.rs:101-141
+    check_matches(coarse, SF_COMPARED_COARSE);
+    check_matches(fine, SF_COMPARED_FINE);
+    MeshComparison c;
+    c.coarse.peak = region_peak(coarse, inside, SF_COMPARED_COARSE);
+    c.fine.peak = region_peak(fine, inside, SF_COMPARED_FINE);
+    c.coarse.elements = elements_inside(coarse, inside);
+    c.fine.elements = elements_inside(fine, inside);
+    if (c.fine.elements <= c.coarse.elements)
+        throw Error(SF_ERR_MODEL_FINE_MESH_NOT_FINER, "the fine mesh has no more elements in the region than the coarse one",
+                    (int64_t)c.coarse.elements, (int64_t)c.fine.elements);
+    c.peak_change = (c.fine.peak.von_mises - c.coarse.peak.von_mises) / c.fine.peak.von_mises;
+    double squared = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double d = c.fine.peak.position[a] - c.coarse.peak.position[a];
+        squared += d * d;
+    }
+    c.peak_distance = std::sqrt(squared);
+    c.coarse.peak_element_volume = holder_volume(coarse, c.coarse.peak);
+    c.fine.peak_element_volume = holder_volume(fine, c.fine.peak);
+    return c;
+}
+
 }  // namespace sf

@@ -58,6 +58,8 @@ _SIG = {
     "ref_problem_element_stiffness": (i32, [vp, i64, P(f64)]),
     "ref_recover": (i32, [vp, P(f64), P(f64), P(f64), P(i64)]),
     "ref_error_estimate": (i32, [vp, P(f64), P(f64), P(f64), P(i64)]),
+    "ref_compare": (i32, [vp, P(f64), u64, P(f64), u64, P(C.c_uint8), P(C.c_uint8), vp, P(f64), u64, P(f64), u64,
+                          P(C.c_uint8), P(C.c_uint8), P(f64), P(i64)]),
     "ref_problem_quadrature": (i32, [vp]),
     "ref_peak": (i32, [P(f64), u64, P(f64), u64, P(f64), u64, P(C.c_uint8), P(C.c_uint8), P(i32), P(u64), P(f64), P(f64)]),
     "ref_nodal_rotations": (i32, [vp, P(f64), P(f64), P(i64)]),
@@ -367,6 +369,35 @@ def pcg_jacobi(rs, cols, vals, rhs, tolerance=1e-9, slice_iterations=0):
     lib().ref_pcg_jacobi(n, ptr(rs, i64), ptr(cols, u32), ptr(vals, f64), ptr(rhs, f64), tolerance, slice_iterations,
                          ptr(x, f64), C.byref(r))
     return x, r
+
+
+def compare(coarse, fine, inside=None):
+    """MeshComparison::between (comparison.rs:101-141) in the oracle. Each side is (Problem,
+    nodal_values (nodes, 7), point_values (elements, points, 10)); `inside` is evaluated here into
+    masks. A dict with status (0, or the refusal's status code) and per side the peak and counts."""
+    sides = []
+    for problem, nodal, points in (coarse, fine):
+        nodal = np.ascontiguousarray(nodal, dtype=np.float64)
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 10)
+        x = problem.nodes()
+        nm = None if inside is None else np.ascontiguousarray([bool(inside(p)) for p in x], dtype=np.uint8)
+        pm = None if inside is None else np.ascontiguousarray([bool(inside(p[:3])) for p in pts], dtype=np.uint8)
+        vm = np.ascontiguousarray(nodal[:, 6])
+        sides.append((problem, vm, pts, nm, pm))
+    values = np.zeros(13)
+    integers = np.zeros(8, dtype=np.int64)
+    u8 = C.c_uint8
+    args = []
+    for problem, vm, pts, nm, pm in sides:
+        args += [problem._p, ptr(vm, f64), 1, ptr(pts, f64), pts.shape[0], None if nm is None else ptr(nm, u8),
+                 None if pm is None else ptr(pm, u8)]
+    status = lib().ref_compare(*args, ptr(values, f64), ptr(integers, i64))
+    out = dict(status=status, a=int(integers[6]), b=int(integers[7]), value=values[12], peak_change=values[10],
+               peak_distance=values[11])
+    for w, name in enumerate(("coarse", "fine")):
+        out[name] = dict(von_mises=values[5 * w], position=values[5 * w + 1:5 * w + 4].copy(), volume=values[5 * w + 4],
+                         elements=int(integers[3 * w]), at_point=bool(integers[3 * w + 1]), index=int(integers[3 * w + 2]))
+    return out
 
 
 def peak(positions, nodal_von_mises, point_values, node_inside=None, point_inside=None):
