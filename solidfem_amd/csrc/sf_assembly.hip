@@ -402,46 +402,97 @@ __global__ void recovery_worst_kernel(size_t nodes, const uint32_t *__restrict__
 // PatchFit::fit (patch.rs:35-119) at each vertex over its incident elements' points in element
 // order; a patch of more than one material is left unfitted. fit: flag (1 fitted, 0 flat, -1 more
 // than one material), centre 3, scale, coefficients 4 x 6 (29 values).
+//
+// One wavefront of 64 threads per vertex. The material test and the scale (a maximum, exact in any
+// order) are reduced over the wavefront. The normal equations are then formed in chunks of 64
+// samples: each lane forms one sample's offsets into LDS, and lane e of the first 40 owns entry e
+// (16 of the matrix, 24 of the right-hand side), adding that entry's products over the chunk in
+// sample order -- each entry is the same running sum, from zero in sample order, as the serial
+// loop forms. Lane 0 solves the 4 x 4 system as the serial loop does.
+constexpr int FIT_LANES = 64;
+
 template <int NQ>
-__global__ void recovery_fit_kernel(size_t first, size_t count, const double *__restrict__ nodes, const uint32_t *__restrict__ starts,
-                                    const uint32_t *__restrict__ elems, const double *__restrict__ points,
-                                    const double *__restrict__ material, double *__restrict__ fit) {
-    const size_t local = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (local >= count) return;
-    const size_t v = first + local;
+__global__ __launch_bounds__(FIT_LANES) void recovery_fit_kernel(size_t first, size_t count, const double *__restrict__ nodes,
+                                                                 const uint32_t *__restrict__ starts, const uint32_t *__restrict__ elems,
+                                                                 const double *__restrict__ points,
+                                                                 const double *__restrict__ material, double *__restrict__ fit) {
+    if ((size_t)blockIdx.x >= count) return;
+    const size_t v = first + blockIdx.x;
+    const int lane = threadIdx.x;
     double *f = fit + v * 29;
-    f[0] = 0.0;
     const double c[3] = {nodes[v * 3], nodes[v * 3 + 1], nodes[v * 3 + 2]};
     const uint32_t k0 = starts[v], k1 = starts[v + 1];
-    if (k0 == k1) return;
-    const double e0 = material[(size_t)elems[k0] * NQ * 2], nu0 = material[(size_t)elems[k0] * NQ * 2 + 1];
+    const uint32_t samples = (k1 - k0) * (uint32_t)NQ;
+    if (samples == 0) {
+        if (lane == 0) f[0] = 0.0;
+        return;
+    }
+    auto point_of = [&](uint32_t sample) { return (size_t)elems[k0 + sample / NQ] * NQ + sample % NQ; };
+    // the material test and the scale pass
+    const size_t p0 = (size_t)elems[k0] * NQ;
+    const long long e0 = __double_as_longlong(material[p0 * 2]), nu0 = __double_as_longlong(material[p0 * 2 + 1]);
+    int mixed = 0;
     double scale = 0.0;
-    for (uint32_t k = k0; k < k1; ++k)
-        for (int q = 0; q < NQ; ++q) {
-            const size_t p = (size_t)elems[k] * NQ + q;
-            if (__double_as_longlong(material[p * 2]) != __double_as_longlong(e0) ||
-                __double_as_longlong(material[p * 2 + 1]) != __double_as_longlong(nu0)) {
-                f[0] = -1.0;  // mixed materials
-                return;
-            }
-            for (int a = 0; a < 3; ++a) scale = fmax(scale, fabs(points[p * 10 + a] - c[a]));
+    for (uint32_t sample = lane; sample < samples; sample += FIT_LANES) {
+        const size_t p = point_of(sample);
+        if (__double_as_longlong(material[p * 2]) != e0 || __double_as_longlong(material[p * 2 + 1]) != nu0) mixed = 1;
+        for (int a = 0; a < 3; ++a) scale = fmax(scale, fabs(points[p * 10 + a] - c[a]));
+    }
+    for (int off = FIT_LANES / 2; off > 0; off >>= 1) {
+        mixed |= __shfl_xor(mixed, off, FIT_LANES);
+        scale = fmax(scale, __shfl_xor(scale, off, FIT_LANES));
+    }
+    if (mixed || scale == 0.0) {
+        if (lane == 0) f[0] = mixed ? -1.0 : 0.0;
+        return;
+    }
+    // the normal equations, entry e of lane e: m[i][j] for e < 16, rhs[i][c] for 16 <= e < 40
+    __shared__ double offsets[FIT_LANES][4], stresses[FIT_LANES][6];
+    const int ei = lane < 16 ? lane / 4 : (lane - 16) / 6;
+    const int ej = lane < 16 ? lane % 4 : (lane - 16) % 6;
+    double entry = 0.0;
+    for (uint32_t base = 0; base < samples; base += FIT_LANES) {
+        const uint32_t sample = base + (uint32_t)lane;
+        if (sample < samples) {
+            const double *pt = points + point_of(sample) * 10;
+            offsets[lane][0] = 1.0;
+            for (int a = 0; a < 3; ++a) offsets[lane][1 + a] = (pt[a] - c[a]) / scale;
+            for (int k = 0; k < 6; ++k) stresses[lane][k] = pt[3 + k];
         }
-    if (scale == 0.0) return;
-    double m[4][4] = {}, rhs[4][6] = {};
-    for (uint32_t k = k0; k < k1; ++k)
-        for (int q = 0; q < NQ; ++q) {
-            const double *pt = points + ((size_t)elems[k] * NQ + q) * 10;
-            const double t[4] = {1.0, (pt[0] - c[0]) / scale, (pt[1] - c[1]) / scale, (pt[2] - c[2]) / scale};
-            for (int i = 0; i < 4; ++i) {
-                for (int j = 0; j < 4; ++j) m[i][j] += t[i] * t[j];
-                for (int cc = 0; cc < 6; ++cc) rhs[i][cc] += t[i] * pt[3 + cc];
+        __syncthreads();
+        const uint32_t chunk = min((uint32_t)FIT_LANES, samples - base);
+        if (lane < 40) {
+            // eight samples' factors are read ahead of their additions, which stay in order
+            const double *second = lane < 16 ? &offsets[0][ej] : &stresses[0][ej];
+            const int stride = lane < 16 ? 4 : 6;
+            uint32_t q = 0;
+            for (; q + 8 <= chunk; q += 8) {
+                double a[8], b[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    a[u] = offsets[q + u][ei];
+                    b[u] = second[(q + u) * stride];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) entry += a[u] * b[u];
             }
+            for (; q < chunk; ++q) entry += offsets[q][ei] * second[q * stride];
         }
+        __syncthreads();
+    }
+    __shared__ double m[4][4], rhs[4][6];
+    if (lane < 16) m[ei][ej] = entry;
+    else if (lane < 40) rhs[ei][ej] = entry;
+    __syncthreads();
+    if (lane != 0) return;
     for (int column = 0; column < 4; ++column) {
         int pivot = column;
         for (int row = column + 1; row < 4; ++row)
             if (fabs(m[row][column]) > fabs(m[pivot][column])) pivot = row;
-        if (fabs(m[pivot][column]) <= 1e-12) return;
+        if (fabs(m[pivot][column]) <= 1e-12) {
+            f[0] = 0.0;
+            return;
+        }
         for (int k = 0; k < 4; ++k) {
             const double x = m[column][k];
             m[column][k] = m[pivot][k];
@@ -771,7 +822,9 @@ DeviceRecovery::DeviceRecovery(const Model &model, const double *u, bool converg
       field_(model.field),
       converged_(converged),
       u_(u, u + 3 * model.mesh.nodes.size()) {
+    StageClock clock;
     const Incidence in = incidence_of(model.mesh, true);
+    clock.lap("recovery: incidence lists (host)");
     xyz_.upload(positions_of(model.mesh), s);
     u_d_.upload(u_, s);
     tets_.upload(tets_of(model.mesh), s);
@@ -781,6 +834,7 @@ DeviceRecovery::DeviceRecovery(const Model &model, const double *u, bool converg
     vat_.upload(in.vert_at, s);
     ends_.upload(in.ends.empty() ? std::vector<uint32_t>{0, 0} : in.ends, s);
     nearest_.upload(nearest_points(nq_), s);
+    clock.lap("recovery: uploads");
     points_.resize(elements_ * nq_ * 10);
     material_.resize(elements_ * nq_ * 2);
     nodevals_.resize(elements_ * 10 * 7);
@@ -827,11 +881,11 @@ void DeviceRecovery::ensure_worst() {
 
 void DeviceRecovery::run_fits(size_t count) {
     if (nq_ == 14)
-        hipLaunchKernelGGL(recovery_fit_kernel<14>, grid_of(count), dim3(256), 0, s_, next_vertex_, count, (const double *)xyz_.ptr,
+        hipLaunchKernelGGL(recovery_fit_kernel<14>, dim3((unsigned)count), dim3(FIT_LANES), 0, s_, next_vertex_, count, (const double *)xyz_.ptr,
                            (const uint32_t *)vstart_.ptr, (const uint32_t *)vat_.ptr, (const double *)points_.ptr,
                            (const double *)material_.ptr, fit_.ptr);
     else
-        hipLaunchKernelGGL(recovery_fit_kernel<4>, grid_of(count), dim3(256), 0, s_, next_vertex_, count, (const double *)xyz_.ptr,
+        hipLaunchKernelGGL(recovery_fit_kernel<4>, dim3((unsigned)count), dim3(FIT_LANES), 0, s_, next_vertex_, count, (const double *)xyz_.ptr,
                            (const uint32_t *)vstart_.ptr, (const uint32_t *)vat_.ptr, (const double *)points_.ptr,
                            (const double *)material_.ptr, fit_.ptr);
     SF_HIP(hipGetLastError());
@@ -883,15 +937,21 @@ void DeviceRecovery::field(double *point_values, double *nodal_values) {
         SF_HIP(hipGetLastError());
         nodal_done_ = true;
     }
+    StageClock clock;
+    SF_HIP(hipStreamSynchronize(s_));
+    clock.lap("recovery: nodal field");
     if (point_values) points_.download(point_values, elements_ * nq_ * 10, s_);
     if (nodal_values) nodal_.download(nodal_values, nodes_ * 7, s_);
     SF_HIP(hipStreamSynchronize(s_));
+    clock.lap("recovery: downloads");
 }
 
 void recover_stress(const Model &model, const double *u, double *point_values, double *nodal_values, hipStream_t s) {
     if (model.mesh.tets.empty()) return;
     DeviceRecovery r(model, u, true, s);
+    StageClock clock;
     r.advance(std::numeric_limits<size_t>::max(), nullptr);
+    clock.lap("recovery: elements and fits");
     r.field(point_values, nodal_values);
 }
 
