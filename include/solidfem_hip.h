@@ -334,6 +334,32 @@ sf_status sf_model_element_stiffness(sf_context *ctx, const sf_model *model, uin
 sf_status sf_assemble_stiffness(sf_context *ctx, const sf_model *model, sf_cancel_fn cancel, void *user,
                                 sf_csr **out);
 
+/* ---- element functions (crates/solidfem-elastic/src/lib.rs:56-71), on the host, one point at a
+ * time: the operations the device kernels apply to every element, for a caller's own element work.
+ * Nodes and displacements are 10 x 3 in the Tet10 node order. */
+/* shape_values, shape_derivatives (shape.rs:35-67) at natural (r, s, t): 10 values; 10 x 3
+ * derivatives, out[3 a + axis] */
+void sf_shape_values(const double natural[3], double out[10]);
+void sf_shape_derivatives(const double natural[3], double out[30]);
+/* Quadrature::points (quadrature.rs:44-77) of the 4- or 14-point rule: r, s, t and weight per
+ * point into `points` (may be NULL); returns the count, 0 for another rule. */
+uint64_t sf_quadrature_points(int32_t rule, double *points);
+/* PointGeometry (element.rs:14-23): physical shape gradients (10 x 3), det J, position */
+typedef struct sf_geometry {
+    double gradients[30];
+    double det_j;
+    double position[3];
+} sf_geometry;
+/* point_geometry (element.rs:25-80): SF_ERR_MODEL_INVERTED_JACOBIAN (det_j and position still
+ * written) when det J is not positive or NaN */
+sf_status sf_point_geometry(const double nodes[30], const double natural[3], sf_geometry *out);
+/* strain_at (element.rs:160-171): [xx, yy, zz, 2xy, 2yz, 2zx] */
+void sf_strain_at(const sf_geometry *geometry, const double displacements[30], double strain[6]);
+/* stress_at (element.rs:174-186): [xx, yy, zz, xy, yz, zx] of an isotropic material */
+void sf_stress_at(double youngs_modulus, double poisson_ratio, const double strain[6], double stress[6]);
+/* von_mises (element.rs:189-192) */
+double sf_von_mises(const double stress[6]);
+
 /* The numbered problem of StudyBuilder::new (study.rs:436-449, assembly.rs:55-81) on the host:
  * free map (node -> free index or UINT32_MAX), restricted right-hand side, the free stiffness's
  * block pattern (block row starts, block columns) and rigid-body modes (rigid_modes.rs:36-83).

@@ -1256,6 +1256,74 @@ def peak_stress(positions, nodal_von_mises, point_values, inside: Optional[Calla
     return PeakStress(out.von_mises, np.array(out.position[:]), bool(out.at_point), out.index)
 
 
+def _doubles(values, count):
+    a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+    if a.size != count:
+        raise ValueError(f"{count} values expected")
+    return a
+
+
+def shape_values(natural) -> np.ndarray:
+    """shape_values (shape.rs:35-45): the 10 Tet10 shape functions at natural (r, s, t)."""
+    out = np.empty(10)
+    library().sf_shape_values(_ptr(_doubles(natural, 3), C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def shape_derivatives(natural) -> np.ndarray:
+    """shape_derivatives (shape.rs:47-67): (10, 3)."""
+    out = np.empty((10, 3))
+    library().sf_shape_derivatives(_ptr(_doubles(natural, 3), C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def quadrature_points(rule: int) -> np.ndarray:
+    """Quadrature::points (quadrature.rs:44-77) of the 4- or 14-point rule: (points, 4) rows of
+    r, s, t and the weight."""
+    n = library().sf_quadrature_points(rule, None)
+    if n == 0:
+        raise ValueError("the rules have 4 or 14 points")
+    out = np.empty((n, 4))
+    library().sf_quadrature_points(rule, _ptr(out, C.c_double))
+    return out
+
+
+@dataclass
+class PointGeometry:
+    """PointGeometry (element.rs:14-23)."""
+    gradients: np.ndarray  # (10, 3)
+    det_j: float
+    position: np.ndarray
+    _raw: object = None
+
+
+def point_geometry(nodes, natural) -> PointGeometry:
+    """point_geometry (element.rs:25-80) of a Tet10 (nodes (10, 3)); ModelError 45 when inverted."""
+    g = _capi.Geometry()
+    _raise(library().sf_point_geometry(_ptr(_doubles(nodes, 30), C.c_double), _ptr(_doubles(natural, 3), C.c_double),
+                                       C.byref(g)))
+    return PointGeometry(np.array(g.gradients[:]).reshape(10, 3), g.det_j, np.array(g.position[:]), g)
+
+
+def strain_at(geometry: PointGeometry, displacements) -> np.ndarray:
+    """strain_at (element.rs:160-171): [xx, yy, zz, 2xy, 2yz, 2zx]."""
+    out = np.empty(6)
+    library().sf_strain_at(C.byref(geometry._raw), _ptr(_doubles(displacements, 30), C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def stress_at(youngs_modulus: float, poisson_ratio: float, strain) -> np.ndarray:
+    """stress_at (element.rs:174-186): [xx, yy, zz, xy, yz, zx]."""
+    out = np.empty(6)
+    library().sf_stress_at(youngs_modulus, poisson_ratio, _ptr(_doubles(strain, 6), C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def von_mises(stress) -> float:
+    """von_mises (element.rs:189-192)."""
+    return library().sf_von_mises(_ptr(_doubles(stress, 6), C.c_double))
+
+
 class Verdict(enum.IntEnum):
     """Verdict (comparison.rs:68-80)."""
     SETTLED = 0

@@ -102,6 +102,10 @@ class Peak(C.Structure):
     _fields_ = [("found", i32), ("at_point", i32), ("index", u64), ("von_mises", f64), ("position", f64 * 3)]
 
 
+class Geometry(C.Structure):
+    _fields_ = [("gradients", f64 * 30), ("det_j", f64), ("position", f64 * 3)]
+
+
 class ComparedSolve(C.Structure):
     _fields_ = [("mesh", vp), ("nodal_von_mises", P(f64)), ("nodes", u64), ("nodal_stride", u64),
                 ("point_values", P(f64)), ("points", u64)]
@@ -262,6 +266,13 @@ SIGNATURES = {
     "sf_error_estimation_elements_done": (u64, [vp]),
     "sf_error_estimation_finish": (i32, [vp, P(ErrorEstimate), P(f64)]),
     "sf_error_estimation_destroy": (None, [vp]),
+    "sf_shape_values": (None, [P(f64), P(f64)]),
+    "sf_shape_derivatives": (None, [P(f64), P(f64)]),
+    "sf_quadrature_points": (u64, [i32, P(f64)]),
+    "sf_point_geometry": (i32, [P(f64), P(f64), P(Geometry)]),
+    "sf_strain_at": (None, [P(Geometry), P(f64), P(f64)]),
+    "sf_stress_at": (None, [f64, f64, P(f64), P(f64)]),
+    "sf_von_mises": (f64, [P(f64)]),
     "sf_mesh_comparison_between": (i32, [P(ComparedSolve), P(ComparedSolve), REGION_FN, vp, P(MeshComparison)]),
     "sf_mesh_comparison_element_ratio": (f64, [P(MeshComparison)]),
     "sf_mesh_comparison_peak_element_ratio": (f64, [P(MeshComparison)]),

@@ -19,7 +19,7 @@ CSRC = PACKAGE / "csrc"
 OUT_DIR = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libsolidfem_hip.so"
 
-SOURCES = ["sf_mesh.cpp", "sf_setup.cpp", "sf_device.hip", "sf_batch.hip", "sf_assembly.hip", "sf_setup_device.hip", "sf_serve.cpp", "sf_noalloc.hip", "sf_capi.cpp"]
+SOURCES = ["sf_mesh.cpp", "sf_element.cpp", "sf_setup.cpp", "sf_device.hip", "sf_batch.hip", "sf_assembly.hip", "sf_setup_device.hip", "sf_serve.cpp", "sf_noalloc.hip", "sf_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
