@@ -214,7 +214,7 @@ typedef struct sf_build_progress {
  * sf_model_fix_node, sf_model_prescribe_displacement and sf_model_add_traction); any other axis
  * outside 0..2 is refused with SF_ERR_MODEL_AXIS_OUT_OF_RANGE. */
 enum { SF_TRACTION_UNIFORM = 0, SF_TRACTION_RANDOM = 1, SF_TRACTION_TORSION = 2 };
-enum { SF_MATERIAL_UNIFORM = 0, SF_MATERIAL_SHELL_CORE = 1 };
+enum { SF_MATERIAL_UNIFORM = 0, SF_MATERIAL_SHELL_CORE = 1, SF_MATERIAL_POINTS = 2 };
 typedef struct sf_workload {
     double origin[3];
     double size[3];
@@ -317,6 +317,16 @@ sf_status sf_model_add_nodal_force(sf_model *model, uint32_t node, const double 
 sf_status sf_model_prescribe_displacement(sf_model *model, uint32_t node, uint32_t axis, double value);
 /* Model::add_traction, study.rs:101-104; face = (tet, face, 6 nodes) as from boundary_faces */
 sf_status sf_model_add_traction(sf_model *model, const uint32_t face[8], const double traction[3]);
+/* A caller's MaterialField (material.rs:35-50): `at` writes the material (E, nu) at a physical
+ * position. */
+typedef void (*sf_material_fn)(const double position[3], double material[2], void *user);
+/* Model::new with a caller's MaterialField: `at` is called once for every integration point of
+ * the 4-point and then of the 14-point rule (element by element, points in rule order, at the
+ * point's position as point_geometry forms it), and the model keeps the values (SF_MATERIAL_POINTS)
+ * for its assembly, recovery and error estimate, whichever rule it solves with. Values are checked
+ * where the reference checks them: an invalid material fails the assembly with
+ * SF_ERR_MODEL_INVALID_MATERIAL. Replaces the model's material. */
+sf_status sf_model_set_material_field(sf_model *model, sf_material_fn at, void *user);
 /* A structured box model (mesh, clamp, tractions, material) from a workload description. */
 sf_status sf_model_from_workload(const sf_workload *workload, sf_model **out);
 void sf_model_destroy(sf_model *model);

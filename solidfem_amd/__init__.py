@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import enum
+import math
 from dataclasses import dataclass, field
 from typing import Callable, Optional
 
@@ -359,6 +360,25 @@ class Model(_Handle):
     def add_traction(self, face, traction):
         _raise(library().sf_model_add_traction(self._ptr, (C.c_uint32 * 8)(*[int(v) for v in face]),
                                                (C.c_double * 3)(*traction)))
+
+    def set_material_field(self, at: Callable[[np.ndarray], tuple]):
+        """Model::new with a caller's MaterialField (material.rs:35-50): `at(position)` gives
+        (youngs_modulus, poisson_ratio); it is evaluated once at every integration point of both
+        rules, and the model keeps the values."""
+        failure = []
+
+        def field(p, out, _user):
+            try:
+                e, nu = at(np.array(p[:3]))
+                out[0], out[1] = float(e), float(nu)
+            except BaseException as exc:  # the callback cannot raise through C
+                failure.append(exc)
+                out[0], out[1] = math.nan, math.nan
+
+        status = library().sf_model_set_material_field(self._ptr, _capi.MATERIAL_FN(field), None)
+        if failure:
+            raise failure[0]
+        _raise(status)
 
     def set_shell_core(self, shell_depth: float, core_youngs: float, core_poisson: float):
         _raise(library().sf_model_set_shell_core(self._ptr, shell_depth, core_youngs, core_poisson))

@@ -17,6 +17,7 @@ vp = C.c_void_p
 
 CANCEL_FN = C.CFUNCTYPE(i32, vp)
 REGION_FN = C.CFUNCTYPE(i32, P(f64), vp)
+MATERIAL_FN = C.CFUNCTYPE(None, P(f64), P(f64), vp)
 APPLY_FN = C.CFUNCTYPE(i32, P(f64), P(f64), u64, vp, vp)
 
 
@@ -266,6 +267,7 @@ SIGNATURES = {
     "sf_error_estimation_elements_done": (u64, [vp]),
     "sf_error_estimation_finish": (i32, [vp, P(ErrorEstimate), P(f64)]),
     "sf_error_estimation_destroy": (None, [vp]),
+    "sf_model_set_material_field": (i32, [vp, MATERIAL_FN, vp]),
     "sf_shape_values": (None, [P(f64), P(f64)]),
     "sf_shape_derivatives": (None, [P(f64), P(f64)]),
     "sf_quadrature_points": (u64, [i32, P(f64)]),

@@ -46,6 +46,8 @@ struct FieldArgs {
     double e, nu, core_e, core_nu, depth;
     double lo[3], hi[3];
     int tr0, tr1;
+    const double *table;  // SF_MATERIAL_POINTS: (E, nu) per (element, point) of a table_nq-point rule
+    int table_nq;
 };
 
 // d(shape a)/d(natural axis) at the point (crates/solidfem-elastic/src/shape.rs:46-67)
@@ -129,11 +131,14 @@ __global__ __launch_bounds__(256) void element_kernel(size_t first, size_t count
             const double det = j[0] * cof[w][0] + j[1] * cof[w][3] + j[2] * cof[w][6];
             if (isnan(det) || det <= 0.0) {
                 bad[w] = 1;
-                atomicMin(&fault[0], (int32_t)e);
+                atomicMin(&fault[0], (int32_t)(e * 64 + (size_t)q * 2));  // element, point, inverted
             }
             // material at the point (material.rs:24-50)
             double E = field.e, nu = field.nu;
-            if (field.kind == SF_MATERIAL_SHELL_CORE) {
+            if (field.kind == SF_MATERIAL_POINTS) {
+                E = field.table[(e * NQ + q) * 2];
+                nu = field.table[(e * NQ + q) * 2 + 1];
+            } else if (field.kind == SF_MATERIAL_SHELL_CORE) {
                 const int axes[2] = {field.tr0, field.tr1};
                 double depth = 1e300;
                 for (int s2 = 0; s2 < 2; ++s2) {
@@ -148,7 +153,7 @@ __global__ __launch_bounds__(256) void element_kernel(size_t first, size_t count
             const bool valid = isfinite(E) && E > 0.0 && nu > -1.0 && nu < 0.5;
             if (!valid) {
                 bad[w] = 2;
-                atomicMin(&fault[0], (int32_t)e);
+                atomicMin(&fault[0], (int32_t)(e * 64 + (size_t)q * 2 + 1));  // element, point, material
             }
             mat[w][0] = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
             mat[w][1] = E / (2.0 * (1.0 + nu));
@@ -268,11 +273,15 @@ __device__ bool thread_geometry(const double xe[10][3], const double nat[3], dou
     return true;
 }
 
-// MaterialField::at (material.rs:35-50), as element_kernel evaluates it
-__device__ void material_at(const FieldArgs &field, const double pos[3], double &E, double &nu) {
+// MaterialField::at (material.rs:35-50), as element_kernel evaluates it, at point q of element e
+// (a caller's field is read from its table of the same rule)
+__device__ void material_at(const FieldArgs &field, const double pos[3], size_t e, int q, double &E, double &nu) {
     E = field.e;
     nu = field.nu;
-    if (field.kind == SF_MATERIAL_SHELL_CORE) {
+    if (field.kind == SF_MATERIAL_POINTS) {
+        E = field.table[(e * (size_t)field.table_nq + (size_t)q) * 2];
+        nu = field.table[(e * (size_t)field.table_nq + (size_t)q) * 2 + 1];
+    } else if (field.kind == SF_MATERIAL_SHELL_CORE) {
         const int axes[2] = {field.tr0, field.tr1};
         double depth = 1e300;
         for (int s2 = 0; s2 < 2; ++s2) {
@@ -338,7 +347,7 @@ __global__ __launch_bounds__(256) void recovery_points_kernel(size_t first, size
     double E, nu;
     bool ok = thread_geometry(xe, nat, grad, pos);
     if (ok) {
-        material_at(field, pos, E, nu);
+        material_at(field, pos, e, q, E, nu);
         ok = isfinite(E) && E > 0.0 && nu > -1.0 && nu < 0.5;
     }
     if (!ok) {
@@ -631,7 +640,7 @@ __global__ __launch_bounds__(256) void estimate_points_kernel(size_t first, size
         return;
     }
     double E, nu;
-    material_at(field, pos, E, nu);
+    material_at(field, pos, e, q, E, nu);
     if (!(isfinite(E) && E > 0.0 && nu > -1.0 && nu < 0.5)) {
         atomicMin(fault, (unsigned long long)e * 32 + 5 + 2 * (unsigned long long)q);
         return;
@@ -665,8 +674,13 @@ __global__ void estimate_sums_kernel(size_t count, const double *__restrict__ te
     sums[slot * 2 + 1] = energy;
 }
 
-FieldArgs field_args(const MaterialField &f) {
+// The kernels' view of a field; a caller's field (SF_MATERIAL_POINTS) reads `table`, the device copy
+// of its values at the points of the `nq`-point rule.
+FieldArgs field_args(const MaterialField &f, const double *table = nullptr, int nq = 4) {
+    if (f.kind == SF_MATERIAL_POINTS && !table) throw Error(SF_ERR_INTERNAL, "a caller's material field has no device table");
     FieldArgs a{};
+    a.table = table;
+    a.table_nq = nq;
     a.kind = f.kind;
     a.e = f.e;
     a.nu = f.nu;
@@ -834,6 +848,7 @@ DeviceRecovery::DeviceRecovery(const Model &model, const double *u, bool converg
     vat_.upload(in.vert_at, s);
     ends_.upload(in.ends.empty() ? std::vector<uint32_t>{0, 0} : in.ends, s);
     nearest_.upload(nearest_points(nq_), s);
+    if (field_.kind == SF_MATERIAL_POINTS) field_table_.upload(field_.points(nq_), s);
     clock.lap("recovery: uploads");
     points_.resize(elements_ * nq_ * 10);
     material_.resize(elements_ * nq_ * 2);
@@ -847,7 +862,7 @@ DeviceRecovery::DeviceRecovery(const Model &model, const double *u, bool converg
 void DeviceRecovery::run_elements(size_t count) {
     const int32_t none = NO_FAULT;
     fault_.upload(&none, 1, s_);
-    const FieldArgs fa = field_args(field_);
+    const FieldArgs fa = field_args(field_, field_table_.ptr, nq_);
     if (nq_ == 14) {
         hipLaunchKernelGGL(recovery_points_kernel<14>, grid_of(count * 14), dim3(256), 0, s_, next_, count, xyz_.ptr, tets_.ptr,
                            u_d_.ptr, fa, points_.ptr, material_.ptr, fault_.ptr);
@@ -967,6 +982,7 @@ DeviceErrorEstimation::DeviceErrorEstimation(const Model &model, DeviceRecovery 
     if (!matches()) return;
     xyz_.upload(positions_of(model.mesh), s);
     tets_.upload(tets_of(model.mesh), s);
+    if (field_.kind == SF_MATERIAL_POINTS) field_table_.upload(field_.points(14), s);
     element_errors_.reserve(elements_);
 }
 
@@ -983,7 +999,7 @@ void DeviceErrorEstimation::run(size_t count) {
         sums_.resize(std::min(SLICE, elements_) * 2);
         fault_.resize(1);
     }
-    const FieldArgs fa = field_args(field_);
+    const FieldArgs fa = field_args(field_, field_table_.ptr, 14);
     std::vector<double> sums;
     for (size_t done = 0; done < count;) {
         const size_t n = std::min(SLICE, count - done), first = next_;
@@ -1116,6 +1132,7 @@ void DeviceAssembly::prepare(const Model &model, const AssemblyPattern &pattern,
     if (elements >= (1u << 25)) throw Error(SF_ERR_INVALID_ARGUMENT, "too many elements for the assembly map");
     field = model.field;
     quadrature = model.quadrature;
+    if (field.kind == SF_MATERIAL_POINTS) field_table.upload(field.points(quadrature == 14 ? 14 : 4), s);
     ke.resize(std::min(elements, CHUNK_ELEMENTS) * 55 * 9);
     fault.resize(1);
     const int32_t none = 0x7fffffff;
@@ -1124,7 +1141,7 @@ void DeviceAssembly::prepare(const Model &model, const AssemblyPattern &pattern,
 }
 
 void DeviceAssembly::add_elements(size_t first, size_t last, double *values, size_t blocks, hipStream_t s) {
-    const FieldArgs fa = field_args(field);
+    const FieldArgs fa = field_args(field, field_table.ptr, quadrature == 14 ? 14 : 4);
     for (size_t c0 = first; c0 < last; c0 += CHUNK_ELEMENTS) {
         const size_t c1 = std::min(last, c0 + CHUNK_ELEMENTS);
         const size_t n = c1 - c0;
@@ -1145,12 +1162,15 @@ void DeviceAssembly::check_fault(hipStream_t s) {
     int32_t f = 0;
     fault.download(&f, 1, s);
     SF_HIP(hipStreamSynchronize(s));
-    if (f != 0x7fffffff)
-        throw Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element has an inverted point or an invalid material", f);
+    // the first refusal in element then point order, the geometry before the material at a point
+    // (element.rs:98-133): element * 64 + point * 2 + (1 for the material)
+    if (f == 0x7fffffff) return;
+    if (f % 2) throw Error(SF_ERR_MODEL_INVALID_MATERIAL, "an element has an invalid material at a point", f / 64, (f % 64) / 2);
+    throw Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element is inverted at a point", f / 64, (f % 64) / 2);
 }
 
 void DeviceAssembly::element_matrices(size_t first, size_t count, double *ke900, hipStream_t s) {
-    const FieldArgs fa = field_args(field);
+    const FieldArgs fa = field_args(field, field_table.ptr, quadrature == 14 ? 14 : 4);
     DeviceBuffer<double> out(std::max<size_t>(count, 1) * 55 * 9);
     const unsigned grid = (unsigned)((count + EPB - 1) / EPB);
     if (quadrature == 14)

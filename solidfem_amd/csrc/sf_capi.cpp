@@ -290,6 +290,36 @@ sf_status sf_model_create(const sf_quad_mesh *mesh, double e, double nu, sf_mode
         *out = m.release();
     });
 }
+sf_status sf_model_set_material_field(sf_model *model, sf_material_fn at, void *user) {
+    return guard([&] {
+        if (!model || !at) throw Error(SF_ERR_INVALID_ARGUMENT, "a material field needs a model and a function");
+        const QuadMesh &mesh = model->model.mesh;
+        auto table = [&](int32_t rule) {
+            double rule_points[14 * 4];
+            const uint64_t nq = sf_quadrature_points(rule, rule_points);
+            auto out = std::make_shared<std::vector<double>>(mesh.tets.size() * nq * 2);
+            for (size_t e = 0; e < mesh.tets.size(); ++e) {
+                double nodes[30];
+                for (int a = 0; a < 10; ++a)
+                    for (int k = 0; k < 3; ++k) nodes[3 * a + k] = mesh.nodes[mesh.tets[e][a]][k];
+                for (uint64_t q = 0; q < nq; ++q) {
+                    // the position of point_geometry (an inverted point is refused later, where
+                    // the reference refuses it, so its position is evaluated all the same)
+                    sf_geometry g;
+                    (void)sf_point_geometry(nodes, rule_points + 4 * q, &g);
+                    at(g.position, out->data() + (e * nq + q) * 2, user);
+                }
+            }
+            return std::shared_ptr<const std::vector<double>>(out);
+        };
+        auto points4 = table(4);
+        auto points14 = table(14);
+        auto &f = model->model.field;
+        f.kind = SF_MATERIAL_POINTS;
+        f.points4 = std::move(points4);
+        f.points14 = std::move(points14);
+    });
+}
 sf_status sf_model_set_shell_core(sf_model *model, double depth, double core_e, double core_nu) {
     return guard([&] {
         auto &f = model->model.field;

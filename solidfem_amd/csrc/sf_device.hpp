@@ -397,6 +397,7 @@ class DeviceRecovery {
     size_t next_ = 0, next_vertex_ = 0;
     bool worst_done_ = false, nodal_done_ = false;
     DeviceBuffer<double> xyz_, u_d_, points_, material_, nodevals_, worst_, fit_, nodal_;
+    DeviceBuffer<double> field_table_;  // a caller's field at the rule's points
     DeviceBuffer<uint32_t> tets_, nstart_, nat_, vstart_, vat_, ends_;
     DeviceBuffer<int8_t> nearest_;
     DeviceBuffer<int32_t> fault_;
@@ -430,6 +431,7 @@ class DeviceErrorEstimation {
     size_t elements_, nodes_, vertices_;
     MaterialField field_;
     DeviceBuffer<double> xyz_, recovered_, terms_, sums_;
+    DeviceBuffer<double> field_table_;  // a caller's field at the 14-point rule's points
     DeviceBuffer<uint32_t> tets_;
     DeviceBuffer<unsigned long long> fault_;
     size_t next_ = 0;
@@ -455,6 +457,7 @@ struct DeviceAssembly {
     DeviceBuffer<uint32_t> contributions;
     DeviceBuffer<double> ke;        // 55 upper blocks x 9 per element of a chunk
     DeviceBuffer<int32_t> fault;    // first failing element, point and kind
+    DeviceBuffer<double> field_table;  // a caller's field at the rule's points
     size_t elements = 0;
     MaterialField field;
     int quadrature = 4;

@@ -73,11 +73,15 @@ extern const int EDGE_PAIRS[6][2];
 // ---- model -------------------------------------------------------------------------------
 
 struct MaterialField {
-    // Uniform (E, nu) or a shell of depth `shell_depth` below the two long-face pairs with a core.
+    // Uniform (E, nu), a shell of depth `shell_depth` below the two long-face pairs with a core, or
+    // a caller's field evaluated at every point of both rules (SF_MATERIAL_POINTS).
     int kind = SF_MATERIAL_UNIFORM;
     double e = 1, nu = 0, core_e = 1, core_nu = 0, shell_depth = 0;
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     int transverse[2] = {1, 2};
+    // SF_MATERIAL_POINTS: (E, nu) per (element, point) of the 4- and the 14-point rule
+    std::shared_ptr<const std::vector<double>> points4, points14;
+    const std::vector<double> &points(int nq) const { return nq == 14 ? *points14 : *points4; }
 };
 
 struct Model {

@@ -523,6 +523,8 @@ Model model_from_workload(const sf_workload &w) {
     clock.lap("boundary faces");
     model.tractions = workload_tractions(model.mesh, boundary, w);
     clock.lap("tractions");
+    if (w.material_kind != SF_MATERIAL_UNIFORM && w.material_kind != SF_MATERIAL_SHELL_CORE)
+        throw Error(SF_ERR_INVALID_ARGUMENT, "a workload's material is uniform or a shell and core");
     model.field.kind = w.material_kind;
     model.field.e = w.youngs;
     model.field.nu = w.poisson;
