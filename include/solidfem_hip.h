@@ -552,8 +552,9 @@ sf_status sf_pcg_restart(sf_pcg *pcg, const double *new_rhs);
  *   nodal_values: per mesh node 7 values -- the recovered nodal stress (6), from the vertex patch
  *     fits, midside means and the worst incident value where a patch is not fitted, and its von
  *     Mises (1) -- Solution::nodal_stress and nodal_von_mises.
- * Either output may be NULL. An inverted element fails with SF_ERR_MODEL_INVERTED_JACOBIAN
- * (a = the first such element). */
+ * Either output may be NULL. The first element that fails, in element then point order, fails the
+ * call: SF_ERR_MODEL_INVERTED_JACOBIAN (a = the element, b = the integration point, or the node
+ * for the node values) or SF_ERR_MODEL_INVALID_MATERIAL (a = the element, b = the point). */
 sf_status sf_recover_stress(sf_context *ctx, const sf_model *model, const double *displacements, double *point_values,
                             double *nodal_values);
 /* The same for a study's current displacements (Recovery as StaticStudy drives it, study.rs). */
@@ -578,8 +579,8 @@ sf_status sf_recovery_create(sf_context *ctx, const sf_model *model, const doubl
 sf_status sf_static_study_begin_recovery(sf_static_study *study, sf_recovery **out);
 /* Recovery::advance (results.rs:477-536): up to max_work more units, an element, then a patch fit;
  * fits start only once every element is done. `cancel` (may be NULL) is read before each unit:
- * SF_ERR_MODEL_CANCELLED, with the units before it done. A faulting element fails with
- * SF_ERR_MODEL_INVERTED_JACOBIAN (a = the element), which stays the next to do. `finished`
+ * SF_ERR_MODEL_CANCELLED, with the units before it done. A faulting element fails as
+ * sf_recover_stress says (a = the element), and stays the next to do. `finished`
  * (may be NULL) is set to whether every unit is done. */
 sf_status sf_recovery_advance(sf_recovery *recovery, uint64_t max_work, sf_cancel_fn cancel, void *user, int32_t *finished);
 /* StaticStudy::continue_recovery (study.rs:275-292): sf_recovery_advance, its time added to the

@@ -345,13 +345,13 @@ __global__ __launch_bounds__(256) void recovery_points_kernel(size_t first, size
     const QPoint qp = (NQ == 14 ? RULE14 : RULE4)[q];
     const double nat[3] = {qp.r, qp.s, qp.t};
     double E, nu;
-    bool ok = thread_geometry(xe, nat, grad, pos);
-    if (ok) {
-        material_at(field, pos, e, q, E, nu);
-        ok = isfinite(E) && E > 0.0 && nu > -1.0 && nu < 0.5;
+    if (!thread_geometry(xe, nat, grad, pos)) {
+        atomicMin(&fault[0], (int32_t)(e * 64 + (size_t)q * 2));  // element, point, inverted
+        return;
     }
-    if (!ok) {
-        atomicMin(&fault[0], (int32_t)e);
+    material_at(field, pos, e, q, E, nu);
+    if (!(isfinite(E) && E > 0.0 && nu > -1.0 && nu < 0.5)) {
+        atomicMin(&fault[0], (int32_t)(e * 64 + (size_t)q * 2 + 1));  // element, point, material
         return;
     }
     double s[6];
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(256) void recovery_nodes_kernel(size_t first, size_
     load_element(nodes, tets, u, e, xe, ue);
     const double nat[3] = {NODE_NATURAL[n][0], NODE_NATURAL[n][1], NODE_NATURAL[n][2]};
     if (!thread_geometry(xe, nat, grad, pos)) {
-        atomicMin(&fault[0], (int32_t)e);
+        atomicMin(&fault[0], (int32_t)(e * 64 + 28 + (size_t)n * 2));  // element, node, inverted
         return;
     }
     const size_t mp = e * NQ + (size_t)nearest[n];
@@ -836,6 +836,7 @@ DeviceRecovery::DeviceRecovery(const Model &model, const double *u, bool converg
       field_(model.field),
       converged_(converged),
       u_(u, u + 3 * model.mesh.nodes.size()) {
+    if (elements_ >= (size_t(1) << 25)) throw Error(SF_ERR_INVALID_ARGUMENT, "too many elements for one recovery");
     StageClock clock;
     const Incidence in = incidence_of(model.mesh, true);
     clock.lap("recovery: incidence lists (host)");
@@ -879,9 +880,14 @@ void DeviceRecovery::run_elements(size_t count) {
     fault_.download(&fault, 1, s_);
     SF_HIP(hipStreamSynchronize(s_));
     if (fault != NO_FAULT) {
-        // the elements before the first faulting one are done; it stays the next to do
-        next_ = (size_t)fault;
-        throw Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element has an inverted point or an invalid material", fault);
+        // the elements before the first faulting one are done; it stays the next to do. The report
+        // is element * 64 + what: 2 q (+1 for the material) at integration point q, then 28 + 2 n
+        // at node n (the order of Recovery::advance's loops, results.rs:477-536)
+        const int64_t e = fault / 64, what = fault % 64;
+        next_ = (size_t)e;
+        if (what < 28 && what % 2)
+            throw Error(SF_ERR_MODEL_INVALID_MATERIAL, "an element has an invalid material at a point", e, what / 2);
+        throw Error(SF_ERR_MODEL_INVERTED_JACOBIAN, "an element is inverted at a point", e, what < 28 ? what / 2 : (what - 28) / 2);
     }
     next_ += count;
 }

@@ -131,3 +131,21 @@ def test_a_field_that_raises_is_reported():
 
     with pytest.raises(KeyError):
         model.set_material_field(broken)
+
+
+def test_a_recovery_refuses_an_invalid_material_at_its_element_and_point(ctx):
+    w = sf.Workload(size=(3.0, 2.0, 2.0), divisions=(3, 2, 2), clamp_axis=0, youngs=2100.0, poisson=0.3)
+    model = sf.Model.from_workload(w)
+    model.set_material_field(lambda p: (2100.0, 0.3) if p[0] < 2.0 else (2100.0, 0.6))
+    x, tets = model.mesh().nodes(), model.mesh().tets()
+    rule = sf.quadrature_points(4)
+    first = next((e, q) for e in range(len(tets)) for q in range(4)
+                 if sf.point_geometry(x[tets[e]], rule[q, :3]).position[0] >= 2.0)
+    u = np.zeros((len(x), 3))
+    with pytest.raises(sf.ModelError) as e:
+        model.recover(ctx, u)
+    assert e.value.status == INVALID_MATERIAL and (e.value.a, e.value.b) == first
+    recovery = model.begin_recovery(ctx, u, converged=True)
+    with pytest.raises(sf.ModelError) as e:
+        recovery.advance(2 ** 62)
+    assert e.value.status == INVALID_MATERIAL and recovery.elements_done() == first[0]
