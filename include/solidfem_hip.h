@@ -456,6 +456,9 @@ void sf_multigrid_destroy(sf_multigrid *mg);
 /* Multigrid::stats / precision, cycle.rs:319-330 */
 sf_status sf_multigrid_stats_get(const sf_multigrid *mg, sf_multigrid_stats *out);
 int32_t sf_multigrid_precision(const sf_multigrid *mg);
+/* Multigrid::threads (cycle.rs:310-316): the host threads the cycle's products run on -- one, the
+ * calling thread, which enqueues them on the device (the bits are the same as with any count). */
+uint64_t sf_multigrid_threads(const sf_multigrid *mg);
 /* Preconditioner::apply for Multigrid, cycle.rs:430-446: one V-cycle on the device, host arrays */
 sf_status sf_multigrid_apply(sf_multigrid *mg, const double *residual, double *output);
 /* One Chebyshev smoother application on `level` from a zero guess, cycle.rs:570-651 */

@@ -825,6 +825,10 @@ class Multigrid(_Handle, _Levels):
     def precision(self) -> Precision:
         return Precision(library().sf_multigrid_precision(self._ptr))
 
+    def threads(self) -> int:
+        """Multigrid::threads (cycle.rs:310-316): one host thread enqueues the device's products."""
+        return library().sf_multigrid_threads(self._ptr)
+
     def level_sizes(self):
         n = C.c_uint64()
         out = np.zeros(3 * 64, dtype=np.uint64)

@@ -222,6 +222,7 @@ SIGNATURES = {
     "sf_multigrid_destroy": (None, [vp]),
     "sf_multigrid_stats_get": (i32, [vp, P(MultigridStats)]),
     "sf_multigrid_precision": (i32, [vp]),
+    "sf_multigrid_threads": (u64, [vp]),
     "sf_multigrid_apply": (i32, [vp, P(f64), P(f64)]),
     "sf_multigrid_smooth": (i32, [vp, u64, P(f64), P(f64)]),
     "sf_multigrid_level_sizes": (i32, [vp, P(u64), u64, P(u64)]),

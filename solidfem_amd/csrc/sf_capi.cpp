@@ -800,6 +800,7 @@ sf_status sf_multigrid_stats_get(const sf_multigrid *mg, sf_multigrid_stats *out
     return guard([&] { *out = mg->mg->stats; });
 }
 int32_t sf_multigrid_precision(const sf_multigrid *mg) { return mg->mg->precision; }
+uint64_t sf_multigrid_threads(const sf_multigrid *mg) { return mg ? 1 : 0; }
 
 sf_status sf_multigrid_apply(sf_multigrid *mg, const double *r, double *z) {
     return guard([&] {
