@@ -100,7 +100,7 @@ def _ptr(array: np.ndarray, ctype):
 
 def _cancel(cancel: Optional[Callable[[], bool]]):
     if cancel is None:
-        return _capi.CANCEL_FN(lambda _u: 0)
+        return _capi.CANCEL_FN()  # NULL: nothing to poll, no call per unit of work
     return _capi.CANCEL_FN(lambda _u: 1 if cancel() else 0)
 
 

@@ -919,9 +919,13 @@ bool DeviceRecovery::advance(size_t max_work, const std::function<bool()> &cance
     // fits start only once every element is done; the cancel check is read once per unit.
     size_t budget = max_work, take = 0;
     bool cancelled = false;
-    while (next_ + take < elements_ && budget > 0) {
+    if (!cancel) {  // nothing to poll
+        take = std::min(budget, elements_ - next_);
+        budget -= take;
+    }
+    while (cancel && next_ + take < elements_ && budget > 0) {
         budget -= 1;
-        if (cancel && cancel()) {
+        if (cancel()) {
             cancelled = true;
             break;
         }
@@ -933,9 +937,13 @@ bool DeviceRecovery::advance(size_t max_work, const std::function<bool()> &cance
         // the worst incident value at a node reads every element's node values, all done now
         ensure_worst();
         take = 0;
-        while (next_vertex_ + take < vertices_ && budget > 0) {
+        if (!cancel) {
+            take = std::min(budget, vertices_ - next_vertex_);
+            budget -= take;
+        }
+        while (cancel && next_vertex_ + take < vertices_ && budget > 0) {
             budget -= 1;
-            if (cancel && cancel()) {
+            if (cancel()) {
                 cancelled = true;
                 break;
             }
@@ -986,9 +994,11 @@ DeviceErrorEstimation::DeviceErrorEstimation(const Model &model, DeviceRecovery 
     // ErrorEstimation::new (error_estimate.rs:94-103) refuses nothing; a recovery of another mesh
     // is refused by advance, so nothing of this mesh is needed then.
     if (!matches()) return;
+    StageClock clock;
     xyz_.upload(positions_of(model.mesh), s);
     tets_.upload(tets_of(model.mesh), s);
     if (field_.kind == SF_MATERIAL_POINTS) field_table_.upload(field_.points(14), s);
+    clock.lap("error estimate: mesh upload");
     element_errors_.reserve(elements_);
 }
 
@@ -1006,6 +1016,7 @@ void DeviceErrorEstimation::run(size_t count) {
         fault_.resize(1);
     }
     const FieldArgs fa = field_args(field_, field_table_.ptr, 14);
+    StageClock clock;
     std::vector<double> sums;
     for (size_t done = 0; done < count;) {
         const size_t n = std::min(SLICE, count - done), first = next_;
@@ -1034,6 +1045,7 @@ void DeviceErrorEstimation::run(size_t count) {
         next_ += good;
         if (fault != NO_ESTIMATE_FAULT) throw refusal(fault, first);
         done += n;
+        clock.lap("error estimate: a slice of elements");
     }
 }
 
@@ -1072,9 +1084,10 @@ bool DeviceErrorEstimation::advance(size_t max_elements, const std::function<boo
         throw Error(SF_ERR_MODEL_NOT_FINISHED, "the recovery is unfinished or of a solve that had not converged");
     size_t budget = max_elements, take = 0;
     bool cancelled = false;
-    while (next_ + take < elements_ && budget > 0) {
+    if (!cancel) take = std::min(budget, elements_ - next_);  // nothing to poll
+    while (cancel && next_ + take < elements_ && budget > 0) {
         budget -= 1;
-        if (cancel && cancel()) {
+        if (cancel()) {
             cancelled = true;
             break;
         }

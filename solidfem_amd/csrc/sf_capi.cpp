@@ -970,7 +970,9 @@ sf_status sf_static_study_begin_recovery(sf_static_study *study, sf_recovery **o
 sf_status sf_recovery_advance(sf_recovery *recovery, uint64_t max_work, sf_cancel_fn cancel, void *user, int32_t *finished) {
     return guard([&] {
         const Cancel c{cancel, user};
-        const bool done = recovery->recovery->advance((size_t)max_work, [&] { return c(); });
+        std::function<bool()> poll;
+        if (cancel) poll = [&] { return c(); };
+        const bool done = recovery->recovery->advance((size_t)max_work, poll);
         if (finished) *finished = done ? 1 : 0;
     });
 }
@@ -1027,7 +1029,9 @@ sf_status sf_error_estimation_advance(sf_error_estimation *estimation, uint64_t 
                                       int32_t *finished) {
     return guard([&] {
         const Cancel c{cancel, user};
-        const bool done = estimation->estimation->advance((size_t)max_elements, [&] { return c(); });
+        std::function<bool()> poll;
+        if (cancel) poll = [&] { return c(); };
+        const bool done = estimation->estimation->advance((size_t)max_elements, poll);
         if (finished) *finished = done ? 1 : 0;
     });
 }
