@@ -693,6 +693,33 @@ class CsrMatrix(_Handle):
                                _ptr(vals, C.c_double)))
         return rs, cols, vals
 
+    def nonzeros(self) -> int:
+        """CsrMatrix::nonzeros (csr.rs:79-82)."""
+        nnz = C.c_uint64()
+        _raise(library().sf_csr_read(self._ptr, None, C.byref(nnz), None, None, None))
+        return nnz.value
+
+    def row(self, row: int):
+        """CsrMatrix::row (csr.rs:157-161): (columns, values) of a scalar row."""
+        rs, cols, vals = self.read()
+        return cols[rs[row]:rs[row + 1]], vals[rs[row]:rs[row + 1]]
+
+    def get(self, row: int, column: int) -> float:
+        """CsrMatrix::get (csr.rs:99-110): the stored value, 0 where there is none."""
+        cols, vals = self.row(row)
+        at = np.searchsorted(cols, column)
+        return float(vals[at]) if at < len(cols) and cols[at] == column else 0.0
+
+    def diagonal(self) -> np.ndarray:
+        """CsrMatrix::diagonal (csr.rs:163-168)."""
+        rs, cols, vals = self.read()
+        out = np.zeros(len(rs) - 1)
+        for i in range(len(rs) - 1):
+            hit = np.nonzero(cols[rs[i]:rs[i + 1]] == i)[0]
+            if hit.size:
+                out[i] = vals[rs[i] + hit[0]]
+        return out
+
     def multiply(self, x) -> np.ndarray:
         xx = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty(self.size)
