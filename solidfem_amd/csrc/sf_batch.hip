@@ -13,7 +13,16 @@ using namespace dev;
 
 static inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
-std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M, bool product) {
+template <typename S>
+bool staged_candidate(const DeviceSliced<S> &a, uint32_t M, size_t vector_size) {
+    const bool long_rows = a.rows > 0 && a.blocks >= 40 * a.rows;
+    const bool forced = forced_variant().load(std::memory_order_relaxed) == STAGED_FORM;
+    return (long_rows || forced) && (size_t)a.rb * M <= 256 && (size_t)STAGE_BLOCKS * a.cb * M * vector_size <= 48 * 1024;
+}
+template bool staged_candidate<double>(const DeviceSliced<double> &, uint32_t, size_t);
+template bool staged_candidate<float>(const DeviceSliced<float> &, uint32_t, size_t);
+
+std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M, bool product, bool staged) {
     // Depths 1, 2, 4 of the lane-group kernels, NODE_FORM + depth for one wavefront per block
     // row, and the forms with two or four columns per lane where M allows. A lane holds BATCH
     // blocks of rb * cb values and BATCH * cb * CPL entries of x: keep that within registers.
@@ -33,6 +42,7 @@ std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M, 
         c.push_back(OCT_FORM + 1);
         c.push_back(OCT_FORM + 2);
     }
+    if (staged && product) c.push_back(STAGED_FORM);
     return c;
 }
 
@@ -77,6 +87,13 @@ static void spmm_shape(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, c
     else spmm_depth<RB, CB, 1>(a, M, x, y, b, epi, s, depth, node_form(batch));
 }
 
+template <int RB, int CB, typename S, typename V>
+static void spmm_staged(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s) {
+    const size_t lds = (size_t)STAGE_BLOCKS * CB * M * sizeof(V);
+    hipLaunchKernelGGL((spmm_staged_kernel<RB, CB, S, V>), dim3((unsigned)((size_t)a.slices * Shape<RB>::NPW)), dim3(256), lds, s,
+                       a.batch_view(), M, epi, x, y, b);
+}
+
 template <typename S, typename V>
 void launch_spmm(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch) {
     if (a.slices == 0 || M == 0) return;
@@ -89,11 +106,27 @@ void launch_spmm(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V
             SF_HIP(hipMemsetAsync(tx.ptr, 0, tx.bytes(), s));
             SF_HIP(hipMemsetAsync(ty.ptr, 0, ty.bytes(), s));
             tuned = tune_batch(s, [&](int depth) { launch_spmm<S, V>(a, M, tx.ptr, ty.ptr, ty.ptr, EPI_STORE, s, depth); },
-                               spmm_candidates(a.rb, a.cb, sizeof(S), M), "",
+                               spmm_candidates(a.rb, a.cb, sizeof(S), M, true, staged_candidate(a, M, sizeof(V))), "",
                                "batch product " + std::to_string(columns_log2(M)) + " " + shape_class(a) + "|" + std::to_string(sizeof(V)));
             SF_HIP(hipStreamSynchronize(s));
         }
         batch = tuned;
+    }
+    if (batch >= STAGED_FORM) {
+        // the staged form where it fits, else one column and four blocks in flight per lane
+        if (staged_candidate(a, M, sizeof(V))) {
+            if (a.rb == 3 && a.cb == 3) spmm_staged<3, 3>(a, M, x, y, b, epi, s);
+            else if (a.rb == 6 && a.cb == 6) spmm_staged<6, 6>(a, M, x, y, b, epi, s);
+            else if (a.rb == 3 && a.cb == 6) spmm_staged<3, 6>(a, M, x, y, b, epi, s);
+            else if (a.rb == 6 && a.cb == 3) spmm_staged<6, 3>(a, M, x, y, b, epi, s);
+            else batch = 4;
+            if (batch >= STAGED_FORM) {
+                SF_HIP(hipGetLastError());
+                return;
+            }
+        } else {
+            batch = 4;
+        }
     }
     if (a.rb == 3 && a.cb == 3) spmm_shape<3, 3>(a, M, x, y, b, epi, s, batch);
     else if (a.rb == 6 && a.cb == 6) spmm_shape<6, 6>(a, M, x, y, b, epi, s, batch);

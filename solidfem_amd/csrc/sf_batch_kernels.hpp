@@ -247,6 +247,62 @@ __device__ __forceinline__ uint32_t node_of_wave(uint32_t order) {
     return __builtin_amdgcn_readfirstlane(group_block(order) * (blockDim.x / WAVE) + threadIdx.x / WAVE);
 }
 
+// The same product with one workgroup per block row, for matrices of few, long rows (the stored
+// transposes of the coarse levels): the workgroup stages a chunk of STAGE_BLOCKS of the row's
+// blocks and the x entries they gather into LDS with loads that are all in flight at once, then
+// thread (p, m) adds the chunk's terms of row entry p and column m in the stored order -- a chunk
+// costs two round trips where a lane walking the row costs one per few blocks. Every sum is the
+// lane-group form's: from zero, blocks in order, block columns in order.
+constexpr int STAGE_BLOCKS = 32;
+template <int RB, int CB, typename S, typename V>
+__global__ __launch_bounds__(256) void spmm_staged_kernel(Sliced<S> a, uint32_t M, int epi, const V *__restrict__ x,
+                                                          V *__restrict__ y, const V *__restrict__ b) {
+    constexpr int NPW = Shape<RB>::NPW;
+    const uint32_t pos = blockIdx.x;
+    if (pos >= a.slices * (uint32_t)NPW) return;  // the whole workgroup leaves together
+    const uint32_t row = a.perm[pos];
+    if (row == NONE) return;
+    const uint32_t slice = pos / NPW;
+    const int j = (int)(pos - slice * NPW);
+    const uint32_t slot0 = a.slice_start[slice], len = a.row_len[pos];
+    const uint32_t *__restrict__ cp = a.cols + (size_t)slot0 * NPW + j;
+    const S *__restrict__ vp = a.vals + ((size_t)slot0 * WAVE + (size_t)j * RB) * CB;
+    __shared__ S values[STAGE_BLOCKS][RB * CB];
+    __shared__ uint32_t columns[STAGE_BLOCKS];
+    extern __shared__ unsigned char staged_x[];  // STAGE_BLOCKS x CB x M entries of x
+    V *xs = reinterpret_cast<V *>(staged_x);
+    const uint32_t t = threadIdx.x;
+    const bool computes = t < (uint32_t)RB * M;
+    const uint32_t p = computes ? t / M : 0, m = computes ? t % M : 0;
+    V sum = (V)0;
+    for (uint32_t k0 = 0; k0 < len; k0 += STAGE_BLOCKS) {
+        const uint32_t n = min((uint32_t)STAGE_BLOCKS, len - k0);
+        for (uint32_t e = t; e < n * RB * CB; e += blockDim.x) {
+            const uint32_t k = e / (RB * CB), r = e - k * (RB * CB);
+            values[k][r] = vp[(size_t)(k0 + k) * WAVE * CB + r];
+        }
+        if (t < n) columns[t] = cp[(size_t)(k0 + t) * NPW];
+        __syncthreads();
+        for (uint32_t e = t; e < n * CB * M; e += blockDim.x) {
+            const uint32_t mm = e % M, kq = e / M;
+            const uint32_t k = kq / CB, q = kq - k * CB;
+            xs[e] = x[((size_t)columns[k] * CB + q) * M + mm];
+        }
+        __syncthreads();
+        if (computes)
+            for (uint32_t k = 0; k < n; ++k)
+#pragma unroll
+                for (int q = 0; q < CB; ++q) sum = sum + widen<V>(values[k][p * CB + q]) * xs[(k * CB + q) * M + m];
+        __syncthreads();
+    }
+    if (!computes) return;
+    const size_t out = ((size_t)row * RB + p) * M + m;
+    V v = sum;
+    if (epi == EPI_RESIDUAL) v = b[out] - sum;
+    else if (epi == EPI_ADD) v = y[out] + sum;
+    y[out] = v;
+}
+
 template <int RB, int CB, int BATCH, typename S, typename V>
 __global__ __launch_bounds__(256) void spmm_node_kernel(Sliced<S> a, uint32_t M, int epi, const V *__restrict__ x,
                                                         V *__restrict__ y, const V *__restrict__ b) {

@@ -899,7 +899,8 @@ class Hierarchy final : public DeviceMultigrid {
             }, spmm_candidates(d.rb, d.rb, sizeof(S), M, false), "", "batch step " + group + shape_class(d.a) + vsize);
             d.mm_depth[lg][1] = tune_batch(s, [&](int batch) {
                 launch_spmm<S, V>(d.a, M, d.mx.ptr, d.mres.ptr, d.mb.ptr, EPI_RESIDUAL, s, batch);
-            }, spmm_candidates(d.rb, d.rb, sizeof(S), M), "", "batch residual " + group + shape_class(d.a) + vsize);
+            }, spmm_candidates(d.rb, d.rb, sizeof(S), M, true, staged_candidate(d.a, M, sizeof(V))), "",
+               "batch residual " + group + shape_class(d.a) + vsize);
             if (d.has_p) {
                 LevelDev<S, V> &c = lv_[l + 1];
                 const size_t cneed = c.n * (size_t)M;
@@ -907,10 +908,12 @@ class Hierarchy final : public DeviceMultigrid {
                 SF_HIP(hipMemsetAsync(c.mx.ptr, 0, cneed * sizeof(V), s));
                 d.mm_depth[lg][2] = tune_batch(s, [&](int batch) {
                     launch_spmm<S, V>(d.r, M, d.mres.ptr, c.mb.ptr, nullptr, EPI_STORE, s, batch);
-                }, spmm_candidates(d.r.rb, d.r.cb, sizeof(S), M), "", "batch restriction " + group + shape_class(d.r) + vsize);
+                }, spmm_candidates(d.r.rb, d.r.cb, sizeof(S), M, true, staged_candidate(d.r, M, sizeof(V))), "",
+                   "batch restriction " + group + shape_class(d.r) + vsize);
                 d.mm_depth[lg][3] = tune_batch(s, [&](int batch) {
                     launch_spmm<S, V>(d.p, M, c.mx.ptr, d.mpx.ptr, nullptr, EPI_ADD, s, batch);
-                }, spmm_candidates(d.p.rb, d.p.cb, sizeof(S), M), "", "batch prolongation " + group + shape_class(d.p) + vsize);
+                }, spmm_candidates(d.p.rb, d.p.cb, sizeof(S), M, true, staged_candidate(d.p, M, sizeof(V))), "",
+                   "batch prolongation " + group + shape_class(d.p) + vsize);
             }
         }
         SF_HIP(hipStreamSynchronize(s));

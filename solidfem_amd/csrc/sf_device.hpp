@@ -135,9 +135,12 @@ constexpr int NODE_FORM = 16;
 constexpr int PAIR_FORM = 32;
 constexpr int QUAD_FORM = 48;
 constexpr int OCT_FORM = 64;  // eight columns per lane
-inline int form_columns(int code) { return code >= OCT_FORM ? 8 : code >= QUAD_FORM ? 4 : code >= PAIR_FORM ? 2 : 1; }
+constexpr int STAGED_FORM = 80;  // products only: one workgroup per block row, chunks staged in LDS
+inline int form_columns(int code) {
+    return code >= STAGED_FORM ? 1 : code >= OCT_FORM ? 8 : code >= QUAD_FORM ? 4 : code >= PAIR_FORM ? 2 : 1;
+}
 inline int form_depth(int code) {
-    return code >= OCT_FORM ? code - OCT_FORM : code >= QUAD_FORM ? code - QUAD_FORM : code >= PAIR_FORM ? code - PAIR_FORM : code >= NODE_FORM ? code - NODE_FORM : code;
+    return code >= STAGED_FORM ? 0 : code >= OCT_FORM ? code - OCT_FORM : code >= QUAD_FORM ? code - QUAD_FORM : code >= PAIR_FORM ? code - PAIR_FORM : code >= NODE_FORM ? code - NODE_FORM : code;
 }
 inline bool node_form(int code) { return code >= NODE_FORM && code < PAIR_FORM; }
 inline int columns_log2(uint32_t m) {
@@ -151,7 +154,12 @@ template <typename S, typename V>
 void launch_spmm(const DeviceSliced<S> &a, uint32_t M, const V *x, V *y, const V *b, int epi, hipStream_t s, int batch = 0);
 // The batch depths worth trying for a matrix's batched product.
 // The variant codes a batched product (`product`) or Chebyshev step is timed over for M columns.
-std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M, bool product = true);
+std::vector<int> spmm_candidates(int rb, int cb, size_t value_size, uint32_t M, bool product = true, bool staged = false);
+// Whether the staged form (STAGED_FORM) is a candidate for products by `a` with M columns of
+// `vector_size` bytes: rows of at least 40 blocks on average (or the form forced), at most 256
+// (row entry, column) pairs per row, and the staged entries of x within 48 KiB of LDS.
+template <typename S>
+bool staged_candidate(const DeviceSliced<S> &a, uint32_t M, size_t vector_size);
 // Tunes a matrix's product on scratch vectors of its shape.
 template <typename S, typename V>
 void tune_spmv(DeviceSliced<S> &a, int epi, hipStream_t s);

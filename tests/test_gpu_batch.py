@@ -200,7 +200,7 @@ def test_solves_outlive_their_context_handle():
 # the batched product forms: lane groups with 1, 2 or 4 blocks in flight, one wavefront per block
 # row, and lanes that take two or four consecutive columns each
 FORMS = {"group-1": 1, "group-4": 4, "node-2": 16 + 2, "pair-1": 32 + 1, "pair-2": 32 + 2, "quad-1": 48 + 1, "quad-2": 48 + 2,
-         "oct-1": 64 + 1, "oct-2": 64 + 2, "group-8": 8}
+         "oct-1": 64 + 1, "oct-2": 64 + 2, "group-8": 8, "staged": 80}
 
 
 @pytest.mark.parametrize("form", list(FORMS))
