@@ -275,14 +275,19 @@ __global__ __launch_bounds__(256) void spmm_staged_kernel(Sliced<S> a, uint32_t 
     const bool computes = t < (uint32_t)RB * M;
     const uint32_t p = computes ? t / M : 0, m = computes ? t % M : 0;
     V sum = (V)0;
+    // a chunk's block columns are loaded during the chunk before it, so its values and its x
+    // entries are fetched together: one round trip per chunk
+    uint32_t column = t < (uint32_t)STAGE_BLOCKS && t < len ? cp[(size_t)t * NPW] : 0;
     for (uint32_t k0 = 0; k0 < len; k0 += STAGE_BLOCKS) {
         const uint32_t n = min((uint32_t)STAGE_BLOCKS, len - k0);
+        if (t < n) columns[t] = column;
+        __syncthreads();
+        const uint32_t k1 = k0 + STAGE_BLOCKS;
+        if (t < (uint32_t)STAGE_BLOCKS && k1 + t < len) column = cp[(size_t)(k1 + t) * NPW];
         for (uint32_t e = t; e < n * RB * CB; e += blockDim.x) {
             const uint32_t k = e / (RB * CB), r = e - k * (RB * CB);
             values[k][r] = vp[(size_t)(k0 + k) * WAVE * CB + r];
         }
-        if (t < n) columns[t] = cp[(size_t)(k0 + t) * NPW];
-        __syncthreads();
         for (uint32_t e = t; e < n * CB * M; e += blockDim.x) {
             const uint32_t mm = e % M, kq = e / M;
             const uint32_t k = kq / CB, q = kq - k * CB;
