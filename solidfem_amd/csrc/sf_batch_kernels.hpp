@@ -471,49 +471,58 @@ __global__ __launch_bounds__(256) void cheb_step_batch_kernel(Sliced<S> a, int l
 }
 
 // The coarsest solve as a product with the f64 inverse for M columns (coarse_inverse_kernel per
-// column): out[i][m] = sum over k ascending of inv(i, k) b[k][m], from zero. A block takes 64
-// rows and all columns; its four waves split the columns, each lane holding up to 16 sums.
+// column): out[i][m] = sum over k ascending of inv(i, k) b[k][m], from zero. A block takes R rows
+// (coarse_rows: R * M' = 256 with M' the power of two at or above M, so one sum per thread) and all
+// the columns: thread t is row t % R, column t / R. Tiles of 32 k of the inverse's R rows and of
+// b pass through two LDS buffers, the next tile's loads in flight while the current one is summed.
 constexpr int COARSE_TILE_K = 32;
+constexpr int COARSE_STAGE = COARSE_TILE_K * 64 / 256;  // tile entries per thread, R and M at most 64
+__host__ __device__ inline uint32_t coarse_rows(uint32_t M) {
+    uint32_t p = 1;
+    while (p < M) p *= 2;
+    return p >= 4 ? 256 / p : 64;
+}
 template <typename V>
-__global__ __launch_bounds__(256) void coarse_inverse_batch_kernel(uint32_t n, uint32_t M, const double *__restrict__ inv,
+__global__ __launch_bounds__(256) void coarse_inverse_batch_kernel(uint32_t n, uint32_t M, uint32_t R, const double *__restrict__ inv,
                                                                    const V *__restrict__ rhs, V *__restrict__ out) {
-    __shared__ double bs[COARSE_TILE_K * 64];
-    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    const uint32_t i = blockIdx.x * WAVE + lane;
-    const uint32_t per = (M + 3) / 4;  // columns per wave, at most 16
-    const uint32_t m0 = wave * per;
-    double sum[16];
+    __shared__ double as[2][COARSE_TILE_K * 64];
+    __shared__ double bs[2][COARSE_TILE_K * 64];
+    const uint32_t t = threadIdx.x, r = t % R, m = t / R;
+    const uint32_t i0 = blockIdx.x * R, i = i0 + r;
+    double pa[COARSE_STAGE], pb[COARSE_STAGE];
+    // tile k0's entries e = t + 256 j: inverse entry (k0 + e / R, i0 + e % R), b entry k0 * M + e
+    auto fetch = [&](uint32_t k0) {
+        const uint32_t kn = n - k0 < (uint32_t)COARSE_TILE_K ? n - k0 : (uint32_t)COARSE_TILE_K;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) sum[c] = 0.0;
+        for (int j = 0; j < COARSE_STAGE; ++j) {
+            const uint32_t e = t + 256 * j, kk = e / R, rr = e % R;
+            pa[j] = kk < kn && i0 + rr < n ? inv[(size_t)(k0 + kk) * n + i0 + rr] : 0.0;
+            pb[j] = e < kn * M ? (double)rhs[(size_t)k0 * M + e] : 0.0;
+        }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < COARSE_STAGE; ++j) {
+            as[buf][t + 256 * j] = pa[j];
+            bs[buf][t + 256 * j] = pb[j];
+        }
+    };
+    fetch(0);
+    put(0);
+    __syncthreads();
+    double sum = 0.0;
+    int buf = 0;
     for (uint32_t k0 = 0; k0 < n; k0 += COARSE_TILE_K) {
         const uint32_t kn = n - k0 < (uint32_t)COARSE_TILE_K ? n - k0 : (uint32_t)COARSE_TILE_K;
-        for (uint32_t e = threadIdx.x; e < kn * M; e += blockDim.x) bs[e] = (double)rhs[(size_t)k0 * M + e];
+        const bool more = k0 + COARSE_TILE_K < n;
+        if (more) fetch(k0 + COARSE_TILE_K);
+        if (m < M && i < n)
+            for (uint32_t kk = 0; kk < kn; ++kk) sum = sum + as[buf][kk * R + r] * bs[buf][kk * M + m];
+        if (more) put(buf ^ 1);
         __syncthreads();
-        if (i < n) {
-            uint32_t kk = 0;
-            for (; kk + 4 <= kn; kk += 4) {  // four loads in flight, terms in ascending k
-                double a[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) a[u] = inv[(size_t)(k0 + kk + u) * n + i];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int c = 0; c < 16; ++c)
-                        if ((uint32_t)c < per && m0 + c < M) sum[c] = sum[c] + a[u] * bs[(kk + u) * M + m0 + c];
-            }
-            for (; kk < kn; ++kk) {
-                const double a = inv[(size_t)(k0 + kk) * n + i];
-#pragma unroll
-                for (int c = 0; c < 16; ++c)
-                    if ((uint32_t)c < per && m0 + c < M) sum[c] = sum[c] + a * bs[kk * M + m0 + c];
-            }
-        }
-        __syncthreads();
+        buf ^= 1;
     }
-    if (i < n)
-#pragma unroll
-        for (int c = 0; c < 16; ++c)
-            if ((uint32_t)c < per && m0 + c < M) out[(size_t)i * M + m0 + c] = (V)sum[c];
+    if (m < M && i < n) out[(size_t)i * M + m] = (V)sum;
 }
 
 // ---- conjugate gradient, M columns -----------------------------------------------------------
@@ -523,7 +532,9 @@ __global__ __launch_bounds__(256) void coarse_inverse_batch_kernel(uint32_t n, u
 // single kernel and each column, the same grid-stride sum from zero; the sums are staged in LDS
 // and reduced by the same pairs as the single kernel's shuffle butterfly (lane l with l + off for
 // off = 32 .. 1, lane 0's value), then its four wave sums in order from zero. Columns read
-// coalesced: thread u takes column u % M of slots t = u / M, u / M + 256 / M, ...
+// coalesced: thread u takes column u % M of slots t = u / M, u / M + 256 / M, ..., eight slots at a
+// time, so that each grid-stride step has their sixteen loads in flight together.
+constexpr int DOT_CHUNK = 8;
 template <int MODE>
 __global__ __launch_bounds__(DOT_THREADS) void dot_partials_batch_kernel(size_t n, uint32_t M, const double *__restrict__ a,
                                                                           const double *__restrict__ b,
@@ -534,11 +545,27 @@ __global__ __launch_bounds__(DOT_THREADS) void dot_partials_batch_kernel(size_t 
     const size_t stride = (size_t)gridDim.x * DOT_THREADS;
     if (u < groups * M) {
         const uint32_t m = u % M;
-        for (uint32_t t = u / M; t < (uint32_t)DOT_THREADS; t += groups) {
-            double v = 0.0;
-            for (size_t i = (size_t)blockIdx.x * DOT_THREADS + t; i < n; i += stride)
-                v = v + (MODE == 0 ? a[i * M + m] * b[i * M + m] : a[i * M + m]);
-            ls[t * M + m] = v;
+        for (uint32_t t0 = u / M; t0 < (uint32_t)DOT_THREADS; t0 += DOT_CHUNK * groups) {
+            double v[DOT_CHUNK];
+#pragma unroll
+            for (int j = 0; j < DOT_CHUNK; ++j) v[j] = 0.0;
+            for (size_t base = (size_t)blockIdx.x * DOT_THREADS; base < n; base += stride) {
+                double x[DOT_CHUNK], y[DOT_CHUNK];
+#pragma unroll
+                for (int j = 0; j < DOT_CHUNK; ++j) {
+                    const size_t i = base + t0 + (size_t)j * groups;
+                    const bool live = t0 + j * groups < (uint32_t)DOT_THREADS && i < n;
+                    x[j] = live ? a[i * M + m] : 0.0;
+                    y[j] = MODE == 0 && live ? b[i * M + m] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < DOT_CHUNK; ++j)  // each slot's terms in ascending i, as alone
+                    if (t0 + j * groups < (uint32_t)DOT_THREADS && base + t0 + (size_t)j * groups < n)
+                        v[j] = v[j] + (MODE == 0 ? x[j] * y[j] : x[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < DOT_CHUNK; ++j)
+                if (t0 + j * groups < (uint32_t)DOT_THREADS) ls[(t0 + j * groups) * M + m] = v[j];
         }
     }
     __syncthreads();

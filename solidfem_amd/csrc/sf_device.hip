@@ -958,8 +958,8 @@ class Hierarchy final : public DeviceMultigrid {
         if (!d.has_p) {
             if (d.has_factor) {
                 if (d.inverse.ptr)
-                    hipLaunchKernelGGL(coarse_inverse_batch_kernel<V>, dim3(grid_for(d.n, WAVE)), dim3(256), 0, s, (uint32_t)d.n, M,
-                                       d.inverse.ptr, b, x);
+                    hipLaunchKernelGGL(coarse_inverse_batch_kernel<V>, dim3(grid_for(d.n, coarse_rows(M))), dim3(256), 0, s, (uint32_t)d.n, M,
+                                       coarse_rows(M), d.inverse.ptr, b, x);
                 else if (d.lds_bytes)
                     hipLaunchKernelGGL(coarse_solve_lds_kernel<V>, dim3(M), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x, M);
                 else
