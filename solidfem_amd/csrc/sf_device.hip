@@ -483,6 +483,22 @@ template void tune_spmv<double, double>(DeviceSliced<double> &, int, hipStream_t
 
 namespace {
 
+// coarse_inverse_small_kernel for n <= COARSE_SMALL rows. Its LDS limit is raised once per
+// process, when a hierarchy forms its inverse (so never under a stream capture).
+template <typename V>
+void raise_coarse_small() {
+    static const bool raised = [] {
+        const int most = (int)coarse_small_lds(COARSE_SMALL);
+        SF_HIP(hipFuncSetAttribute((const void *)coarse_inverse_small_kernel<V>, hipFuncAttributeMaxDynamicSharedMemorySize, most));
+        return true;
+    }();
+    (void)raised;
+}
+template <typename V>
+void launch_coarse_small(uint32_t n, const double *inv, const V *b, V *x, hipStream_t s) {
+    hipLaunchKernelGGL(coarse_inverse_small_kernel<V>, dim3(1), dim3(COARSE_SMALL_THREADS), coarse_small_lds(n), s, n, inv, b, x);
+}
+
 // Chebyshev scalars computed in f64 as the reference's cycle does (cycle.rs:578-650,
 // crates/solidfem-solve/src/gpu/layout.rs:445-468).
 void chebyshev(double lmin, double lmax, size_t degree, double &theta, std::vector<std::pair<double, double>> &steps) {
@@ -645,6 +661,7 @@ class Hierarchy final : public DeviceMultigrid {
                     }
                     hipLaunchKernelGGL(factor_inverse_entries_kernel, dim3(grid_for(n * n, 256)), dim3(256), 0, s, (uint32_t)n,
                                        (const double *)linv.ptr, d.inverse.ptr);
+                    if (n <= COARSE_SMALL) raise_coarse_small<V>();
                     SF_HIP(hipGetLastError());
                     SF_HIP(hipStreamSynchronize(s));  // linv is freed on return
                 }
@@ -1111,7 +1128,9 @@ class Hierarchy final : public DeviceMultigrid {
         LevelDev<S, V> &d = lv_[l];
         if (!d.has_p) {
             if (d.has_factor) {
-                if (d.inverse.ptr)
+                if (d.inverse.ptr && d.n <= COARSE_SMALL)
+                    launch_coarse_small<V>((uint32_t)d.n, d.inverse.ptr, b, x, s);
+                else if (d.inverse.ptr)
                     hipLaunchKernelGGL(coarse_inverse_kernel<V>, dim3(grid_for(d.n, 256)), dim3(256), 0, s, (uint32_t)d.n, d.inverse.ptr, b, x);
                 else if (d.lds_bytes)
                     hipLaunchKernelGGL(coarse_solve_lds_kernel<V>, dim3(1), dim3(WAVE), d.lds_bytes, s, (uint32_t)d.n, d.packed.ptr, b, x, 1u);

@@ -647,6 +647,38 @@ __global__ __launch_bounds__(256) void coarse_inverse_kernel(uint32_t n, const d
     if (i < n) out[i] = (V)sum;
 }
 
+// The same product for a coarse level of at most COARSE_SMALL rows (the plate's has 102): one
+// workgroup brings the whole inverse (at most 128 KB) and the right-hand side into LDS with every
+// load in flight at once, then thread i adds its row's terms in ascending k from zero.
+constexpr uint32_t COARSE_SMALL = 128;
+constexpr uint32_t COARSE_SMALL_THREADS = 1024;
+constexpr size_t coarse_small_lds(uint32_t n) { return ((size_t)n * n + n) * sizeof(double); }
+template <typename V>
+__global__ __launch_bounds__(COARSE_SMALL_THREADS) void coarse_inverse_small_kernel(uint32_t n, const double *__restrict__ inv,
+                                                                                     const V *__restrict__ rhs, V *__restrict__ out) {
+    extern __shared__ double ls[];  // the n x n inverse, then the n right-hand-side entries
+    constexpr int PER = COARSE_SMALL * COARSE_SMALL / COARSE_SMALL_THREADS;
+    const uint32_t t = threadIdx.x, nn = n * n;
+    double a[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const uint32_t e = t + COARSE_SMALL_THREADS * j;
+        a[j] = e < nn ? inv[e] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const uint32_t e = t + COARSE_SMALL_THREADS * j;
+        if (e < nn) ls[e] = a[j];
+    }
+    if (t < n) ls[nn + t] = (double)rhs[t];
+    __syncthreads();
+    if (t < n) {
+        double sum = 0.0;
+        for (uint32_t k = 0; k < n; ++k) sum = sum + ls[(size_t)k * n + t] * ls[nn + k];
+        out[t] = (V)sum;
+    }
+}
+
 // ---- conjugate gradient vectors -------------------------------------------------------------
 
 // Scalars of a solve, resident on the device.
